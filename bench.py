@@ -1,0 +1,130 @@
+"""Throughput of the batched Soccar tick step on one MI355X.
+
+Workload (one "step" = one tick of every arena): 4096 independent Soccar 2v2
+arenas, 4 Octane cars + ball each, on the synthetic arena mesh, cars driven by
+the seeded control process of include/rsg_workload.h (redraw every U{2..60}
+ticks), kickoff auto-reset on goal.  State is resident in HBM for the whole
+timed region.
+
+Prints one JSON line: whole-job arena-ticks/s, the roofline of the dominant
+kernel (HBM-bound: algorithmic bytes per arena-tick = full state read+write +
+controls) and the reference CPU baseline (RocketSim compiled from source, one
+Arena per thread) timed on this host on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "arena-ticks/s, 2v2 batched vs plain GPU vs CPU; adjoint rollouts/s"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per arena-tick, 2v2: 2 x (4 cars x 220 B + ball 36 B +
+# pads 272 B + tick 8 B) + 4 x 24 B controls.
+BYTES_PER_ARENA_TICK_2V2 = 2488
+
+
+def cpu_baseline(target_seconds=10.0):
+    """Reference Arena::Step, one arena per std::thread, all host threads."""
+    from oracle import ref
+    from rocketsim_amd import arena_mesh
+    if not ref.available():
+        return None
+    ref.init([arena_mesh.soccar_cmf()])
+    threads = min(16, os.cpu_count() or 1)
+    # calibrate on a short run, then size the sample to ~target_seconds
+    n, secs = ref.run_workload(threads, 1, 2, 2, 0, 500, 12345)
+    rate = n / max(secs, 1e-9)
+    ticks = int(max(1000, min(200000, rate * target_seconds / threads)))
+    n, secs = ref.run_workload(threads, 1, 2, 2, 0, ticks, 12345)
+    return {
+        "value": n / secs, "unit": "arena-ticks/s", "cores": threads, "kind": "reference",
+        "sample": f"{threads} threads x 1 Soccar 2v2 Octane arena x {ticks} ticks, same control process, "
+                  f"kickoff reset on goal ({secs:.1f} s wall)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--arenas", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); arenas are independent, "
+              "so N GPUs would each step their own shard of the arena range with no per-tick exchange.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from rocketsim_amd import arena_mesh, native
+    from rocketsim_amd.batch import ArenaBatch
+    from rocketsim_amd.state import MODE_SOCCAR
+
+    native.lib()
+    native.check(native.lib().rsg_init_mesh(arena_mesh.soccar_cmf(), len(arena_mesh.soccar_cmf())))
+    batch = ArenaBatch.uniform(args.arenas, 2, 2, 0, mode=MODE_SOCCAR, seed=args.seed,
+                               synth_controls=True, auto_reset=True)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    for _ in range(args.warmup):
+        batch.step(1, sp)
+    torch.cuda.synchronize()
+
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        batch.step(1, sp)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = start.elapsed_time(end)
+
+    arena_ticks = args.arenas * args.steps
+    value = arena_ticks / wall
+    kernel_ms = ev_ms / args.steps  # one kernel launch per step
+    achieved = (args.arenas * BYTES_PER_ARENA_TICK_2V2) / (kernel_ms * 1e-3) / 1e9
+    goals = batch.get_goals()
+    batch.close()
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "arena-ticks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"soccar 2v2 x {args.arenas} arenas, Octane, seeded controls, auto-reset on goal",
+                   "arenas": args.arenas, "cars_per_arena": 4, "tick_rate": 120,
+                   "goals_during_run": int(sum(goals[0]) + sum(goals[1]))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "rsg_step_arena_kernel", "kernel_ms": kernel_ms,
+                     "bytes_per_arena_tick": BYTES_PER_ARENA_TICK_2V2},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,103 @@
+/*
+ * rsg.h -- C ABI of the MI355X batched RocketSim tick step (librsg.so).
+ *
+ * The reference exposes a C++ class API only (src/Sim/Arena/Arena.h,
+ * src/Sim/Car/Car.h, src/Sim/Ball/Ball.h); third-party bindings wrap that C++
+ * (README.md:31-35).  This header is the drop-in boundary for its hot path,
+ * Arena::Step(ticks) (src/Sim/Arena/Arena.cpp:689-756): a batch of N
+ * independent arenas lives in GPU memory and is stepped by hand-written HIP
+ * kernels.  Every entry point returns 0 on success and a negative code on
+ * error (no exceptions cross the boundary); rsg_last_error() describes the
+ * last failure on the calling thread.  The C++ facade in
+ * rocketsim_amd/csrc/rsg_arena_api.h rethrows errors as std::runtime_error
+ * like the reference's RS_ERR_CLOSE (src/Framework.h:67-72).
+ *
+ * Threading: one host thread per batch handle; all work is ordered on the
+ * stream passed to rsg_batch_step (NULL = the default stream).
+ */
+#ifndef RSG_H
+#define RSG_H
+
+#include <stdint.h>
+#include "rsg_state.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rsg_batch rsg_batch;
+
+/* Description of a batch (replaces Arena::Create + AddCar,
+ * src/Sim/Arena/Arena.cpp:50-62, 416-540).  Cars of arena a are
+ * car_team[off_a .. off_a + cars_per_arena[a]) with off_a the prefix sum. */
+typedef struct rsg_batch_desc {
+    int n_arenas;
+    int mode;                     /* RSG_MODE_SOCCAR or RSG_MODE_THE_VOID */
+    float tick_rate;              /* ticks per second, 120 in every workload */
+    uint64_t seed;                /* seeds kickoff shuffles, respawns and the synthetic controls */
+    int auto_reset;               /* kickoff reset after a goal (Arena.cpp:748-752 + 113-223) */
+    int synth_controls;           /* draw controls on device each tick (include/rsg_workload.h) */
+    const int* cars_per_arena;    /* n_arenas entries (each 0..RSG_MAX_CARS) */
+    const uint8_t* car_team;      /* RSG_TEAM_*, one per car */
+    const uint8_t* car_preset;    /* RSG_PRESET_*, one per car */
+} rsg_batch_desc;
+
+/* Last error message of the calling thread ("" if none). */
+const char* rsg_last_error(void);
+
+/* Replaces RocketSim::InitFromMem (src/RocketSim.h:32, src/RocketSim.cpp:92-185)
+ * for the Soccar arena: one collision mesh in the reference's .cmf format
+ * (src/CollisionMeshFile/CollisionMeshFile.cpp:11-61).  Must be called once
+ * before creating a SOCCAR batch. */
+int rsg_init_mesh(const void* cmf_data, uint64_t size);
+
+/* Replaces Arena::Create + AddCar (src/Sim/Arena/Arena.h:71,111).  Cars start
+ * at random kickoff positions (Arena::ResetToRandomKickoff). */
+rsg_batch* rsg_batch_create(const rsg_batch_desc* desc);
+void rsg_batch_destroy(rsg_batch* b);
+
+int rsg_batch_num_arenas(const rsg_batch* b);
+/* Total number of cars and index of arena a's first car in flattened arrays. */
+int rsg_batch_num_cars(const rsg_batch* b);
+int rsg_batch_car_offset(const rsg_batch* b, int arena);
+
+/* Replaces Arena::Step(ticks) (src/Sim/Arena/Arena.h:132) for every arena of
+ * the batch.  `stream` is a hipStream_t (NULL for the default stream). */
+int rsg_batch_step(rsg_batch* b, int ticks, void* stream);
+/* Waits for all work on the batch's last stream. */
+int rsg_batch_sync(rsg_batch* b);
+
+/* Full-state access (Car::Get/SetState Car.h:151-152, Ball::Get/SetState
+ * Ball.h:68-69, BoostPad::Get/SetState BoostPad.h:45-46), including the
+ * per-wheel values the reference keeps in btVehicleRL between ticks.
+ * Flattened arrays: cars in (arena, car) order, pads 34 per arena. */
+int rsg_batch_get_cars(rsg_batch* b, rsg_car_state* out);
+int rsg_batch_set_cars(rsg_batch* b, const rsg_car_state* in);
+int rsg_batch_get_balls(rsg_batch* b, rsg_ball_state* out);
+int rsg_batch_set_balls(rsg_batch* b, const rsg_ball_state* in);
+int rsg_batch_get_pads(rsg_batch* b, rsg_pad_state* out);
+int rsg_batch_set_pads(rsg_batch* b, const rsg_pad_state* in);
+/* Car::controls (Car.h:149), held for the following rsg_batch_step calls. */
+int rsg_batch_set_controls(rsg_batch* b, const rsg_controls* in);
+int rsg_batch_get_controls(rsg_batch* b, rsg_controls* out);
+/* Arena::tickCount (Arena.h:61). */
+int rsg_batch_get_tick_counts(rsg_batch* b, uint64_t* out);
+int rsg_batch_set_tick_counts(rsg_batch* b, const uint64_t* in);
+
+/* Per-arena goal events, replacing the goal-score callback
+ * (Arena::SetGoalScoreCallback, Arena.h:24,102): goals scored by blue (ball
+ * in the orange net), by orange, and the tick of the last goal. */
+int rsg_batch_get_goals(rsg_batch* b, uint32_t* blue, uint32_t* orange, uint64_t* last_goal_tick);
+
+/* Arena::ResetToRandomKickoff (Arena.cpp:113-223) for one arena, or all
+ * arenas when arena < 0; the kickoff order is drawn from `seed`. */
+int rsg_batch_reset_kickoff(rsg_batch* b, int arena, uint64_t seed);
+
+/* Arena::IsBallScored (Arena.cpp:893-901) per arena. */
+int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSG_H */

@@ -1,0 +1,118 @@
+/*
+ * rsg_state.h -- plain-old-data state records exchanged across the C ABI.
+ *
+ * These records are the teacher-forcing surface of the batched tick step: one
+ * record holds everything a car, the ball or a boost pad carries from one tick
+ * to the next, so that a state written with rsg_batch_set_* and stepped once is
+ * a full restatement of the reference tick on the same input.
+ *
+ * Field meanings follow the reference's public structs:
+ *   CarState      src/Sim/Car/Car.h:17-120
+ *   BallState     src/Sim/Ball/Ball.h:18-57
+ *   BoostPadState src/Sim/BoostPad/BoostPad.h:23-31
+ *   CarControls   src/Sim/CarControls.h:7-33
+ *   BallHitInfo   src/Sim/BallHitInfo/BallHitInfo.h:9-27
+ * plus the per-wheel vehicle values the reference keeps inside btVehicleRL
+ * between ticks (src/Sim/btVehicleRL/btVehicleRL.h:9-27): engine force, brake,
+ * steer angle, lateral/longitudinal friction and extra pushback.  The
+ * reference's Car::SetState does not touch those, so a faithful one-tick
+ * comparison has to carry them explicitly.
+ *
+ * Units are the reference's public units: positions in uu, velocities in uu/s,
+ * angular velocity in rad/s.  Rotation is the column-major RotMat of the
+ * reference (src/Math/MathTypes/MathTypes.h:169-170): rot[0..2] = forward,
+ * rot[3..5] = right, rot[6..8] = up.
+ */
+#ifndef RSG_STATE_H
+#define RSG_STATE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Game mode ids, same order as src/Sim/GameMode.h:7-17. Only SOCCAR and
+ * THE_VOID are supported by the batched step. */
+enum { RSG_MODE_SOCCAR = 0, RSG_MODE_THE_VOID = 5 };
+
+/* Car hitbox presets, same order as src/Sim/Car/CarConfig/CarConfig.cpp:20-100. */
+enum {
+    RSG_PRESET_OCTANE = 0, RSG_PRESET_DOMINUS = 1, RSG_PRESET_PLANK = 2,
+    RSG_PRESET_BREAKOUT = 3, RSG_PRESET_HYBRID = 4, RSG_PRESET_MERC = 5,
+    RSG_PRESET_COUNT = 6
+};
+
+enum { RSG_TEAM_BLUE = 0, RSG_TEAM_ORANGE = 1 };
+
+typedef struct rsg_controls {
+    float throttle, steer, pitch, yaw, roll;
+    uint8_t jump, boost, handbrake, _pad;
+} rsg_controls; /* 24 bytes */
+
+/* Bit positions inside rsg_car_state.flags */
+enum {
+    RSG_CF_ON_GROUND = 1u << 0,
+    RSG_CF_WHEEL0 = 1u << 1, RSG_CF_WHEEL1 = 1u << 2,
+    RSG_CF_WHEEL2 = 1u << 3, RSG_CF_WHEEL3 = 1u << 4,
+    RSG_CF_HAS_JUMPED = 1u << 5,
+    RSG_CF_HAS_DOUBLE_JUMPED = 1u << 6,
+    RSG_CF_HAS_FLIPPED = 1u << 7,
+    RSG_CF_IS_FLIPPING = 1u << 8,
+    RSG_CF_IS_JUMPING = 1u << 9,
+    RSG_CF_IS_BOOSTING = 1u << 10,
+    RSG_CF_SUPERSONIC = 1u << 11,
+    RSG_CF_AUTO_FLIPPING = 1u << 12,
+    RSG_CF_WORLD_CONTACT = 1u << 13,
+    RSG_CF_DEMOED = 1u << 14,
+    RSG_CF_HIT_VALID = 1u << 15
+};
+
+typedef struct rsg_car_state {
+    uint64_t tick_count_since_update;
+    uint64_t hit_tick;              /* BallHitInfo.tickCountWhenHit */
+    uint64_t hit_extra_tick;        /* BallHitInfo.tickCountWhenExtraImpulseApplied */
+    float pos[3];
+    float rot[9];
+    float vel[3];
+    float ang_vel[3];
+    uint32_t flags;                 /* RSG_CF_* */
+    float flip_rel_torque[3];
+    float jump_time, flip_time;
+    float air_time, air_time_since_jump;
+    float boost, time_since_boosted, boosting_time;
+    float supersonic_time, handbrake_val;
+    float auto_flip_timer, auto_flip_torque_scale;
+    float world_contact_normal[3];
+    uint32_t car_contact_other_id;
+    float car_contact_cooldown;
+    float demo_respawn_timer;
+    float hit_rel_pos[3], hit_ball_pos[3], hit_extra_vel[3];
+    rsg_controls last_controls;
+    /* per-wheel vehicle values carried between ticks (bullet units) */
+    float wheel_engine_force, wheel_brake, wheel_steer;
+    float wheel_lat_friction[4], wheel_long_friction[4];
+    float wheel_extra_pushback[4];
+} rsg_car_state;
+
+typedef struct rsg_ball_state {
+    uint64_t tick_count_since_update;
+    float pos[3];
+    float rot[9];
+    float vel[3];
+    float ang_vel[3];
+} rsg_ball_state;
+
+typedef struct rsg_pad_state {
+    uint32_t is_active;
+    float cooldown;
+    uint32_t prev_locked_car_id;
+} rsg_pad_state;
+
+enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8 };
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSG_STATE_H */

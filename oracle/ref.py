@@ -1,0 +1,164 @@
+"""ctypes loader for the oracle: the reference RocketSim compiled from its own
+sources into oracle/_ref/librsref.so (recipe: oracle/Makefile, C entry points:
+oracle/ref_harness.cpp).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may use this module, and only as the checker or
+the timed CPU baseline -- never as the product path.
+"""
+import ctypes as C
+import os
+
+from rocketsim_amd.state import BallState, CarState, Controls, PadState
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_ref", "librsref.so")
+
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        L.rsref_last_error.restype = C.c_char_p
+        L.rsref_init.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_int]
+        L.rsref_arena_create.restype = C.c_void_p
+        L.rsref_arena_create.argtypes = [C.c_int, C.c_float]
+        L.rsref_arena_destroy.argtypes = [C.c_void_p]
+        L.rsref_add_car.restype = C.c_uint32
+        L.rsref_add_car.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.rsref_num_cars.argtypes = [C.c_void_p]
+        L.rsref_car_id.restype = C.c_uint32
+        L.rsref_car_id.argtypes = [C.c_void_p, C.c_int]
+        L.rsref_set_controls.argtypes = [C.c_void_p, C.c_int, C.POINTER(Controls)]
+        L.rsref_get_car.argtypes = [C.c_void_p, C.c_int, C.POINTER(CarState)]
+        L.rsref_set_car.argtypes = [C.c_void_p, C.c_int, C.POINTER(CarState)]
+        L.rsref_get_ball.argtypes = [C.c_void_p, C.POINTER(BallState)]
+        L.rsref_set_ball.argtypes = [C.c_void_p, C.POINTER(BallState)]
+        L.rsref_num_pads.argtypes = [C.c_void_p]
+        L.rsref_get_pad.argtypes = [C.c_void_p, C.c_int, C.POINTER(PadState)]
+        L.rsref_set_pad.argtypes = [C.c_void_p, C.c_int, C.POINTER(PadState)]
+        L.rsref_get_tick_count.restype = C.c_uint64
+        L.rsref_get_tick_count.argtypes = [C.c_void_p]
+        L.rsref_set_tick_count.argtypes = [C.c_void_p, C.c_uint64]
+        L.rsref_step.argtypes = [C.c_void_p, C.c_int]
+        L.rsref_is_ball_scored.argtypes = [C.c_void_p]
+        L.rsref_reset_kickoff.argtypes = [C.c_void_p, C.c_int]
+        L.rsref_pad_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int)]
+        L.rsref_run_workload.restype = C.c_double
+        L.rsref_run_workload.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_uint64, C.POINTER(C.c_uint64)]
+        _lib = L
+    return _lib
+
+
+_initialised = False
+
+
+def init(meshes=()):
+    """RocketSim::InitFromMem with the given .cmf blobs (bytes) for SOCCAR."""
+    global _initialised
+    L = lib()
+    n = len(meshes)
+    bufs = [C.create_string_buffer(m, len(m)) for m in meshes]
+    ptrs = (C.c_void_p * max(n, 1))(*[C.cast(b, C.c_void_p) for b in bufs])
+    sizes = (C.c_uint64 * max(n, 1))(*[len(m) for m in meshes])
+    if L.rsref_init(ptrs, sizes, n) != 0:
+        raise RuntimeError(L.rsref_last_error().decode())
+    _initialised = True
+
+
+class RefArena:
+    """One reference Arena with cars added in a fixed order."""
+
+    def __init__(self, mode, tick_rate=120.0):
+        L = lib()
+        self.h = L.rsref_arena_create(mode, tick_rate)
+        if not self.h:
+            raise RuntimeError(L.rsref_last_error().decode())
+        self.L = L
+
+    def add_car(self, team, preset):
+        cid = self.L.rsref_add_car(self.h, team, preset)
+        if cid == 0:
+            raise RuntimeError(self.L.rsref_last_error().decode())
+        return cid
+
+    @property
+    def num_cars(self):
+        return self.L.rsref_num_cars(self.h)
+
+    def get_car(self, i):
+        s = CarState()
+        self.L.rsref_get_car(self.h, i, C.byref(s))
+        return s
+
+    def set_car(self, i, s):
+        self.L.rsref_set_car(self.h, i, C.byref(s))
+
+    def get_ball(self):
+        s = BallState()
+        self.L.rsref_get_ball(self.h, C.byref(s))
+        return s
+
+    def set_ball(self, s):
+        self.L.rsref_set_ball(self.h, C.byref(s))
+
+    def get_pads(self):
+        out = []
+        for i in range(self.L.rsref_num_pads(self.h)):
+            p = PadState()
+            self.L.rsref_get_pad(self.h, i, C.byref(p))
+            out.append(p)
+        return out
+
+    def set_pads(self, pads):
+        for i, p in enumerate(pads):
+            self.L.rsref_set_pad(self.h, i, C.byref(p))
+
+    def set_controls(self, i, c):
+        self.L.rsref_set_controls(self.h, i, C.byref(c))
+
+    @property
+    def tick_count(self):
+        return self.L.rsref_get_tick_count(self.h)
+
+    @tick_count.setter
+    def tick_count(self, t):
+        self.L.rsref_set_tick_count(self.h, t)
+
+    def step(self, ticks=1):
+        if self.L.rsref_step(self.h, ticks) != 0:
+            raise RuntimeError(self.L.rsref_last_error().decode())
+
+    def is_ball_scored(self):
+        return bool(self.L.rsref_is_ball_scored(self.h))
+
+    def reset_kickoff(self, seed=-1):
+        self.L.rsref_reset_kickoff(self.h, seed)
+
+    def close(self):
+        if self.h:
+            self.L.rsref_arena_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_workload(threads, arenas_per_thread, n_blue, n_orange, preset, ticks, seed):
+    """Timed CPU baseline: returns (arena_ticks, wall_seconds)."""
+    L = lib()
+    n = C.c_uint64(0)
+    secs = L.rsref_run_workload(threads, arenas_per_thread, n_blue, n_orange, preset, ticks, seed, C.byref(n))
+    return n.value, secs

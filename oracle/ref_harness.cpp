@@ -1,0 +1,336 @@
+// ref_harness.cpp -- C entry points around the reference RocketSim, compiled
+// from the reference's own sources (see Makefile) into oracle/_ref/librsref.so.
+//
+// TEST INFRASTRUCTURE ONLY.  This is the oracle: tests, the smoke check and the
+// CPU-baseline leg of the benchmark load it to check or time against; the
+// product path never links or calls it.
+//
+// What it exposes:
+//   * mesh init from an in-memory .cmf blob   (RocketSim::InitFromMem, src/RocketSim.cpp:92-185)
+//   * arena create / add car / destroy        (src/Sim/Arena/Arena.cpp:50-62,416-540)
+//   * full-state get/set incl. the per-wheel values btVehicleRL keeps between
+//     ticks (src/Sim/btVehicleRL/btVehicleRL.h:9-27), which Car::SetState does
+//     not touch (src/Sim/Car/Car.cpp:23-36)
+//   * Arena::Step (src/Sim/Arena/Arena.cpp:689-756)
+//   * a multi-threaded timed run of the seeded workload (one Arena per thread)
+//     for the CPU baseline.
+
+#include "RocketSim.h"
+#include "rsg_state.h"
+#include "rsg_workload.h"
+
+#include <atomic>
+#include <chrono>
+#include <thread>
+#include <vector>
+
+using namespace RocketSim;
+
+namespace {
+
+const CarConfig* preset_config(int p) {
+    switch (p) {
+        case RSG_PRESET_OCTANE: return &CAR_CONFIG_OCTANE;
+        case RSG_PRESET_DOMINUS: return &CAR_CONFIG_DOMINUS;
+        case RSG_PRESET_PLANK: return &CAR_CONFIG_PLANK;
+        case RSG_PRESET_BREAKOUT: return &CAR_CONFIG_BREAKOUT;
+        case RSG_PRESET_HYBRID: return &CAR_CONFIG_HYBRID;
+        case RSG_PRESET_MERC: return &CAR_CONFIG_MERC;
+        default: return nullptr;
+    }
+}
+
+struct RefArena {
+    Arena* arena = nullptr;
+    std::vector<Car*> cars;  // creation order
+};
+
+thread_local std::string g_err;
+
+void vset(Vec& v, const float* f) { v = Vec(f[0], f[1], f[2]); }
+void vget(const Vec& v, float* f) { f[0] = v.x; f[1] = v.y; f[2] = v.z; }
+void rset(RotMat& m, const float* f) {
+    m.forward = Vec(f[0], f[1], f[2]);
+    m.right = Vec(f[3], f[4], f[5]);
+    m.up = Vec(f[6], f[7], f[8]);
+}
+void rget(const RotMat& m, float* f) {
+    vget(m.forward, f); vget(m.right, f + 3); vget(m.up, f + 6);
+}
+void cset(CarControls& c, const rsg_controls& s) {
+    c.throttle = s.throttle; c.steer = s.steer; c.pitch = s.pitch; c.yaw = s.yaw; c.roll = s.roll;
+    c.jump = s.jump; c.boost = s.boost; c.handbrake = s.handbrake;
+}
+void cget(const CarControls& c, rsg_controls& s) {
+    s.throttle = c.throttle; s.steer = c.steer; s.pitch = c.pitch; s.yaw = c.yaw; s.roll = c.roll;
+    s.jump = c.jump; s.boost = c.boost; s.handbrake = c.handbrake; s._pad = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rsref_last_error() { return g_err.c_str(); }
+
+// blobs: n .cmf files for SOCCAR (may be 0 for a THE_VOID-only process).
+int rsref_init(const uint8_t* const* blobs, const uint64_t* sizes, int n) {
+    try {
+        std::map<GameMode, std::vector<FileData>> m;
+        for (int i = 0; i < n; i++)
+            m[GameMode::SOCCAR].push_back(FileData(blobs[i], blobs[i] + sizes[i]));
+        RocketSim::InitFromMem(m, true);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void* rsref_arena_create(int mode, float tick_rate) {
+    try {
+        auto* ra = new RefArena();
+        ra->arena = Arena::Create((GameMode)mode, ArenaConfig(), tick_rate);
+        // The solver info's timestep is only written by the first world step
+        // (btDiscreteDynamicsWorld.cpp:414); before that it holds Bullet's
+        // default 1/60, which the wheel pushback of the very first tick reads
+        // (btVehicleRL.cpp:186-194).  Start it at the arena's tick time so a
+        // freshly created arena behaves like one that has already stepped.
+        ra->arena->_bulletWorld.getSolverInfo().m_timeStep = ra->arena->tickTime;
+        return ra;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void rsref_arena_destroy(void* h) {
+    auto* ra = (RefArena*)h;
+    if (!ra) return;
+    delete ra->arena;
+    delete ra;
+}
+
+// Returns the car id (>0), or 0 on error.
+uint32_t rsref_add_car(void* h, int team, int preset) {
+    auto* ra = (RefArena*)h;
+    const CarConfig* cfg = preset_config(preset);
+    if (!cfg) { g_err = "bad preset"; return 0; }
+    Car* c = ra->arena->AddCar((Team)team, *cfg);
+    ra->cars.push_back(c);
+    return c->id;
+}
+
+int rsref_num_cars(void* h) { return (int)((RefArena*)h)->cars.size(); }
+uint32_t rsref_car_id(void* h, int i) { return ((RefArena*)h)->cars[i]->id; }
+
+void rsref_set_controls(void* h, int i, const rsg_controls* c) {
+    cset(((RefArena*)h)->cars[i]->controls, *c);
+}
+
+void rsref_get_car(void* h, int i, rsg_car_state* s) {
+    Car* car = ((RefArena*)h)->cars[i];
+    CarState cs = car->GetState();
+    std::memset(s, 0, sizeof(*s));
+    s->tick_count_since_update = cs.tickCountSinceUpdate;
+    vget(cs.pos, s->pos); rget(cs.rotMat, s->rot); vget(cs.vel, s->vel); vget(cs.angVel, s->ang_vel);
+    uint32_t f = 0;
+    if (cs.isOnGround) f |= RSG_CF_ON_GROUND;
+    for (int w = 0; w < 4; w++) if (cs.wheelsWithContact[w]) f |= (RSG_CF_WHEEL0 << w);
+    if (cs.hasJumped) f |= RSG_CF_HAS_JUMPED;
+    if (cs.hasDoubleJumped) f |= RSG_CF_HAS_DOUBLE_JUMPED;
+    if (cs.hasFlipped) f |= RSG_CF_HAS_FLIPPED;
+    if (cs.isFlipping) f |= RSG_CF_IS_FLIPPING;
+    if (cs.isJumping) f |= RSG_CF_IS_JUMPING;
+    if (cs.isBoosting) f |= RSG_CF_IS_BOOSTING;
+    if (cs.isSupersonic) f |= RSG_CF_SUPERSONIC;
+    if (cs.isAutoFlipping) f |= RSG_CF_AUTO_FLIPPING;
+    if (cs.worldContact.hasContact) f |= RSG_CF_WORLD_CONTACT;
+    if (cs.isDemoed) f |= RSG_CF_DEMOED;
+    if (cs.ballHitInfo.isValid) f |= RSG_CF_HIT_VALID;
+    s->flags = f;
+    vget(cs.flipRelTorque, s->flip_rel_torque);
+    s->jump_time = cs.jumpTime; s->flip_time = cs.flipTime;
+    s->air_time = cs.airTime; s->air_time_since_jump = cs.airTimeSinceJump;
+    s->boost = cs.boost; s->time_since_boosted = cs.timeSinceBoosted; s->boosting_time = cs.boostingTime;
+    s->supersonic_time = cs.supersonicTime; s->handbrake_val = cs.handbrakeVal;
+    s->auto_flip_timer = cs.autoFlipTimer; s->auto_flip_torque_scale = cs.autoFlipTorqueScale;
+    vget(cs.worldContact.contactNormal, s->world_contact_normal);
+    s->car_contact_other_id = cs.carContact.otherCarID;
+    s->car_contact_cooldown = cs.carContact.cooldownTimer;
+    s->demo_respawn_timer = cs.demoRespawnTimer;
+    vget(cs.ballHitInfo.relativePosOnBall, s->hit_rel_pos);
+    vget(cs.ballHitInfo.ballPos, s->hit_ball_pos);
+    vget(cs.ballHitInfo.extraHitVel, s->hit_extra_vel);
+    s->hit_tick = cs.ballHitInfo.tickCountWhenHit;
+    s->hit_extra_tick = cs.ballHitInfo.tickCountWhenExtraImpulseApplied;
+    cget(cs.lastControls, s->last_controls);
+    auto& wi = car->_bulletVehicle.m_wheelInfo;
+    s->wheel_engine_force = wi[0].m_engineForce;
+    s->wheel_brake = wi[0].m_brake;
+    s->wheel_steer = wi[0].m_steerAngle;
+    for (int w = 0; w < 4; w++) {
+        s->wheel_lat_friction[w] = wi[w].m_latFriction;
+        s->wheel_long_friction[w] = wi[w].m_longFriction;
+        s->wheel_extra_pushback[w] = wi[w].m_extraPushback;
+    }
+}
+
+void rsref_set_car(void* h, int i, const rsg_car_state* s) {
+    Car* car = ((RefArena*)h)->cars[i];
+    CarState cs;
+    vset(cs.pos, s->pos); rset(cs.rotMat, s->rot); vset(cs.vel, s->vel); vset(cs.angVel, s->ang_vel);
+    uint32_t f = s->flags;
+    cs.isOnGround = f & RSG_CF_ON_GROUND;
+    for (int w = 0; w < 4; w++) cs.wheelsWithContact[w] = f & (RSG_CF_WHEEL0 << w);
+    cs.hasJumped = f & RSG_CF_HAS_JUMPED;
+    cs.hasDoubleJumped = f & RSG_CF_HAS_DOUBLE_JUMPED;
+    cs.hasFlipped = f & RSG_CF_HAS_FLIPPED;
+    cs.isFlipping = f & RSG_CF_IS_FLIPPING;
+    cs.isJumping = f & RSG_CF_IS_JUMPING;
+    cs.isBoosting = f & RSG_CF_IS_BOOSTING;
+    cs.isSupersonic = f & RSG_CF_SUPERSONIC;
+    cs.isAutoFlipping = f & RSG_CF_AUTO_FLIPPING;
+    cs.worldContact.hasContact = f & RSG_CF_WORLD_CONTACT;
+    cs.isDemoed = f & RSG_CF_DEMOED;
+    cs.ballHitInfo.isValid = f & RSG_CF_HIT_VALID;
+    vset(cs.flipRelTorque, s->flip_rel_torque);
+    cs.jumpTime = s->jump_time; cs.flipTime = s->flip_time;
+    cs.airTime = s->air_time; cs.airTimeSinceJump = s->air_time_since_jump;
+    cs.boost = s->boost; cs.timeSinceBoosted = s->time_since_boosted; cs.boostingTime = s->boosting_time;
+    cs.supersonicTime = s->supersonic_time; cs.handbrakeVal = s->handbrake_val;
+    cs.autoFlipTimer = s->auto_flip_timer; cs.autoFlipTorqueScale = s->auto_flip_torque_scale;
+    vset(cs.worldContact.contactNormal, s->world_contact_normal);
+    cs.carContact.otherCarID = s->car_contact_other_id;
+    cs.carContact.cooldownTimer = s->car_contact_cooldown;
+    cs.demoRespawnTimer = s->demo_respawn_timer;
+    vset(cs.ballHitInfo.relativePosOnBall, s->hit_rel_pos);
+    vset(cs.ballHitInfo.ballPos, s->hit_ball_pos);
+    vset(cs.ballHitInfo.extraHitVel, s->hit_extra_vel);
+    cs.ballHitInfo.tickCountWhenHit = s->hit_tick;
+    cs.ballHitInfo.tickCountWhenExtraImpulseApplied = s->hit_extra_tick;
+    cset(cs.lastControls, s->last_controls);
+    car->SetState(cs);
+    car->_internalState.tickCountSinceUpdate = s->tick_count_since_update;
+    auto& wi = car->_bulletVehicle.m_wheelInfo;
+    for (int w = 0; w < 4; w++) {
+        wi[w].m_engineForce = s->wheel_engine_force;
+        wi[w].m_brake = s->wheel_brake;
+        wi[w].m_steerAngle = (w < 2) ? s->wheel_steer : 0.f;
+        wi[w].m_latFriction = s->wheel_lat_friction[w];
+        wi[w].m_longFriction = s->wheel_long_friction[w];
+        wi[w].m_extraPushback = s->wheel_extra_pushback[w];
+    }
+}
+
+void rsref_get_ball(void* h, rsg_ball_state* s) {
+    BallState bs = ((RefArena*)h)->arena->ball->GetState();
+    std::memset(s, 0, sizeof(*s));
+    s->tick_count_since_update = bs.tickCountSinceUpdate;
+    vget(bs.pos, s->pos); rget(bs.rotMat, s->rot); vget(bs.vel, s->vel); vget(bs.angVel, s->ang_vel);
+}
+
+void rsref_set_ball(void* h, const rsg_ball_state* s) {
+    Ball* b = ((RefArena*)h)->arena->ball;
+    BallState bs;
+    vset(bs.pos, s->pos); rset(bs.rotMat, s->rot); vset(bs.vel, s->vel); vset(bs.angVel, s->ang_vel);
+    b->SetState(bs);
+    b->_internalState.tickCountSinceUpdate = s->tick_count_since_update;
+}
+
+int rsref_num_pads(void* h) { return (int)((RefArena*)h)->arena->_boostPads.size(); }
+
+void rsref_get_pad(void* h, int i, rsg_pad_state* s) {
+    BoostPadState ps = ((RefArena*)h)->arena->_boostPads[i]->GetState();
+    s->is_active = ps.isActive; s->cooldown = ps.cooldown; s->prev_locked_car_id = ps.prevLockedCarID;
+}
+
+void rsref_set_pad(void* h, int i, const rsg_pad_state* s) {
+    BoostPadState ps;
+    ps.isActive = s->is_active != 0; ps.cooldown = s->cooldown; ps.prevLockedCarID = s->prev_locked_car_id;
+    ((RefArena*)h)->arena->_boostPads[i]->SetState(ps);
+}
+
+uint64_t rsref_get_tick_count(void* h) { return ((RefArena*)h)->arena->tickCount; }
+void rsref_set_tick_count(void* h, uint64_t t) { ((RefArena*)h)->arena->tickCount = t; }
+
+int rsref_step(void* h, int ticks) {
+    try {
+        ((RefArena*)h)->arena->Step(ticks);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int rsref_is_ball_scored(void* h) { return ((RefArena*)h)->arena->IsBallScored() ? 1 : 0; }
+
+void rsref_reset_kickoff(void* h, int seed) { ((RefArena*)h)->arena->ResetToRandomKickoff(seed); }
+
+// Pad static config, for checking the tables restated by the product.
+void rsref_pad_config(void* h, int i, float* pos3, int* is_big) {
+    BoostPad* p = ((RefArena*)h)->arena->_boostPads[i];
+    vget(p->config.pos, pos3);
+    *is_big = p->config.isBig;
+}
+
+// Timed CPU run of the seeded workload: `threads` std::threads, each owning
+// `arenas_per_thread` arenas of (n_blue, n_orange) cars of `preset`, stepped
+// `ticks` ticks one tick at a time with the rsg_workload.h control process and
+// a kickoff reset whenever a goal is scored.  Returns wall seconds; writes the
+// total number of arena-ticks to *arena_ticks.
+double rsref_run_workload(int threads, int arenas_per_thread, int n_blue, int n_orange,
+                          int preset, int ticks, uint64_t seed, uint64_t* arena_ticks) {
+    std::atomic<uint64_t> total{0};
+    std::atomic<int> ready{0};
+    std::atomic<bool> go{false};
+    std::vector<std::thread> pool;
+    std::vector<std::chrono::steady_clock::time_point> finish(threads);
+    for (int t = 0; t < threads; t++) {
+        pool.emplace_back([&, t]() {
+            std::vector<RefArena> arenas(arenas_per_thread);
+            std::vector<std::vector<rsg_ctrl_gen>> gens(arenas_per_thread);
+            for (int a = 0; a < arenas_per_thread; a++) {
+                arenas[a].arena = Arena::Create(GameMode::SOCCAR, ArenaConfig(), 120.f);
+                for (int i = 0; i < n_blue + n_orange; i++) {
+                    Car* c = arenas[a].arena->AddCar(i < n_blue ? Team::BLUE : Team::ORANGE,
+                                                    *preset_config(preset));
+                    arenas[a].cars.push_back(c);
+                }
+                arenas[a].arena->ResetToRandomKickoff((int)(seed + t * 7919 + a));
+                gens[a].assign(n_blue + n_orange, rsg_ctrl_gen{0, 0});
+            }
+            ready++;
+            while (!go.load()) std::this_thread::yield();
+            uint64_t done = 0;
+            for (int k = 0; k < ticks; k++) {
+                for (int a = 0; a < arenas_per_thread; a++) {
+                    uint32_t arena_id = (uint32_t)(t * arenas_per_thread + a);
+                    for (size_t i = 0; i < arenas[a].cars.size(); i++) {
+                        rsg_controls c;
+                        // Keep the previous draw unless a redraw is due.
+                        cget(arenas[a].cars[i]->controls, c);
+                        rsg_ctrl_next(seed, arena_id, (uint32_t)i, &gens[a][i], &c);
+                        cset(arenas[a].cars[i]->controls, c);
+                    }
+                    arenas[a].arena->Step(1);
+                    if (arenas[a].arena->IsBallScored())
+                        arenas[a].arena->ResetToRandomKickoff((int)(seed + k));
+                    done++;
+                }
+            }
+            finish[t] = std::chrono::steady_clock::now();
+            total += done;
+            for (auto& ra : arenas) delete ra.arena;
+        });
+    }
+    while (ready.load() < threads) std::this_thread::yield();
+    auto t0 = std::chrono::steady_clock::now();
+    go = true;
+    for (auto& th : pool) th.join();
+    auto t1 = *std::max_element(finish.begin(), finish.end());
+    *arena_ticks = total.load();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+"""Procedural Soccar arena collision mesh, written in the reference's
+CollisionMeshFile (.cmf) format.
+
+The reference ships no arena meshes (RocketSim.cpp loads them at run time,
+src/RocketSim.cpp:92-185), so every workload here uses one synthetic arena:
+
+  * playing volume x in [-4096, 4096], y in [-5120, 5120], z in [0, 2044] uu;
+  * the floor (z = 0), ceiling (z = 2048) and side walls (x = +-4096) are the
+    reference's static planes (src/Sim/Arena/Arena.cpp:1036-1068) and are not
+    part of the mesh;
+  * the mesh holds the back walls (y = +-5120), 45 degree corner bevels of
+    1152 uu, 256 uu fillets where walls meet floor and ceiling (also along the
+    side walls), and both goals: a mouth 1786 wide x 642.8 tall cut into the
+    back wall and an 880 deep box behind it;
+  * about 8k triangles, every triangle's normal pointing into the play space.
+
+.cmf layout (src/CollisionMeshFile/CollisionMeshFile.cpp:11-61): int32
+numTris, int32 numVerts, numTris x int32[3] vertex indices, numVerts x
+float32[3] vertices, in Bullet units (uu / 50, no scaling on load,
+CollisionMeshFile.cpp:63-73).
+"""
+import math
+import struct
+
+import numpy as np
+
+EXTENT_X = 4096.0
+EXTENT_Y = 5120.0
+HEIGHT = 2044.0
+FILLET_R = 256.0
+BEVEL = 1152.0
+GOAL_HALF_W = 1786.0 / 2
+GOAL_H = 642.8
+GOAL_DEPTH = 880.0
+UU_TO_BT = 1.0 / 50.0
+
+FILLET_SEGS = 8      # arc segments per fillet
+WALL_SEGS = 6        # vertical wall segments between the fillets
+EDGE_STEP = 164.0    # target edge length along the walls
+
+
+def _outline():
+    """Arena outline (counter-clockwise seen from above) as a list of corners."""
+    X, Y, B = EXTENT_X, EXTENT_Y, BEVEL
+    return [
+        (X, -(Y - B)), (X, Y - B),          # +x side wall
+        (X - B, Y),                         # bevel to the +y back wall
+        (-(X - B), Y),                      # +y back wall
+        (-X, Y - B),                        # bevel
+        (-X, -(Y - B)),                     # -x side wall
+        (-(X - B), -Y),                     # bevel
+        (X - B, -Y),                        # -y back wall
+    ]
+
+
+def _profile():
+    """(inset d, height z) samples of the wall cross-section, bottom to top."""
+    pts = []
+    for k in range(FILLET_SEGS + 1):
+        t = (math.pi / 2) * k / FILLET_SEGS
+        pts.append((FILLET_R - FILLET_R * math.sin(t), FILLET_R - FILLET_R * math.cos(t)))
+    z0, z1 = FILLET_R, HEIGHT - FILLET_R
+    # goal crossbar height must be a profile sample so the mouth is cut cleanly
+    zs = list(np.linspace(z0, z1, WALL_SEGS + 1)[1:-1]) + [GOAL_H]
+    for z in sorted(zs):
+        pts.append((0.0, float(z)))
+    for k in range(FILLET_SEGS + 1):
+        t = (math.pi / 2) * k / FILLET_SEGS
+        pts.append((FILLET_R - FILLET_R * math.cos(t), z1 + FILLET_R * math.sin(t)))
+    return pts
+
+
+def _offset_polygon(corners, d):
+    """Inward offset of a convex CCW polygon by distance d (mitred corners)."""
+    n = len(corners)
+    P = np.array(corners, dtype=np.float64)
+    normals = []
+    for i in range(n):
+        a, b = P[i], P[(i + 1) % n]
+        e = b - a
+        e /= np.linalg.norm(e)
+        normals.append(np.array([-e[1], e[0]]))  # left normal = inward for CCW
+    out = []
+    for i in range(n):
+        n0 = normals[(i - 1) % n]
+        n1 = normals[i]
+        m = n0 + n1
+        m /= np.linalg.norm(m)
+        scale = d / max(np.dot(m, n1), 1e-9)
+        out.append(P[i] + m * scale)
+    return out
+
+
+def build_mesh():
+    """Returns (vertices [V,3] float32 in uu, triangles [T,3] int32)."""
+    corners = _outline()
+    prof = _profile()
+    n_edges = len(corners)
+    rings = [_offset_polygon(corners, d) for d, _ in prof]
+    verts = []
+    tris = []
+    vid = {}
+
+    def vertex(p):
+        key = (round(p[0], 4), round(p[1], 4), round(p[2], 4))
+        if key not in vid:
+            vid[key] = len(verts)
+            verts.append(p)
+        return vid[key]
+
+    interior = np.array([0.0, 0.0, HEIGHT / 2])
+
+    def add_quad(a, b, c, d, inward_ref):
+        # a-b-c-d in order around the quad; orient both triangles towards inward_ref
+        for tri in ((a, b, c), (a, c, d)):
+            p = [np.asarray(verts[i]) for i in tri]
+            nrm = np.cross(p[1] - p[0], p[2] - p[0])
+            if np.linalg.norm(nrm) < 1e-9:
+                continue
+            cen = (p[0] + p[1] + p[2]) / 3
+            if np.dot(nrm, inward_ref - cen) < 0:
+                tri = (tri[0], tri[2], tri[1])
+            tris.append(tri)
+
+    for e in range(n_edges):
+        c0, c1 = corners[e], corners[(e + 1) % n_edges]
+        is_side = abs(c0[0] - c1[0]) < 1e-6 and abs(abs(c0[0]) - EXTENT_X) < 1e-6
+        is_back = abs(c0[1] - c1[1]) < 1e-6 and abs(abs(c0[1]) - EXTENT_Y) < 1e-6
+        length = math.dist(c0, c1)
+        # split the back walls exactly at the goal posts
+        if is_back:
+            xa, xb = c0[0], c1[0]
+            cuts = sorted({xa, xb, -GOAL_HALF_W, GOAL_HALF_W})
+            cuts = [x for x in cuts if min(xa, xb) - 1e-6 <= x <= max(xa, xb) + 1e-6]
+            ts = []
+            for k in range(len(cuts) - 1):
+                seg = cuts[k + 1] - cuts[k]
+                m = max(1, int(round(seg / EDGE_STEP)))
+                ts += [(cuts[k] + seg * j / m - xa) / (xb - xa) for j in range(m)]
+            ts.append(1.0)
+            ts = sorted(set(round(t, 9) for t in ts))
+        else:
+            m = max(1, int(round(length / EDGE_STEP)))
+            ts = [j / m for j in range(m + 1)]
+        for k in range(len(prof) - 1):
+            (d0, z0), (d1, z1) = prof[k], prof[k + 1]
+            if is_side and d0 == 0.0 and d1 == 0.0:
+                continue  # flat side wall: the x = +-4096 planes cover it
+            r0a, r0b = rings[k][e], rings[k][(e + 1) % n_edges]
+            r1a, r1b = rings[k + 1][e], rings[k + 1][(e + 1) % n_edges]
+            for j in range(len(ts) - 1):
+                ta, tb = ts[j], ts[j + 1]
+                pa0 = r0a + (r0b - r0a) * ta
+                pb0 = r0a + (r0b - r0a) * tb
+                pa1 = r1a + (r1b - r1a) * ta
+                pb1 = r1a + (r1b - r1a) * tb
+                if is_back:
+                    xm = 0.5 * (pa0[0] + pb0[0])
+                    zm = 0.5 * (z0 + z1)
+                    if abs(xm) < GOAL_HALF_W and zm < GOAL_H:
+                        continue  # goal mouth
+                q = [vertex((float(pa0[0]), float(pa0[1]), float(z0))),
+                     vertex((float(pb0[0]), float(pb0[1]), float(z0))),
+                     vertex((float(pb1[0]), float(pb1[1]), float(z1))),
+                     vertex((float(pa1[0]), float(pa1[1]), float(z1)))]
+                add_quad(*q, interior)
+
+    # goals: side walls, back wall and ceiling of an 880 deep box behind each mouth
+    def grid_quads(origin, du, dv, nu, nv, inward_ref):
+        origin, du, dv = (np.asarray(v, dtype=np.float64) for v in (origin, du, dv))
+        for i in range(nu):
+            for j in range(nv):
+                p00 = origin + du * (i / nu) + dv * (j / nv)
+                p10 = origin + du * ((i + 1) / nu) + dv * (j / nv)
+                p11 = origin + du * ((i + 1) / nu) + dv * ((j + 1) / nv)
+                p01 = origin + du * (i / nu) + dv * ((j + 1) / nv)
+                q = [vertex(tuple(float(x) for x in p)) for p in (p00, p10, p11, p01)]
+                add_quad(*q, inward_ref)
+
+    for s in (1.0, -1.0):
+        y0 = s * EXTENT_Y
+        y1 = s * (EXTENT_Y + GOAL_DEPTH)
+        inside = np.array([0.0, s * (EXTENT_Y + GOAL_DEPTH / 2), GOAL_H / 2])
+        nw, nd, nh = 8, 4, 3
+        grid_quads((-GOAL_HALF_W, y0, 0), (0, y1 - y0, 0), (0, 0, GOAL_H), nd, nh, inside)  # left
+        grid_quads((GOAL_HALF_W, y0, 0), (0, y1 - y0, 0), (0, 0, GOAL_H), nd, nh, inside)   # right
+        grid_quads((-GOAL_HALF_W, y1, 0), (2 * GOAL_HALF_W, 0, 0), (0, 0, GOAL_H), nw, nh, inside)  # back
+        grid_quads((-GOAL_HALF_W, y0, GOAL_H), (2 * GOAL_HALF_W, 0, 0), (0, y1 - y0, 0), nw, nd, inside)  # roof
+
+    V = np.array(verts, dtype=np.float32)
+    T = np.array(tris, dtype=np.int32)
+    return V, T
+
+
+def to_cmf(verts_uu, tris):
+    """Serialise to .cmf bytes (vertices in Bullet units)."""
+    v_bt = (np.asarray(verts_uu, dtype=np.float64) * UU_TO_BT).astype(np.float32)
+    out = struct.pack("<ii", len(tris), len(v_bt))
+    out += np.ascontiguousarray(tris, dtype="<i4").tobytes()
+    out += np.ascontiguousarray(v_bt, dtype="<f4").tobytes()
+    return out
+
+
+def from_cmf(data):
+    """Parse .cmf bytes -> (vertices [V,3] float32 in Bullet units, tris [T,3] int32)."""
+    nt, nv = struct.unpack_from("<ii", data, 0)
+    if nt <= 0 or nv <= 0 or max(nt, nv) > 1000 * 1000:
+        raise ValueError(f"bad .cmf counts ({nt}, {nv})")
+    need = 8 + nt * 12 + nv * 12
+    if len(data) < need:
+        raise ValueError(".cmf data truncated")
+    tris = np.frombuffer(data, dtype="<i4", count=nt * 3, offset=8).reshape(nt, 3)
+    verts = np.frombuffer(data, dtype="<f4", count=nv * 3, offset=8 + nt * 12).reshape(nv, 3)
+    if tris.min() < 0 or tris.max() >= nv:
+        raise ValueError("bad .cmf triangle vertex index")
+    return verts, tris
+
+
+_CACHE = None
+
+
+def soccar_cmf():
+    """The synthetic Soccar arena as .cmf bytes (deterministic, cached)."""
+    global _CACHE
+    if _CACHE is None:
+        V, T = build_mesh()
+        _CACHE = to_cmf(V, T)
+    return _CACHE
+
+
+if __name__ == "__main__":
+    V, T = build_mesh()
+    print("verts", len(V), "tris", len(T))
+    print("bounds", V.min(0), V.max(0))

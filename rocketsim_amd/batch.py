@@ -1,0 +1,125 @@
+"""Python front end of the batched tick step (wraps include/rsg.h).
+
+`ArenaBatch` is the batched counterpart of the reference's Arena
+(src/Sim/Arena/Arena.h): `step(ticks)` replaces `Arena::Step(ticks)`
+(src/Sim/Arena/Arena.cpp:689-756) for every arena of the batch at once, and
+the get/set methods replace Car/Ball/BoostPad Get/SetState.
+"""
+import ctypes as C
+
+from . import native
+from .state import (BallState, BatchDesc, CarState, Controls, MODE_SOCCAR, MODE_THE_VOID,
+                    NUM_PADS, PadState)
+
+
+class ArenaBatch:
+    def __init__(self, cars_per_arena, teams, presets, mode=MODE_SOCCAR, tick_rate=120.0,
+                 seed=0, auto_reset=False, synth_controls=False):
+        L = native.lib()
+        n = len(cars_per_arena)
+        total = sum(cars_per_arena)
+        if len(teams) != total or len(presets) != total:
+            raise ValueError("teams/presets must have one entry per car")
+        self._cpa = (C.c_int * n)(*cars_per_arena)
+        self._team = (C.c_uint8 * max(total, 1))(*teams)
+        self._preset = (C.c_uint8 * max(total, 1))(*presets)
+        d = BatchDesc(n_arenas=n, mode=mode, tick_rate=tick_rate, seed=seed,
+                      auto_reset=int(auto_reset), synth_controls=int(synth_controls),
+                      cars_per_arena=self._cpa, car_team=self._team, car_preset=self._preset)
+        h = L.rsg_batch_create(C.byref(d))
+        if not h:
+            raise RuntimeError(L.rsg_last_error().decode())
+        self.h = h
+        self.L = L
+        self.n_arenas = n
+        self.n_cars = total
+        self.cars_per_arena = list(cars_per_arena)
+        self.offsets = [L.rsg_batch_car_offset(h, a) for a in range(n)]
+
+    @classmethod
+    def uniform(cls, n_arenas, n_blue, n_orange, preset=0, **kw):
+        per = n_blue + n_orange
+        teams = ([0] * n_blue + [1] * n_orange) * n_arenas
+        return cls([per] * n_arenas, teams, [preset] * (per * n_arenas), **kw)
+
+    # -- stepping ---------------------------------------------------------
+    def step(self, ticks=1, stream=None):
+        native.check(self.L.rsg_batch_step(self.h, ticks, stream))
+
+    def sync(self):
+        native.check(self.L.rsg_batch_sync(self.h))
+
+    def reset_kickoff(self, arena=-1, seed=0):
+        native.check(self.L.rsg_batch_reset_kickoff(self.h, arena, seed))
+
+    # -- state ------------------------------------------------------------
+    def get_cars(self):
+        out = (CarState * max(self.n_cars, 1))()
+        native.check(self.L.rsg_batch_get_cars(self.h, out))
+        return out
+
+    def set_cars(self, states):
+        arr = (CarState * max(self.n_cars, 1))(*states)
+        native.check(self.L.rsg_batch_set_cars(self.h, arr))
+
+    def get_balls(self):
+        out = (BallState * self.n_arenas)()
+        native.check(self.L.rsg_batch_get_balls(self.h, out))
+        return out
+
+    def set_balls(self, states):
+        arr = (BallState * self.n_arenas)(*states)
+        native.check(self.L.rsg_batch_set_balls(self.h, arr))
+
+    def get_pads(self):
+        out = (PadState * (NUM_PADS * self.n_arenas))()
+        native.check(self.L.rsg_batch_get_pads(self.h, out))
+        return out
+
+    def set_pads(self, states):
+        arr = (PadState * (NUM_PADS * self.n_arenas))(*states)
+        native.check(self.L.rsg_batch_set_pads(self.h, arr))
+
+    def set_controls(self, controls):
+        arr = (Controls * max(self.n_cars, 1))(*controls)
+        native.check(self.L.rsg_batch_set_controls(self.h, arr))
+
+    def get_controls(self):
+        out = (Controls * max(self.n_cars, 1))()
+        native.check(self.L.rsg_batch_get_controls(self.h, out))
+        return out
+
+    def get_tick_counts(self):
+        out = (C.c_uint64 * self.n_arenas)()
+        native.check(self.L.rsg_batch_get_tick_counts(self.h, out))
+        return list(out)
+
+    def set_tick_counts(self, ticks):
+        arr = (C.c_uint64 * self.n_arenas)(*ticks)
+        native.check(self.L.rsg_batch_set_tick_counts(self.h, arr))
+
+    def get_goals(self):
+        b = (C.c_uint32 * self.n_arenas)()
+        o = (C.c_uint32 * self.n_arenas)()
+        t = (C.c_uint64 * self.n_arenas)()
+        native.check(self.L.rsg_batch_get_goals(self.h, b, o, t))
+        return list(b), list(o), list(t)
+
+    def is_ball_scored(self):
+        out = (C.c_uint8 * self.n_arenas)()
+        native.check(self.L.rsg_batch_is_ball_scored(self.h, out))
+        return [bool(x) for x in out]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rsg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["ArenaBatch", "MODE_SOCCAR", "MODE_THE_VOID"]

@@ -1,0 +1,42 @@
+"""Build recipes for the native library (librsg.so, HIP for gfx950).
+
+The library is built in-tree (rocketsim_amd/lib/) so it travels with the
+source tree; nothing is installed into site-packages.
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+LIB_PATH = os.path.join(LIBDIR, "librsg.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off keeps a*b+c as two rounded operations like the reference's
+# x86-64 build; packed FP32 instructions are disabled (they are not trusted on
+# this hardware pool).
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build_native(verbose=True):
+    os.makedirs(LIBDIR, exist_ok=True)
+    src = os.path.join(CSRC, "rsg_batch.hip")
+    tmp = LIB_PATH + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-I" + os.path.join(ROOT, "include"), src, "-o", tmp]
+    if verbose:
+        print("[build]", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building librsg.so")
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    build_native()

@@ -1,0 +1,32 @@
+// rsg_arena.h -- the working copy of one arena during a tick.
+#pragma once
+
+#include "rsg_sim.h"
+
+namespace rsg {
+
+constexpr int MAX_CARS = RSG_MAX_CARS;
+
+struct ArenaLocal {
+    Car cars[MAX_CARS];
+    int n_cars;
+    Ball ball;
+    Pad pads[PAD_NUM];
+    uint64_t tick;
+    uint32_t goals_blue, goals_orange;
+    uint64_t last_goal_tick;
+    uint32_t resets;
+    uint32_t status;
+    SolverInfo si;
+};
+
+struct TickParams {
+    int mode;
+    float dt;
+    uint64_t seed;
+    uint32_t arena_id;
+    bool synth_controls;
+    bool auto_reset;
+};
+
+}  // namespace rsg

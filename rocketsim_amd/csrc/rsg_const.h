@@ -1,0 +1,349 @@
+// rsg_const.h -- game constants, curves, presets and spawn/pad tables of the
+// Soccar tick step, restated as data.
+//
+// Values follow the reference exactly (they are facts of the game, not code):
+//   src/RLConst.h:10-527            game constants, curves, pad and spawn tables
+//   src/Sim/Car/CarConfig/CarConfig.cpp:20-100   hitbox / wheel presets
+//   src/Sim/MutatorConfig/MutatorConfig.{h,cpp}  Soccar mutator defaults
+//   src/BulletLink.h:12,15                       uu <-> Bullet metres (x50)
+// Expressions are written in the same types as the reference so that constants
+// computed in double and narrowed to float round identically.
+#pragma once
+
+#include "rsg_math.h"
+
+namespace rsg {
+
+constexpr float BT_TO_UU = 50.f;
+constexpr float UU_TO_BT = 1.f / 50.f;
+
+constexpr double PI_D = 3.14159265358979323846;
+constexpr float GRAVITY_Z = -650.f;
+constexpr float ARENA_EXTENT_X = 4096, ARENA_EXTENT_Y = 5120, ARENA_HEIGHT = 2048;
+constexpr float ARENA_COLLISION_BASE_FRICTION = 0.6f, ARENA_COLLISION_BASE_RESTITUTION = 0.3f;
+constexpr float CAR_MASS_BT = 180.f;
+constexpr float BALL_MASS_BT = CAR_MASS_BT / 6.f;
+constexpr float CAR_COLLISION_FRICTION = 0.3f, CAR_COLLISION_RESTITUTION = 0.1f;
+constexpr float CARBALL_COLLISION_FRICTION = 2.0f, CARBALL_COLLISION_RESTITUTION = 0.0f;
+constexpr float CARWORLD_COLLISION_FRICTION = 0.3f, CARWORLD_COLLISION_RESTITUTION = 0.3f;
+constexpr float CARCAR_COLLISION_FRICTION = 0.09f, CARCAR_COLLISION_RESTITUTION = 0.1f;
+constexpr float BALL_REST_Z = 93.15f;
+constexpr float BALL_MAX_ANG_SPEED = 6.f;
+constexpr float BALL_DRAG = 0.03f;
+constexpr float BALL_FRICTION = 0.35f;
+constexpr float BALL_RESTITUTION = 0.6f;
+constexpr float CAR_MAX_SPEED = 2300.f;
+constexpr float BALL_MAX_SPEED = 6000.f;
+constexpr float BOOST_MAX = 100.f;
+constexpr float BOOST_USED_PER_SECOND = BOOST_MAX / 3;
+constexpr float BOOST_MIN_TIME = 0.1f;
+constexpr float BOOST_ACCEL_GROUND = 2975 / 3.f;
+constexpr float BOOST_ACCEL_AIR = 3175 / 3.f;
+constexpr float BOOST_SPAWN_AMOUNT = BOOST_MAX / 3;
+constexpr float RECHARGE_BOOST_PER_SECOND = 10;
+constexpr float RECHARGE_BOOST_DELAY = 0.25;
+constexpr float CAR_MAX_ANG_SPEED = 5.5f;
+constexpr float SUPERSONIC_START_SPEED = 2200.f;
+constexpr float SUPERSONIC_MAINTAIN_MIN_SPEED = SUPERSONIC_START_SPEED - 100.f;
+constexpr float SUPERSONIC_MAINTAIN_MAX_TIME = 1.f;
+constexpr float POWERSLIDE_RISE_RATE = 5, POWERSLIDE_FALL_RATE = 2;
+constexpr float THROTTLE_TORQUE_AMOUNT = CAR_MASS_BT * 400.f;
+constexpr float BRAKE_TORQUE_AMOUNT = CAR_MASS_BT * (14.25f + (1.f / 3.f));
+constexpr float STOPPING_FORWARD_VEL = 25.f;
+constexpr float COASTING_BRAKE_FACTOR = 0.15f;
+constexpr float BRAKING_NO_THROTTLE_SPEED_THRESH = 0.01f;
+constexpr float THROTTLE_DEADZONE = 0.001f;
+constexpr float THROTTLE_AIR_ACCEL = 200 / 3.f;
+constexpr float JUMP_ACCEL = 4375.f / 3.f;
+constexpr float JUMP_IMMEDIATE_FORCE = 875.f / 3.f;
+constexpr float JUMP_MIN_TIME = 0.025f;
+constexpr float JUMP_RESET_TIME_PAD = (1 / 40.f);
+constexpr float JUMP_MAX_TIME = 0.2f;
+constexpr float JUMP_PRE_MIN_ACCEL_SCALE = 0.62f;  // src/Sim/Car/Car.cpp:566
+constexpr float DOUBLEJUMP_MAX_DELAY = 1.25f;
+constexpr float FLIP_Z_DAMP_120 = 0.35f;
+constexpr float FLIP_Z_DAMP_START = 0.15f;
+constexpr float FLIP_Z_DAMP_END = 0.21f;
+constexpr float FLIP_TORQUE_TIME = 0.65f;
+constexpr float FLIP_PITCHLOCK_EXTRA_TIME = 0.3f;
+constexpr float FLIP_INITIAL_VEL_SCALE = 500.f;
+constexpr float FLIP_TORQUE_X = 260.f, FLIP_TORQUE_Y = 224.f;
+constexpr float FLIP_FORWARD_IMPULSE_MAX_SPEED_SCALE = 1.f;
+constexpr float FLIP_SIDE_IMPULSE_MAX_SPEED_SCALE = 1.9f;
+constexpr float FLIP_BACKWARD_IMPULSE_MAX_SPEED_SCALE = 2.5f;
+constexpr float FLIP_BACKWARD_IMPULSE_SCALE_X = 16.f / 15.f;
+constexpr float BALL_COLLISION_RADIUS_SOCCAR = 91.25f;
+constexpr float SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y = 5124.25f;
+constexpr float CAR_TORQUE_SCALE = 2 * PI_D / (1 << 16) * 1000;
+constexpr float CAR_AUTOFLIP_IMPULSE = 200;
+constexpr float CAR_AUTOFLIP_TORQUE = 50;
+constexpr float CAR_AUTOFLIP_TIME = 0.4f;
+constexpr float CAR_AUTOFLIP_NORMZ_THRESH = 0.70710678118654752440;
+constexpr float CAR_AUTOFLIP_ROLL_THRESH = 2.8f;
+constexpr float CAR_AUTOROLL_FORCE = 100;
+constexpr float CAR_AUTOROLL_TORQUE = 80;
+constexpr float BALL_CAR_EXTRA_IMPULSE_Z_SCALE = 0.35f;
+constexpr float BALL_CAR_EXTRA_IMPULSE_FORWARD_SCALE = 0.65f;
+constexpr float BALL_CAR_EXTRA_IMPULSE_MAXDELTAVEL_UU = 4600.f;
+constexpr float CAR_SPAWN_REST_Z = 17.f;
+constexpr float CAR_RESPAWN_Z = 36.f;
+constexpr float BUMP_COOLDOWN_TIME = 0.25f;
+constexpr float BUMP_MIN_FORWARD_DIST = 64.5f;
+constexpr float DEMO_RESPAWN_TIME = 3.f;
+
+// CAR_AIR_CONTROL_TORQUE / _DAMPING, order pitch, yaw, roll.
+constexpr float AIR_TORQUE_P = 130, AIR_TORQUE_Y = 95, AIR_TORQUE_R = 400;
+constexpr float AIR_DAMP_P = 30, AIR_DAMP_Y = 20, AIR_DAMP_R = 50;
+
+// btVehicleRL tuning (src/RLConst.h BTVehicle namespace)
+constexpr float SUSPENSION_FORCE_SCALE_FRONT = 36.f - (1.f / 4.f);
+constexpr float SUSPENSION_FORCE_SCALE_BACK = 54.f + (1.f / 4.f) + (1.5f / 100.f);
+constexpr float SUSPENSION_STIFFNESS = 500.f;
+constexpr float WHEELS_DAMPING_COMPRESSION = 25.f;
+constexpr float WHEELS_DAMPING_RELAXATION = 40.f;
+constexpr float MAX_SUSPENSION_TRAVEL = 12.f;
+constexpr float SUSPENSION_SUBTRACTION = 0.05f;
+constexpr float ROLLING_FRICTION_SCALE_MAGIC = 113.73963f;  // src/Sim/btVehicleRL/btVehicleRL.cpp:359
+
+// Boost pads (src/RLConst.h BoostPads namespace)
+constexpr float PAD_CYL_HEIGHT = 95, PAD_CYL_RAD_BIG = 208, PAD_CYL_RAD_SMALL = 144;
+constexpr float PAD_BOX_HEIGHT = 64, PAD_BOX_RAD_BIG = 160, PAD_BOX_RAD_SMALL = 120;
+constexpr float PAD_COOLDOWN_BIG = 10, PAD_COOLDOWN_SMALL = 4;
+constexpr float PAD_BOOST_AMOUNT_BIG = 100, PAD_BOOST_AMOUNT_SMALL = 12;
+constexpr int PAD_NUM_BIG = 6, PAD_NUM_SMALL = 28, PAD_NUM = 34;
+
+// Boost pad grid (src/Sim/BoostPad/BoostPadGrid/BoostPadGrid.h)
+constexpr float PAD_GRID_EXTENT_Z = PAD_CYL_HEIGHT + 250.f;
+constexpr int PAD_GRID_CELLS_X = 8, PAD_GRID_CELLS_Y = 10;
+constexpr int PAD_GRID_CELL_SIZE_X = (int)(4096.f / (8 / 2));
+constexpr int PAD_GRID_CELL_SIZE_Y = (int)(5120.f / (10 / 2));
+
+// Bullet solver / world settings used by the reference arena
+// (src/Sim/Arena/Arena.cpp:473-475, BulletDynamics/ConstraintSolver/btContactSolverInfo.h:77-113)
+constexpr int SOLVER_ITERATIONS = 10;
+constexpr float SOLVER_ERP2 = 0.8f;
+constexpr float SOLVER_SPLIT_TURN_ERP = 0.1f;
+constexpr float SOLVER_RESTITUTION_VEL_THRESHOLD = 0.2f;
+constexpr float FLT_EPS = 1.1920928955078125e-07f;
+constexpr float ANGULAR_MOTION_THRESHOLD = 0.5f * 1.57079632679489661923f;
+constexpr float CONTACT_BREAKING_THRESHOLD = 0.02f;  // gContactBreakingThreshold
+constexpr float CONVEX_DISTANCE_MARGIN = 0.04f;
+
+// Piecewise-linear curves (LinearPieceCurve, src/Math/Math.cpp:5-35).
+struct Curve {
+    int n;
+    float x[6];
+    float y[6];
+};
+
+RSG_FN float curve_eval(const Curve& c, float input, float default_out = 1.f) {
+    if (c.n == 0) return default_out;
+    if (input <= c.x[0]) return c.y[0];
+    for (int i = 1; i < c.n; i++) {
+        if (c.x[i] > input) {
+            float range = c.x[i] - c.x[i - 1];
+            float diff = c.y[i] - c.y[i - 1];
+            float t = (input - c.x[i - 1]) / range;
+            return c.y[i - 1] + diff * t;
+        }
+    }
+    return c.y[c.n - 1];
+}
+
+#if defined(__HIPCC__)
+#define RSG_CONST_TABLE __constant__ const
+#else
+#define RSG_CONST_TABLE static const
+#endif
+
+// Curves are tiny; keep them as inline constexpr data usable on host and device.
+RSG_FN Curve curve_steer_angle() { return Curve{6, {0, 500, 1000, 1500, 1750, 3000}, {0.53356f, 0.31930f, 0.18203f, 0.10570f, 0.08507f, 0.03454f}}; }
+RSG_FN Curve curve_powerslide_steer() { return Curve{2, {0, 2500}, {0.39235f, 0.12610f}}; }
+RSG_FN Curve curve_drive_torque() { return Curve{3, {0, 1400, 1410}, {1.0f, 0.1f, 0.0f}}; }
+RSG_FN Curve curve_non_sticky() { return Curve{3, {0, 0.7075f, 1}, {0.1f, 0.5f, 1.0f}}; }
+RSG_FN Curve curve_lat_friction() { return Curve{2, {0, 1}, {1.0f, 0.2f}}; }
+RSG_FN Curve curve_long_friction() { return Curve{0, {0}, {0}}; }
+RSG_FN Curve curve_handbrake_lat() { return Curve{1, {0}, {0.1f}}; }
+RSG_FN Curve curve_handbrake_long() { return Curve{2, {0, 1}, {0.5f, 0.9f}}; }
+RSG_FN Curve curve_ball_extra_impulse() { return Curve{4, {0, 500.f, 2300.f, 4600.f}, {0.65f, 0.65f, 0.55f, 0.30f}}; }
+RSG_FN Curve curve_bump_ground() { return Curve{3, {0.f, 1400.f, 2200.f}, {(5.f / 6.f), 1100.f, 1530.f}}; }
+RSG_FN Curve curve_bump_air() { return Curve{3, {0.f, 1400.f, 2200.f}, {(5.f / 6.f), 1390.f, 1945.f}}; }
+RSG_FN Curve curve_bump_upward() { return Curve{3, {0.f, 1400.f, 2200.f}, {(2.f / 6.f), 278.f, 417.f}}; }
+
+// Car presets (CarConfig.cpp:20-100): hitbox size, hitbox offset, front/back
+// wheel radius, suspension rest length, connection point.
+struct PresetRaw {
+    float hitbox[3], offset[3];
+    float front_rad, back_rad, front_rest, back_rest;
+    float front_conn[3], back_conn[3];
+};
+
+RSG_FN PresetRaw preset_raw(int p) {
+    switch (p) {
+        default:
+        case 0: return PresetRaw{{120.507f, 86.6994f, 38.6591f}, {13.8757f, 0, 20.755f}, 12.50f, 15.00f, 38.755f, 37.055f, {51.25f, 25.90f, 20.755f}, {-33.75f, 29.50f, 20.755f}};
+        case 1: return PresetRaw{{130.427f, 85.7799f, 33.8f}, {9.f, 0, 15.75f}, 12.00f, 13.50f, 33.95f, 33.85f, {50.30f, 31.10f, 15.75f}, {-34.75f, 33.00f, 15.75f}};
+        case 2: return PresetRaw{{131.32f, 87.1704f, 31.8944f}, {9.00857f, 0, 12.0942f}, 12.50f, 17.00f, 31.9242f, 27.9242f, {49.97f, 27.80f, 12.0942f}, {-35.43f, 20.28f, 12.0942f}};
+        case 3: return PresetRaw{{133.992f, 83.021f, 32.8f}, {12.5f, 0, 11.75f}, 13.50f, 15.00f, 29.7f, 29.666f, {51.50f, 26.67f, 11.75f}, {-35.75f, 35.00f, 11.75f}};
+        case 4: return PresetRaw{{129.519f, 84.6879f, 36.6591f}, {13.8757f, 0, 20.755f}, 12.50f, 15.00f, 38.755f, 37.055f, {51.25f, 25.90f, 20.755f}, {-34.00f, 29.50f, 20.755f}};
+        case 5: return PresetRaw{{123.22f, 79.2103f, 44.1591f}, {11.3757f, 0, 21.505f}, 15.00f, 15.00f, 39.505f, 39.105f, {51.25f, 25.90f, 21.505f}, {-33.75f, 29.50f, 21.505f}};
+    }
+}
+
+// Derived per-preset physics values in Bullet units, computed the way the
+// reference's setup does (src/Sim/Car/Car.cpp:195-292,
+// BulletCollision/CollisionShapes/btBoxShape.cpp:17-45,
+// btConvexInternalShape.h:63-78, btCompoundShape.cpp:48-82,159-179).
+struct CarPhys {
+    V3 box_implicit;      // half extents minus the default margin
+    float box_margin;     // safe margin (<= 0.04)
+    V3 box_offset;        // child offset (hitboxPosOffset in m)
+    V3 local_aabb_half;   // compound local AABB half extents
+    V3 local_aabb_center; // compound local AABB centre
+    V3 inv_inertia_local;
+    float inv_mass;
+    // wheels: index 0,1 front (1 = left), 2,3 back (3 = left)
+    V3 wheel_conn[4];
+    float wheel_rest[4];  // suspension rest length (m)
+    float wheel_radius[4];
+    float wheel_force_scale[4];
+    float max_susp_travel; // m
+    float angular_motion_disc;
+};
+
+RSG_FN CarPhys make_car_phys(int preset) {
+    PresetRaw r = preset_raw(preset);
+    CarPhys c;
+    V3 size = v3(r.hitbox[0], r.hitbox[1], r.hitbox[2]);
+    V3 half = (size * UU_TO_BT) / 2.f;
+    c.box_implicit = half - v3(CONVEX_DISTANCE_MARGIN, CONVEX_DISTANCE_MARGIN, CONVEX_DISTANCE_MARGIN);
+    float mind = half.x;
+    int ax = 0;
+    // btVector3::minAxis: x<y ? (x<z ? 0 : 2) : (y<z ? 1 : 2)
+    ax = half.x < half.y ? (half.x < half.z ? 0 : 2) : (half.y < half.z ? 1 : 2);
+    mind = get(half, ax);
+    float safe = 0.1f * mind;
+    c.box_margin = safe < CONVEX_DISTANCE_MARGIN ? safe : CONVEX_DISTANCE_MARGIN;
+    c.box_offset = v3(r.offset[0], r.offset[1], r.offset[2]) * UU_TO_BT;
+    V3 he = c.box_implicit + v3(c.box_margin, c.box_margin, c.box_margin);
+    V3 lmin = c.box_offset - he, lmax = c.box_offset + he;
+    c.local_aabb_half = 0.5f * (lmax - lmin);
+    c.local_aabb_center = 0.5f * (lmax + lmin);
+    float lx = 2.f * he.x, ly = 2.f * he.y, lz = 2.f * he.z;
+    V3 inertia = v3(CAR_MASS_BT / (12.0f) * (ly * ly + lz * lz),
+                    CAR_MASS_BT / (12.0f) * (lx * lx + lz * lz),
+                    CAR_MASS_BT / (12.0f) * (lx * lx + ly * ly));
+    c.inv_inertia_local = v3(1.f / inertia.x, 1.f / inertia.y, 1.f / inertia.z);
+    c.inv_mass = 1.f / CAR_MASS_BT;
+    for (int i = 0; i < 4; i++) {
+        bool front = i < 2;
+        bool left = i % 2;
+        V3 conn = front ? v3(r.front_conn[0], r.front_conn[1], r.front_conn[2])
+                        : v3(r.back_conn[0], r.back_conn[1], r.back_conn[2]);
+        if (left) conn.y *= -1;
+        c.wheel_conn[i] = conn * UU_TO_BT;
+        float rest = front ? r.front_rest : r.back_rest;
+        rest -= MAX_SUSPENSION_TRAVEL;
+        c.wheel_rest[i] = rest * UU_TO_BT;
+        c.wheel_radius[i] = (front ? r.front_rad : r.back_rad) * UU_TO_BT;
+        c.wheel_force_scale[i] = front ? SUSPENSION_FORCE_SCALE_FRONT : SUSPENSION_FORCE_SCALE_BACK;
+    }
+    c.max_susp_travel = ((MAX_SUSPENSION_TRAVEL * UU_TO_BT) * 100) / 100;
+    // Bounding-sphere radius of the compound plus its centre offset
+    // (btCollisionShape::getAngularMotionDisc, btCollisionShape.cpp:151-157):
+    // computed from the compound's local AABB of the child box.
+    {
+        // btCollisionShape::getBoundingSphere on the identity-transformed
+        // compound AABB (compound margin is 0).
+        V3 amin = c.local_aabb_center - c.local_aabb_half;
+        V3 amax = c.local_aabb_center + c.local_aabb_half;
+        float radius = len(amax - amin) * 0.5f;
+        V3 center = (amin + amax) * 0.5f;
+        c.angular_motion_disc = radius + len(center);
+    }
+    return c;
+}
+
+// Ball (src/Sim/Ball/Ball.cpp:53-115, btSphereShape.cpp:37-59)
+struct BallPhys {
+    float radius;      // m
+    float inv_mass;
+    float inv_inertia;
+    float angular_motion_disc;
+};
+
+RSG_FN BallPhys make_ball_phys() {
+    BallPhys b;
+    b.radius = BALL_COLLISION_RADIUS_SOCCAR * UU_TO_BT;
+    b.inv_mass = 1.f / BALL_MASS_BT;
+    float elem = 0.4f * BALL_MASS_BT * b.radius * b.radius;
+    b.inv_inertia = 1.f / elem;
+    // RocketSim adds 0.08 to the sphere's bounding radius
+    // (BulletCollision/CollisionShapes/btCollisionShape.cpp:130-134).
+    b.angular_motion_disc = (float)((double)b.radius + 0.08);
+    return b;
+}
+
+// Car spawn tables (src/RLConst.h:388-427), blue side; orange is mirrored.
+struct SpawnPos { float x, y, yaw; };
+RSG_FN SpawnPos kickoff_spawn(int i) {
+    switch (i) {
+        default:
+        case 0: return SpawnPos{-2048, -2560, (float)(0.78539816339744830962 * 1)};
+        case 1: return SpawnPos{2048, -2560, (float)(0.78539816339744830962 * 3)};
+        case 2: return SpawnPos{-256, -3840, (float)(0.78539816339744830962 * 2)};
+        case 3: return SpawnPos{256, -3840, (float)(0.78539816339744830962 * 2)};
+        case 4: return SpawnPos{0, -4608, (float)(0.78539816339744830962 * 2)};
+    }
+}
+RSG_FN SpawnPos respawn_spawn(int i) {
+    switch (i) {
+        default:
+        case 0: return SpawnPos{-2304, -4608, (float)(PI_D / 2)};
+        case 1: return SpawnPos{-2688, -4608, (float)(PI_D / 2)};
+        case 2: return SpawnPos{2304, -4608, (float)(PI_D / 2)};
+        case 3: return SpawnPos{2688, -4608, (float)(PI_D / 2)};
+    }
+}
+
+// Boost pad locations (src/RLConst.h:264-309): 6 big then 28 small.
+RSG_FN V3 pad_location(int i) {
+    switch (i) {
+        case 0: return v3(-3584.f, 0.f, 73.f);
+        case 1: return v3(3584.f, 0.f, 73.f);
+        case 2: return v3(-3072.f, 4096.f, 73.f);
+        case 3: return v3(3072.f, 4096.f, 73.f);
+        case 4: return v3(-3072.f, -4096.f, 73.f);
+        case 5: return v3(3072.f, -4096.f, 73.f);
+        case 6: return v3(0.f, -4240.f, 70.f);
+        case 7: return v3(-1792.f, -4184.f, 70.f);
+        case 8: return v3(1792.f, -4184.f, 70.f);
+        case 9: return v3(-940.f, -3308.f, 70.f);
+        case 10: return v3(940.f, -3308.f, 70.f);
+        case 11: return v3(0.f, -2816.f, 70.f);
+        case 12: return v3(-3584.f, -2484.f, 70.f);
+        case 13: return v3(3584.f, -2484.f, 70.f);
+        case 14: return v3(-1788.f, -2300.f, 70.f);
+        case 15: return v3(1788.f, -2300.f, 70.f);
+        case 16: return v3(-2048.f, -1036.f, 70.f);
+        case 17: return v3(0.f, -1024.f, 70.f);
+        case 18: return v3(2048.f, -1036.f, 70.f);
+        case 19: return v3(-1024.f, 0.f, 70.f);
+        case 20: return v3(1024.f, 0.f, 70.f);
+        case 21: return v3(-2048.f, 1036.f, 70.f);
+        case 22: return v3(0.f, 1024.f, 70.f);
+        case 23: return v3(2048.f, 1036.f, 70.f);
+        case 24: return v3(-1788.f, 2300.f, 70.f);
+        case 25: return v3(1788.f, 2300.f, 70.f);
+        case 26: return v3(-3584.f, 2484.f, 70.f);
+        case 27: return v3(3584.f, 2484.f, 70.f);
+        case 28: return v3(0.f, 2816.f, 70.f);
+        case 29: return v3(-940.f, 3308.f, 70.f);
+        case 30: return v3(940.f, 3308.f, 70.f);
+        case 31: return v3(-1792.f, 4184.f, 70.f);
+        case 32: return v3(1792.f, 4184.f, 70.f);
+        default: return v3(0.f, 4240.f, 70.f);
+    }
+}
+
+}  // namespace rsg

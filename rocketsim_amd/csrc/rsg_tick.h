@@ -1,0 +1,511 @@
+// rsg_tick.h -- one arena tick (Arena::Step body, src/Sim/Arena/Arena.cpp:689-756)
+// over an arena held in registers / private memory, plus the gather/scatter
+// between that working copy and the struct-of-arrays state in HBM.
+#pragma once
+
+#include "rsg_sim.h"
+#include "rsg_collide.h"
+
+namespace rsg {
+
+// ---------------------------------------------------------------------------
+// gather / scatter
+
+RSG_FN float wf(uint32_t u) { return __builtin_bit_cast(float, u); }
+RSG_FN uint32_t fw(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+struct CarRef {
+    uint32_t* base;
+    int stride;  // words between consecutive word planes
+    RSG_FN uint32_t& u(int w) const { return base[(size_t)w * stride]; }
+    RSG_FN float f(int w) const { return wf(u(w)); }
+    RSG_FN void sf(int w, float v) const { u(w) = fw(v); }
+    RSG_FN V3 v(int w) const { return v3(f(w), f(w + 1), f(w + 2)); }
+    RSG_FN void sv(int w, V3 x) const { sf(w, x.x); sf(w + 1, x.y); sf(w + 2, x.z); }
+    RSG_FN uint64_t u64(int w) const { return (uint64_t)u(w) | ((uint64_t)u(w + 1) << 32); }
+    RSG_FN void su64(int w, uint64_t x) const { u(w) = (uint32_t)x; u(w + 1) = (uint32_t)(x >> 32); }
+};
+
+RSG_FN CarRef car_ref(const BatchDev& d, int arena, int slot) {
+    CarRef r;
+    r.base = d.car + (size_t)slot * d.n + arena;
+    r.stride = d.max_cars * d.n;
+    return r;
+}
+RSG_FN CarRef ball_ref(const BatchDev& d, int arena) { CarRef r; r.base = d.ball + arena; r.stride = d.n; return r; }
+RSG_FN CarRef pad_ref(const BatchDev& d, int arena, int pad) {
+    CarRef r; r.base = d.pad + (size_t)pad * d.n + arena; r.stride = PAD_NUM * d.n; return r;
+}
+RSG_FN CarRef arena_ref(const BatchDev& d, int arena) { CarRef r; r.base = d.arena + arena; r.stride = d.n; return r; }
+
+RSG_FN Ctrl load_ctrl(const CarRef& r, int w) {
+    Ctrl c;
+    c.throttle = r.f(w); c.steer = r.f(w + 1); c.pitch = r.f(w + 2); c.yaw = r.f(w + 3); c.roll = r.f(w + 4);
+    uint32_t b = r.u(w + 5);
+    c.jump = b & CTRL_JUMP; c.boost = b & CTRL_BOOST; c.handbrake = b & CTRL_HANDBRAKE;
+    return c;
+}
+RSG_FN void store_ctrl(const CarRef& r, int w, const Ctrl& c) {
+    r.sf(w, c.throttle); r.sf(w + 1, c.steer); r.sf(w + 2, c.pitch); r.sf(w + 3, c.yaw); r.sf(w + 4, c.roll);
+    r.u(w + 5) = (c.jump ? CTRL_JUMP : 0u) | (c.boost ? CTRL_BOOST : 0u) | (c.handbrake ? CTRL_HANDBRAKE : 0u);
+}
+
+RSG_FN M3 load_rot(const CarRef& r, int w) { return m3(r.v(w), r.v(w + 3), r.v(w + 6)); }
+RSG_FN void store_rot(const CarRef& r, int w, const M3& m) { r.sv(w, m.r[0]); r.sv(w + 3, m.r[1]); r.sv(w + 6, m.r[2]); }
+
+RSG_FN void load_car(const CarRef& r, Car& c) {
+    c.b.pos = r.v(CW_POS);
+    c.b.rot = load_rot(r, CW_ROT);
+    c.b.vel = r.v(CW_VEL);
+    c.b.ang = r.v(CW_ANGVEL);
+    c.flags = r.u(CW_FLAGS);
+    c.flip_torque = r.v(CW_FLIP_TORQUE);
+    c.jump_time = r.f(CW_JUMP_TIME);
+    c.flip_time = r.f(CW_FLIP_TIME);
+    c.air_time = r.f(CW_AIR_TIME);
+    c.air_time_since_jump = r.f(CW_AIR_TIME_SINCE_JUMP);
+    c.boost = r.f(CW_BOOST);
+    c.time_since_boosted = r.f(CW_TIME_SINCE_BOOSTED);
+    c.boosting_time = r.f(CW_BOOSTING_TIME);
+    c.supersonic_time = r.f(CW_SUPERSONIC_TIME);
+    c.handbrake = r.f(CW_HANDBRAKE_VAL);
+    c.af_timer = r.f(CW_AUTOFLIP_TIMER);
+    c.af_scale = r.f(CW_AUTOFLIP_SCALE);
+    c.world_normal = r.v(CW_WORLD_NORMAL);
+    c.other_id = r.u(CW_OTHER_CAR_ID);
+    c.car_cooldown = r.f(CW_CAR_COOLDOWN);
+    c.demo_timer = r.f(CW_DEMO_TIMER);
+    c.hit_rel_pos = r.v(CW_HIT_REL_POS);
+    c.hit_ball_pos = r.v(CW_HIT_BALL_POS);
+    c.hit_extra_vel = r.v(CW_HIT_EXTRA_VEL);
+    c.hit_tick = r.u64(CW_HIT_TICK);
+    c.hit_extra_tick = r.u64(CW_HIT_EXTRA_TICK);
+    c.ticks_since_set = r.u64(CW_TICKS_SINCE_SET);
+    c.last = load_ctrl(r, CW_LAST_CTRL);
+    c.engine = r.f(CW_ENGINE_FORCE);
+    c.brake = r.f(CW_BRAKE);
+    c.steer = r.f(CW_STEER);
+    for (int i = 0; i < 4; i++) {
+        c.lat[i] = r.f(CW_LAT_FRICTION + i);
+        c.lon[i] = r.f(CW_LONG_FRICTION + i);
+        c.push[i] = r.f(CW_PUSHBACK + i);
+    }
+    c.ctrl = load_ctrl(r, CW_CTRL);
+    c.gen_redraw = r.u(CW_GEN_REDRAW);
+    c.gen_left = r.u(CW_GEN_TICKS_LEFT);
+    uint32_t cfg = r.u(CW_CONFIG);
+    c.preset = cfg & 0xFF;
+    c.team = (cfg >> 8) & 0xFF;
+    c.id = r.u(CW_ID);
+    c.vel_cache = v3zero();
+}
+
+RSG_FN void store_car(const CarRef& r, const Car& c) {
+    r.sv(CW_POS, c.b.pos);
+    store_rot(r, CW_ROT, c.b.rot);
+    r.sv(CW_VEL, c.b.vel);
+    r.sv(CW_ANGVEL, c.b.ang);
+    r.u(CW_FLAGS) = c.flags;
+    r.sv(CW_FLIP_TORQUE, c.flip_torque);
+    r.sf(CW_JUMP_TIME, c.jump_time);
+    r.sf(CW_FLIP_TIME, c.flip_time);
+    r.sf(CW_AIR_TIME, c.air_time);
+    r.sf(CW_AIR_TIME_SINCE_JUMP, c.air_time_since_jump);
+    r.sf(CW_BOOST, c.boost);
+    r.sf(CW_TIME_SINCE_BOOSTED, c.time_since_boosted);
+    r.sf(CW_BOOSTING_TIME, c.boosting_time);
+    r.sf(CW_SUPERSONIC_TIME, c.supersonic_time);
+    r.sf(CW_HANDBRAKE_VAL, c.handbrake);
+    r.sf(CW_AUTOFLIP_TIMER, c.af_timer);
+    r.sf(CW_AUTOFLIP_SCALE, c.af_scale);
+    r.sv(CW_WORLD_NORMAL, c.world_normal);
+    r.u(CW_OTHER_CAR_ID) = c.other_id;
+    r.sf(CW_CAR_COOLDOWN, c.car_cooldown);
+    r.sf(CW_DEMO_TIMER, c.demo_timer);
+    r.sv(CW_HIT_REL_POS, c.hit_rel_pos);
+    r.sv(CW_HIT_BALL_POS, c.hit_ball_pos);
+    r.sv(CW_HIT_EXTRA_VEL, c.hit_extra_vel);
+    r.su64(CW_HIT_TICK, c.hit_tick);
+    r.su64(CW_HIT_EXTRA_TICK, c.hit_extra_tick);
+    r.su64(CW_TICKS_SINCE_SET, c.ticks_since_set);
+    store_ctrl(r, CW_LAST_CTRL, c.last);
+    r.sf(CW_ENGINE_FORCE, c.engine);
+    r.sf(CW_BRAKE, c.brake);
+    r.sf(CW_STEER, c.steer);
+    for (int i = 0; i < 4; i++) {
+        r.sf(CW_LAT_FRICTION + i, c.lat[i]);
+        r.sf(CW_LONG_FRICTION + i, c.lon[i]);
+        r.sf(CW_PUSHBACK + i, c.push[i]);
+    }
+    store_ctrl(r, CW_CTRL, c.ctrl);
+    r.u(CW_GEN_REDRAW) = c.gen_redraw;
+    r.u(CW_GEN_TICKS_LEFT) = c.gen_left;
+}
+
+RSG_FN void car_init_body(Car& c) {
+    CarPhys cp = make_car_phys(c.preset);
+    c.b.force = v3zero();
+    c.b.torque = v3zero();
+    c.b.inv_i_local = cp.inv_inertia_local;
+    c.b.inv_mass = cp.inv_mass;
+    c.b.no_rot = false;
+    c.b.active = true;
+    body_update_inertia(c.b);
+}
+
+RSG_FN void ball_init_body(Ball& b) {
+    BallPhys bp = make_ball_phys();
+    b.b.force = v3zero();
+    b.b.torque = v3zero();
+    b.b.inv_i_local = v3(bp.inv_inertia, bp.inv_inertia, bp.inv_inertia);
+    b.b.inv_mass = bp.inv_mass;
+    b.b.no_rot = true;
+    b.b.active = true;
+    body_update_inertia(b.b);
+    b.vel_cache = v3zero();
+}
+
+RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
+    CarRef ar = arena_ref(d, a);
+    A.tick = ar.u64(AW_TICK);
+    A.n_cars = (int)ar.u(AW_NUM_CARS);
+    A.goals_blue = ar.u(AW_GOALS_BLUE);
+    A.goals_orange = ar.u(AW_GOALS_ORANGE);
+    A.last_goal_tick = ar.u64(AW_LAST_GOAL_TICK);
+    A.resets = ar.u(AW_RESETS);
+    A.status = ar.u(AW_STATUS);
+    A.si.erp = 0.2f;
+    A.si.time_step = d.tick_time;
+    for (int i = 0; i < A.n_cars; i++) {
+        load_car(car_ref(d, a, i), A.cars[i]);
+        car_init_body(A.cars[i]);
+    }
+    CarRef br = ball_ref(d, a);
+    A.ball.b.pos = br.v(BW_POS);
+    A.ball.b.rot = load_rot(br, BW_ROT);
+    A.ball.b.vel = br.v(BW_VEL);
+    A.ball.b.ang = br.v(BW_ANGVEL);
+    A.ball.ticks_since_set = br.u64(BW_TICKS_SINCE_SET);
+    ball_init_body(A.ball);
+    if (d.mode == RSG_MODE_SOCCAR) {
+        for (int p = 0; p < PAD_NUM; p++) {
+            CarRef pr = pad_ref(d, a, p);
+            A.pads[p].active = pr.u(PW_ACTIVE) != 0;
+            A.pads[p].cooldown = pr.f(PW_COOLDOWN);
+            A.pads[p].prev_locked = pr.u(PW_PREV_LOCKED);
+            A.pads[p].cur_locked = -1;
+        }
+    }
+}
+
+RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
+    CarRef ar = arena_ref(d, a);
+    ar.su64(AW_TICK, A.tick);
+    ar.u(AW_GOALS_BLUE) = A.goals_blue;
+    ar.u(AW_GOALS_ORANGE) = A.goals_orange;
+    ar.su64(AW_LAST_GOAL_TICK, A.last_goal_tick);
+    ar.u(AW_RESETS) = A.resets;
+    ar.u(AW_STATUS) = A.status;
+    for (int i = 0; i < A.n_cars; i++) store_car(car_ref(d, a, i), A.cars[i]);
+    CarRef br = ball_ref(d, a);
+    br.sv(BW_POS, A.ball.b.pos);
+    store_rot(br, BW_ROT, A.ball.b.rot);
+    br.sv(BW_VEL, A.ball.b.vel);
+    br.sv(BW_ANGVEL, A.ball.b.ang);
+    br.su64(BW_TICKS_SINCE_SET, A.ball.ticks_since_set);
+    if (d.mode == RSG_MODE_SOCCAR) {
+        for (int p = 0; p < PAD_NUM; p++) {
+            CarRef pr = pad_ref(d, a, p);
+            pr.u(PW_ACTIVE) = A.pads[p].active ? 1u : 0u;
+            pr.sf(PW_COOLDOWN, A.pads[p].cooldown);
+            pr.u(PW_PREV_LOCKED) = A.pads[p].prev_locked;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kickoff reset (Arena::ResetToRandomKickoff, src/Sim/Arena/Arena.cpp:113-223)
+// with a counter-based shuffle of the 5 kickoff slots in place of the
+// reference's std::default_random_engine.  Cars are taken in slot order per
+// team.
+RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id) {
+    int order[5] = {0, 1, 2, 3, 4};
+    uint64_t h = rsg_draw(seed ^ 0x4B1C0FFull, arena_id, 0xFFFFu, A.resets, 7);
+    for (int i = 4; i > 0; i--) {
+        int j = (int)((h >> (i * 8)) % (uint64_t)(i + 1));
+        int t = order[i]; order[i] = order[j]; order[j] = t;
+    }
+    A.resets++;
+    int idx_team[2] = {0, 0};
+    int num_at_respawn[4] = {0, 0, 0, 0};
+    for (int ci = 0; ci < A.n_cars; ci++) {
+        Car& c = A.cars[ci];
+        int k = idx_team[c.team]++;
+        SpawnPos sp;
+        if (k < 5) {
+            sp = kickoff_spawn(order[k]);
+        } else {
+            int ri = (k - 5) % 5;
+            ri = ri < 4 ? ri : 3;
+            sp = respawn_spawn(ri);
+            sp.y += 250.f * num_at_respawn[ri];
+            num_at_respawn[ri]++;
+        }
+        car_respawn_state(c, 0, BOOST_SPAWN_AMOUNT);
+        V3 pos = v3(sp.x, sp.y, CAR_SPAWN_REST_Z);
+        float yaw = sp.yaw;
+        if (c.team != RSG_TEAM_BLUE) {
+            pos = pos * v3(-1, -1, 1);
+            yaw = (float)((double)yaw + PI_D);
+        }
+        c.b.pos = pos * UU_TO_BT;
+        c.b.rot = yaw_basis(yaw);
+        body_update_inertia(c.b);
+        c.flags = RSG_CF_ON_GROUND;
+    }
+    // BallState() then SetState (Ball.cpp:36-51)
+    A.ball.b.pos = v3(0, 0, BALL_REST_Z) * UU_TO_BT;
+    A.ball.b.rot = m3_identity();
+    A.ball.b.vel = v3zero();
+    A.ball.b.ang = v3zero();
+    A.ball.ticks_since_set = 0;
+    A.ball.vel_cache = v3zero();
+    body_update_inertia(A.ball.b);
+    for (int p = 0; p < PAD_NUM; p++) {
+        A.pads[p].active = true;
+        A.pads[p].cooldown = 0;
+        A.pads[p].prev_locked = 0;
+        A.pads[p].cur_locked = -1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boost pads (src/Sim/BoostPad/BoostPad.cpp, BoostPadGrid.cpp)
+
+RSG_FN bool pad_is_big(int p) { return p < PAD_NUM_BIG; }
+
+// BoostPadGrid::Add cell of each pad; the grid maps each occupied (x,y) cell
+// to one pad (src/Sim/BoostPad/BoostPadGrid/BoostPadGrid.cpp:27-41).
+RSG_FN int pad_cell_x(V3 p) { return (int)(p.x / PAD_GRID_CELL_SIZE_X + (PAD_GRID_CELLS_X / 2)); }
+RSG_FN int pad_cell_y(V3 p) { return (int)(p.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2)); }
+
+RSG_FN void pad_check_collide(Pad& pad, int p, const Car& c, int car_idx, const CarPhys& cp) {
+    V3 pos_uu = pad_location(p);
+    V3 pos_bt = pos_uu * UU_TO_BT;
+    V3 cpos = c.b.pos;
+    bool colliding = false;
+    if (pad.prev_locked == c.id) {
+        float box_rad = (pad_is_big(p) ? PAD_BOX_RAD_BIG : PAD_BOX_RAD_SMALL) * UU_TO_BT;
+        V3 bmin = pos_bt - v3(box_rad, box_rad, 0);
+        V3 bmax = pos_bt + v3(box_rad, box_rad, PAD_BOX_HEIGHT * UU_TO_BT);
+        // car->_rigidBody.getAabb: compound AABB without contact threshold
+        V3 center = c.b.rot * cp.local_aabb_center + c.b.pos;
+        M3 ab = c.b.rot;
+        for (int r = 0; r < 3; r++) ab.r[r] = v3(fabsf(ab.r[r].x), fabsf(ab.r[r].y), fabsf(ab.r[r].z));
+        V3 ext = ab * cp.local_aabb_half;
+        V3 cmin = center - ext, cmax = center + ext;
+        colliding = (bmax.x > cmin.x && bmax.y > cmin.y && bmax.z > cmin.z) &&
+                    (bmin.x < cmax.x && bmin.y < cmax.y && bmin.z < cmax.z);
+    } else {
+        float rad = (pad_is_big(p) ? PAD_CYL_RAD_BIG : PAD_CYL_RAD_SMALL) * UU_TO_BT;
+        float dx = cpos.x - pos_bt.x, dy = cpos.y - pos_bt.y;
+        if ((dx * dx + dy * dy) < (rad * rad))
+            colliding = fabsf(cpos.z - pos_bt.z) < (PAD_CYL_HEIGHT * UU_TO_BT);
+    }
+    if (colliding) pad.cur_locked = car_idx;
+}
+
+RSG_FN void pads_check_car(ArenaLocal& A, int ci) {
+    const Car& c = A.cars[ci];
+    if (flag(c.flags, RSG_CF_DEMOED) || c.boost >= BOOST_MAX) return;
+    V3 cp_uu = c.b.pos * BT_TO_UU;
+    if (cp_uu.z > PAD_GRID_EXTENT_Z) return;
+    int ix = (int)(cp_uu.x / PAD_GRID_CELL_SIZE_X + (PAD_GRID_CELLS_X / 2));
+    int iy = (int)(cp_uu.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2));
+    CarPhys cphys = make_car_phys(c.preset);
+    int x0 = ix - 1 > 0 ? ix - 1 : 0, x1 = ix + 1 < PAD_GRID_CELLS_X - 1 ? ix + 1 : PAD_GRID_CELLS_X - 1;
+    int y0 = iy - 1 > 0 ? iy - 1 : 0, y1 = iy + 1 < PAD_GRID_CELLS_Y - 1 ? iy + 1 : PAD_GRID_CELLS_Y - 1;
+    for (int i = x0; i <= x1; i++)
+        for (int j = y0; j <= y1; j++)
+            for (int p = 0; p < PAD_NUM; p++) {
+                V3 pl = pad_location(p);
+                if (pad_cell_x(pl) == i && pad_cell_y(pl) == j) pad_check_collide(A.pads[p], p, c, ci, cphys);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// The tick
+
+template <class MeshT>
+RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshT& mesh) {
+    const float dt = P.dt;
+    const bool soccar = P.mode == RSG_MODE_SOCCAR;
+
+    if (P.synth_controls) {
+        for (int i = 0; i < A.n_cars; i++) {
+            Car& c = A.cars[i];
+            rsg_ctrl_gen g = {c.gen_redraw, c.gen_left};
+            rsg_controls rc;
+            rc.throttle = c.ctrl.throttle; rc.steer = c.ctrl.steer; rc.pitch = c.ctrl.pitch;
+            rc.yaw = c.ctrl.yaw; rc.roll = c.ctrl.roll;
+            rc.jump = c.ctrl.jump; rc.boost = c.ctrl.boost; rc.handbrake = c.ctrl.handbrake;
+            rsg_ctrl_next(P.seed, P.arena_id, (uint32_t)i, &g, &rc);
+            c.gen_redraw = g.redraw; c.gen_left = g.ticks_left;
+            c.ctrl.throttle = rc.throttle; c.ctrl.steer = rc.steer; c.ctrl.pitch = rc.pitch;
+            c.ctrl.yaw = rc.yaw; c.ctrl.roll = rc.roll;
+            c.ctrl.jump = rc.jump; c.ctrl.boost = rc.boost; c.ctrl.handbrake = rc.handbrake;
+        }
+    }
+
+    // 1. ball sleep flag (Arena.cpp:694-700)
+    A.ball.b.active = !(len2(A.ball.b.vel) == 0.f && len2(A.ball.b.ang) == 0.f);
+
+    World<MeshT> world(A, mesh, soccar);
+
+    // 2. car pre-tick (Car.cpp:58-131)
+    for (int ci = 0; ci < A.n_cars; ci++) {
+        Car& c = A.cars[ci];
+        CarPhys cp = make_car_phys(c.preset);
+        c.ctrl.throttle = clampf(c.ctrl.throttle, -1, 1);
+        c.ctrl.steer = clampf(c.ctrl.steer, -1, 1);
+        c.ctrl.pitch = clampf(c.ctrl.pitch, -1, 1);
+        c.ctrl.yaw = clampf(c.ctrl.yaw, -1, 1);
+        c.ctrl.roll = clampf(c.ctrl.roll, -1, 1);
+
+        if (flag(c.flags, RSG_CF_DEMOED)) {
+            c.demo_timer = rs_max(c.demo_timer - dt, 0.f);
+            if (c.demo_timer == 0) {
+                int slot = (int)(rsg_draw(P.seed ^ 0xDE40ull, P.arena_id, (uint32_t)ci, (uint32_t)A.tick, 9) % 4u);
+                car_respawn_state(c, slot, BOOST_SPAWN_AMOUNT);
+            }
+            c.b.active = false;  // DISABLE_SIMULATION + no contact response this tick
+        } else {
+            c.b.active = true;
+        }
+        if (flag(c.flags, RSG_CF_DEMOED)) continue;
+
+        // updateVehicleFirst (btVehicleRL.cpp:215-228)
+        for (int i = 0; i < 4; i++) wheel_update_transform(c, cp, i, c.w[i]);
+        for (int i = 0; i < 4; i++) vehicle_ray_cast(c, cp, i, world, 1 + ci, A.si);
+        vehicle_calc_friction(c, world, dt);
+
+        bool jump_pressed = c.ctrl.jump && !c.last.jump;
+        int n_contact = 0;
+        for (int i = 0; i < 4; i++) {
+            setflag(c.flags, RSG_CF_WHEEL0 << i, c.w[i].in_contact);
+            n_contact += c.w[i].in_contact ? 1 : 0;
+        }
+        setflag(c.flags, RSG_CF_ON_GROUND, n_contact >= 3);
+        float fwd_speed_uu = dot(c.b.vel, car_forward(c)) * BT_TO_UU;
+        car_update_wheels(c, cp, dt, n_contact, fwd_speed_uu);
+        if (n_contact < 3)
+            car_update_air_torque(c, dt, n_contact == 0);
+        else
+            setflag(c.flags, RSG_CF_IS_FLIPPING, false);
+        car_update_jump(c, dt, jump_pressed);
+        car_update_auto_flip(c, dt, jump_pressed);
+        car_update_double_jump_or_flip(c, dt, jump_pressed, fwd_speed_uu);
+        if (c.ctrl.throttle != 0.f && ((n_contact > 0 && n_contact < 4) || flag(c.flags, RSG_CF_WORLD_CONTACT)))
+            car_update_auto_roll(c, n_contact);
+        setflag(c.flags, RSG_CF_WORLD_CONTACT, false);
+        vehicle_second(c, cp, dt);
+        car_update_boost(c, dt);
+    }
+
+    // 3. pads pre-tick (BoostPad.cpp:51-60)
+    if (soccar && A.n_cars > 0) {
+        for (int p = 0; p < PAD_NUM; p++) {
+            Pad& pad = A.pads[p];
+            if (pad.cooldown > 0) pad.cooldown = rs_max(pad.cooldown - dt, 0.f);
+            pad.active = (pad.cooldown == 0);
+            pad.cur_locked = -1;
+        }
+    }
+
+    // 4. world step (btDiscreteDynamicsWorld::stepSimulation)
+    world.step(dt, P);
+
+    // 5. cars post-tick (Car.cpp:133-193) + pad collisions
+    for (int ci = 0; ci < A.n_cars; ci++) {
+        Car& c = A.cars[ci];
+        if (!flag(c.flags, RSG_CF_DEMOED)) {
+            float s2 = len2(c.b.vel * BT_TO_UU);
+            bool ss = flag(c.flags, RSG_CF_SUPERSONIC);
+            if (ss && c.supersonic_time < SUPERSONIC_MAINTAIN_MAX_TIME)
+                ss = s2 >= SUPERSONIC_MAINTAIN_MIN_SPEED * SUPERSONIC_MAINTAIN_MIN_SPEED;
+            else
+                ss = s2 >= SUPERSONIC_START_SPEED * SUPERSONIC_START_SPEED;
+            setflag(c.flags, RSG_CF_SUPERSONIC, ss);
+            if (ss) c.supersonic_time += dt; else c.supersonic_time = 0;
+            if (c.car_cooldown > 0) c.car_cooldown = rs_max(c.car_cooldown - dt, 0.f);
+            c.last = c.ctrl;
+
+            // _FinishPhysicsTick
+            if (!is_zero(c.vel_cache)) {
+                c.b.vel += c.vel_cache;
+                c.vel_cache = v3zero();
+            }
+            V3 v = c.b.vel, w = c.b.ang;
+            if (len2(v) > (CAR_MAX_SPEED * UU_TO_BT) * (CAR_MAX_SPEED * UU_TO_BT))
+                v = normalized(v) * (CAR_MAX_SPEED * UU_TO_BT);
+            if (len2(w) > (CAR_MAX_ANG_SPEED * CAR_MAX_ANG_SPEED))
+                w = normalized(w) * CAR_MAX_ANG_SPEED;
+            c.b.vel = v;
+            c.b.ang = w;
+            c.ticks_since_set++;
+        }
+        if (soccar) pads_check_car(A, ci);
+    }
+
+    // 6. pads post-tick (BoostPad.cpp:88-107)
+    if (soccar && A.n_cars > 0) {
+        for (int p = 0; p < PAD_NUM; p++) {
+            Pad& pad = A.pads[p];
+            uint32_t locked_id = 0;
+            if (pad.cur_locked >= 0) {
+                Car& c = A.cars[pad.cur_locked];
+                locked_id = c.id;
+                if (pad.active) {
+                    float add = pad_is_big(p) ? PAD_BOOST_AMOUNT_BIG : PAD_BOOST_AMOUNT_SMALL;
+                    c.boost = rs_min(c.boost + add, BOOST_MAX);
+                    pad.active = false;
+                    pad.cooldown = pad_is_big(p) ? PAD_COOLDOWN_BIG : PAD_COOLDOWN_SMALL;
+                }
+            }
+            pad.prev_locked = locked_id;
+        }
+    }
+
+    // 7. ball finish (Ball.cpp:117-143)
+    {
+        Body& b = A.ball.b;
+        if (!is_zero(A.ball.vel_cache)) {
+            b.vel += A.ball.vel_cache;
+            A.ball.vel_cache = v3zero();
+        }
+        V3 v = b.vel, w = b.ang;
+        float max_bt = BALL_MAX_SPEED * UU_TO_BT;
+        if (len2(v) > max_bt * max_bt) v = normalized(v) * max_bt;
+        if (len2(w) > (BALL_MAX_ANG_SPEED * BALL_MAX_ANG_SPEED)) w = normalized(w) * BALL_MAX_ANG_SPEED;
+        b.vel = v;
+        b.ang = w;
+        A.ball.ticks_since_set++;
+    }
+
+    A.si.time_step = dt;
+
+    // 8. goal test (Arena.cpp:893-901) and optional kickoff reset
+    bool scored = false;
+    if (soccar) {
+        float y = A.ball.b.pos.y * BT_TO_UU;
+        if (fabsf(y) > (SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y + BALL_COLLISION_RADIUS_SOCCAR)) {
+            scored = true;
+            if (y > 0) A.goals_blue++; else A.goals_orange++;
+            A.last_goal_tick = A.tick;
+        }
+    }
+    A.tick++;
+    if (scored && P.auto_reset) arena_kickoff_reset(A, P.seed, P.arena_id);
+}
+
+}  // namespace rsg

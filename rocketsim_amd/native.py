@@ -1,0 +1,70 @@
+"""Loader for librsg.so (the HIP tick kernels behind include/rsg.h).
+
+There is no CPU fallback: if the library is missing or cannot be loaded, every
+entry point raises.
+"""
+import ctypes as C
+import os
+
+from .state import BallState, BatchDesc, CarState, Controls, PadState
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "librsg.so")
+
+_lib = None
+
+
+class NativeMissing(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeMissing(f"{LIB_PATH} is not built; run `python -m rocketsim_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.rsg_last_error.restype = C.c_char_p
+        L.rsg_init_mesh.argtypes = [C.c_void_p, C.c_uint64]
+        L.rsg_batch_create.restype = C.c_void_p
+        L.rsg_batch_create.argtypes = [C.POINTER(BatchDesc)]
+        L.rsg_batch_destroy.argtypes = [C.c_void_p]
+        for name in ("rsg_batch_num_arenas", "rsg_batch_num_cars"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.rsg_batch_car_offset.argtypes = [C.c_void_p, C.c_int]
+        L.rsg_batch_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.rsg_batch_sync.argtypes = [C.c_void_p]
+        L.rsg_batch_get_cars.argtypes = [C.c_void_p, C.POINTER(CarState)]
+        L.rsg_batch_set_cars.argtypes = [C.c_void_p, C.POINTER(CarState)]
+        L.rsg_batch_get_balls.argtypes = [C.c_void_p, C.POINTER(BallState)]
+        L.rsg_batch_set_balls.argtypes = [C.c_void_p, C.POINTER(BallState)]
+        L.rsg_batch_get_pads.argtypes = [C.c_void_p, C.POINTER(PadState)]
+        L.rsg_batch_set_pads.argtypes = [C.c_void_p, C.POINTER(PadState)]
+        L.rsg_batch_set_controls.argtypes = [C.c_void_p, C.POINTER(Controls)]
+        L.rsg_batch_get_controls.argtypes = [C.c_void_p, C.POINTER(Controls)]
+        L.rsg_batch_get_tick_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.rsg_batch_set_tick_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.rsg_batch_get_goals.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint64)]
+        L.rsg_batch_reset_kickoff.argtypes = [C.c_void_p, C.c_int, C.c_uint64]
+        L.rsg_batch_is_ball_scored.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        raise RuntimeError(lib().rsg_last_error().decode())
+    return rc
+
+
+# Every symbol include/rsg.h declares, for the ABI export test.
+EXPORTED = [
+    "rsg_last_error", "rsg_init_mesh", "rsg_batch_create", "rsg_batch_destroy",
+    "rsg_batch_num_arenas", "rsg_batch_num_cars", "rsg_batch_car_offset",
+    "rsg_batch_step", "rsg_batch_sync",
+    "rsg_batch_get_cars", "rsg_batch_set_cars", "rsg_batch_get_balls", "rsg_batch_set_balls",
+    "rsg_batch_get_pads", "rsg_batch_set_pads", "rsg_batch_set_controls", "rsg_batch_get_controls",
+    "rsg_batch_get_tick_counts", "rsg_batch_set_tick_counts", "rsg_batch_get_goals",
+    "rsg_batch_reset_kickoff", "rsg_batch_is_ball_scored",
+]

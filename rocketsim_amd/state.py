@@ -1,0 +1,98 @@
+"""ctypes mirrors of include/rsg_state.h: the state records of the C ABI.
+
+Field meanings follow the reference's public structs (CarState
+src/Sim/Car/Car.h:17-120, BallState src/Sim/Ball/Ball.h:18-57, BoostPadState
+src/Sim/BoostPad/BoostPad.h:23-31, CarControls src/Sim/CarControls.h:7-33),
+plus the per-wheel vehicle values carried between ticks
+(src/Sim/btVehicleRL/btVehicleRL.h:9-27).
+"""
+import ctypes as C
+
+MODE_SOCCAR = 0
+MODE_THE_VOID = 5
+
+PRESET_OCTANE, PRESET_DOMINUS, PRESET_PLANK, PRESET_BREAKOUT, PRESET_HYBRID, PRESET_MERC = range(6)
+PRESET_NAMES = ["octane", "dominus", "plank", "breakout", "hybrid", "merc"]
+TEAM_BLUE, TEAM_ORANGE = 0, 1
+NUM_PADS = 34
+MAX_CARS = 8
+
+# rsg_car_state.flags bits
+CF_ON_GROUND = 1 << 0
+CF_WHEEL = [1 << 1, 1 << 2, 1 << 3, 1 << 4]
+CF_HAS_JUMPED = 1 << 5
+CF_HAS_DOUBLE_JUMPED = 1 << 6
+CF_HAS_FLIPPED = 1 << 7
+CF_IS_FLIPPING = 1 << 8
+CF_IS_JUMPING = 1 << 9
+CF_IS_BOOSTING = 1 << 10
+CF_SUPERSONIC = 1 << 11
+CF_AUTO_FLIPPING = 1 << 12
+CF_WORLD_CONTACT = 1 << 13
+CF_DEMOED = 1 << 14
+CF_HIT_VALID = 1 << 15
+
+F3 = C.c_float * 3
+F4 = C.c_float * 4
+F9 = C.c_float * 9
+
+
+class Controls(C.Structure):
+    _fields_ = [
+        ("throttle", C.c_float), ("steer", C.c_float), ("pitch", C.c_float),
+        ("yaw", C.c_float), ("roll", C.c_float),
+        ("jump", C.c_uint8), ("boost", C.c_uint8), ("handbrake", C.c_uint8), ("_pad", C.c_uint8),
+    ]
+
+
+class CarState(C.Structure):
+    _fields_ = [
+        ("tick_count_since_update", C.c_uint64),
+        ("hit_tick", C.c_uint64),
+        ("hit_extra_tick", C.c_uint64),
+        ("pos", F3), ("rot", F9), ("vel", F3), ("ang_vel", F3),
+        ("flags", C.c_uint32),
+        ("flip_rel_torque", F3),
+        ("jump_time", C.c_float), ("flip_time", C.c_float),
+        ("air_time", C.c_float), ("air_time_since_jump", C.c_float),
+        ("boost", C.c_float), ("time_since_boosted", C.c_float), ("boosting_time", C.c_float),
+        ("supersonic_time", C.c_float), ("handbrake_val", C.c_float),
+        ("auto_flip_timer", C.c_float), ("auto_flip_torque_scale", C.c_float),
+        ("world_contact_normal", F3),
+        ("car_contact_other_id", C.c_uint32),
+        ("car_contact_cooldown", C.c_float),
+        ("demo_respawn_timer", C.c_float),
+        ("hit_rel_pos", F3), ("hit_ball_pos", F3), ("hit_extra_vel", F3),
+        ("last_controls", Controls),
+        ("wheel_engine_force", C.c_float), ("wheel_brake", C.c_float), ("wheel_steer", C.c_float),
+        ("wheel_lat_friction", F4), ("wheel_long_friction", F4), ("wheel_extra_pushback", F4),
+    ]
+
+
+class BallState(C.Structure):
+    _fields_ = [
+        ("tick_count_since_update", C.c_uint64),
+        ("pos", F3), ("rot", F9), ("vel", F3), ("ang_vel", F3),
+    ]
+
+
+class PadState(C.Structure):
+    _fields_ = [("is_active", C.c_uint32), ("cooldown", C.c_float), ("prev_locked_car_id", C.c_uint32)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [
+        ("n_arenas", C.c_int), ("mode", C.c_int), ("tick_rate", C.c_float),
+        ("seed", C.c_uint64), ("auto_reset", C.c_int), ("synth_controls", C.c_int),
+        ("cars_per_arena", C.POINTER(C.c_int)),
+        ("car_team", C.POINTER(C.c_uint8)),
+        ("car_preset", C.POINTER(C.c_uint8)),
+    ]
+
+
+IDENTITY_ROT = (1, 0, 0, 0, 1, 0, 0, 0, 1)
+
+
+def car_state_fields():
+    """Names of the continuous and discrete fields compared by the parity tests."""
+    return [f[0] for f in CarState._fields_]

@@ -1,0 +1,30 @@
+"""Writes the teacher-forced one-tick fixtures under tests/golden/ from the
+oracle (the reference RocketSim built from source, oracle/_ref/librsref.so).
+
+    python tests/golden/make_fixtures.py
+
+Each fixture holds inputs and expected outputs only (raw state records of
+include/rsg_state.h); GPU tests replay them when the oracle library is not
+present on the machine.
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from oracle import ref  # noqa: E402
+import parity  # noqa: E402
+
+
+def main():
+    ref.init([])
+    void = parity.sample_void(ref, n_traj=24, ticks_per_traj=8, seed=20261202)
+    parity.save_samples(os.path.join(HERE, "void_tf.npz"), void)
+    print("void_tf.npz:", len(void), "samples")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,286 @@
+"""Teacher-forced one-tick parity: sampling from the oracle and comparison.
+
+A sample is one tick of the reference Arena::Step (src/Sim/Arena/Arena.cpp:689-756)
+taken from an oracle trajectory: the full state before the tick (cars incl.
+the per-wheel values btVehicleRL keeps between ticks, ball, pads, tick count),
+the controls held during it, and the full state after it.  The GPU batch is
+set to the "before" state of many samples at once (one sample per arena),
+stepped once, and compared with the "after" state.
+
+Tolerances (north star: 1e-3 relative on position, velocity and angular
+velocity, 1e-4 on rotation; discrete state exact):
+  pos      |d| <= 1e-3 * max(|ref|, 1 uu)
+  vel      |d| <= 1e-3 * max(|ref|, 1 uu/s)
+  ang_vel  |d| <= 1e-3 * max(|ref|, 1e-2 rad/s)
+  rot      |d| <= 1e-4 (absolute, unit vectors)
+  other continuous fields (timers, boost, wheel values, hit info)
+           |d| <= 1e-3 * max(|ref|, 1)
+  flags, ids, tick counters, boolean controls: exact
+"""
+import ctypes as C
+import io
+import math
+import random
+
+import numpy as np
+
+from rocketsim_amd.state import (BallState, CarState, Controls, PadState, CF_HAS_JUMPED,
+                                 CF_ON_GROUND, MODE_SOCCAR, MODE_THE_VOID, NUM_PADS)
+
+EXACT_CAR = {"flags", "car_contact_other_id", "hit_tick", "hit_extra_tick", "tick_count_since_update",
+             "last_controls.jump", "last_controls.boost", "last_controls.handbrake", "last_controls._pad"}
+REL_FLOOR = {"pos": 1.0, "vel": 1.0, "ang_vel": 1e-2}
+ROT_TOL = 1e-4
+REL_TOL = 1e-3
+
+
+def flat_fields(s, prefix=""):
+    out = {}
+    for name, _ in s._fields_:
+        v = getattr(s, name)
+        if hasattr(v, "_fields_"):
+            out.update(flat_fields(v, prefix + name + "."))
+        elif hasattr(v, "__len__"):
+            out[prefix + name] = [float(x) for x in v]
+        else:
+            out[prefix + name] = v
+    return out
+
+
+def compare_car(ref, got):
+    """Return a list of (field, ref, got, err) mismatches."""
+    bad = []
+    fr, fg = flat_fields(ref), flat_fields(got)
+    for k, rv in fr.items():
+        gv = fg[k]
+        if k in EXACT_CAR:
+            if rv != gv:
+                bad.append((k, rv, gv, None))
+            continue
+        a = np.atleast_1d(np.asarray(rv, dtype=np.float64))
+        b = np.atleast_1d(np.asarray(gv, dtype=np.float64))
+        if k == "rot":
+            err = np.max(np.abs(a - b))
+            if not err <= ROT_TOL:
+                bad.append((k, rv, gv, err))
+            continue
+        floor = REL_FLOOR.get(k, 1.0)
+        err = np.max(np.abs(a - b) / np.maximum(np.abs(a), floor))
+        if not err <= REL_TOL:
+            bad.append((k, rv, gv, err))
+    return bad
+
+
+def compare_ball(ref, got):
+    bad = []
+    for k in ("pos", "vel", "ang_vel"):
+        a = np.asarray(getattr(ref, k)[:], dtype=np.float64)
+        b = np.asarray(getattr(got, k)[:], dtype=np.float64)
+        err = np.max(np.abs(a - b) / np.maximum(np.abs(a), REL_FLOOR[k]))
+        if not err <= REL_TOL:
+            bad.append(("ball." + k, list(a), list(b), err))
+    a = np.asarray(ref.rot[:]); b = np.asarray(got.rot[:])
+    if np.max(np.abs(a - b)) > ROT_TOL:
+        bad.append(("ball.rot", list(a), list(b), np.max(np.abs(a - b))))
+    if ref.tick_count_since_update != got.tick_count_since_update:
+        bad.append(("ball.tick_count_since_update", ref.tick_count_since_update, got.tick_count_since_update, None))
+    return bad
+
+
+def compare_pads(ref, got):
+    bad = []
+    for i, (r, g) in enumerate(zip(ref, got)):
+        if r.is_active != g.is_active or r.prev_locked_car_id != g.prev_locked_car_id:
+            bad.append((f"pad{i}", (r.is_active, r.prev_locked_car_id), (g.is_active, g.prev_locked_car_id), None))
+        if abs(r.cooldown - g.cooldown) > 1e-4 * max(1.0, abs(r.cooldown)):
+            bad.append((f"pad{i}.cooldown", r.cooldown, g.cooldown, abs(r.cooldown - g.cooldown)))
+    return bad
+
+
+class Sample:
+    __slots__ = ("mode", "teams", "presets", "cars_in", "ball_in", "pads_in", "ctrls", "tick_in",
+                 "cars_out", "ball_out", "pads_out", "tick_out")
+
+
+def _rand_quat(rng):
+    u1, u2, u3 = rng.random(), rng.random(), rng.random()
+    return (math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+            math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3))
+
+
+def quat_to_rotmat(q):
+    """Unit quaternion -> RotMat words (forward, right, up columns)."""
+    x, y, z, w = q
+    m = [[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+         [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+         [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]]
+    return [m[0][0], m[1][0], m[2][0], m[0][1], m[1][1], m[2][1], m[0][2], m[1][2], m[2][2]]
+
+
+def rand_controls(rng):
+    c = Controls()
+    for f in ("throttle", "steer", "pitch", "yaw", "roll"):
+        setattr(c, f, rng.uniform(-1, 1))
+    c.jump = rng.random() < 0.5
+    c.boost = rng.random() < 0.5
+    c.handbrake = rng.random() < 0.5
+    return c
+
+
+def sample_void(ref, n_traj, ticks_per_traj, seed):
+    """Airborne cars and ball flight in THE_VOID (no contacts): samples from
+    oracle trajectories with random controls redrawn every tick."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n_traj):
+        A = ref.RefArena(MODE_THE_VOID)
+        ncar = rng.randint(1, 4)
+        teams = [rng.randint(0, 1) for _ in range(ncar)]
+        presets = [rng.randint(0, 5) for _ in range(ncar)]
+        for i in range(ncar):
+            A.add_car(teams[i], presets[i])
+        for i in range(ncar):
+            s = A.get_car(i)
+            s.pos[:] = [i * 1000.0 + rng.uniform(-100, 100), rng.uniform(-100, 100), rng.uniform(800, 1800)]
+            s.rot[:] = quat_to_rotmat(_rand_quat(rng))
+            s.vel[:] = [rng.gauss(0, 500) for _ in range(3)]
+            s.ang_vel[:] = [rng.gauss(0, 2) for _ in range(3)]
+            s.boost = rng.uniform(0, 100)
+            s.flags = CF_HAS_JUMPED if rng.random() < 0.5 else 0
+            s.air_time_since_jump = rng.uniform(0, 2)
+            A.set_car(i, s)
+        b = A.get_ball()
+        b.pos[:] = [-3000, 0, 1000]
+        b.vel[:] = [rng.gauss(0, 300) for _ in range(3)]
+        b.ang_vel[:] = [rng.gauss(0, 1) for _ in range(3)]
+        A.set_ball(b)
+        for _ in range(ticks_per_traj):
+            out.append(_one_tick(A, MODE_THE_VOID, teams, presets, [rand_controls(rng) for _ in range(ncar)]))
+        A.close()
+    return out
+
+
+def _one_tick(A, mode, teams, presets, ctrls):
+    s = Sample()
+    s.mode = mode
+    s.teams = list(teams)
+    s.presets = list(presets)
+    n = len(teams)
+    s.cars_in = [A.get_car(i) for i in range(n)]
+    s.ball_in = A.get_ball()
+    s.pads_in = A.get_pads() if mode == MODE_SOCCAR else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
+    s.tick_in = A.tick_count
+    s.ctrls = [Controls.from_buffer_copy(c) for c in ctrls]
+    for i in range(n):
+        A.set_controls(i, ctrls[i])
+    A.step(1)
+    s.cars_out = [A.get_car(i) for i in range(n)]
+    s.ball_out = A.get_ball()
+    s.pads_out = A.get_pads() if mode == MODE_SOCCAR else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
+    s.tick_out = A.tick_count
+    return s
+
+
+# -- fixture (de)serialisation: raw struct bytes in an .npz ------------------
+
+def _pack(structs, cls):
+    return np.frombuffer(b"".join(bytes(x) for x in structs), dtype=np.uint8) if structs else np.zeros(0, np.uint8)
+
+
+def _unpack(arr, cls):
+    raw = arr.tobytes()
+    sz = C.sizeof(cls)
+    return [cls.from_buffer_copy(raw[i:i + sz]) for i in range(0, len(raw), sz)]
+
+
+def save_samples(path, samples):
+    ncars = np.array([len(s.teams) for s in samples], dtype=np.int32)
+    np.savez_compressed(
+        path,
+        mode=np.array([s.mode for s in samples], dtype=np.int32),
+        ncars=ncars,
+        teams=np.array(sum((s.teams for s in samples), []), dtype=np.uint8),
+        presets=np.array(sum((s.presets for s in samples), []), dtype=np.uint8),
+        cars_in=_pack(sum((s.cars_in for s in samples), []), CarState),
+        cars_out=_pack(sum((s.cars_out for s in samples), []), CarState),
+        ctrls=_pack(sum((s.ctrls for s in samples), []), Controls),
+        ball_in=_pack([s.ball_in for s in samples], BallState),
+        ball_out=_pack([s.ball_out for s in samples], BallState),
+        pads_in=_pack(sum((list(s.pads_in) for s in samples), []), PadState),
+        pads_out=_pack(sum((list(s.pads_out) for s in samples), []), PadState),
+        tick_in=np.array([s.tick_in for s in samples], dtype=np.uint64),
+        tick_out=np.array([s.tick_out for s in samples], dtype=np.uint64),
+    )
+
+
+def load_samples(path):
+    z = np.load(path, allow_pickle=False)
+    ncars = z["ncars"]
+    teams, presets = list(z["teams"]), list(z["presets"])
+    cars_in, cars_out = _unpack(z["cars_in"], CarState), _unpack(z["cars_out"], CarState)
+    ctrls = _unpack(z["ctrls"], Controls)
+    ball_in, ball_out = _unpack(z["ball_in"], BallState), _unpack(z["ball_out"], BallState)
+    pads_in, pads_out = _unpack(z["pads_in"], PadState), _unpack(z["pads_out"], PadState)
+    out = []
+    k = 0
+    for i, n in enumerate(ncars):
+        s = Sample()
+        s.mode = int(z["mode"][i])
+        s.teams = [int(t) for t in teams[k:k + n]]
+        s.presets = [int(p) for p in presets[k:k + n]]
+        s.cars_in, s.cars_out, s.ctrls = cars_in[k:k + n], cars_out[k:k + n], ctrls[k:k + n]
+        s.ball_in, s.ball_out = ball_in[i], ball_out[i]
+        s.pads_in = pads_in[i * NUM_PADS:(i + 1) * NUM_PADS]
+        s.pads_out = pads_out[i * NUM_PADS:(i + 1) * NUM_PADS]
+        s.tick_in, s.tick_out = int(z["tick_in"][i]), int(z["tick_out"][i])
+        out.append(s)
+        k += n
+    return out
+
+
+def run_on_gpu(samples, mode):
+    """Step every sample once on the GPU, one sample per arena."""
+    from rocketsim_amd.batch import ArenaBatch
+    cpa = [len(s.teams) for s in samples]
+    batch = ArenaBatch(cpa, sum((s.teams for s in samples), []), sum((s.presets for s in samples), []),
+                       mode=mode)
+    try:
+        batch.set_cars(sum((list(s.cars_in) for s in samples), []))
+        batch.set_balls([s.ball_in for s in samples])
+        batch.set_pads(sum((list(s.pads_in) for s in samples), []))
+        batch.set_controls(sum((list(s.ctrls) for s in samples), []))
+        batch.set_tick_counts([s.tick_in for s in samples])
+        batch.step(1)
+        cars = batch.get_cars()
+        balls = batch.get_balls()
+        pads = batch.get_pads()
+        ticks = batch.get_tick_counts()
+    finally:
+        batch.close()
+    results = []
+    k = 0
+    for i, s in enumerate(samples):
+        n = len(s.teams)
+        results.append((cars[k:k + n], balls[i], pads[i * NUM_PADS:(i + 1) * NUM_PADS], ticks[i]))
+        k += n
+    return results
+
+
+def check_samples(samples, results, max_report=10):
+    """Return (n_bad_samples, messages)."""
+    msgs = []
+    nbad = 0
+    for i, (s, (cars, ball, pads, tick)) in enumerate(zip(samples, results)):
+        bad = []
+        for j in range(len(s.teams)):
+            bad += [(f"car{j}." + b[0],) + b[1:] for b in compare_car(s.cars_out[j], cars[j])]
+        bad += compare_ball(s.ball_out, ball)
+        if s.mode == MODE_SOCCAR:
+            bad += compare_pads(s.pads_out, pads)
+        if tick != s.tick_out:
+            bad.append(("tick", s.tick_out, tick, None))
+        if bad:
+            nbad += 1
+            if len(msgs) < max_report:
+                msgs.append(f"sample {i}: " + "; ".join(f"{b[0]} ref={b[1]} got={b[2]} err={b[3]}" for b in bad[:6]))
+    return nbad, msgs
