@@ -18,6 +18,7 @@
 #include "RocketSim.h"
 #include "rsg_state.h"
 #include "rsg_workload.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/BroadphaseCollision/btRSBroadphase.h"
 
 #include <atomic>
 #include <chrono>
@@ -331,6 +332,89 @@ double rsref_run_workload(int threads, int arenas_per_thread, int n_blue, int n_
     auto t1 = *std::max_element(finish.begin(), finish.end());
     *arena_ticks = total.load();
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+
+// Debug view of the last tick's contact manifolds, in the dispatcher's array
+// order.  Body codes: 0 = ball, 1 + i = car i (creation order), -1 - s =
+// static object s (_worldCollisionRBs order: meshes, then planes).
+struct rsref_point {
+    float pos_a[3], pos_b[3], normal[3];
+    float dist, friction, restitution, applied_impulse;
+    int32_t special;
+    int32_t part0, index0, part1, index1;
+};
+
+static int body_code(RefArena* ra, const btCollisionObject* o) {
+    if (o == &ra->arena->ball->_rigidBody) return 0;
+    for (size_t i = 0; i < ra->cars.size(); i++)
+        if (o == &ra->cars[i]->_rigidBody) return 1 + (int)i;
+    auto& st = ra->arena->_worldCollisionRBs;
+    for (size_t i = 0; i < st.size(); i++)
+        if (o == st[i]) return -1 - (int)i;
+    return -1000;
+}
+
+int rsref_num_manifolds(void* h) {
+    auto* ra = (RefArena*)h;
+    return ra->arena->_bulletWorldParams.collisionDispatcher.getNumManifolds();
+}
+
+// Returns the number of points; writes body codes and up to 4 points.
+int rsref_get_manifold(void* h, int m, int* body0, int* body1, float* threshold, rsref_point* pts) {
+    auto* ra = (RefArena*)h;
+    btPersistentManifold* mf = ra->arena->_bulletWorldParams.collisionDispatcher.getManifoldByIndexInternal(m);
+    *body0 = body_code(ra, mf->getBody0());
+    *body1 = body_code(ra, mf->getBody1());
+    *threshold = mf->getContactBreakingThreshold();
+    int n = mf->getNumContacts();
+    for (int i = 0; i < n; i++) {
+        const btManifoldPoint& cp = mf->getContactPoint(i);
+        rsref_point& p = pts[i];
+        for (int k = 0; k < 3; k++) {
+            p.pos_a[k] = cp.m_positionWorldOnA[k];
+            p.pos_b[k] = cp.m_positionWorldOnB[k];
+            p.normal[k] = cp.m_normalWorldOnB[k];
+        }
+        p.dist = cp.m_distance1;
+        p.friction = cp.m_combinedFriction;
+        p.restitution = cp.m_combinedRestitution;
+        p.applied_impulse = cp.m_appliedImpulse;
+        p.special = cp.m_isSpecial;
+        p.part0 = cp.m_partId0; p.index0 = cp.m_index0;
+        p.part1 = cp.m_partId1; p.index1 = cp.m_index1;
+    }
+    return n;
+}
+
+// Broadphase cell (i, j, k) of a dynamic body's proxy, and its position in
+// the proxy's own cell list (dynHandles order).
+int rsref_get_proxy_cell(void* h, int body, int* ijk, int* order_in_cell, int* list_len) {
+    auto* ra = (RefArena*)h;
+    btCollisionObject* o = body == 0 ? (btCollisionObject*)&ra->arena->ball->_rigidBody
+                                     : (btCollisionObject*)&ra->cars[body - 1]->_rigidBody;
+    auto* bp = (btRSBroadphase*)ra->arena->_bulletWorldParams.broadphase;
+    auto* px = (btRSBroadphaseProxy*)o->getBroadphaseHandle();
+    ijk[0] = px->iIdx; ijk[1] = px->jIdx; ijk[2] = px->kIdx;
+    auto& cell = bp->cells[px->cellIdx];
+    *list_len = (int)cell.dynHandles.size();
+    *order_in_cell = -1;
+    for (size_t i = 0; i < cell.dynHandles.size(); i++)
+        if (cell.dynHandles[i] == px) *order_in_cell = (int)i;
+    return px->cellIdx;
+}
+
+// dynHandles of a given cell as body codes.
+int rsref_get_cell_list(void* h, int cell_idx, int* out, int max_out) {
+    auto* ra = (RefArena*)h;
+    auto* bp = (btRSBroadphase*)ra->arena->_bulletWorldParams.broadphase;
+    auto& cell = bp->cells[cell_idx];
+    int n = 0;
+    for (auto* px : cell.dynHandles) {
+        if (n < max_out) out[n] = body_code(ra, (const btCollisionObject*)px->m_clientObject);
+        n++;
+    }
+    return n;
 }
 
 }  // extern "C"
