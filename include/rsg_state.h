@@ -93,6 +93,11 @@ typedef struct rsg_car_state {
     float wheel_engine_force, wheel_brake, wheel_steer;
     float wheel_lat_friction[4], wheel_long_friction[4];
     float wheel_extra_pushback[4];
+    /* broadphase registration of the car's proxy (btRSBroadphase): the cell
+     * index it is registered in and its registration order among the arena's
+     * dynamic proxies; only the relative order of bp_seq values matters */
+    int32_t bp_cell;
+    uint32_t bp_seq;
 } rsg_car_state;
 
 typedef struct rsg_ball_state {
@@ -101,6 +106,8 @@ typedef struct rsg_ball_state {
     float rot[9];
     float vel[3];
     float ang_vel[3];
+    int32_t bp_cell;                /* as in rsg_car_state */
+    uint32_t bp_seq;
 } rsg_ball_state;
 
 typedef struct rsg_pad_state {

@@ -19,6 +19,7 @@
 #include "rsg_state.h"
 #include "rsg_workload.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/BroadphaseCollision/btRSBroadphase.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btBvhTriangleMeshShape.h"
 
 #include <atomic>
 #include <chrono>
@@ -44,7 +45,35 @@ const CarConfig* preset_config(int p) {
 struct RefArena {
     Arena* arena = nullptr;
     std::vector<Car*> cars;  // creation order
+    // Broadphase registration bookkeeping of the dynamic proxies (0 = ball,
+    // 1 + i = car i): the cell each proxy is registered in, and the order of
+    // its last (re-)registration, which is the order of the cell lists.
+    std::vector<int> bp_cell;
+    std::vector<uint32_t> bp_seq;
+    uint32_t bp_counter = 0;
 };
+
+btCollisionObject* body_object(RefArena* ra, int body) {
+    return body == 0 ? (btCollisionObject*)&ra->arena->ball->_rigidBody
+                     : (btCollisionObject*)&ra->cars[body - 1]->_rigidBody;
+}
+
+int proxy_cell(RefArena* ra, int body) {
+    return ((btRSBroadphaseProxy*)body_object(ra, body)->getBroadphaseHandle())->cellIdx;
+}
+
+// After a tick: a proxy whose cell changed was re-registered (btRSBroadphase::setAabb,
+// which runs once per proxy per tick in collision-object order).
+void track_bp(RefArena* ra) {
+    int nb = (int)ra->bp_cell.size();
+    for (int i = 0; i < nb; i++) {
+        int c = proxy_cell(ra, i);
+        if (c != ra->bp_cell[i]) {
+            ra->bp_cell[i] = c;
+            if (nb > 1) ra->bp_seq[i] = ++ra->bp_counter;
+        }
+    }
+}
 
 thread_local std::string g_err;
 
@@ -97,6 +126,8 @@ void* rsref_arena_create(int mode, float tick_rate) {
         // (btVehicleRL.cpp:186-194).  Start it at the arena's tick time so a
         // freshly created arena behaves like one that has already stepped.
         ra->arena->_bulletWorld.getSolverInfo().m_timeStep = ra->arena->tickTime;
+        ra->bp_cell.push_back(proxy_cell(ra, 0));
+        ra->bp_seq.push_back(0);
         return ra;
     } catch (std::exception& e) {
         g_err = e.what();
@@ -118,6 +149,8 @@ uint32_t rsref_add_car(void* h, int team, int preset) {
     if (!cfg) { g_err = "bad preset"; return 0; }
     Car* c = ra->arena->AddCar((Team)team, *cfg);
     ra->cars.push_back(c);
+    ra->bp_cell.push_back(proxy_cell(ra, (int)ra->cars.size()));
+    ra->bp_seq.push_back(++ra->bp_counter);
     return c->id;
 }
 
@@ -174,6 +207,9 @@ void rsref_get_car(void* h, int i, rsg_car_state* s) {
         s->wheel_long_friction[w] = wi[w].m_longFriction;
         s->wheel_extra_pushback[w] = wi[w].m_extraPushback;
     }
+    auto* ra = (RefArena*)h;
+    s->bp_cell = ra->bp_cell[1 + i];
+    s->bp_seq = ra->bp_seq[1 + i];
 }
 
 void rsref_set_car(void* h, int i, const rsg_car_state* s) {
@@ -228,6 +264,9 @@ void rsref_get_ball(void* h, rsg_ball_state* s) {
     std::memset(s, 0, sizeof(*s));
     s->tick_count_since_update = bs.tickCountSinceUpdate;
     vget(bs.pos, s->pos); rget(bs.rotMat, s->rot); vget(bs.vel, s->vel); vget(bs.angVel, s->ang_vel);
+    auto* ra = (RefArena*)h;
+    s->bp_cell = ra->bp_cell[0];
+    s->bp_seq = ra->bp_seq[0];
 }
 
 void rsref_set_ball(void* h, const rsg_ball_state* s) {
@@ -256,7 +295,11 @@ void rsref_set_tick_count(void* h, uint64_t t) { ((RefArena*)h)->arena->tickCoun
 
 int rsref_step(void* h, int ticks) {
     try {
-        ((RefArena*)h)->arena->Step(ticks);
+        auto* ra = (RefArena*)h;
+        for (int t = 0; t < ticks; t++) {
+            ra->arena->Step(1);
+            track_bp(ra);
+        }
         return 0;
     } catch (std::exception& e) {
         g_err = e.what();
@@ -415,6 +458,135 @@ int rsref_get_cell_list(void* h, int cell_idx, int* out, int max_out) {
         n++;
     }
     return n;
+}
+
+
+// Triangle indices the reference's mesh BVH reports, in report order, for an
+// AABB query (processAllTriangles) and for a ray query (performRaycast).
+namespace {
+struct RecordTris : public btTriangleCallback {
+    int* out; int max_out; int n = 0;
+    void processTriangle(btVector3*, int, int tri) override {
+        if (n < max_out) out[n] = tri;
+        n++;
+    }
+};
+}
+
+int rsref_mesh_query_aabb(const float* mn, const float* mx, int* out, int max_out) {
+    auto& meshes = RocketSim::GetArenaCollisionShapes(GameMode::SOCCAR);
+    if (meshes.empty()) return -1;
+    RecordTris cb; cb.out = out; cb.max_out = max_out;
+    meshes[0]->processAllTriangles(&cb, btVector3(mn[0], mn[1], mn[2]), btVector3(mx[0], mx[1], mx[2]));
+    return cb.n;
+}
+
+int rsref_mesh_query_ray(const float* from, const float* to, int* out, int max_out) {
+    auto& meshes = RocketSim::GetArenaCollisionShapes(GameMode::SOCCAR);
+    if (meshes.empty()) return -1;
+    RecordTris cb; cb.out = out; cb.max_out = max_out;
+    meshes[0]->performRaycast(&cb, btVector3(from[0], from[1], from[2]), btVector3(to[0], to[1], to[2]));
+    return cb.n;
+}
+
+// Internal-edge info of mesh triangle `tri` (btTriangleInfoMap entry): returns
+// 0 and writes the three edge angles (v0v1, v1v2, v2v0) and flags, or -1 when
+// the triangle has no entry.
+int rsref_tri_info(int tri, float* angles, uint32_t* flags) {
+    auto& meshes = RocketSim::GetArenaCollisionShapes(GameMode::SOCCAR);
+    if (meshes.empty()) return -2;
+    const btTriangleInfoMap* map = meshes[0]->getTriangleInfoMap();
+    if (!map) return -2;
+    int hash = (0 << (31 - MAX_NUM_PARTS_IN_BITS)) | tri;
+    const btTriangleInfo* info = map->find(hash);
+    if (!info) return -1;
+    angles[0] = info->m_edgeV0V1Angle;
+    angles[1] = info->m_edgeV1V2Angle;
+    angles[2] = info->m_edgeV2V0Angle;
+    *flags = (uint32_t)info->m_flags;
+    return 0;
+}
+
+// Static objects registered in a broadphase cell, as a bit mask over
+// _worldCollisionRBs order, and their list order packed 4 bits each.
+int rsref_cell_statics(void* h, int cell_idx, uint32_t* order_packed) {
+    auto* ra = (RefArena*)h;
+    auto* bp = (btRSBroadphase*)ra->arena->_bulletWorldParams.broadphase;
+    auto& cell = bp->cells[cell_idx];
+    int mask = 0;
+    uint32_t packed = 0;
+    int n = 0;
+    for (auto* px : cell.staticHandles) {
+        int code = body_code(ra, (const btCollisionObject*)px->m_clientObject);
+        int s = -1 - code;
+        mask |= 1 << s;
+        if (n < 8) packed |= (uint32_t)s << (4 * n);
+        n++;
+    }
+    *order_packed = packed;
+    return mask;
+}
+
+// The reference's own triangle test for broadphase cell (i, j, k).
+int rsref_cell_has_tris(void* h, int i, int j, int k) {
+    auto* ra = (RefArena*)h;
+    auto* bp = (btRSBroadphase*)ra->arena->_bulletWorldParams.broadphase;
+    auto& meshes = RocketSim::GetArenaCollisionShapes(GameMode::SOCCAR);
+    btVector3 cmin = bp->GetCellMinPos(i, j, k);
+    btVector3 cmax = cmin + btVector3(bp->cellSize, bp->cellSize, bp->cellSize);
+    int out[1];
+    RecordTris cb; cb.out = out; cb.max_out = 0;
+    meshes[0]->processAllTriangles(&cb, cmin, cmax);
+    return cb.n;
+}
+
+// Fresh broadphase with one static proxy for static object s at the given
+// AABB; returns whether that proxy is registered in cell `cell_idx`.
+int rsref_fresh_static_cells(void* h, int s, const float* mn, const float* mx, uint8_t* out_cells) {
+    auto* ra = (RefArena*)h;
+    auto* bpo = (btRSBroadphase*)ra->arena->_bulletWorldParams.broadphase;
+    btHashedOverlappingPairCache cache;
+    btRSBroadphase bp(bpo->minPos, bpo->maxPos, bpo->cellSize, &cache, 16);
+    btCollisionObject* o = ra->arena->_worldCollisionRBs[s];
+    btBroadphaseProxy* px = bp.createProxy(btVector3(mn[0], mn[1], mn[2]), btVector3(mx[0], mx[1], mx[2]),
+                                           o->getCollisionShape()->getShapeType(), o, 2, -3, nullptr);
+    for (int c = 0; c < bp.totalCells; c++) {
+        out_cells[c] = 0;
+        for (auto* q : bp.cells[c].staticHandles) if (q == px) out_cells[c] = 1;
+    }
+    return bp.totalCells;
+}
+
+int rsref_debug_add_loop(void* h, const float* mn, const float* mx, int* out, int max_out) {
+    auto* ra = (RefArena*)h;
+    auto* bp = (btRSBroadphase*)ra->arena->_bulletWorldParams.broadphase;
+    btVector3 aabbMin(mn[0], mn[1], mn[2]), aabbMax(mx[0], mx[1], mx[2]);
+    for (int i = 0; i < 3; i++) aabbMax[i] = btMin(aabbMax[i], bp->maxPos[i]);
+    int iMin, jMin, kMin, iMax, jMax, kMax;
+    bp->GetCellIndices(aabbMin, iMin, jMin, kMin);
+    bp->GetCellIndices(aabbMax, iMax, jMax, kMax);
+    auto& meshes = RocketSim::GetArenaCollisionShapes(GameMode::SOCCAR);
+    auto* tm = (btTriangleMeshShape*)meshes[0];
+    int n = 0;
+    out[n++] = iMin; out[n++] = jMin; out[n++] = kMin; out[n++] = iMax; out[n++] = jMax; out[n++] = kMax;
+    for (int i = iMin; i <= iMax; i++)
+        for (int j = jMin; j <= jMax; j++)
+            for (int k = kMin; k <= kMax; k++) {
+                btVector3 cellMin = bp->GetCellMinPos(i, j, k);
+                btVector3 cellMax = cellMin + btVector3(bp->cellSize, bp->cellSize, bp->cellSize);
+                int o[1];
+                RecordTris cb; cb.out = o; cb.max_out = 0;
+                tm->processAllTriangles(&cb, cellMin, cellMax);
+                if (n + 4 <= max_out) { out[n++] = i; out[n++] = j; out[n++] = k; out[n++] = cb.n; }
+            }
+    return n;
+}
+
+// Broadphase proxy AABB of static object s (_worldCollisionRBs order).
+void rsref_static_aabb(void* h, int s, float* mn, float* mx) {
+    auto* ra = (RefArena*)h;
+    auto* px = (btRSBroadphaseProxy*)ra->arena->_worldCollisionRBs[s]->getBroadphaseHandle();
+    for (int k = 0; k < 3; k++) { mn[k] = px->m_aabbMin[k]; mx[k] = px->m_aabbMax[k]; }
 }
 
 }  // extern "C"

@@ -1,0 +1,149 @@
+// rsg_api_convert.h -- conversion between the C ABI state records
+// (include/rsg_state.h, public units: uu, uu/s, column-major RotMat) and the
+// word-plane records in memory (Bullet units, basis rows), as the reference's
+// Car/Ball Get/SetState convert (src/Sim/Car/Car.cpp:10-36,
+// src/Sim/Ball/Ball.cpp:27-51).  Host code only.
+#pragma once
+
+#include <cstring>
+
+#include "../../include/rsg_state.h"
+#include "rsg_tick.h"
+
+namespace rsg {
+
+inline void rot_api_to_rows(const float* api, M3& m) {
+    // api: forward, right, up columns (RotMat, src/Math/MathTypes/MathTypes.h:169-170)
+    for (int r = 0; r < 3; r++) m.r[r] = v3(api[0 + r], api[3 + r], api[6 + r]);
+}
+inline void rot_rows_to_api(const M3& m, float* api) {
+    for (int r = 0; r < 3; r++) {
+        api[0 + r] = m.r[r].x;
+        api[3 + r] = m.r[r].y;
+        api[6 + r] = m.r[r].z;
+    }
+}
+
+inline Ctrl ctrl_from_api(const rsg_controls& c) {
+    Ctrl o;
+    o.throttle = c.throttle; o.steer = c.steer; o.pitch = c.pitch; o.yaw = c.yaw; o.roll = c.roll;
+    o.jump = c.jump != 0; o.boost = c.boost != 0; o.handbrake = c.handbrake != 0;
+    return o;
+}
+inline rsg_controls ctrl_to_api(const Ctrl& c) {
+    rsg_controls o;
+    o.throttle = c.throttle; o.steer = c.steer; o.pitch = c.pitch; o.yaw = c.yaw; o.roll = c.roll;
+    o.jump = c.jump; o.boost = c.boost; o.handbrake = c.handbrake; o._pad = 0;
+    return o;
+}
+
+inline void car_to_api(const CarRef& r, rsg_car_state& s) {
+    Car c;
+    load_car(r, c);
+    std::memset(&s, 0, sizeof(s));
+    s.tick_count_since_update = c.ticks_since_set;
+    s.hit_tick = c.hit_tick;
+    s.hit_extra_tick = c.hit_extra_tick;
+    V3 p = c.b.pos * BT_TO_UU, v = c.b.vel * BT_TO_UU;
+    s.pos[0] = p.x; s.pos[1] = p.y; s.pos[2] = p.z;
+    rot_rows_to_api(c.b.rot, s.rot);
+    s.vel[0] = v.x; s.vel[1] = v.y; s.vel[2] = v.z;
+    s.ang_vel[0] = c.b.ang.x; s.ang_vel[1] = c.b.ang.y; s.ang_vel[2] = c.b.ang.z;
+    s.flags = c.flags;
+    s.flip_rel_torque[0] = c.flip_torque.x; s.flip_rel_torque[1] = c.flip_torque.y; s.flip_rel_torque[2] = c.flip_torque.z;
+    s.jump_time = c.jump_time; s.flip_time = c.flip_time;
+    s.air_time = c.air_time; s.air_time_since_jump = c.air_time_since_jump;
+    s.boost = c.boost; s.time_since_boosted = c.time_since_boosted; s.boosting_time = c.boosting_time;
+    s.supersonic_time = c.supersonic_time; s.handbrake_val = c.handbrake;
+    s.auto_flip_timer = c.af_timer; s.auto_flip_torque_scale = c.af_scale;
+    s.world_contact_normal[0] = c.world_normal.x; s.world_contact_normal[1] = c.world_normal.y; s.world_contact_normal[2] = c.world_normal.z;
+    s.car_contact_other_id = c.other_id;
+    s.car_contact_cooldown = c.car_cooldown;
+    s.demo_respawn_timer = c.demo_timer;
+    s.hit_rel_pos[0] = c.hit_rel_pos.x; s.hit_rel_pos[1] = c.hit_rel_pos.y; s.hit_rel_pos[2] = c.hit_rel_pos.z;
+    s.hit_ball_pos[0] = c.hit_ball_pos.x; s.hit_ball_pos[1] = c.hit_ball_pos.y; s.hit_ball_pos[2] = c.hit_ball_pos.z;
+    s.hit_extra_vel[0] = c.hit_extra_vel.x; s.hit_extra_vel[1] = c.hit_extra_vel.y; s.hit_extra_vel[2] = c.hit_extra_vel.z;
+    s.last_controls = ctrl_to_api(c.last);
+    s.wheel_engine_force = c.engine; s.wheel_brake = c.brake; s.wheel_steer = c.steer;
+    for (int w = 0; w < 4; w++) {
+        s.wheel_lat_friction[w] = c.lat[w];
+        s.wheel_long_friction[w] = c.lon[w];
+        s.wheel_extra_pushback[w] = c.push[w];
+    }
+    s.bp_cell = (int32_t)r.u(CW_BP_CELL);
+    s.bp_seq = r.u(CW_BP_SEQ);
+}
+
+inline void car_from_api(const CarRef& r, const rsg_car_state& s) {
+    Car c;
+    load_car(r, c);
+    c.b.pos = v3(s.pos[0], s.pos[1], s.pos[2]) * UU_TO_BT;
+    rot_api_to_rows(s.rot, c.b.rot);
+    c.b.vel = v3(s.vel[0], s.vel[1], s.vel[2]) * UU_TO_BT;
+    c.b.ang = v3(s.ang_vel[0], s.ang_vel[1], s.ang_vel[2]);
+    c.flags = s.flags;
+    c.flip_torque = v3(s.flip_rel_torque[0], s.flip_rel_torque[1], s.flip_rel_torque[2]);
+    c.jump_time = s.jump_time; c.flip_time = s.flip_time;
+    c.air_time = s.air_time; c.air_time_since_jump = s.air_time_since_jump;
+    c.boost = s.boost; c.time_since_boosted = s.time_since_boosted; c.boosting_time = s.boosting_time;
+    c.supersonic_time = s.supersonic_time; c.handbrake = s.handbrake_val;
+    c.af_timer = s.auto_flip_timer; c.af_scale = s.auto_flip_torque_scale;
+    c.world_normal = v3(s.world_contact_normal[0], s.world_contact_normal[1], s.world_contact_normal[2]);
+    c.other_id = s.car_contact_other_id;
+    c.car_cooldown = s.car_contact_cooldown;
+    c.demo_timer = s.demo_respawn_timer;
+    c.hit_rel_pos = v3(s.hit_rel_pos[0], s.hit_rel_pos[1], s.hit_rel_pos[2]);
+    c.hit_ball_pos = v3(s.hit_ball_pos[0], s.hit_ball_pos[1], s.hit_ball_pos[2]);
+    c.hit_extra_vel = v3(s.hit_extra_vel[0], s.hit_extra_vel[1], s.hit_extra_vel[2]);
+    c.hit_tick = s.hit_tick;
+    c.hit_extra_tick = s.hit_extra_tick;
+    c.ticks_since_set = s.tick_count_since_update;
+    c.last = ctrl_from_api(s.last_controls);
+    c.engine = s.wheel_engine_force; c.brake = s.wheel_brake; c.steer = s.wheel_steer;
+    for (int w = 0; w < 4; w++) {
+        c.lat[w] = s.wheel_lat_friction[w];
+        c.lon[w] = s.wheel_long_friction[w];
+        c.push[w] = s.wheel_extra_pushback[w];
+    }
+    store_car(r, c);
+    r.u(CW_BP_CELL) = (uint32_t)(s.bp_cell >= 0 && s.bp_cell < BP_NUM_CELLS ? s.bp_cell : 0);
+    r.u(CW_BP_SEQ) = s.bp_seq;
+}
+
+inline void ball_to_api(const CarRef& r, rsg_ball_state& s) {
+    std::memset(&s, 0, sizeof(s));
+    V3 p = r.v(BW_POS) * BT_TO_UU, v = r.v(BW_VEL) * BT_TO_UU, w = r.v(BW_ANGVEL);
+    s.pos[0] = p.x; s.pos[1] = p.y; s.pos[2] = p.z;
+    rot_rows_to_api(load_rot(r, BW_ROT), s.rot);
+    s.vel[0] = v.x; s.vel[1] = v.y; s.vel[2] = v.z;
+    s.ang_vel[0] = w.x; s.ang_vel[1] = w.y; s.ang_vel[2] = w.z;
+    s.tick_count_since_update = r.u64(BW_TICKS_SINCE_SET);
+    s.bp_cell = (int32_t)r.u(BW_BP_CELL);
+    s.bp_seq = r.u(BW_BP_SEQ);
+}
+
+inline void ball_from_api(const CarRef& r, const rsg_ball_state& s) {
+    r.sv(BW_POS, v3(s.pos[0], s.pos[1], s.pos[2]) * UU_TO_BT);
+    M3 m;
+    rot_api_to_rows(s.rot, m);
+    store_rot(r, BW_ROT, m);
+    r.sv(BW_VEL, v3(s.vel[0], s.vel[1], s.vel[2]) * UU_TO_BT);
+    r.sv(BW_ANGVEL, v3(s.ang_vel[0], s.ang_vel[1], s.ang_vel[2]));
+    r.su64(BW_TICKS_SINCE_SET, s.tick_count_since_update);
+    r.u(BW_BP_CELL) = (uint32_t)(s.bp_cell >= 0 && s.bp_cell < BP_NUM_CELLS ? s.bp_cell : 0);
+    r.u(BW_BP_SEQ) = s.bp_seq;
+}
+
+inline void pad_to_api(const CarRef& r, rsg_pad_state& s) {
+    s.is_active = r.u(PW_ACTIVE);
+    s.cooldown = r.f(PW_COOLDOWN);
+    s.prev_locked_car_id = r.u(PW_PREV_LOCKED);
+}
+
+inline void pad_from_api(const CarRef& r, const rsg_pad_state& s) {
+    r.u(PW_ACTIVE) = s.is_active ? 1u : 0u;
+    r.sf(PW_COOLDOWN, s.cooldown);
+    r.u(PW_PREV_LOCKED) = s.prev_locked_car_id;
+}
+
+}  // namespace rsg

@@ -18,6 +18,9 @@ struct ArenaLocal {
     uint32_t resets;
     uint32_t status;
     SolverInfo si;
+    // broadphase registration of the dynamic proxies (0 = ball, 1 + i = car i)
+    int bp_cell[1 + MAX_CARS];
+    uint32_t bp_seq[1 + MAX_CARS];
 };
 
 struct TickParams {

@@ -18,7 +18,9 @@
 #include <vector>
 
 #include "../../include/rsg.h"
+#include "rsg_mesh_build.h"
 #include "rsg_tick.h"
+#include "rsg_api_convert.h"
 
 using namespace rsg;
 
@@ -37,12 +39,39 @@ int fail(const std::string& msg) {
         if (_e != hipSuccess) return fail(std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
+// The Soccar collision mesh, shared read-only by every batch of the process
+// (RocketSim keeps its arena meshes in process-wide storage the same way,
+// src/RocketSim.cpp:92-185).  Device arrays are allocated once per init.
+struct DeviceMesh {
+    bool ready = false;
+    HostMesh host;
+    float* verts = nullptr;
+    int32_t* tris = nullptr;
+    QNode* nodes = nullptr;
+    QSubtree* subtrees = nullptr;
+    TriInfo* tri_info = nullptr;
+    uint8_t* cell_statics = nullptr;
+    MeshView view{};
+};
+DeviceMesh g_mesh;
+std::mutex g_mesh_mutex;
+
+template <class T>
+int upload_array(T** dst, const std::vector<T>& src) {
+    if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
+    size_t bytes = src.size() * sizeof(T);
+    if (bytes == 0) return 0;
+    HIP_OK(hipMalloc((void**)dst, bytes));
+    HIP_OK(hipMemcpy(*dst, src.data(), bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // kernels
 
-__global__ void __launch_bounds__(64) rsg_step_arena_kernel(BatchDev d, int ticks) {
+__global__ void __launch_bounds__(64) rsg_step_arena_kernel(BatchDev d, MeshView mesh, int ticks) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= d.n) return;
     ArenaLocal A;
@@ -54,8 +83,8 @@ __global__ void __launch_bounds__(64) rsg_step_arena_kernel(BatchDev d, int tick
     P.arena_id = (uint32_t)a;
     P.synth_controls = d.synth_controls != 0;
     P.auto_reset = d.auto_reset != 0;
-    NoMesh mesh;
-    for (int t = 0; t < ticks; t++) arena_tick(A, P, mesh);
+    const MeshView* mp = d.mode == RSG_MODE_SOCCAR ? &mesh : nullptr;
+    for (int t = 0; t < ticks; t++) arena_tick(A, P, mp);
     store_arena(d, a, A);
 }
 
@@ -121,31 +150,6 @@ int download(rsg_batch* b) {
     return 0;
 }
 
-void rot_api_to_rows(const float* api, M3& m) {
-    // api: forward, right, up columns (RotMat, src/Math/MathTypes/MathTypes.h:169-170)
-    for (int r = 0; r < 3; r++) m.r[r] = v3(api[0 + r], api[3 + r], api[6 + r]);
-}
-void rot_rows_to_api(const M3& m, float* api) {
-    for (int r = 0; r < 3; r++) {
-        api[0 + r] = m.r[r].x;
-        api[3 + r] = m.r[r].y;
-        api[6 + r] = m.r[r].z;
-    }
-}
-
-Ctrl ctrl_from_api(const rsg_controls& c) {
-    Ctrl o;
-    o.throttle = c.throttle; o.steer = c.steer; o.pitch = c.pitch; o.yaw = c.yaw; o.roll = c.roll;
-    o.jump = c.jump != 0; o.boost = c.boost != 0; o.handbrake = c.handbrake != 0;
-    return o;
-}
-rsg_controls ctrl_to_api(const Ctrl& c) {
-    rsg_controls o;
-    o.throttle = c.throttle; o.steer = c.steer; o.pitch = c.pitch; o.yaw = c.yaw; o.roll = c.roll;
-    o.jump = c.jump; o.boost = c.boost; o.handbrake = c.handbrake; o._pad = 0;
-    return o;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -166,6 +170,10 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
         return nullptr;
     }
     if (!desc->cars_per_arena) { fail("rsg_batch_create: cars_per_arena is required"); return nullptr; }
+    if (desc->mode == RSG_MODE_SOCCAR && !g_mesh.ready) {
+        fail("rsg_batch_create: no arena collision mesh loaded (call rsg_init_mesh first)");
+        return nullptr;
+    }
     auto* b = new rsg_batch();
     b->dev.n = desc->n_arenas;
     b->dev.mode = desc->mode;
@@ -226,10 +234,23 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
             cr.su64(CW_HIT_EXTRA_TICK, ~0ull);
             cr.sf(CW_BOOST, BOOST_SPAWN_AMOUNT);
             store_rot(cr, CW_ROT, m3_identity());
+            // proxy created at the identity transform (Car::_BulletSetup,
+            // src/Sim/Car/Car.cpp:212-233), AABB without contact threshold
+            CarPhys cp = make_car_phys(b->preset[k]);
+            cr.u(CW_BP_CELL) = (uint32_t)bp_cell_of(cp.local_aabb_center - cp.local_aabb_half);
+            cr.u(CW_BP_SEQ) = (uint32_t)(1 + i);
         }
         CarRef br = ball_ref(h, a);
         br.sv(BW_POS, v3(0, 0, BALL_REST_Z) * UU_TO_BT);
         store_rot(br, BW_ROT, m3_identity());
+        {
+            // proxy created at (0, 0, radius) (Ball::_BulletSetup, src/Sim/Ball/Ball.cpp:85-114)
+            BallPhys bp = make_ball_phys();
+            float ext = bp.radius + 0.08f;
+            V3 c = v3(0, 0, bp.radius);
+            br.u(BW_BP_CELL) = (uint32_t)bp_cell_of(c - v3(ext, ext, ext));
+            br.u(BW_BP_SEQ) = 0;
+        }
         for (int p = 0; p < PAD_NUM; p++) pad_ref(h, a, p).u(PW_ACTIVE) = 1;
     }
     b->host_dirty = true;
@@ -263,7 +284,12 @@ int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     int block = 64;
     int grid = (b->dev.n + block - 1) / block;
-    hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(block), 0, s, b->dev, ticks);
+    MeshView mv{};
+    if (b->dev.mode == RSG_MODE_SOCCAR) {
+        if (!g_mesh.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
+        mv = g_mesh.view;
+    }
+    hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(block), 0, s, b->dev, mv, ticks);
     HIP_OK(hipGetLastError());
     b->last_stream = s;
     b->dev_newer = true;
@@ -293,44 +319,9 @@ int rsg_batch_get_cars(rsg_batch* b, rsg_car_state* out) {
     if (!b || !out) return fail("rsg_batch_get_cars: null argument");
     if (download(b) != 0) return -1;
     BatchDev h = host_view(b);
-    for (int a = 0; a < b->dev.n; a++) {
-        ArenaLocal A;
-        load_arena(h, a, A);
-        for (int i = 0; i < A.n_cars; i++) {
-            const Car& c = A.cars[i];
-            rsg_car_state& s = out[b->car_offset[a] + i];
-            std::memset(&s, 0, sizeof(s));
-            s.tick_count_since_update = c.ticks_since_set;
-            s.hit_tick = c.hit_tick;
-            s.hit_extra_tick = c.hit_extra_tick;
-            V3 p = c.b.pos * BT_TO_UU, v = c.b.vel * BT_TO_UU;
-            s.pos[0] = p.x; s.pos[1] = p.y; s.pos[2] = p.z;
-            rot_rows_to_api(c.b.rot, s.rot);
-            s.vel[0] = v.x; s.vel[1] = v.y; s.vel[2] = v.z;
-            s.ang_vel[0] = c.b.ang.x; s.ang_vel[1] = c.b.ang.y; s.ang_vel[2] = c.b.ang.z;
-            s.flags = c.flags;
-            s.flip_rel_torque[0] = c.flip_torque.x; s.flip_rel_torque[1] = c.flip_torque.y; s.flip_rel_torque[2] = c.flip_torque.z;
-            s.jump_time = c.jump_time; s.flip_time = c.flip_time;
-            s.air_time = c.air_time; s.air_time_since_jump = c.air_time_since_jump;
-            s.boost = c.boost; s.time_since_boosted = c.time_since_boosted; s.boosting_time = c.boosting_time;
-            s.supersonic_time = c.supersonic_time; s.handbrake_val = c.handbrake;
-            s.auto_flip_timer = c.af_timer; s.auto_flip_torque_scale = c.af_scale;
-            s.world_contact_normal[0] = c.world_normal.x; s.world_contact_normal[1] = c.world_normal.y; s.world_contact_normal[2] = c.world_normal.z;
-            s.car_contact_other_id = c.other_id;
-            s.car_contact_cooldown = c.car_cooldown;
-            s.demo_respawn_timer = c.demo_timer;
-            s.hit_rel_pos[0] = c.hit_rel_pos.x; s.hit_rel_pos[1] = c.hit_rel_pos.y; s.hit_rel_pos[2] = c.hit_rel_pos.z;
-            s.hit_ball_pos[0] = c.hit_ball_pos.x; s.hit_ball_pos[1] = c.hit_ball_pos.y; s.hit_ball_pos[2] = c.hit_ball_pos.z;
-            s.hit_extra_vel[0] = c.hit_extra_vel.x; s.hit_extra_vel[1] = c.hit_extra_vel.y; s.hit_extra_vel[2] = c.hit_extra_vel.z;
-            s.last_controls = ctrl_to_api(c.last);
-            s.wheel_engine_force = c.engine; s.wheel_brake = c.brake; s.wheel_steer = c.steer;
-            for (int w = 0; w < 4; w++) {
-                s.wheel_lat_friction[w] = c.lat[w];
-                s.wheel_long_friction[w] = c.lon[w];
-                s.wheel_extra_pushback[w] = c.push[w];
-            }
-        }
-    }
+    for (int a = 0; a < b->dev.n; a++)
+        for (int i = 0; i < b->cars_per_arena[a]; i++)
+            car_to_api(car_ref(h, a, i), out[b->car_offset[a] + i]);
     return 0;
 }
 
@@ -340,39 +331,7 @@ int rsg_batch_set_cars(rsg_batch* b, const rsg_car_state* in) {
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++) {
         for (int i = 0; i < b->cars_per_arena[a]; i++) {
-            const rsg_car_state& s = in[b->car_offset[a] + i];
-            CarRef r = car_ref(h, a, i);
-            Car c;
-            load_car(r, c);
-            c.b.pos = v3(s.pos[0], s.pos[1], s.pos[2]) * UU_TO_BT;
-            rot_api_to_rows(s.rot, c.b.rot);
-            c.b.vel = v3(s.vel[0], s.vel[1], s.vel[2]) * UU_TO_BT;
-            c.b.ang = v3(s.ang_vel[0], s.ang_vel[1], s.ang_vel[2]);
-            c.flags = s.flags;
-            c.flip_torque = v3(s.flip_rel_torque[0], s.flip_rel_torque[1], s.flip_rel_torque[2]);
-            c.jump_time = s.jump_time; c.flip_time = s.flip_time;
-            c.air_time = s.air_time; c.air_time_since_jump = s.air_time_since_jump;
-            c.boost = s.boost; c.time_since_boosted = s.time_since_boosted; c.boosting_time = s.boosting_time;
-            c.supersonic_time = s.supersonic_time; c.handbrake = s.handbrake_val;
-            c.af_timer = s.auto_flip_timer; c.af_scale = s.auto_flip_torque_scale;
-            c.world_normal = v3(s.world_contact_normal[0], s.world_contact_normal[1], s.world_contact_normal[2]);
-            c.other_id = s.car_contact_other_id;
-            c.car_cooldown = s.car_contact_cooldown;
-            c.demo_timer = s.demo_respawn_timer;
-            c.hit_rel_pos = v3(s.hit_rel_pos[0], s.hit_rel_pos[1], s.hit_rel_pos[2]);
-            c.hit_ball_pos = v3(s.hit_ball_pos[0], s.hit_ball_pos[1], s.hit_ball_pos[2]);
-            c.hit_extra_vel = v3(s.hit_extra_vel[0], s.hit_extra_vel[1], s.hit_extra_vel[2]);
-            c.hit_tick = s.hit_tick;
-            c.hit_extra_tick = s.hit_extra_tick;
-            c.ticks_since_set = s.tick_count_since_update;
-            c.last = ctrl_from_api(s.last_controls);
-            c.engine = s.wheel_engine_force; c.brake = s.wheel_brake; c.steer = s.wheel_steer;
-            for (int w = 0; w < 4; w++) {
-                c.lat[w] = s.wheel_lat_friction[w];
-                c.lon[w] = s.wheel_long_friction[w];
-                c.push[w] = s.wheel_extra_pushback[w];
-            }
-            store_car(r, c);
+            car_from_api(car_ref(h, a, i), in[b->car_offset[a] + i]);
         }
     }
     b->host_dirty = true;
@@ -384,15 +343,7 @@ int rsg_batch_get_balls(rsg_batch* b, rsg_ball_state* out) {
     if (download(b) != 0) return -1;
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++) {
-        CarRef r = ball_ref(h, a);
-        rsg_ball_state& s = out[a];
-        std::memset(&s, 0, sizeof(s));
-        V3 p = r.v(BW_POS) * BT_TO_UU, v = r.v(BW_VEL) * BT_TO_UU, w = r.v(BW_ANGVEL);
-        s.pos[0] = p.x; s.pos[1] = p.y; s.pos[2] = p.z;
-        rot_rows_to_api(load_rot(r, BW_ROT), s.rot);
-        s.vel[0] = v.x; s.vel[1] = v.y; s.vel[2] = v.z;
-        s.ang_vel[0] = w.x; s.ang_vel[1] = w.y; s.ang_vel[2] = w.z;
-        s.tick_count_since_update = r.u64(BW_TICKS_SINCE_SET);
+        ball_to_api(ball_ref(h, a), out[a]);
     }
     return 0;
 }
@@ -402,15 +353,7 @@ int rsg_batch_set_balls(rsg_batch* b, const rsg_ball_state* in) {
     if (download(b) != 0) return -1;
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++) {
-        CarRef r = ball_ref(h, a);
-        const rsg_ball_state& s = in[a];
-        r.sv(BW_POS, v3(s.pos[0], s.pos[1], s.pos[2]) * UU_TO_BT);
-        M3 m;
-        rot_api_to_rows(s.rot, m);
-        store_rot(r, BW_ROT, m);
-        r.sv(BW_VEL, v3(s.vel[0], s.vel[1], s.vel[2]) * UU_TO_BT);
-        r.sv(BW_ANGVEL, v3(s.ang_vel[0], s.ang_vel[1], s.ang_vel[2]));
-        r.su64(BW_TICKS_SINCE_SET, s.tick_count_since_update);
+        ball_from_api(ball_ref(h, a), in[a]);
     }
     b->host_dirty = true;
     return 0;
@@ -421,13 +364,7 @@ int rsg_batch_get_pads(rsg_batch* b, rsg_pad_state* out) {
     if (download(b) != 0) return -1;
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++)
-        for (int p = 0; p < PAD_NUM; p++) {
-            CarRef r = pad_ref(h, a, p);
-            rsg_pad_state& s = out[a * PAD_NUM + p];
-            s.is_active = r.u(PW_ACTIVE);
-            s.cooldown = r.f(PW_COOLDOWN);
-            s.prev_locked_car_id = r.u(PW_PREV_LOCKED);
-        }
+        for (int p = 0; p < PAD_NUM; p++) pad_to_api(pad_ref(h, a, p), out[a * PAD_NUM + p]);
     return 0;
 }
 
@@ -436,13 +373,7 @@ int rsg_batch_set_pads(rsg_batch* b, const rsg_pad_state* in) {
     if (download(b) != 0) return -1;
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++)
-        for (int p = 0; p < PAD_NUM; p++) {
-            CarRef r = pad_ref(h, a, p);
-            const rsg_pad_state& s = in[a * PAD_NUM + p];
-            r.u(PW_ACTIVE) = s.is_active ? 1u : 0u;
-            r.sf(PW_COOLDOWN, s.cooldown);
-            r.u(PW_PREV_LOCKED) = s.prev_locked_car_id;
-        }
+        for (int p = 0; p < PAD_NUM; p++) pad_from_api(pad_ref(h, a, p), in[a * PAD_NUM + p]);
     b->host_dirty = true;
     return 0;
 }
@@ -511,12 +442,27 @@ int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out) {
 }
 
 int rsg_init_mesh(const void* cmf_data, uint64_t size) {
-    if (!cmf_data || size < 8) return fail("rsg_init_mesh: bad .cmf data");
-    int32_t nt, nv;
-    std::memcpy(&nt, cmf_data, 4);
-    std::memcpy(&nv, (const char*)cmf_data + 4, 4);
-    if (nt <= 0 || nv <= 0 || nt > 1000000 || nv > 1000000 || size < 8 + (uint64_t)nt * 12 + (uint64_t)nv * 12)
-        return fail("rsg_init_mesh: bad .cmf triangle/vertex count");
+    std::lock_guard<std::mutex> lock(g_mesh_mutex);
+    HostMesh m;
+    std::string err = build_host_mesh(cmf_data, size, m);
+    if (!err.empty()) return fail("rsg_init_mesh: invalid collision mesh file (" + err + ")");
+    (void)hipDeviceSynchronize();
+    g_mesh.ready = false;
+    g_mesh.host = std::move(m);
+    const HostMesh& H = g_mesh.host;
+    if (upload_array(&g_mesh.verts, H.verts) || upload_array(&g_mesh.tris, H.tris) ||
+        upload_array(&g_mesh.nodes, H.nodes) || upload_array(&g_mesh.subtrees, H.subtrees) ||
+        upload_array(&g_mesh.tri_info, H.tri_info) || upload_array(&g_mesh.cell_statics, H.cell_statics))
+        return -1;
+    MeshView v = H.view();
+    v.verts = g_mesh.verts;
+    v.tris = g_mesh.tris;
+    v.nodes = g_mesh.nodes;
+    v.subtrees = g_mesh.subtrees;
+    v.tri_info = g_mesh.tri_info;
+    v.cell_statics = g_mesh.cell_statics;
+    g_mesh.view = v;
+    g_mesh.ready = true;
     return 0;
 }
 

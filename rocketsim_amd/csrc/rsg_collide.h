@@ -1,28 +1,156 @@
-// rsg_collide.h -- the world step of one arena: gravity, damping, collision
-// detection, contact solve and integration
+// rsg_collide.h -- the world step of one arena
 // (btDiscreteDynamicsWorld::stepSimulation / internalSingleStepSimulation,
-// BulletDynamics/Dynamics/btDiscreteDynamicsWorld.cpp:325-437).
+// BulletDynamics/Dynamics/btDiscreteDynamicsWorld.cpp:325-437):
+//   gravity -> damping + predicted poses -> AABBs -> broadphase pairs ->
+//   narrowphase + contact callbacks -> islands -> contact solve -> integrate.
+//
+// Pair order, manifold order and the island sort follow the reference
+// exactly, since the Gauss-Seidel solve depends on the order of its rows:
+//   * pairs: for each dynamic proxy in handle order (ball, cars), the static
+//     objects registered in its broadphase cell, then the other dynamic
+//     proxies of the cell (btRSBroadphase.cpp:392-468);
+//   * manifolds: created in dispatch order (btCollisionDispatcher.cpp:68-115),
+//     compound children release theirs when their AABB stops overlapping
+//     (btCompoundCollisionAlgorithm.cpp:300-338);
+//   * islands: union-find with path compression over the pairs, manifolds
+//     sorted by island id with Bullet's (unstable) quicksort
+//     (btSimulationIslandManager.cpp:196-445, LinearMath/btAlignedObjectArray.h).
 #pragma once
 
-#include "rsg_arena.h"
+#include "rsg_solver.h"
 
 namespace rsg {
 
-// Placeholder mesh type for arenas without a static world (THE_VOID).
-struct NoMesh {};
+constexpr int MAX_BODIES = 1 + RSG_MAX_CARS;
+constexpr int MAX_PAIRS = MAX_BODIES * NUM_STATICS + MAX_BODIES * (MAX_BODIES - 1) / 2;
 
-template <class MeshT>
+struct Pair {
+    int a, b;  // body codes, a has the lower unique id (statics < ball < cars)
+};
+
+// Bullet's btAlignedObjectArray::quickSortInternal (Hoare partition on the
+// middle element) over manifold indices keyed by island id.
+RSG_FN void quicksort_by_key(int* idx, const int* key, int lo, int hi) {
+    // explicit stack instead of recursion (same visiting order)
+    int stack_lo[32], stack_hi[32];
+    int sp = 0;
+    stack_lo[sp] = lo; stack_hi[sp] = hi; sp++;
+    while (sp > 0) {
+        sp--;
+        int l = stack_lo[sp], h = stack_hi[sp];
+        int i = l, j = h;
+        int x = key[idx[(l + h) / 2]];
+        do {
+            while (key[idx[i]] < x) i++;
+            while (x < key[idx[j]]) j--;
+            if (i <= j) {
+                int t = idx[i]; idx[i] = idx[j]; idx[j] = t;
+                i++; j--;
+            }
+        } while (i <= j);
+        // recursion order: (l, j) first, then (i, h) -- push in reverse
+        if (i < h) { stack_lo[sp] = i; stack_hi[sp] = h; sp++; }
+        if (l < j) { stack_lo[sp] = l; stack_hi[sp] = j; sp++; }
+    }
+}
+
 struct World {
     ArenaLocal& A;
-    const MeshT& mesh;
+    const MeshView* mesh;
     bool soccar;
 
-    RSG_FN World(ArenaLocal& a, const MeshT& m, bool s) : A(a), mesh(m), soccar(s) {}
+    V3 aabb_min[MAX_BODIES], aabb_max[MAX_BODIES];
+    int order[MAX_BODIES];  // dynamic proxies in cell-list order
+    Pair pairs[MAX_PAIRS];
+    int n_pairs;
+    Manifold mf[MAX_MANIFOLDS];
+    int n_mf;
+    bool overflow;
+
+    RSG_FN World(ArenaLocal& a, const MeshView* m, bool s) : A(a), mesh(m), soccar(s), n_pairs(0), n_mf(0), overflow(false) {}
 
     // Dynamic body index: 0 = ball, 1 + i = car i.
     RSG_FN Body* body_ptr(int i) { return i == 0 ? &A.ball.b : &A.cars[i - 1].b; }
     RSG_FN int num_bodies() const { return 1 + A.n_cars; }
+    RSG_FN bool is_car(int k) const { return k >= 1; }
+    // contact response: demoed cars have CF_NO_CONTACT_RESPONSE this tick
+    RSG_FN bool has_response(int code) {
+        if (code < 0) return true;
+        if (code == 0) return true;
+        return !flag(A.cars[code - 1].flags, RSG_CF_DEMOED) && A.cars[code - 1].b.active;
+    }
+    RSG_FN bool active(int code) {
+        if (code < 0) return false;
+        return body_ptr(code)->active;
+    }
 
+    RSG_FN Xf xf_of(int code) {
+        if (code >= 0) { Body* b = body_ptr(code); return xf_from(b->pos, b->rot); }
+        if (code == -1 - ST_MESH) return xf_ident();
+        PlaneDef p = plane_def(-1 - code);
+        return xf_from(p.origin, m3_identity());
+    }
+
+    RSG_FN float threshold_of(int code) {
+        if (code == 0) return make_ball_phys().angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
+        if (code > 0) return make_car_phys(A.cars[code - 1].preset).angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
+        return 1e30f;  // planes and the arena mesh have huge bounding spheres
+    }
+
+    RSG_FN int new_manifold(int b0, int b1) {
+        if (n_mf >= MAX_MANIFOLDS) { overflow = true; return -1; }
+        Manifold& m = mf[n_mf];
+        m.b0 = b0; m.b1 = b1;
+        m.threshold = rs_min(threshold_of(b0), threshold_of(b1));
+        m.n = 0;
+        return n_mf++;
+    }
+    RSG_FN void release_manifold(int i) {
+        int last = n_mf - 1;
+        if (i != last) mf[i] = mf[last];
+        n_mf--;
+    }
+
+    // -----------------------------------------------------------------------
+    // AABBs (btCollisionWorld::updateSingleAabb): shape AABB at the current
+    // and predicted pose, merged, +- the global contact threshold.
+    RSG_FN void shape_aabb(int code, V3 pos, const M3& rot, V3& mn, V3& mx) {
+        if (code == 0) {
+            float r = make_ball_phys().radius + 0.08f;  // btSphereShape::getAabb (RocketSim +0.08)
+            mn = pos - v3(r, r, r);
+            mx = pos + v3(r, r, r);
+        } else {
+            CarPhys cp = make_car_phys(A.cars[code - 1].preset);
+            M3 ab = abs_m3(rot);
+            V3 c = rot * cp.local_aabb_center + pos;
+            V3 e = v3(dot(ab.r[0], cp.local_aabb_half), dot(ab.r[1], cp.local_aabb_half), dot(ab.r[2], cp.local_aabb_half));
+            mn = c - e;
+            mx = c + e;
+        }
+    }
+
+    // Child box AABB of a car (btBoxShape::getAabb at hull transform, no threshold).
+    RSG_FN void car_child_aabb(int code, V3& mn, V3& mx) {
+        Car& c = A.cars[code - 1];
+        CarPhys cp = make_car_phys(c.preset);
+        Xf x = xf_mul(xf_from(c.b.pos, c.b.rot), xf_from(cp.box_offset, m3_identity()));
+        V3 he = cp.box_implicit + v3(cp.box_margin, cp.box_margin, cp.box_margin);
+        M3 ab = abs_m3(x.R);
+        V3 e = v3(dot(ab.r[0], he), dot(ab.r[1], he), dot(ab.r[2], he));
+        mn = x.t - e;
+        mx = x.t + e;
+    }
+
+    RSG_FN bool static_in_cell(int s, int c) {
+        if (!soccar) return false;
+        return (mesh->cell_statics[c] >> s) & 1u;
+    }
+
+    // -----------------------------------------------------------------------
+    // Ray cast for the wheels (btDefaultVehicleRaycaster::castRay,
+    // BulletDynamics/Vehicle/btDefaultVehicleRaycaster.cpp:32-53): closest hit
+    // over the statics registered in the ray start's cell, then the dynamic
+    // bodies of that cell.
     RSG_FN RayHit raycast(V3 from, V3 to, int self_body) {
         RayHit h;
         h.hit = false;
@@ -31,40 +159,327 @@ struct World {
         h.fraction = 1.f;
         h.point = to;
         h.normal = v3zero();
-        (void)from; (void)self_body;
+        (void)self_body;
+        if (!soccar) return h;
+        int c = bp_cell_of(from);
+        float frac = 1.f;
+        V3 n = v3zero();
+        int hit_obj = -100;
+        for (int s = 0; s < NUM_STATICS; s++) {
+            if (!static_in_cell(s, c)) continue;
+            if (frac == 0.f) break;
+            if (s == ST_MESH) {
+                struct RayCB {
+                    const MeshView* m; V3 from, to; float* frac; V3* n; bool hit;
+                    RSG_FN void operator()(int t) {
+                        V3 v[3];
+                        mesh_triangle(*m, t, v);
+                        if (ray_triangle(from, to, v[0], v[1], v[2], *frac, *n)) hit = true;
+                    }
+                } cb{mesh, from, to, &frac, &n, false};
+                mesh_query_ray(*mesh, from, to, cb);
+                if (cb.hit) hit_obj = -1 - s;
+            } else {
+                if (ray_plane(s, from, to, frac, n)) hit_obj = -1 - s;
+            }
+        }
+        if (hit_obj != -100) {
+            h.hit = true;
+            h.is_static = true;
+            h.fraction = frac;
+            float sm = 1.f - frac;
+            h.point = from * sm + to * frac;
+            h.normal = normalized(n);
+        }
         return h;
     }
 
-    // btDiscreteDynamicsWorld::stepSimulation(dt, 0, dt)
+    // -----------------------------------------------------------------------
+    // Contact point creation (btManifoldResult::addContactPoint) for a
+    // manifold whose body0 is dynamic `b0`; `normal` points from B to A,
+    // `point_b` lies on B, depth < 0 is penetration.
+    RSG_FN int add_contact(int mi, V3 normal, V3 point_b, float depth, int tri) {
+        Manifold& m = mf[mi];
+        if (depth > m.threshold) return -1;
+        V3 point_a = point_b + normal * depth;
+        CPoint p;
+        p.local_a = xf_inv_apply(xf_of(m.b0), point_a);
+        p.local_b = xf_inv_apply(xf_of(m.b1), point_b);
+        p.pos_a = point_a;
+        p.pos_b = point_b;
+        p.normal = normal;
+        p.dist = depth;
+        p.special = false;
+        p.tri = tri;
+        // combined friction / restitution (btManifoldResult.cpp:56-77)
+        float fa, ra, fb, rb;
+        body_material(m.b0, fa, ra);
+        body_material(m.b1, fb, rb);
+        bool stat = m.b0 < 0 || m.b1 < 0;
+        p.friction = stat ? (fa < fb ? fa : fb) : fa * fb;
+        p.restitution = stat ? (ra > rb ? ra : rb) : ra * rb;
+        int idx = manifold_add_point(m, p);
+        contact_added(m.p[idx], m.b0, m.b1);
+        return idx;
+    }
+
+    RSG_FN void body_material(int code, float& f, float& r) {
+        if (code < 0) { f = ARENA_COLLISION_BASE_FRICTION; r = ARENA_COLLISION_BASE_RESTITUTION; }
+        else if (code == 0) { f = BALL_FRICTION; r = BALL_RESTITUTION; }
+        else { f = CAR_COLLISION_FRICTION; r = CAR_COLLISION_RESTITUTION; }
+    }
+
+    // Arena::_BulletContactAddedCallback (src/Sim/Arena/Arena.cpp:244-314)
+    RSG_FN void contact_added(CPoint& cp, int b0, int b1) {
+        if (!has_response(b0) || !has_response(b1)) return;
+        if (b0 > 0 && b1 < 0) {
+            // car + world (Arena.cpp:407-414)
+            Car& c = A.cars[b0 - 1];
+            setflag(c.flags, RSG_CF_WORLD_CONTACT, true);
+            c.world_normal = cp.normal;
+            cp.friction = CARWORLD_COLLISION_FRICTION;
+            cp.restitution = CARWORLD_COLLISION_RESTITUTION;
+        } else if (b0 == 0 && b1 < 0) {
+            cp.special = true;
+        }
+        if (b1 == -1 - ST_MESH && cp.tri >= 0) adjust_internal_edge(cp, *mesh, cp.tri);
+    }
+
+    RSG_FN void refresh(int mi) {
+        Manifold& m = mf[mi];
+        if (m.n == 0) return;
+        manifold_refresh(m, xf_of(m.b0), xf_of(m.b1));
+    }
+
+    // -----------------------------------------------------------------------
+    // Narrowphase
+
+    // btConvexPlaneCollisionAlgorithm::processCollision for the ball
+    RSG_FN void ball_vs_plane(int mi, int s) {
+        PlaneDef pd = plane_def(s);
+        Xf px = xf_from(pd.origin, m3_identity());
+        Body& b = A.ball.b;
+        Xf bx = xf_from(b.pos, b.rot);
+        Xf plane_in_convex = xf_mul(xf_inverse(bx), px);
+        Xf convex_in_plane = xf_mul(xf_inverse(px), bx);
+        V3 dir = plane_in_convex.R * -pd.n;
+        V3 vtx;
+        float r = make_ball_phys().radius;
+        if (len2(dir) < FLT_EPS * FLT_EPS) vtx = normalized(v3(-1, -1, -1)) * r;
+        else vtx = r * normalized(dir);
+        V3 vin = xf_apply(convex_in_plane, vtx);
+        float distance = dot(pd.n, vin) - 0.f;
+        if (distance < mf[mi].threshold + 0.f) {
+            V3 proj = vin - distance * pd.n;
+            V3 w = xf_apply(px, proj);
+            V3 nB = px.R * pd.n;
+            add_contact(mi, nB, w, distance, -1);
+        }
+        refresh(mi);
+    }
+
+    // Child box of a car vs a plane (same algorithm, box support with margin)
+    RSG_FN void car_vs_plane(int mi, int code, int s) {
+        PlaneDef pd = plane_def(s);
+        Xf px = xf_from(pd.origin, m3_identity());
+        Car& c = A.cars[code - 1];
+        CarPhys cp = make_car_phys(c.preset);
+        Xf cx = xf_mul(xf_from(c.b.pos, c.b.rot), xf_from(cp.box_offset, m3_identity()));
+        Xf plane_in_convex = xf_mul(xf_inverse(cx), px);
+        Xf convex_in_plane = xf_mul(xf_inverse(px), cx);
+        V3 dir = plane_in_convex.R * -pd.n;
+        V3 he = cp.box_implicit + v3(cp.box_margin, cp.box_margin, cp.box_margin);
+        V3 vtx = v3(dir.x >= 0 ? he.x : -he.x, dir.y >= 0 ? he.y : -he.y, dir.z >= 0 ? he.z : -he.z);
+        V3 vin = xf_apply(convex_in_plane, vtx);
+        float distance = dot(pd.n, vin) - 0.f;
+        if (distance < mf[mi].threshold + 0.f) {
+            V3 proj = vin - distance * pd.n;
+            V3 w = xf_apply(px, proj);
+            V3 nB = px.R * pd.n;
+            add_contact(mi, nB, w, distance, -1);
+        }
+        refresh(mi);
+    }
+
+    // Sphere vs one mesh triangle (SphereTriangleDetector::collide, with the
+    // closest-point-on-triangle capsule test RocketSim uses).
+    RSG_FN void sphere_triangle(int mi, const V3 v[3], int tri) {
+        Body& b = A.ball.b;
+        V3 center = b.pos;  // triangle space = world (mesh at identity)
+        float radius = make_ball_phys().radius;
+        float thr = mf[mi].threshold + 0.f;
+        float rwt = radius + thr;
+        V3 normal = cross(v[1] - v[0], v[2] - v[0]);
+        float l2 = len2(normal);
+        bool has = false;
+        V3 cpt;
+        if (l2 >= FLT_EPS * FLT_EPS) {
+            normal = normal * (1.f / sqrtf(l2));
+            V3 p1c = center - v[0];
+            float dfp = dot(p1c, normal);
+            if (dfp < 0.f) { dfp *= -1.f; normal = normal * -1.f; }
+            if (dfp < rwt) {
+                if (point_in_triangle(v, normal, center)) {
+                    has = true;
+                    cpt = center - normal * dfp;
+                } else {
+                    float min_d2 = rwt * rwt;
+                    V3 ne = closest_point_triangle(center, v[0], v[1], v[2]);
+                    V3 dd = ne - center;
+                    float d2 = dot(dd, dd);
+                    if (d2 < min_d2) { has = true; cpt = ne; }
+                }
+            }
+        }
+        if (!has) return;
+        V3 c2c = center - cpt;
+        float d2 = len2(c2c);
+        if (!(d2 < rwt * rwt)) return;
+        V3 rn, pt;
+        float depth;
+        if (d2 > FLT_EPS) {
+            float dist = sqrtf(d2);
+            rn = normalized(c2c);
+            pt = cpt;
+            depth = -(radius - dist);
+        } else {
+            rn = normal;
+            pt = cpt;
+            depth = -radius;
+        }
+        add_contact(mi, rn, pt, depth, tri);
+    }
+
+    // SphereTriangleDetector::pointInTriangle (RocketSim variant)
+    RSG_FN static bool point_in_triangle(const V3 v[3], V3 normal, V3 p) {
+        V3 e0 = v[1] - v[0], e1 = v[2] - v[1], e2 = v[0] - v[2];
+        V3 n0 = cross(e0, normal), n1 = cross(e1, normal), n2 = cross(e2, normal);
+        float r0 = dot(p - v[0], n0), r1 = dot(p - v[1], n1), r2 = dot(p - v[2], n2);
+        return (r0 <= 0 && r1 <= 0 && r2 <= 0) || (r0 > 0 && r1 > 0 && r2 > 0);
+    }
+
+    // closestPointTriangle (Ericson, Real-Time Collision Detection 5.1.5)
+    RSG_FN static V3 closest_point_triangle(V3 p, V3 a, V3 b, V3 c) {
+        V3 ab = b - a, ac = c - a, ap = p - a;
+        float d1 = dot(ab, ap), d2 = dot(ac, ap);
+        if (d1 <= 0.f && d2 <= 0.f) return a;
+        V3 bp = p - b;
+        float d3 = dot(ab, bp), d4 = dot(ac, bp);
+        if (d3 >= 0.f && d4 <= d3) return b;
+        V3 cp = p - c;
+        float d5 = dot(ab, cp), d6 = dot(ac, cp);
+        if (d6 >= 0.f && d5 <= d6) return c;
+        float vc = d1 * d4 - d3 * d2;
+        if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) { float v = d1 / (d1 - d3); return a + v * ab; }
+        float vb = d5 * d2 - d1 * d6;
+        if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) { float v = d2 / (d2 - d6); return a + v * ac; }
+        float va = d3 * d6 - d5 * d4;
+        if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+            float v = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+            return b + v * (c - b);
+        }
+        float denom = 1.f / (va + vb + vc);
+        float v = vb * denom, w = vc * denom;
+        return a + v * ab + w * ac;
+    }
+
+    // btConvexConcaveCollisionAlgorithm::processCollision for the ball
+    RSG_FN void ball_vs_mesh(int mi) {
+        Body& b = A.ball.b;
+        float r = make_ball_phys().radius + 0.08f;
+        V3 amin = b.pos - v3(r, r, r), amax = b.pos + v3(r, r, r);  // convex AABB in mesh space, + margin 0
+        struct CB {
+            World* w; int mi; V3 amin, amax;
+            RSG_FN void operator()(int t) { w->ball_triangle(mi, t, amin, amax); }
+        } cb{this, mi, amin, amax};
+        mesh_query_aabb(*mesh, amin, amax, cb);
+        refresh(mi);
+    }
+
+    RSG_FN void ball_triangle(int mi, int t, V3 amin, V3 amax) {
+        V3 v[3];
+        mesh_triangle(*mesh, t, v);
+        if (!triangle_aabb_overlap(v, amin, amax)) return;
+        // early out on the triangle plane (btConvexConcaveCollisionAlgorithm.cpp:86-125)
+        {
+            V3 tn = normalized(cross(v[1] - v[0], v[2] - v[0]));
+            Body& b = A.ball.b;
+            float r = make_ball_phys().radius;
+            V3 d = inverse(b.rot) * tn;  // getBasis().inverse() * n
+            V3 lp = (len2(d) < FLT_EPS * FLT_EPS) ? normalized(v3(-1, -1, -1)) * r : r * normalized(d);
+            V3 wp = b.rot * lp + b.pos;
+            float thr = mf[mi].threshold + 0.f;
+            float dist = dot(tn, v[0]) - dot(tn, wp);
+            if (dist > thr) return;
+            tn = tn * -1.f;
+            d = inverse(b.rot) * tn;
+            lp = (len2(d) < FLT_EPS * FLT_EPS) ? normalized(v3(-1, -1, -1)) * r : r * normalized(d);
+            wp = b.rot * lp + b.pos;
+            dist = dot(tn, v[0]) - dot(tn, wp);
+            if (dist > thr) return;
+        }
+        sphere_triangle(mi, v, t);
+    }
+
+    // -----------------------------------------------------------------------
+    // The step
+
     RSG_FN void step(float dt, const TickParams& P) {
         (void)P;
         const int nb = num_bodies();
-        // applyGravity: active bodies only (btDiscreteDynamicsWorld.cpp:265-276)
         const V3 g = v3(0.f, 0.f, GRAVITY_Z) * UU_TO_BT;
         for (int i = 0; i < nb; i++) {
             Body& b = *body_ptr(i);
             if (b.active) b.force += g * (1.f / b.inv_mass);
         }
-        // predictUnconstraintMotion: damping on every dynamic body
-        // (btRigidBody::applyDamping, btRigidBody.cpp:153-207); the ball has
-        // linear damping BALL_DRAG, cars none.
+        // predictUnconstraintMotion (damping on every dynamic body)
         {
             Body& bb = A.ball.b;
             bb.vel *= powf(1.f - BALL_DRAG, dt);
             bb.ang *= powf(1.f - 0.f, dt);
         }
-        // Solver body write-back for every body in an active island
-        // (btSequentialImpulseConstraintSolver.cpp:458-491, 1866-1890,
-        // btSolverBody.h:265-287): v += F/m dt, w += (T * invI) dt.
+        // AABBs and broadphase cells
         for (int i = 0; i < nb; i++) {
             Body& b = *body_ptr(i);
-            if (!b.active) continue;
-            V3 ext_f = b.force * b.inv_mass * dt;
-            V3 ext_t = mul_vm(b.torque, b.inv_iw) * dt;
-            b.vel = (b.vel + v3zero()) + ext_f;
-            b.ang = (b.ang + v3zero()) + ext_t;
+            V3 pp; M3 pr;
+            integrate_transform(b.pos, b.rot, b.vel, b.ang, dt, b.no_rot, pp, pr);
+            V3 mn, mx, mn2, mx2;
+            shape_aabb(i, b.pos, b.rot, mn, mx);
+            V3 t = v3(CONTACT_BREAKING_THRESHOLD, CONTACT_BREAKING_THRESHOLD, CONTACT_BREAKING_THRESHOLD);
+            mn = mn - t; mx = mx + t;
+            shape_aabb(i, pp, pr, mn2, mx2);
+            mn2 = mn2 - t; mx2 = mx2 + t;
+            aabb_min[i] = v3(fminf(mn.x, mn2.x), fminf(mn.y, mn2.y), fminf(mn.z, mn2.z));
+            aabb_max[i] = v3(fmaxf(mx.x, mx2.x), fmaxf(mx.y, mx2.y), fmaxf(mx.z, mx2.z));
+            // btRSBroadphase::setAabb: a proxy whose cell changed is removed
+            // from its old cells and appended to the new ones, i.e. it moves
+            // to the end of every cell list it is in.
+            int c = bp_cell_of(aabb_min[i]);
+            if (c != A.bp_cell[i]) {
+                A.bp_cell[i] = c;
+                if (nb > 1) {
+                    uint32_t mxs = 0;
+                    for (int q = 0; q < nb; q++) mxs = A.bp_seq[q] > mxs ? A.bp_seq[q] : mxs;
+                    A.bp_seq[i] = mxs + 1;
+                }
+            }
         }
-        // integrateTransforms: active bodies (btDiscreteDynamicsWorld.cpp:889-983)
+        // cell-list order of the dynamic proxies (registration order)
+        for (int i = 0; i < nb; i++) order[i] = i;
+        for (int i = 1; i < nb; i++) {
+            int v = order[i], j = i - 1;
+            while (j >= 0 && A.bp_seq[order[j]] > A.bp_seq[v]) { order[j + 1] = order[j]; j--; }
+            order[j + 1] = v;
+        }
+        // keep the order numbers small: rank them 0..nb-1
+        for (int r = 0; r < nb; r++) A.bp_seq[order[r]] = (uint32_t)r;
+        find_pairs();
+        dispatch_pairs();
+#ifdef RSG_HOST_DEBUG_HOOK
+        RSG_HOST_DEBUG_HOOK(*this);
+#endif
+        solve_islands(dt);
+        // integrateTransforms: active bodies
         for (int i = 0; i < nb; i++) {
             Body& b = *body_ptr(i);
             if (!b.active) continue;
@@ -74,12 +489,285 @@ struct World {
             b.rot = nr;
             body_update_inertia(b);
         }
-        // clearForces
         for (int i = 0; i < nb; i++) {
             Body& b = *body_ptr(i);
             b.force = v3zero();
             b.torque = v3zero();
         }
+    }
+
+    // btRSBroadphase::calculateOverlappingPairs
+    RSG_FN void find_pairs() {
+        n_pairs = 0;
+        const int nb = num_bodies();
+        for (int p = 0; p < nb; p++) {
+            int c = A.bp_cell[p];
+            for (int s = 0; s < NUM_STATICS; s++) {
+                if (!static_in_cell(s, c)) continue;
+                V3 smin, smax;
+                static_proxy_aabb(mesh, s, smin, smax);
+                if (aabb_overlap(aabb_min[p], aabb_max[p], smin, smax)) {
+                    pairs[n_pairs].a = -1 - s;
+                    pairs[n_pairs].b = p;
+                    n_pairs++;
+                }
+            }
+            if (nb > 1) {
+                // the cell's dynamic list: every proxy registered within one
+                // cell of `c`, in registration order
+                int cnt = 0;
+                for (int q = 0; q < nb; q++)
+                    if (bp_cells_adjacent(A.bp_cell[q], c)) cnt++;
+                if (cnt > 1) {
+                    for (int r = 0; r < nb; r++) {
+                        int q = order[r];
+                        if (q == p || !bp_cells_adjacent(A.bp_cell[q], c)) continue;
+                        if (!aabb_overlap(aabb_min[p], aabb_max[p], aabb_min[q], aabb_max[q])) continue;
+                        int lo = p < q ? p : q, hi = p < q ? q : p;
+                        bool found = false;
+                        for (int k = 0; k < n_pairs; k++)
+                            if (pairs[k].a == lo && pairs[k].b == hi) found = true;
+                        if (found) continue;
+                        pairs[n_pairs].a = lo;
+                        pairs[n_pairs].b = hi;
+                        n_pairs++;
+                    }
+                }
+            }
+        }
+    }
+
+    RSG_FN bool needs_collision(int a, int b) { return active(a) || active(b); }
+
+    RSG_FN void dispatch_pairs() {
+        n_mf = 0;
+        for (int k = 0; k < n_pairs; k++) {
+            int a = pairs[k].a, b = pairs[k].b;
+            if (!needs_collision(a, b)) continue;
+            if (a < 0) {
+                int s = -1 - a;
+                if (b == 0) {
+                    if (s == ST_MESH) {
+                        int mi = new_manifold(0, a);
+                        if (mi >= 0) ball_vs_mesh(mi);
+                    } else {
+                        int mi = new_manifold(0, a);
+                        if (mi >= 0) ball_vs_plane(mi, s);
+                    }
+                } else {
+                    // compound car vs static: the child algorithm's manifold is
+                    // created up front and released if the child AABB misses
+                    int mi = new_manifold(b, a);
+                    if (mi < 0) continue;
+                    V3 cmin, cmax, smin, smax;
+                    car_child_aabb(b, cmin, cmax);
+                    static_shape_aabb(mesh, s, smin, smax);
+                    bool ov = aabb_overlap(cmin, cmax, smin, smax);
+                    if (ov) {
+                        if (s == ST_MESH) {
+                            // box vs triangles: not implemented yet
+                        } else {
+                            car_vs_plane(mi, b, s);
+                        }
+                    }
+                    if (!ov) release_manifold(mi);
+                }
+            } else {
+                // dynamic-dynamic pairs (ball-car, car-car): not implemented yet
+            }
+        }
+    }
+
+    // -----------------------------------------------------------------------
+    // Islands + solve
+
+    RSG_FN void solve_islands(float dt) {
+        const int nb = num_bodies();
+        // union-find over non-static objects (index = body index)
+        int uf_id[MAX_BODIES], uf_sz[MAX_BODIES];
+        for (int i = 0; i < nb; i++) { uf_id[i] = i; uf_sz[i] = 1; }
+        for (int k = 0; k < n_pairs; k++) {
+            int a = pairs[k].a, b = pairs[k].b;
+            if (a < 0 || b < 0) continue;
+            if (!has_response(a) || !has_response(b)) continue;
+            int i = uf_find(uf_id, a), j = uf_find(uf_id, b);
+            if (i == j) continue;
+            uf_id[i] = j;
+            uf_sz[j] += uf_sz[i];
+        }
+        int tag[MAX_BODIES];
+        for (int i = 0; i < nb; i++) tag[i] = uf_find(uf_id, i);
+        // sortIslands: elements sorted by island id (quicksort)
+        int elem[MAX_BODIES], ekey[MAX_BODIES];
+        for (int i = 0; i < nb; i++) { elem[i] = i; ekey[i] = uf_find(uf_id, i); }
+        if (nb > 1) quicksort_by_key(elem, ekey, 0, nb - 1);
+        // activation: wake islands that contain an active (non sleeping) body
+        // (btSimulationIslandManager::buildIslands)
+        bool sleeping[MAX_BODIES];
+        for (int i = 0; i < nb; i++) sleeping[i] = (i == 0) ? !A.ball.b.active : false;
+        // cars: ACTIVE_TAG (awake) or DISABLE_SIMULATION (demoed, never sleeps)
+        for (int s0 = 0; s0 < nb;) {
+            int id = ekey[elem[s0]];
+            int e0 = s0;
+            while (e0 < nb && ekey[elem[e0]] == id) e0++;
+            bool all_sleeping = true;
+            for (int q = s0; q < e0; q++) {
+                int bi = elem[q];
+                bool active_tag = (bi == 0) ? A.ball.b.active : A.cars[bi - 1].b.active;
+                if (active_tag) { all_sleeping = false; break; }
+            }
+            if (!all_sleeping) {
+                for (int q = s0; q < e0; q++) {
+                    int bi = elem[q];
+                    if (bi == 0 && sleeping[0]) {
+                        sleeping[0] = false;
+                        A.ball.b.active = true;  // WANTS_DEACTIVATION: integrated, but got no gravity
+                    }
+                }
+            }
+            s0 = e0;
+        }
+        // island manifolds in dispatcher order
+        int im[MAX_MANIFOLDS], ikey[MAX_MANIFOLDS];
+        int n_im = 0;
+        for (int m = 0; m < n_mf; m++) {
+            int b0 = mf[m].b0, b1 = mf[m].b1;
+            bool b0_sleep = b0 < 0 ? true : (b0 == 0 ? sleeping[0] : false);
+            bool b1_sleep = b1 < 0 ? true : (b1 == 0 ? sleeping[0] : false);
+            if (b0_sleep && b1_sleep) continue;
+            if (!(has_response(b0) && has_response(b1))) continue;
+            if (b0 < 0 && b1 < 0) continue;
+            im[n_im] = m;
+            ikey[m] = b0 >= 0 ? tag[b0] : tag[b1];
+            n_im++;
+        }
+        if (n_im > 1) quicksort_by_key(im, ikey, 0, n_im - 1);
+        // collect bodies of non-sleeping islands, in island order
+        int bodies[MAX_BODIES];
+        int n_bodies = 0;
+        for (int s0 = 0; s0 < nb;) {
+            int id = ekey[elem[s0]];
+            int e0 = s0;
+            bool island_sleeping = true;
+            while (e0 < nb && ekey[elem[e0]] == id) {
+                int bi = elem[e0];
+                if (body_ptr(bi)->active) island_sleeping = false;
+                e0++;
+            }
+            if (!island_sleeping)
+                for (int q = s0; q < e0; q++) bodies[n_bodies++] = elem[q];
+            s0 = e0;
+        }
+        // manifolds whose island is processed (all islands are batched)
+        Solver S;
+        solver_reset(S, dt);
+        struct BodyAt {
+            World* w;
+            RSG_FN const Body* operator()(int k) { return w->body_ptr(k); }
+        } body_at{this};
+        for (int i = 0; i < n_bodies; i++) solver_body_id(S, bodies[i], body_at);
+        for (int q = 0; q < n_im; q++) {
+            int m = im[q];
+            // manifolds of sleeping islands are not solved
+            int b0 = mf[m].b0, b1 = mf[m].b1;
+            int owner = b0 >= 0 ? b0 : b1;
+            bool in = false;
+            for (int i = 0; i < n_bodies; i++) if (bodies[i] == owner) in = true;
+            if (!in) continue;
+            convert_manifold(S, mf[m], body_at);
+        }
+        for (int i = 0; i < n_bodies; i++) {
+            int k = bodies[i];
+            if (S.special[k].num > 0) convert_special(S, k, body_at);
+        }
+        solver_iterate(S);
+        // write back (btSequentialImpulseConstraintSolver::writeBackBodies)
+        for (int i = 0; i < S.n_bodies; i++) {
+            SBody& sb = S.bodies[i];
+            if (sb.body < 0) continue;
+            Body& b = *body_ptr(sb.body);
+            sb.lin = sb.lin + sb.dv;
+            sb.ang = sb.ang + sb.dw;
+            if (sb.push.x != 0.f || sb.push.y != 0 || sb.push.z != 0 || sb.turn.x != 0.f || sb.turn.y != 0 || sb.turn.z != 0) {
+                V3 np; M3 nr;
+                integrate_transform(sb.xf.t, sb.xf.R, sb.push, sb.turn * SOLVER_SPLIT_TURN_ERP, dt, b.no_rot, np, nr);
+                sb.xf.t = np;
+                sb.xf.R = nr;
+            }
+            b.vel = sb.lin + sb.ext_f;
+            b.ang = sb.ang + sb.ext_t;
+            b.pos = sb.xf.t;
+            b.rot = sb.xf.R;
+        }
+        if (S.overflow) overflow = true;
+    }
+
+    RSG_FN static int uf_find(int* id, int x) {
+        while (x != id[x]) {
+            id[x] = id[id[x]];
+            x = id[x];
+        }
+        return x;
+    }
+
+    template <class BodyAt>
+    RSG_FN void convert_manifold(Solver& S, const Manifold& m, BodyAt& body_at) {
+        int ia = solver_body_id(S, m.b0, body_at);
+        int ib = solver_body_id(S, m.b1, body_at);
+        SBody& A_ = S.bodies[ia];
+        SBody& B_ = S.bodies[ib];
+        bool a_zero = fuzzy_zero(A_.inv_mass), b_zero = fuzzy_zero(B_.inv_mass);
+        if (a_zero && b_zero) return;
+        const Body* rb0 = m.b0 >= 0 ? body_ptr(m.b0) : nullptr;
+        const Body* rb1 = m.b1 >= 0 ? body_ptr(m.b1) : nullptr;
+        Xf x0 = xf_of(m.b0), x1 = xf_of(m.b1);
+        for (int j = 0; j < m.n; j++) {
+            const CPoint& cp = m.p[j];
+            if (S.n_rows >= MAX_ROWS) { S.overflow = true; return; }
+            int ci = S.n_rows++;
+            SRow& c = S.rows[ci];
+            c.a = ia; c.b = ib;
+            V3 rel1 = cp.pos_a - x0.t;
+            V3 rel2 = cp.pos_b - x1.t;
+            setup_contact_row(S, c, ia, ib, rb0, rb1, cp.normal, cp.dist, cp.friction, cp.restitution, rel1, rel2);
+            c.special = cp.special;
+            c.friction_index = S.n_fric;
+            if (cp.special) {
+                for (int i = 0; i < 2; i++) {
+                    int code = i ? m.b1 : m.b0;
+                    if (code >= 0) {
+                        SpecialInfo& sp = S.special[code];
+                        sp.num++;
+                        sp.friction = cp.friction;
+                        sp.restitution = cp.restitution;
+                        sp.total_normal += cp.normal;
+                        sp.total_dist += len(i ? rel2 : rel1);
+                    }
+                }
+            }
+            add_friction_for(S, ia, ib, rb0, rb1, cp.normal, cp.friction, rel1, rel2, ci);
+        }
+    }
+
+    template <class BodyAt>
+    RSG_FN void convert_special(Solver& S, int k, BodyAt& body_at) {
+        SpecialInfo& sp = S.special[k];
+        float distance = sp.total_dist / (float)sp.num;
+        V3 normal = sp.total_normal * (1.f / (float)sp.num);
+        int ia = solver_body_id(S, k, body_at);
+        int ib = solver_body_id(S, -1000, body_at);  // fixed body
+        if (S.n_rows >= MAX_ROWS) { S.overflow = true; return; }
+        int ci = S.n_rows++;
+        SRow& c = S.rows[ci];
+        c.a = ia; c.b = ib;
+        V3 rel1 = normal * -distance;
+        V3 rel2 = v3zero();
+        const Body* rb0 = body_ptr(k);
+        setup_contact_row(S, c, ia, ib, rb0, nullptr, normal, distance, sp.friction, sp.restitution, rel1, rel2);
+        c.special = false;
+        c.friction_index = S.n_fric;
+        add_friction_for(S, ia, ib, rb0, nullptr, normal, sp.friction, rel1, rel2, ci);
+        sp.num = 0;
     }
 };
 

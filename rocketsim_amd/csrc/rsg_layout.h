@@ -57,12 +57,15 @@ enum CarWord : int {
     CW_GEN_TICKS_LEFT = 82,
     CW_CONFIG = 83,        // preset | team << 8
     CW_ID = 84,
-    CAR_WORDS = 85
+    CW_BP_CELL = 85,       // broadphase cell the proxy is registered in
+    CW_BP_SEQ = 86,        // registration order among the arena's dynamic proxies
+    CAR_WORDS = 87
 };
 
 enum BallWord : int {
     BW_POS = 0, BW_ROT = 3, BW_VEL = 12, BW_ANGVEL = 15, BW_TICKS_SINCE_SET = 18,
-    BALL_WORDS = 20
+    BW_BP_CELL = 20, BW_BP_SEQ = 21,
+    BALL_WORDS = 22
 };
 
 enum PadWord : int { PW_ACTIVE = 0, PW_COOLDOWN = 1, PW_PREV_LOCKED = 2, PAD_WORDS = 3 };

@@ -100,6 +100,12 @@ RSG_FN void load_car(const CarRef& r, Car& c) {
     c.vel_cache = v3zero();
 }
 
+RSG_FN void load_bp(const CarRef& r, int w_cell, int w_seq, int& cell, uint32_t& seq) {
+    cell = (int)r.u(w_cell);
+    seq = r.u(w_seq);
+    if (cell < 0 || cell >= BP_NUM_CELLS) cell = 0;
+}
+
 RSG_FN void store_car(const CarRef& r, const Car& c) {
     r.sv(CW_POS, c.b.pos);
     store_rot(r, CW_ROT, c.b.rot);
@@ -177,8 +183,10 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.si.erp = 0.2f;
     A.si.time_step = d.tick_time;
     for (int i = 0; i < A.n_cars; i++) {
-        load_car(car_ref(d, a, i), A.cars[i]);
+        CarRef cr = car_ref(d, a, i);
+        load_car(cr, A.cars[i]);
         car_init_body(A.cars[i]);
+        load_bp(cr, CW_BP_CELL, CW_BP_SEQ, A.bp_cell[1 + i], A.bp_seq[1 + i]);
     }
     CarRef br = ball_ref(d, a);
     A.ball.b.pos = br.v(BW_POS);
@@ -187,6 +195,7 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.ball.b.ang = br.v(BW_ANGVEL);
     A.ball.ticks_since_set = br.u64(BW_TICKS_SINCE_SET);
     ball_init_body(A.ball);
+    load_bp(br, BW_BP_CELL, BW_BP_SEQ, A.bp_cell[0], A.bp_seq[0]);
     if (d.mode == RSG_MODE_SOCCAR) {
         for (int p = 0; p < PAD_NUM; p++) {
             CarRef pr = pad_ref(d, a, p);
@@ -206,8 +215,15 @@ RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
     ar.su64(AW_LAST_GOAL_TICK, A.last_goal_tick);
     ar.u(AW_RESETS) = A.resets;
     ar.u(AW_STATUS) = A.status;
-    for (int i = 0; i < A.n_cars; i++) store_car(car_ref(d, a, i), A.cars[i]);
+    for (int i = 0; i < A.n_cars; i++) {
+        CarRef cr = car_ref(d, a, i);
+        store_car(cr, A.cars[i]);
+        cr.u(CW_BP_CELL) = (uint32_t)A.bp_cell[1 + i];
+        cr.u(CW_BP_SEQ) = A.bp_seq[1 + i];
+    }
     CarRef br = ball_ref(d, a);
+    br.u(BW_BP_CELL) = (uint32_t)A.bp_cell[0];
+    br.u(BW_BP_SEQ) = A.bp_seq[0];
     br.sv(BW_POS, A.ball.b.pos);
     store_rot(br, BW_ROT, A.ball.b.rot);
     br.sv(BW_VEL, A.ball.b.vel);
@@ -336,8 +352,7 @@ RSG_FN void pads_check_car(ArenaLocal& A, int ci) {
 // ---------------------------------------------------------------------------
 // The tick
 
-template <class MeshT>
-RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshT& mesh) {
+RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh) {
     const float dt = P.dt;
     const bool soccar = P.mode == RSG_MODE_SOCCAR;
 
@@ -360,7 +375,7 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshT& mesh) {
     // 1. ball sleep flag (Arena.cpp:694-700)
     A.ball.b.active = !(len2(A.ball.b.vel) == 0.f && len2(A.ball.b.ang) == 0.f);
 
-    World<MeshT> world(A, mesh, soccar);
+    World world(A, mesh, soccar && mesh != nullptr);
 
     // 2. car pre-tick (Car.cpp:58-131)
     for (int ci = 0; ci < A.n_cars; ci++) {

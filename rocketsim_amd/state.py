@@ -66,6 +66,7 @@ class CarState(C.Structure):
         ("last_controls", Controls),
         ("wheel_engine_force", C.c_float), ("wheel_brake", C.c_float), ("wheel_steer", C.c_float),
         ("wheel_lat_friction", F4), ("wheel_long_friction", F4), ("wheel_extra_pushback", F4),
+        ("bp_cell", C.c_int32), ("bp_seq", C.c_uint32),
     ]
 
 
@@ -73,6 +74,7 @@ class BallState(C.Structure):
     _fields_ = [
         ("tick_count_since_update", C.c_uint64),
         ("pos", F3), ("rot", F9), ("vel", F3), ("ang_vel", F3),
+        ("bp_cell", C.c_int32), ("bp_seq", C.c_uint32),
     ]
 
 

@@ -28,7 +28,11 @@ from rocketsim_amd.state import (BallState, CarState, Controls, PadState, CF_HAS
                                  CF_ON_GROUND, MODE_SOCCAR, MODE_THE_VOID, NUM_PADS)
 
 EXACT_CAR = {"flags", "car_contact_other_id", "hit_tick", "hit_extra_tick", "tick_count_since_update",
-             "last_controls.jump", "last_controls.boost", "last_controls.handbrake", "last_controls._pad"}
+             "last_controls.jump", "last_controls.boost", "last_controls.handbrake", "last_controls._pad",
+             "bp_cell"}
+# Registration order numbers: only their relative order across the arena's
+# bodies is state (compared by bp_order below).
+SKIP_CAR = {"bp_seq"}
 REL_FLOOR = {"pos": 1.0, "vel": 1.0, "ang_vel": 1e-2}
 ROT_TOL = 1e-4
 REL_TOL = 1e-3
@@ -53,6 +57,8 @@ def compare_car(ref, got):
     fr, fg = flat_fields(ref), flat_fields(got)
     for k, rv in fr.items():
         gv = fg[k]
+        if k in SKIP_CAR:
+            continue
         if k in EXACT_CAR:
             if rv != gv:
                 bad.append((k, rv, gv, None))
@@ -84,7 +90,15 @@ def compare_ball(ref, got):
         bad.append(("ball.rot", list(a), list(b), np.max(np.abs(a - b))))
     if ref.tick_count_since_update != got.tick_count_since_update:
         bad.append(("ball.tick_count_since_update", ref.tick_count_since_update, got.tick_count_since_update, None))
+    if ref.bp_cell != got.bp_cell:
+        bad.append(("ball.bp_cell", ref.bp_cell, got.bp_cell, None))
     return bad
+
+
+def bp_order(ball, cars):
+    """Broadphase registration order of (ball, cars...) as a permutation."""
+    seq = [ball.bp_seq] + [c.bp_seq for c in cars]
+    return sorted(range(len(seq)), key=lambda i: (seq[i], i))
 
 
 def compare_pads(ref, got):
@@ -275,6 +289,8 @@ def check_samples(samples, results, max_report=10):
         for j in range(len(s.teams)):
             bad += [(f"car{j}." + b[0],) + b[1:] for b in compare_car(s.cars_out[j], cars[j])]
         bad += compare_ball(s.ball_out, ball)
+        if bp_order(s.ball_out, s.cars_out) != bp_order(ball, cars):
+            bad.append(("bp_order", bp_order(s.ball_out, s.cars_out), bp_order(ball, cars), None))
         if s.mode == MODE_SOCCAR:
             bad += compare_pads(s.pads_out, pads)
         if tick != s.tick_out:
