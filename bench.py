@@ -37,11 +37,13 @@ def cpu_baseline(target_seconds=10.0):
         return None
     ref.init([arena_mesh.soccar_cmf()])
     threads = min(16, os.cpu_count() or 1)
-    # calibrate on a short run, then size the sample to ~target_seconds
-    n, secs = ref.run_workload(threads, 1, 2, 2, 0, 500, 12345)
-    rate = n / max(secs, 1e-9)
-    ticks = int(max(1000, min(200000, rate * target_seconds / threads)))
-    n, secs = ref.run_workload(threads, 1, 2, 2, 0, ticks, 12345)
+    # grow the sample until it takes at least half the target wall time
+    ticks = 2000
+    while True:
+        n, secs = ref.run_workload(threads, 1, 2, 2, 0, ticks, 12345)
+        if secs >= 0.5 * target_seconds or ticks >= 4_000_000:
+            break
+        ticks = int(ticks * min(50.0, max(2.0, target_seconds / max(secs, 1e-3))))
     return {
         "value": n / secs, "unit": "arena-ticks/s", "cores": threads, "kind": "reference",
         "sample": f"{threads} threads x 1 Soccar 2v2 Octane arena x {ticks} ticks, same control process, "

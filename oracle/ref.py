@@ -62,9 +62,16 @@ def lib():
 _initialised = False
 
 
-def init(meshes=()):
-    """RocketSim::InitFromMem with the given .cmf blobs (bytes) for SOCCAR."""
+def init(meshes=None):
+    """RocketSim::InitFromMem with the given .cmf blobs (bytes) for SOCCAR,
+    by default the procedural Soccar arena of rocketsim_amd.arena_mesh.  The
+    reference initialises once per process; later calls are no-ops."""
     global _initialised
+    if _initialised:
+        return
+    if meshes is None:
+        from rocketsim_amd import arena_mesh
+        meshes = [arena_mesh.soccar_cmf()]
     L = lib()
     n = len(meshes)
     bufs = [C.create_string_buffer(m, len(m)) for m in meshes]

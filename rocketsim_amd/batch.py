@@ -12,10 +12,28 @@ from .state import (BallState, BatchDesc, CarState, Controls, MODE_SOCCAR, MODE_
                     NUM_PADS, PadState)
 
 
+_mesh_loaded = False
+
+
+def init_mesh(cmf=None):
+    """Load the Soccar collision mesh (RocketSim::Init / InitFromMem,
+    src/RocketSim.cpp:92-185): `cmf` is the bytes of one .cmf file; by default
+    the procedural arena of rocketsim_amd.arena_mesh."""
+    global _mesh_loaded
+    if cmf is None:
+        from . import arena_mesh
+        cmf = arena_mesh.soccar_cmf()
+    buf = C.create_string_buffer(bytes(cmf), len(cmf))
+    native.check(native.lib().rsg_init_mesh(buf, len(cmf)))
+    _mesh_loaded = True
+
+
 class ArenaBatch:
     def __init__(self, cars_per_arena, teams, presets, mode=MODE_SOCCAR, tick_rate=120.0,
                  seed=0, auto_reset=False, synth_controls=False):
         L = native.lib()
+        if mode == MODE_SOCCAR and not _mesh_loaded:
+            init_mesh()
         n = len(cars_per_arena)
         total = sum(cars_per_arena)
         if len(teams) != total or len(presets) != total:

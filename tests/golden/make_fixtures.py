@@ -20,10 +20,13 @@ import parity  # noqa: E402
 
 
 def main():
-    ref.init([])
+    ref.init()
     void = parity.sample_void(ref, n_traj=24, ticks_per_traj=8, seed=20261202)
     parity.save_samples(os.path.join(HERE, "void_tf.npz"), void)
     print("void_tf.npz:", len(void), "samples")
+    ball = parity.sample_soccar_ball(ref, n_traj=32, ticks_per_traj=96, seed=20261203, stride=8)
+    parity.save_samples(os.path.join(HERE, "soccar_ball_tf.npz"), ball)
+    print("soccar_ball_tf.npz:", len(ball), "samples")
 
 
 if __name__ == "__main__":

@@ -174,6 +174,52 @@ def sample_void(ref, n_traj, ticks_per_traj, seed):
     return out
 
 
+def _soccar_scenario(ref, rng, kind, teams, presets):
+    """A Soccar arena at a random kickoff with the ball thrown at the floor,
+    a side wall, a corner or into a goal (the arena mesh and the four planes)."""
+    A = ref.RefArena(MODE_SOCCAR)
+    for t, p in zip(teams, presets):
+        A.add_car(t, p)
+    A.reset_kickoff(rng.randint(0, 1000))
+    b = A.get_ball()
+    sx, sy = rng.choice([-1, 1]), rng.choice([-1, 1])
+    if kind == "drop":
+        b.pos[:] = [rng.uniform(-3500, 3500), rng.uniform(-4500, 4500), rng.uniform(150, 1800)]
+        b.vel[:] = [rng.gauss(0, 800), rng.gauss(0, 800), rng.gauss(0, 600)]
+    elif kind == "wall":
+        b.pos[:] = [sx * rng.uniform(3000, 3900), rng.uniform(-3000, 3000), rng.uniform(150, 1500)]
+        b.vel[:] = [sx * rng.uniform(500, 3000), rng.gauss(0, 500), rng.gauss(0, 500)]
+    elif kind == "corner":
+        b.pos[:] = [sx * rng.uniform(2800, 3700), sy * rng.uniform(3800, 4800), rng.uniform(150, 1500)]
+        b.vel[:] = [sx * rng.uniform(500, 3000), sy * rng.uniform(500, 3000), rng.gauss(0, 500)]
+    else:  # goal
+        b.pos[:] = [rng.uniform(-1200, 1200), sy * rng.uniform(4200, 5000), rng.uniform(150, 900)]
+        b.vel[:] = [rng.gauss(0, 800), sy * rng.uniform(500, 3000), rng.gauss(0, 500)]
+    b.ang_vel[:] = [rng.gauss(0, 2) for _ in range(3)]
+    A.set_ball(b)
+    return A
+
+
+def sample_soccar_ball(ref, n_traj, ticks_per_traj, seed, stride=1):
+    """Ball against the arena (floor, walls, corners, goals) with two idle
+    cars resting on their wheels at kickoff: every `stride`-th tick of each
+    trajectory becomes a sample."""
+    rng = random.Random(seed)
+    kinds = ("drop", "wall", "corner", "goal")
+    out = []
+    for k in range(n_traj):
+        teams, presets = [0, 1], [rng.randint(0, 5), rng.randint(0, 5)]
+        A = _soccar_scenario(ref, rng, kinds[k % len(kinds)], teams, presets)
+        zero = [Controls() for _ in teams]
+        for t in range(ticks_per_traj):
+            if t % stride == 0:
+                out.append(_one_tick(A, MODE_SOCCAR, teams, presets, zero))
+            else:
+                A.step(1)
+        A.close()
+    return out
+
+
 def _one_tick(A, mode, teams, presets, ctrls):
     s = Sample()
     s.mode = mode

@@ -15,7 +15,7 @@ def _oracle():
     from oracle import ref
     if not ref.available():
         return None
-    ref.init([])
+    ref.init()
     return ref
 
 
