@@ -80,6 +80,16 @@ int rsg_batch_set_pads(rsg_batch* b, const rsg_pad_state* in);
 /* Car::controls (Car.h:149), held for the following rsg_batch_step calls. */
 int rsg_batch_set_controls(rsg_batch* b, const rsg_controls* in);
 int rsg_batch_get_controls(rsg_batch* b, rsg_controls* out);
+/* The order the per-car updates run in: the iteration order of the
+ * reference's Arena::_cars, an std::unordered_set<Car*> (Arena.h:35) walked
+ * by Arena::Step's pre-tick and post-tick/boost-pad loops (Arena.cpp:706-734)
+ * and by ResetToRandomKickoff (Arena.cpp:150-152).  Its order follows the
+ * cars' heap addresses; with libstdc++ and no bucket collision it is the
+ * reverse of creation order, which is the default here.  `order` holds, per
+ * arena (same flattening as the car arrays), a permutation of the arena's
+ * car indices; anything else is rejected. */
+int rsg_batch_set_car_order(rsg_batch* b, const int32_t* order);
+int rsg_batch_get_car_order(rsg_batch* b, int32_t* order);
 /* Arena::tickCount (Arena.h:61). */
 int rsg_batch_get_tick_counts(rsg_batch* b, uint64_t* out);
 int rsg_batch_set_tick_counts(rsg_batch* b, const uint64_t* in);

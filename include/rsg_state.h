@@ -65,7 +65,14 @@ enum {
     RSG_CF_AUTO_FLIPPING = 1u << 12,
     RSG_CF_WORLD_CONTACT = 1u << 13,
     RSG_CF_DEMOED = 1u << 14,
-    RSG_CF_HIT_VALID = 1u << 15
+    RSG_CF_HIT_VALID = 1u << 15,
+    /* The car's rigid body has simulation disabled and no contact response
+     * (btCollisionObject::CF_NO_CONTACT_RESPONSE + DISABLE_SIMULATION).  The
+     * reference sets this in the pre-tick update of a demolished car and
+     * clears it in that of a live one (src/Sim/Car/Car.cpp:68-84); it
+     * persists on the body in between, so the wheel rays of cars updated
+     * earlier in a tick see the value left by the previous tick. */
+    RSG_CF_BODY_DISABLED = 1u << 16
 };
 
 typedef struct rsg_car_state {

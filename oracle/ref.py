@@ -49,6 +49,7 @@ def lib():
         L.rsref_get_tick_count.argtypes = [C.c_void_p]
         L.rsref_set_tick_count.argtypes = [C.c_void_p, C.c_uint64]
         L.rsref_step.argtypes = [C.c_void_p, C.c_int]
+        L.rsref_car_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
         L.rsref_is_ball_scored.argtypes = [C.c_void_p]
         L.rsref_reset_kickoff.argtypes = [C.c_void_p, C.c_int]
         L.rsref_pad_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int)]
@@ -132,6 +133,12 @@ class RefArena:
 
     def set_controls(self, i, c):
         self.L.rsref_set_controls(self.h, i, C.byref(c))
+
+    def car_order(self):
+        """Iteration order of the reference's car set (creation indices)."""
+        out = (C.c_int32 * 8)()
+        n = self.L.rsref_car_order(self.h, out)
+        return list(out)[:n]
 
     @property
     def tick_count(self):

@@ -20,6 +20,13 @@
 #include "rsg_workload.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/BroadphaseCollision/btRSBroadphase.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btBvhTriangleMeshShape.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btBoxShape.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btSphereShape.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btTriangleShape.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btGjkPairDetector.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btVoronoiSimplexSolver.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btGjkEpaPenetrationDepthSolver.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btSubSimplexConvexCast.h"
 
 #include <atomic>
 #include <chrono>
@@ -181,6 +188,7 @@ void rsref_get_car(void* h, int i, rsg_car_state* s) {
     if (cs.worldContact.hasContact) f |= RSG_CF_WORLD_CONTACT;
     if (cs.isDemoed) f |= RSG_CF_DEMOED;
     if (cs.ballHitInfo.isValid) f |= RSG_CF_HIT_VALID;
+    if (car->_rigidBody.m_collisionFlags & btCollisionObject::CF_NO_CONTACT_RESPONSE) f |= RSG_CF_BODY_DISABLED;
     s->flags = f;
     vget(cs.flipRelTorque, s->flip_rel_torque);
     s->jump_time = cs.jumpTime; s->flip_time = cs.flipTime;
@@ -248,6 +256,13 @@ void rsref_set_car(void* h, int i, const rsg_car_state* s) {
     cset(cs.lastControls, s->last_controls);
     car->SetState(cs);
     car->_internalState.tickCountSinceUpdate = s->tick_count_since_update;
+    if (f & RSG_CF_BODY_DISABLED) {
+        car->_rigidBody.m_activationState1 = DISABLE_SIMULATION;
+        car->_rigidBody.m_collisionFlags |= btCollisionObject::CF_NO_CONTACT_RESPONSE;
+    } else {
+        car->_rigidBody.m_activationState1 = ACTIVE_TAG;
+        car->_rigidBody.m_collisionFlags &= ~btCollisionObject::CF_NO_CONTACT_RESPONSE;
+    }
     auto& wi = car->_bulletVehicle.m_wheelInfo;
     for (int w = 0; w < 4; w++) {
         wi[w].m_engineForce = s->wheel_engine_force;
@@ -582,6 +597,83 @@ int rsref_debug_add_loop(void* h, const float* mn, const float* mx, int* out, in
     return n;
 }
 
+// The reference's GJK pair detector on (box, shape B): box given by its full
+// half extents (btBoxShape constructor semantics), at transform xa (basis rows
+// then origin); B is a sphere (kind 0, radius r) or a triangle (kind 2, verts).
+// Returns 1 with normal / point on B / distance when a contact is reported.
+namespace {
+struct CaptureResult : public btDiscreteCollisionDetectorInterface::Result {
+    bool has = false; btVector3 n, p; btScalar d = 0;
+    void setShapeIdentifiersA(int, int) override {}
+    void setShapeIdentifiersB(int, int) override {}
+    void addContactPoint(const btVector3& normal, const btVector3& point, btScalar depth) override {
+        has = true; n = normal; p = point; d = depth;
+    }
+};
+btTransform xf_from_floats(const float* x) {
+    btMatrix3x3 m(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8]);
+    return btTransform(m, btVector3(x[9], x[10], x[11]));
+}
+}
+
+int rsref_gjk(const float* box_half, const float* xa, int kind_b, float radius, const float* tri,
+              const float* xb, float max_dist2, float* out_n, float* out_p, float* out_d, int* method) {
+    btBoxShape box(btVector3(box_half[0], box_half[1], box_half[2]));
+    btSphereShape sph(radius);
+    btTriangleShape tr(btVector3(tri[0], tri[1], tri[2]), btVector3(tri[3], tri[4], tri[5]), btVector3(tri[6], tri[7], tri[8]));
+    tr.setMargin(0);
+    const btConvexShape* b = kind_b == 0 ? (const btConvexShape*)&sph : (const btConvexShape*)&tr;
+    btVoronoiSimplexSolver simplex;
+    btGjkEpaPenetrationDepthSolver pd;
+    btGjkPairDetector det(&box, b, &simplex, &pd);
+    det.setMinkowskiA(&box);
+    det.setMinkowskiB(b);
+    btGjkPairDetector::ClosestPointInput in;
+    in.m_maximumDistanceSquared = max_dist2;
+    in.m_transformA = xf_from_floats(xa);
+    in.m_transformB = xf_from_floats(xb);
+    CaptureResult r;
+    det.getClosestPoints(in, r);
+    *method = det.m_lastUsedMethod * 100 + det.m_degenerateSimplex;
+    if (!r.has) return 0;
+    for (int k = 0; k < 3; k++) { out_n[k] = r.n[k]; out_p[k] = r.p[k]; }
+    *out_d = r.d;
+    return 1;
+}
+
+// The reference's ray vs convex test (btSubsimplexConvexCast of a point, as in
+// btCollisionWorld::rayTestSingleInternal): shape kind 0 sphere (radius r),
+// 1 box (full half extents).  Returns 1 and fraction / normal on a hit.
+int rsref_ray_convex(int kind, float radius, const float* box_half, const float* xb,
+                     const float* from, const float* to, float* frac, float* normal) {
+    btSphereShape point(0.f);
+    point.setMargin(0.f);
+    btSphereShape sph(radius);
+    btBoxShape box(btVector3(box_half[0], box_half[1], box_half[2]));
+    const btConvexShape* b = kind == 0 ? (const btConvexShape*)&sph : (const btConvexShape*)&box;
+    btVoronoiSimplexSolver simplex;
+    btSubsimplexConvexCast caster(&point, b, &simplex);
+    btConvexCast::CastResult res;
+    res.m_fraction = 1.f;
+    btTransform ft(btMatrix3x3::getIdentity(), btVector3(from[0], from[1], from[2]));
+    btTransform tt(btMatrix3x3::getIdentity(), btVector3(to[0], to[1], to[2]));
+    btTransform bt = xf_from_floats(xb);
+    if (!caster.calcTimeOfImpact(ft, tt, bt, bt, res)) return 0;
+    if (!(res.m_normal.length2() > btScalar(0.0001))) return 0;
+    res.m_normal.normalize();
+    *frac = res.m_fraction;
+    for (int k = 0; k < 3; k++) normal[k] = res.m_normal[k];
+    return 1;
+}
+
+// Box margin and implicit half extents the btBoxShape constructor derives.
+void rsref_box_dims(const float* half, float* implicit, float* margin) {
+    btBoxShape box(btVector3(half[0], half[1], half[2]));
+    btVector3 im = box.getImplicitShapeDimensions();
+    for (int k = 0; k < 3; k++) implicit[k] = im[k];
+    *margin = box.getMargin();
+}
+
 // Broadphase proxy AABB of static object s (_worldCollisionRBs order).
 void rsref_static_aabb(void* h, int s, float* mn, float* mx) {
     auto* ra = (RefArena*)h;
@@ -590,3 +682,79 @@ void rsref_static_aabb(void* h, int s, float* mn, float* mx) {
 }
 
 }  // extern "C"
+
+// Wheel w of car i after the last step: ground body code (as in
+// rsref_get_manifold, -2000 when not in contact) and the friction impulse.
+extern "C" int rsref_wheel_debug(void* h, int car, int w, float* impulse, float* extra) {
+    auto* ra = (RefArena*)h;
+    Car* c = ra->cars[car];
+    const btWheelInfoRL& wi = c->_bulletVehicle.m_wheelInfo[w];
+    for (int k = 0; k < 3; k++) impulse[k] = wi.m_impulse[k];
+    extra[0] = wi.m_wheelsSuspensionForce;
+    extra[1] = wi.m_raycastInfo.m_suspensionLength;
+    for (int k = 0; k < 3; k++) {
+        extra[2 + k] = wi.m_raycastInfo.m_hardPointWS[k];
+        extra[5 + k] = wi.m_raycastInfo.m_contactPointWS[k];
+        extra[8 + k] = wi.m_raycastInfo.m_contactNormalWS[k];
+    }
+    const btCollisionObject* g = (const btCollisionObject*)wi.m_raycastInfo.m_groundObject;
+    return g ? body_code(ra, g) : -2000;
+}
+
+// Iteration order of Arena::_cars (an unordered_set<Car*>) as indices into
+// the creation order; it is the order of the per-car update loops.
+extern "C" int rsref_car_order(void* h, int32_t* out) {
+    auto* ra = (RefArena*)h;
+    int k = 0;
+    for (Car* c : ra->arena->_cars) {
+        int idx = -1;
+        for (size_t i = 0; i < ra->cars.size(); i++)
+            if (ra->cars[i] == c) idx = (int)i;
+        out[k++] = idx;
+    }
+    return k;
+}
+
+// World transform of car i's hitbox child (rows of the basis, then origin),
+// as btCollisionWorld's compound ray test forms it.
+extern "C" void rsref_car_child_xf(void* h, int car, float* out) {
+    auto* ra = (RefArena*)h;
+    Car* c = ra->cars[car];
+    btTransform t = c->_rigidBody.getWorldTransform() * c->_compoundShape.getChildTransform(0);
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) out[i * 3 + k] = t.getBasis()[i][k];
+    for (int k = 0; k < 3; k++) out[9 + k] = t.getOrigin()[k];
+}
+
+// btDefaultVehicleRaycaster::castRay through the arena's world (ignoring car
+// `ignore_car`, -1 for none): returns the hit body code, -2000 on no hit.
+extern "C" int rsref_world_ray(void* h, const float* from, const float* to, int ignore_car, float* frac, float* normal) {
+    auto* ra = (RefArena*)h;
+    btVector3 f(from[0], from[1], from[2]), t(to[0], to[1], to[2]);
+    Car* c = ra->cars[ignore_car];
+    ra->arena->_bulletWorld.updateAabbs();
+    btVehicleRaycaster::btVehicleRaycasterResult res;
+    const btCollisionObject* o = (const btCollisionObject*)c->_bulletVehicleRaycaster.castRay(f, t, &c->_rigidBody, res);
+    if (!o) return -2000;
+    *frac = res.m_distFraction;
+    for (int k = 0; k < 3; k++) normal[k] = res.m_hitNormalInWorld[k];
+    return body_code(ra, o);
+}
+
+// btCollisionWorld::rayTestSingle of one car's collision object (its
+// compound shape at its world transform), as the world ray test runs it.
+extern "C" int rsref_ray_single_car(void* h, const float* from, const float* to, int car, float* frac, float* normal) {
+    auto* ra = (RefArena*)h;
+    btVector3 f(from[0], from[1], from[2]), t(to[0], to[1], to[2]);
+    Car* c = ra->cars[car];
+    btTransform ft(btMatrix3x3::getIdentity(), f), tt(btMatrix3x3::getIdentity(), t);
+    btCollisionWorld::ClosestRayResultCallback cb(f, t, nullptr);
+    btCollisionWorld::rayTestSingle(ft, tt, &c->_rigidBody, c->_rigidBody.getCollisionShape(),
+                                    c->_rigidBody.getWorldTransform(), cb);
+    if (!cb.hasHit()) return 0;
+    btVector3 n = cb.m_hitNormalWorld;
+    n.normalize();
+    *frac = cb.m_closestHitFraction;
+    for (int k = 0; k < 3; k++) normal[k] = n[k];
+    return 1;
+}

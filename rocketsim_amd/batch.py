@@ -107,6 +107,16 @@ class ArenaBatch:
         native.check(self.L.rsg_batch_get_controls(self.h, out))
         return out
 
+    def get_car_order(self):
+        """Per-arena car update order (flattened like the car arrays)."""
+        out = (C.c_int32 * max(1, self.n_cars))()
+        native.check(self.L.rsg_batch_get_car_order(self.h, out))
+        return list(out)[:self.n_cars]
+
+    def set_car_order(self, order):
+        arr = (C.c_int32 * max(1, self.n_cars))(*order)
+        native.check(self.L.rsg_batch_set_car_order(self.h, arr))
+
     def get_tick_counts(self):
         out = (C.c_uint64 * self.n_arenas)()
         native.check(self.L.rsg_batch_get_tick_counts(self.h, out))

@@ -46,7 +46,7 @@ inline void car_to_api(const CarRef& r, rsg_car_state& s) {
     s.hit_extra_tick = c.hit_extra_tick;
     V3 p = c.b.pos * BT_TO_UU, v = c.b.vel * BT_TO_UU;
     s.pos[0] = p.x; s.pos[1] = p.y; s.pos[2] = p.z;
-    rot_rows_to_api(c.b.rot, s.rot);
+    rot_rows_to_api(c.api_rot, s.rot);
     s.vel[0] = v.x; s.vel[1] = v.y; s.vel[2] = v.z;
     s.ang_vel[0] = c.b.ang.x; s.ang_vel[1] = c.b.ang.y; s.ang_vel[2] = c.b.ang.z;
     s.flags = c.flags;
@@ -79,6 +79,7 @@ inline void car_from_api(const CarRef& r, const rsg_car_state& s) {
     load_car(r, c);
     c.b.pos = v3(s.pos[0], s.pos[1], s.pos[2]) * UU_TO_BT;
     rot_api_to_rows(s.rot, c.b.rot);
+    c.api_rot = c.b.rot;
     c.b.vel = v3(s.vel[0], s.vel[1], s.vel[2]) * UU_TO_BT;
     c.b.ang = v3(s.ang_vel[0], s.ang_vel[1], s.ang_vel[2]);
     c.flags = s.flags;

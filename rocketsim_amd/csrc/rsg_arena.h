@@ -21,6 +21,10 @@ struct ArenaLocal {
     // broadphase registration of the dynamic proxies (0 = ball, 1 + i = car i)
     int bp_cell[1 + MAX_CARS];
     uint32_t bp_seq[1 + MAX_CARS];
+    // Order in which the per-car updates run (Arena::_cars is an
+    // unordered_set<Car*>, src/Sim/Arena/Arena.h:35; it is iterated for the
+    // pre-tick, post-tick/boost-pad and kickoff loops).
+    uint8_t order[MAX_CARS];
 };
 
 struct TickParams {

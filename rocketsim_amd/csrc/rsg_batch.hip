@@ -225,6 +225,10 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     for (int a = 0; a < b->dev.n; a++) {
         CarRef ar = arena_ref(h, a);
         ar.u(AW_NUM_CARS) = (uint32_t)b->cars_per_arena[a];
+        uint32_t ow = 0;  // default car update order: reverse creation
+        for (int k = 0; k < b->cars_per_arena[a]; k++)
+            ow |= (uint32_t)(b->cars_per_arena[a] - 1 - k) << (4 * k);
+        ar.u(AW_CAR_ORDER) = ow;
         for (int i = 0; i < b->cars_per_arena[a]; i++) {
             int k = b->car_offset[a] + i;
             CarRef cr = car_ref(h, a, i);
@@ -396,6 +400,43 @@ int rsg_batch_get_controls(rsg_batch* b, rsg_controls* out) {
     for (int a = 0; a < b->dev.n; a++)
         for (int i = 0; i < b->cars_per_arena[a]; i++)
             out[b->car_offset[a] + i] = ctrl_to_api(load_ctrl(car_ref(h, a, i), CW_CTRL));
+    return 0;
+}
+
+int rsg_batch_set_car_order(rsg_batch* b, const int32_t* order) {
+    if (!b || !order) return fail("rsg_batch_set_car_order: null argument");
+    for (int a = 0; a < b->dev.n; a++) {
+        const int n = b->cars_per_arena[a];
+        uint32_t seen = 0;
+        for (int k = 0; k < n; k++) {
+            int32_t v = order[b->car_offset[a] + k];
+            if (v < 0 || v >= n || (seen >> v & 1u))
+                return fail("rsg_batch_set_car_order: arena " + std::to_string(a) +
+                            " order is not a permutation of its car indices");
+            seen |= 1u << v;
+        }
+    }
+    if (download(b) != 0) return -1;
+    BatchDev h = host_view(b);
+    for (int a = 0; a < b->dev.n; a++) {
+        uint32_t ow = 0;
+        for (int k = 0; k < b->cars_per_arena[a]; k++)
+            ow |= (uint32_t)order[b->car_offset[a] + k] << (4 * k);
+        arena_ref(h, a).u(AW_CAR_ORDER) = ow;
+    }
+    b->host_dirty = true;
+    return 0;
+}
+
+int rsg_batch_get_car_order(rsg_batch* b, int32_t* order) {
+    if (!b || !order) return fail("rsg_batch_get_car_order: null argument");
+    if (download(b) != 0) return -1;
+    BatchDev h = host_view(b);
+    for (int a = 0; a < b->dev.n; a++) {
+        uint32_t ow = arena_ref(h, a).u(AW_CAR_ORDER);
+        for (int k = 0; k < b->cars_per_arena[a]; k++)
+            order[b->car_offset[a] + k] = (int32_t)((ow >> (4 * k)) & 0xFu);
+    }
     return 0;
 }
 

@@ -18,6 +18,8 @@
 #pragma once
 
 #include "rsg_solver.h"
+#include "rsg_gjk.h"
+#include "rsg_boxbox.h"
 
 namespace rsg {
 
@@ -73,11 +75,13 @@ struct World {
     RSG_FN Body* body_ptr(int i) { return i == 0 ? &A.ball.b : &A.cars[i - 1].b; }
     RSG_FN int num_bodies() const { return 1 + A.n_cars; }
     RSG_FN bool is_car(int k) const { return k >= 1; }
-    // contact response: demoed cars have CF_NO_CONTACT_RESPONSE this tick
+    // Contact response: a car demolished before this tick (or respawning in it)
+    // has CF_NO_CONTACT_RESPONSE for the whole tick (Car::_PreTickUpdate,
+    // src/Sim/Car/Car.cpp:67-83); a demolition during the tick takes effect on
+    // the next one, so this reads the pre-tick state kept in Body::active.
     RSG_FN bool has_response(int code) {
-        if (code < 0) return true;
-        if (code == 0) return true;
-        return !flag(A.cars[code - 1].flags, RSG_CF_DEMOED) && A.cars[code - 1].b.active;
+        if (code <= 0) return true;
+        return A.cars[code - 1].b.active;
     }
     RSG_FN bool active(int code) {
         if (code < 0) return false;
@@ -159,13 +163,13 @@ struct World {
         h.fraction = 1.f;
         h.point = to;
         h.normal = v3zero();
-        (void)self_body;
-        if (!soccar) return h;
+        if (!soccar && num_bodies() < 2) return h;
         int c = bp_cell_of(from);
         float frac = 1.f;
         V3 n = v3zero();
         int hit_obj = -100;
-        for (int s = 0; s < NUM_STATICS; s++) {
+        // static objects registered in the cell, in registration order
+        for (int s = 0; s < NUM_STATICS && soccar; s++) {
             if (!static_in_cell(s, c)) continue;
             if (frac == 0.f) break;
             if (s == ST_MESH) {
@@ -183,9 +187,44 @@ struct World {
                 if (ray_plane(s, from, to, frac, n)) hit_obj = -1 - s;
             }
         }
+        // then the dynamic proxies registered in the cell (cell-list order),
+        // except the casting car (btCollisionWorld::RayResultCallback::needsCollision)
+        const int nb = num_bodies();
+        int ord[MAX_BODIES];
+        for (int i = 0; i < nb; i++) ord[i] = i;
+        for (int i = 1; i < nb; i++) {
+            int v = ord[i], j = i - 1;
+            while (j >= 0 && A.bp_seq[ord[j]] > A.bp_seq[v]) { ord[j + 1] = ord[j]; j--; }
+            ord[j + 1] = v;
+        }
+        for (int r = 0; r < nb; r++) {
+            int q = ord[r];
+            if (!bp_cells_adjacent(A.bp_cell[q], c)) continue;
+            if (frac == 0.f) break;
+            if (q == self_body) continue;
+            float f;
+            V3 nn;
+            if (q == 0) {
+                CShape sph = shape_sphere(make_ball_phys().radius);
+                if (ray_convex(from, to, sph, xf_from(A.ball.b.pos, A.ball.b.rot), f, nn)) {
+                    if (len2(nn) > 0.0001f && f < frac) { frac = f; n = normalized(nn); hit_obj = q; }
+                }
+            } else {
+                CShape box = car_box_shape(q);
+#ifdef RSG_HOST_RAY_HOOK
+                RSG_HOST_RAY_HOOK(from, to, box, car_child_xf(q));
+#endif
+                if (ray_convex(from, to, box, car_child_xf(q), f, nn)) {
+                    if (len2(nn) > 0.0001f && f < frac) { frac = f; n = normalized(nn); hit_obj = q; }
+                }
+            }
+        }
         if (hit_obj != -100) {
+            // btDefaultVehicleRaycaster::castRay: the closest hit must respond
+            if (hit_obj >= 0 && !has_response(hit_obj)) return h;
             h.hit = true;
-            h.is_static = true;
+            h.is_static = hit_obj < 0;
+            h.body = hit_obj >= 0 ? hit_obj : -1;
             h.fraction = frac;
             float sm = 1.f - frac;
             h.point = from * sm + to * frac;
@@ -240,9 +279,88 @@ struct World {
             cp.friction = CARWORLD_COLLISION_FRICTION;
             cp.restitution = CARWORLD_COLLISION_RESTITUTION;
         } else if (b0 == 0 && b1 < 0) {
+            // ball + world: averaged "special" row (Arena.cpp:296-304)
             cp.special = true;
+        } else if (b0 > 0 && b1 == 0) {
+            car_ball_hit(cp, b0 - 1);
+        } else if (b0 > 0 && b1 > 0) {
+            car_car_hit(cp, b0 - 1, b1 - 1);
         }
         if (b1 == -1 - ST_MESH && cp.tri >= 0) adjust_internal_edge(cp, *mesh, cp.tri);
+    }
+
+    // Arena::_BtCallback_OnCarBallCollision + Ball::_OnHit
+    // (src/Sim/Arena/Arena.cpp:316-322, src/Sim/Ball/Ball.cpp:236-290).  The
+    // manifold's body0 is the car, so the point on the ball is local point B.
+    RSG_FN void car_ball_hit(CPoint& cp, int ci) {
+        Car& c = A.cars[ci];
+        Body& bb = A.ball.b;
+        cp.friction = CARBALL_COLLISION_FRICTION;
+        cp.restitution = CARBALL_COLLISION_RESTITUTION;
+        setflag(c.flags, RSG_CF_HIT_VALID, true);
+        c.hit_rel_pos = cp.local_b * BT_TO_UU;
+        c.hit_tick = A.tick;
+        V3 ball_pos = bb.pos * BT_TO_UU;
+        c.hit_ball_pos = ball_pos;
+        c.hit_extra_vel = v3zero();
+        if ((A.tick > c.hit_extra_tick + 1) || (c.hit_extra_tick > A.tick)) {
+            c.hit_extra_tick = A.tick;
+            V3 fwd = car_forward(c);
+            V3 rel_pos = ball_pos - c.b.pos * BT_TO_UU;
+            V3 rel_vel = bb.vel * BT_TO_UU - c.b.vel * BT_TO_UU;
+            float rel_speed = rs_min(len(rel_vel), BALL_CAR_EXTRA_IMPULSE_MAXDELTAVEL_UU);
+            if (rel_speed > 0) {
+                V3 hit_dir = rs_normalized(rel_pos * v3(1, 1, BALL_CAR_EXTRA_IMPULSE_Z_SCALE));
+                V3 fwd_adj = fwd * dot(hit_dir, fwd) * (1 - BALL_CAR_EXTRA_IMPULSE_FORWARD_SCALE);
+                hit_dir = rs_normalized(hit_dir - fwd_adj);
+                V3 added = (hit_dir * rel_speed) * curve_eval(curve_ball_extra_impulse(), rel_speed) * 1.f;
+                c.hit_extra_vel = added;
+                A.ball.vel_cache += added * UU_TO_BT;
+            }
+        }
+    }
+
+    // Arena::_BtCallback_OnCarCarCollision (src/Sim/Arena/Arena.cpp:324-395):
+    // bumps and demolitions, tested from both cars.  car1 is the manifold's
+    // body0 (point A), car2 its body1 (point B).
+    RSG_FN void car_car_hit(CPoint& cp, int c1, int c2) {
+        cp.friction = CARCAR_COLLISION_FRICTION;
+        cp.restitution = CARCAR_COLLISION_RESTITUTION;
+        for (int it = 0; it < 2; it++) {
+            bool swapped = it == 1;
+            Car& me = A.cars[swapped ? c2 : c1];
+            Car& other = A.cars[swapped ? c1 : c2];
+            if (flag(me.flags, RSG_CF_DEMOED) || flag(other.flags, RSG_CF_DEMOED)) return;
+            if (me.other_id == other.id && me.car_cooldown > 0) continue;
+            V3 vel = me.b.vel * BT_TO_UU;
+            V3 other_vel = other.b.vel * BT_TO_UU;
+            V3 delta_pos = other.b.pos * BT_TO_UU - me.b.pos * BT_TO_UU;
+            if (dot(vel, delta_pos) > 0) {
+                V3 vel_dir = rs_normalized(vel);
+                V3 dir_to_other = rs_normalized(delta_pos);
+                float speed_towards = dot(vel, dir_to_other);
+                float other_away = dot(other_vel, vel_dir);
+                if (speed_towards > other_away) {
+                    V3 local = swapped ? cp.local_b : cp.local_a;
+                    if ((local.x * BT_TO_UU) > BUMP_MIN_FORWARD_DIST) {
+                        bool is_demo = flag(me.flags, RSG_CF_SUPERSONIC);
+                        if (is_demo) is_demo = me.team != other.team;
+                        if (is_demo) {
+                            setflag(other.flags, RSG_CF_DEMOED, true);
+                            other.demo_timer = DEMO_RESPAWN_TIME;
+                        } else {
+                            bool ground = flag(other.flags, RSG_CF_ON_GROUND);
+                            float base = curve_eval(ground ? curve_bump_ground() : curve_bump_air(), speed_towards);
+                            V3 up = ground ? car_up(other) : v3(0, 0, 1);
+                            V3 imp = vel_dir * base + up * curve_eval(curve_bump_upward(), speed_towards) * 1.f;
+                            other.vel_cache += imp * UU_TO_BT;
+                        }
+                        me.other_id = other.id;
+                        me.car_cooldown = BUMP_COOLDOWN_TIME;
+                    }
+                }
+            }
+        }
     }
 
     RSG_FN void refresh(int mi) {
@@ -383,6 +501,87 @@ struct World {
         return a + v * ab + w * ac;
     }
 
+    RSG_FN Xf car_child_xf(int code) {
+        Car& c = A.cars[code - 1];
+        CarPhys cp = make_car_phys(c.preset);
+        return xf_mul(xf_from(c.b.pos, c.b.rot), xf_from(cp.box_offset, m3_identity()));
+    }
+    RSG_FN CShape car_box_shape(int code) {
+        CarPhys cp = make_car_phys(A.cars[code - 1].preset);
+        return shape_box(cp.box_implicit, cp.box_margin);
+    }
+
+    // Car hitbox vs the arena mesh: btConvexConcaveCollisionAlgorithm with the
+    // per-triangle early out, then btConvexConvexAlgorithm (GJK) per triangle.
+    RSG_FN void car_vs_mesh(int mi, int code) {
+        Xf cx = car_child_xf(code);
+        CarPhys cp = make_car_phys(A.cars[code - 1].preset);
+        V3 he = cp.box_implicit + v3(cp.box_margin, cp.box_margin, cp.box_margin);
+        M3 ab = abs_m3(cx.R);
+        V3 e = v3(dot(ab.r[0], he), dot(ab.r[1], he), dot(ab.r[2], he));
+        V3 amin = cx.t - e, amax = cx.t + e;
+        struct CB {
+            World* w; int mi; int code; V3 amin, amax; Xf cx; V3 he;
+            RSG_FN void operator()(int t) { w->car_triangle(mi, code, t, amin, amax, cx, he); }
+        } cb{this, mi, code, amin, amax, cx, he};
+        mesh_query_aabb(*mesh, amin, amax, cb);
+        refresh(mi);
+    }
+
+    RSG_FN void car_triangle(int mi, int code, int t, V3 amin, V3 amax, const Xf& cx, V3 he) {
+        V3 v[3];
+        mesh_triangle(*mesh, t, v);
+        if (!triangle_aabb_overlap(v, amin, amax)) return;
+        float thr = mf[mi].threshold + 0.f;
+        {
+            V3 tn = normalized(cross(v[1] - v[0], v[2] - v[0]));
+            M3 inv = inverse(cx.R);
+            for (int side = 0; side < 2; side++) {
+                V3 d = inv * tn;
+                V3 lp = v3(fsel(d.x, he.x, -he.x), fsel(d.y, he.y, -he.y), fsel(d.z, he.z, -he.z));
+                V3 wp = xf_apply(cx, lp);
+                float dist = dot(tn, v[0]) - dot(tn, wp);
+                if (dist > thr) return;
+                tn = tn * -1.f;
+            }
+        }
+        CShape box = car_box_shape(code);
+        CShape tri = shape_triangle(v);
+        float md = box.margin + tri.margin + mf[mi].threshold;
+        md *= md;
+        V3 n, pb;
+        float dist;
+        if (gjk_closest(box, tri, cx, xf_ident(), md, n, pb, dist)) add_contact(mi, n, pb, dist, t);
+    }
+
+    // Car hitbox vs the ball: btConvexConvexAlgorithm (GJK) on the box and the
+    // sphere; the manifold (car, ball) exists once the child AABBs overlap.
+    RSG_FN void car_vs_ball(int mi, int code) {
+        Xf cx = car_child_xf(code);
+        CShape box = car_box_shape(code);
+        CShape sph = shape_sphere(make_ball_phys().radius);
+        float md = box.margin + sph.margin + mf[mi].threshold;
+        md *= md;
+        V3 n, pb;
+        float dist;
+        if (gjk_closest(box, sph, cx, xf_from(A.ball.b.pos, A.ball.b.rot), md, n, pb, dist))
+            add_contact(mi, n, pb, dist, -1);
+        refresh(mi);
+    }
+
+    // Car hitbox vs car hitbox: btBoxBoxCollisionAlgorithm (dBoxBox2) with
+    // box0 = the car of manifold body0.
+    RSG_FN void car_vs_car(int mi, int c0, int c1) {
+        Xf x0 = car_child_xf(c0), x1 = car_child_xf(c1);
+        CarPhys p0 = make_car_phys(A.cars[c0 - 1].preset), p1 = make_car_phys(A.cars[c1 - 1].preset);
+        V3 h0 = p0.box_implicit + v3(p0.box_margin, p0.box_margin, p0.box_margin);
+        V3 h1 = p1.box_implicit + v3(p1.box_margin, p1.box_margin, p1.box_margin);
+        BoxBoxContacts bc;
+        box_box(x0, h0, x1, h1, bc);
+        for (int i = 0; i < bc.n; i++) add_contact(mi, bc.normal, bc.point[i], bc.depth[i], -1);
+        refresh(mi);
+    }
+
     // btConvexConcaveCollisionAlgorithm::processCollision for the ball
     RSG_FN void ball_vs_mesh(int mi) {
         Body& b = A.ball.b;
@@ -438,6 +637,7 @@ struct World {
             bb.vel *= powf(1.f - BALL_DRAG, dt);
             bb.ang *= powf(1.f - 0.f, dt);
         }
+        RSG_PROF_MARK(2);
         // AABBs and broadphase cells
         for (int i = 0; i < nb; i++) {
             Body& b = *body_ptr(i);
@@ -474,11 +674,14 @@ struct World {
         // keep the order numbers small: rank them 0..nb-1
         for (int r = 0; r < nb; r++) A.bp_seq[order[r]] = (uint32_t)r;
         find_pairs();
+        RSG_PROF_MARK(3);
         dispatch_pairs();
+        RSG_PROF_MARK(4);
 #ifdef RSG_HOST_DEBUG_HOOK
         RSG_HOST_DEBUG_HOOK(*this);
 #endif
         solve_islands(dt);
+        RSG_PROF_MARK(5);
         // integrateTransforms: active bodies
         for (int i = 0; i < nb; i++) {
             Body& b = *body_ptr(i);
@@ -564,16 +767,43 @@ struct World {
                     static_shape_aabb(mesh, s, smin, smax);
                     bool ov = aabb_overlap(cmin, cmax, smin, smax);
                     if (ov) {
-                        if (s == ST_MESH) {
-                            // box vs triangles: not implemented yet
-                        } else {
-                            car_vs_plane(mi, b, s);
-                        }
+                        if (s == ST_MESH) car_vs_mesh(mi, b);
+                        else car_vs_plane(mi, b, s);
                     }
                     if (!ov) release_manifold(mi);
                 }
+            } else if (a == 0) {
+                // ball + car compound (swapped compound algorithm): the child
+                // convex-convex algorithm creates its manifold (car, ball) on
+                // its first run, which needs the child AABB to overlap the ball's
+                V3 cmin, cmax;
+                car_child_aabb(b, cmin, cmax);
+                float r = make_ball_phys().radius + 0.08f;
+                V3 bmin = A.ball.b.pos - v3(r, r, r), bmax = A.ball.b.pos + v3(r, r, r);
+                if (aabb_overlap(cmin, cmax, bmin, bmax)) {
+                    int mi = new_manifold(b, 0);
+                    if (mi >= 0) car_vs_ball(mi, b);
+                }
             } else {
-                // dynamic-dynamic pairs (ball-car, car-car): not implemented yet
+                // car + car: compound-compound without AABB trees falls back to
+                // nested compound algorithms whose box-box child owns a
+                // manifold (car b, car a) created with the pair; it is released
+                // unless both child AABB tests pass.
+                int mi = new_manifold(b, a);
+                if (mi < 0) continue;
+                V3 amin, amax, bcmin, bcmax, bmin, bmax;
+                car_child_aabb(a, amin, amax);
+                Body& bb = A.cars[b - 1].b;
+                shape_aabb(b, bb.pos, bb.rot, bcmin, bcmax);
+                bool keep = false;
+                if (aabb_overlap(amin, amax, bcmin, bcmax)) {
+                    car_child_aabb(b, bmin, bmax);
+                    if (aabb_overlap(bmin, bmax, amin, amax)) {
+                        car_vs_car(mi, b, a);
+                        keep = true;
+                    }
+                }
+                if (!keep) release_manifold(mi);
             }
         }
     }

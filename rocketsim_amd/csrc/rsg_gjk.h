@@ -320,6 +320,66 @@ struct VoronoiSimplex {
 };
 
 // ---------------------------------------------------------------------------
+// Ray cast against a convex shape: btSubsimplexConvexCast of a point
+// (BulletCollision/NarrowPhaseCollision/btSubSimplexConvexCast.cpp:27-140) as
+// run by btCollisionWorld::rayTestSingleInternal for convex shapes
+// (BulletCollision/CollisionDispatch/btCollisionWorld.cpp:267-312).  The shape
+// does not move; supports include its margin.  Returns true with the hit
+// fraction and the (unit) hit normal.
+
+RSG_FN V3 support_with_margin(const CShape& s, V3 d) {
+    if (s.kind == SHAPE_BOX) {
+        V3 h = s.half + v3(s.margin, s.margin, s.margin);
+        return v3(fsel(d.x, h.x, -h.x), fsel(d.y, h.y, -h.y), fsel(d.z, h.z, -h.z));
+    }
+    // sphere: margin (= radius) along the normalised direction
+    V3 n = d;
+    if (len2(n) < FLT_EPS * FLT_EPS) n = v3(-1.f, -1.f, -1.f);
+    n = normalized(n);
+    return v3zero() + s.margin * n;
+}
+
+RSG_FN bool ray_convex(V3 from, V3 to, const CShape& B, const Xf& tb, float& fraction, V3& normal) {
+    VoronoiSimplex sx;
+    sx.reset();
+    V3 r = to - from;  // linVelA - linVelB (the shape is static)
+    float lambda = 0.f;
+    V3 ia = from;      // interpolated origin of the ray point
+    Xf ib = tb;
+    V3 sup_a = from;
+    V3 sup_b = xf_apply(tb, support_with_margin(B, mul_vm(r, tb.R)));
+    V3 v = sup_a - sup_b;
+    int max_iter = 32;          // float build: 32 iterations, epsilon 1e-4
+    V3 n = v3zero();
+    float dist2 = len2(v);
+    while (dist2 > 0.0001f && max_iter--) {
+        sup_a = ia;
+        sup_b = xf_apply(ib, support_with_margin(B, mul_vm(v, ib.R)));
+        V3 w = sup_a - sup_b;
+        float vdw = dot(v, w);
+        if (lambda > 1.f) return false;
+        if (vdw > 0.f) {
+            float vdr = dot(v, r);
+            if (vdr >= -(FLT_EPS * FLT_EPS)) return false;
+            lambda = lambda - vdw / vdr;
+            float sm = 1.f - lambda;
+            ia = from * sm + to * lambda;
+            ib.t = tb.t * sm + tb.t * lambda;
+            w = sup_a - sup_b;
+            n = v;
+        }
+        if (!sx.in_simplex(w)) sx.add_vertex(w, sup_a, sup_b);
+        if (sx.closest(v)) dist2 = len2(v);
+        else dist2 = 0.f;
+    }
+    V3 nn = len2(n) >= FLT_EPS * FLT_EPS ? normalized(n) : v3zero();
+    if (dot(nn, r) >= -0.f) return false;
+    fraction = lambda;
+    normal = nn;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // btGjkEpaSolver2: GJK (Presson) + EPA, used when the cores overlap
 
 constexpr float G2_ACCURACY = 0.0001f, G2_MIN_DISTANCE = 0.0001f, G2_DUP_EPS = 0.0001f;
@@ -860,10 +920,7 @@ RSG_FN bool g2_penetration(const CShape& s0, const Xf& t0, const CShape& s1, con
     res.witness[0] = res.witness[1] = v3zero();
     int gs = gjk.evaluate(-guess);
     if (gs == 1) {
-        epa.init(pool + 4);
-        // EPA vertices live after the four GJK entries: shift its indices
-        // by addressing the shared pool from the start.
-        epa.sv = pool;
+        epa.init(pool);  // EPA vertices follow the four GJK entries of the pool
         int es = epa.evaluate(gjk, -guess);
         if (es != 9) {
             V3 w0 = v3zero();

@@ -59,7 +59,10 @@ enum CarWord : int {
     CW_ID = 84,
     CW_BP_CELL = 85,       // broadphase cell the proxy is registered in
     CW_BP_SEQ = 86,        // registration order among the arena's dynamic proxies
-    CAR_WORDS = 87
+    CW_DEMO_BODY_ROT = 87, // 9  rigid-body basis while demolished (CW_ROT then
+                           //    holds the basis last reported, which the
+                           //    reference freezes at demolition)
+    CAR_WORDS = 96
 };
 
 enum BallWord : int {
@@ -78,7 +81,8 @@ enum ArenaWord : int {
     AW_LAST_GOAL_TICK = 5, // 2
     AW_RESETS = 7,
     AW_STATUS = 8,      // bit 0: non-finite state seen
-    ARENA_WORDS = 9
+    AW_CAR_ORDER = 9,   // car update order: slot k = car index in bits 4k..4k+3
+    ARENA_WORDS = 10
 };
 
 // Controls bit word

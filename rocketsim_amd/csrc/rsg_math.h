@@ -17,6 +17,11 @@
 #define RSG_FN inline
 #endif
 
+// Phase marks for an instrumented build (no-ops otherwise).
+#ifndef RSG_PROF_MARK
+#define RSG_PROF_MARK(id)
+#endif
+
 namespace rsg {
 
 struct V3 {
@@ -48,7 +53,13 @@ RSG_FN float get(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 RSG_FN void set(V3& a, int i, float v) { if (i == 0) a.x = v; else if (i == 1) a.y = v; else a.z = v; }
 
 // v / |v| (Bullet normalize(): multiply by the reciprocal of the length).
+// A host build may substitute its own rounding of this step (RSG_NORMALIZE_HOOK)
+// to study how sensitive a trajectory is to the last bits of a normal.
+#ifdef RSG_NORMALIZE_HOOK
+RSG_FN V3 normalized(V3 a) { return RSG_NORMALIZE_HOOK(a); }
+#else
 RSG_FN V3 normalized(V3 a) { return a * (1.f / len(a)); }
+#endif
 
 // Bullet safeNormalize(): leave as (1,0,0) when the length is ~0
 // (LinearMath/btVector3.h:287-302).

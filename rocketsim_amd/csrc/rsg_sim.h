@@ -98,6 +98,8 @@ struct WheelT {
 
 struct Car {
     Body b;
+    M3 api_rot;        // CarState::rotMat: the body basis as of the last post-tick
+                       // update of a car that was not demolished
     int preset, team;
     uint32_t id;
     uint32_t flags;
@@ -710,10 +712,13 @@ RSG_FN void car_respawn_state(Car& c, int slot, float boost_amount) {
     c.vel_cache = v3zero();
     c.b.pos = pos_uu * UU_TO_BT;
     c.b.rot = yaw_basis(yaw);
+    c.api_rot = c.b.rot;
     c.b.vel = v3zero();
     c.b.ang = v3zero();
     body_update_inertia(c.b);
-    c.flags = RSG_CF_ON_GROUND;  // CarState default isOnGround = true
+    // CarState default isOnGround = true; the body flag is not part of the
+    // state SetState writes
+    c.flags = RSG_CF_ON_GROUND | (keep.flags & RSG_CF_BODY_DISABLED);
     c.boost = boost_amount;
     c.hit_tick = ~0ull;
     c.hit_extra_tick = ~0ull;

@@ -55,10 +55,11 @@ RSG_FN void store_rot(const CarRef& r, int w, const M3& m) { r.sv(w, m.r[0]); r.
 
 RSG_FN void load_car(const CarRef& r, Car& c) {
     c.b.pos = r.v(CW_POS);
-    c.b.rot = load_rot(r, CW_ROT);
+    c.api_rot = load_rot(r, CW_ROT);
     c.b.vel = r.v(CW_VEL);
     c.b.ang = r.v(CW_ANGVEL);
     c.flags = r.u(CW_FLAGS);
+    c.b.rot = flag(c.flags, RSG_CF_DEMOED) ? load_rot(r, CW_DEMO_BODY_ROT) : c.api_rot;
     c.flip_torque = r.v(CW_FLIP_TORQUE);
     c.jump_time = r.f(CW_JUMP_TIME);
     c.flip_time = r.f(CW_FLIP_TIME);
@@ -108,7 +109,8 @@ RSG_FN void load_bp(const CarRef& r, int w_cell, int w_seq, int& cell, uint32_t&
 
 RSG_FN void store_car(const CarRef& r, const Car& c) {
     r.sv(CW_POS, c.b.pos);
-    store_rot(r, CW_ROT, c.b.rot);
+    store_rot(r, CW_ROT, c.api_rot);
+    if (flag(c.flags, RSG_CF_DEMOED)) store_rot(r, CW_DEMO_BODY_ROT, c.b.rot);
     r.sv(CW_VEL, c.b.vel);
     r.sv(CW_ANGVEL, c.b.ang);
     r.u(CW_FLAGS) = c.flags;
@@ -180,12 +182,29 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.last_goal_tick = ar.u64(AW_LAST_GOAL_TICK);
     A.resets = ar.u(AW_RESETS);
     A.status = ar.u(AW_STATUS);
+    {
+        // Anything but a permutation of 0..n_cars-1 falls back to the
+        // default (reverse creation) order, so no index leaves the arena.
+        uint32_t ow = ar.u(AW_CAR_ORDER), seen = 0;
+        bool ok = true;
+        for (int k = 0; k < MAX_CARS; k++) {
+            uint32_t v = (ow >> (4 * k)) & 0xFu;
+            A.order[k] = (uint8_t)v;
+            if (k < A.n_cars) {
+                ok = ok && v < (uint32_t)A.n_cars && !(seen >> v & 1u);
+                seen |= 1u << (v & 31u);
+            }
+        }
+        if (!ok)
+            for (int k = 0; k < MAX_CARS; k++) A.order[k] = (uint8_t)(k < A.n_cars ? A.n_cars - 1 - k : k);
+    }
     A.si.erp = 0.2f;
     A.si.time_step = d.tick_time;
     for (int i = 0; i < A.n_cars; i++) {
         CarRef cr = car_ref(d, a, i);
         load_car(cr, A.cars[i]);
         car_init_body(A.cars[i]);
+        A.cars[i].b.active = !flag(A.cars[i].flags, RSG_CF_BODY_DISABLED);
         load_bp(cr, CW_BP_CELL, CW_BP_SEQ, A.bp_cell[1 + i], A.bp_seq[1 + i]);
     }
     CarRef br = ball_ref(d, a);
@@ -215,6 +234,11 @@ RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
     ar.su64(AW_LAST_GOAL_TICK, A.last_goal_tick);
     ar.u(AW_RESETS) = A.resets;
     ar.u(AW_STATUS) = A.status;
+    {
+        uint32_t ow = 0;
+        for (int k = 0; k < MAX_CARS; k++) ow |= (uint32_t)(A.order[k] & 0xFu) << (4 * k);
+        ar.u(AW_CAR_ORDER) = ow;
+    }
     for (int i = 0; i < A.n_cars; i++) {
         CarRef cr = car_ref(d, a, i);
         store_car(cr, A.cars[i]);
@@ -245,21 +269,21 @@ RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
 // reference's std::default_random_engine.  Cars are taken in slot order per
 // team.
 RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id) {
-    int order[5] = {0, 1, 2, 3, 4};
+    int slots[5] = {0, 1, 2, 3, 4};
     uint64_t h = rsg_draw(seed ^ 0x4B1C0FFull, arena_id, 0xFFFFu, A.resets, 7);
     for (int i = 4; i > 0; i--) {
         int j = (int)((h >> (i * 8)) % (uint64_t)(i + 1));
-        int t = order[i]; order[i] = order[j]; order[j] = t;
+        int t = slots[i]; slots[i] = slots[j]; slots[j] = t;
     }
     A.resets++;
     int idx_team[2] = {0, 0};
     int num_at_respawn[4] = {0, 0, 0, 0};
-    for (int ci = 0; ci < A.n_cars; ci++) {
-        Car& c = A.cars[ci];
+    for (int oi = 0; oi < A.n_cars; oi++) {
+        Car& c = A.cars[A.order[oi]];
         int k = idx_team[c.team]++;
         SpawnPos sp;
         if (k < 5) {
-            sp = kickoff_spawn(order[k]);
+            sp = kickoff_spawn(slots[k]);
         } else {
             int ri = (k - 5) % 5;
             ri = ri < 4 ? ri : 3;
@@ -276,8 +300,9 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
         }
         c.b.pos = pos * UU_TO_BT;
         c.b.rot = yaw_basis(yaw);
+        c.api_rot = c.b.rot;
         body_update_inertia(c.b);
-        c.flags = RSG_CF_ON_GROUND;
+        c.flags = RSG_CF_ON_GROUND | (c.flags & RSG_CF_BODY_DISABLED);
     }
     // BallState() then SetState (Ball.cpp:36-51)
     A.ball.b.pos = v3(0, 0, BALL_REST_Z) * UU_TO_BT;
@@ -377,8 +402,10 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
 
     World world(A, mesh, soccar && mesh != nullptr);
 
-    // 2. car pre-tick (Car.cpp:58-131)
-    for (int ci = 0; ci < A.n_cars; ci++) {
+    RSG_PROF_MARK(0);
+    // 2. car pre-tick (Car.cpp:58-131), in the arena's car update order
+    for (int oi = 0; oi < A.n_cars; oi++) {
+        const int ci = A.order[oi];
         Car& c = A.cars[ci];
         CarPhys cp = make_car_phys(c.preset);
         c.ctrl.throttle = clampf(c.ctrl.throttle, -1, 1);
@@ -393,10 +420,11 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
                 int slot = (int)(rsg_draw(P.seed ^ 0xDE40ull, P.arena_id, (uint32_t)ci, (uint32_t)A.tick, 9) % 4u);
                 car_respawn_state(c, slot, BOOST_SPAWN_AMOUNT);
             }
-            c.b.active = false;  // DISABLE_SIMULATION + no contact response this tick
+            c.b.active = false;  // DISABLE_SIMULATION + no contact response
         } else {
             c.b.active = true;
         }
+        setflag(c.flags, RSG_CF_BODY_DISABLED, !c.b.active);
         if (flag(c.flags, RSG_CF_DEMOED)) continue;
 
         // updateVehicleFirst (btVehicleRL.cpp:215-228)
@@ -427,6 +455,7 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
         car_update_boost(c, dt);
     }
 
+    RSG_PROF_MARK(1);
     // 3. pads pre-tick (BoostPad.cpp:51-60)
     if (soccar && A.n_cars > 0) {
         for (int p = 0; p < PAD_NUM; p++) {
@@ -439,11 +468,15 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
 
     // 4. world step (btDiscreteDynamicsWorld::stepSimulation)
     world.step(dt, P);
+    RSG_PROF_MARK(7);
 
-    // 5. cars post-tick (Car.cpp:133-193) + pad collisions
-    for (int ci = 0; ci < A.n_cars; ci++) {
+    // 5. cars post-tick (Car.cpp:133-193) + pad collisions, per car in the
+    //    car update order (Arena.cpp:720-734)
+    for (int oi = 0; oi < A.n_cars; oi++) {
+        const int ci = A.order[oi];
         Car& c = A.cars[ci];
         if (!flag(c.flags, RSG_CF_DEMOED)) {
+            c.api_rot = c.b.rot;
             float s2 = len2(c.b.vel * BT_TO_UU);
             bool ss = flag(c.flags, RSG_CF_SUPERSONIC);
             if (ss && c.supersonic_time < SUPERSONIC_MAINTAIN_MAX_TIME)
@@ -521,6 +554,7 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
     }
     A.tick++;
     if (scored && P.auto_reset) arena_kickoff_reset(A, P.seed, P.arena_id);
+    RSG_PROF_MARK(8);
 }
 
 }  // namespace rsg

@@ -31,6 +31,7 @@ CF_AUTO_FLIPPING = 1 << 12
 CF_WORLD_CONTACT = 1 << 13
 CF_DEMOED = 1 << 14
 CF_HIT_VALID = 1 << 15
+CF_BODY_DISABLED = 1 << 16
 
 F3 = C.c_float * 3
 F4 = C.c_float * 4

@@ -27,6 +27,12 @@ def main():
     ball = parity.sample_soccar_ball(ref, n_traj=32, ticks_per_traj=96, seed=20261203, stride=8)
     parity.save_samples(os.path.join(HERE, "soccar_ball_tf.npz"), ball)
     print("soccar_ball_tf.npz:", len(ball), "samples")
+    drive = parity.sample_soccar_drive(ref, n_traj=8, ticks_per_traj=240, seed=20261204, n_cars=4, stride=5)
+    parity.save_samples(os.path.join(HERE, "soccar_drive_tf.npz"), drive)
+    print("soccar_drive_tf.npz:", len(drive), "samples")
+    melee = parity.sample_soccar_melee(ref, n_traj=24, ticks_per_traj=100, seed=20261205, n_cars=4, stride=2)
+    parity.save_samples(os.path.join(HERE, "soccar_melee_tf.npz"), melee)
+    print("soccar_melee_tf.npz:", len(melee), "samples")
 
 
 if __name__ == "__main__":

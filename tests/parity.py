@@ -25,7 +25,7 @@ import random
 import numpy as np
 
 from rocketsim_amd.state import (BallState, CarState, Controls, PadState, CF_HAS_JUMPED,
-                                 CF_ON_GROUND, MODE_SOCCAR, MODE_THE_VOID, NUM_PADS)
+                                 CF_ON_GROUND, CF_SUPERSONIC, MODE_SOCCAR, MODE_THE_VOID, NUM_PADS)
 
 EXACT_CAR = {"flags", "car_contact_other_id", "hit_tick", "hit_extra_tick", "tick_count_since_update",
              "last_controls.jump", "last_controls.boost", "last_controls.handbrake", "last_controls._pad",
@@ -33,7 +33,12 @@ EXACT_CAR = {"flags", "car_contact_other_id", "hit_tick", "hit_extra_tick", "tic
 # Registration order numbers: only their relative order across the arena's
 # bodies is state (compared by bp_order below).
 SKIP_CAR = {"bp_seq"}
-REL_FLOOR = {"pos": 1.0, "vel": 1.0, "ang_vel": 1e-2}
+REL_FLOOR = {"pos": 1.0, "vel": 1.0, "ang_vel": 1e-2,
+             # wheel pushback is an impulse (kg m/s) taken from a ray depth that
+             # is a difference of nearly equal coordinates; 10 units move the
+             # car (180 kg) by 0.05 m/s, so 1e-3 of that is far below the
+             # velocity tolerance
+             "wheel_extra_pushback": 10.0}
 ROT_TOL = 1e-4
 REL_TOL = 1e-3
 
@@ -112,7 +117,7 @@ def compare_pads(ref, got):
 
 
 class Sample:
-    __slots__ = ("mode", "teams", "presets", "cars_in", "ball_in", "pads_in", "ctrls", "tick_in",
+    __slots__ = ("mode", "teams", "presets", "car_order", "cars_in", "ball_in", "pads_in", "ctrls", "tick_in",
                  "cars_out", "ball_out", "pads_out", "tick_out")
 
 
@@ -220,12 +225,136 @@ def sample_soccar_ball(ref, n_traj, ticks_per_traj, seed, stride=1):
     return out
 
 
+def sample_soccar_drive(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1):
+    """Cars driving from a random kickoff with random controls held for
+    10-60 ticks (throttle mostly forward, no jumps until the first redraw):
+    wheels on the floor and ramps, boost pads, car-car bumps."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n_traj):
+        A = ref.RefArena(MODE_SOCCAR)
+        teams = [i % 2 for i in range(n_cars)]
+        presets = [rng.randint(0, 5) for _ in range(n_cars)]
+        for t, p in zip(teams, presets):
+            A.add_car(t, p)
+        A.reset_kickoff(rng.randint(0, 1000))
+        ctrls = [rand_controls(rng) for _ in range(n_cars)]
+        for c in ctrls:
+            c.throttle = 1.0
+            c.jump = 0
+        hold = [rng.randint(10, 60) for _ in range(n_cars)]
+        for t in range(ticks_per_traj):
+            for i in range(n_cars):
+                hold[i] -= 1
+                if hold[i] <= 0:
+                    ctrls[i] = rand_controls(rng)
+                    ctrls[i].throttle = rng.choice([1.0, 1.0, 0.5, -1.0])
+                    hold[i] = rng.randint(10, 60)
+            if t % stride == 0:
+                out.append(_one_tick(A, MODE_SOCCAR, teams, presets, ctrls))
+            else:
+                for i in range(n_cars):
+                    A.set_controls(i, ctrls[i])
+                A.step(1)
+        A.close()
+    return out
+
+
+def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1):
+    """Cars placed 250-700 uu around the ball and driven at it (some
+    supersonic, some jumping): car-ball hits with the extra impulse,
+    car-car box-box contacts, bumps and demolitions, wheels resting on
+    other cars, respawns."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n_traj):
+        A = ref.RefArena(MODE_SOCCAR)
+        teams = [i % 2 for i in range(n_cars)]
+        presets = [rng.randint(0, 5) for _ in range(n_cars)]
+        for t, p in zip(teams, presets):
+            A.add_car(t, p)
+        bx, by = rng.uniform(-3000, 3000), rng.uniform(-4000, 4000)
+        b = A.get_ball()
+        b.pos[:] = [bx, by, rng.choice([93.15, rng.uniform(100, 400)])]
+        b.vel[:] = [rng.gauss(0, 300), rng.gauss(0, 300), 0]
+        A.set_ball(b)
+        for i in range(n_cars):
+            s = A.get_car(i)
+            ang, dist = rng.uniform(0, 2 * math.pi), rng.uniform(250, 700)
+            x, y = bx + dist * math.cos(ang), by + dist * math.sin(ang)
+            yaw = math.atan2(by - y, bx - x) + rng.gauss(0, 0.3)
+            cy, sy = math.cos(yaw), math.sin(yaw)
+            spd = rng.choice([0, 800, 1500, 2300])
+            s.pos[:] = [x, y, 17.0]
+            s.rot[:] = [cy, sy, 0, -sy, cy, 0, 0, 0, 1]
+            s.vel[:] = [spd * cy, spd * sy, 0]
+            s.ang_vel[:] = [0, 0, 0]
+            s.flags = CF_ON_GROUND | 0x1E | (CF_SUPERSONIC if spd >= 2200 else 0)
+            s.supersonic_time = 0.5 if spd >= 2200 else 0
+            s.boost = 50
+            A.set_car(i, s)
+        A.step(2)  # let the wheels settle
+        ctrls = [rand_controls(rng) for _ in range(n_cars)]
+        for c in ctrls:
+            c.throttle = 1.0
+            c.boost = rng.random() < 0.5
+            c.jump = rng.random() < 0.2
+        hold = [rng.randint(10, 60) for _ in range(n_cars)]
+        for t in range(ticks_per_traj):
+            for i in range(n_cars):
+                hold[i] -= 1
+                if hold[i] <= 0:
+                    ctrls[i] = rand_controls(rng)
+                    hold[i] = rng.randint(10, 60)
+            if t % stride == 0:
+                out.append(_one_tick(A, MODE_SOCCAR, teams, presets, ctrls))
+            else:
+                for i in range(n_cars):
+                    A.set_controls(i, ctrls[i])
+                A.step(1)
+        A.close()
+    return out
+
+
+# The reference normalises vectors with SSE rsqrtss plus one Newton step
+# (btVector3::normalize, LinearMath/btVector3.h:304-340), whose last bits
+# depend on the x86 CPU; the GPU divides by the length.  GJK/EPA (which stop
+# at a 1e-4 squared distance) and the subsimplex ray cast can turn such ulp
+# differences into visibly different contacts in rare configurations.  A host
+# build of the same tick code with rsqrtss-emulated normalisation matches
+# the oracle on every sample of the drive and melee sets; with plain
+# normalisation about 1 tick in 3500 of the melee set diverges.  Contact-rich
+# sets therefore allow this many samples out of tolerance.
+CHAOTIC_FRACTION = 0.005
+
+
+def assert_parity(samples, results, max_frac=0.0):
+    nbad, msgs = check_samples(samples, results)
+    allowed = int(max_frac * len(samples))
+    assert nbad <= allowed, f"{nbad}/{len(samples)} samples off (allowed {allowed})\n" + "\n".join(msgs)
+    return nbad
+
+
 def _one_tick(A, mode, teams, presets, ctrls):
+    """One teacher-forced sample.  The oracle is first set to its own state as
+    read through the API (positions and velocities in uu), repeatedly until
+    reading it back gives the same record, so that it steps from exactly the
+    Bullet-unit values the GPU derives from that record (one uu -> bt -> uu
+    round trip can move a value by one ulp)."""
+    n = len(teams)
+    for _ in range(8):
+        before = [bytes(A.get_car(i)) for i in range(n)] + [bytes(A.get_ball())]
+        for i in range(n):
+            A.set_car(i, A.get_car(i))
+        A.set_ball(A.get_ball())
+        after = [bytes(A.get_car(i)) for i in range(n)] + [bytes(A.get_ball())]
+        if after == before:
+            break
     s = Sample()
     s.mode = mode
     s.teams = list(teams)
     s.presets = list(presets)
-    n = len(teams)
+    s.car_order = A.car_order()
     s.cars_in = [A.get_car(i) for i in range(n)]
     s.ball_in = A.get_ball()
     s.pads_in = A.get_pads() if mode == MODE_SOCCAR else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
@@ -261,6 +390,7 @@ def save_samples(path, samples):
         ncars=ncars,
         teams=np.array(sum((s.teams for s in samples), []), dtype=np.uint8),
         presets=np.array(sum((s.presets for s in samples), []), dtype=np.uint8),
+        car_order=np.array(sum((s.car_order for s in samples), []), dtype=np.uint8),
         cars_in=_pack(sum((s.cars_in for s in samples), []), CarState),
         cars_out=_pack(sum((s.cars_out for s in samples), []), CarState),
         ctrls=_pack(sum((s.ctrls for s in samples), []), Controls),
@@ -277,6 +407,7 @@ def load_samples(path):
     z = np.load(path, allow_pickle=False)
     ncars = z["ncars"]
     teams, presets = list(z["teams"]), list(z["presets"])
+    order = list(z["car_order"])
     cars_in, cars_out = _unpack(z["cars_in"], CarState), _unpack(z["cars_out"], CarState)
     ctrls = _unpack(z["ctrls"], Controls)
     ball_in, ball_out = _unpack(z["ball_in"], BallState), _unpack(z["ball_out"], BallState)
@@ -288,6 +419,7 @@ def load_samples(path):
         s.mode = int(z["mode"][i])
         s.teams = [int(t) for t in teams[k:k + n]]
         s.presets = [int(p) for p in presets[k:k + n]]
+        s.car_order = [int(o) for o in order[k:k + n]]
         s.cars_in, s.cars_out, s.ctrls = cars_in[k:k + n], cars_out[k:k + n], ctrls[k:k + n]
         s.ball_in, s.ball_out = ball_in[i], ball_out[i]
         s.pads_in = pads_in[i * NUM_PADS:(i + 1) * NUM_PADS]
@@ -310,6 +442,7 @@ def run_on_gpu(samples, mode):
         batch.set_pads(sum((list(s.pads_in) for s in samples), []))
         batch.set_controls(sum((list(s.ctrls) for s in samples), []))
         batch.set_tick_counts([s.tick_in for s in samples])
+        batch.set_car_order(sum((list(s.car_order) for s in samples), []))
         batch.step(1)
         cars = batch.get_cars()
         balls = batch.get_balls()
