@@ -27,6 +27,10 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per arena-tick, 2v2: 2 x (4 cars x 220 B + ball 36 B +
 # pads 272 B + tick 8 B) + 4 x 24 B controls.
 BYTES_PER_ARENA_TICK_2V2 = 2488
+# HBM bytes per launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE passes
+# over `bench.py --profile-probe` (counter factor 1.0 from a 64 MiB copy),
+# keyed by (arenas, ticks per launch, arenas per workgroup).
+HBM_TRAFFIC_PER_LAUNCH = {(4096, 8, 4): (825453 + 516258) * 1024}
 
 
 def cpu_baseline(target_seconds=10.0):
@@ -54,11 +58,18 @@ def cpu_baseline(target_seconds=10.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=512)
+    ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--arenas", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--arenas-per-block", type=int, default=0,
+                    help="arenas per workgroup of the step kernel (0 = spread over every SIMD)")
+    ap.add_argument("--chunk", type=int, default=8,
+                    help="ticks per kernel launch (the arena state stays on chip across a chunk)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-probe", action="store_true",
+                    help="for rocprofv3 --pmc passes: fill and copy a 64 MiB buffer (calibrates the HBM byte "
+                         "counters), then two launches of --chunk ticks; prints no JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -76,19 +87,37 @@ def main():
     native.check(native.lib().rsg_init_mesh(arena_mesh.soccar_cmf(), len(arena_mesh.soccar_cmf())))
     batch = ArenaBatch.uniform(args.arenas, 2, 2, 0, mode=MODE_SOCCAR, seed=args.seed,
                                synth_controls=True, auto_reset=True)
+    batch.set_launch(args.arenas_per_block)
+    per_block = batch.get_launch()
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
 
-    for _ in range(args.warmup):
-        batch.step(1, sp)
+    if args.profile_probe:
+        src = torch.empty(64 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        for _ in range(2):
+            batch.step(args.chunk, sp)
+            torch.cuda.synchronize()
+        batch.close()
+        print("profile probe done", flush=True)
+        return
+
+    chunk = max(1, min(args.chunk, args.steps))
+    # K steps = K // chunk launches of `chunk` ticks plus one launch of the rest
+    launches = [chunk] * (args.steps // chunk) + ([args.steps % chunk] if args.steps % chunk else [])
+    for _ in range(max(1, args.warmup // chunk)):
+        batch.step(chunk, sp)
     torch.cuda.synchronize()
 
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start.record(stream)
-    for _ in range(args.steps):
-        batch.step(1, sp)
+    for n in launches:
+        batch.step(n, sp)
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -96,7 +125,7 @@ def main():
 
     arena_ticks = args.arenas * args.steps
     value = arena_ticks / wall
-    kernel_ms = ev_ms / args.steps  # one kernel launch per step
+    kernel_ms = ev_ms / args.steps  # per tick (one launch covers `chunk` ticks)
     achieved = (args.arenas * BYTES_PER_ARENA_TICK_2V2) / (kernel_ms * 1e-3) / 1e9
     goals = batch.get_goals()
     batch.close()
@@ -118,9 +147,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"soccar 2v2 x {args.arenas} arenas, Octane, seeded controls, auto-reset on goal",
                    "arenas": args.arenas, "cars_per_arena": 4, "tick_rate": 120,
+                   "arenas_per_block": per_block, "ticks_per_launch": chunk,
                    "goals_during_run": int(sum(goals[0]) + sum(goals[1]))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": HBM_TRAFFIC_PER_LAUNCH.get((args.arenas, chunk, per_block)),
+                     "traffic_unit": "B per launch",
                      "kernel": "rsg_step_arena_kernel", "kernel_ms": kernel_ms,
                      "bytes_per_arena_tick": BYTES_PER_ARENA_TICK_2V2},
         "cpu_baseline": cpu,

@@ -8,9 +8,9 @@
  * independent arenas lives in GPU memory and is stepped by hand-written HIP
  * kernels.  Every entry point returns 0 on success and a negative code on
  * error (no exceptions cross the boundary); rsg_last_error() describes the
- * last failure on the calling thread.  The C++ facade in
- * rocketsim_amd/csrc/rsg_arena_api.h rethrows errors as std::runtime_error
- * like the reference's RS_ERR_CLOSE (src/Framework.h:67-72).
+ * last failure on the calling thread.  The C++ facade in include/rsg.hpp
+ * rethrows errors as std::runtime_error like the reference's RS_ERR_CLOSE
+ * (src/Framework.h:67-72).
  *
  * Threading: one host thread per batch handle; all work is ordered on the
  * stream passed to rsg_batch_step (NULL = the default stream).
@@ -90,6 +90,12 @@ int rsg_batch_get_controls(rsg_batch* b, rsg_controls* out);
  * car indices; anything else is rejected. */
 int rsg_batch_set_car_order(rsg_batch* b, const int32_t* order);
 int rsg_batch_get_car_order(rsg_batch* b, int32_t* order);
+/* Launch geometry of rsg_batch_step (no counterpart in the reference, which
+ * steps one arena per call): arenas per workgroup, 1..64, or 0 for the
+ * default that spreads the batch over every SIMD of the device.  Results do
+ * not depend on it.  rsg_batch_get_launch returns the value in use. */
+int rsg_batch_set_launch(rsg_batch* b, int arenas_per_block);
+int rsg_batch_get_launch(rsg_batch* b);
 /* Arena::tickCount (Arena.h:61). */
 int rsg_batch_get_tick_counts(rsg_batch* b, uint64_t* out);
 int rsg_batch_set_tick_counts(rsg_batch* b, const uint64_t* in);

@@ -107,6 +107,13 @@ class ArenaBatch:
         native.check(self.L.rsg_batch_get_controls(self.h, out))
         return out
 
+    def set_launch(self, arenas_per_block):
+        """Arenas per workgroup of the step kernel (0 = device default)."""
+        native.check(self.L.rsg_batch_set_launch(self.h, int(arenas_per_block)))
+
+    def get_launch(self):
+        return native.check(self.L.rsg_batch_get_launch(self.h))
+
     def get_car_order(self):
         """Per-arena car update order (flattened like the car arrays)."""
         out = (C.c_int32 * max(1, self.n_cars))()

@@ -71,7 +71,13 @@ int upload_array(T** dst, const std::vector<T>& src) {
 // ---------------------------------------------------------------------------
 // kernels
 
-__global__ void __launch_bounds__(64) rsg_step_arena_kernel(BatchDev d, MeshView mesh, int ticks) {
+// Waves per SIMD the step kernel is compiled for (register budget).
+#ifndef RSG_STEP_WAVES_PER_EU
+#define RSG_STEP_WAVES_PER_EU 1
+#endif
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
+rsg_step_arena_kernel(BatchDev d, MeshView mesh, int ticks) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= d.n) return;
     ArenaLocal A;
@@ -110,6 +116,7 @@ struct rsg_batch {
     bool host_dirty = false;  // host mirror newer than device
     bool dev_newer = false;   // device newer than host mirror
     hipStream_t last_stream = nullptr;
+    int arenas_per_block = 0;  // 0: derived from the device's SIMD count
 };
 
 namespace {
@@ -280,13 +287,42 @@ int rsg_batch_car_offset(const rsg_batch* b, int arena) {
     return b->car_offset[arena];
 }
 
+// Arenas per workgroup.  One lane simulates one arena, and the kernel's
+// register and scratch footprint allows one wave per SIMD, so a launch is
+// latency bound: spreading the arenas over every SIMD (partial waves) beats
+// packing 64 lanes into few waves.  The default gives each SIMD of the
+// device at most one wave; rsg_batch_set_launch overrides it.
+static int launch_block(rsg_batch* b) {
+    if (b->arenas_per_block > 0) return b->arenas_per_block;
+    static int simds = 0;
+    if (simds == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        simds = 4 * cus;
+    }
+    int per = (b->dev.n + simds - 1) / simds;
+    return per < 1 ? 1 : (per > 64 ? 64 : per);
+}
+
+int rsg_batch_set_launch(rsg_batch* b, int arenas_per_block) {
+    if (!b) return fail("null batch");
+    if (arenas_per_block < 0 || arenas_per_block > 64)
+        return fail("rsg_batch_set_launch: arenas_per_block must be 0 (auto) or 1..64");
+    b->arenas_per_block = arenas_per_block;
+    return 0;
+}
+
+int rsg_batch_get_launch(rsg_batch* b) { return b ? launch_block(b) : fail("null batch"); }
+
 int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
     if (!b) return fail("null batch");
     if (ticks < 0) return fail("rsg_batch_step: ticks must be >= 0");
     if (ticks == 0) return 0;
     if (upload(b) != 0) return -1;
     hipStream_t s = (hipStream_t)stream;
-    int block = 64;
+    int block = launch_block(b);
     int grid = (b->dev.n + block - 1) / block;
     MeshView mv{};
     if (b->dev.mode == RSG_MODE_SOCCAR) {

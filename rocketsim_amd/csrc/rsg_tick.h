@@ -366,12 +366,14 @@ RSG_FN void pads_check_car(ArenaLocal& A, int ci) {
     CarPhys cphys = make_car_phys(c.preset);
     int x0 = ix - 1 > 0 ? ix - 1 : 0, x1 = ix + 1 < PAD_GRID_CELLS_X - 1 ? ix + 1 : PAD_GRID_CELLS_X - 1;
     int y0 = iy - 1 > 0 ? iy - 1 : 0, y1 = iy + 1 < PAD_GRID_CELLS_Y - 1 ? iy + 1 : PAD_GRID_CELLS_Y - 1;
-    for (int i = x0; i <= x1; i++)
-        for (int j = y0; j <= y1; j++)
-            for (int p = 0; p < PAD_NUM; p++) {
-                V3 pl = pad_location(p);
-                if (pad_cell_x(pl) == i && pad_cell_y(pl) == j) pad_check_collide(A.pads[p], p, c, ci, cphys);
-            }
+    // Each grid cell holds at most one pad and a pad's test touches only its
+    // own state, so visiting the pads whose cell lies in the 3x3 window in
+    // pad order is the same as visiting the window's cells.
+    for (int p = 0; p < PAD_NUM; p++) {
+        V3 pl = pad_location(p);
+        int px = pad_cell_x(pl), py = pad_cell_y(pl);
+        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A.pads[p], p, c, ci, cphys);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,43 @@
+// C++ host code written against include/rsg.hpp the way reference code is
+// written against Arena.h: create an arena, set a state, Step, read states.
+// Usage: arena_step_demo <soccar.cmf> <ticks>
+// Prints the ball and car 0 positions after `ticks` ticks, one per line.
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iterator>
+#include <vector>
+
+#include "rsg.hpp"
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s <soccar.cmf> <ticks>\n", argv[0]);
+        return 2;
+    }
+    try {
+        std::ifstream f(argv[1], std::ios::binary);
+        std::vector<char> cmf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        rsg_amd::InitFromMem(cmf.data(), cmf.size());
+        auto arena = rsg_amd::Arena::Create(rsg_amd::GameMode::SOCCAR,
+                                            {{rsg_amd::Team::BLUE, RSG_PRESET_OCTANE},
+                                             {rsg_amd::Team::ORANGE, RSG_PRESET_DOMINUS}});
+        arena->ResetToRandomKickoff(7);
+        rsg_amd::BallState b = arena->GetBallState();
+        b.vel[0] = 300.f; b.vel[1] = -200.f; b.vel[2] = 650.f;
+        arena->SetBallState(b);
+        rsg_amd::CarControls c{};
+        c.throttle = 1.f; c.boost = 1;
+        arena->SetCarControls(0, c);
+        arena->Step(std::atoi(argv[2]));
+        b = arena->GetBallState();
+        rsg_amd::CarState s = arena->GetCarState(0);
+        std::printf("ball %.6f %.6f %.6f\n", b.pos[0], b.pos[1], b.pos[2]);
+        std::printf("car0 %.6f %.6f %.6f\n", s.pos[0], s.pos[1], s.pos[2]);
+        std::printf("tick %llu\n", (unsigned long long)arena->TickCount());
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
