@@ -30,7 +30,7 @@ BYTES_PER_ARENA_TICK_2V2 = 2488
 # HBM bytes per launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE passes
 # over `bench.py --profile-probe` (counter factor 1.0 from a 64 MiB copy),
 # keyed by (arenas, ticks per launch, arenas per workgroup).
-HBM_TRAFFIC_PER_LAUNCH = {(4096, 8, 4): (825453 + 516258) * 1024}
+HBM_TRAFFIC_PER_LAUNCH = {("rsg_step_arena_kernel", 4096, 8, 4): (825453 + 516258) * 1024}
 
 
 def cpu_baseline(target_seconds=10.0):
@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--arenas-per-block", type=int, default=0,
                     help="arenas per workgroup of the step kernel (0 = spread over every SIMD)")
+    ap.add_argument("--kernel", choices=["auto", "scratch", "lds"], default="auto",
+                    help="where the step kernel keeps each arena's working copy")
     ap.add_argument("--chunk", type=int, default=8,
                     help="ticks per kernel launch (the arena state stays on chip across a chunk)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -87,8 +89,10 @@ def main():
     native.check(native.lib().rsg_init_mesh(arena_mesh.soccar_cmf(), len(arena_mesh.soccar_cmf())))
     batch = ArenaBatch.uniform(args.arenas, 2, 2, 0, mode=MODE_SOCCAR, seed=args.seed,
                                synth_controls=True, auto_reset=True)
+    batch.set_kernel({"auto": 0, "scratch": 1, "lds": 2}[args.kernel])
     batch.set_launch(args.arenas_per_block)
     per_block = batch.get_launch()
+    kernel_name = {1: "rsg_step_arena_kernel", 2: "rsg_step_arena_lds_kernel"}[batch.get_kernel()]
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
 
@@ -147,13 +151,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"soccar 2v2 x {args.arenas} arenas, Octane, seeded controls, auto-reset on goal",
                    "arenas": args.arenas, "cars_per_arena": 4, "tick_rate": 120,
-                   "arenas_per_block": per_block, "ticks_per_launch": chunk,
+                   "arenas_per_block": per_block, "ticks_per_launch": chunk, "kernel": kernel_name,
                    "goals_during_run": int(sum(goals[0]) + sum(goals[1]))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": HBM_TRAFFIC_PER_LAUNCH.get((args.arenas, chunk, per_block)),
+                     "traffic": HBM_TRAFFIC_PER_LAUNCH.get((kernel_name, args.arenas, chunk, per_block)),
                      "traffic_unit": "B per launch",
-                     "kernel": "rsg_step_arena_kernel", "kernel_ms": kernel_ms,
+                     "kernel": kernel_name, "kernel_ms": kernel_ms,
                      "bytes_per_arena_tick": BYTES_PER_ARENA_TICK_2V2},
         "cpu_baseline": cpu,
     }

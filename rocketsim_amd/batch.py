@@ -111,6 +111,13 @@ class ArenaBatch:
         """Arenas per workgroup of the step kernel (0 = device default)."""
         native.check(self.L.rsg_batch_set_launch(self.h, int(arenas_per_block)))
 
+    def set_kernel(self, kernel):
+        """KERNEL_AUTO, KERNEL_SCRATCH or KERNEL_LDS (rocketsim_amd.state)."""
+        native.check(self.L.rsg_batch_set_kernel(self.h, int(kernel)))
+
+    def get_kernel(self):
+        return native.check(self.L.rsg_batch_get_kernel(self.h))
+
     def get_launch(self):
         return native.check(self.L.rsg_batch_get_launch(self.h))
 

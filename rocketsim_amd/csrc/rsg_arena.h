@@ -7,8 +7,10 @@ namespace rsg {
 
 constexpr int MAX_CARS = RSG_MAX_CARS;
 
+// The car array comes last: an arena with n cars touches only the first
+// offsetof(ArenaLocal, cars) + n * sizeof(Car) bytes, which lets a kernel
+// pack arenas of a batch with at most n cars per arena at that stride.
 struct ArenaLocal {
-    Car cars[MAX_CARS];
     int n_cars;
     Ball ball;
     Pad pads[PAD_NUM];
@@ -25,6 +27,7 @@ struct ArenaLocal {
     // unordered_set<Car*>, src/Sim/Arena/Arena.h:35; it is iterated for the
     // pre-tick, post-tick/boost-pad and kickoff loops).
     uint8_t order[MAX_CARS];
+    Car cars[MAX_CARS];
 };
 
 struct TickParams {

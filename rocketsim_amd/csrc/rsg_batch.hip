@@ -76,12 +76,10 @@ int upload_array(T** dst, const std::vector<T>& src) {
 #define RSG_STEP_WAVES_PER_EU 1
 #endif
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
-rsg_step_arena_kernel(BatchDev d, MeshView mesh, int ticks) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= d.n) return;
-    ArenaLocal A;
+RSG_FN void step_arena(const BatchDev& d, const MeshView& mesh, int a, int ticks, ArenaLocal& A) {
+    RSG_PROF_MARK(15);
     load_arena(d, a, A);
+    RSG_PROF_MARK(15);
     TickParams P;
     P.mode = d.mode;
     P.dt = d.tick_time;
@@ -92,6 +90,27 @@ rsg_step_arena_kernel(BatchDev d, MeshView mesh, int ticks) {
     const MeshView* mp = d.mode == RSG_MODE_SOCCAR ? &mesh : nullptr;
     for (int t = 0; t < ticks; t++) arena_tick(A, P, mp);
     store_arena(d, a, A);
+}
+
+// One lane per arena; the arena's working copy lives in the lane's private
+// (scratch) memory.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
+rsg_step_arena_kernel(BatchDev d, MeshView mesh, int ticks) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= d.n) return;
+    ArenaLocal A;
+    step_arena(d, mesh, a, ticks, A);
+}
+
+// One lane per arena; the arena's working copy lives in LDS, `stride` bytes
+// per lane (the ArenaLocal prefix up to the batch's largest car count).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
+rsg_step_arena_lds_kernel(BatchDev d, MeshView mesh, int ticks, int stride) {
+    extern __shared__ __align__(16) unsigned char rsg_lds[];
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= d.n) return;
+    ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(rsg_lds + (size_t)threadIdx.x * (size_t)stride);
+    step_arena(d, mesh, a, ticks, A);
 }
 
 __global__ void __launch_bounds__(64) rsg_reset_kernel(BatchDev d, int only_arena, uint64_t seed) {
@@ -117,6 +136,7 @@ struct rsg_batch {
     bool dev_newer = false;   // device newer than host mirror
     hipStream_t last_stream = nullptr;
     int arenas_per_block = 0;  // 0: derived from the device's SIMD count
+    int kernel = RSG_KERNEL_AUTO;
 };
 
 namespace {
@@ -287,13 +307,19 @@ int rsg_batch_car_offset(const rsg_batch* b, int arena) {
     return b->car_offset[arena];
 }
 
-// Arenas per workgroup.  One lane simulates one arena, and the kernel's
-// register and scratch footprint allows one wave per SIMD, so a launch is
+// Launch geometry.  One lane simulates one arena and the kernel runs at one
+// wave per SIMD (its register and private-memory footprint), so a launch is
 // latency bound: spreading the arenas over every SIMD (partial waves) beats
-// packing 64 lanes into few waves.  The default gives each SIMD of the
-// device at most one wave; rsg_batch_set_launch overrides it.
-static int launch_block(rsg_batch* b) {
-    if (b->arenas_per_block > 0) return b->arenas_per_block;
+// packing 64 lanes into few waves.
+//
+// Two kernels: the arena's working copy in private scratch, or in LDS.  LDS
+// cuts the latency of every state access but holds only ~40 KB per wave at
+// one wave per SIMD (160 KB per CU), i.e. a few arenas per wave; the LDS
+// kernel is chosen when that still covers the batch in one round of waves.
+namespace {
+constexpr size_t LDS_PER_WAVE = 40 * 1024;
+
+int device_simds() {
     static int simds = 0;
     if (simds == 0) {
         int dev = 0, cus = 0;
@@ -302,9 +328,42 @@ static int launch_block(rsg_batch* b) {
             cus = 256;
         simds = 4 * cus;
     }
-    int per = (b->dev.n + simds - 1) / simds;
-    return per < 1 ? 1 : (per > 64 ? 64 : per);
+    return simds;
 }
+
+int lds_stride(const rsg_batch* b) {
+    size_t bytes = offsetof(ArenaLocal, cars) + (size_t)b->dev.max_cars * sizeof(Car);
+    return (int)((bytes + 15) & ~(size_t)15);
+}
+
+int lds_max_per_block(const rsg_batch* b) {
+    int m = (int)(LDS_PER_WAVE / (size_t)lds_stride(b));
+    return m < 1 ? 0 : (m > 64 ? 64 : m);
+}
+
+struct Launch {
+    int kernel;  // RSG_KERNEL_SCRATCH or RSG_KERNEL_LDS
+    int per_block;
+};
+
+Launch choose_launch(const rsg_batch* b) {
+    const int simds = device_simds();
+    int spread = (b->dev.n + simds - 1) / simds;  // arenas per wave at one wave per SIMD
+    spread = spread < 1 ? 1 : (spread > 64 ? 64 : spread);
+    Launch L;
+    L.kernel = b->kernel;
+    if (L.kernel == RSG_KERNEL_AUTO)
+        L.kernel = (lds_max_per_block(b) >= spread) ? RSG_KERNEL_LDS : RSG_KERNEL_SCRATCH;
+    if (b->arenas_per_block > 0) {
+        L.per_block = b->arenas_per_block;
+    } else {
+        L.per_block = spread;
+    }
+    if (L.kernel == RSG_KERNEL_LDS && L.per_block > lds_max_per_block(b)) L.per_block = lds_max_per_block(b);
+    if (L.per_block < 1) L.per_block = 1;
+    return L;
+}
+}  // namespace
 
 int rsg_batch_set_launch(rsg_batch* b, int arenas_per_block) {
     if (!b) return fail("null batch");
@@ -314,7 +373,19 @@ int rsg_batch_set_launch(rsg_batch* b, int arenas_per_block) {
     return 0;
 }
 
-int rsg_batch_get_launch(rsg_batch* b) { return b ? launch_block(b) : fail("null batch"); }
+int rsg_batch_get_launch(rsg_batch* b) { return b ? choose_launch(b).per_block : fail("null batch"); }
+
+int rsg_batch_set_kernel(rsg_batch* b, int kernel) {
+    if (!b) return fail("null batch");
+    if (kernel != RSG_KERNEL_AUTO && kernel != RSG_KERNEL_SCRATCH && kernel != RSG_KERNEL_LDS)
+        return fail("rsg_batch_set_kernel: unknown kernel");
+    if (kernel == RSG_KERNEL_LDS && lds_max_per_block(b) < 1)
+        return fail("rsg_batch_set_kernel: an arena of this batch does not fit the LDS budget");
+    b->kernel = kernel;
+    return 0;
+}
+
+int rsg_batch_get_kernel(rsg_batch* b) { return b ? choose_launch(b).kernel : fail("null batch"); }
 
 int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
     if (!b) return fail("null batch");
@@ -322,14 +393,21 @@ int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
     if (ticks == 0) return 0;
     if (upload(b) != 0) return -1;
     hipStream_t s = (hipStream_t)stream;
-    int block = launch_block(b);
+    Launch L = choose_launch(b);
+    int block = L.per_block;
     int grid = (b->dev.n + block - 1) / block;
     MeshView mv{};
     if (b->dev.mode == RSG_MODE_SOCCAR) {
         if (!g_mesh.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
         mv = g_mesh.view;
     }
-    hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(block), 0, s, b->dev, mv, ticks);
+    if (L.kernel == RSG_KERNEL_LDS) {
+        int stride = lds_stride(b);
+        size_t lds = (size_t)block * (size_t)stride;
+        hipLaunchKernelGGL(rsg_step_arena_lds_kernel, dim3(grid), dim3(block), lds, s, b->dev, mv, ticks, stride);
+    } else {
+        hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(block), 0, s, b->dev, mv, ticks);
+    }
     HIP_OK(hipGetLastError());
     b->last_stream = s;
     b->dev_newer = true;

@@ -431,7 +431,11 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
 
         // updateVehicleFirst (btVehicleRL.cpp:215-228)
         for (int i = 0; i < 4; i++) wheel_update_transform(c, cp, i, c.w[i]);
-        for (int i = 0; i < 4; i++) vehicle_ray_cast(c, cp, i, world, 1 + ci, A.si);
+        RSG_PROF_MARK(1);
+        for (int i = 0; i < 4; i++) {
+            vehicle_ray_cast(c, cp, i, world, 1 + ci, A.si);
+            RSG_PROF_MARK(6);
+        }
         vehicle_calc_friction(c, world, dt);
 
         bool jump_pressed = c.ctrl.jump && !c.last.jump;

@@ -33,6 +33,9 @@ CF_DEMOED = 1 << 14
 CF_HIT_VALID = 1 << 15
 CF_BODY_DISABLED = 1 << 16
 
+# rsg_batch_set_kernel
+KERNEL_AUTO, KERNEL_SCRATCH, KERNEL_LDS = 0, 1, 2
+
 F3 = C.c_float * 3
 F4 = C.c_float * 4
 F9 = C.c_float * 9
