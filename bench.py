@@ -64,7 +64,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--arenas-per-block", type=int, default=0,
                     help="arenas per workgroup of the step kernel (0 = spread over every SIMD)")
-    ap.add_argument("--kernel", choices=["auto", "scratch", "lds"], default="auto",
+    ap.add_argument("--kernel", choices=["auto", "scratch", "lds", "coop"], default="auto",
                     help="where the step kernel keeps each arena's working copy")
     ap.add_argument("--chunk", type=int, default=8,
                     help="ticks per kernel launch (the arena state stays on chip across a chunk)")
@@ -89,10 +89,11 @@ def main():
     native.check(native.lib().rsg_init_mesh(arena_mesh.soccar_cmf(), len(arena_mesh.soccar_cmf())))
     batch = ArenaBatch.uniform(args.arenas, 2, 2, 0, mode=MODE_SOCCAR, seed=args.seed,
                                synth_controls=True, auto_reset=True)
-    batch.set_kernel({"auto": 0, "scratch": 1, "lds": 2}[args.kernel])
+    batch.set_kernel({"auto": 0, "scratch": 1, "lds": 2, "coop": 3}[args.kernel])
     batch.set_launch(args.arenas_per_block)
     per_block = batch.get_launch()
-    kernel_name = {1: "rsg_step_arena_kernel", 2: "rsg_step_arena_lds_kernel"}[batch.get_kernel()]
+    kernel_name = {1: "rsg_step_arena_kernel", 2: "rsg_step_arena_lds_kernel",
+                   3: "rsg_step_arena_coop_kernel"}[batch.get_kernel()]
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
 

@@ -96,12 +96,16 @@ int rsg_batch_get_car_order(rsg_batch* b, int32_t* order);
  * not depend on it.  rsg_batch_get_launch returns the value in use. */
 int rsg_batch_set_launch(rsg_batch* b, int arenas_per_block);
 int rsg_batch_get_launch(rsg_batch* b);
-/* Where the step kernel keeps each arena's working copy (no counterpart in
- * the reference): RSG_KERNEL_SCRATCH (private memory, any batch),
- * RSG_KERNEL_LDS (on-chip LDS, a few arenas per wave) or RSG_KERNEL_AUTO
- * (LDS when the batch fits one round of waves that way).  Results do not
- * depend on it; rsg_batch_get_kernel returns the kernel in use. */
-enum { RSG_KERNEL_AUTO = 0, RSG_KERNEL_SCRATCH = 1, RSG_KERNEL_LDS = 2 };
+/* Step kernel variant (no counterpart in the reference):
+ *   RSG_KERNEL_SCRATCH  one lane per arena, working copy in private memory;
+ *   RSG_KERNEL_LDS      one lane per arena, working copy in LDS;
+ *   RSG_KERNEL_COOP     4 lanes per car slot per arena, working copy in LDS,
+ *                       the wheel rays of a tick cast by all lanes at once;
+ *   RSG_KERNEL_AUTO     the first of COOP, LDS, SCRATCH whose footprint
+ *                       covers the batch in one round of waves.
+ * Results do not depend on it; rsg_batch_get_kernel returns the kernel in
+ * use.  (rsg_batch_set_launch's arenas_per_block counts arenas, not lanes.) */
+enum { RSG_KERNEL_AUTO = 0, RSG_KERNEL_SCRATCH = 1, RSG_KERNEL_LDS = 2, RSG_KERNEL_COOP = 3 };
 int rsg_batch_set_kernel(rsg_batch* b, int kernel);
 int rsg_batch_get_kernel(rsg_batch* b);
 /* Arena::tickCount (Arena.h:61). */

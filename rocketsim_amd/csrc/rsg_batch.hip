@@ -113,6 +113,43 @@ rsg_step_arena_lds_kernel(BatchDev d, MeshView mesh, int ticks, int stride) {
     step_arena(d, mesh, a, ticks, A);
 }
 
+// Cooperative variant: `lanes` lanes per arena (4 per car slot), arena in
+// LDS.  Lane 0 of each arena runs the tick; at the start of each pre-tick all
+// lanes cast the arena's wheel rays, one ray per lane, into LDS (unless a car
+// respawns that tick, when lane 0 casts them in the loop as usual).  Per-arena
+// LDS block: ArenaLocal prefix, ray results, one flag word.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
+rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int arena_bytes, int lanes) {
+    extern __shared__ __align__(16) unsigned char rsg_lds[];
+    const int slot = (int)threadIdx.x / lanes;
+    const int k = (int)threadIdx.x - slot * lanes;
+    const int a = (int)blockIdx.x * ((int)blockDim.x / lanes) + slot;
+    const bool live = a < d.n;
+    unsigned char* base = rsg_lds + (size_t)slot * (size_t)stride;
+    ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(base);
+    RayHit* hits = reinterpret_cast<RayHit*>(base + arena_bytes);
+    int* ahead = reinterpret_cast<int*>(base + arena_bytes + (size_t)(lanes) * sizeof(RayHit));
+    TickParams P;
+    P.mode = d.mode;
+    P.dt = d.tick_time;
+    P.seed = d.seed;
+    P.arena_id = (uint32_t)a;
+    P.synth_controls = d.synth_controls != 0;
+    P.auto_reset = d.auto_reset != 0;
+    const MeshView* mp = d.mode == RSG_MODE_SOCCAR ? &mesh : nullptr;
+    if (live && k == 0) load_arena(d, a, A);
+    __syncthreads();
+    for (int t = 0; t < ticks; t++) {
+        if (live && k == 0) *ahead = tick_begin(A, P) ? 1 : 0;
+        __syncthreads();
+        if (live && *ahead && k < 4 * A.n_cars) tick_cast_ray(A, P, mp, k >> 2, k & 3, hits[k]);
+        __syncthreads();
+        if (live && k == 0) tick_rest(A, P, mp, *ahead ? hits : nullptr);
+        __syncthreads();
+    }
+    if (live && k == 0) store_arena(d, a, A);
+}
+
 __global__ void __launch_bounds__(64) rsg_reset_kernel(BatchDev d, int only_arena, uint64_t seed) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= d.n) return;
@@ -341,6 +378,19 @@ int lds_max_per_block(const rsg_batch* b) {
     return m < 1 ? 0 : (m > 64 ? 64 : m);
 }
 
+// Cooperative kernel: 4 lanes per car slot, arena + ray results + flag in LDS.
+int coop_lanes(const rsg_batch* b) { return 4 * b->dev.max_cars; }
+int coop_stride(const rsg_batch* b) {
+    size_t bytes = (size_t)lds_stride(b) + (size_t)coop_lanes(b) * sizeof(RayHit) + 16;
+    return (int)((bytes + 15) & ~(size_t)15);
+}
+int coop_max_per_block(const rsg_batch* b) {
+    int by_lanes = 64 / coop_lanes(b);
+    int by_lds = (int)(LDS_PER_WAVE / (size_t)coop_stride(b));
+    int m = by_lanes < by_lds ? by_lanes : by_lds;
+    return m < 1 ? 0 : m;
+}
+
 struct Launch {
     int kernel;  // RSG_KERNEL_SCRATCH or RSG_KERNEL_LDS
     int per_block;
@@ -352,14 +402,18 @@ Launch choose_launch(const rsg_batch* b) {
     spread = spread < 1 ? 1 : (spread > 64 ? 64 : spread);
     Launch L;
     L.kernel = b->kernel;
-    if (L.kernel == RSG_KERNEL_AUTO)
-        L.kernel = (lds_max_per_block(b) >= spread) ? RSG_KERNEL_LDS : RSG_KERNEL_SCRATCH;
+    if (L.kernel == RSG_KERNEL_AUTO) {
+        if (coop_max_per_block(b) >= spread) L.kernel = RSG_KERNEL_COOP;
+        else if (lds_max_per_block(b) >= spread) L.kernel = RSG_KERNEL_LDS;
+        else L.kernel = RSG_KERNEL_SCRATCH;
+    }
     if (b->arenas_per_block > 0) {
         L.per_block = b->arenas_per_block;
     } else {
         L.per_block = spread;
     }
     if (L.kernel == RSG_KERNEL_LDS && L.per_block > lds_max_per_block(b)) L.per_block = lds_max_per_block(b);
+    if (L.kernel == RSG_KERNEL_COOP && L.per_block > coop_max_per_block(b)) L.per_block = coop_max_per_block(b);
     if (L.per_block < 1) L.per_block = 1;
     return L;
 }
@@ -377,10 +431,13 @@ int rsg_batch_get_launch(rsg_batch* b) { return b ? choose_launch(b).per_block :
 
 int rsg_batch_set_kernel(rsg_batch* b, int kernel) {
     if (!b) return fail("null batch");
-    if (kernel != RSG_KERNEL_AUTO && kernel != RSG_KERNEL_SCRATCH && kernel != RSG_KERNEL_LDS)
+    if (kernel != RSG_KERNEL_AUTO && kernel != RSG_KERNEL_SCRATCH && kernel != RSG_KERNEL_LDS &&
+        kernel != RSG_KERNEL_COOP)
         return fail("rsg_batch_set_kernel: unknown kernel");
     if (kernel == RSG_KERNEL_LDS && lds_max_per_block(b) < 1)
         return fail("rsg_batch_set_kernel: an arena of this batch does not fit the LDS budget");
+    if (kernel == RSG_KERNEL_COOP && coop_max_per_block(b) < 1)
+        return fail("rsg_batch_set_kernel: an arena of this batch does not fit the cooperative kernel");
     b->kernel = kernel;
     return 0;
 }
@@ -401,7 +458,13 @@ int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
         if (!g_mesh.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
         mv = g_mesh.view;
     }
-    if (L.kernel == RSG_KERNEL_LDS) {
+    if (L.kernel == RSG_KERNEL_COOP) {
+        int lanes = coop_lanes(b);
+        int stride = coop_stride(b);
+        size_t lds = (size_t)block * (size_t)stride;
+        hipLaunchKernelGGL(rsg_step_arena_coop_kernel, dim3(grid), dim3(block * lanes), lds, s, b->dev, mv, ticks,
+                           stride, lds_stride(b), lanes);
+    } else if (L.kernel == RSG_KERNEL_LDS) {
         int stride = lds_stride(b);
         size_t lds = (size_t)block * (size_t)stride;
         hipLaunchKernelGGL(rsg_step_arena_lds_kernel, dim3(grid), dim3(block), lds, s, b->dev, mv, ticks, stride);

@@ -103,6 +103,21 @@ struct World {
         if (code <= 0) return true;
         return A.cars[code - 1].b.active;
     }
+    // Contact response of a ray's closest hit.  Serially this is the body's
+    // current flag.  When all wheel rays are cast at the start of the
+    // pre-tick (ray_caster_pos >= 0: the casting car's position in the car
+    // update order), a car updated before the caster already carries this
+    // tick's flag (no response iff demolished), one updated after it still
+    // carries the previous tick's.
+    int ray_caster_pos = -1;
+    RSG_FN bool ray_target_responds(int code) {
+        if (code <= 0) return true;
+        const Car& t = A.cars[code - 1];
+        if (ray_caster_pos < 0) return t.b.active;
+        int pos = 0;
+        while (pos < A.n_cars && A.order[pos] != code - 1) pos++;
+        return pos < ray_caster_pos ? !flag(t.flags, RSG_CF_DEMOED) : t.b.active;
+    }
     RSG_FN bool active(int code) {
         if (code < 0) return false;
         return body_ptr(code)->active;
@@ -249,7 +264,7 @@ struct World {
         RSG_PROF_MARK(9);
         if (hit_obj != -100) {
             // btDefaultVehicleRaycaster::castRay: the closest hit must respond
-            if (hit_obj >= 0 && !has_response(hit_obj)) return h;
+            if (hit_obj >= 0 && !ray_target_responds(hit_obj)) return h;
             h.hit = true;
             h.is_static = hit_obj < 0;
             h.body = hit_obj >= 0 ? hit_obj : -1;

@@ -607,9 +607,22 @@ RSG_FN void vehicle_calc_friction(Car& c, World& world, float dt) {
     }
 }
 
-// btVehicleRL::rayCast (btVehicleRL.cpp:120-209)
+// The suspension ray of wheel i (btVehicleRL.cpp:123-131): from the wheel's
+// hard point along the suspension direction, rest + travel + radius - 0.05.
+RSG_FN void wheel_ray(const Car& c, const CarPhys& cp, int i, V3& source, V3& target) {
+    WheelT w;
+    wheel_transforms_ws(c, cp, i, w);
+    float ray_len = cp.wheel_rest[i] + cp.max_susp_travel + cp.wheel_radius[i] - SUSPENSION_SUBTRACTION;
+    source = w.hard_point;
+    target = source + (w.dir_ws * ray_len);
+}
+
+// btVehicleRL::rayCast (btVehicleRL.cpp:120-209).  `pre`, when given, is this
+// wheel's ray result cast earlier in the same pre-tick (nothing it depends on
+// has changed since).
 template <class World>
-RSG_FN void vehicle_ray_cast(Car& c, const CarPhys& cp, int i, World& world, int self_body, const SolverInfo& si) {
+RSG_FN void vehicle_ray_cast(Car& c, const CarPhys& cp, int i, World& world, int self_body, const SolverInfo& si,
+                             const RayHit* pre) {
     WheelT& w = c.w[i];
     wheel_transforms_ws(c, cp, i, w);
     float travel = cp.max_susp_travel;
@@ -620,7 +633,7 @@ RSG_FN void vehicle_ray_cast(Car& c, const CarPhys& cp, int i, World& world, int
     V3 target = source + (w.dir_ws * ray_len);
     w.contact_point = target;
     w.ground = -1;
-    RayHit h = world.raycast(source, target, self_body);
+    RayHit h = pre ? *pre : world.raycast(source, target, self_body);
     V3 up = car_up(c);
     if (h.hit) {
         w.contact_point = h.point;

@@ -379,10 +379,11 @@ RSG_FN void pads_check_car(ArenaLocal& A, int ci) {
 // ---------------------------------------------------------------------------
 // The tick
 
-RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh) {
-    const float dt = P.dt;
-    const bool soccar = P.mode == RSG_MODE_SOCCAR;
-
+// Start of Arena::Step: the synthetic controls and the ball's sleep flag.
+// Returns whether every wheel ray of the tick may be cast before the per-car
+// pre-tick loop: true unless a demolished car respawns this tick, which moves
+// it in the middle of the loop.
+RSG_FN bool tick_begin(ArenaLocal& A, const TickParams& P) {
     if (P.synth_controls) {
         for (int i = 0; i < A.n_cars; i++) {
             Car& c = A.cars[i];
@@ -401,6 +402,34 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
 
     // 1. ball sleep flag (Arena.cpp:694-700)
     A.ball.b.active = !(len2(A.ball.b.vel) == 0.f && len2(A.ball.b.ang) == 0.f);
+
+    for (int i = 0; i < A.n_cars; i++) {
+        const Car& c = A.cars[i];
+        if (flag(c.flags, RSG_CF_DEMOED) && rs_max(c.demo_timer - P.dt, 0.f) == 0.f) return false;
+    }
+    return true;
+}
+
+// Wheel ray `wheel` of car `ci` as the pre-tick loop will cast it, computed
+// ahead of the loop (valid when tick_begin returned true): poses, broadphase
+// cells and contact-response flags are those the car sees at its turn.
+RSG_FN void tick_cast_ray(ArenaLocal& A, const TickParams& P, const MeshView* mesh, int ci, int wheel, RayHit& out) {
+    World world(A, mesh, P.mode == RSG_MODE_SOCCAR && mesh != nullptr);
+    int pos = 0;
+    while (pos < A.n_cars && A.order[pos] != ci) pos++;
+    world.ray_caster_pos = pos;
+    const Car& c = A.cars[ci];
+    CarPhys cp = make_car_phys(c.preset);
+    V3 source, target;
+    wheel_ray(c, cp, wheel, source, target);
+    out = world.raycast(source, target, 1 + ci);
+}
+
+// The rest of Arena::Step after tick_begin.  `pre` holds every wheel's ray
+// result (car-major, 4 per car) when cast ahead, else null.
+RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, const RayHit* pre) {
+    const float dt = P.dt;
+    const bool soccar = P.mode == RSG_MODE_SOCCAR;
 
     World world(A, mesh, soccar && mesh != nullptr);
 
@@ -433,7 +462,7 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
         for (int i = 0; i < 4; i++) wheel_update_transform(c, cp, i, c.w[i]);
         RSG_PROF_MARK(1);
         for (int i = 0; i < 4; i++) {
-            vehicle_ray_cast(c, cp, i, world, 1 + ci, A.si);
+            vehicle_ray_cast(c, cp, i, world, 1 + ci, A.si, pre ? &pre[ci * 4 + i] : nullptr);
             RSG_PROF_MARK(6);
         }
         vehicle_calc_friction(c, world, dt);
@@ -561,6 +590,12 @@ RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
     A.tick++;
     if (scored && P.auto_reset) arena_kickoff_reset(A, P.seed, P.arena_id);
     RSG_PROF_MARK(8);
+}
+
+// Arena::Step (src/Sim/Arena/Arena.cpp:689-756), one tick.
+RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh) {
+    tick_begin(A, P);
+    tick_rest(A, P, mesh, nullptr);
 }
 
 }  // namespace rsg

@@ -34,7 +34,7 @@ CF_HIT_VALID = 1 << 15
 CF_BODY_DISABLED = 1 << 16
 
 # rsg_batch_set_kernel
-KERNEL_AUTO, KERNEL_SCRATCH, KERNEL_LDS = 0, 1, 2
+KERNEL_AUTO, KERNEL_SCRATCH, KERNEL_LDS, KERNEL_COOP = 0, 1, 2, 3
 
 F3 = C.c_float * 3
 F4 = C.c_float * 4

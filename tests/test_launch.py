@@ -3,7 +3,7 @@ LDS) nor the launch geometry (arenas per workgroup) changes results: the same
 batch stepped each way ends bit-identical."""
 import pytest
 
-from rocketsim_amd.state import KERNEL_AUTO, KERNEL_LDS, KERNEL_SCRATCH, MODE_SOCCAR
+from rocketsim_amd.state import KERNEL_AUTO, KERNEL_COOP, KERNEL_LDS, KERNEL_SCRATCH, MODE_SOCCAR
 
 
 @pytest.mark.gpu
@@ -12,7 +12,8 @@ def test_kernel_and_launch_bit_identical(cars):
     from rocketsim_amd.batch import ArenaBatch
     outs = []
     for kernel, per_block in ((KERNEL_SCRATCH, 64), (KERNEL_SCRATCH, 1), (KERNEL_SCRATCH, 7),
-                              (KERNEL_LDS, 1), (KERNEL_LDS, 3), (KERNEL_AUTO, 0)):
+                              (KERNEL_LDS, 1), (KERNEL_LDS, 3), (KERNEL_COOP, 1), (KERNEL_COOP, 2),
+                              (KERNEL_AUTO, 0)):
         b = ArenaBatch.uniform(300, cars[0], cars[1], 0, mode=MODE_SOCCAR, seed=99, synth_controls=True,
                                auto_reset=True)
         try:
