@@ -30,7 +30,8 @@ BYTES_PER_ARENA_TICK_2V2 = 2488
 # HBM bytes per launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE passes
 # over `bench.py --profile-probe` (counter factor 1.0 from a 64 MiB copy),
 # keyed by (arenas, ticks per launch, arenas per workgroup).
-HBM_TRAFFIC_PER_LAUNCH = {("rsg_step_arena_kernel", 4096, 8, 4): (825453 + 516258) * 1024}
+HBM_TRAFFIC_PER_LAUNCH = {("rsg_step_arena_kernel", 4096, 8, 4): (825453 + 516258) * 1024,
+                          ("rsg_step_arena_coop_kernel", 4096, 8, 4): (328856 + 570600) * 1024}
 
 
 def cpu_baseline(target_seconds=10.0):
@@ -89,13 +90,19 @@ def main():
     native.check(native.lib().rsg_init_mesh(arena_mesh.soccar_cmf(), len(arena_mesh.soccar_cmf())))
     batch = ArenaBatch.uniform(args.arenas, 2, 2, 0, mode=MODE_SOCCAR, seed=args.seed,
                                synth_controls=True, auto_reset=True)
-    batch.set_kernel({"auto": 0, "scratch": 1, "lds": 2, "coop": 3}[args.kernel])
-    batch.set_launch(args.arenas_per_block)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    tuned = None
+    if args.kernel == "auto" and args.arenas_per_block == 0 and not args.profile_probe:
+        # kernel and launch shape from the autotuner (times each on a copy of the state)
+        k, pb, ms = batch.autotune(8, sp)
+        tuned = {"kernel": k, "arenas_per_block": pb, "ms_per_tick": ms}
+    else:
+        batch.set_kernel({"auto": 0, "scratch": 1, "lds": 2, "coop": 3}[args.kernel])
+        batch.set_launch(args.arenas_per_block)
     per_block = batch.get_launch()
     kernel_name = {1: "rsg_step_arena_kernel", 2: "rsg_step_arena_lds_kernel",
                    3: "rsg_step_arena_coop_kernel"}[batch.get_kernel()]
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
 
     if args.profile_probe:
         src = torch.empty(64 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -153,6 +160,7 @@ def main():
         "config": {"workload": f"soccar 2v2 x {args.arenas} arenas, Octane, seeded controls, auto-reset on goal",
                    "arenas": args.arenas, "cars_per_arena": 4, "tick_rate": 120,
                    "arenas_per_block": per_block, "ticks_per_launch": chunk, "kernel": kernel_name,
+                   "autotuned": tuned is not None,
                    "goals_during_run": int(sum(goals[0]) + sum(goals[1]))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

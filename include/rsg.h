@@ -108,6 +108,12 @@ int rsg_batch_get_launch(rsg_batch* b);
 enum { RSG_KERNEL_AUTO = 0, RSG_KERNEL_SCRATCH = 1, RSG_KERNEL_LDS = 2, RSG_KERNEL_COOP = 3 };
 int rsg_batch_set_kernel(rsg_batch* b, int kernel);
 int rsg_batch_get_kernel(rsg_batch* b);
+/* Autotuner (no counterpart in the reference): times every feasible kernel
+ * and arenas-per-workgroup choice for `ticks` ticks on a device copy of the
+ * batch state (the batch itself is not advanced), keeps the fastest as the
+ * batch's kernel and launch shape, and reports it (any out pointer may be
+ * NULL; out_ms is per tick).  Allocates temporary device memory. */
+int rsg_batch_autotune(rsg_batch* b, int ticks, void* stream, int* out_kernel, int* out_per_block, float* out_ms);
 /* Arena::tickCount (Arena.h:61). */
 int rsg_batch_get_tick_counts(rsg_batch* b, uint64_t* out);
 int rsg_batch_set_tick_counts(rsg_batch* b, const uint64_t* in);

@@ -111,6 +111,13 @@ class ArenaBatch:
         """Arenas per workgroup of the step kernel (0 = device default)."""
         native.check(self.L.rsg_batch_set_launch(self.h, int(arenas_per_block)))
 
+    def autotune(self, ticks=8, stream=None):
+        """Time every kernel / launch shape on a copy of the state, keep the
+        fastest; returns (kernel, arenas_per_block, ms_per_tick)."""
+        k, pb, ms = C.c_int(), C.c_int(), C.c_float()
+        native.check(self.L.rsg_batch_autotune(self.h, ticks, stream, C.byref(k), C.byref(pb), C.byref(ms)))
+        return k.value, pb.value, ms.value
+
     def set_kernel(self, kernel):
         """KERNEL_AUTO, KERNEL_SCRATCH or KERNEL_LDS (rocketsim_amd.state)."""
         native.check(self.L.rsg_batch_set_kernel(self.h, int(kernel)))

@@ -114,10 +114,16 @@ rsg_step_arena_lds_kernel(BatchDev d, MeshView mesh, int ticks, int stride) {
 }
 
 // Cooperative variant: `lanes` lanes per arena (4 per car slot), arena in
-// LDS.  Lane 0 of each arena runs the tick; at the start of each pre-tick all
-// lanes cast the arena's wheel rays, one ray per lane, into LDS (unless a car
-// respawns that tick, when lane 0 casts them in the loop as usual).  Per-arena
-// LDS block: ArenaLocal prefix, ray results, one flag word.
+// LDS.  Lane 0 of each arena runs the serial parts of the tick; the lanes
+// share:
+//   * the wheel rays, one per lane, cast at the start of the pre-tick into
+//     LDS (unless a car respawns that tick: then lane 0 casts them in the
+//     loop as usual);
+//   * the car pre-ticks, one car per lane, when they are independent of
+//     their order (no wheel on a car);
+//   * the car post-ticks, one car per lane, and the boost-pad tests, pads
+//     spread over the lanes.
+// Per-arena LDS block: ArenaLocal prefix, ray results, two flag words.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
 rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int arena_bytes, int lanes) {
     extern __shared__ __align__(16) unsigned char rsg_lds[];
@@ -137,6 +143,7 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
     P.synth_controls = d.synth_controls != 0;
     P.auto_reset = d.auto_reset != 0;
     const MeshView* mp = d.mode == RSG_MODE_SOCCAR ? &mesh : nullptr;
+    int* concurrent = ahead + 1;
     if (live && k == 0) load_arena(d, a, A);
     __syncthreads();
     for (int t = 0; t < ticks; t++) {
@@ -144,7 +151,31 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
         __syncthreads();
         if (live && *ahead && k < 4 * A.n_cars) tick_cast_ray(A, P, mp, k >> 2, k & 3, hits[k]);
         __syncthreads();
-        if (live && k == 0) tick_rest(A, P, mp, *ahead ? hits : nullptr);
+        if (live && k == 0) {
+            const RayHit* pre = *ahead ? hits : nullptr;
+            *concurrent = car_pre_ticks_independent(A, pre) ? 1 : 0;
+            if (!*concurrent) {
+                World world(A, mp, d.mode == RSG_MODE_SOCCAR);
+                for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], pre);
+            }
+        }
+        __syncthreads();
+        if (live && *concurrent && k < A.n_cars) {
+            World world(A, mp, d.mode == RSG_MODE_SOCCAR);
+            car_pre_tick(A, P, world, k, hits);
+        }
+        __syncthreads();
+        if (live && k == 0) {
+            pads_pre_tick(A, P);
+            tick_world(A, P, mp);
+        }
+        __syncthreads();
+        if (live && k < A.n_cars) car_post_tick(A, P, k);
+        __syncthreads();
+        if (live && d.mode == RSG_MODE_SOCCAR)
+            for (int p = k; p < PAD_NUM; p += lanes) pad_check_cars(A, p);
+        __syncthreads();
+        if (live && k == 0) tick_end(A, P);
         __syncthreads();
     }
     if (live && k == 0) store_arena(d, a, A);
@@ -381,7 +412,7 @@ int lds_max_per_block(const rsg_batch* b) {
 // Cooperative kernel: 4 lanes per car slot, arena + ray results + flag in LDS.
 int coop_lanes(const rsg_batch* b) { return 4 * b->dev.max_cars; }
 int coop_stride(const rsg_batch* b) {
-    size_t bytes = (size_t)lds_stride(b) + (size_t)coop_lanes(b) * sizeof(RayHit) + 16;
+    size_t bytes = (size_t)lds_stride(b) + (size_t)coop_lanes(b) * sizeof(RayHit) + 2 * sizeof(int);
     return (int)((bytes + 15) & ~(size_t)15);
 }
 int coop_max_per_block(const rsg_batch* b) {
@@ -444,17 +475,13 @@ int rsg_batch_set_kernel(rsg_batch* b, int kernel) {
 
 int rsg_batch_get_kernel(rsg_batch* b) { return b ? choose_launch(b).kernel : fail("null batch"); }
 
-int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
-    if (!b) return fail("null batch");
-    if (ticks < 0) return fail("rsg_batch_step: ticks must be >= 0");
-    if (ticks == 0) return 0;
-    if (upload(b) != 0) return -1;
-    hipStream_t s = (hipStream_t)stream;
-    Launch L = choose_launch(b);
+namespace {
+// Enqueues `ticks` ticks of the batch state `dev` with launch L.
+int launch_step(const rsg_batch* b, const BatchDev& dev, const Launch& L, int ticks, hipStream_t s) {
     int block = L.per_block;
-    int grid = (b->dev.n + block - 1) / block;
+    int grid = (dev.n + block - 1) / block;
     MeshView mv{};
-    if (b->dev.mode == RSG_MODE_SOCCAR) {
+    if (dev.mode == RSG_MODE_SOCCAR) {
         if (!g_mesh.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
         mv = g_mesh.view;
     }
@@ -462,18 +489,103 @@ int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
         int lanes = coop_lanes(b);
         int stride = coop_stride(b);
         size_t lds = (size_t)block * (size_t)stride;
-        hipLaunchKernelGGL(rsg_step_arena_coop_kernel, dim3(grid), dim3(block * lanes), lds, s, b->dev, mv, ticks,
+        hipLaunchKernelGGL(rsg_step_arena_coop_kernel, dim3(grid), dim3(block * lanes), lds, s, dev, mv, ticks,
                            stride, lds_stride(b), lanes);
     } else if (L.kernel == RSG_KERNEL_LDS) {
         int stride = lds_stride(b);
         size_t lds = (size_t)block * (size_t)stride;
-        hipLaunchKernelGGL(rsg_step_arena_lds_kernel, dim3(grid), dim3(block), lds, s, b->dev, mv, ticks, stride);
+        hipLaunchKernelGGL(rsg_step_arena_lds_kernel, dim3(grid), dim3(block), lds, s, dev, mv, ticks, stride);
     } else {
-        hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(block), 0, s, b->dev, mv, ticks);
+        hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(block), 0, s, dev, mv, ticks);
     }
     HIP_OK(hipGetLastError());
+    return 0;
+}
+}  // namespace
+
+int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
+    if (!b) return fail("null batch");
+    if (ticks < 0) return fail("rsg_batch_step: ticks must be >= 0");
+    if (ticks == 0) return 0;
+    if (upload(b) != 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (launch_step(b, b->dev, choose_launch(b), ticks, s) != 0) return -1;
     b->last_stream = s;
     b->dev_newer = true;
+    return 0;
+}
+
+// Autotuner: every feasible (kernel, arenas per workgroup) is timed on a
+// device copy of the batch state for `ticks` ticks after one warm-up launch,
+// and the fastest is kept.  The batch's own state is not advanced.
+int rsg_batch_autotune(rsg_batch* b, int ticks, void* stream, int* out_kernel, int* out_per_block, float* out_ms) {
+    if (!b) return fail("null batch");
+    if (ticks <= 0) return fail("rsg_batch_autotune: ticks must be > 0");
+    if (upload(b) != 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    BatchDev tmp = b->dev;
+    tmp.car = tmp.ball = tmp.pad = tmp.arena = nullptr;
+    size_t cw = car_words(b) * 4, bw = ball_words(b) * 4, pw = pad_words(b) * 4, aw = arena_words(b) * 4;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (tmp.car) (void)hipFree(tmp.car);
+        if (tmp.ball) (void)hipFree(tmp.ball);
+        if (tmp.pad) (void)hipFree(tmp.pad);
+        if (tmp.arena) (void)hipFree(tmp.arena);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&tmp.car, cw ? cw : 4)) != hipSuccess || (e = hipMalloc(&tmp.ball, bw)) != hipSuccess ||
+        (e = hipMalloc(&tmp.pad, pw ? pw : 4)) != hipSuccess || (e = hipMalloc(&tmp.arena, aw)) != hipSuccess ||
+        (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+        cleanup();
+        return fail(std::string("rsg_batch_autotune: ") + hipGetErrorString(e));
+    }
+    struct Cand { Launch L; float ms; };
+    std::vector<Launch> cands;
+    const int shapes[] = {1, 2, 4, 8, 16, 32, 64};
+    for (int kernel : {RSG_KERNEL_COOP, RSG_KERNEL_LDS, RSG_KERNEL_SCRATCH}) {
+        int maxpb = kernel == RSG_KERNEL_COOP ? coop_max_per_block(b)
+                  : kernel == RSG_KERNEL_LDS ? lds_max_per_block(b) : 64;
+        for (int pb : shapes) {
+            if (pb > maxpb) continue;
+            if ((b->dev.n + pb - 1) / pb < 1) continue;
+            Launch L;
+            L.kernel = kernel;
+            L.per_block = pb;
+            cands.push_back(L);
+        }
+    }
+    Launch best = choose_launch(b);
+    float best_ms = -1.f;
+    for (const Launch& L : cands) {
+        // fresh copy of the state for every candidate
+        if ((e = hipMemcpyAsync(tmp.car, b->dev.car, cw, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(tmp.ball, b->dev.ball, bw, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(tmp.pad, b->dev.pad, pw, hipMemcpyDeviceToDevice, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(tmp.arena, b->dev.arena, aw, hipMemcpyDeviceToDevice, s)) != hipSuccess) {
+            cleanup();
+            return fail(std::string("rsg_batch_autotune: ") + hipGetErrorString(e));
+        }
+        if (launch_step(b, tmp, L, 1, s) != 0) { cleanup(); return -1; }  // warm-up
+        (void)hipEventRecord(e0, s);
+        if (launch_step(b, tmp, L, ticks, s) != 0) { cleanup(); return -1; }
+        (void)hipEventRecord(e1, s);
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) {
+            cleanup();
+            return fail(std::string("rsg_batch_autotune: ") + hipGetErrorString(e));
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (best_ms < 0.f || ms < best_ms) { best_ms = ms; best = L; }
+    }
+    cleanup();
+    b->kernel = best.kernel;
+    b->arenas_per_block = best.per_block;
+    if (out_kernel) *out_kernel = best.kernel;
+    if (out_per_block) *out_per_block = best.per_block;
+    if (out_ms) *out_ms = best_ms / (float)ticks;
     return 0;
 }
 

@@ -356,26 +356,6 @@ RSG_FN void pad_check_collide(Pad& pad, int p, const Car& c, int car_idx, const 
     if (colliding) pad.cur_locked = car_idx;
 }
 
-RSG_FN void pads_check_car(ArenaLocal& A, int ci) {
-    const Car& c = A.cars[ci];
-    if (flag(c.flags, RSG_CF_DEMOED) || c.boost >= BOOST_MAX) return;
-    V3 cp_uu = c.b.pos * BT_TO_UU;
-    if (cp_uu.z > PAD_GRID_EXTENT_Z) return;
-    int ix = (int)(cp_uu.x / PAD_GRID_CELL_SIZE_X + (PAD_GRID_CELLS_X / 2));
-    int iy = (int)(cp_uu.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2));
-    CarPhys cphys = make_car_phys(c.preset);
-    int x0 = ix - 1 > 0 ? ix - 1 : 0, x1 = ix + 1 < PAD_GRID_CELLS_X - 1 ? ix + 1 : PAD_GRID_CELLS_X - 1;
-    int y0 = iy - 1 > 0 ? iy - 1 : 0, y1 = iy + 1 < PAD_GRID_CELLS_Y - 1 ? iy + 1 : PAD_GRID_CELLS_Y - 1;
-    // Each grid cell holds at most one pad and a pad's test touches only its
-    // own state, so visiting the pads whose cell lies in the 3x3 window in
-    // pad order is the same as visiting the window's cells.
-    for (int p = 0; p < PAD_NUM; p++) {
-        V3 pl = pad_location(p);
-        int px = pad_cell_x(pl), py = pad_cell_y(pl);
-        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A.pads[p], p, c, ci, cphys);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // The tick
 
@@ -425,121 +405,144 @@ RSG_FN void tick_cast_ray(ArenaLocal& A, const TickParams& P, const MeshView* me
     out = world.raycast(source, target, 1 + ci);
 }
 
-// The rest of Arena::Step after tick_begin.  `pre` holds every wheel's ray
+// Car pre-tick (Car.cpp:58-131) of car `ci`.  `pre` holds every wheel's ray
 // result (car-major, 4 per car) when cast ahead, else null.
-RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, const RayHit* pre) {
+RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int ci, const RayHit* pre) {
+    const float dt = P.dt;
+    Car& c = A.cars[ci];
+    CarPhys cp = make_car_phys(c.preset);
+    c.ctrl.throttle = clampf(c.ctrl.throttle, -1, 1);
+    c.ctrl.steer = clampf(c.ctrl.steer, -1, 1);
+    c.ctrl.pitch = clampf(c.ctrl.pitch, -1, 1);
+    c.ctrl.yaw = clampf(c.ctrl.yaw, -1, 1);
+    c.ctrl.roll = clampf(c.ctrl.roll, -1, 1);
+
+    if (flag(c.flags, RSG_CF_DEMOED)) {
+        c.demo_timer = rs_max(c.demo_timer - dt, 0.f);
+        if (c.demo_timer == 0) {
+            int slot = (int)(rsg_draw(P.seed ^ 0xDE40ull, P.arena_id, (uint32_t)ci, (uint32_t)A.tick, 9) % 4u);
+            car_respawn_state(c, slot, BOOST_SPAWN_AMOUNT);
+        }
+        c.b.active = false;  // DISABLE_SIMULATION + no contact response
+    } else {
+        c.b.active = true;
+    }
+    setflag(c.flags, RSG_CF_BODY_DISABLED, !c.b.active);
+    if (flag(c.flags, RSG_CF_DEMOED)) return;
+
+    // updateVehicleFirst (btVehicleRL.cpp:215-228)
+    for (int i = 0; i < 4; i++) wheel_update_transform(c, cp, i, c.w[i]);
+    RSG_PROF_MARK(1);
+    for (int i = 0; i < 4; i++) {
+        vehicle_ray_cast(c, cp, i, world, 1 + ci, A.si, pre ? &pre[ci * 4 + i] : nullptr);
+        RSG_PROF_MARK(6);
+    }
+    vehicle_calc_friction(c, world, dt);
+
+    bool jump_pressed = c.ctrl.jump && !c.last.jump;
+    int n_contact = 0;
+    for (int i = 0; i < 4; i++) {
+        setflag(c.flags, RSG_CF_WHEEL0 << i, c.w[i].in_contact);
+        n_contact += c.w[i].in_contact ? 1 : 0;
+    }
+    setflag(c.flags, RSG_CF_ON_GROUND, n_contact >= 3);
+    float fwd_speed_uu = dot(c.b.vel, car_forward(c)) * BT_TO_UU;
+    car_update_wheels(c, cp, dt, n_contact, fwd_speed_uu);
+    if (n_contact < 3)
+        car_update_air_torque(c, dt, n_contact == 0);
+    else
+        setflag(c.flags, RSG_CF_IS_FLIPPING, false);
+    car_update_jump(c, dt, jump_pressed);
+    car_update_auto_flip(c, dt, jump_pressed);
+    car_update_double_jump_or_flip(c, dt, jump_pressed, fwd_speed_uu);
+    if (c.ctrl.throttle != 0.f && ((n_contact > 0 && n_contact < 4) || flag(c.flags, RSG_CF_WORLD_CONTACT)))
+        car_update_auto_roll(c, n_contact);
+    setflag(c.flags, RSG_CF_WORLD_CONTACT, false);
+    vehicle_second(c, cp, dt);
+    car_update_boost(c, dt);
+}
+
+// Whether the car pre-ticks of this tick are independent of their order, so
+// that they may run concurrently: the rays were cast ahead and no wheel rests
+// on a car (tire friction reads the velocity of a car ground, which that
+// car's own pre-tick changes).
+RSG_FN bool car_pre_ticks_independent(const ArenaLocal& A, const RayHit* pre) {
+    if (!pre) return false;
+    for (int k = 0; k < 4 * A.n_cars; k++)
+        if (pre[k].hit && !pre[k].is_static && pre[k].body >= 1) return false;
+    return true;
+}
+
+// Pads pre-tick (BoostPad.cpp:51-60).
+RSG_FN void pads_pre_tick(ArenaLocal& A, const TickParams& P) {
+    if (P.mode != RSG_MODE_SOCCAR || A.n_cars == 0) return;
+    for (int p = 0; p < PAD_NUM; p++) {
+        Pad& pad = A.pads[p];
+        if (pad.cooldown > 0) pad.cooldown = rs_max(pad.cooldown - P.dt, 0.f);
+        pad.active = (pad.cooldown == 0);
+        pad.cur_locked = -1;
+    }
+}
+
+// Car post-tick (Car.cpp:133-193) and _FinishPhysicsTick of car `ci`.
+RSG_FN void car_post_tick(ArenaLocal& A, const TickParams& P, int ci) {
+    const float dt = P.dt;
+    Car& c = A.cars[ci];
+    if (flag(c.flags, RSG_CF_DEMOED)) return;
+    c.api_rot = c.b.rot;
+    float s2 = len2(c.b.vel * BT_TO_UU);
+    bool ss = flag(c.flags, RSG_CF_SUPERSONIC);
+    if (ss && c.supersonic_time < SUPERSONIC_MAINTAIN_MAX_TIME)
+        ss = s2 >= SUPERSONIC_MAINTAIN_MIN_SPEED * SUPERSONIC_MAINTAIN_MIN_SPEED;
+    else
+        ss = s2 >= SUPERSONIC_START_SPEED * SUPERSONIC_START_SPEED;
+    setflag(c.flags, RSG_CF_SUPERSONIC, ss);
+    if (ss) c.supersonic_time += dt; else c.supersonic_time = 0;
+    if (c.car_cooldown > 0) c.car_cooldown = rs_max(c.car_cooldown - dt, 0.f);
+    c.last = c.ctrl;
+
+    // _FinishPhysicsTick
+    if (!is_zero(c.vel_cache)) {
+        c.b.vel += c.vel_cache;
+        c.vel_cache = v3zero();
+    }
+    V3 v = c.b.vel, w = c.b.ang;
+    if (len2(v) > (CAR_MAX_SPEED * UU_TO_BT) * (CAR_MAX_SPEED * UU_TO_BT))
+        v = normalized(v) * (CAR_MAX_SPEED * UU_TO_BT);
+    if (len2(w) > (CAR_MAX_ANG_SPEED * CAR_MAX_ANG_SPEED))
+        w = normalized(w) * CAR_MAX_ANG_SPEED;
+    c.b.vel = v;
+    c.b.ang = w;
+    c.ticks_since_set++;
+}
+
+// Boost pad p against every car, in the car update order (the reference
+// checks each car against the pads of its 3x3 grid window right after the
+// car's post-tick, Arena.cpp:720-734 and BoostPadGrid.cpp:5-25; a pad's
+// test reads only that car's pose, boost and flags, which no later post-tick
+// changes, and the last colliding car locks the pad).
+RSG_FN void pad_check_cars(ArenaLocal& A, int p) {
+    V3 pl = pad_location(p);
+    int px = pad_cell_x(pl), py = pad_cell_y(pl);
+    for (int oi = 0; oi < A.n_cars; oi++) {
+        const int ci = A.order[oi];
+        const Car& c = A.cars[ci];
+        if (flag(c.flags, RSG_CF_DEMOED) || c.boost >= BOOST_MAX) continue;
+        V3 cp_uu = c.b.pos * BT_TO_UU;
+        if (cp_uu.z > PAD_GRID_EXTENT_Z) continue;
+        int ix = (int)(cp_uu.x / PAD_GRID_CELL_SIZE_X + (PAD_GRID_CELLS_X / 2));
+        int iy = (int)(cp_uu.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2));
+        int x0 = ix - 1 > 0 ? ix - 1 : 0, x1 = ix + 1 < PAD_GRID_CELLS_X - 1 ? ix + 1 : PAD_GRID_CELLS_X - 1;
+        int y0 = iy - 1 > 0 ? iy - 1 : 0, y1 = iy + 1 < PAD_GRID_CELLS_Y - 1 ? iy + 1 : PAD_GRID_CELLS_Y - 1;
+        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A.pads[p], p, c, ci, make_car_phys(c.preset));
+    }
+}
+
+// End of Arena::Step: pads post-tick, ball finish, goal test, tick count,
+// kickoff reset on goal.
+RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
     const float dt = P.dt;
     const bool soccar = P.mode == RSG_MODE_SOCCAR;
-
-    World world(A, mesh, soccar && mesh != nullptr);
-
-    RSG_PROF_MARK(0);
-    // 2. car pre-tick (Car.cpp:58-131), in the arena's car update order
-    for (int oi = 0; oi < A.n_cars; oi++) {
-        const int ci = A.order[oi];
-        Car& c = A.cars[ci];
-        CarPhys cp = make_car_phys(c.preset);
-        c.ctrl.throttle = clampf(c.ctrl.throttle, -1, 1);
-        c.ctrl.steer = clampf(c.ctrl.steer, -1, 1);
-        c.ctrl.pitch = clampf(c.ctrl.pitch, -1, 1);
-        c.ctrl.yaw = clampf(c.ctrl.yaw, -1, 1);
-        c.ctrl.roll = clampf(c.ctrl.roll, -1, 1);
-
-        if (flag(c.flags, RSG_CF_DEMOED)) {
-            c.demo_timer = rs_max(c.demo_timer - dt, 0.f);
-            if (c.demo_timer == 0) {
-                int slot = (int)(rsg_draw(P.seed ^ 0xDE40ull, P.arena_id, (uint32_t)ci, (uint32_t)A.tick, 9) % 4u);
-                car_respawn_state(c, slot, BOOST_SPAWN_AMOUNT);
-            }
-            c.b.active = false;  // DISABLE_SIMULATION + no contact response
-        } else {
-            c.b.active = true;
-        }
-        setflag(c.flags, RSG_CF_BODY_DISABLED, !c.b.active);
-        if (flag(c.flags, RSG_CF_DEMOED)) continue;
-
-        // updateVehicleFirst (btVehicleRL.cpp:215-228)
-        for (int i = 0; i < 4; i++) wheel_update_transform(c, cp, i, c.w[i]);
-        RSG_PROF_MARK(1);
-        for (int i = 0; i < 4; i++) {
-            vehicle_ray_cast(c, cp, i, world, 1 + ci, A.si, pre ? &pre[ci * 4 + i] : nullptr);
-            RSG_PROF_MARK(6);
-        }
-        vehicle_calc_friction(c, world, dt);
-
-        bool jump_pressed = c.ctrl.jump && !c.last.jump;
-        int n_contact = 0;
-        for (int i = 0; i < 4; i++) {
-            setflag(c.flags, RSG_CF_WHEEL0 << i, c.w[i].in_contact);
-            n_contact += c.w[i].in_contact ? 1 : 0;
-        }
-        setflag(c.flags, RSG_CF_ON_GROUND, n_contact >= 3);
-        float fwd_speed_uu = dot(c.b.vel, car_forward(c)) * BT_TO_UU;
-        car_update_wheels(c, cp, dt, n_contact, fwd_speed_uu);
-        if (n_contact < 3)
-            car_update_air_torque(c, dt, n_contact == 0);
-        else
-            setflag(c.flags, RSG_CF_IS_FLIPPING, false);
-        car_update_jump(c, dt, jump_pressed);
-        car_update_auto_flip(c, dt, jump_pressed);
-        car_update_double_jump_or_flip(c, dt, jump_pressed, fwd_speed_uu);
-        if (c.ctrl.throttle != 0.f && ((n_contact > 0 && n_contact < 4) || flag(c.flags, RSG_CF_WORLD_CONTACT)))
-            car_update_auto_roll(c, n_contact);
-        setflag(c.flags, RSG_CF_WORLD_CONTACT, false);
-        vehicle_second(c, cp, dt);
-        car_update_boost(c, dt);
-    }
-
-    RSG_PROF_MARK(1);
-    // 3. pads pre-tick (BoostPad.cpp:51-60)
-    if (soccar && A.n_cars > 0) {
-        for (int p = 0; p < PAD_NUM; p++) {
-            Pad& pad = A.pads[p];
-            if (pad.cooldown > 0) pad.cooldown = rs_max(pad.cooldown - dt, 0.f);
-            pad.active = (pad.cooldown == 0);
-            pad.cur_locked = -1;
-        }
-    }
-
-    // 4. world step (btDiscreteDynamicsWorld::stepSimulation)
-    world.step(dt, P);
-    RSG_PROF_MARK(7);
-
-    // 5. cars post-tick (Car.cpp:133-193) + pad collisions, per car in the
-    //    car update order (Arena.cpp:720-734)
-    for (int oi = 0; oi < A.n_cars; oi++) {
-        const int ci = A.order[oi];
-        Car& c = A.cars[ci];
-        if (!flag(c.flags, RSG_CF_DEMOED)) {
-            c.api_rot = c.b.rot;
-            float s2 = len2(c.b.vel * BT_TO_UU);
-            bool ss = flag(c.flags, RSG_CF_SUPERSONIC);
-            if (ss && c.supersonic_time < SUPERSONIC_MAINTAIN_MAX_TIME)
-                ss = s2 >= SUPERSONIC_MAINTAIN_MIN_SPEED * SUPERSONIC_MAINTAIN_MIN_SPEED;
-            else
-                ss = s2 >= SUPERSONIC_START_SPEED * SUPERSONIC_START_SPEED;
-            setflag(c.flags, RSG_CF_SUPERSONIC, ss);
-            if (ss) c.supersonic_time += dt; else c.supersonic_time = 0;
-            if (c.car_cooldown > 0) c.car_cooldown = rs_max(c.car_cooldown - dt, 0.f);
-            c.last = c.ctrl;
-
-            // _FinishPhysicsTick
-            if (!is_zero(c.vel_cache)) {
-                c.b.vel += c.vel_cache;
-                c.vel_cache = v3zero();
-            }
-            V3 v = c.b.vel, w = c.b.ang;
-            if (len2(v) > (CAR_MAX_SPEED * UU_TO_BT) * (CAR_MAX_SPEED * UU_TO_BT))
-                v = normalized(v) * (CAR_MAX_SPEED * UU_TO_BT);
-            if (len2(w) > (CAR_MAX_ANG_SPEED * CAR_MAX_ANG_SPEED))
-                w = normalized(w) * CAR_MAX_ANG_SPEED;
-            c.b.vel = v;
-            c.b.ang = w;
-            c.ticks_since_set++;
-        }
-        if (soccar) pads_check_car(A, ci);
-    }
-
     // 6. pads post-tick (BoostPad.cpp:88-107)
     if (soccar && A.n_cars > 0) {
         for (int p = 0; p < PAD_NUM; p++) {
@@ -590,6 +593,34 @@ RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, 
     A.tick++;
     if (scored && P.auto_reset) arena_kickoff_reset(A, P.seed, P.arena_id);
     RSG_PROF_MARK(8);
+}
+
+// The world step of the tick (btDiscreteDynamicsWorld::stepSimulation).
+RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const MeshView* mesh) {
+    World world(A, mesh, P.mode == RSG_MODE_SOCCAR && mesh != nullptr);
+    world.step(P.dt, P);
+    RSG_PROF_MARK(7);
+}
+
+// The rest of Arena::Step after tick_begin, serially.  `pre` holds every
+// wheel's ray result (car-major, 4 per car) when cast ahead, else null.
+RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, const RayHit* pre) {
+    {
+        World world(A, mesh, P.mode == RSG_MODE_SOCCAR && mesh != nullptr);
+        RSG_PROF_MARK(0);
+        // 2. car pre-tick, in the arena's car update order
+        for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], pre);
+    }
+    RSG_PROF_MARK(1);
+    // 3. pads pre-tick
+    pads_pre_tick(A, P);
+    // 4. world step
+    tick_world(A, P, mesh);
+    // 5. cars post-tick + pad collisions
+    for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
+    if (P.mode == RSG_MODE_SOCCAR)
+        for (int p = 0; p < PAD_NUM; p++) pad_check_cars(A, p);
+    tick_end(A, P);
 }
 
 // Arena::Step (src/Sim/Arena/Arena.cpp:689-756), one tick.

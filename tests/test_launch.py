@@ -26,3 +26,28 @@ def test_kernel_and_launch_bit_identical(cars):
             b.close()
     for o in outs[1:]:
         assert o == outs[0]
+
+
+@pytest.mark.gpu
+def test_autotune_leaves_state_and_results_unchanged():
+    """The autotuner times candidates on a copy of the state: the batch is not
+    advanced, and stepping afterwards gives the same result as without it."""
+    from rocketsim_amd.batch import ArenaBatch
+    outs = []
+    for tune in (False, True):
+        b = ArenaBatch.uniform(500, 2, 2, 0, mode=MODE_SOCCAR, seed=5, synth_controls=True, auto_reset=True)
+        try:
+            b.step(30)
+            before = b"".join(bytes(c) for c in b.get_cars())
+            if tune:
+                kernel, per_block, ms = b.autotune(4)
+                assert kernel in (KERNEL_SCRATCH, KERNEL_LDS, KERNEL_COOP)
+                assert 1 <= per_block <= 64 and ms > 0
+                assert b.get_kernel() == kernel and b.get_launch() == per_block
+                assert b"".join(bytes(c) for c in b.get_cars()) == before
+                assert b.get_tick_counts() == [30] * 500
+            b.step(60)
+            outs.append((b"".join(bytes(c) for c in b.get_cars()), b"".join(bytes(x) for x in b.get_balls())))
+        finally:
+            b.close()
+    assert outs[0] == outs[1]
