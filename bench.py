@@ -9,9 +9,16 @@ timed region.
 Prints one JSON line: whole-job arena-ticks/s, the roofline of the dominant
 kernel (HBM-bound: algorithmic bytes per arena-tick = full state read+write +
 controls) and the reference CPU baseline (RocketSim compiled from source, one
-Arena per thread) timed on this host on a bounded sample.
+Arena per thread, 2v2) timed on this host on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+The batched path is the default: the kernel and launch shape come from the
+autotuner, ticks run in 8-tick launches captured in one hipGraph.
+--compare-plain also times the plain one-arena-per-thread kernel (private
+scratch, 64 arenas per workgroup, one tick per launch) on the same workload.
+--workload mixed is the uneven batch: 32768 arenas, 40/35/25 % 1v1/2v2/3v3,
+six hitbox presets, random boost.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload soccar2v2|mixed] [--compare-plain]
 """
 import argparse
 import json
@@ -56,19 +63,86 @@ def cpu_baseline(target_seconds=10.0):
     }
 
 
+def make_batch(workload, n_arenas, seed):
+    """The batch of a workload, its algorithmic bytes per arena-tick and a
+    description.  soccar2v2: n 2v2 Octane arenas.  mixed: n arenas, 40 % 1v1 /
+    35 % 2v2 / 25 % 3v3, presets uniform over the six hitboxes, boost
+    U[0, 100] (seeded)."""
+    import random
+    from rocketsim_amd.batch import ArenaBatch
+    from rocketsim_amd.state import MODE_SOCCAR
+    if workload == "soccar2v2":
+        b = ArenaBatch.uniform(n_arenas, 2, 2, 0, mode=MODE_SOCCAR, seed=seed, synth_controls=True, auto_reset=True)
+        return b, float(BYTES_PER_ARENA_TICK_2V2), \
+            f"soccar 2v2 x {n_arenas} arenas, Octane, seeded controls, auto-reset on goal", 4.0
+    rng = random.Random(seed)
+    counts, teams, presets = [], [], []
+    for _ in range(n_arenas):
+        u = rng.random()
+        per_team = 1 if u < 0.40 else (2 if u < 0.75 else 3)
+        counts.append(2 * per_team)
+        teams += [0] * per_team + [1] * per_team
+        presets += [rng.randrange(6) for _ in range(2 * per_team)]
+    b = ArenaBatch(counts, teams, presets, mode=MODE_SOCCAR, seed=seed, synth_controls=True, auto_reset=True)
+    cars = b.get_cars()
+    for c in cars:
+        c.boost = rng.uniform(0.0, 100.0)
+    b.set_cars(list(cars))
+    # per arena-tick: 2 x (cars x 220 + 316) + cars x 24 (see BYTES_PER_ARENA_TICK_2V2)
+    bytes_avg = sum(2 * (c * 220 + 316) + c * 24 for c in counts) / n_arenas
+    return b, bytes_avg, (f"soccar mixed x {n_arenas} arenas (40/35/25 % 1v1/2v2/3v3), six presets, "
+                          f"boost U[0,100], seeded controls, auto-reset on goal"), sum(counts) / n_arenas
+
+
+def run_timed(batch, steps, warmup, chunk, use_graph, stream):
+    """Warm up, then time exactly `steps` ticks (one graph replay, or launches
+    of `chunk` ticks) between two synchronisations.  Returns (wall s, event ms)."""
+    import torch
+    sp = stream.cuda_stream
+    chunk = max(1, min(chunk, steps))
+    launches = [chunk] * (steps // chunk) + ([steps % chunk] if steps % chunk else [])
+    for _ in range(max(1, warmup // chunk)):
+        batch.step(chunk, sp)
+    graph = use_graph and steps % chunk == 0
+    if graph:
+        batch.graph_create(chunk, steps // chunk)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    if graph:
+        batch.graph_launch(sp)
+    else:
+        for n in launches:
+            batch.step(n, sp)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, start.elapsed_time(end), chunk, graph
+
+
+KERNEL_NAMES = {1: "rsg_step_arena_kernel", 2: "rsg_step_arena_lds_kernel", 3: "rsg_step_arena_coop_kernel"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=512)
     ap.add_argument("--warmup", type=int, default=16)
-    ap.add_argument("--arenas", type=int, default=4096)
+    ap.add_argument("--workload", choices=["soccar2v2", "mixed"], default="soccar2v2")
+    ap.add_argument("--arenas", type=int, default=None, help="default 4096 (soccar2v2) or 32768 (mixed)")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--arenas-per-block", type=int, default=0,
-                    help="arenas per workgroup of the step kernel (0 = spread over every SIMD)")
+                    help="arenas per workgroup of the step kernel (0 = autotune, or the kernel's default)")
     ap.add_argument("--kernel", choices=["auto", "scratch", "lds", "coop"], default="auto",
-                    help="where the step kernel keeps each arena's working copy")
+                    help="where the step kernel keeps each arena's working copy (auto = autotune)")
     ap.add_argument("--chunk", type=int, default=8,
                     help="ticks per kernel launch (the arena state stays on chip across a chunk)")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of one captured hipGraph")
+    ap.add_argument("--compare-plain", action="store_true",
+                    help="also time the plain one-arena-per-thread kernel (scratch, 64 arenas per workgroup, "
+                         "one tick per launch) on the same workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-probe", action="store_true",
                     help="for rocprofv3 --pmc passes: fill and copy a 64 MiB buffer (calibrates the HBM byte "
@@ -80,16 +154,14 @@ def main():
               "so N GPUs would each step their own shard of the arena range with no per-tick exchange.",
               file=sys.stderr)
         sys.exit(2)
+    n_arenas = args.arenas or (4096 if args.workload == "soccar2v2" else 32768)
 
     import torch
     from rocketsim_amd import arena_mesh, native
-    from rocketsim_amd.batch import ArenaBatch
-    from rocketsim_amd.state import MODE_SOCCAR
 
     native.lib()
     native.check(native.lib().rsg_init_mesh(arena_mesh.soccar_cmf(), len(arena_mesh.soccar_cmf())))
-    batch = ArenaBatch.uniform(args.arenas, 2, 2, 0, mode=MODE_SOCCAR, seed=args.seed,
-                               synth_controls=True, auto_reset=True)
+    batch, bytes_per_at, desc, cars_per_arena = make_batch(args.workload, n_arenas, args.seed)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     tuned = None
@@ -101,8 +173,7 @@ def main():
         batch.set_kernel({"auto": 0, "scratch": 1, "lds": 2, "coop": 3}[args.kernel])
         batch.set_launch(args.arenas_per_block)
     per_block = batch.get_launch()
-    kernel_name = {1: "rsg_step_arena_kernel", 2: "rsg_step_arena_lds_kernel",
-                   3: "rsg_step_arena_coop_kernel"}[batch.get_kernel()]
+    kernel_name = KERNEL_NAMES[batch.get_kernel()]
 
     if args.profile_probe:
         src = torch.empty(64 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -117,30 +188,24 @@ def main():
         print("profile probe done", flush=True)
         return
 
-    chunk = max(1, min(args.chunk, args.steps))
-    # K steps = K // chunk launches of `chunk` ticks plus one launch of the rest
-    launches = [chunk] * (args.steps // chunk) + ([args.steps % chunk] if args.steps % chunk else [])
-    for _ in range(max(1, args.warmup // chunk)):
-        batch.step(chunk, sp)
-    torch.cuda.synchronize()
-
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    start.record(stream)
-    for n in launches:
-        batch.step(n, sp)
-    end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = start.elapsed_time(end)
-
-    arena_ticks = args.arenas * args.steps
+    wall, ev_ms, chunk, graph = run_timed(batch, args.steps, args.warmup, args.chunk, not args.no_graph, stream)
+    arena_ticks = n_arenas * args.steps
     value = arena_ticks / wall
     kernel_ms = ev_ms / args.steps  # per tick (one launch covers `chunk` ticks)
-    achieved = (args.arenas * BYTES_PER_ARENA_TICK_2V2) / (kernel_ms * 1e-3) / 1e9
+    achieved = (n_arenas * bytes_per_at) / (kernel_ms * 1e-3) / 1e9
     goals = batch.get_goals()
     batch.close()
+
+    plain = None
+    if args.compare_plain:
+        pb_batch, _, _, _ = make_batch(args.workload, n_arenas, args.seed)
+        pb_batch.set_kernel(1)
+        pb_batch.set_launch(64)
+        p_wall, p_ev, _, _ = run_timed(pb_batch, args.steps, args.warmup, 1, False, stream)
+        pb_batch.close()
+        plain = {"kernel": "rsg_step_arena_kernel", "arenas_per_block": 64, "ticks_per_launch": 1,
+                 "value": arena_ticks / p_wall, "ms_per_tick": p_ev / args.steps,
+                 "speedup": (arena_ticks / wall) / (arena_ticks / p_wall)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
@@ -157,19 +222,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"soccar 2v2 x {args.arenas} arenas, Octane, seeded controls, auto-reset on goal",
-                   "arenas": args.arenas, "cars_per_arena": 4, "tick_rate": 120,
+        "config": {"workload": desc, "arenas": n_arenas, "cars_per_arena": cars_per_arena, "tick_rate": 120,
                    "arenas_per_block": per_block, "ticks_per_launch": chunk, "kernel": kernel_name,
-                   "autotuned": tuned is not None,
+                   "autotuned": tuned is not None, "hip_graph": graph,
                    "goals_during_run": int(sum(goals[0]) + sum(goals[1]))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": HBM_TRAFFIC_PER_LAUNCH.get((kernel_name, args.arenas, chunk, per_block)),
+                     "traffic": HBM_TRAFFIC_PER_LAUNCH.get((kernel_name, n_arenas, chunk, per_block))
+                     if args.workload == "soccar2v2" else None,
                      "traffic_unit": "B per launch",
                      "kernel": kernel_name, "kernel_ms": kernel_ms,
-                     "bytes_per_arena_tick": BYTES_PER_ARENA_TICK_2V2},
+                     "bytes_per_arena_tick": bytes_per_at},
         "cpu_baseline": cpu,
     }
+    if plain is not None:
+        line["plain_gpu"] = plain
     print(json.dumps(line), flush=True)
 
 

@@ -114,6 +114,15 @@ int rsg_batch_get_kernel(rsg_batch* b);
  * batch's kernel and launch shape, and reports it (any out pointer may be
  * NULL; out_ms is per tick).  Allocates temporary device memory. */
 int rsg_batch_autotune(rsg_batch* b, int ticks, void* stream, int* out_kernel, int* out_per_block, float* out_ms);
+/* hipGraph of multi-tick chunks (no counterpart in the reference): captures
+ * `launches` step launches of `ticks_per_launch` ticks each, with the batch's
+ * current kernel and launch shape; rsg_batch_graph_launch replays it on a
+ * stream (the same as that many rsg_batch_step calls), and
+ * rsg_batch_graph_ticks returns the ticks one replay advances.  Re-create the
+ * graph after changing the kernel or launch shape. */
+int rsg_batch_graph_create(rsg_batch* b, int ticks_per_launch, int launches);
+int rsg_batch_graph_launch(rsg_batch* b, void* stream);
+int rsg_batch_graph_ticks(rsg_batch* b);
 /* Arena::tickCount (Arena.h:61). */
 int rsg_batch_get_tick_counts(rsg_batch* b, uint64_t* out);
 int rsg_batch_set_tick_counts(rsg_batch* b, const uint64_t* in);

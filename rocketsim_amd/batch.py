@@ -111,6 +111,13 @@ class ArenaBatch:
         """Arenas per workgroup of the step kernel (0 = device default)."""
         native.check(self.L.rsg_batch_set_launch(self.h, int(arenas_per_block)))
 
+    def graph_create(self, ticks_per_launch, launches):
+        """Capture `launches` launches of `ticks_per_launch` ticks in a hipGraph."""
+        native.check(self.L.rsg_batch_graph_create(self.h, ticks_per_launch, launches))
+
+    def graph_launch(self, stream=None):
+        native.check(self.L.rsg_batch_graph_launch(self.h, stream))
+
     def autotune(self, ticks=8, stream=None):
         """Time every kernel / launch shape on a copy of the state, keep the
         fastest; returns (kernel, arenas_per_block, ms_per_tick)."""

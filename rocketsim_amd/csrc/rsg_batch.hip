@@ -205,6 +205,11 @@ struct rsg_batch {
     hipStream_t last_stream = nullptr;
     int arenas_per_block = 0;  // 0: derived from the device's SIMD count
     int kernel = RSG_KERNEL_AUTO;
+    // captured multi-launch step (rsg_batch_graph_*)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    int graph_ticks = 0;
 };
 
 namespace {
@@ -365,6 +370,9 @@ void rsg_batch_destroy(rsg_batch* b) {
     if (b->dev.ball) (void)hipFree(b->dev.ball);
     if (b->dev.pad) (void)hipFree(b->dev.pad);
     if (b->dev.arena) (void)hipFree(b->dev.arena);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     delete b;
 }
 
@@ -514,6 +522,46 @@ int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
     b->dev_newer = true;
     return 0;
 }
+
+// hipGraph of `launches` step launches of `ticks_per_launch` ticks each, with
+// the batch's current kernel and launch shape, captured from a private stream.
+int rsg_batch_graph_create(rsg_batch* b, int ticks_per_launch, int launches) {
+    if (!b) return fail("null batch");
+    if (ticks_per_launch <= 0 || launches <= 0) return fail("rsg_batch_graph_create: counts must be > 0");
+    if (upload(b) != 0) return -1;
+    if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    if (!b->capture_stream) HIP_OK(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+    // the step reads device state written by earlier work on other streams
+    if (b->last_stream) HIP_OK(hipStreamSynchronize(b->last_stream));
+    HIP_OK(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+    Launch L = choose_launch(b);
+    int rc = 0;
+    for (int i = 0; i < launches && rc == 0; i++) rc = launch_step(b, b->dev, L, ticks_per_launch, b->capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
+    if (rc != 0) { if (g) (void)hipGraphDestroy(g); return -1; }
+    if (e != hipSuccess) return fail(std::string("rsg_batch_graph_create: ") + hipGetErrorString(e));
+    b->graph = g;
+    HIP_OK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    b->graph_ticks = ticks_per_launch * launches;
+    return 0;
+}
+
+// Replays the captured graph on `stream` (advances every arena by the
+// graph's tick count).
+int rsg_batch_graph_launch(rsg_batch* b, void* stream) {
+    if (!b) return fail("null batch");
+    if (!b->graph_exec) return fail("rsg_batch_graph_launch: no graph (call rsg_batch_graph_create)");
+    if (upload(b) != 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    HIP_OK(hipGraphLaunch(b->graph_exec, s));
+    b->last_stream = s;
+    b->dev_newer = true;
+    return 0;
+}
+
+int rsg_batch_graph_ticks(rsg_batch* b) { return b ? b->graph_ticks : fail("null batch"); }
 
 // Autotuner: every feasible (kernel, arenas per workgroup) is timed on a
 // device copy of the batch state for `ticks` ticks after one warm-up launch,

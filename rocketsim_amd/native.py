@@ -46,6 +46,9 @@ def lib():
         L.rsg_batch_get_launch.argtypes = [C.c_void_p]
         L.rsg_batch_set_kernel.argtypes = [C.c_void_p, C.c_int]
         L.rsg_batch_get_kernel.argtypes = [C.c_void_p]
+        L.rsg_batch_graph_create.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.rsg_batch_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+        L.rsg_batch_graph_ticks.argtypes = [C.c_void_p]
         L.rsg_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.POINTER(C.c_float)]
         L.rsg_batch_set_car_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
@@ -75,6 +78,7 @@ EXPORTED = [
     "rsg_batch_get_pads", "rsg_batch_set_pads", "rsg_batch_set_controls", "rsg_batch_get_controls",
     "rsg_batch_set_car_order", "rsg_batch_get_car_order", "rsg_batch_set_launch", "rsg_batch_get_launch",
     "rsg_batch_set_kernel", "rsg_batch_get_kernel", "rsg_batch_autotune",
+    "rsg_batch_graph_create", "rsg_batch_graph_launch", "rsg_batch_graph_ticks",
     "rsg_batch_get_tick_counts", "rsg_batch_set_tick_counts", "rsg_batch_get_goals",
     "rsg_batch_reset_kickoff", "rsg_batch_is_ball_scored",
 ]
