@@ -51,6 +51,7 @@ struct DeviceMesh {
     QSubtree* subtrees = nullptr;
     TriInfo* tri_info = nullptr;
     uint8_t* cell_statics = nullptr;
+    uint32_t* occ = nullptr;
     MeshView view{};
 };
 DeviceMesh g_mesh;
@@ -830,7 +831,8 @@ int rsg_init_mesh(const void* cmf_data, uint64_t size) {
     const HostMesh& H = g_mesh.host;
     if (upload_array(&g_mesh.verts, H.verts) || upload_array(&g_mesh.tris, H.tris) ||
         upload_array(&g_mesh.nodes, H.nodes) || upload_array(&g_mesh.subtrees, H.subtrees) ||
-        upload_array(&g_mesh.tri_info, H.tri_info) || upload_array(&g_mesh.cell_statics, H.cell_statics))
+        upload_array(&g_mesh.tri_info, H.tri_info) || upload_array(&g_mesh.cell_statics, H.cell_statics) ||
+        upload_array(&g_mesh.occ, H.occ))
         return -1;
     MeshView v = H.view();
     v.verts = g_mesh.verts;
@@ -839,6 +841,7 @@ int rsg_init_mesh(const void* cmf_data, uint64_t size) {
     v.subtrees = g_mesh.subtrees;
     v.tri_info = g_mesh.tri_info;
     v.cell_statics = g_mesh.cell_statics;
+    v.occ = g_mesh.occ;
     g_mesh.view = v;
     g_mesh.ready = true;
     return 0;

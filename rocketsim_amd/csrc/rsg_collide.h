@@ -219,7 +219,7 @@ struct World {
                 mesh_query_ray(*mesh, from, to, cb);
                 if (cb.hit) hit_obj = -1 - s;
             } else {
-                if (ray_plane(s, from, to, frac, n)) hit_obj = -1 - s;
+                if (ray_plane(s, mesh->plane_unit, from, to, frac, n)) hit_obj = -1 - s;
             }
         }
         RSG_PROF_MARK(10);
@@ -417,7 +417,7 @@ struct World {
 
     // btConvexPlaneCollisionAlgorithm::processCollision for the ball
     RSG_FN void ball_vs_plane(int mi, int s) {
-        PlaneDef pd = plane_def(s);
+        PlaneDef pd = plane_def(s, mesh->plane_unit);
         Xf px = xf_from(pd.origin, m3_identity());
         Body& b = A.ball.b;
         Xf bx = xf_from(b.pos, b.rot);
@@ -441,7 +441,7 @@ struct World {
 
     // Child box of a car vs a plane (same algorithm, box support with margin)
     RSG_FN void car_vs_plane(int mi, int code, int s) {
-        PlaneDef pd = plane_def(s);
+        PlaneDef pd = plane_def(s, mesh->plane_unit);
         Xf px = xf_from(pd.origin, m3_identity());
         Car& c = A.cars[code - 1];
         CarPhys cp = make_car_phys(c.preset);

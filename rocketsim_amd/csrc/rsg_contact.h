@@ -31,14 +31,19 @@ struct PlaneDef {
     bool backwards;
 };
 
-RSG_FN PlaneDef plane_def(int s) {
+// `unit` is the stored length of the axis-aligned normals: the reference
+// keeps planeNormal.normalized() (btStaticPlaneShape.cpp:21), whose SSE
+// reciprocal-square-root normalisation of a unit axis is the host CPU's
+// rsqrtss(1) after one Newton step (0.99999994 on common x86 cores), measured
+// once on the host (MeshView::plane_unit).
+RSG_FN PlaneDef plane_def(int s, float unit = 1.f) {
     PlaneDef p;
     switch (s) {
         default:
-        case ST_FLOOR: p.n = v3(0, 0, 1); p.origin = v3(0, 0, 0) * UU_TO_BT; p.axis = 2; p.backwards = false; break;
-        case ST_CEIL: p.n = v3(0, 0, -1); p.origin = v3(0, 0, ARENA_HEIGHT) * UU_TO_BT; p.axis = 2; p.backwards = true; break;
-        case ST_WALL_NX: p.n = v3(1, 0, 0); p.origin = v3(-ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = false; break;
-        case ST_WALL_PX: p.n = v3(-1, 0, 0); p.origin = v3(ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = true; break;
+        case ST_FLOOR: p.n = v3(0, 0, unit); p.origin = v3(0, 0, 0) * UU_TO_BT; p.axis = 2; p.backwards = false; break;
+        case ST_CEIL: p.n = v3(0, 0, -unit); p.origin = v3(0, 0, ARENA_HEIGHT) * UU_TO_BT; p.axis = 2; p.backwards = true; break;
+        case ST_WALL_NX: p.n = v3(unit, 0, 0); p.origin = v3(-ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = false; break;
+        case ST_WALL_PX: p.n = v3(-unit, 0, 0); p.origin = v3(ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = true; break;
     }
     return p;
 }
@@ -409,8 +414,8 @@ RSG_FN void plane_space(V3 n, V3& p, V3& q) {
 
 // Ray vs static plane: the plane shape emits two triangles around the ray's
 // AABB (btStaticPlaneShape::processAllTriangles) which are ray-tested.
-RSG_FN bool ray_plane(int s, V3 from_w, V3 to_w, float& hit_fraction, V3& hit_normal) {
-    PlaneDef pd = plane_def(s);
+RSG_FN bool ray_plane(int s, float unit, V3 from_w, V3 to_w, float& hit_fraction, V3& hit_normal) {
+    PlaneDef pd = plane_def(s, unit);
     // world -> plane local (identity basis)
     V3 from = from_w - pd.origin, to = to_w - pd.origin;
     V3 amin = v3(fminf(from.x, to.x), fminf(from.y, to.y), fminf(from.z, to.z));

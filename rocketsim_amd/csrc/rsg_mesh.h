@@ -55,7 +55,48 @@ struct MeshView {
     V3 aabb_min, aabb_max;   // local AABB of the mesh (no margin)
     float tri_info_equal_vertex_threshold, tri_info_edge_distance_threshold,
           tri_info_convex_epsilon, tri_info_planar_epsilon, tri_info_max_edge_angle;
+    // Occupancy bitmask on a uniform grid over the tree bounds: bit set iff a
+    // triangle's box (grown by MESH_OCC_SLACK) touches the cell.  Lets a query
+    // whose box touches no marked cell skip the tree walk (no triangle there
+    // can yield a contact or a ray hit).  May be null.
+    const uint32_t* occ;
+    int occ_nx, occ_ny, occ_nz;
+    V3 occ_origin;
+    float occ_inv_cell;
+    // Length of the static planes' stored unit normals (see plane_def).
+    float plane_unit;
 };
+
+constexpr float MESH_OCC_CELL = 2.f;     // Bullet units
+constexpr float MESH_OCC_SLACK = 0.01f;
+
+// True when no triangle of the mesh touches the box [lo, hi]: every grid
+// cell the (slightly grown) box touches is unmarked.  Boxes touching many
+// cells, or with non-finite bounds, report false (walk the tree).
+RSG_FN bool mesh_region_empty(const MeshView& m, V3 lo, V3 hi) {
+    if (!m.occ) return false;
+    if (!(lo.x <= hi.x && lo.y <= hi.y && lo.z <= hi.z)) return false;
+    V3 a = (lo - v3(MESH_OCC_SLACK, MESH_OCC_SLACK, MESH_OCC_SLACK) - m.occ_origin) * m.occ_inv_cell;
+    V3 b = (hi + v3(MESH_OCC_SLACK, MESH_OCC_SLACK, MESH_OCC_SLACK) - m.occ_origin) * m.occ_inv_cell;
+    if (!(fabsf(a.x) < 1e6f && fabsf(a.y) < 1e6f && fabsf(a.z) < 1e6f && fabsf(b.x) < 1e6f &&
+          fabsf(b.y) < 1e6f && fabsf(b.z) < 1e6f))
+        return false;
+    int i0 = (int)floorf(a.x), j0 = (int)floorf(a.y), k0 = (int)floorf(a.z);
+    int i1 = (int)floorf(b.x), j1 = (int)floorf(b.y), k1 = (int)floorf(b.z);
+    i0 = i0 < 0 ? 0 : i0; j0 = j0 < 0 ? 0 : j0; k0 = k0 < 0 ? 0 : k0;
+    i1 = i1 >= m.occ_nx ? m.occ_nx - 1 : i1;
+    j1 = j1 >= m.occ_ny ? m.occ_ny - 1 : j1;
+    k1 = k1 >= m.occ_nz ? m.occ_nz - 1 : k1;
+    if (i0 > i1 || j0 > j1 || k0 > k1) return true;  // entirely outside the grid, which covers the mesh
+    if ((i1 - i0 + 1) * (j1 - j0 + 1) * (k1 - k0 + 1) > 64) return false;
+    for (int i = i0; i <= i1; i++)
+        for (int j = j0; j <= j1; j++)
+            for (int k = k0; k <= k1; k++) {
+                int bit = (i * m.occ_ny + j) * m.occ_nz + k;
+                if (m.occ[bit >> 5] >> (bit & 31) & 1u) return false;
+            }
+    return true;
+}
 
 RSG_FN void quantize(const MeshView& m, uint16_t out[3], V3 p, int is_max) {
     V3 v = (p - m.bvh_min) * m.quant;
@@ -102,6 +143,7 @@ RSG_FN void mesh_triangle(const MeshView& m, int t, V3 out[3]) {
 // cb(int tri) is called for every leaf whose quantized box overlaps.
 template <class CB>
 RSG_FN void mesh_query_aabb(const MeshView& m, V3 qmin_f, V3 qmax_f, CB& cb) {
+    if (mesh_region_empty(m, qmin_f, qmax_f)) return;
     uint16_t qmin[3], qmax[3];
     quantize_clamp(m, qmin, qmin_f, 0);
     quantize_clamp(m, qmax, qmax_f, 1);
@@ -131,6 +173,7 @@ RSG_FN void mesh_query_ray(const MeshView& m, V3 from, V3 to, CB& cb) {
     int sign[3] = {inv.x < 0.f, inv.y < 0.f, inv.z < 0.f};
     V3 rmin = v3(fminf(from.x, to.x), fminf(from.y, to.y), fminf(from.z, to.z));
     V3 rmax = v3(fmaxf(from.x, to.x), fmaxf(from.y, to.y), fmaxf(from.z, to.z));
+    if (mesh_region_empty(m, rmin, rmax)) return;
     uint16_t qmin[3], qmax[3];
     quantize_clamp(m, qmin, rmin, 0);
     quantize_clamp(m, qmax, rmax, 1);

@@ -9,6 +9,7 @@
 // the node order it produces is the order in which contacts are generated.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <immintrin.h>
 #include <cstring>
@@ -29,6 +30,9 @@ struct HostMesh {
     std::vector<uint8_t> cell_statics;  // BP_NUM_CELLS: bit s = static s registered
     V3 bvh_min, bvh_max, quant;
     V3 aabb_min, aabb_max;
+    std::vector<uint32_t> occ;          // MeshView::occ bitmask
+    int occ_dims[3] = {0, 0, 0};
+    float plane_unit = 1.f;             // MeshView::plane_unit
 
     MeshView view() const {
         MeshView v{};
@@ -43,6 +47,11 @@ struct HostMesh {
         v.n_subtrees = (int)subtrees.size();
         v.bvh_min = bvh_min; v.bvh_max = bvh_max; v.quant = quant;
         v.aabb_min = aabb_min; v.aabb_max = aabb_max;
+        v.occ = occ.empty() ? nullptr : occ.data();
+        v.occ_nx = occ_dims[0]; v.occ_ny = occ_dims[1]; v.occ_nz = occ_dims[2];
+        v.occ_origin = bvh_min;
+        v.occ_inv_cell = 1.f / MESH_OCC_CELL;
+        v.plane_unit = plane_unit;
         return v;
     }
 };
@@ -416,6 +425,41 @@ inline void build_cell_statics(HostMesh& M) {
     }
 }
 
+// Occupancy grid (MeshView::occ): mark every cell a triangle's box, grown by
+// MESH_OCC_SLACK, touches (same cell arithmetic as mesh_region_empty).
+inline void build_occupancy(HostMesh& M) {
+    V3 ext = M.bvh_max - M.bvh_min;
+    M.occ_dims[0] = (int)std::ceil(ext.x / MESH_OCC_CELL) + 1;
+    M.occ_dims[1] = (int)std::ceil(ext.y / MESH_OCC_CELL) + 1;
+    M.occ_dims[2] = (int)std::ceil(ext.z / MESH_OCC_CELL) + 1;
+    size_t bits = (size_t)M.occ_dims[0] * M.occ_dims[1] * M.occ_dims[2];
+    M.occ.assign((bits + 31) / 32, 0u);
+    const float inv = 1.f / MESH_OCC_CELL;
+    const int nt = (int)M.tris.size() / 3;
+    for (int t = 0; t < nt; t++) {
+        V3 lo = v3(1e30f, 1e30f, 1e30f), hi = v3(-1e30f, -1e30f, -1e30f);
+        for (int j = 0; j < 3; j++) {
+            int vi = M.tris[3 * t + j];
+            V3 p = v3(M.verts[3 * vi], M.verts[3 * vi + 1], M.verts[3 * vi + 2]);
+            lo = v3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
+            hi = v3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
+        }
+        V3 a = (lo - v3(MESH_OCC_SLACK, MESH_OCC_SLACK, MESH_OCC_SLACK) - M.bvh_min) * inv;
+        V3 b = (hi + v3(MESH_OCC_SLACK, MESH_OCC_SLACK, MESH_OCC_SLACK) - M.bvh_min) * inv;
+        int i0 = std::max(0, (int)std::floor(a.x)), j0 = std::max(0, (int)std::floor(a.y)),
+            k0 = std::max(0, (int)std::floor(a.z));
+        int i1 = std::min(M.occ_dims[0] - 1, (int)std::floor(b.x)),
+            j1 = std::min(M.occ_dims[1] - 1, (int)std::floor(b.y)),
+            k1 = std::min(M.occ_dims[2] - 1, (int)std::floor(b.z));
+        for (int i = i0; i <= i1; i++)
+            for (int j = j0; j <= j1; j++)
+                for (int k = k0; k <= k1; k++) {
+                    size_t bit = ((size_t)i * M.occ_dims[1] + j) * M.occ_dims[2] + k;
+                    M.occ[bit >> 5] |= 1u << (bit & 31);
+                }
+    }
+}
+
 // Full preparation of the arena mesh from .cmf bytes.
 inline std::string build_host_mesh(const void* data, uint64_t size, HostMesh& M) {
     std::string err = parse_cmf(data, size, M);
@@ -424,6 +468,8 @@ inline std::string build_host_mesh(const void* data, uint64_t size, HostMesh& M)
     b.build();
     build_tri_info(M);
     build_cell_statics(M);
+    build_occupancy(M);
+    M.plane_unit = host_sse_normalized(v3(0.f, 0.f, 1.f)).z;
     return "";
 }
 

@@ -33,6 +33,10 @@ def main():
     melee = parity.sample_soccar_melee(ref, n_traj=24, ticks_per_traj=100, seed=20261205, n_cars=4, stride=2)
     parity.save_samples(os.path.join(HERE, "soccar_melee_tf.npz"), melee)
     print("soccar_melee_tf.npz:", len(melee), "samples")
+    walls = parity.sample_soccar_melee(ref, n_traj=24, ticks_per_traj=100, seed=20261206, n_cars=4, stride=2,
+                                       near_walls=True)
+    parity.save_samples(os.path.join(HERE, "soccar_walls_tf.npz"), walls)
+    print("soccar_walls_tf.npz:", len(walls), "samples")
 
 
 if __name__ == "__main__":

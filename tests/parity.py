@@ -260,11 +260,12 @@ def sample_soccar_drive(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1):
     return out
 
 
-def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1):
+def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1, near_walls=False):
     """Cars placed 250-700 uu around the ball and driven at it (some
     supersonic, some jumping): car-ball hits with the extra impulse,
     car-car box-box contacts, bumps and demolitions, wheels resting on
-    other cars, respawns."""
+    other cars.  With near_walls the ball starts by a side wall, a back wall
+    or a corner, so cars drive onto the mesh's fillets and walls."""
     rng = random.Random(seed)
     out = []
     for _ in range(n_traj):
@@ -274,6 +275,10 @@ def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1):
         for t, p in zip(teams, presets):
             A.add_car(t, p)
         bx, by = rng.uniform(-3000, 3000), rng.uniform(-4000, 4000)
+        if near_walls:
+            bx, by = rng.choice([-1, 1]) * rng.uniform(3300, 3800), rng.uniform(-4500, 4500)
+            if rng.random() < 0.4:
+                bx, by = rng.uniform(-3600, 3600), rng.choice([-1, 1]) * rng.uniform(4300, 4800)
         b = A.get_ball()
         b.pos[:] = [bx, by, rng.choice([93.15, rng.uniform(100, 400)])]
         b.vel[:] = [rng.gauss(0, 300), rng.gauss(0, 300), 0]

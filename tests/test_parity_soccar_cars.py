@@ -24,7 +24,7 @@ def _oracle():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["soccar_drive_tf.npz", "soccar_melee_tf.npz"])
+@pytest.mark.parametrize("name", ["soccar_drive_tf.npz", "soccar_melee_tf.npz", "soccar_walls_tf.npz"])
 def test_soccar_cars_fixture_parity(name):
     samples = parity.load_samples(os.path.join(HERE, "golden", name))
     results = parity.run_on_gpu(samples, MODE_SOCCAR)
@@ -47,5 +47,16 @@ def test_soccar_melee_live_oracle_parity():
     if ref is None:
         pytest.skip("oracle library not built on this machine")
     samples = parity.sample_soccar_melee(ref, n_traj=40, ticks_per_traj=90, seed=37, n_cars=4, stride=1)
+    results = parity.run_on_gpu(samples, MODE_SOCCAR)
+    parity.assert_parity(samples, results, parity.CHAOTIC_FRACTION)
+
+
+@pytest.mark.gpu
+def test_soccar_walls_live_oracle_parity():
+    ref = _oracle()
+    if ref is None:
+        pytest.skip("oracle library not built on this machine")
+    samples = parity.sample_soccar_melee(ref, n_traj=30, ticks_per_traj=90, seed=41, n_cars=6, stride=1,
+                                         near_walls=True)
     results = parity.run_on_gpu(samples, MODE_SOCCAR)
     parity.assert_parity(samples, results, parity.CHAOTIC_FRACTION)
