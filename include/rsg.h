@@ -136,6 +136,16 @@ int rsg_batch_get_goals(rsg_batch* b, uint32_t* blue, uint32_t* orange, uint64_t
  * arenas when arena < 0; the kickoff order is drawn from `seed`. */
 int rsg_batch_reset_kickoff(rsg_batch* b, int arena, uint64_t seed);
 
+/* Game events (rsg_event, include/rsg_state.h) in place of the goal-score
+ * and car-bump callbacks (Arena.h:24-25,99-108).  rsg_batch_enable_events
+ * allocates a device ring of `capacity` events (0 frees it; events are off by
+ * default).  rsg_batch_poll_events waits for the batch's stream, copies up to
+ * max_out events in (arena, per-arena order) order into `out`, empties the
+ * ring and returns the number copied; *n_lost (if not NULL) receives how many
+ * were dropped because the ring or `out` was full. */
+int rsg_batch_enable_events(rsg_batch* b, int capacity);
+int rsg_batch_poll_events(rsg_batch* b, rsg_event* out, int max_out, int* n_lost);
+
 /* Arena::IsBallScored (Arena.cpp:893-901) per arena. */
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
 

@@ -125,6 +125,22 @@ typedef struct rsg_pad_state {
 
 enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8 };
 
+/* A game event, replacing the reference's callbacks: the goal-score callback
+ * (Arena::SetGoalScoreCallback, src/Sim/Arena/Arena.h:24,102; fired at the
+ * end of the tick, Arena.cpp:748-752) and the car-bump callback
+ * (Arena::SetCarBumpCallback, Arena.h:25,108; fired for bumps and
+ * demolitions during collision handling, Arena.cpp:377-400). */
+enum { RSG_EVENT_GOAL = 1, RSG_EVENT_BUMP = 2, RSG_EVENT_DEMO = 3 };
+typedef struct rsg_event {
+    uint32_t arena;
+    uint32_t type;      /* RSG_EVENT_* */
+    uint64_t tick;      /* Arena::tickCount of the tick it happened in */
+    uint32_t a;         /* GOAL: scoring team (RSG_TEAM_*); BUMP/DEMO: bumper car id */
+    uint32_t b;         /* BUMP/DEMO: victim car id; GOAL: 0 */
+    uint32_t seq;       /* per-arena sequence number (order of the callbacks) */
+    uint32_t _pad;
+} rsg_event;
+
 #ifdef __cplusplus
 }
 #endif

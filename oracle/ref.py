@@ -50,6 +50,8 @@ def lib():
         L.rsref_set_tick_count.argtypes = [C.c_void_p, C.c_uint64]
         L.rsref_step.argtypes = [C.c_void_p, C.c_int]
         L.rsref_car_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+        from rocketsim_amd.state import Event
+        L.rsref_pop_events.argtypes = [C.c_void_p, C.POINTER(Event), C.c_int]
         L.rsref_is_ball_scored.argtypes = [C.c_void_p]
         L.rsref_reset_kickoff.argtypes = [C.c_void_p, C.c_int]
         L.rsref_pad_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int)]
@@ -133,6 +135,13 @@ class RefArena:
 
     def set_controls(self, i, c):
         self.L.rsref_set_controls(self.h, i, C.byref(c))
+
+    def pop_events(self):
+        """Goal / bump / demo callbacks fired since the last call."""
+        from rocketsim_amd.state import Event
+        out = (Event * 256)()
+        n = self.L.rsref_pop_events(self.h, out, 256)
+        return list(out[:n])
 
     def car_order(self):
         """Iteration order of the reference's car set (creation indices)."""

@@ -58,6 +58,9 @@ struct RefArena {
     std::vector<int> bp_cell;
     std::vector<uint32_t> bp_seq;
     uint32_t bp_counter = 0;
+    // goal-score and car-bump callbacks, in the order the reference fires them
+    std::vector<rsg_event> events;
+    uint32_t event_seq = 0;
 };
 
 btCollisionObject* body_object(RefArena* ra, int body) {
@@ -135,6 +138,26 @@ void* rsref_arena_create(int mode, float tick_rate) {
         ra->arena->_bulletWorld.getSolverInfo().m_timeStep = ra->arena->tickTime;
         ra->bp_cell.push_back(proxy_cell(ra, 0));
         ra->bp_seq.push_back(0);
+        ra->arena->SetCarBumpCallback([](Arena* arena, Car* bumper, Car* victim, bool is_demo, void* ui) {
+            auto* r = (RefArena*)ui;
+            rsg_event e{};
+            e.type = is_demo ? RSG_EVENT_DEMO : RSG_EVENT_BUMP;
+            e.tick = arena->tickCount;
+            e.a = bumper->id;
+            e.b = victim->id;
+            e.seq = r->event_seq++;
+            r->events.push_back(e);
+        }, ra);
+        if ((GameMode)mode != GameMode::THE_VOID)
+            ra->arena->SetGoalScoreCallback([](Arena* arena, Team team, void* ui) {
+                auto* r = (RefArena*)ui;
+                rsg_event e{};
+                e.type = RSG_EVENT_GOAL;
+                e.tick = arena->tickCount;
+                e.a = (uint32_t)team;
+                e.seq = r->event_seq++;
+                r->events.push_back(e);
+            }, ra);
         return ra;
     } catch (std::exception& e) {
         g_err = e.what();
@@ -757,4 +780,14 @@ extern "C" int rsref_ray_single_car(void* h, const float* from, const float* to,
     *frac = cb.m_closestHitFraction;
     for (int k = 0; k < 3; k++) normal[k] = n[k];
     return 1;
+}
+
+// Events recorded from the reference's callbacks since the last call
+// (rsg_event records; `arena` is 0).  Returns the number copied.
+extern "C" int rsref_pop_events(void* h, rsg_event* out, int max_out) {
+    auto* ra = (RefArena*)h;
+    int n = (int)ra->events.size() < max_out ? (int)ra->events.size() : max_out;
+    for (int i = 0; i < n; i++) out[i] = ra->events[i];
+    ra->events.clear();
+    return n;
 }

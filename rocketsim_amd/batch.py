@@ -8,7 +8,7 @@ the get/set methods replace Car/Ball/BoostPad Get/SetState.
 import ctypes as C
 
 from . import native
-from .state import (BallState, BatchDesc, CarState, Controls, MODE_SOCCAR, MODE_THE_VOID,
+from .state import (BallState, BatchDesc, CarState, Controls, Event, MODE_SOCCAR, MODE_THE_VOID,
                     NUM_PADS, PadState)
 
 
@@ -110,6 +110,18 @@ class ArenaBatch:
     def set_launch(self, arenas_per_block):
         """Arenas per workgroup of the step kernel (0 = device default)."""
         native.check(self.L.rsg_batch_set_launch(self.h, int(arenas_per_block)))
+
+    def enable_events(self, capacity=1 << 16):
+        """Goal / bump / demolition events into a device ring of `capacity`."""
+        native.check(self.L.rsg_batch_enable_events(self.h, capacity))
+
+    def poll_events(self, max_out=1 << 16):
+        """Events since the last poll, ordered by (arena, sequence); returns
+        (list of Event, number lost)."""
+        out = (Event * max_out)()
+        lost = C.c_int()
+        n = native.check(self.L.rsg_batch_poll_events(self.h, out, max_out, C.byref(lost)))
+        return list(out[:n]), lost.value
 
     def graph_create(self, ticks_per_launch, launches):
         """Capture `launches` launches of `ticks_per_launch` ticks in a hipGraph."""

@@ -27,8 +27,36 @@ struct ArenaLocal {
     // unordered_set<Car*>, src/Sim/Arena/Arena.h:35; it is iterated for the
     // pre-tick, post-tick/boost-pad and kickoff loops).
     uint8_t order[MAX_CARS];
+    // event output (the batch's ring, or null) and this arena's sequence
+    rsg_event* ev_buf;
+    uint32_t* ev_count;
+    uint32_t ev_cap;
+    uint32_t ev_seq;
+    uint32_t arena_id;
     Car cars[MAX_CARS];
 };
+
+// Appends an event to the batch's ring (dropped when the ring is full; the
+// counter keeps counting, so the host sees how many were lost).
+RSG_FN void emit_event(ArenaLocal& A, uint32_t type, uint32_t a, uint32_t b) {
+    uint32_t seq = A.ev_seq++;
+    if (!A.ev_buf) return;
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t idx = atomicAdd(A.ev_count, 1u);
+#else
+    uint32_t idx = (*A.ev_count)++;
+#endif
+    if (idx >= A.ev_cap) return;
+    rsg_event e;
+    e.arena = A.arena_id;
+    e.type = type;
+    e.tick = A.tick;
+    e.a = a;
+    e.b = b;
+    e.seq = seq;
+    e._pad = 0;
+    A.ev_buf[idx] = e;
+}
 
 struct TickParams {
     int mode;

@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -371,6 +372,8 @@ void rsg_batch_destroy(rsg_batch* b) {
     if (b->dev.ball) (void)hipFree(b->dev.ball);
     if (b->dev.pad) (void)hipFree(b->dev.pad);
     if (b->dev.arena) (void)hipFree(b->dev.arena);
+    if (b->dev.ev_buf) (void)hipFree(b->dev.ev_buf);
+    if (b->dev.ev_count) (void)hipFree(b->dev.ev_count);
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
@@ -776,6 +779,43 @@ int rsg_batch_get_car_order(rsg_batch* b, int32_t* order) {
             order[b->car_offset[a] + k] = (int32_t)((ow >> (4 * k)) & 0xFu);
     }
     return 0;
+}
+
+int rsg_batch_enable_events(rsg_batch* b, int capacity) {
+    if (!b) return fail("null batch");
+    if (capacity < 0) return fail("rsg_batch_enable_events: capacity must be >= 0");
+    if (b->last_stream) HIP_OK(hipStreamSynchronize(b->last_stream));
+    if (b->dev.ev_buf) { (void)hipFree(b->dev.ev_buf); b->dev.ev_buf = nullptr; }
+    if (b->dev.ev_count) { (void)hipFree(b->dev.ev_count); b->dev.ev_count = nullptr; }
+    b->dev.ev_cap = 0;
+    if (capacity == 0) return 0;
+    HIP_OK(hipMalloc((void**)&b->dev.ev_buf, (size_t)capacity * sizeof(rsg_event)));
+    HIP_OK(hipMalloc((void**)&b->dev.ev_count, sizeof(uint32_t)));
+    uint32_t zero = 0;
+    HIP_OK(hipMemcpy(b->dev.ev_count, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    b->dev.ev_cap = (uint32_t)capacity;
+    return 0;
+}
+
+int rsg_batch_poll_events(rsg_batch* b, rsg_event* out, int max_out, int* n_lost) {
+    if (!b) return fail("null batch");
+    if (n_lost) *n_lost = 0;
+    if (!b->dev.ev_buf) return 0;
+    if (b->last_stream) HIP_OK(hipStreamSynchronize(b->last_stream));
+    uint32_t count = 0;
+    HIP_OK(hipMemcpy(&count, b->dev.ev_count, sizeof(count), hipMemcpyDeviceToHost));
+    uint32_t stored = count < b->dev.ev_cap ? count : b->dev.ev_cap;
+    std::vector<rsg_event> ev(stored);
+    if (stored) HIP_OK(hipMemcpy(ev.data(), b->dev.ev_buf, stored * sizeof(rsg_event), hipMemcpyDeviceToHost));
+    uint32_t zero = 0;
+    HIP_OK(hipMemcpy(b->dev.ev_count, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    std::sort(ev.begin(), ev.end(), [](const rsg_event& x, const rsg_event& y) {
+        return x.arena != y.arena ? x.arena < y.arena : x.seq < y.seq;
+    });
+    int n = (int)stored < max_out ? (int)stored : (max_out > 0 ? max_out : 0);
+    if (out && n) std::memcpy(out, ev.data(), (size_t)n * sizeof(rsg_event));
+    if (n_lost) *n_lost = (int)(count - (uint32_t)n);
+    return n;
 }
 
 int rsg_batch_get_tick_counts(rsg_batch* b, uint64_t* out) {

@@ -388,6 +388,7 @@ struct World {
                     if ((local.x * BT_TO_UU) > BUMP_MIN_FORWARD_DIST) {
                         bool is_demo = flag(me.flags, RSG_CF_SUPERSONIC);
                         if (is_demo) is_demo = me.team != other.team;
+                        emit_event(A, is_demo ? RSG_EVENT_DEMO : RSG_EVENT_BUMP, me.id, other.id);
                         if (is_demo) {
                             setflag(other.flags, RSG_CF_DEMOED, true);
                             other.demo_timer = DEMO_RESPAWN_TIME;

@@ -82,7 +82,8 @@ enum ArenaWord : int {
     AW_RESETS = 7,
     AW_STATUS = 8,      // bit 0: non-finite state seen
     AW_CAR_ORDER = 9,   // car update order: slot k = car index in bits 4k..4k+3
-    ARENA_WORDS = 10
+    AW_EVENT_SEQ = 10,  // number of events the arena has emitted
+    ARENA_WORDS = 11
 };
 
 // Controls bit word
@@ -100,6 +101,10 @@ struct BatchDev {
     int synth_controls;// draw controls from the seeded process each tick
     uint64_t seed;
     float tick_time;
+    // event ring (null when events are off): append index from *ev_count
+    rsg_event* ev_buf;
+    uint32_t* ev_count;
+    uint32_t ev_cap;
 };
 
 }  // namespace rsg

@@ -182,6 +182,11 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.last_goal_tick = ar.u64(AW_LAST_GOAL_TICK);
     A.resets = ar.u(AW_RESETS);
     A.status = ar.u(AW_STATUS);
+    A.ev_buf = d.ev_buf;
+    A.ev_count = d.ev_count;
+    A.ev_cap = d.ev_cap;
+    A.ev_seq = ar.u(AW_EVENT_SEQ);
+    A.arena_id = (uint32_t)a;
     {
         // Anything but a permutation of 0..n_cars-1 falls back to the
         // default (reverse creation) order, so no index leaves the arena.
@@ -234,6 +239,7 @@ RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
     ar.su64(AW_LAST_GOAL_TICK, A.last_goal_tick);
     ar.u(AW_RESETS) = A.resets;
     ar.u(AW_STATUS) = A.status;
+    ar.u(AW_EVENT_SEQ) = A.ev_seq;
     {
         uint32_t ow = 0;
         for (int k = 0; k < MAX_CARS; k++) ow |= (uint32_t)(A.order[k] & 0xFu) << (4 * k);
@@ -588,6 +594,7 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
             scored = true;
             if (y > 0) A.goals_blue++; else A.goals_orange++;
             A.last_goal_tick = A.tick;
+            emit_event(A, RSG_EVENT_GOAL, y > 0 ? (uint32_t)RSG_TEAM_BLUE : (uint32_t)RSG_TEAM_ORANGE, 0u);
         }
     }
     A.tick++;

@@ -6,7 +6,7 @@ entry point raises.
 import ctypes as C
 import os
 
-from .state import BallState, BatchDesc, CarState, Controls, PadState
+from .state import BallState, BatchDesc, CarState, Controls, Event, PadState
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "librsg.so")
@@ -46,6 +46,8 @@ def lib():
         L.rsg_batch_get_launch.argtypes = [C.c_void_p]
         L.rsg_batch_set_kernel.argtypes = [C.c_void_p, C.c_int]
         L.rsg_batch_get_kernel.argtypes = [C.c_void_p]
+        L.rsg_batch_enable_events.argtypes = [C.c_void_p, C.c_int]
+        L.rsg_batch_poll_events.argtypes = [C.c_void_p, C.POINTER(Event), C.c_int, C.POINTER(C.c_int)]
         L.rsg_batch_graph_create.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.rsg_batch_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
         L.rsg_batch_graph_ticks.argtypes = [C.c_void_p]
@@ -79,6 +81,7 @@ EXPORTED = [
     "rsg_batch_set_car_order", "rsg_batch_get_car_order", "rsg_batch_set_launch", "rsg_batch_get_launch",
     "rsg_batch_set_kernel", "rsg_batch_get_kernel", "rsg_batch_autotune",
     "rsg_batch_graph_create", "rsg_batch_graph_launch", "rsg_batch_graph_ticks",
+    "rsg_batch_enable_events", "rsg_batch_poll_events",
     "rsg_batch_get_tick_counts", "rsg_batch_set_tick_counts", "rsg_batch_get_goals",
     "rsg_batch_reset_kickoff", "rsg_batch_is_ball_scored",
 ]

@@ -41,6 +41,18 @@ F4 = C.c_float * 4
 F9 = C.c_float * 9
 
 
+EVENT_GOAL, EVENT_BUMP, EVENT_DEMO = 1, 2, 3
+
+
+class Event(C.Structure):
+    """rsg_event (include/rsg_state.h)."""
+    _fields_ = [("arena", C.c_uint32), ("type", C.c_uint32), ("tick", C.c_uint64), ("a", C.c_uint32),
+                ("b", C.c_uint32), ("seq", C.c_uint32), ("_pad", C.c_uint32)]
+
+    def key(self):
+        return (self.arena, self.type, self.tick, self.a, self.b, self.seq)
+
+
 class Controls(C.Structure):
     _fields_ = [
         ("throttle", C.c_float), ("steer", C.c_float), ("pitch", C.c_float),

@@ -118,7 +118,10 @@ def compare_pads(ref, got):
 
 class Sample:
     __slots__ = ("mode", "teams", "presets", "car_order", "cars_in", "ball_in", "pads_in", "ctrls", "tick_in",
-                 "cars_out", "ball_out", "pads_out", "tick_out")
+                 "cars_out", "ball_out", "pads_out", "tick_out", "events")
+
+    def __init__(self):
+        self.events = None  # goal/bump/demo events of the tick (live oracle samples)
 
 
 def _rand_quat(rng):
@@ -367,7 +370,9 @@ def _one_tick(A, mode, teams, presets, ctrls):
     s.ctrls = [Controls.from_buffer_copy(c) for c in ctrls]
     for i in range(n):
         A.set_controls(i, ctrls[i])
+    A.pop_events()
     A.step(1)
+    s.events = [(e.type, e.tick, e.a, e.b) for e in A.pop_events()]
     s.cars_out = [A.get_car(i) for i in range(n)]
     s.ball_out = A.get_ball()
     s.pads_out = A.get_pads() if mode == MODE_SOCCAR else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
@@ -448,18 +453,25 @@ def run_on_gpu(samples, mode):
         batch.set_controls(sum((list(s.ctrls) for s in samples), []))
         batch.set_tick_counts([s.tick_in for s in samples])
         batch.set_car_order(sum((list(s.car_order) for s in samples), []))
+        batch.enable_events(1 << 16)
         batch.step(1)
+        events, lost = batch.poll_events(1 << 16)
+        assert lost == 0
         cars = batch.get_cars()
         balls = batch.get_balls()
         pads = batch.get_pads()
         ticks = batch.get_tick_counts()
     finally:
         batch.close()
+    per_arena = {}
+    for e in events:
+        per_arena.setdefault(e.arena, []).append((e.type, e.tick, e.a, e.b))
     results = []
     k = 0
     for i, s in enumerate(samples):
         n = len(s.teams)
-        results.append((cars[k:k + n], balls[i], pads[i * NUM_PADS:(i + 1) * NUM_PADS], ticks[i]))
+        results.append((cars[k:k + n], balls[i], pads[i * NUM_PADS:(i + 1) * NUM_PADS], ticks[i],
+                        per_arena.get(i, [])))
         k += n
     return results
 
@@ -468,8 +480,11 @@ def check_samples(samples, results, max_report=10):
     """Return (n_bad_samples, messages)."""
     msgs = []
     nbad = 0
-    for i, (s, (cars, ball, pads, tick)) in enumerate(zip(samples, results)):
+    for i, (s, res) in enumerate(zip(samples, results)):
+        cars, ball, pads, tick = res[:4]
         bad = []
+        if s.events is not None and len(res) > 4 and s.events != res[4]:
+            bad.append(("events", s.events, res[4], None))
         for j in range(len(s.teams)):
             bad += [(f"car{j}." + b[0],) + b[1:] for b in compare_car(s.cars_out[j], cars[j])]
         bad += compare_ball(s.ball_out, ball)

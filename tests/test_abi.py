@@ -10,7 +10,7 @@ import tempfile
 import pytest
 
 from rocketsim_amd import native
-from rocketsim_amd.state import BallState, BatchDesc, CarState, Controls, PadState
+from rocketsim_amd.state import BallState, BatchDesc, CarState, Controls, Event, PadState
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -32,7 +32,7 @@ def test_library_exports_all_symbols():
 
 
 RECORDS = {"rsg_controls": Controls, "rsg_car_state": CarState, "rsg_ball_state": BallState,
-           "rsg_pad_state": PadState, "rsg_batch_desc": BatchDesc}
+           "rsg_pad_state": PadState, "rsg_batch_desc": BatchDesc, "rsg_event": Event}
 
 
 def test_ctypes_layout_matches_c_header():
