@@ -122,6 +122,44 @@ def run_timed(batch, steps, warmup, chunk, use_graph, stream):
     return wall, start.elapsed_time(end), chunk, graph
 
 
+def bench_adjoint(n_cars=65536, ticks=120, reps=5, seed=7):
+    """Differentiable air control (65536 airborne cars, 120 ticks, checkpoints
+    every 16 ticks): rollouts/s of forward + hand-written adjoint through the
+    torch autograd Function."""
+    import torch
+    from rocketsim_amd.adjoint import air_rollout
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    pos = torch.stack([torch.rand(n_cars, device="cuda", generator=g) * 4000 - 2000,
+                       torch.rand(n_cars, device="cuda", generator=g) * 4000 - 2000,
+                       800 + 1000 * torch.rand(n_cars, device="cuda", generator=g)], -1)
+    quat = torch.randn(n_cars, 4, device="cuda", generator=g)
+    quat = quat / quat.norm(dim=-1, keepdim=True)
+    state0 = torch.cat([pos, quat, torch.randn(n_cars, 3, device="cuda", generator=g) * 500,
+                        torch.randn(n_cars, 3, device="cuda", generator=g) * 2], -1)
+    params = torch.randn(ticks, n_cars, 4, device="cuda", generator=g, requires_grad=True)
+    tq = torch.randn(n_cars, 4, device="cuda", generator=g)
+    target = torch.cat([tq / tq.norm(dim=-1, keepdim=True), pos + 300 * torch.randn(n_cars, 3, device="cuda",
+                                                                                   generator=g)], -1)
+
+    def one():
+        controls = torch.cat([torch.tanh(params[..., :3]), torch.sigmoid(params[..., 3:])], -1)
+        loss, _ = air_rollout(state0, controls, target)
+        loss.sum().backward()
+        return loss
+
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    return {"value": n_cars / wall, "unit": "rollouts/s", "cars": n_cars, "ticks": ticks, "ckpt_every": 16,
+            "ms_per_batch": wall * 1e3,
+            "workload": "airborne cars, z U[800,1800], random orientation, v N(0,500), w N(0,2); "
+                        "controls tanh/sigmoid of parameters; loss geodesic angle + 1e-4 |dp|^2; fwd + adjoint"}
+
+
 KERNEL_NAMES = {1: "rsg_step_arena_kernel", 2: "rsg_step_arena_lds_kernel", 3: "rsg_step_arena_coop_kernel"}
 
 
@@ -144,6 +182,7 @@ def main():
                     help="also time the plain one-arena-per-thread kernel (scratch, 64 arenas per workgroup, "
                          "one tick per launch) on the same workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable air-control line")
     ap.add_argument("--profile-probe", action="store_true",
                     help="for rocprofv3 --pmc passes: fill and copy a 64 MiB buffer (calibrates the HBM byte "
                          "counters), then two launches of --chunk ticks; prints no JSON")
@@ -207,6 +246,8 @@ def main():
                  "value": arena_ticks / p_wall, "ms_per_tick": p_ev / args.steps,
                  "speedup": (arena_ticks / wall) / (arena_ticks / p_wall)}
 
+    adjoint = None if args.no_adjoint else bench_adjoint()
+
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     line = {
@@ -237,6 +278,8 @@ def main():
     }
     if plain is not None:
         line["plain_gpu"] = plain
+    if adjoint is not None:
+        line["adjoint"] = adjoint
     print(json.dumps(line), flush=True)
 
 

@@ -146,6 +146,29 @@ int rsg_batch_reset_kickoff(rsg_batch* b, int arena, uint64_t seed);
 int rsg_batch_enable_events(rsg_batch* b, int capacity);
 int rsg_batch_poll_events(rsg_batch* b, rsg_event* out, int max_out, int* n_lost);
 
+/* Differentiable air control (no counterpart in the reference; its forward
+ * model is the reference's tick of an airborne car: Car::_UpdateAirTorque
+ * Car.cpp:584-667, Car::_UpdateBoost Car.cpp:486-530 with the boost input
+ * relaxed to [0, 1], the world step's integration and the speed caps of
+ * Car::_FinishPhysicsTick).  All pointers are device memory, structure of
+ * arrays by word, n cars:
+ *   state   [13][n]  position (uu), quaternion (x, y, z, w), velocity (uu/s),
+ *                    angular velocity (rad/s)
+ *   controls [ticks][4][n]  pitch, yaw, roll in [-1, 1], boost in [0, 1]
+ *   target  [7][n]   quaternion, position (uu)
+ *   checkpoints [ceil(ticks / ckpt_every)][13][n]  (written by the rollout,
+ *                    read by the backward pass; ckpt_every <= 32)
+ * rsg_air_rollout steps every car `ticks` ticks and writes the final state
+ * and loss = 2 acos|<q, q_target>| + 1e-4 |p - p_target|^2 (uu) when the
+ * pointers are not NULL.  rsg_air_rollout_backward recomputes each segment
+ * from its checkpoint and writes d(sum dloss * loss)/dcontrols and, if not
+ * NULL, /dstate0. */
+int rsg_air_rollout(int n, int ticks, int ckpt_every, float tick_rate, const float* state0, const float* controls,
+                    const float* target, float* state_out, float* loss, float* checkpoints, void* stream);
+int rsg_air_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, const float* controls,
+                             const float* target, const float* checkpoints, const float* dloss, float* dcontrols,
+                             float* dstate0, void* stream);
+
 /* Arena::IsBallScored (Arena.cpp:893-901) per arena. */
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
 

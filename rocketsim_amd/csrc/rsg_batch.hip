@@ -22,6 +22,7 @@
 #include "rsg_mesh_build.h"
 #include "rsg_tick.h"
 #include "rsg_api_convert.h"
+#include "rsg_adjoint.h"
 
 using namespace rsg;
 
@@ -181,6 +182,89 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
         __syncthreads();
     }
     if (live && k == 0) store_arena(d, a, A);
+}
+
+// ---------------------------------------------------------------------------
+// Airborne-car rollouts and their adjoint (rsg_adjoint.h).  One lane per car;
+// arrays are structure-of-arrays by word: state [13][n] (position uu,
+// quaternion x y z w, velocity uu/s, angular velocity rad/s), controls
+// [ticks][4][n] (pitch, yaw, roll, boost), target [7][n] (quaternion,
+// position uu), checkpoints [ceil(ticks / K)][13][n] in Bullet units.
+
+constexpr int AIR_MAX_K = 32;
+
+RSG_FN AirState air_load(const float* a, int n, int i, bool uu) {
+    float k = uu ? UU_TO_BT : 1.f;
+    AirState s;
+    s.p = v3(a[0 * n + i], a[1 * n + i], a[2 * n + i]) * k;
+    s.q = q4(a[3 * n + i], a[4 * n + i], a[5 * n + i], a[6 * n + i]);
+    s.v = v3(a[7 * n + i], a[8 * n + i], a[9 * n + i]) * k;
+    s.w = v3(a[10 * n + i], a[11 * n + i], a[12 * n + i]);
+    return s;
+}
+
+RSG_FN void air_store(float* a, int n, int i, const AirState& s, float kpv) {
+    a[0 * n + i] = s.p.x * kpv; a[1 * n + i] = s.p.y * kpv; a[2 * n + i] = s.p.z * kpv;
+    a[3 * n + i] = s.q.x; a[4 * n + i] = s.q.y; a[5 * n + i] = s.q.z; a[6 * n + i] = s.q.w;
+    a[7 * n + i] = s.v.x * kpv; a[8 * n + i] = s.v.y * kpv; a[9 * n + i] = s.v.z * kpv;
+    a[10 * n + i] = s.w.x; a[11 * n + i] = s.w.y; a[12 * n + i] = s.w.z;
+}
+
+RSG_FN AirCtrl air_ctrl(const float* c, int n, int i, int t) {
+    const float* b = c + (size_t)t * 4 * n;
+    AirCtrl k;
+    k.pitch = b[i]; k.yaw = b[n + i]; k.roll = b[2 * n + i]; k.boost = b[3 * n + i];
+    return k;
+}
+
+__global__ void __launch_bounds__(256) rsg_air_forward_kernel(int n, int ticks, int K, float dt, const float* s0,
+                                                              const float* ctrl, const float* target, float* s_out,
+                                                              float* loss, float* ckpt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    AirState s = air_load(s0, n, i, true);
+    for (int t = 0; t < ticks; t++) {
+        if (ckpt && t % K == 0) air_store(ckpt + (size_t)(t / K) * AIR_STATE_WORDS * n, n, i, s, 1.f);
+        air_step(s, air_ctrl(ctrl, n, i, t), dt);
+    }
+    if (s_out) air_store(s_out, n, i, s, BT_TO_UU);
+    if (loss) {
+        Q4 tq = q4(target[i], target[n + i], target[2 * n + i], target[3 * n + i]);
+        V3 tp = v3(target[4 * n + i], target[5 * n + i], target[6 * n + i]) * UU_TO_BT;
+        loss[i] = air_loss(s, tq, tp);
+    }
+}
+
+__global__ void __launch_bounds__(256) rsg_air_backward_kernel(int n, int ticks, int K, float dt, const float* ctrl,
+                                                               const float* target, const float* ckpt,
+                                                               const float* dloss, float* dctrl, float* ds0) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int nseg = (ticks + K - 1) / K;
+    AirState st[AIR_MAX_K];
+    // final state: recompute the last segment from its checkpoint
+    AirState s = air_load(ckpt + (size_t)(nseg - 1) * AIR_STATE_WORDS * n, n, i, false);
+    for (int t = (nseg - 1) * K; t < ticks; t++) air_step(s, air_ctrl(ctrl, n, i, t), dt);
+    Q4 tq = q4(target[i], target[n + i], target[2 * n + i], target[3 * n + i]);
+    V3 tp = v3(target[4 * n + i], target[5 * n + i], target[6 * n + i]) * UU_TO_BT;
+    AirState g;
+    air_loss_grad(s, tq, tp, dloss[i], g);
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        int t0 = seg * K, t1 = t0 + K < ticks ? t0 + K : ticks;
+        AirState x = air_load(ckpt + (size_t)seg * AIR_STATE_WORDS * n, n, i, false);
+        for (int t = t0; t < t1; t++) {
+            st[t - t0] = x;
+            air_step(x, air_ctrl(ctrl, n, i, t), dt);
+        }
+        for (int t = t1 - 1; t >= t0; t--) {
+            AirCtrl gc = {0.f, 0.f, 0.f, 0.f};
+            air_step_adjoint(st[t - t0], air_ctrl(ctrl, n, i, t), dt, g, gc);
+            float* b = dctrl + (size_t)t * 4 * n;
+            b[i] = gc.pitch; b[n + i] = gc.yaw; b[2 * n + i] = gc.roll; b[3 * n + i] = gc.boost;
+        }
+    }
+    // initial-state gradient in interface units (uu for position and velocity)
+    if (ds0) air_store(ds0, n, i, g, UU_TO_BT);
 }
 
 __global__ void __launch_bounds__(64) rsg_reset_kernel(BatchDev d, int only_arena, uint64_t seed) {
@@ -566,6 +650,36 @@ int rsg_batch_graph_launch(rsg_batch* b, void* stream) {
 }
 
 int rsg_batch_graph_ticks(rsg_batch* b) { return b ? b->graph_ticks : fail("null batch"); }
+
+int rsg_air_rollout(int n, int ticks, int ckpt_every, float tick_rate, const float* state0, const float* controls,
+                    const float* target, float* state_out, float* loss, float* checkpoints, void* stream) {
+    if (n < 0 || ticks <= 0) return fail("rsg_air_rollout: need n >= 0 and ticks > 0");
+    if (ckpt_every < 1 || ckpt_every > AIR_MAX_K) return fail("rsg_air_rollout: ckpt_every must be 1..32");
+    if (tick_rate <= 0.f) return fail("rsg_air_rollout: tick_rate must be > 0");
+    if (!state0 || !controls || (loss && !target)) return fail("rsg_air_rollout: null argument");
+    if (n == 0) return 0;
+    int block = 256, grid = (n + block - 1) / block;
+    hipLaunchKernelGGL(rsg_air_forward_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream, n, ticks,
+                       ckpt_every, 1.f / tick_rate, state0, controls, target, state_out, loss, checkpoints);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int rsg_air_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, const float* controls,
+                             const float* target, const float* checkpoints, const float* dloss, float* dcontrols,
+                             float* dstate0, void* stream) {
+    if (n < 0 || ticks <= 0) return fail("rsg_air_rollout_backward: need n >= 0 and ticks > 0");
+    if (ckpt_every < 1 || ckpt_every > AIR_MAX_K) return fail("rsg_air_rollout_backward: ckpt_every must be 1..32");
+    if (tick_rate <= 0.f) return fail("rsg_air_rollout_backward: tick_rate must be > 0");
+    if (!controls || !target || !checkpoints || !dloss || !dcontrols)
+        return fail("rsg_air_rollout_backward: null argument");
+    if (n == 0) return 0;
+    int block = 256, grid = (n + block - 1) / block;
+    hipLaunchKernelGGL(rsg_air_backward_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream, n, ticks,
+                       ckpt_every, 1.f / tick_rate, controls, target, checkpoints, dloss, dcontrols, dstate0);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
 
 // Autotuner: every feasible (kernel, arenas per workgroup) is timed on a
 // device copy of the batch state for `ticks` ticks after one warm-up launch,
