@@ -160,6 +160,50 @@ def bench_adjoint(n_cars=65536, ticks=120, reps=5, seed=7):
                         "controls tanh/sigmoid of parameters; loss geodesic angle + 1e-4 |dp|^2; fwd + adjoint"}
 
 
+def bench_ball_pred(n=4096, T=720, reps=3, seed=11):
+    """Ball prediction (BallPredTracker over the GPU): n trackers predicting
+    T ticks (6 s) of the Soccar ball, from random states in the field.  Full
+    re-prediction (ForceUpdateAllPred) and the incremental update of 8 ticks
+    that a bot calling UpdatePredManual every 8 ticks pays."""
+    import random
+    import torch
+    from rocketsim_amd.ball_pred import BallPredTracker
+    from rocketsim_amd.state import BallState, MODE_SOCCAR
+    rng = random.Random(seed)
+    states = []
+    for _ in range(n):
+        s = BallState()
+        s.pos[:] = [rng.uniform(-3800, 3800), rng.uniform(-4800, 4800), rng.uniform(100, 1900)]
+        s.vel[:] = [rng.gauss(0, 1200), rng.gauss(0, 1200), rng.gauss(0, 800)]
+        s.ang_vel[:] = [rng.gauss(0, 2) for _ in range(3)]
+        s.rot[:] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+        states.append(s)
+    tr = BallPredTracker(mode=MODE_SOCCAR, n=n, num_pred_ticks=T)
+    try:
+        tr.force_update_all_pred(states)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            tr.force_update_all_pred(states)
+        torch.cuda.synchronize()
+        full = (time.perf_counter() - t0) / reps
+        # incremental: the current state is the prediction 8 ticks ahead
+        pd = tr.pred_data_all()
+        cur = [pd[i * T + 8] for i in range(n)]
+        ticks = [8] * n
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr.update_pred_manual(cur, ticks)
+        torch.cuda.synchronize()
+        inc = time.perf_counter() - t0
+    finally:
+        tr.close()
+    return {"value": n * (T - 1) / full, "unit": "ball-ticks/s", "trackers": n, "num_pred_ticks": T,
+            "ms_full_update": full * 1e3, "ms_update_8_ticks": inc * 1e3,
+            "workload": "Soccar ball, pos U(field), vel N(0,1200/800), w N(0,2); ForceUpdateAllPred; "
+                        "then UpdatePredManual 8 ticks on"}
+
+
 KERNEL_NAMES = {1: "rsg_step_arena_kernel", 2: "rsg_step_arena_lds_kernel", 3: "rsg_step_arena_coop_kernel"}
 
 
@@ -183,6 +227,7 @@ def main():
                          "one tick per launch) on the same workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable air-control line")
+    ap.add_argument("--no-ball-pred", action="store_true", help="skip the ball-prediction line")
     ap.add_argument("--profile-probe", action="store_true",
                     help="for rocprofv3 --pmc passes: fill and copy a 64 MiB buffer (calibrates the HBM byte "
                          "counters), then two launches of --chunk ticks; prints no JSON")
@@ -247,6 +292,7 @@ def main():
                  "speedup": (arena_ticks / wall) / (arena_ticks / p_wall)}
 
     adjoint = None if args.no_adjoint else bench_adjoint()
+    ball_pred = None if args.no_ball_pred else bench_ball_pred()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
@@ -280,6 +326,8 @@ def main():
         line["plain_gpu"] = plain
     if adjoint is not None:
         line["adjoint"] = adjoint
+    if ball_pred is not None:
+        line["ball_pred"] = ball_pred
     print(json.dumps(line), flush=True)
 
 

@@ -172,6 +172,57 @@ int rsg_air_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, 
 /* Arena::IsBallScored (Arena.cpp:893-901) per arena. */
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
 
+/* ---- Ball prediction: a batch of n BallPredTracker
+ * (src/Sim/BallPredTracker/BallPredTracker.h:7-36).  Tracker i owns a
+ * ball-only arena of `mode` (the reference's tracker makes its own car-less
+ * copy of the arena, BallPredTracker.cpp:5-12) and num_pred_ticks predicted
+ * ball states (predData), kept on the device.  Updates run on `stream`
+ * (NULL = the default stream); the getters wait for the last update. */
+typedef struct rsg_ball_pred rsg_ball_pred;
+
+/* BallPredTracker::BallPredTracker (BallPredTracker.cpp:5-12) without the
+ * first update: the prediction is empty until an update runs.  mode is
+ * RSG_MODE_SOCCAR (needs rsg_init_mesh) or RSG_MODE_THE_VOID;
+ * num_pred_ticks >= 1. */
+rsg_ball_pred* rsg_ball_pred_create(int mode, int n, int num_pred_ticks, float tick_rate);
+void rsg_ball_pred_destroy(rsg_ball_pred* p);
+
+/* ForceUpdateAllPred (BallPredTracker.cpp:63-71) for every tracker from the
+ * host states initial[n] (their bp_cell / bp_seq are not used). */
+int rsg_ball_pred_force_update(rsg_ball_pred* p, const rsg_ball_state* initial, void* stream);
+
+/* UpdatePredManual (BallPredTracker.cpp:25-61) for every tracker: cur[n] and
+ * ticks_since_last_update[n] are host arrays.  A tracker whose prediction at
+ * that tick Matches (Ball.cpp:12-17) the current state drops the consumed
+ * ticks and predicts only the new tail; any other re-predicts in full. */
+int rsg_ball_pred_update_manual(rsg_ball_pred* p, const rsg_ball_state* cur, const int32_t* ticks_since_last_update,
+                                void* stream);
+
+/* UpdatePredFromArena (BallPredTracker.cpp:18-23) with tracker i following
+ * arena i of batch b (same n, mode and tick rate): the ball state and tick
+ * count are read on the device, with no host round trip. */
+int rsg_ball_pred_update_from_batch(rsg_ball_pred* p, rsg_batch* b, void* stream);
+
+/* predData of one tracker: writes min(size, max_out) states, tick 0 first, and
+ * returns the prediction's size (0 before the first update).  The states'
+ * bp_cell / bp_seq are 0 (BallState carries no broadphase record). */
+int rsg_ball_pred_get(rsg_ball_pred* p, int tracker, rsg_ball_state* out, int max_out);
+
+/* Every tracker's prediction, tracker-major: out[i * num_pred_ticks + t]. */
+int rsg_ball_pred_get_all(rsg_ball_pred* p, rsg_ball_state* out);
+
+/* GetBallStateForTime (BallPredTracker.cpp:73-79): the state at index
+ * clamp(pred_time / tick_time, 0, size - 1), truncated; an error while the
+ * prediction is empty. */
+int rsg_ball_pred_state_for_time(rsg_ball_pred* p, int tracker, float pred_time, rsg_ball_state* out);
+
+/* Launch shape of the updates: trackers per 64-lane wave (1..64), or 0 for
+ * the default, ceil(n / SIMDs) so that every SIMD holds a wave. */
+int rsg_ball_pred_set_launch(rsg_ball_pred* p, int trackers_per_wave);
+
+/* Device view of the predictions for consumers on the GPU (no copy). */
+int rsg_ball_pred_view(rsg_ball_pred* p, rsg_ball_pred_view_t* out);
+
 #ifdef __cplusplus
 }
 #endif

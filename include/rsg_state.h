@@ -131,6 +131,28 @@ enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8 };
  * (Arena::SetCarBumpCallback, Arena.h:25,108; fired for bumps and
  * demolitions during collision handling, Arena.cpp:377-400). */
 enum { RSG_EVENT_GOAL = 1, RSG_EVENT_BUMP = 2, RSG_EVENT_DEMO = 3 };
+
+/* Ball prediction records on the device (rsg_ball_pred_view): one row of
+ * RSG_PRED_WORDS word planes per predicted tick, in the public units of
+ * rsg_ball_state (BallState, src/Sim/Ball/Ball.h:18-57). */
+enum {
+    RSG_PRED_POS = 0,                 /* 3 floats, uu */
+    RSG_PRED_ROT = 3,                 /* 9 floats, forward, right, up columns */
+    RSG_PRED_VEL = 12,                /* 3 floats, uu/s */
+    RSG_PRED_ANG_VEL = 15,            /* 3 floats, rad/s */
+    RSG_PRED_TICKS_SINCE_UPDATE = 18, /* uint64 as two words, low first */
+    RSG_PRED_WORDS = 20
+};
+typedef struct rsg_ball_pred_view_t {
+    const float* traj;      /* [num_pred_ticks][RSG_PRED_WORDS][n]: word w of
+                               tracker i's tick t is
+                               traj[(row * RSG_PRED_WORDS + w) * n + i] with
+                               row = (head[i] + t) % num_pred_ticks */
+    const int32_t* head;    /* [n] first row of each tracker's ring */
+    const int32_t* size;    /* [n] predicted ticks held (predData.size()) */
+    int n;
+    int num_pred_ticks;
+} rsg_ball_pred_view_t;
 typedef struct rsg_event {
     uint32_t arena;
     uint32_t type;      /* RSG_EVENT_* */

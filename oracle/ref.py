@@ -55,6 +55,14 @@ def lib():
         L.rsref_is_ball_scored.argtypes = [C.c_void_p]
         L.rsref_reset_kickoff.argtypes = [C.c_void_p, C.c_int]
         L.rsref_pad_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int)]
+        L.rsref_ball_pred_create.restype = C.c_void_p
+        L.rsref_ball_pred_create.argtypes = [C.c_void_p, C.c_int]
+        L.rsref_ball_pred_destroy.argtypes = [C.c_void_p]
+        L.rsref_ball_pred_update_from_arena.argtypes = [C.c_void_p, C.c_void_p]
+        L.rsref_ball_pred_update_manual.argtypes = [C.c_void_p, C.POINTER(BallState), C.c_int]
+        L.rsref_ball_pred_force.argtypes = [C.c_void_p, C.POINTER(BallState)]
+        L.rsref_ball_pred_get.argtypes = [C.c_void_p, C.POINTER(BallState), C.c_int]
+        L.rsref_ball_pred_for_time.argtypes = [C.c_void_p, C.c_float, C.POINTER(BallState)]
         L.rsref_run_workload.restype = C.c_double
         L.rsref_run_workload.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.c_uint64, C.POINTER(C.c_uint64)]
@@ -170,6 +178,46 @@ class RefArena:
     def close(self):
         if self.h:
             self.L.rsref_arena_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RefBallPredTracker:
+    """The reference's BallPredTracker (src/Sim/BallPredTracker/BallPredTracker.h)
+    over a RefArena; constructing it predicts from the arena's ball."""
+
+    def __init__(self, arena, num_pred_ticks):
+        self.L = lib()
+        self.h = self.L.rsref_ball_pred_create(arena.h, num_pred_ticks)
+        self.num_pred_ticks = num_pred_ticks
+
+    def update_pred_from_arena(self, arena):
+        self.L.rsref_ball_pred_update_from_arena(self.h, arena.h)
+
+    def update_pred_manual(self, state, ticks_since_last_update):
+        self.L.rsref_ball_pred_update_manual(self.h, C.byref(state), ticks_since_last_update)
+
+    def force_update_all_pred(self, state):
+        self.L.rsref_ball_pred_force(self.h, C.byref(state))
+
+    def pred_data(self):
+        out = (BallState * self.num_pred_ticks)()
+        n = self.L.rsref_ball_pred_get(self.h, out, self.num_pred_ticks)
+        return list(out[:n])
+
+    def get_ball_state_for_time(self, t):
+        out = BallState()
+        self.L.rsref_ball_pred_for_time(self.h, t, C.byref(out))
+        return out
+
+    def close(self):
+        if self.h:
+            self.L.rsref_ball_pred_destroy(self.h)
             self.h = None
 
     def __del__(self):

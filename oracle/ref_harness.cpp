@@ -13,9 +13,12 @@
 //     not touch (src/Sim/Car/Car.cpp:23-36)
 //   * Arena::Step (src/Sim/Arena/Arena.cpp:689-756)
 //   * a multi-threaded timed run of the seeded workload (one Arena per thread)
-//     for the CPU baseline.
+//     for the CPU baseline
+//   * BallPredTracker (src/Sim/BallPredTracker/BallPredTracker.cpp) over an
+//     arena of this harness.
 
 #include "RocketSim.h"
+#include "Sim/BallPredTracker/BallPredTracker.h"
 #include "rsg_state.h"
 #include "rsg_workload.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/BroadphaseCollision/btRSBroadphase.h"
@@ -790,4 +793,45 @@ extern "C" int rsref_pop_events(void* h, rsg_event* out, int max_out) {
     for (int i = 0; i < n; i++) out[i] = ra->events[i];
     ra->events.clear();
     return n;
+}
+
+// ---------------------------------------------------------------------------
+// BallPredTracker (src/Sim/BallPredTracker/BallPredTracker.h:7-36).
+
+static BallState ball_state_from(const rsg_ball_state* s) {
+    BallState bs;
+    vset(bs.pos, s->pos); rset(bs.rotMat, s->rot); vset(bs.vel, s->vel); vset(bs.angVel, s->ang_vel);
+    bs.tickCountSinceUpdate = s->tick_count_since_update;
+    return bs;
+}
+
+static void ball_state_to(const BallState& bs, rsg_ball_state* s) {
+    std::memset(s, 0, sizeof(*s));
+    s->tick_count_since_update = bs.tickCountSinceUpdate;
+    vget(bs.pos, s->pos); rget(bs.rotMat, s->rot); vget(bs.vel, s->vel); vget(bs.angVel, s->ang_vel);
+}
+
+// new BallPredTracker(arena, numPredTicks): predicts from the arena's ball at once.
+extern "C" void* rsref_ball_pred_create(void* h, int num_pred_ticks) {
+    return new BallPredTracker(((RefArena*)h)->arena, (size_t)num_pred_ticks);
+}
+extern "C" void rsref_ball_pred_destroy(void* p) { delete (BallPredTracker*)p; }
+extern "C" void rsref_ball_pred_update_from_arena(void* p, void* h) {
+    ((BallPredTracker*)p)->UpdatePredFromArena(((RefArena*)h)->arena);
+}
+extern "C" void rsref_ball_pred_update_manual(void* p, const rsg_ball_state* s, int ticks_since) {
+    ((BallPredTracker*)p)->UpdatePredManual(ball_state_from(s), ticks_since);
+}
+extern "C" void rsref_ball_pred_force(void* p, const rsg_ball_state* s) {
+    ((BallPredTracker*)p)->ForceUpdateAllPred(ball_state_from(s));
+}
+// predData: copies min(size, max_out) states and returns the size.
+extern "C" int rsref_ball_pred_get(void* p, rsg_ball_state* out, int max_out) {
+    auto* t = (BallPredTracker*)p;
+    int n = (int)t->predData.size();
+    for (int i = 0; i < n && i < max_out; i++) ball_state_to(t->predData[i], &out[i]);
+    return n;
+}
+extern "C" void rsref_ball_pred_for_time(void* p, float pred_time, rsg_ball_state* out) {
+    ball_state_to(((BallPredTracker*)p)->GetBallStateForTime(pred_time), out);
 }

@@ -50,6 +50,8 @@ class ArenaBatch:
         self.h = h
         self.L = L
         self.n_arenas = n
+        self.mode = mode
+        self.tick_rate = tick_rate
         self.n_cars = total
         self.cars_per_arena = list(cars_per_arena)
         self.offsets = [L.rsg_batch_car_offset(h, a) for a in range(n)]

@@ -135,6 +135,43 @@ inline void ball_from_api(const CarRef& r, const rsg_ball_state& s) {
     r.u(BW_BP_SEQ) = s.bp_seq;
 }
 
+// A new arena's records (Arena::Create + AddCar, src/Sim/Arena/Arena.cpp:50-62,
+// 416-540): cars with their config and spawn boost, the ball at rest, pads
+// active; the proxies registered where the reference creates them.
+inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_t* team, const uint8_t* preset) {
+    CarRef ar = arena_ref(h, a);
+    ar.u(AW_NUM_CARS) = (uint32_t)n_cars;
+    uint32_t ow = 0;  // default car update order: reverse creation
+    for (int k = 0; k < n_cars; k++) ow |= (uint32_t)(n_cars - 1 - k) << (4 * k);
+    ar.u(AW_CAR_ORDER) = ow;
+    for (int i = 0; i < n_cars; i++) {
+        CarRef cr = car_ref(h, a, i);
+        cr.u(CW_CONFIG) = (uint32_t)preset[i] | ((uint32_t)team[i] << 8);
+        cr.u(CW_ID) = (uint32_t)(i + 1);
+        cr.su64(CW_HIT_TICK, ~0ull);
+        cr.su64(CW_HIT_EXTRA_TICK, ~0ull);
+        cr.sf(CW_BOOST, BOOST_SPAWN_AMOUNT);
+        store_rot(cr, CW_ROT, m3_identity());
+        // proxy created at the identity transform (Car::_BulletSetup,
+        // src/Sim/Car/Car.cpp:212-233), AABB without contact threshold
+        CarPhys cp = make_car_phys(preset[i]);
+        cr.u(CW_BP_CELL) = (uint32_t)bp_cell_of(cp.local_aabb_center - cp.local_aabb_half);
+        cr.u(CW_BP_SEQ) = (uint32_t)(1 + i);
+    }
+    CarRef br = ball_ref(h, a);
+    br.sv(BW_POS, v3(0, 0, BALL_REST_Z) * UU_TO_BT);
+    store_rot(br, BW_ROT, m3_identity());
+    {
+        // proxy created at (0, 0, radius) (Ball::_BulletSetup, src/Sim/Ball/Ball.cpp:85-114)
+        BallPhys bp = make_ball_phys();
+        float ext = bp.radius + 0.08f;
+        V3 c = v3(0, 0, bp.radius);
+        br.u(BW_BP_CELL) = (uint32_t)bp_cell_of(c - v3(ext, ext, ext));
+        br.u(BW_BP_SEQ) = 0;
+    }
+    for (int p = 0; p < PAD_NUM; p++) pad_ref(h, a, p).u(PW_ACTIVE) = 1;
+}
+
 inline void pad_to_api(const CarRef& r, rsg_pad_state& s) {
     s.is_active = r.u(PW_ACTIVE);
     s.cooldown = r.f(PW_COOLDOWN);

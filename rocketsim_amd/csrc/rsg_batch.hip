@@ -2,9 +2,15 @@
 // include/rsg.h.
 //
 // Kernel map:
-//   rsg_step_arena_kernel   one lane per arena, K ticks per launch, arena state
-//                           in registers/private memory between ticks
-//   rsg_reset_kernel        kickoff reset of one / all arenas
+//   rsg_step_arena_kernel       one lane per arena, K ticks per launch, arena
+//                               state in private (scratch) memory
+//   rsg_step_arena_lds_kernel   the same with the arena state in LDS
+//   rsg_step_arena_coop_kernel  4 lanes per car slot share the tick's
+//                               independent parts, arena state in LDS
+//   rsg_ball_pred_kernel        ball prediction trackers (rsg_ball_pred.h)
+//   rsg_air_forward_kernel,     differentiable air control: rollout with
+//   rsg_air_backward_kernel     checkpoints, hand-written adjoint
+//   rsg_reset_kernel            kickoff reset of one / all arenas
 //
 // Host side keeps a mirror of the SoA state for the get/set API; it is
 // uploaded lazily before the next step and downloaded lazily on the next get.
@@ -23,6 +29,7 @@
 #include "rsg_tick.h"
 #include "rsg_api_convert.h"
 #include "rsg_adjoint.h"
+#include "rsg_ball_pred.h"
 
 using namespace rsg;
 
@@ -267,6 +274,20 @@ __global__ void __launch_bounds__(256) rsg_air_backward_kernel(int n, int ticks,
     if (ds0) air_store(ds0, n, i, g, UU_TO_BT);
 }
 
+// Ball prediction (rsg_ball_pred.h): one update of every tracker,
+// `per_wave` trackers per 64-lane block (the rest of the wave idles, so small
+// batches still reach every SIMD).  Tracker i's prediction arena (a
+// ball-only arena of `pa`) is held in LDS, `stride` bytes per tracker.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
+rsg_ball_pred_kernel(BatchDev pa, MeshView mesh, PredDev p, int kind, BatchDev src, int stride, int per_wave) {
+    extern __shared__ __align__(16) unsigned char rsg_lds[];
+    if ((int)threadIdx.x >= per_wave) return;
+    const int i = blockIdx.x * per_wave + threadIdx.x;
+    if (i >= pa.n) return;
+    ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(rsg_lds + (size_t)threadIdx.x * (size_t)stride);
+    ball_pred_update(pa, pa.mode == RSG_MODE_SOCCAR ? &mesh : nullptr, p, kind, src, i, A);
+}
+
 __global__ void __launch_bounds__(64) rsg_reset_kernel(BatchDev d, int only_arena, uint64_t seed) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= d.n) return;
@@ -408,41 +429,9 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     }
     // Host-side initial state: cars with config, ball at rest, pads active.
     BatchDev h = host_view(b);
-    for (int a = 0; a < b->dev.n; a++) {
-        CarRef ar = arena_ref(h, a);
-        ar.u(AW_NUM_CARS) = (uint32_t)b->cars_per_arena[a];
-        uint32_t ow = 0;  // default car update order: reverse creation
-        for (int k = 0; k < b->cars_per_arena[a]; k++)
-            ow |= (uint32_t)(b->cars_per_arena[a] - 1 - k) << (4 * k);
-        ar.u(AW_CAR_ORDER) = ow;
-        for (int i = 0; i < b->cars_per_arena[a]; i++) {
-            int k = b->car_offset[a] + i;
-            CarRef cr = car_ref(h, a, i);
-            cr.u(CW_CONFIG) = (uint32_t)b->preset[k] | ((uint32_t)b->team[k] << 8);
-            cr.u(CW_ID) = (uint32_t)(i + 1);
-            cr.su64(CW_HIT_TICK, ~0ull);
-            cr.su64(CW_HIT_EXTRA_TICK, ~0ull);
-            cr.sf(CW_BOOST, BOOST_SPAWN_AMOUNT);
-            store_rot(cr, CW_ROT, m3_identity());
-            // proxy created at the identity transform (Car::_BulletSetup,
-            // src/Sim/Car/Car.cpp:212-233), AABB without contact threshold
-            CarPhys cp = make_car_phys(b->preset[k]);
-            cr.u(CW_BP_CELL) = (uint32_t)bp_cell_of(cp.local_aabb_center - cp.local_aabb_half);
-            cr.u(CW_BP_SEQ) = (uint32_t)(1 + i);
-        }
-        CarRef br = ball_ref(h, a);
-        br.sv(BW_POS, v3(0, 0, BALL_REST_Z) * UU_TO_BT);
-        store_rot(br, BW_ROT, m3_identity());
-        {
-            // proxy created at (0, 0, radius) (Ball::_BulletSetup, src/Sim/Ball/Ball.cpp:85-114)
-            BallPhys bp = make_ball_phys();
-            float ext = bp.radius + 0.08f;
-            V3 c = v3(0, 0, bp.radius);
-            br.u(BW_BP_CELL) = (uint32_t)bp_cell_of(c - v3(ext, ext, ext));
-            br.u(BW_BP_SEQ) = 0;
-        }
-        for (int p = 0; p < PAD_NUM; p++) pad_ref(h, a, p).u(PW_ACTIVE) = 1;
-    }
+    for (int a = 0; a < b->dev.n; a++)
+        init_arena_record(h, a, b->cars_per_arena[a], b->team.data() + b->car_offset[a],
+                          b->preset.data() + b->car_offset[a]);
     b->host_dirty = true;
     if (upload(b) != 0) { rsg_batch_destroy(b); return nullptr; }
     if (rsg_batch_reset_kickoff(b, -1, desc->seed) != 0) { rsg_batch_destroy(b); return nullptr; }
@@ -971,6 +960,260 @@ int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out) {
         out[a] = (b->dev.mode == RSG_MODE_SOCCAR &&
                   fabsf(y) > (SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y + BALL_COLLISION_RADIUS_SOCCAR)) ? 1 : 0;
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Ball prediction
+
+}  // extern "C"
+
+struct rsg_ball_pred {
+    rsg_batch* arena = nullptr;  // the trackers' ball-only arenas (arena i = tracker i)
+    int n = 0, T = 0;
+    uint32_t* traj = nullptr;    // [T][RSG_PRED_WORDS][n]
+    int32_t* meta = nullptr;     // head [n], size [n], last [n]
+    uint32_t* cur = nullptr;     // [RSG_PRED_WORDS][n] staged states
+    int32_t* ticks = nullptr;    // [n] staged ticks since the last update
+    hipStream_t last_stream = nullptr;
+    bool pending = false;        // an update was enqueued since the last wait
+    int per_wave = 0;            // trackers per wave, 0 = derived from n and the SIMD count
+};
+
+namespace {
+
+int pred_stride() {
+    return (int)((offsetof(ArenaLocal, cars) + 15) & ~(size_t)15);
+}
+
+int pred_per_wave(int n) {
+    int per = (n + device_simds() - 1) / device_simds();
+    return per < 1 ? 1 : (per > 64 ? 64 : per);
+}
+
+int pred_wait(rsg_ball_pred* p) {
+    if (p->pending) {
+        HIP_OK(hipStreamSynchronize(p->last_stream));
+        p->pending = false;
+    }
+    return 0;
+}
+
+int pred_launch(rsg_ball_pred* p, int kind, const BatchDev* src, hipStream_t s) {
+    rsg_batch* a = p->arena;
+    if (upload(a) != 0) return -1;
+    MeshView mv{};
+    if (a->dev.mode == RSG_MODE_SOCCAR) {
+        if (!g_mesh.ready) return fail("rsg_ball_pred: no arena mesh (call rsg_init_mesh first)");
+        mv = g_mesh.view;
+    }
+    PredDev d;
+    d.traj = p->traj;
+    d.head = p->meta;
+    d.size = p->meta + p->n;
+    d.last = p->meta + 2 * p->n;
+    d.cur = p->cur;
+    d.ticks = p->ticks;
+    d.T = p->T;
+    int per = p->per_wave > 0 ? p->per_wave : pred_per_wave(p->n);
+    int stride = pred_stride();
+    int grid = (p->n + per - 1) / per;
+    hipLaunchKernelGGL(rsg_ball_pred_kernel, dim3(grid), dim3(64), (size_t)per * (size_t)stride, s, a->dev, mv, d,
+                       kind, src ? *src : a->dev, stride, per);
+    HIP_OK(hipGetLastError());
+    a->dev_newer = true;
+    p->last_stream = s;
+    p->pending = true;
+    return 0;
+}
+
+// Host states -> staged word planes (Ball::GetState units, as given).
+int pred_stage(rsg_ball_pred* p, const rsg_ball_state* in, const int32_t* ticks) {
+    const size_t n = (size_t)p->n;
+    std::vector<uint32_t> w((size_t)RSG_PRED_WORDS * n);
+    for (size_t i = 0; i < n; i++) {
+        const rsg_ball_state& b = in[i];
+        float f[18];
+        std::memcpy(f, b.pos, 12);
+        std::memcpy(f + 3, b.rot, 36);
+        std::memcpy(f + 12, b.vel, 12);
+        std::memcpy(f + 15, b.ang_vel, 12);
+        for (int k = 0; k < 18; k++) std::memcpy(&w[k * n + i], &f[k], 4);
+        w[18 * n + i] = (uint32_t)b.tick_count_since_update;
+        w[19 * n + i] = (uint32_t)(b.tick_count_since_update >> 32);
+    }
+    if (pred_wait(p) != 0) return -1;  // the previous update may still read the staging buffers
+    HIP_OK(hipMemcpy(p->cur, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+    if (ticks) HIP_OK(hipMemcpy(p->ticks, ticks, n * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+void pred_row_to_api(const uint32_t* r, size_t s, rsg_ball_state& o) {
+    std::memset(&o, 0, sizeof(o));
+    float f[18];
+    for (int k = 0; k < 18; k++) std::memcpy(&f[k], &r[k * s], 4);
+    std::memcpy(o.pos, f, 12);
+    std::memcpy(o.rot, f + 3, 36);
+    std::memcpy(o.vel, f + 12, 12);
+    std::memcpy(o.ang_vel, f + 15, 12);
+    o.tick_count_since_update = (uint64_t)r[18 * s] | ((uint64_t)r[19 * s] << 32);
+}
+
+// One tracker's ring -> predData order.
+int pred_read(rsg_ball_pred* p, int tracker, rsg_ball_state* out, int max_out, int* size_out) {
+    if (pred_wait(p) != 0) return -1;
+    int32_t hs[2];
+    HIP_OK(hipMemcpy(&hs[0], p->meta + tracker, 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(&hs[1], p->meta + p->n + tracker, 4, hipMemcpyDeviceToHost));
+    const int head = hs[0], size = hs[1];
+    *size_out = size;
+    int m = size < max_out ? size : max_out;
+    if (m <= 0) return 0;
+    // the tracker's column: T * RSG_PRED_WORDS words at a pitch of n words
+    std::vector<uint32_t> col((size_t)p->T * RSG_PRED_WORDS);
+    HIP_OK(hipMemcpy2D(col.data(), 4, p->traj + tracker, (size_t)p->n * 4, 4, (size_t)p->T * RSG_PRED_WORDS,
+                       hipMemcpyDeviceToHost));
+    for (int t = 0; t < m; t++) {
+        int row = (head + t) % p->T;
+        pred_row_to_api(col.data() + (size_t)row * RSG_PRED_WORDS, 1, out[t]);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+rsg_ball_pred* rsg_ball_pred_create(int mode, int n, int num_pred_ticks, float tick_rate) {
+    if (n <= 0) { fail("rsg_ball_pred_create: n must be > 0"); return nullptr; }
+    if (num_pred_ticks < 1) { fail("rsg_ball_pred_create: num_pred_ticks must be >= 1"); return nullptr; }
+    if ((size_t)num_pred_ticks * RSG_PRED_WORDS * (size_t)n > ((size_t)1 << 36)) {
+        fail("rsg_ball_pred_create: prediction buffer above 256 GiB");
+        return nullptr;
+    }
+    std::vector<int> zeros((size_t)n, 0);
+    rsg_batch_desc desc;
+    std::memset(&desc, 0, sizeof(desc));
+    desc.n_arenas = n;
+    desc.mode = mode;
+    desc.tick_rate = tick_rate;
+    desc.cars_per_arena = zeros.data();
+    rsg_batch* a = rsg_batch_create(&desc);
+    if (!a) return nullptr;
+    auto* p = new rsg_ball_pred();
+    p->arena = a;
+    p->n = n;
+    p->T = num_pred_ticks;
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&p->traj, (size_t)num_pred_ticks * RSG_PRED_WORDS * n * 4)) != hipSuccess ||
+        (e = hipMalloc(&p->meta, (size_t)3 * n * 4)) != hipSuccess ||
+        (e = hipMalloc(&p->cur, (size_t)RSG_PRED_WORDS * n * 4)) != hipSuccess ||
+        (e = hipMalloc(&p->ticks, (size_t)n * 4)) != hipSuccess ||
+        (e = hipMemset(p->meta, 0, (size_t)3 * n * 4)) != hipSuccess) {
+        fail(std::string("rsg_ball_pred_create: ") + hipGetErrorString(e));
+        rsg_ball_pred_destroy(p);
+        return nullptr;
+    }
+    return p;
+}
+
+void rsg_ball_pred_destroy(rsg_ball_pred* p) {
+    if (!p) return;
+    (void)hipDeviceSynchronize();
+    if (p->traj) (void)hipFree(p->traj);
+    if (p->meta) (void)hipFree(p->meta);
+    if (p->cur) (void)hipFree(p->cur);
+    if (p->ticks) (void)hipFree(p->ticks);
+    rsg_batch_destroy(p->arena);
+    delete p;
+}
+
+int rsg_ball_pred_force_update(rsg_ball_pred* p, const rsg_ball_state* initial, void* stream) {
+    if (!p || !initial) return fail("rsg_ball_pred_force_update: null argument");
+    if (pred_stage(p, initial, nullptr) != 0) return -1;
+    return pred_launch(p, PRED_FORCE, nullptr, (hipStream_t)stream);
+}
+
+int rsg_ball_pred_update_manual(rsg_ball_pred* p, const rsg_ball_state* cur, const int32_t* ticks_since_last_update,
+                                void* stream) {
+    if (!p || !cur || !ticks_since_last_update) return fail("rsg_ball_pred_update_manual: null argument");
+    if (pred_stage(p, cur, ticks_since_last_update) != 0) return -1;
+    return pred_launch(p, PRED_MANUAL, nullptr, (hipStream_t)stream);
+}
+
+int rsg_ball_pred_update_from_batch(rsg_ball_pred* p, rsg_batch* b, void* stream) {
+    if (!p || !b) return fail("rsg_ball_pred_update_from_batch: null argument");
+    if (b->dev.n != p->n || b->dev.mode != p->arena->dev.mode || b->dev.tick_time != p->arena->dev.tick_time)
+        return fail("rsg_ball_pred_update_from_batch: the batch's arena count, mode or tick rate differs");
+    if (upload(b) != 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    // the batch's state may have been written on another stream
+    if (b->last_stream && b->last_stream != s) HIP_OK(hipStreamSynchronize(b->last_stream));
+    return pred_launch(p, PRED_FROM_BATCH, &b->dev, s);
+}
+
+int rsg_ball_pred_get(rsg_ball_pred* p, int tracker, rsg_ball_state* out, int max_out) {
+    if (!p || (!out && max_out > 0)) return fail("rsg_ball_pred_get: null argument");
+    if (tracker < 0 || tracker >= p->n) return fail("rsg_ball_pred_get: tracker out of range");
+    int size = 0;
+    if (pred_read(p, tracker, out, max_out, &size) != 0) return -1;
+    return size;
+}
+
+int rsg_ball_pred_get_all(rsg_ball_pred* p, rsg_ball_state* out) {
+    if (!p || !out) return fail("rsg_ball_pred_get_all: null argument");
+    if (pred_wait(p) != 0) return -1;
+    const size_t n = (size_t)p->n, T = (size_t)p->T;
+    std::vector<uint32_t> traj(T * RSG_PRED_WORDS * n);
+    std::vector<int32_t> meta(2 * n);
+    HIP_OK(hipMemcpy(traj.data(), p->traj, traj.size() * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(meta.data(), p->meta, meta.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        for (size_t t = 0; t < T; t++) {
+            rsg_ball_state& o = out[i * T + t];
+            if ((int)t >= meta[n + i]) { std::memset(&o, 0, sizeof(o)); continue; }
+            size_t row = ((size_t)meta[i] + t) % T;
+            pred_row_to_api(traj.data() + row * RSG_PRED_WORDS * n + i, n, o);
+        }
+    }
+    return 0;
+}
+
+int rsg_ball_pred_state_for_time(rsg_ball_pred* p, int tracker, float pred_time, rsg_ball_state* out) {
+    if (!p || !out) return fail("rsg_ball_pred_state_for_time: null argument");
+    if (tracker < 0 || tracker >= p->n) return fail("rsg_ball_pred_state_for_time: tracker out of range");
+    if (pred_wait(p) != 0) return -1;
+    int32_t size = 0;
+    HIP_OK(hipMemcpy(&size, p->meta + p->n + tracker, 4, hipMemcpyDeviceToHost));
+    if (size <= 0)
+        return fail("rsg_ball_pred_state_for_time: predicted ball data is empty, update the prediction first");
+    // RS_CLAMP(predTime / tickTime, 0, size - 1) in float, truncated to int
+    float x = pred_time / p->arena->dev.tick_time;
+    float lo = x > 0 ? x : 0;
+    float hi = (float)(size - 1);
+    int index = (int)(lo < hi ? lo : hi);
+    std::vector<rsg_ball_state> tmp((size_t)index + 1);
+    int got = 0;
+    if (pred_read(p, tracker, tmp.data(), index + 1, &got) != 0) return -1;
+    *out = tmp[(size_t)index];
+    return 0;
+}
+
+int rsg_ball_pred_set_launch(rsg_ball_pred* p, int trackers_per_wave) {
+    if (!p) return fail("null tracker");
+    if (trackers_per_wave < 0 || trackers_per_wave > 64)
+        return fail("rsg_ball_pred_set_launch: trackers_per_wave must be within [0, 64]");
+    p->per_wave = trackers_per_wave;
+    return 0;
+}
+
+int rsg_ball_pred_view(rsg_ball_pred* p, rsg_ball_pred_view_t* out) {
+    if (!p || !out) return fail("rsg_ball_pred_view: null argument");
+    out->traj = reinterpret_cast<const float*>(p->traj);
+    out->head = p->meta;
+    out->size = p->meta + p->n;
+    out->n = p->n;
+    out->num_pred_ticks = p->T;
     return 0;
 }
 

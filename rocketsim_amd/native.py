@@ -6,7 +6,7 @@ entry point raises.
 import ctypes as C
 import os
 
-from .state import BallState, BatchDesc, CarState, Controls, Event, PadState
+from .state import BallPredView, BallState, BatchDesc, CarState, Controls, Event, PadState
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "librsg.so")
@@ -64,6 +64,18 @@ def lib():
                                           C.POINTER(C.c_uint64)]
         L.rsg_batch_reset_kickoff.argtypes = [C.c_void_p, C.c_int, C.c_uint64]
         L.rsg_batch_is_ball_scored.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+        L.rsg_ball_pred_create.restype = C.c_void_p
+        L.rsg_ball_pred_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float]
+        L.rsg_ball_pred_destroy.argtypes = [C.c_void_p]
+        L.rsg_ball_pred_force_update.argtypes = [C.c_void_p, C.POINTER(BallState), C.c_void_p]
+        L.rsg_ball_pred_update_manual.argtypes = [C.c_void_p, C.POINTER(BallState), C.POINTER(C.c_int32),
+                                                  C.c_void_p]
+        L.rsg_ball_pred_update_from_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rsg_ball_pred_get.argtypes = [C.c_void_p, C.c_int, C.POINTER(BallState), C.c_int]
+        L.rsg_ball_pred_get_all.argtypes = [C.c_void_p, C.POINTER(BallState)]
+        L.rsg_ball_pred_state_for_time.argtypes = [C.c_void_p, C.c_int, C.c_float, C.POINTER(BallState)]
+        L.rsg_ball_pred_view.argtypes = [C.c_void_p, C.POINTER(BallPredView)]
+        L.rsg_ball_pred_set_launch.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -87,4 +99,7 @@ EXPORTED = [
     "rsg_batch_enable_events", "rsg_batch_poll_events", "rsg_air_rollout", "rsg_air_rollout_backward",
     "rsg_batch_get_tick_counts", "rsg_batch_set_tick_counts", "rsg_batch_get_goals",
     "rsg_batch_reset_kickoff", "rsg_batch_is_ball_scored",
+    "rsg_ball_pred_create", "rsg_ball_pred_destroy", "rsg_ball_pred_force_update", "rsg_ball_pred_update_manual",
+    "rsg_ball_pred_update_from_batch", "rsg_ball_pred_get", "rsg_ball_pred_get_all",
+    "rsg_ball_pred_state_for_time", "rsg_ball_pred_view", "rsg_ball_pred_set_launch",
 ]

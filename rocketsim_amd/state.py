@@ -94,6 +94,15 @@ class BallState(C.Structure):
     ]
 
 
+# Ball prediction record words (include/rsg_state.h RSG_PRED_*)
+PRED_POS, PRED_ROT, PRED_VEL, PRED_ANG_VEL, PRED_TICKS_SINCE_UPDATE, PRED_WORDS = 0, 3, 12, 15, 18, 20
+
+
+class BallPredView(C.Structure):
+    _fields_ = [("traj", C.c_void_p), ("head", C.c_void_p), ("size", C.c_void_p),
+                ("n", C.c_int), ("num_pred_ticks", C.c_int)]
+
+
 class PadState(C.Structure):
     _fields_ = [("is_active", C.c_uint32), ("cooldown", C.c_float), ("prev_locked_car_id", C.c_uint32)]
 

@@ -1,4 +1,5 @@
-"""Writes the teacher-forced one-tick fixtures under tests/golden/ from the
+"""Writes the fixtures under tests/golden/ (teacher-forced one-tick samples
+and ball-prediction tracker runs) from the
 oracle (the reference RocketSim built from source, oracle/_ref/librsref.so).
 
     python tests/golden/make_fixtures.py
@@ -16,6 +17,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
 from oracle import ref  # noqa: E402
+import ball_pred_cases  # noqa: E402
 import parity  # noqa: E402
 
 
@@ -37,6 +39,10 @@ def main():
                                        near_walls=True)
     parity.save_samples(os.path.join(HERE, "soccar_walls_tf.npz"), walls)
     print("soccar_walls_tf.npz:", len(walls), "samples")
+    states = ball_pred_cases.initial_states(ref, 10, seed=20261207)
+    run = ball_pred_cases.reference_run(ref, states, 120)
+    ball_pred_cases.save(os.path.join(HERE, "soccar_ball_pred.npz"), run, 120)
+    print("soccar_ball_pred.npz:", len(states), "trackers")
 
 
 if __name__ == "__main__":

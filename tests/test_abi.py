@@ -10,7 +10,8 @@ import tempfile
 import pytest
 
 from rocketsim_amd import native
-from rocketsim_amd.state import BallState, BatchDesc, CarState, Controls, Event, PadState
+from rocketsim_amd import state
+from rocketsim_amd.state import BallPredView, BallState, BatchDesc, CarState, Controls, Event, PadState
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -32,7 +33,12 @@ def test_library_exports_all_symbols():
 
 
 RECORDS = {"rsg_controls": Controls, "rsg_car_state": CarState, "rsg_ball_state": BallState,
-           "rsg_pad_state": PadState, "rsg_batch_desc": BatchDesc, "rsg_event": Event}
+           "rsg_pad_state": PadState, "rsg_batch_desc": BatchDesc, "rsg_event": Event,
+           "rsg_ball_pred_view_t": BallPredView}
+# enum constants of include/rsg_state.h mirrored in rocketsim_amd.state
+CONSTANTS = {"RSG_PRED_POS": "PRED_POS", "RSG_PRED_ROT": "PRED_ROT", "RSG_PRED_VEL": "PRED_VEL",
+             "RSG_PRED_ANG_VEL": "PRED_ANG_VEL", "RSG_PRED_TICKS_SINCE_UPDATE": "PRED_TICKS_SINCE_UPDATE",
+             "RSG_PRED_WORDS": "PRED_WORDS"}
 
 
 def test_ctypes_layout_matches_c_header():
@@ -41,6 +47,8 @@ def test_ctypes_layout_matches_c_header():
         lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
         for fname, _ in py._fields_:
             lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    for cname, pyname in CONSTANTS.items():
+        lines.append(f'printf("const {cname} %d\\n", (int){cname});')
     lines.append("return 0;}")
     with tempfile.TemporaryDirectory() as d:
         src = os.path.join(d, "layout.c")
@@ -53,6 +61,9 @@ def test_ctypes_layout_matches_c_header():
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
     for line in out.strip().splitlines():
         cname, field, value = line.split()
+        if cname == "const":
+            assert getattr(state, CONSTANTS[field]) == int(value), (field, value)
+            continue
         py = RECORDS[cname]
         got = C.sizeof(py) if field == "sizeof" else getattr(py, field).offset
         assert got == int(value), (cname, field, got, value)
