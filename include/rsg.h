@@ -172,6 +172,34 @@ int rsg_air_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, 
 /* Arena::IsBallScored (Arena.cpp:893-901) per arena. */
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
 
+/* ---- Snapshots in the reference's byte format (Arena::Serialize,
+ * src/Sim/Arena/Arena.cpp:546-573; Arena::DeserializeNew, :575-644).
+ *
+ * rsg_snapshot_serialize writes the reference's bytes for one arena, then an
+ * extension block the reference's reader ignores, holding the full state
+ * records (hidden per-wheel values, flags, counters, broadphase records), so
+ * the reference can load the file and this library restores it exactly.
+ * Returns the byte count; writes only when cap is large enough (call with
+ * out = NULL to size the buffer).  Host work only, no GPU. */
+int64_t rsg_snapshot_serialize(const rsg_arena_snapshot* s, uint8_t* out, uint64_t cap);
+
+/* Reads the reference's bytes (with or without the extension block).  Only
+ * SOCCAR and THE_VOID with the default ArenaConfig, MutatorConfig and preset
+ * car configs are accepted; anything else is an error.  Without the
+ * extension, states are as Car/Ball::SetState leave them in a new arena
+ * (tick counters and per-wheel values 0, broadphase records of newly created
+ * proxies).  Host work only. */
+int rsg_snapshot_deserialize(const uint8_t* data, uint64_t size, rsg_arena_snapshot* out);
+
+/* One arena of a batch as a snapshot: cars in its car update order. */
+int rsg_batch_get_snapshot(rsg_batch* b, int arena, rsg_arena_snapshot* out);
+
+/* Loads a snapshot into an arena slot.  Mode, tick time and car count must
+ * match the slot; its cars, taken in ascending id order (the reference's
+ * creation order), must have the slots' teams and presets.  The snapshot's
+ * car order becomes the arena's car update order and its ids the cars' ids. */
+int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s);
+
 /* ---- Ball prediction: a batch of n BallPredTracker
  * (src/Sim/BallPredTracker/BallPredTracker.h:7-36).  Tracker i owns a
  * ball-only arena of `mode` (the reference's tracker makes its own car-less

@@ -111,6 +111,25 @@ public:
     std::vector<uint8_t> IsBallScored() { std::vector<uint8_t> v(NumArenas()); check(rsg_batch_is_ball_scored(h_, v.data())); return v; }
     void SetLaunch(int arenas_per_block) { check(rsg_batch_set_launch(h_, arenas_per_block)); }
 
+    // Arena::Serialize (Arena.cpp:546-573) of one arena: the reference's bytes
+    // plus this library's extension block.
+    std::vector<uint8_t> Serialize(int arena) {
+        rsg_arena_snapshot s;
+        check(rsg_batch_get_snapshot(h_, arena, &s));
+        int64_t n = rsg_snapshot_serialize(&s, nullptr, 0);
+        if (n < 0) check((int)n);
+        std::vector<uint8_t> out((size_t)n);
+        if (rsg_snapshot_serialize(&s, out.data(), out.size()) < 0) check(-1);
+        return out;
+    }
+    // Arena::DeserializeNew (Arena.cpp:575-644) into an arena slot of the
+    // same mode, tick rate, car count, teams and presets.
+    void Deserialize(int arena, const std::vector<uint8_t>& bytes) {
+        rsg_arena_snapshot s;
+        check(rsg_snapshot_deserialize(bytes.data(), bytes.size(), &s));
+        check(rsg_batch_set_snapshot(h_, arena, &s));
+    }
+
     rsg_batch* handle() const { return h_; }
 
 private:
@@ -140,6 +159,8 @@ public:
     uint64_t TickCount() { return b_.GetTickCounts()[0]; }
     void ResetToRandomKickoff(int seed = -1) { b_.ResetToRandomKickoff(0, (uint64_t)(int64_t)seed); }
     bool IsBallScored() { return b_.IsBallScored()[0] != 0; }
+    std::vector<uint8_t> Serialize() { return b_.Serialize(0); }
+    void Deserialize(const std::vector<uint8_t>& bytes) { b_.Deserialize(0, bytes); }
     ArenaBatch& Batch() { return b_; }
 
 private:

@@ -123,7 +123,7 @@ typedef struct rsg_pad_state {
     uint32_t prev_locked_car_id;
 } rsg_pad_state;
 
-enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8 };
+enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8, RSG_SNAPSHOT_RAW_WORDS = 1024 };
 
 /* A game event, replacing the reference's callbacks: the goal-score callback
  * (Arena::SetGoalScoreCallback, src/Sim/Arena/Arena.h:24,102; fired at the
@@ -131,6 +131,37 @@ enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8 };
  * (Arena::SetCarBumpCallback, Arena.h:25,108; fired for bumps and
  * demolitions during collision handling, Arena.cpp:377-400). */
 enum { RSG_EVENT_GOAL = 1, RSG_EVENT_BUMP = 2, RSG_EVENT_DEMO = 3 };
+
+/* Everything Arena::Serialize writes for one arena (src/Sim/Arena/Arena.cpp:
+ * 546-573), plus the state records' hidden values: the unit of the
+ * reference's snapshot format (rsg_snapshot_serialize / _deserialize). */
+typedef struct rsg_arena_snapshot {
+    int32_t mode;                       /* RSG_MODE_* (GameMode) */
+    float tick_time;                    /* Arena::tickTime, seconds */
+    uint64_t tick_count;                /* Arena::tickCount */
+    uint32_t last_car_id;               /* Arena::_lastCarID */
+    int32_t n_cars;
+    /* cars in the arena's car update order (the reference's _cars iteration
+     * order, which Serialize also follows) */
+    uint32_t car_id[RSG_MAX_CARS];
+    uint8_t car_team[RSG_MAX_CARS];
+    uint8_t car_preset[RSG_MAX_CARS];  /* RSG_PRESET_* */
+    uint8_t _pad[6];
+    rsg_controls controls[RSG_MAX_CARS];   /* Car::controls */
+    rsg_car_state cars[RSG_MAX_CARS];
+    rsg_ball_state ball;
+    rsg_pad_state pads[RSG_NUM_PADS];
+    /* This library's exact records of the arena, opaque: positions and
+     * velocities in its internal units, the values the reference keeps
+     * outside CarState / BallState (per-wheel vehicle values, contact flags,
+     * tick counters, broadphase records) and the arena's counters.  Filled by
+     * rsg_batch_get_snapshot and by reading bytes that carry the extension
+     * block (raw_words = 0 otherwise).  When present, rsg_batch_set_snapshot
+     * restores from them, so a snapshot resumes bit for bit. */
+    uint32_t raw_layout;
+    uint32_t raw_words;
+    uint32_t raw[RSG_SNAPSHOT_RAW_WORDS];
+} rsg_arena_snapshot;
 
 /* Ball prediction records on the device (rsg_ball_pred_view): one row of
  * RSG_PRED_WORDS word planes per predicted tick, in the public units of

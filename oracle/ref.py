@@ -63,6 +63,14 @@ def lib():
         L.rsref_ball_pred_force.argtypes = [C.c_void_p, C.POINTER(BallState)]
         L.rsref_ball_pred_get.argtypes = [C.c_void_p, C.POINTER(BallState), C.c_int]
         L.rsref_ball_pred_for_time.argtypes = [C.c_void_p, C.c_float, C.POINTER(BallState)]
+        L.rsref_serialize.restype = C.c_int64
+        L.rsref_serialize.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+        L.rsref_deserialize.restype = C.c_void_p
+        L.rsref_deserialize.argtypes = [C.c_char_p, C.c_uint64]
+        L.rsref_last_car_id.restype = C.c_uint32
+        L.rsref_last_car_id.argtypes = [C.c_void_p]
+        L.rsref_car_team.argtypes = [C.c_void_p, C.c_int]
+        L.rsref_get_controls.argtypes = [C.c_void_p, C.c_int, C.POINTER(Controls)]
         L.rsref_run_workload.restype = C.c_double
         L.rsref_run_workload.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.c_uint64, C.POINTER(C.c_uint64)]
@@ -150,6 +158,34 @@ class RefArena:
         out = (Event * 256)()
         n = self.L.rsref_pop_events(self.h, out, 256)
         return list(out[:n])
+
+    def serialize(self):
+        """Arena::Serialize bytes (src/Sim/Arena/Arena.cpp:546-573)."""
+        n = self.L.rsref_serialize(self.h, None, 0)
+        buf = (C.c_uint8 * n)()
+        self.L.rsref_serialize(self.h, buf, n)
+        return bytes(buf)
+
+    @classmethod
+    def deserialize(cls, data):
+        """Arena::DeserializeNew (Arena.cpp:575-644); cars in id order."""
+        self = cls.__new__(cls)
+        self.L = lib()
+        self.h = self.L.rsref_deserialize(bytes(data), len(data))
+        if not self.h:
+            raise RuntimeError(self.L.rsref_last_error().decode())
+        return self
+
+    def last_car_id(self):
+        return self.L.rsref_last_car_id(self.h)
+
+    def car_team(self, i):
+        return self.L.rsref_car_team(self.h, i)
+
+    def get_controls(self, i):
+        c = Controls()
+        self.L.rsref_get_controls(self.h, i, C.byref(c))
+        return c
 
     def car_order(self):
         """Iteration order of the reference's car set (creation indices)."""

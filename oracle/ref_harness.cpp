@@ -31,6 +31,7 @@
 #include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btGjkEpaPenetrationDepthSolver.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btSubSimplexConvexCast.h"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <thread>
@@ -835,3 +836,43 @@ extern "C" int rsref_ball_pred_get(void* p, rsg_ball_state* out, int max_out) {
 extern "C" void rsref_ball_pred_for_time(void* p, float pred_time, rsg_ball_state* out) {
     ball_state_to(((BallPredTracker*)p)->GetBallStateForTime(pred_time), out);
 }
+
+// ---------------------------------------------------------------------------
+// Arena::Serialize / Arena::DeserializeNew (src/Sim/Arena/Arena.cpp:546-644).
+
+// Serialized bytes of the arena; copies them when cap suffices, returns the size.
+extern "C" int64_t rsref_serialize(void* h, uint8_t* out, uint64_t cap) {
+    DataStreamOut s;
+    ((RefArena*)h)->arena->Serialize(s);
+    if (out && cap >= s.data.size()) std::memcpy(out, s.data.data(), s.data.size());
+    return (int64_t)s.data.size();
+}
+
+// A new harness arena from serialized bytes (cars in file order = creation order).
+extern "C" void* rsref_deserialize(const uint8_t* data, uint64_t size) {
+    try {
+        DataStreamIn s;
+        s.data.assign(data, data + size);
+        auto* ra = new RefArena();
+        ra->arena = Arena::DeserializeNew(s);
+        ra->arena->_bulletWorld.getSolverInfo().m_timeStep = ra->arena->tickTime;
+        ra->bp_cell.push_back(proxy_cell(ra, 0));
+        ra->bp_seq.push_back(0);
+        // file order: the _carIDMap is keyed by id; recover creation order by id
+        std::vector<Car*> cars(ra->arena->_cars.begin(), ra->arena->_cars.end());
+        std::sort(cars.begin(), cars.end(), [](Car* a, Car* b) { return a->id < b->id; });
+        for (Car* c : cars) {
+            ra->cars.push_back(c);
+            ra->bp_cell.push_back(proxy_cell(ra, (int)ra->cars.size()));
+            ra->bp_seq.push_back(++ra->bp_counter);
+        }
+        return ra;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+extern "C" uint32_t rsref_last_car_id(void* h) { return ((RefArena*)h)->arena->_lastCarID; }
+extern "C" int rsref_car_team(void* h, int i) { return (int)((RefArena*)h)->cars[i]->team; }
+extern "C" void rsref_get_controls(void* h, int i, rsg_controls* c) { cget(((RefArena*)h)->cars[i]->controls, *c); }

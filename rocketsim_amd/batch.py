@@ -8,7 +8,7 @@ the get/set methods replace Car/Ball/BoostPad Get/SetState.
 import ctypes as C
 
 from . import native
-from .state import (BallState, BatchDesc, CarState, Controls, Event, MODE_SOCCAR, MODE_THE_VOID,
+from .state import (ArenaSnapshot, BallState, BatchDesc, CarState, Controls, Event, MODE_SOCCAR, MODE_THE_VOID,
                     NUM_PADS, PadState)
 
 
@@ -26,6 +26,23 @@ def init_mesh(cmf=None):
     buf = C.create_string_buffer(bytes(cmf), len(cmf))
     native.check(native.lib().rsg_init_mesh(buf, len(cmf)))
     _mesh_loaded = True
+
+
+def serialize_snapshot(snap):
+    """Arena::Serialize (src/Sim/Arena/Arena.cpp:546-573) bytes of an
+    ArenaSnapshot, followed by this library's extension block."""
+    L = native.lib()
+    n = native.check(L.rsg_snapshot_serialize(C.byref(snap), None, 0))
+    buf = (C.c_uint8 * n)()
+    native.check(L.rsg_snapshot_serialize(C.byref(snap), buf, n))
+    return bytes(buf)
+
+
+def deserialize_snapshot(data):
+    """Arena::DeserializeNew (Arena.cpp:575-644) of snapshot bytes."""
+    out = ArenaSnapshot()
+    native.check(native.lib().rsg_snapshot_deserialize(bytes(data), len(data), C.byref(out)))
+    return out
 
 
 class ArenaBatch:
@@ -61,6 +78,22 @@ class ArenaBatch:
         per = n_blue + n_orange
         teams = ([0] * n_blue + [1] * n_orange) * n_arenas
         return cls([per] * n_arenas, teams, [preset] * (per * n_arenas), **kw)
+
+    # -- snapshots (Arena::Serialize / DeserializeNew) ---------------------
+    def get_snapshot(self, arena):
+        out = ArenaSnapshot()
+        native.check(self.L.rsg_batch_get_snapshot(self.h, arena, C.byref(out)))
+        return out
+
+    def set_snapshot(self, arena, snap):
+        native.check(self.L.rsg_batch_set_snapshot(self.h, arena, C.byref(snap)))
+
+    def serialize_arena(self, arena):
+        """The arena in the reference's snapshot byte format."""
+        return serialize_snapshot(self.get_snapshot(arena))
+
+    def deserialize_arena(self, arena, data):
+        self.set_snapshot(arena, deserialize_snapshot(data))
 
     # -- stepping ---------------------------------------------------------
     def step(self, ticks=1, stream=None):

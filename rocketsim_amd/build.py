@@ -15,8 +15,8 @@ LIB_PATH = os.path.join(LIBDIR, "librsg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off keeps a*b+c as two rounded operations like the reference's
-# x86-64 build; packed FP32 instructions are disabled (they are not trusted on
-# this hardware pool).
+# x86-64 build; packed FP32 instructions are disabled (some of their forms have
+# given wrong results on gfx950 without an error).
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -30,6 +30,7 @@
 #include "rsg_api_convert.h"
 #include "rsg_adjoint.h"
 #include "rsg_ball_pred.h"
+#include "rsg_serialize.h"
 
 using namespace rsg;
 
@@ -960,6 +961,141 @@ int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out) {
         out[a] = (b->dev.mode == RSG_MODE_SOCCAR &&
                   fabsf(y) > (SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y + BALL_COLLISION_RADIUS_SOCCAR)) ? 1 : 0;
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Snapshots (Arena::Serialize / DeserializeNew)
+
+int64_t rsg_snapshot_serialize(const rsg_arena_snapshot* s, uint8_t* out, uint64_t cap) {
+    if (!s) return fail("rsg_snapshot_serialize: null snapshot");
+    if (s->raw_words > RSG_SNAPSHOT_RAW_WORDS) return fail("rsg_snapshot_serialize: raw_words out of range");
+    std::vector<uint8_t> bytes;
+    std::string err = snapshot_serialize(*s, bytes);
+    if (!err.empty()) return fail("rsg_snapshot_serialize: " + err);
+    if (out && cap >= bytes.size()) std::memcpy(out, bytes.data(), bytes.size());
+    return (int64_t)bytes.size();
+}
+
+int rsg_snapshot_deserialize(const uint8_t* data, uint64_t size, rsg_arena_snapshot* out) {
+    if (!data || !out) return fail("rsg_snapshot_deserialize: null argument");
+    std::string err = snapshot_deserialize(data, (size_t)size, *out);
+    if (!err.empty()) return fail("rsg_snapshot_deserialize: " + err);
+    return 0;
+}
+
+int rsg_batch_get_snapshot(rsg_batch* b, int arena, rsg_arena_snapshot* out) {
+    if (!b || !out) return fail("rsg_batch_get_snapshot: null argument");
+    if (arena < 0 || arena >= b->dev.n) return fail("rsg_batch_get_snapshot: arena out of range");
+    if (download(b) != 0) return -1;
+    BatchDev h = host_view(b);
+    std::memset(out, 0, sizeof(*out));
+    CarRef ar = arena_ref(h, arena);
+    out->mode = b->dev.mode;
+    out->tick_time = b->dev.tick_time;
+    out->tick_count = ar.u64(AW_TICK);
+    const int n = b->cars_per_arena[arena];
+    out->n_cars = n;
+    uint32_t ow = ar.u(AW_CAR_ORDER), last_id = 0;
+    for (int k = 0; k < n; k++) {
+        int slot = (int)((ow >> (4 * k)) & 0xFu);
+        CarRef cr = car_ref(h, arena, slot);
+        uint32_t cfg = cr.u(CW_CONFIG);
+        out->car_id[k] = cr.u(CW_ID);
+        out->car_preset[k] = (uint8_t)(cfg & 0xFF);
+        out->car_team[k] = (uint8_t)((cfg >> 8) & 0xFF);
+        out->controls[k] = ctrl_to_api(load_ctrl(cr, CW_CTRL));
+        car_to_api(cr, out->cars[k]);
+        last_id = out->car_id[k] > last_id ? out->car_id[k] : last_id;
+    }
+    out->last_car_id = last_id;
+    ball_to_api(ball_ref(h, arena), out->ball);
+    if (b->dev.mode == RSG_MODE_SOCCAR)
+        for (int p = 0; p < PAD_NUM; p++) pad_to_api(pad_ref(h, arena, p), out->pads[p]);
+    // raw records, cars in snapshot order
+    BatchDev v = raw_view(out->raw, n);
+    for (int k = 0; k < n; k++) {
+        CarRef src = car_ref(h, arena, (int)((ow >> (4 * k)) & 0xFu)), dst = car_ref(v, 0, k);
+        for (int w = 0; w < CAR_WORDS; w++) dst.u(w) = src.u(w);
+    }
+    {
+        CarRef src = ball_ref(h, arena), dst = ball_ref(v, 0);
+        for (int w = 0; w < BALL_WORDS; w++) dst.u(w) = src.u(w);
+        CarRef sa = arena_ref(h, arena), da = arena_ref(v, 0);
+        for (int w = 0; w < ARENA_WORDS; w++) da.u(w) = sa.u(w);
+        uint32_t ident = 0;
+        for (int k = 0; k < n; k++) ident |= (uint32_t)k << (4 * k);
+        da.u(AW_CAR_ORDER) = ident;
+    }
+    out->raw_layout = raw_layout_tag();
+    out->raw_words = raw_words_for(n);
+    return 0;
+}
+
+int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s) {
+    if (!b || !s) return fail("rsg_batch_set_snapshot: null argument");
+    if (arena < 0 || arena >= b->dev.n) return fail("rsg_batch_set_snapshot: arena out of range");
+    if (s->mode != b->dev.mode) return fail("rsg_batch_set_snapshot: game mode differs from the batch's");
+    if (s->tick_time != b->dev.tick_time) return fail("rsg_batch_set_snapshot: tick time differs from the batch's");
+    const int n = b->cars_per_arena[arena];
+    if (s->n_cars != n) return fail("rsg_batch_set_snapshot: car count differs from the arena slot's");
+    // Cars go to the slots in creation (ascending id) order; each must have
+    // the slot's team and preset.
+    int by_id[RSG_MAX_CARS], slot_of[RSG_MAX_CARS];
+    for (int k = 0; k < n; k++) by_id[k] = k;
+    std::sort(by_id, by_id + n, [&](int x, int y) { return s->car_id[x] < s->car_id[y]; });
+    for (int i = 0; i < n; i++) {
+        int k = by_id[i];
+        if (i > 0 && s->car_id[k] == s->car_id[by_id[i - 1]]) return fail("rsg_batch_set_snapshot: repeated car id");
+        int g = b->car_offset[arena] + i;
+        if (s->car_team[k] != b->team[g] || s->car_preset[k] != b->preset[g])
+            return fail("rsg_batch_set_snapshot: car " + std::to_string(s->car_id[k]) +
+                        " has another team or preset than slot " + std::to_string(i));
+        slot_of[k] = i;
+    }
+    const bool raw = s->raw_words > 0;
+    if (raw && (s->raw_layout != raw_layout_tag() || s->raw_words != raw_words_for(n)))
+        return fail("rsg_batch_set_snapshot: raw records of another layout or car count");
+    std::vector<uint32_t> rw(s->raw, s->raw + (raw ? s->raw_words : 0));
+    BatchDev v = raw_view(rw.data(), n);
+    if (raw)
+        for (int k = 0; k < n; k++) {
+            CarRef rc = car_ref(v, 0, k);
+            uint32_t cfg = rc.u(CW_CONFIG);
+            if (rc.u(CW_ID) != s->car_id[k] || (cfg & 0xFF) != s->car_preset[k] ||
+                ((cfg >> 8) & 0xFF) != s->car_team[k])
+                return fail("rsg_batch_set_snapshot: raw records disagree with the snapshot's cars");
+        }
+    if (download(b) != 0) return -1;
+    BatchDev h = host_view(b);
+    CarRef ar = arena_ref(h, arena);
+    uint32_t ow = 0;
+    for (int k = 0; k < n; k++) {
+        ow |= (uint32_t)slot_of[k] << (4 * k);
+        CarRef cr = car_ref(h, arena, slot_of[k]);
+        if (raw) {
+            CarRef rc = car_ref(v, 0, k);
+            for (int w = 0; w < CAR_WORDS; w++) cr.u(w) = rc.u(w);
+        } else {
+            car_from_api(cr, s->cars[k]);
+            cr.u(CW_ID) = s->car_id[k];
+            store_ctrl(cr, CW_CTRL, ctrl_from_api(s->controls[k]));
+        }
+    }
+    if (raw) {
+        CarRef rb = ball_ref(v, 0), db = ball_ref(h, arena);
+        for (int w = 0; w < BALL_WORDS; w++) db.u(w) = rb.u(w);
+        CarRef ra = arena_ref(v, 0);
+        for (int w = 0; w < ARENA_WORDS; w++)
+            if (w != AW_NUM_CARS && w != AW_CAR_ORDER) ar.u(w) = ra.u(w);
+    } else {
+        ar.su64(AW_TICK, s->tick_count);
+        ball_from_api(ball_ref(h, arena), s->ball);
+    }
+    ar.u(AW_CAR_ORDER) = ow;
+    if (b->dev.mode == RSG_MODE_SOCCAR)
+        for (int p = 0; p < PAD_NUM; p++) pad_from_api(pad_ref(h, arena, p), s->pads[p]);
+    b->host_dirty = true;
     return 0;
 }
 

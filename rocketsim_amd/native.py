@@ -6,7 +6,7 @@ entry point raises.
 import ctypes as C
 import os
 
-from .state import BallPredView, BallState, BatchDesc, CarState, Controls, Event, PadState
+from .state import ArenaSnapshot, BallPredView, BallState, BatchDesc, CarState, Controls, Event, PadState
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "librsg.so")
@@ -76,6 +76,11 @@ def lib():
         L.rsg_ball_pred_state_for_time.argtypes = [C.c_void_p, C.c_int, C.c_float, C.POINTER(BallState)]
         L.rsg_ball_pred_view.argtypes = [C.c_void_p, C.POINTER(BallPredView)]
         L.rsg_ball_pred_set_launch.argtypes = [C.c_void_p, C.c_int]
+        L.rsg_snapshot_serialize.restype = C.c_int64
+        L.rsg_snapshot_serialize.argtypes = [C.POINTER(ArenaSnapshot), C.c_void_p, C.c_uint64]
+        L.rsg_snapshot_deserialize.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(ArenaSnapshot)]
+        L.rsg_batch_get_snapshot.argtypes = [C.c_void_p, C.c_int, C.POINTER(ArenaSnapshot)]
+        L.rsg_batch_set_snapshot.argtypes = [C.c_void_p, C.c_int, C.POINTER(ArenaSnapshot)]
         _lib = L
     return _lib
 
@@ -102,4 +107,5 @@ EXPORTED = [
     "rsg_ball_pred_create", "rsg_ball_pred_destroy", "rsg_ball_pred_force_update", "rsg_ball_pred_update_manual",
     "rsg_ball_pred_update_from_batch", "rsg_ball_pred_get", "rsg_ball_pred_get_all",
     "rsg_ball_pred_state_for_time", "rsg_ball_pred_view", "rsg_ball_pred_set_launch",
+    "rsg_snapshot_serialize", "rsg_snapshot_deserialize", "rsg_batch_get_snapshot", "rsg_batch_set_snapshot",
 ]

@@ -16,6 +16,7 @@ PRESET_NAMES = ["octane", "dominus", "plank", "breakout", "hybrid", "merc"]
 TEAM_BLUE, TEAM_ORANGE = 0, 1
 NUM_PADS = 34
 MAX_CARS = 8
+SNAPSHOT_RAW_WORDS = 1024
 
 # rsg_car_state.flags bits
 CF_ON_GROUND = 1 << 0
@@ -105,6 +106,20 @@ class BallPredView(C.Structure):
 
 class PadState(C.Structure):
     _fields_ = [("is_active", C.c_uint32), ("cooldown", C.c_float), ("prev_locked_car_id", C.c_uint32)]
+
+
+class ArenaSnapshot(C.Structure):
+    """rsg_arena_snapshot (include/rsg_state.h): what Arena::Serialize writes
+    for one arena (src/Sim/Arena/Arena.cpp:546-573) plus the hidden values."""
+    _fields_ = [
+        ("mode", C.c_int32), ("tick_time", C.c_float), ("tick_count", C.c_uint64),
+        ("last_car_id", C.c_uint32), ("n_cars", C.c_int32),
+        ("car_id", C.c_uint32 * MAX_CARS), ("car_team", C.c_uint8 * MAX_CARS),
+        ("car_preset", C.c_uint8 * MAX_CARS), ("_pad", C.c_uint8 * 6),
+        ("controls", Controls * MAX_CARS), ("cars", CarState * MAX_CARS),
+        ("ball", BallState), ("pads", PadState * NUM_PADS),
+        ("raw_layout", C.c_uint32), ("raw_words", C.c_uint32), ("raw", C.c_uint32 * SNAPSHOT_RAW_WORDS),
+    ]
 
 
 class BatchDesc(C.Structure):

@@ -57,5 +57,6 @@ def test_facade_matches_python_layer(tmp_path):
         assert vals["ball"] == pytest.approx(list(ball.pos), abs=1e-3)
         assert vals["car0"] == pytest.approx(list(car.pos), abs=1e-3)
         assert int(vals["tick"][0]) == b.get_tick_counts()[0]
+        assert vals["resume"][0] == 1.0 and vals["resume"][1] > 0
     finally:
         b.close()
