@@ -172,6 +172,31 @@ int rsg_air_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, 
 /* Arena::IsBallScored (Arena.cpp:893-901) per arena. */
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
 
+/* ---- Game-event tracking: one GameEventTracker
+ * (src/Sim/GameEventTracker/GameEventTracker.cpp:51-166) per arena of a
+ * SOCCAR batch, on the device.  Shots, goals (scorer and passer) and saves go
+ * to the tracker's own event ring (RSG_EVENT_SHOT / _SCORER / _SAVE). */
+typedef struct rsg_game_events rsg_game_events;
+
+void rsg_game_event_config_default(rsg_game_event_config* out);
+
+/* One tracker per arena of b (n arenas, SOCCAR only: the reference's
+ * IsBallProbablyGoingIn supports no other mode here); `capacity` events are
+ * kept between polls (later ones are counted as lost). */
+rsg_game_events* rsg_game_events_create(rsg_batch* b, const rsg_game_event_config* cfg, int capacity);
+void rsg_game_events_destroy(rsg_game_events* t);
+
+/* GameEventTracker::Update (:51-158) for every arena of b, on `stream`.
+ * Call it at the interval the reference's users call Update (every tick, or
+ * at the tick-skip of an RL loop). */
+int rsg_game_events_update(rsg_game_events* t, rsg_batch* b, void* stream);
+
+/* Events since the last poll, sorted by arena and then by occurrence. */
+int rsg_game_events_poll(rsg_game_events* t, rsg_event* out, int max_out, int* n_lost);
+
+/* ResetPersistentInfo (:160-166) for one arena, or all with arena = -1. */
+int rsg_game_events_reset(rsg_game_events* t, int arena);
+
 /* ---- Snapshots in the reference's byte format (Arena::Serialize,
  * src/Sim/Arena/Arena.cpp:546-573; Arena::DeserializeNew, :575-644).
  *

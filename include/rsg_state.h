@@ -131,6 +131,23 @@ enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8, RSG_SNAPSHOT_RAW_WORDS = 1024 };
  * (Arena::SetCarBumpCallback, Arena.h:25,108; fired for bumps and
  * demolitions during collision handling, Arena.cpp:377-400). */
 enum { RSG_EVENT_GOAL = 1, RSG_EVENT_BUMP = 2, RSG_EVENT_DEMO = 3 };
+/* Game-event tracker events (GameEventTracker's shot, goal and save
+ * callbacks, src/Sim/GameEventTracker/GameEventTracker.h:6-8): `a` is the
+ * shooter / scorer / saver car id, `b` the passer's id or 0. */
+enum { RSG_EVENT_SHOT = 4, RSG_EVENT_SCORER = 5, RSG_EVENT_SAVE = 6 };
+
+/* GameEventTrackerConfig (GameEventTracker.h:10-41); the defaults are the
+ * reference's (rsg_game_event_config_default). */
+typedef struct rsg_game_event_config {
+    float shot_min_speed;            /* uu/s toward the net, 1750 */
+    float shot_touch_min_delay;      /* s since the shooter's touch, 0.3 */
+    float pred_score_extra_margin;   /* uu, 0 */
+    float shot_event_cooldown;       /* s between shots, 1 */
+    float shot_min_score_time;       /* s, 2 */
+    float goal_max_touch_time;       /* s, 4 */
+    float pass_max_touch_time;       /* s, 2 */
+    int32_t auto_state_set_detection;/* 1 */
+} rsg_game_event_config;
 
 /* Everything Arena::Serialize writes for one arena (src/Sim/Arena/Arena.cpp:
  * 546-573), plus the state records' hidden values: the unit of the

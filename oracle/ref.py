@@ -71,6 +71,13 @@ def lib():
         L.rsref_last_car_id.argtypes = [C.c_void_p]
         L.rsref_car_team.argtypes = [C.c_void_p, C.c_int]
         L.rsref_get_controls.argtypes = [C.c_void_p, C.c_int, C.POINTER(Controls)]
+        from rocketsim_amd.state import GameEventConfig
+        L.rsref_tracker_create.restype = C.c_void_p
+        L.rsref_tracker_create.argtypes = [C.POINTER(GameEventConfig)]
+        L.rsref_tracker_destroy.argtypes = [C.c_void_p]
+        L.rsref_tracker_update.argtypes = [C.c_void_p, C.c_void_p]
+        L.rsref_tracker_reset.argtypes = [C.c_void_p]
+        L.rsref_tracker_pop.argtypes = [C.c_void_p, C.POINTER(Event), C.c_int]
         L.rsref_run_workload.restype = C.c_double
         L.rsref_run_workload.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.c_uint64, C.POINTER(C.c_uint64)]
@@ -254,6 +261,38 @@ class RefBallPredTracker:
     def close(self):
         if self.h:
             self.L.rsref_ball_pred_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RefGameEventTracker:
+    """The reference's GameEventTracker (src/Sim/GameEventTracker/) with its
+    shot, goal and save callbacks recorded as Event records."""
+
+    def __init__(self, config):
+        self.L = lib()
+        self.h = self.L.rsref_tracker_create(C.byref(config))
+
+    def update(self, arena):
+        self.L.rsref_tracker_update(self.h, arena.h)
+
+    def reset_persistent_info(self):
+        self.L.rsref_tracker_reset(self.h)
+
+    def pop(self):
+        from rocketsim_amd.state import Event
+        out = (Event * 256)()
+        n = self.L.rsref_tracker_pop(self.h, out, 256)
+        return list(out[:n])
+
+    def close(self):
+        if self.h:
+            self.L.rsref_tracker_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -19,6 +19,7 @@
 
 #include "RocketSim.h"
 #include "Sim/BallPredTracker/BallPredTracker.h"
+#include "Sim/GameEventTracker/GameEventTracker.h"
 #include "rsg_state.h"
 #include "rsg_workload.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/BroadphaseCollision/btRSBroadphase.h"
@@ -876,3 +877,56 @@ extern "C" void* rsref_deserialize(const uint8_t* data, uint64_t size) {
 extern "C" uint32_t rsref_last_car_id(void* h) { return ((RefArena*)h)->arena->_lastCarID; }
 extern "C" int rsref_car_team(void* h, int i) { return (int)((RefArena*)h)->cars[i]->team; }
 extern "C" void rsref_get_controls(void* h, int i, rsg_controls* c) { cget(((RefArena*)h)->cars[i]->controls, *c); }
+
+// ---------------------------------------------------------------------------
+// GameEventTracker (src/Sim/GameEventTracker/GameEventTracker.h:44-106) with
+// its shot / goal / save callbacks recorded as rsg_event (arena 0).
+
+struct RefTracker {
+    GameEventTracker t;
+    std::vector<rsg_event> events;
+    uint32_t seq = 0;
+};
+
+static void tracker_push(RefTracker* rt, Arena* arena, uint32_t type, Car* x, Car* y) {
+    rsg_event e{};
+    e.type = type;
+    e.tick = arena->tickCount;
+    e.a = x ? x->id : 0;
+    e.b = y ? y->id : 0;
+    e.seq = rt->seq++;
+    rt->events.push_back(e);
+}
+
+extern "C" void* rsref_tracker_create(const rsg_game_event_config* c) {
+    auto* rt = new RefTracker();
+    GameEventTrackerConfig& k = rt->t.config;
+    k.shotMinSpeed = c->shot_min_speed;
+    k.shotTouchMinDelay = c->shot_touch_min_delay;
+    k.predScoreExtraMargin = c->pred_score_extra_margin;
+    k.shotEventCooldown = c->shot_event_cooldown;
+    k.shotMinScoreTime = c->shot_min_score_time;
+    k.goalMaxTouchTime = c->goal_max_touch_time;
+    k.passMaxTouchTime = c->pass_max_touch_time;
+    rt->t.autoStateSetDetection = c->auto_state_set_detection != 0;
+    rt->t.SetShotCallback([](Arena* a, Car* shooter, Car* passer, void* ui) {
+        tracker_push((RefTracker*)ui, a, RSG_EVENT_SHOT, shooter, passer);
+    }, rt);
+    rt->t.SetGoalCallback([](Arena* a, Car* scorer, Car* passer, void* ui) {
+        tracker_push((RefTracker*)ui, a, RSG_EVENT_SCORER, scorer, passer);
+    }, rt);
+    rt->t.SetSaveCallback([](Arena* a, Car* saver, void* ui) {
+        tracker_push((RefTracker*)ui, a, RSG_EVENT_SAVE, saver, nullptr);
+    }, rt);
+    return rt;
+}
+extern "C" void rsref_tracker_destroy(void* p) { delete (RefTracker*)p; }
+extern "C" void rsref_tracker_update(void* p, void* h) { ((RefTracker*)p)->t.Update(((RefArena*)h)->arena); }
+extern "C" void rsref_tracker_reset(void* p) { ((RefTracker*)p)->t.ResetPersistentInfo(); }
+extern "C" int rsref_tracker_pop(void* p, rsg_event* out, int max_out) {
+    auto* rt = (RefTracker*)p;
+    int n = (int)rt->events.size() < max_out ? (int)rt->events.size() : max_out;
+    for (int i = 0; i < n; i++) out[i] = rt->events[i];
+    rt->events.clear();
+    return n;
+}

@@ -31,6 +31,7 @@
 #include "rsg_adjoint.h"
 #include "rsg_ball_pred.h"
 #include "rsg_serialize.h"
+#include "rsg_game_events.h"
 
 using namespace rsg;
 
@@ -287,6 +288,13 @@ rsg_ball_pred_kernel(BatchDev pa, MeshView mesh, PredDev p, int kind, BatchDev s
     if (i >= pa.n) return;
     ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(rsg_lds + (size_t)threadIdx.x * (size_t)stride);
     ball_pred_update(pa, pa.mode == RSG_MODE_SOCCAR ? &mesh : nullptr, p, kind, src, i, A);
+}
+
+// GameEventTracker::Update for every arena (rsg_game_events.h), one lane each.
+__global__ void __launch_bounds__(256) rsg_game_events_kernel(BatchDev d, GameEventDev g, rsg_game_event_config cfg) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= d.n) return;
+    game_events_update(d, g, cfg, a);
 }
 
 __global__ void __launch_bounds__(64) rsg_reset_kernel(BatchDev d, int only_arena, uint64_t seed) {
@@ -961,6 +969,133 @@ int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out) {
         out[a] = (b->dev.mode == RSG_MODE_SOCCAR &&
                   fabsf(y) > (SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y + BALL_COLLISION_RADIUS_SOCCAR)) ? 1 : 0;
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Game-event tracking
+
+}  // extern "C"
+
+struct rsg_game_events {
+    int n = 0;
+    rsg_game_event_config cfg{};
+    float* cooldown = nullptr;
+    uint32_t* flags = nullptr;
+    uint64_t* last_count = nullptr;
+    uint32_t* seq = nullptr;
+    rsg_event* ev_buf = nullptr;
+    uint32_t* ev_count = nullptr;
+    uint32_t ev_cap = 0;
+    hipStream_t last_stream = nullptr;
+};
+
+extern "C" {
+
+void rsg_game_event_config_default(rsg_game_event_config* out) {
+    if (!out) return;
+    out->shot_min_speed = 1750.f;
+    out->shot_touch_min_delay = 0.3f;
+    out->pred_score_extra_margin = 0.f;
+    out->shot_event_cooldown = 1.0f;
+    out->shot_min_score_time = 2.0f;
+    out->goal_max_touch_time = 4.0f;
+    out->pass_max_touch_time = 2.0f;
+    out->auto_state_set_detection = 1;
+}
+
+rsg_game_events* rsg_game_events_create(rsg_batch* b, const rsg_game_event_config* cfg, int capacity) {
+    if (!b) { fail("rsg_game_events_create: null batch"); return nullptr; }
+    if (b->dev.mode != RSG_MODE_SOCCAR) { fail("rsg_game_events_create: SOCCAR batches only"); return nullptr; }
+    if (capacity < 1) { fail("rsg_game_events_create: capacity must be >= 1"); return nullptr; }
+    auto* t = new rsg_game_events();
+    t->n = b->dev.n;
+    if (cfg) t->cfg = *cfg;
+    else rsg_game_event_config_default(&t->cfg);
+    t->ev_cap = (uint32_t)capacity;
+    const size_t n = (size_t)t->n;
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&t->cooldown, n * 4)) != hipSuccess || (e = hipMalloc(&t->flags, n * 4)) != hipSuccess ||
+        (e = hipMalloc(&t->last_count, n * 8)) != hipSuccess || (e = hipMalloc(&t->seq, n * 4)) != hipSuccess ||
+        (e = hipMalloc(&t->ev_buf, (size_t)capacity * sizeof(rsg_event))) != hipSuccess ||
+        (e = hipMalloc(&t->ev_count, 4)) != hipSuccess || (e = hipMemset(t->cooldown, 0, n * 4)) != hipSuccess ||
+        (e = hipMemset(t->flags, 0, n * 4)) != hipSuccess || (e = hipMemset(t->last_count, 0, n * 8)) != hipSuccess ||
+        (e = hipMemset(t->seq, 0, n * 4)) != hipSuccess || (e = hipMemset(t->ev_count, 0, 4)) != hipSuccess) {
+        fail(std::string("rsg_game_events_create: ") + hipGetErrorString(e));
+        rsg_game_events_destroy(t);
+        return nullptr;
+    }
+    return t;
+}
+
+void rsg_game_events_destroy(rsg_game_events* t) {
+    if (!t) return;
+    (void)hipDeviceSynchronize();
+    if (t->cooldown) (void)hipFree(t->cooldown);
+    if (t->flags) (void)hipFree(t->flags);
+    if (t->last_count) (void)hipFree(t->last_count);
+    if (t->seq) (void)hipFree(t->seq);
+    if (t->ev_buf) (void)hipFree(t->ev_buf);
+    if (t->ev_count) (void)hipFree(t->ev_count);
+    delete t;
+}
+
+int rsg_game_events_update(rsg_game_events* t, rsg_batch* b, void* stream) {
+    if (!t || !b) return fail("rsg_game_events_update: null argument");
+    if (b->dev.n != t->n || b->dev.mode != RSG_MODE_SOCCAR)
+        return fail("rsg_game_events_update: not the batch the tracker was made for");
+    if (upload(b) != 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (b->last_stream && b->last_stream != s) HIP_OK(hipStreamSynchronize(b->last_stream));
+    GameEventDev g;
+    g.cooldown = t->cooldown;
+    g.flags = t->flags;
+    g.last_count = t->last_count;
+    g.seq = t->seq;
+    g.ev_buf = t->ev_buf;
+    g.ev_count = t->ev_count;
+    g.ev_cap = t->ev_cap;
+    int block = 256, grid = (t->n + block - 1) / block;
+    hipLaunchKernelGGL(rsg_game_events_kernel, dim3(grid), dim3(block), 0, s, b->dev, g, t->cfg);
+    HIP_OK(hipGetLastError());
+    t->last_stream = s;
+    return 0;
+}
+
+int rsg_game_events_poll(rsg_game_events* t, rsg_event* out, int max_out, int* n_lost) {
+    if (!t) return fail("rsg_game_events_poll: null tracker");
+    if (n_lost) *n_lost = 0;
+    if (t->last_stream) HIP_OK(hipStreamSynchronize(t->last_stream));
+    else HIP_OK(hipDeviceSynchronize());
+    uint32_t count = 0;
+    HIP_OK(hipMemcpy(&count, t->ev_count, sizeof(count), hipMemcpyDeviceToHost));
+    uint32_t stored = count < t->ev_cap ? count : t->ev_cap;
+    std::vector<rsg_event> ev(stored);
+    if (stored) HIP_OK(hipMemcpy(ev.data(), t->ev_buf, stored * sizeof(rsg_event), hipMemcpyDeviceToHost));
+    uint32_t zero = 0;
+    HIP_OK(hipMemcpy(t->ev_count, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    std::sort(ev.begin(), ev.end(), [](const rsg_event& x, const rsg_event& y) {
+        return x.arena != y.arena ? x.arena < y.arena : x.seq < y.seq;
+    });
+    int n = (int)stored < max_out ? (int)stored : (max_out > 0 ? max_out : 0);
+    if (out && n) std::memcpy(out, ev.data(), (size_t)n * sizeof(rsg_event));
+    if (n_lost) *n_lost = (int)(count - (uint32_t)n);
+    return n;
+}
+
+int rsg_game_events_reset(rsg_game_events* t, int arena) {
+    if (!t) return fail("rsg_game_events_reset: null tracker");
+    if (arena < -1 || arena >= t->n) return fail("rsg_game_events_reset: arena out of range");
+    if (t->last_stream) HIP_OK(hipStreamSynchronize(t->last_stream));
+    int a0 = arena < 0 ? 0 : arena, a1 = arena < 0 ? t->n : arena + 1;
+    // ResetPersistentInfo: cooldown 0, not shot, not scored last (the shot's
+    // goal team is kept)
+    std::vector<uint32_t> fl((size_t)(a1 - a0));
+    HIP_OK(hipMemcpy(fl.data(), t->flags + a0, fl.size() * 4, hipMemcpyDeviceToHost));
+    for (auto& f : fl) f &= GE_SHOT_GOAL_ORANGE;
+    std::vector<float> zero((size_t)(a1 - a0), 0.f);
+    HIP_OK(hipMemcpy(t->flags + a0, fl.data(), fl.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(t->cooldown + a0, zero.data(), zero.size() * 4, hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -6,7 +6,8 @@ entry point raises.
 import ctypes as C
 import os
 
-from .state import ArenaSnapshot, BallPredView, BallState, BatchDesc, CarState, Controls, Event, PadState
+from .state import (ArenaSnapshot, BallPredView, BallState, BatchDesc, CarState, Controls, Event, GameEventConfig,
+                    PadState)
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "librsg.so")
@@ -76,6 +77,13 @@ def lib():
         L.rsg_ball_pred_state_for_time.argtypes = [C.c_void_p, C.c_int, C.c_float, C.POINTER(BallState)]
         L.rsg_ball_pred_view.argtypes = [C.c_void_p, C.POINTER(BallPredView)]
         L.rsg_ball_pred_set_launch.argtypes = [C.c_void_p, C.c_int]
+        L.rsg_game_event_config_default.argtypes = [C.POINTER(GameEventConfig)]
+        L.rsg_game_events_create.restype = C.c_void_p
+        L.rsg_game_events_create.argtypes = [C.c_void_p, C.POINTER(GameEventConfig), C.c_int]
+        L.rsg_game_events_destroy.argtypes = [C.c_void_p]
+        L.rsg_game_events_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rsg_game_events_poll.argtypes = [C.c_void_p, C.POINTER(Event), C.c_int, C.POINTER(C.c_int)]
+        L.rsg_game_events_reset.argtypes = [C.c_void_p, C.c_int]
         L.rsg_snapshot_serialize.restype = C.c_int64
         L.rsg_snapshot_serialize.argtypes = [C.POINTER(ArenaSnapshot), C.c_void_p, C.c_uint64]
         L.rsg_snapshot_deserialize.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(ArenaSnapshot)]
@@ -108,4 +116,6 @@ EXPORTED = [
     "rsg_ball_pred_update_from_batch", "rsg_ball_pred_get", "rsg_ball_pred_get_all",
     "rsg_ball_pred_state_for_time", "rsg_ball_pred_view", "rsg_ball_pred_set_launch",
     "rsg_snapshot_serialize", "rsg_snapshot_deserialize", "rsg_batch_get_snapshot", "rsg_batch_set_snapshot",
+    "rsg_game_event_config_default", "rsg_game_events_create", "rsg_game_events_destroy", "rsg_game_events_update",
+    "rsg_game_events_poll", "rsg_game_events_reset",
 ]

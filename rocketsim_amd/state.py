@@ -122,6 +122,25 @@ class ArenaSnapshot(C.Structure):
     ]
 
 
+EVENT_SHOT, EVENT_SCORER, EVENT_SAVE = 4, 5, 6
+
+
+class GameEventConfig(C.Structure):
+    """rsg_game_event_config: GameEventTrackerConfig
+    (src/Sim/GameEventTracker/GameEventTracker.h:10-41), reference defaults."""
+    _fields_ = [("shot_min_speed", C.c_float), ("shot_touch_min_delay", C.c_float),
+                ("pred_score_extra_margin", C.c_float), ("shot_event_cooldown", C.c_float),
+                ("shot_min_score_time", C.c_float), ("goal_max_touch_time", C.c_float),
+                ("pass_max_touch_time", C.c_float), ("auto_state_set_detection", C.c_int32)]
+
+    def __init__(self, **kw):
+        defaults = dict(shot_min_speed=1750.0, shot_touch_min_delay=0.3, pred_score_extra_margin=0.0,
+                        shot_event_cooldown=1.0, shot_min_score_time=2.0, goal_max_touch_time=4.0,
+                        pass_max_touch_time=2.0, auto_state_set_detection=1)
+        defaults.update(kw)
+        super().__init__(**defaults)
+
+
 class BatchDesc(C.Structure):
     _fields_ = [
         ("n_arenas", C.c_int), ("mode", C.c_int), ("tick_rate", C.c_float),
