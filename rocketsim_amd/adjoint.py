@@ -1,7 +1,11 @@
-"""torch.autograd.Function over the differentiable air-control rollout of
-librsg.so (rsg_air_rollout / rsg_air_rollout_backward, include/rsg.h).
+"""Autograd over the differentiable air-control rollout of librsg.so
+(rsg_air_rollout / rsg_air_rollout_backward, include/rsg.h).
 
-    loss, final = AirRollout.apply(state0, controls, target)
+    loss, final = air_rollout(state0, controls, target)
+
+`air_rollout` goes through the LibTorch C++ torch::autograd::Function of the
+rsg_torch extension (csrc/rsg_torch.cpp); `AirRollout` is the same Function
+written in Python over the same two entry points.
 
 state0 [n, 13] (position uu, quaternion x y z w, velocity uu/s, angular
 velocity rad/s), controls [ticks, n, 4] (pitch, yaw, roll in [-1, 1], boost
@@ -10,6 +14,9 @@ The forward pass keeps a checkpoint of every car's state every `ckpt_every`
 ticks; the backward pass recomputes each segment from its checkpoint and runs
 the hand-written adjoint kernel.  Gradients flow to controls and state0.
 """
+import importlib.util
+import os
+
 import torch
 
 from . import native
@@ -60,5 +67,32 @@ class AirRollout(torch.autograd.Function):
         return ds0.t().contiguous(), dctrl.permute(0, 2, 1).contiguous(), None, None, None
 
 
+_ext = None
+TORCH_EXT_PATH = os.path.join(native.PKG, "lib", "rsg_torch.so")
+
+
+def torch_ext():
+    """The rsg_torch extension module (built by rocketsim_amd.build)."""
+    global _ext
+    if _ext is None:
+        native.lib()
+        if not os.path.exists(TORCH_EXT_PATH):
+            raise native.NativeMissing(f"{TORCH_EXT_PATH} is not built; run `python -m rocketsim_amd.build`")
+        spec = importlib.util.spec_from_file_location("rsg_torch", TORCH_EXT_PATH)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _ext = mod
+    return _ext
+
+
 def air_rollout(state0, controls, target, ckpt_every=CKPT_EVERY, tick_rate=120.0):
+    """(loss [n], final state [n, 13]) through the LibTorch C++ Function, on
+    the current stream."""
+    stream = torch.cuda.current_stream().cuda_stream
+    loss, final = torch_ext().air_rollout(state0, controls, target, ckpt_every, tick_rate, stream)
+    return loss, final
+
+
+def air_rollout_py(state0, controls, target, ckpt_every=CKPT_EVERY, tick_rate=120.0):
+    """The same through the Python autograd Function."""
     return AirRollout.apply(state0, controls, target, ckpt_every, tick_rate)

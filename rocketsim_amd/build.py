@@ -38,5 +38,37 @@ def build_native(verbose=True):
     return LIB_PATH
 
 
+TORCH_EXT_PATH = os.path.join(LIBDIR, "rsg_torch.so")
+
+
+def build_torch_ext(verbose=True):
+    """The LibTorch autograd Function module (csrc/rsg_torch.cpp), linked to
+    librsg.so beside it.  Host C++ only: it calls the C ABI, which launches the
+    HIP kernels."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension
+    os.makedirs(LIBDIR, exist_ok=True)
+    src = os.path.join(CSRC, "rsg_torch.cpp")
+    tmp = TORCH_EXT_PATH + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", src, "-o", tmp,
+           "-I" + os.path.join(ROOT, "include"), "-I" + sysconfig.get_paths()["include"],
+           "-D_GLIBCXX_USE_CXX11_ABI=" + str(int(torch._C._GLIBCXX_USE_CXX11_ABI)),
+           "-DTORCH_EXTENSION_NAME=rsg_torch", "-DTORCH_API_INCLUDE_EXTENSION_H"]
+    cmd += ["-I" + p for p in cpp_extension.include_paths()]
+    cmd += ["-L" + p for p in cpp_extension.library_paths()]
+    cmd += ["-L" + LIBDIR, "-lrsg", "-Wl,-rpath,$ORIGIN", "-ltorch", "-ltorch_cpu", "-lc10", "-ltorch_python"]
+    if verbose:
+        print("[build]", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building rsg_torch.so")
+    os.replace(tmp, TORCH_EXT_PATH)
+    return TORCH_EXT_PATH
+
+
 if __name__ == "__main__":
     build_native()
+    build_torch_ext()

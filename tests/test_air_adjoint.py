@@ -139,6 +139,13 @@ def test_fp64_autograd_matches_central_differences():
         assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
 
 
+def test_torch_extension_loads():
+    """The LibTorch C++ Function module links against librsg.so and torch and
+    exports air_rollout (no GPU work)."""
+    from rocketsim_amd.adjoint import torch_ext
+    assert hasattr(torch_ext(), "air_rollout")
+
+
 def _gpu_problem(n, seed):
     s0, ctrl, tgt = air_ref.random_problem(n, TICKS, seed)
     return s0, ctrl, tgt, s0.float().cuda(), ctrl.float().cuda(), tgt.float().cuda()
@@ -222,3 +229,21 @@ def test_air_gradients_match_fp64_and_central_differences():
         fd = (lp - lm) / (2 * h)
         an = (gc * d).sum(dim=(0, 2))
         assert ((fd - an).abs() <= 1e-3 * fd.abs().clamp(min=1e-2)).float().mean() >= 0.99
+
+
+@pytest.mark.gpu
+def test_cpp_function_equals_python_function():
+    """The LibTorch C++ autograd Function and the Python one run the same
+    kernels: losses, final states and gradients are identical."""
+    from rocketsim_amd.adjoint import air_rollout, air_rollout_py
+    n = 256
+    _, _, _, s0g, ctrlg, tgtg = _gpu_problem(n, 13)
+    out = []
+    for fn in (air_rollout, air_rollout_py):
+        s0 = s0g.clone().requires_grad_(True)
+        c = ctrlg.clone().requires_grad_(True)
+        loss, final = fn(s0, c, tgtg)
+        (loss * torch.linspace(0.5, 1.5, n, device=loss.device)).sum().backward()
+        out.append((loss.detach(), final.detach(), c.grad, s0.grad))
+    for a, b in zip(out[0], out[1]):
+        assert torch.equal(a, b)
