@@ -29,7 +29,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "arena-ticks/s, 2v2 batched vs plain GPU vs CPU; adjoint rollouts/s"
+METRIC = "arena-ticks/s, 2v2 batched vs plain GPU vs CPU; adjoint and shot rollouts/s"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per arena-tick, 2v2: 2 x (4 cars x 220 B + ball 36 B +
 # pads 272 B + tick 8 B) + 4 x 24 B controls.
@@ -159,6 +159,41 @@ def bench_adjoint(n_cars=65536, ticks=120, reps=5, seed=7):
                         "controls tanh/sigmoid of parameters; loss geodesic angle + 1e-4 |dp|^2; fwd + adjoint"}
 
 
+def bench_shot(n=16384, ticks=90, reps=5, seed=13):
+    """Differentiable aerial shot (16384 car+ball pairs, 90 ticks, checkpoints
+    every 16 ticks): rollouts/s of forward + adjoint through the car-ball hit,
+    through the LibTorch autograd Function."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import shot_ref
+    from rocketsim_amd.adjoint import shot_rollout
+    s0, ctrl, tgt = shot_ref.random_problem(n, ticks, seed)
+    state0, target = s0.float().cuda(), tgt.float().cuda()
+    params = torch.atanh(ctrl[..., :3].clamp(-0.999, 0.999)).float().cuda()
+    params = torch.cat([params, torch.logit(ctrl[..., 3:].clamp(1e-3, 1 - 1e-3)).float().cuda()], -1)
+    params.requires_grad_(True)
+
+    def one():
+        controls = torch.cat([torch.tanh(params[..., :3]), torch.sigmoid(params[..., 3:])], -1)
+        loss, _, flags = shot_rollout(state0, controls, target)
+        loss.sum().backward()
+        return flags
+
+    flags = one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    fl = flags.cpu()
+    return {"value": n / wall, "unit": "rollouts/s", "pairs": n, "ticks": ticks, "ckpt_every": 16,
+            "ms_per_batch": wall * 1e3, "touched": int(((fl & 1) != 0).sum()),
+            "outside_model": int(((fl & 6) != 0).sum()),
+            "workload": "ball x U[-3000,3000], y U[-4000,4000], z U[300,1500] uu, v N(0,800); car airborne "
+                        "250-600 uu from the ball, facing it; loss |ball - (0,5120,321)| at tick 90; fwd + adjoint"}
+
+
 def bench_ball_pred(n=4096, T=720, reps=3, seed=11):
     """Ball prediction (BallPredTracker over the GPU): n trackers predicting
     T ticks (6 s) of the Soccar ball, from random states in the field.  Full
@@ -226,6 +261,7 @@ def main():
                          "one tick per launch) on the same workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable air-control line")
+    ap.add_argument("--no-shot", action="store_true", help="skip the differentiable aerial-shot line")
     ap.add_argument("--no-ball-pred", action="store_true", help="skip the ball-prediction line")
     ap.add_argument("--profile-probe", action="store_true",
                     help="for rocprofv3 --pmc passes: fill and copy a 64 MiB buffer (calibrates the HBM byte "
@@ -291,6 +327,7 @@ def main():
                  "speedup": (arena_ticks / wall) / (arena_ticks / p_wall)}
 
     adjoint = None if args.no_adjoint else bench_adjoint()
+    shot = None if args.no_shot else bench_shot()
     ball_pred = None if args.no_ball_pred else bench_ball_pred()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline()
@@ -325,6 +362,8 @@ def main():
         line["plain_gpu"] = plain
     if adjoint is not None:
         line["adjoint"] = adjoint
+    if shot is not None:
+        line["shot"] = shot
     if ball_pred is not None:
         line["ball_pred"] = ball_pred
     print(json.dumps(line), flush=True)

@@ -169,6 +169,33 @@ int rsg_air_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, 
                              const float* target, const float* checkpoints, const float* dloss, float* dcontrols,
                              float* dstate0, void* stream);
 
+/* Differentiable aerial shot: one airborne car and the ball per pair, through
+ * the car-ball hit (no counterpart in the reference; the forward model is the
+ * reference's tick for an airborne Octane and a flying ball with no world
+ * contact: the air control and boost above, ball damping and gravity, the
+ * car hitbox vs ball contact with Ball::_OnHit's extra impulse (Ball.cpp:
+ * 236-290) and its sequential-impulse solve, integration and the speed caps;
+ * rocketsim_amd/csrc/rsg_shot.h).  Device memory, structure of arrays, n pairs:
+ *   state   [22][n]  car position (uu), quaternion, velocity (uu/s), angular
+ *                    velocity; ball position (uu), velocity (uu/s), angular
+ *                    velocity
+ *   controls [ticks][4][n]  pitch, yaw, roll in [-1, 1], boost in [0, 1]
+ *   target  [3][n]   where the ball should end (uu); loss = |ball - target|
+ *   checkpoints [ceil(ticks / ckpt_every)][23][n]  (ckpt_every <= 32)
+ *   flags   [n]      bit 0: the car touched the ball; bit 1: the ball centre
+ *                    entered the hitbox core (not modelled); bit 2: the ball
+ *                    or car came near the floor, ceiling or walls (not
+ *                    modelled: the forward result there is not the reference's)
+ * The backward pass recomputes each segment from its checkpoint and, per
+ * tick, propagates one tangent per step input to form the exact
+ * vector-Jacobian product for the contact events of the forward pass. */
+int rsg_shot_rollout(int n, int ticks, int ckpt_every, float tick_rate, const float* state0, const float* controls,
+                     const float* target, float* state_out, float* loss, float* checkpoints, uint32_t* flags,
+                     void* stream);
+int rsg_shot_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, const float* controls,
+                              const float* target, const float* checkpoints, const float* dloss, float* dcontrols,
+                              float* dstate0, void* stream);
+
 /* Arena::IsBallScored (Arena.cpp:893-901) per arena. */
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
 

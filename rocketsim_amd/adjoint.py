@@ -93,6 +93,19 @@ def air_rollout(state0, controls, target, ckpt_every=CKPT_EVERY, tick_rate=120.0
     return loss, final
 
 
+def shot_rollout(state0, controls, target, ckpt_every=CKPT_EVERY, tick_rate=120.0):
+    """Differentiable aerial shot through the LibTorch C++ Function:
+    state0 [n, 22] (car position uu, quaternion, velocity uu/s, angular
+    velocity; ball position uu, velocity uu/s, angular velocity), controls
+    [ticks, n, 4], target [n, 3] (uu) -> (loss [n] = |ball - target| (uu),
+    final state [n, 22], flags [n]: 1 = the car touched the ball, 2 = the ball
+    entered the hitbox core, 4 = near the floor, ceiling or walls; the last two
+    are outside the model)."""
+    stream = torch.cuda.current_stream().cuda_stream
+    loss, final, flags = torch_ext().shot_rollout(state0, controls, target, ckpt_every, tick_rate, stream)
+    return loss, final, flags
+
+
 def air_rollout_py(state0, controls, target, ckpt_every=CKPT_EVERY, tick_rate=120.0):
     """The same through the Python autograd Function."""
     return AirRollout.apply(state0, controls, target, ckpt_every, tick_rate)

@@ -10,6 +10,10 @@
 //   rsg_ball_pred_kernel        ball prediction trackers (rsg_ball_pred.h)
 //   rsg_air_forward_kernel,     differentiable air control: rollout with
 //   rsg_air_backward_kernel     checkpoints, hand-written adjoint
+//   rsg_shot_forward_kernel,    differentiable aerial shot (rsg_shot.h):
+//   rsg_shot_backward_kernel    rollout with checkpoints; backward with one
+//                               lane per tangent input of each pair
+//   rsg_game_events_kernel      GameEventTracker (rsg_game_events.h)
 //   rsg_reset_kernel            kickoff reset of one / all arenas
 //
 // Host side keeps a mirror of the SoA state for the get/set API; it is
@@ -32,6 +36,7 @@
 #include "rsg_ball_pred.h"
 #include "rsg_serialize.h"
 #include "rsg_game_events.h"
+#include "rsg_shot.h"
 
 using namespace rsg;
 
@@ -288,6 +293,88 @@ rsg_ball_pred_kernel(BatchDev pa, MeshView mesh, PredDev p, int kind, BatchDev s
     if (i >= pa.n) return;
     ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(rsg_lds + (size_t)threadIdx.x * (size_t)stride);
     ball_pred_update(pa, pa.mode == RSG_MODE_SOCCAR ? &mesh : nullptr, p, kind, src, i, A);
+}
+
+// Differentiable aerial shot (rsg_shot.h): one lane per car-ball pair.
+__global__ void __launch_bounds__(256) rsg_shot_forward_kernel(int n, int ticks, int K, float dt, const float* state0,
+                                                               const float* ctrl, const float* target,
+                                                               float* state_out, float* loss, float* ckpt,
+                                                               uint32_t* flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ShotConsts k = shot_consts(dt);
+    ShotState<float> s;
+    shot_load(state0, n, i, s);
+    uint32_t fl = 0;
+    V3 tg = v3(target[i], target[n + i], target[2 * n + i]);
+    float l = shot_forward(s, ctrl, n, i, ticks, K, ckpt, tg, k, fl);
+    if (state_out) shot_store(state_out, n, i, s);
+    if (loss) loss[i] = l;
+    if (flags) flags[i] = fl;
+}
+
+// Backward of the shot: one group of SHOT_GROUP lanes per pair, lane j < 26
+// carrying the tangent along input j of every tick (22 state words, then
+// pitch, yaw, roll, boost), so a pair's 26 passes per tick run side by side
+// instead of one after another.  Every lane of a group recomputes the same
+// segment from its checkpoint; lane 0 keeps the segment's states in LDS
+// ([K][SHOT_LDS_WORDS] per group), and after each tick the group gathers the
+// new state adjoint from lanes 0-21 with cross-lane shuffles.  A group spans
+// half a wave, so no workgroup barrier is needed: a group whose pair is past
+// n leaves at once.
+constexpr int SHOT_GROUP = 32;
+constexpr int SHOT_LDS_WORDS = 24;   // a checkpoint's 23 words, padded
+
+__global__ void __launch_bounds__(256) rsg_shot_backward_kernel(int n, int ticks, int K, float dt, const float* ctrl,
+                                                                const float* target, const float* ckpt,
+                                                                const float* dloss, float* dctrl, float* ds0) {
+    extern __shared__ float shot_seg[];
+    const int lane = threadIdx.x & (SHOT_GROUP - 1);
+    const int grp = threadIdx.x / SHOT_GROUP;
+    const int i = blockIdx.x * (blockDim.x / SHOT_GROUP) + grp;
+    if (i >= n) return;
+    float* seg = shot_seg + (size_t)grp * K * SHOT_LDS_WORDS;
+    ShotConsts k = shot_consts(dt);
+    V3 tg = v3(target[i], target[n + i], target[2 * n + i]);
+    const int nseg = (ticks + K - 1) / K;
+    ShotState<float> s;
+    int last_extra;
+    shot_ckpt_load(ckpt + (size_t)(nseg - 1) * SHOT_CKPT_WORDS * n, n, i, s, last_extra);
+    uint32_t fl = 0;
+    for (int t = (nseg - 1) * K; t < ticks; t++) shot_step(s, shot_ctrl(ctrl, n, i, t), t, last_extra, fl, k);
+    float g[SHOT_STATE_WORDS];
+    shot_loss_grad(s, tg, dloss[i], g);
+    float mine = 0.f;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int t0 = sg * K, t1 = t0 + K < ticks ? t0 + K : ticks;
+        ShotState<float> x;
+        int lx;
+        shot_ckpt_load(ckpt + (size_t)sg * SHOT_CKPT_WORDS * n, n, i, x, lx);
+        for (int t = t0; t < t1; t++) {
+            if (lane == 0) {
+                float* r = seg + (t - t0) * SHOT_LDS_WORDS;
+                for (int w = 0; w < SHOT_STATE_WORDS; w++) r[w] = shot_word(x, w);
+                r[SHOT_STATE_WORDS] = __int_as_float(lx);
+            }
+            shot_step(x, shot_ctrl(ctrl, n, i, t), t, lx, fl, k);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        for (int t = t1 - 1; t >= t0; t--) {
+            const float* r = seg + (t - t0) * SHOT_LDS_WORDS;
+            ShotState<float> st;
+            for (int w = 0; w < SHOT_STATE_WORDS; w++) shot_word(st, w) = r[w];
+            const int le = __float_as_int(r[SHOT_STATE_WORDS]);
+            mine = shot_vjp_entry(st, shot_ctrl(ctrl, n, i, t), t, le, k, g, lane);
+            for (int w = 0; w < SHOT_STATE_WORDS; w++) g[w] = __shfl(mine, w, SHOT_GROUP);
+            if (lane >= SHOT_STATE_WORDS && lane < SHOT_INPUTS)
+                dctrl[(size_t)t * 4 * n + (size_t)(lane - SHOT_STATE_WORDS) * n + i] = mine;
+        }
+        // the next segment's states overwrite this one's
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (ds0 && lane < SHOT_STATE_WORDS) ds0[(size_t)lane * n + i] = mine * shot_unit(lane);
 }
 
 // GameEventTracker::Update for every arena (rsg_game_events.h), one lane each.
@@ -674,6 +761,39 @@ int rsg_air_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, 
     if (n == 0) return 0;
     int block = 256, grid = (n + block - 1) / block;
     hipLaunchKernelGGL(rsg_air_backward_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream, n, ticks,
+                       ckpt_every, 1.f / tick_rate, controls, target, checkpoints, dloss, dcontrols, dstate0);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int rsg_shot_rollout(int n, int ticks, int ckpt_every, float tick_rate, const float* state0, const float* controls,
+                     const float* target, float* state_out, float* loss, float* checkpoints, uint32_t* flags,
+                     void* stream) {
+    if (n <= 0 || ticks <= 0) return fail("rsg_shot_rollout: n and ticks must be > 0");
+    if (ckpt_every < 1 || ckpt_every > SHOT_MAX_K) return fail("rsg_shot_rollout: ckpt_every must be within [1, 32]");
+    if (!(tick_rate >= 15.f && tick_rate <= 120.f)) return fail("rsg_shot_rollout: tick_rate must be within [15, 120]");
+    if (!state0 || !controls || !target || !checkpoints) return fail("rsg_shot_rollout: null argument");
+    // 64-lane workgroups spread the pairs over every CU
+    int block = 64, grid = (n + block - 1) / block;
+    hipLaunchKernelGGL(rsg_shot_forward_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream, n, ticks, ckpt_every,
+                       1.f / tick_rate, state0, controls, target, state_out, loss, checkpoints, flags);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int rsg_shot_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, const float* controls,
+                              const float* target, const float* checkpoints, const float* dloss, float* dcontrols,
+                              float* dstate0, void* stream) {
+    if (n <= 0 || ticks <= 0) return fail("rsg_shot_rollout_backward: n and ticks must be > 0");
+    if (ckpt_every < 1 || ckpt_every > SHOT_MAX_K)
+        return fail("rsg_shot_rollout_backward: ckpt_every must be within [1, 32]");
+    if (!(tick_rate >= 15.f && tick_rate <= 120.f))
+        return fail("rsg_shot_rollout_backward: tick_rate must be within [15, 120]");
+    if (!controls || !target || !checkpoints || !dloss || !dcontrols)
+        return fail("rsg_shot_rollout_backward: null argument");
+    const int block = 256, per_block = block / SHOT_GROUP, grid = (n + per_block - 1) / per_block;
+    const size_t lds = (size_t)per_block * ckpt_every * SHOT_LDS_WORDS * sizeof(float);
+    hipLaunchKernelGGL(rsg_shot_backward_kernel, dim3(grid), dim3(block), lds, (hipStream_t)stream, n, ticks,
                        ckpt_every, 1.f / tick_rate, controls, target, checkpoints, dloss, dcontrols, dstate0);
     HIP_OK(hipGetLastError());
     return 0;

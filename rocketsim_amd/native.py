@@ -50,6 +50,8 @@ def lib():
         P = C.c_void_p
         L.rsg_air_rollout.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, P, P, P, P, P, P, P]
         L.rsg_air_rollout_backward.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, P, P, P, P, P, P, P]
+        L.rsg_shot_rollout.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, P, P, P, P, P, P, P, P]
+        L.rsg_shot_rollout_backward.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, P, P, P, P, P, P, P]
         L.rsg_batch_enable_events.argtypes = [C.c_void_p, C.c_int]
         L.rsg_batch_poll_events.argtypes = [C.c_void_p, C.POINTER(Event), C.c_int, C.POINTER(C.c_int)]
         L.rsg_batch_graph_create.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -110,6 +112,7 @@ EXPORTED = [
     "rsg_batch_set_kernel", "rsg_batch_get_kernel", "rsg_batch_autotune",
     "rsg_batch_graph_create", "rsg_batch_graph_launch", "rsg_batch_graph_ticks",
     "rsg_batch_enable_events", "rsg_batch_poll_events", "rsg_air_rollout", "rsg_air_rollout_backward",
+    "rsg_shot_rollout", "rsg_shot_rollout_backward",
     "rsg_batch_get_tick_counts", "rsg_batch_set_tick_counts", "rsg_batch_get_goals",
     "rsg_batch_reset_kickoff", "rsg_batch_is_ball_scored",
     "rsg_ball_pred_create", "rsg_ball_pred_destroy", "rsg_ball_pred_force_update", "rsg_ball_pred_update_manual",
