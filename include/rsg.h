@@ -32,7 +32,7 @@ typedef struct rsg_batch rsg_batch;
  * car_team[off_a .. off_a + cars_per_arena[a]) with off_a the prefix sum. */
 typedef struct rsg_batch_desc {
     int n_arenas;
-    int mode;                     /* RSG_MODE_SOCCAR or RSG_MODE_THE_VOID */
+    int mode;                     /* RSG_MODE_SOCCAR, RSG_MODE_HEATSEEKER or RSG_MODE_THE_VOID */
     float tick_rate;              /* ticks per second, 120 in every workload */
     uint64_t seed;                /* seeds kickoff shuffles, respawns and the synthetic controls */
     int auto_reset;               /* kickoff reset after a goal (Arena.cpp:748-752 + 113-223) */

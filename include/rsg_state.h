@@ -32,9 +32,10 @@
 extern "C" {
 #endif
 
-/* Game mode ids, same order as src/Sim/GameMode.h:7-17. Only SOCCAR and
- * THE_VOID are supported by the batched step. */
-enum { RSG_MODE_SOCCAR = 0, RSG_MODE_THE_VOID = 5 };
+/* Game mode ids, same order as src/Sim/GameMode.h:7-17. SOCCAR, HEATSEEKER
+ * (the Soccar arena with the seeking ball) and THE_VOID are supported by the
+ * batched step. */
+enum { RSG_MODE_SOCCAR = 0, RSG_MODE_HEATSEEKER = 2, RSG_MODE_THE_VOID = 5 };
 
 /* Car hitbox presets, same order as src/Sim/Car/CarConfig/CarConfig.cpp:20-100. */
 enum {
@@ -115,6 +116,10 @@ typedef struct rsg_ball_state {
     float ang_vel[3];
     int32_t bp_cell;                /* as in rsg_car_state */
     uint32_t bp_seq;
+    /* BallState::HeatseekerInfo (src/Sim/Ball/Ball.h:24-31): the net the
+     * ball seeks (+1 the orange net, -1 the blue net, 0 none), its target
+     * speed (uu/s) and the time since the last speed-up touch (s) */
+    float hs_y_target_dir, hs_cur_target_speed, hs_time_since_hit;
 } rsg_ball_state;
 
 typedef struct rsg_pad_state {

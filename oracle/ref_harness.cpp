@@ -121,8 +121,11 @@ const char* rsref_last_error() { return g_err.c_str(); }
 int rsref_init(const uint8_t* const* blobs, const uint64_t* sizes, int n) {
     try {
         std::map<GameMode, std::vector<FileData>> m;
-        for (int i = 0; i < n; i++)
+        for (int i = 0; i < n; i++) {
+            // Heatseeker plays in the Soccar arena
             m[GameMode::SOCCAR].push_back(FileData(blobs[i], blobs[i] + sizes[i]));
+            m[GameMode::HEATSEEKER].push_back(FileData(blobs[i], blobs[i] + sizes[i]));
+        }
         RocketSim::InitFromMem(m, true);
         return 0;
     } catch (std::exception& e) {
@@ -310,12 +313,18 @@ void rsref_get_ball(void* h, rsg_ball_state* s) {
     auto* ra = (RefArena*)h;
     s->bp_cell = ra->bp_cell[0];
     s->bp_seq = ra->bp_seq[0];
+    s->hs_y_target_dir = bs.hsInfo.yTargetDir;
+    s->hs_cur_target_speed = bs.hsInfo.curTargetSpeed;
+    s->hs_time_since_hit = bs.hsInfo.timeSinceHit;
 }
 
 void rsref_set_ball(void* h, const rsg_ball_state* s) {
     Ball* b = ((RefArena*)h)->arena->ball;
     BallState bs;
     vset(bs.pos, s->pos); rset(bs.rotMat, s->rot); vset(bs.vel, s->vel); vset(bs.angVel, s->ang_vel);
+    bs.hsInfo.yTargetDir = s->hs_y_target_dir;
+    bs.hsInfo.curTargetSpeed = s->hs_cur_target_speed;
+    bs.hsInfo.timeSinceHit = s->hs_time_since_hit;
     b->SetState(bs);
     b->_internalState.tickCountSinceUpdate = s->tick_count_since_update;
 }

@@ -8,7 +8,7 @@ the get/set methods replace Car/Ball/BoostPad Get/SetState.
 import ctypes as C
 
 from . import native
-from .state import (ArenaSnapshot, BallState, BatchDesc, CarState, Controls, Event, MODE_SOCCAR, MODE_THE_VOID,
+from .state import (ArenaSnapshot, BallState, BatchDesc, CarState, Controls, Event, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID,
                     NUM_PADS, PadState)
 
 
@@ -49,7 +49,7 @@ class ArenaBatch:
     def __init__(self, cars_per_arena, teams, presets, mode=MODE_SOCCAR, tick_rate=120.0,
                  seed=0, auto_reset=False, synth_controls=False):
         L = native.lib()
-        if mode == MODE_SOCCAR and not _mesh_loaded:
+        if mode in (MODE_SOCCAR, MODE_HEATSEEKER) and not _mesh_loaded:
             init_mesh()
         n = len(cars_per_arena)
         total = sum(cars_per_arena)
@@ -225,4 +225,4 @@ class ArenaBatch:
             pass
 
 
-__all__ = ["ArenaBatch", "MODE_SOCCAR", "MODE_THE_VOID"]
+__all__ = ["ArenaBatch", "MODE_HEATSEEKER", "MODE_SOCCAR", "MODE_THE_VOID"]

@@ -121,6 +121,9 @@ inline void ball_to_api(const CarRef& r, rsg_ball_state& s) {
     s.tick_count_since_update = r.u64(BW_TICKS_SINCE_SET);
     s.bp_cell = (int32_t)r.u(BW_BP_CELL);
     s.bp_seq = r.u(BW_BP_SEQ);
+    s.hs_y_target_dir = r.f(BW_HS);
+    s.hs_cur_target_speed = r.f(BW_HS + 1);
+    s.hs_time_since_hit = r.f(BW_HS + 2);
 }
 
 inline void ball_from_api(const CarRef& r, const rsg_ball_state& s) {
@@ -133,12 +136,16 @@ inline void ball_from_api(const CarRef& r, const rsg_ball_state& s) {
     r.su64(BW_TICKS_SINCE_SET, s.tick_count_since_update);
     r.u(BW_BP_CELL) = (uint32_t)(s.bp_cell >= 0 && s.bp_cell < BP_NUM_CELLS ? s.bp_cell : 0);
     r.u(BW_BP_SEQ) = s.bp_seq;
+    r.sf(BW_HS, s.hs_y_target_dir);
+    r.sf(BW_HS + 1, s.hs_cur_target_speed);
+    r.sf(BW_HS + 2, s.hs_time_since_hit);
 }
 
 // A new arena's records (Arena::Create + AddCar, src/Sim/Arena/Arena.cpp:50-62,
 // 416-540): cars with their config and spawn boost, the ball at rest, pads
 // active; the proxies registered where the reference creates them.
 inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_t* team, const uint8_t* preset) {
+    const int mode = h.mode;
     CarRef ar = arena_ref(h, a);
     ar.u(AW_NUM_CARS) = (uint32_t)n_cars;
     uint32_t ow = 0;  // default car update order: reverse creation
@@ -150,7 +157,7 @@ inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_
         cr.u(CW_ID) = (uint32_t)(i + 1);
         cr.su64(CW_HIT_TICK, ~0ull);
         cr.su64(CW_HIT_EXTRA_TICK, ~0ull);
-        cr.sf(CW_BOOST, BOOST_SPAWN_AMOUNT);
+        cr.sf(CW_BOOST, spawn_boost(mode));
         store_rot(cr, CW_ROT, m3_identity());
         // proxy created at the identity transform (Car::_BulletSetup,
         // src/Sim/Car/Car.cpp:212-233), AABB without contact threshold
@@ -161,6 +168,7 @@ inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_
     CarRef br = ball_ref(h, a);
     br.sv(BW_POS, v3(0, 0, BALL_REST_Z) * UU_TO_BT);
     store_rot(br, BW_ROT, m3_identity());
+    br.sf(BW_HS + 1, HS_INITIAL_TARGET_SPEED);
     {
         // proxy created at (0, 0, radius) (Ball::_BulletSetup, src/Sim/Ball/Ball.cpp:85-114)
         BallPhys bp = make_ball_phys();
@@ -169,7 +177,8 @@ inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_
         br.u(BW_BP_CELL) = (uint32_t)bp_cell_of(c - v3(ext, ext, ext));
         br.u(BW_BP_SEQ) = 0;
     }
-    for (int p = 0; p < PAD_NUM; p++) pad_ref(h, a, p).u(PW_ACTIVE) = 1;
+    if (mode_has_arena(mode))
+        for (int p = 0; p < PAD_NUM; p++) pad_ref(h, a, p).u(PW_ACTIVE) = 1;
 }
 
 inline void pad_to_api(const CarRef& r, rsg_pad_state& s) {

@@ -12,6 +12,7 @@ constexpr int MAX_CARS = RSG_MAX_CARS;
 // pack arenas of a batch with at most n cars per arena at that stride.
 struct ArenaLocal {
     int n_cars;
+    int mode;           // RSG_MODE_*
     Ball ball;
     Pad pads[PAD_NUM];
     uint64_t tick;

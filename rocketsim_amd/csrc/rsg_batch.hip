@@ -104,7 +104,7 @@ RSG_FN void step_arena(const BatchDev& d, const MeshView& mesh, int a, int ticks
     P.arena_id = (uint32_t)a;
     P.synth_controls = d.synth_controls != 0;
     P.auto_reset = d.auto_reset != 0;
-    const MeshView* mp = d.mode == RSG_MODE_SOCCAR ? &mesh : nullptr;
+    const MeshView* mp = mode_has_arena(d.mode) ? &mesh : nullptr;
     for (int t = 0; t < ticks; t++) arena_tick(A, P, mp);
     store_arena(d, a, A);
 }
@@ -159,7 +159,7 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
     P.arena_id = (uint32_t)a;
     P.synth_controls = d.synth_controls != 0;
     P.auto_reset = d.auto_reset != 0;
-    const MeshView* mp = d.mode == RSG_MODE_SOCCAR ? &mesh : nullptr;
+    const MeshView* mp = mode_has_arena(d.mode) ? &mesh : nullptr;
     int* concurrent = ahead + 1;
     if (live && k == 0) load_arena(d, a, A);
     __syncthreads();
@@ -172,13 +172,13 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
             const RayHit* pre = *ahead ? hits : nullptr;
             *concurrent = car_pre_ticks_independent(A, pre) ? 1 : 0;
             if (!*concurrent) {
-                World world(A, mp, d.mode == RSG_MODE_SOCCAR);
+                World world(A, mp, mode_has_arena(d.mode));
                 for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], pre);
             }
         }
         __syncthreads();
         if (live && *concurrent && k < A.n_cars) {
-            World world(A, mp, d.mode == RSG_MODE_SOCCAR);
+            World world(A, mp, mode_has_arena(d.mode));
             car_pre_tick(A, P, world, k, hits);
         }
         __syncthreads();
@@ -189,7 +189,7 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
         __syncthreads();
         if (live && k < A.n_cars) car_post_tick(A, P, k);
         __syncthreads();
-        if (live && d.mode == RSG_MODE_SOCCAR)
+        if (live && mode_has_arena(d.mode))
             for (int p = k; p < PAD_NUM; p += lanes) pad_check_cars(A, p);
         __syncthreads();
         if (live && k == 0) tick_end(A, P);
@@ -464,8 +464,8 @@ const char* rsg_last_error(void) { return g_err.c_str(); }
 
 rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     if (!desc || desc->n_arenas <= 0) { fail("rsg_batch_create: n_arenas must be > 0"); return nullptr; }
-    if (desc->mode != RSG_MODE_SOCCAR && desc->mode != RSG_MODE_THE_VOID) {
-        fail("rsg_batch_create: only SOCCAR and THE_VOID are supported");
+    if (desc->mode != RSG_MODE_SOCCAR && desc->mode != RSG_MODE_HEATSEEKER && desc->mode != RSG_MODE_THE_VOID) {
+        fail("rsg_batch_create: only SOCCAR, HEATSEEKER and THE_VOID are supported");
         return nullptr;
     }
     if (!(desc->tick_rate >= 15.f && desc->tick_rate <= 120.f)) {
@@ -473,7 +473,7 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
         return nullptr;
     }
     if (!desc->cars_per_arena) { fail("rsg_batch_create: cars_per_arena is required"); return nullptr; }
-    if (desc->mode == RSG_MODE_SOCCAR && !g_mesh.ready) {
+    if (mode_has_arena(desc->mode) && !g_mesh.ready) {
         fail("rsg_batch_create: no arena collision mesh loaded (call rsg_init_mesh first)");
         return nullptr;
     }
@@ -662,7 +662,7 @@ int launch_step(const rsg_batch* b, const BatchDev& dev, const Launch& L, int ti
     int block = L.per_block;
     int grid = (dev.n + block - 1) / block;
     MeshView mv{};
-    if (dev.mode == RSG_MODE_SOCCAR) {
+    if (mode_has_arena(dev.mode)) {
         if (!g_mesh.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
         mv = g_mesh.view;
     }
@@ -1086,7 +1086,7 @@ int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out) {
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++) {
         float y = ball_ref(h, a).v(BW_POS).y * BT_TO_UU;
-        out[a] = (b->dev.mode == RSG_MODE_SOCCAR &&
+        out[a] = (mode_has_arena(b->dev.mode) &&
                   fabsf(y) > (SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y + BALL_COLLISION_RADIUS_SOCCAR)) ? 1 : 0;
     }
     return 0;
@@ -1265,7 +1265,7 @@ int rsg_batch_get_snapshot(rsg_batch* b, int arena, rsg_arena_snapshot* out) {
     }
     out->last_car_id = last_id;
     ball_to_api(ball_ref(h, arena), out->ball);
-    if (b->dev.mode == RSG_MODE_SOCCAR)
+    if (mode_has_arena(b->dev.mode))
         for (int p = 0; p < PAD_NUM; p++) pad_to_api(pad_ref(h, arena, p), out->pads[p]);
     // raw records, cars in snapshot order
     BatchDev v = raw_view(out->raw, n);
@@ -1348,7 +1348,7 @@ int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s)
         ball_from_api(ball_ref(h, arena), s->ball);
     }
     ar.u(AW_CAR_ORDER) = ow;
-    if (b->dev.mode == RSG_MODE_SOCCAR)
+    if (mode_has_arena(b->dev.mode))
         for (int p = 0; p < PAD_NUM; p++) pad_from_api(pad_ref(h, arena, p), s->pads[p]);
     b->host_dirty = true;
     return 0;

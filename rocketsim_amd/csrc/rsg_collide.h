@@ -322,7 +322,9 @@ struct World {
             cp.friction = CARWORLD_COLLISION_FRICTION;
             cp.restitution = CARWORLD_COLLISION_RESTITUTION;
         } else if (b0 == 0 && b1 < 0) {
-            // ball + world: averaged "special" row (Arena.cpp:296-304)
+            // ball + world: Ball::_OnWorldCollision, then the averaged
+            // "special" row (Arena.cpp:296-304)
+            if (A.mode == RSG_MODE_HEATSEEKER) ball_world_hit_heatseeker(cp.normal);
             cp.special = true;
         } else if (b0 > 0 && b1 == 0) {
             car_ball_hit(cp, b0 - 1);
@@ -330,6 +332,25 @@ struct World {
             car_car_hit(cp, b0 - 1, b1 - 1);
         }
         if (b1 == -1 - ST_MESH && cp.tri >= 0) adjust_internal_edge(cp, *mesh, cp.tri);
+    }
+
+    // Ball::_OnWorldCollision for Heatseeker (src/Sim/Ball/Ball.cpp:326-358):
+    // a hit on the back wall of the net the ball seeks turns it round, with
+    // an extra impulse towards the other net.  `normal` points from the world
+    // to the ball.
+    RSG_FN void ball_world_hit_heatseeker(V3 normal) {
+        Ball& ball = A.ball;
+        if (ball.hs_dir == 0) return;
+        V3 pos = ball.b.pos * BT_TO_UU;
+        float rel_ny = normal.y * ball.hs_dir, rel_y = pos.y * ball.hs_dir;
+        if (rel_ny <= -HS_WALL_BOUNCE_CHANGE_Y_NORMAL && rel_y >= ARENA_EXTENT_Y - HS_WALL_BOUNCE_CHANGE_Y_THRESH) {
+            ball.hs_dir *= -1;
+            V3 vel = ball.b.vel * BT_TO_UU;
+            V3 to_goal = rs_normalized(v3(0, HS_TARGET_Y * ball.hs_dir, HS_TARGET_Z) - pos);
+            V3 dir = to_goal * (1 - HS_WALL_BOUNCE_UP_FRAC) + v3(0, 0, 1) * HS_WALL_BOUNCE_UP_FRAC;
+            V3 impulse = dir * len(vel) * HS_WALL_BOUNCE_FORCE_SCALE;
+            ball.vel_cache += impulse * UU_TO_BT;
+        }
     }
 
     // Arena::_BtCallback_OnCarBallCollision + Ball::_OnHit
@@ -359,6 +380,18 @@ struct World {
                 V3 added = (hit_dir * rel_speed) * curve_eval(curve_ball_extra_impulse(), rel_speed) * 1.f;
                 c.hit_extra_vel = added;
                 A.ball.vel_cache += added * UU_TO_BT;
+            }
+            if (A.mode == RSG_MODE_HEATSEEKER) {
+                // Ball.cpp:291-299: a touch by the other team speeds the
+                // ball up (at most once a second) and retargets it
+                Ball& ball = A.ball;
+                bool can_increase = (ball.hs_time > HS_MIN_SPEEDUP_INTERVAL) || (ball.hs_dir == 0);
+                float new_dir = c.team == RSG_TEAM_BLUE ? 1.f : -1.f;
+                if (can_increase && new_dir != ball.hs_dir) {
+                    ball.hs_time = 0;
+                    ball.hs_speed = rs_min(ball.hs_speed + HS_TARGET_SPEED_INCREMENT, HS_MAX_SPEED);
+                }
+                ball.hs_dir = new_dir;
             }
         }
     }

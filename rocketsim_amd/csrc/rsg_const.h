@@ -11,6 +11,7 @@
 #pragma once
 
 #include "rsg_math.h"
+#include "../../include/rsg_state.h"
 
 namespace rsg {
 
@@ -305,6 +306,34 @@ RSG_FN SpawnPos respawn_spawn(int i) {
         case 3: return SpawnPos{2688, -4608, (float)(PI_D / 2)};
     }
 }
+// Heatseeker kickoff spawns (src/RLConst.h:365-371): 4 slots, all facing +y.
+RSG_FN SpawnPos kickoff_spawn_heatseeker(int i) {
+    switch (i) {
+        default:
+        case 0: return SpawnPos{-1000, -4620, (float)(PI_D / 2)};
+        case 1: return SpawnPos{1000, -4620, (float)(PI_D / 2)};
+        case 2: return SpawnPos{-2000, -4620, (float)(PI_D / 2)};
+        case 3: return SpawnPos{2000, -4620, (float)(PI_D / 2)};
+    }
+}
+
+// Modes played in the Soccar arena: its mesh, boost pads and goals
+// (Arena.cpp:478-520, 893-901).
+RSG_FN bool mode_has_arena(int mode) { return mode == RSG_MODE_SOCCAR || mode == RSG_MODE_HEATSEEKER; }
+
+// MutatorConfig defaults per mode (src/Sim/MutatorConfig/MutatorConfig.cpp:
+// 20-40): Heatseeker spawns cars with full boost and never drains it.
+RSG_FN float spawn_boost(int mode) { return mode == RSG_MODE_HEATSEEKER ? BOOST_MAX : BOOST_SPAWN_AMOUNT; }
+RSG_FN float boost_used_per_second(int mode) { return mode == RSG_MODE_HEATSEEKER ? 0.f : BOOST_USED_PER_SECOND; }
+
+// Heatseeker ball (src/RLConst.h:168-188).
+constexpr float HS_INITIAL_TARGET_SPEED = 2900, HS_TARGET_SPEED_INCREMENT = 85, HS_MIN_SPEEDUP_INTERVAL = 1,
+                HS_TARGET_Y = 5120, HS_TARGET_Z = 320, HS_HORIZONTAL_BLEND = 1.45f, HS_VERTICAL_BLEND = 0.78f,
+                HS_SPEED_BLEND = 0.3f, HS_MAX_TURN_PITCH = (float)(7000 * PI_D / (1 << 15)), HS_MAX_SPEED = 4600,
+                HS_WALL_BOUNCE_CHANGE_Y_THRESH = 300, HS_WALL_BOUNCE_CHANGE_Y_NORMAL = 0.5f,
+                HS_WALL_BOUNCE_FORCE_SCALE = 1 / 3.f, HS_WALL_BOUNCE_UP_FRAC = 0.3f;
+constexpr float HS_BALL_START_X = -1000, HS_BALL_START_Y = -2220, HS_BALL_START_Z = 92.75f, HS_BALL_START_VY = -65,
+                HS_BALL_START_VZ = 650;
 
 // Boost pad locations (src/RLConst.h:264-309): 6 big then 28 small.
 RSG_FN V3 pad_location(int i) {

@@ -68,7 +68,8 @@ enum CarWord : int {
 enum BallWord : int {
     BW_POS = 0, BW_ROT = 3, BW_VEL = 12, BW_ANGVEL = 15, BW_TICKS_SINCE_SET = 18,
     BW_BP_CELL = 20, BW_BP_SEQ = 21,
-    BALL_WORDS = 22
+    BW_HS = 22,          // 3  Heatseeker: target net, target speed, time since hit
+    BALL_WORDS = 25
 };
 
 enum PadWord : int { PW_ACTIVE = 0, PW_COOLDOWN = 1, PW_PREV_LOCKED = 2, PAD_WORDS = 3 };

@@ -265,13 +265,13 @@ inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<u
         }
     }
 
-    // BallState::Serialize (src/Sim/Ball/Ball.cpp:19-21), heatseeker and
-    // dropshot fields at their defaults (Ball.h:24-47)
+    // BallState::Serialize (src/Sim/Ball/Ball.cpp:19-21), dropshot fields at
+    // their defaults (Ball.h:34-47)
     o.count(12);
     o.vec(s.ball.pos); o.rot(s.ball.rot); o.vec(s.ball.vel); o.vec(s.ball.ang_vel);
-    o.put<float>(0.f);
-    o.put<float>(2900.f);  // RLConst::Heatseeker::INITIAL_TARGET_SPEED
-    o.put<float>(0.f);
+    o.put<float>(s.ball.hs_y_target_dir);
+    o.put<float>(s.ball.hs_cur_target_speed);
+    o.put<float>(s.ball.hs_time_since_hit);
     o.put<int32_t>(1);
     o.put<float>(0.f);
     o.put<float>(0.f);
@@ -425,9 +425,12 @@ inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_ar
 
     if (in.get<uint32_t>() != 12) return "ball state field count is not 12";
     in.vec(s.ball.pos); in.rot(s.ball.rot); in.vec(s.ball.vel); in.vec(s.ball.ang_vel);
+    s.ball.hs_y_target_dir = in.get<float>();
+    s.ball.hs_cur_target_speed = in.get<float>();
+    s.ball.hs_time_since_hit = in.get<float>();
     {
-        // heatseeker / dropshot fields: not simulated in these modes
-        uint8_t skip[4 + 4 + 4 + 4 + 4 + 4 + 1 + 8];
+        // dropshot fields: not simulated in these modes
+        uint8_t skip[4 + 4 + 4 + 1 + 8];
         in.raw(skip, sizeof(skip));
     }
     {

@@ -124,6 +124,8 @@ struct Ball {
     Body b;
     uint64_t ticks_since_set;
     V3 vel_cache;
+    // BallState::HeatseekerInfo (src/Sim/Ball/Ball.h:24-31)
+    float hs_dir, hs_speed, hs_time;
 };
 
 struct Pad {
@@ -347,7 +349,7 @@ RSG_FN void car_update_wheels(Car& c, const CarPhys& cp, float dt, int n_contact
 }
 
 // _UpdateBoost (Car.cpp:486-533), Soccar mutators (no recharge).
-RSG_FN void car_update_boost(Car& c, float dt) {
+RSG_FN void car_update_boost(Car& c, float dt, float used_per_second) {
     bool has_boost = c.boost > 0;
     bool boosting = flag(c.flags, RSG_CF_IS_BOOSTING);
     if (has_boost) {
@@ -362,7 +364,7 @@ RSG_FN void car_update_boost(Car& c, float dt) {
     setflag(c.flags, RSG_CF_IS_BOOSTING, boosting);
     if (boosting) c.boosting_time += dt; else c.boosting_time = 0;
     if (boosting) {
-        c.boost = rs_max(c.boost - BOOST_USED_PER_SECOND * dt, 0.f);
+        c.boost = rs_max(c.boost - used_per_second * dt, 0.f);
         float accel = flag(c.flags, RSG_CF_ON_GROUND) ? BOOST_ACCEL_GROUND : BOOST_ACCEL_AIR;
         body_apply_central_force(c.b, accel * UU_TO_BT * car_forward(c) * CAR_MASS_BT);
         c.time_since_boosted = 0;

@@ -177,6 +177,7 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     CarRef ar = arena_ref(d, a);
     A.tick = ar.u64(AW_TICK);
     A.n_cars = (int)ar.u(AW_NUM_CARS);
+    A.mode = d.mode;
     A.goals_blue = ar.u(AW_GOALS_BLUE);
     A.goals_orange = ar.u(AW_GOALS_ORANGE);
     A.last_goal_tick = ar.u64(AW_LAST_GOAL_TICK);
@@ -219,8 +220,11 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.ball.b.ang = br.v(BW_ANGVEL);
     A.ball.ticks_since_set = br.u64(BW_TICKS_SINCE_SET);
     ball_init_body(A.ball);
+    A.ball.hs_dir = br.f(BW_HS);
+    A.ball.hs_speed = br.f(BW_HS + 1);
+    A.ball.hs_time = br.f(BW_HS + 2);
     load_bp(br, BW_BP_CELL, BW_BP_SEQ, A.bp_cell[0], A.bp_seq[0]);
-    if (d.mode == RSG_MODE_SOCCAR) {
+    if (mode_has_arena(d.mode)) {
         for (int p = 0; p < PAD_NUM; p++) {
             CarRef pr = pad_ref(d, a, p);
             A.pads[p].active = pr.u(PW_ACTIVE) != 0;
@@ -259,7 +263,10 @@ RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
     br.sv(BW_VEL, A.ball.b.vel);
     br.sv(BW_ANGVEL, A.ball.b.ang);
     br.su64(BW_TICKS_SINCE_SET, A.ball.ticks_since_set);
-    if (d.mode == RSG_MODE_SOCCAR) {
+    br.sf(BW_HS, A.ball.hs_dir);
+    br.sf(BW_HS + 1, A.ball.hs_speed);
+    br.sf(BW_HS + 2, A.ball.hs_time);
+    if (mode_has_arena(d.mode)) {
         for (int p = 0; p < PAD_NUM; p++) {
             CarRef pr = pad_ref(d, a, p);
             pr.u(PW_ACTIVE) = A.pads[p].active ? 1u : 0u;
@@ -275,9 +282,11 @@ RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
 // reference's std::default_random_engine.  Cars are taken in slot order per
 // team.
 RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id) {
+    const bool hs = A.mode == RSG_MODE_HEATSEEKER;
+    const int n_slots = hs ? 4 : 5;
     int slots[5] = {0, 1, 2, 3, 4};
     uint64_t h = rsg_draw(seed ^ 0x4B1C0FFull, arena_id, 0xFFFFu, A.resets, 7);
-    for (int i = 4; i > 0; i--) {
+    for (int i = n_slots - 1; i > 0; i--) {
         int j = (int)((h >> (i * 8)) % (uint64_t)(i + 1));
         int t = slots[i]; slots[i] = slots[j]; slots[j] = t;
     }
@@ -288,16 +297,16 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
         Car& c = A.cars[A.order[oi]];
         int k = idx_team[c.team]++;
         SpawnPos sp;
-        if (k < 5) {
-            sp = kickoff_spawn(slots[k]);
+        if (k < n_slots) {
+            sp = hs ? kickoff_spawn_heatseeker(slots[k]) : kickoff_spawn(slots[k]);
         } else {
-            int ri = (k - 5) % 5;
+            int ri = (k - n_slots) % n_slots;
             ri = ri < 4 ? ri : 3;
             sp = respawn_spawn(ri);
             sp.y += 250.f * num_at_respawn[ri];
             num_at_respawn[ri]++;
         }
-        car_respawn_state(c, 0, BOOST_SPAWN_AMOUNT);
+        car_respawn_state(c, 0, spawn_boost(A.mode));
         V3 pos = v3(sp.x, sp.y, CAR_SPAWN_REST_Z);
         float yaw = sp.yaw;
         if (c.team != RSG_TEAM_BLUE) {
@@ -310,11 +319,21 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
         body_update_inertia(c.b);
         c.flags = RSG_CF_ON_GROUND | (c.flags & RSG_CF_BODY_DISABLED);
     }
-    // BallState() then SetState (Ball.cpp:36-51)
+    // BallState() then SetState (Ball.cpp:36-51); Heatseeker starts the ball
+    // rolling towards a random side's corner (Arena.cpp:200-204), the side
+    // from the same counter-based draw
     A.ball.b.pos = v3(0, 0, BALL_REST_Z) * UU_TO_BT;
     A.ball.b.rot = m3_identity();
     A.ball.b.vel = v3zero();
     A.ball.b.ang = v3zero();
+    if (hs) {
+        const float sy = ((h >> 56) & 1u) ? 1.f : -1.f;
+        A.ball.b.pos = v3(HS_BALL_START_X, HS_BALL_START_Y * sy, HS_BALL_START_Z) * UU_TO_BT;
+        A.ball.b.vel = v3(0.f, HS_BALL_START_VY * sy, HS_BALL_START_VZ) * UU_TO_BT;
+    }
+    A.ball.hs_dir = 0;
+    A.ball.hs_speed = HS_INITIAL_TARGET_SPEED;
+    A.ball.hs_time = 0;
     A.ball.ticks_since_set = 0;
     A.ball.vel_cache = v3zero();
     body_update_inertia(A.ball.b);
@@ -400,7 +419,7 @@ RSG_FN bool tick_begin(ArenaLocal& A, const TickParams& P) {
 // ahead of the loop (valid when tick_begin returned true): poses, broadphase
 // cells and contact-response flags are those the car sees at its turn.
 RSG_FN void tick_cast_ray(ArenaLocal& A, const TickParams& P, const MeshView* mesh, int ci, int wheel, RayHit& out) {
-    World world(A, mesh, P.mode == RSG_MODE_SOCCAR && mesh != nullptr);
+    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
     int pos = 0;
     while (pos < A.n_cars && A.order[pos] != ci) pos++;
     world.ray_caster_pos = pos;
@@ -427,7 +446,7 @@ RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int c
         c.demo_timer = rs_max(c.demo_timer - dt, 0.f);
         if (c.demo_timer == 0) {
             int slot = (int)(rsg_draw(P.seed ^ 0xDE40ull, P.arena_id, (uint32_t)ci, (uint32_t)A.tick, 9) % 4u);
-            car_respawn_state(c, slot, BOOST_SPAWN_AMOUNT);
+            car_respawn_state(c, slot, spawn_boost(P.mode));
         }
         c.b.active = false;  // DISABLE_SIMULATION + no contact response
     } else {
@@ -465,7 +484,7 @@ RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int c
         car_update_auto_roll(c, n_contact);
     setflag(c.flags, RSG_CF_WORLD_CONTACT, false);
     vehicle_second(c, cp, dt);
-    car_update_boost(c, dt);
+    car_update_boost(c, dt, boost_used_per_second(P.mode));
 }
 
 // Whether the car pre-ticks of this tick are independent of their order, so
@@ -481,7 +500,7 @@ RSG_FN bool car_pre_ticks_independent(const ArenaLocal& A, const RayHit* pre) {
 
 // Pads pre-tick (BoostPad.cpp:51-60).
 RSG_FN void pads_pre_tick(ArenaLocal& A, const TickParams& P) {
-    if (P.mode != RSG_MODE_SOCCAR || A.n_cars == 0) return;
+    if (!mode_has_arena(P.mode) || A.n_cars == 0) return;
     for (int p = 0; p < PAD_NUM; p++) {
         Pad& pad = A.pads[p];
         if (pad.cooldown > 0) pad.cooldown = rs_max(pad.cooldown - P.dt, 0.f);
@@ -548,7 +567,7 @@ RSG_FN void pad_check_cars(ArenaLocal& A, int p) {
 // kickoff reset on goal.
 RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
     const float dt = P.dt;
-    const bool soccar = P.mode == RSG_MODE_SOCCAR;
+    const bool soccar = mode_has_arena(P.mode);
     // 6. pads post-tick (BoostPad.cpp:88-107)
     if (soccar && A.n_cars > 0) {
         for (int p = 0; p < PAD_NUM; p++) {
@@ -602,9 +621,71 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
     RSG_PROF_MARK(8);
 }
 
-// The world step of the tick (btDiscreteDynamicsWorld::stepSimulation).
+// Angle::FromVec (src/Math/MathTypes/MathTypes.cpp:80-100): yaw and pitch
+// of a direction.
+RSG_FN void hs_angle_from_vec(V3 f, float& yaw, float& pitch) {
+    const float eps = 1.1920928955078125e-07f;
+    if (fabsf(f.y) > eps || fabsf(f.x) > eps) {
+        yaw = atan2f(f.y, f.x);
+        float d2 = sqrtf(f.x * f.x + f.y * f.y);
+        pitch = -atan2f(-f.z, d2);
+    } else {
+        yaw = 0;
+        pitch = f.z > eps ? (float)(PI_D / 2) : (f.z < -eps ? (float)(-PI_D / 2) : 0.f);
+    }
+}
+
+// Math::WrapNormalizeFloat (src/Math/Math.cpp:66-73).
+RSG_FN float hs_wrap(float v, float mm) {
+    float r = fmodf(v, mm * 2);
+    if (r > mm) r -= mm * 2;
+    else if (r < -mm) r += mm * 2;
+    return r;
+}
+
+// Math::RoundAngleUE3 (src/Math/Math.cpp:75-91): the rotator's 16-bit
+// units, rounded down to multiples of 4.
+RSG_FN float hs_round_ue3(float a) {
+    constexpr float to_ints = (float)((double)(1 << 15) / PI_D);
+    constexpr float back = (1.f / to_ints) * 4.f;
+    int r = ((int)(a * to_ints) >> 2) & (0x4000 - 1);
+    return (float)r * back;
+}
+
+// Ball::_PreTickUpdate for Heatseeker (src/Sim/Ball/Ball.cpp:161-206): once
+// a team has touched it, the ball turns its velocity towards the target net
+// and blends its speed towards the target speed.
+RSG_FN void ball_pre_tick_heatseeker(Ball& ball, float dt) {
+    if (ball.hs_dir == 0) return;
+    const float pi_f = (float)PI_D, half_pi_f = (float)(PI_D / 2);
+    V3 pos = ball.b.pos * BT_TO_UU, vel = ball.b.vel * BT_TO_UU;
+    float vyaw, vpitch, gyaw, gpitch;
+    hs_angle_from_vec(vel, vyaw, vpitch);
+    hs_angle_from_vec(v3(0, HS_TARGET_Y * ball.hs_dir, HS_TARGET_Z) - pos, gyaw, gpitch);
+    // Angle::GetDeltaTo, normalised (MathTypes.h:305-310)
+    float dyaw = hs_wrap(gyaw - vyaw, pi_f), dpitch = hs_wrap(gpitch - vpitch, half_pi_f);
+    float speed = len(vel);
+    float base = (speed / HS_MAX_SPEED) * dt;
+    float yaw = vyaw + dyaw * base * HS_HORIZONTAL_BLEND;
+    float pitch = vpitch + dpitch * base * HS_VERTICAL_BLEND;
+    yaw = hs_wrap(yaw, pi_f);
+    pitch = hs_wrap(pitch, half_pi_f);
+    pitch = clampf(pitch, -HS_MAX_TURN_PITCH, HS_MAX_TURN_PITCH);
+    yaw = hs_round_ue3(yaw);
+    pitch = hs_round_ue3(pitch);
+    float new_speed = speed + ((ball.hs_speed - speed) * HS_SPEED_BLEND);
+    // Angle::GetForwardVec (MathTypes.cpp:102-110)
+    float cp = cosf(-pitch), cy = cosf(yaw), sy = sinf(yaw), sp = sinf(-pitch);
+    V3 dir = v3(cp * cy, cp * sy, -sp);
+    ball.b.vel = (dir * new_speed) * UU_TO_BT;
+    ball.hs_time += dt;
+}
+
+// The world step of the tick (btDiscreteDynamicsWorld::stepSimulation),
+// preceded by the ball's pre-tick, which runs after the pads' (Arena.cpp:715).
 RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const MeshView* mesh) {
-    World world(A, mesh, P.mode == RSG_MODE_SOCCAR && mesh != nullptr);
+    if (P.mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(A.ball, P.dt);
+    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
     world.step(P.dt, P);
     RSG_PROF_MARK(7);
 }
@@ -613,7 +694,7 @@ RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const MeshView* mesh)
 // wheel's ray result (car-major, 4 per car) when cast ahead, else null.
 RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, const RayHit* pre) {
     {
-        World world(A, mesh, P.mode == RSG_MODE_SOCCAR && mesh != nullptr);
+        World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
         RSG_PROF_MARK(0);
         // 2. car pre-tick, in the arena's car update order
         for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], pre);
@@ -625,7 +706,7 @@ RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, 
     tick_world(A, P, mesh);
     // 5. cars post-tick + pad collisions
     for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
-    if (P.mode == RSG_MODE_SOCCAR)
+    if (mode_has_arena(P.mode))
         for (int p = 0; p < PAD_NUM; p++) pad_check_cars(A, p);
     tick_end(A, P);
 }

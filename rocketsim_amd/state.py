@@ -9,6 +9,7 @@ plus the per-wheel vehicle values carried between ticks
 import ctypes as C
 
 MODE_SOCCAR = 0
+MODE_HEATSEEKER = 2
 MODE_THE_VOID = 5
 
 PRESET_OCTANE, PRESET_DOMINUS, PRESET_PLANK, PRESET_BREAKOUT, PRESET_HYBRID, PRESET_MERC = range(6)
@@ -92,6 +93,8 @@ class BallState(C.Structure):
         ("tick_count_since_update", C.c_uint64),
         ("pos", F3), ("rot", F9), ("vel", F3), ("ang_vel", F3),
         ("bp_cell", C.c_int32), ("bp_seq", C.c_uint32),
+        # Heatseeker: target net (+1 orange, -1 blue, 0 none), target speed, time since hit
+        ("hs_y_target_dir", C.c_float), ("hs_cur_target_speed", C.c_float), ("hs_time_since_hit", C.c_float),
     ]
 
 

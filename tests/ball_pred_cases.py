@@ -4,6 +4,7 @@ BallPredTracker (src/Sim/BallPredTracker/BallPredTracker.cpp) in the oracle.
 
 Test infrastructure only.
 """
+import ctypes as C
 import random
 
 import numpy as np
@@ -114,7 +115,8 @@ def as_array(states):
 
 
 def from_array(arr):
-    return [BallState.from_buffer_copy(arr[i].tobytes()) for i in range(arr.shape[0])]
+    # records written before the Heatseeker fields were appended are zero-padded
+    return [BallState.from_buffer_copy(arr[i].tobytes().ljust(C.sizeof(BallState), b"\0")) for i in range(arr.shape[0])]
 
 
 def save(path, run, T):

@@ -25,7 +25,7 @@ import random
 import numpy as np
 
 from rocketsim_amd.state import (BallState, CarState, Controls, PadState, CF_HAS_JUMPED,
-                                 CF_ON_GROUND, CF_SUPERSONIC, MODE_SOCCAR, MODE_THE_VOID, NUM_PADS)
+                                 CF_ON_GROUND, CF_SUPERSONIC, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID, NUM_PADS)
 
 EXACT_CAR = {"flags", "car_contact_other_id", "hit_tick", "hit_extra_tick", "tick_count_since_update",
              "last_controls.jump", "last_controls.boost", "last_controls.handbrake", "last_controls._pad",
@@ -97,6 +97,12 @@ def compare_ball(ref, got):
         bad.append(("ball.tick_count_since_update", ref.tick_count_since_update, got.tick_count_since_update, None))
     if ref.bp_cell != got.bp_cell:
         bad.append(("ball.bp_cell", ref.bp_cell, got.bp_cell, None))
+    # Heatseeker state: the target net and speed are discrete, the time a sum of tick times
+    if (ref.hs_y_target_dir, ref.hs_cur_target_speed) != (got.hs_y_target_dir, got.hs_cur_target_speed):
+        bad.append(("ball.hs", (ref.hs_y_target_dir, ref.hs_cur_target_speed),
+                    (got.hs_y_target_dir, got.hs_cur_target_speed), None))
+    if abs(ref.hs_time_since_hit - got.hs_time_since_hit) > 1e-5 * max(1.0, abs(ref.hs_time_since_hit)):
+        bad.append(("ball.hs_time_since_hit", ref.hs_time_since_hit, got.hs_time_since_hit, None))
     return bad
 
 
@@ -365,7 +371,7 @@ def _one_tick(A, mode, teams, presets, ctrls):
     s.car_order = A.car_order()
     s.cars_in = [A.get_car(i) for i in range(n)]
     s.ball_in = A.get_ball()
-    s.pads_in = A.get_pads() if mode == MODE_SOCCAR else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
+    s.pads_in = A.get_pads() if mode in (MODE_SOCCAR, MODE_HEATSEEKER) else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
     s.tick_in = A.tick_count
     s.ctrls = [Controls.from_buffer_copy(c) for c in ctrls]
     for i in range(n):
@@ -375,7 +381,7 @@ def _one_tick(A, mode, teams, presets, ctrls):
     s.events = [(e.type, e.tick, e.a, e.b) for e in A.pop_events()]
     s.cars_out = [A.get_car(i) for i in range(n)]
     s.ball_out = A.get_ball()
-    s.pads_out = A.get_pads() if mode == MODE_SOCCAR else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
+    s.pads_out = A.get_pads() if mode in (MODE_SOCCAR, MODE_HEATSEEKER) else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
     s.tick_out = A.tick_count
     return s
 
@@ -386,10 +392,14 @@ def _pack(structs, cls):
     return np.frombuffer(b"".join(bytes(x) for x in structs), dtype=np.uint8) if structs else np.zeros(0, np.uint8)
 
 
-def _unpack(arr, cls):
+def _unpack(arr, cls, count=None):
+    """Records of `cls` from packed bytes.  With `count`, the stored record
+    size is len / count, and records written before fields were appended to
+    the struct are zero-padded (the ball's Heatseeker fields)."""
     raw = arr.tobytes()
     sz = C.sizeof(cls)
-    return [cls.from_buffer_copy(raw[i:i + sz]) for i in range(0, len(raw), sz)]
+    rec = len(raw) // count if count else sz
+    return [cls.from_buffer_copy(raw[i:i + rec].ljust(sz, b"\0")) for i in range(0, len(raw), rec)]
 
 
 def save_samples(path, samples):
@@ -420,7 +430,7 @@ def load_samples(path):
     order = list(z["car_order"])
     cars_in, cars_out = _unpack(z["cars_in"], CarState), _unpack(z["cars_out"], CarState)
     ctrls = _unpack(z["ctrls"], Controls)
-    ball_in, ball_out = _unpack(z["ball_in"], BallState), _unpack(z["ball_out"], BallState)
+    ball_in, ball_out = _unpack(z["ball_in"], BallState, len(ncars)), _unpack(z["ball_out"], BallState, len(ncars))
     pads_in, pads_out = _unpack(z["pads_in"], PadState), _unpack(z["pads_out"], PadState)
     out = []
     k = 0
@@ -490,7 +500,7 @@ def check_samples(samples, results, max_report=10):
         bad += compare_ball(s.ball_out, ball)
         if bp_order(s.ball_out, s.cars_out) != bp_order(ball, cars):
             bad.append(("bp_order", bp_order(s.ball_out, s.cars_out), bp_order(ball, cars), None))
-        if s.mode == MODE_SOCCAR:
+        if s.mode in (MODE_SOCCAR, MODE_HEATSEEKER):
             bad += compare_pads(s.pads_out, pads)
         if tick != s.tick_out:
             bad.append(("tick", s.tick_out, tick, None))
