@@ -105,16 +105,17 @@ inline void preset_config_words(int preset, float out[1 + 3 + 3 + 5 + 5]) {
     *o++ = r.back_rad;
 }
 
-// The MutatorConfig of Soccar and THE_VOID (MutatorConfig.h:16-75 with
-// MutatorConfig.cpp:5-40), in MUTATOR_CONFIG_SERIALIZATION_FIELDS order.
-inline void write_default_mutators(ByteOut& o) {
+// The default MutatorConfig of `mode` (MutatorConfig.h:16-75 with
+// MutatorConfig.cpp:5-40: Heatseeker spawns with 100 boost and uses none), in
+// MUTATOR_CONFIG_SERIALIZATION_FIELDS order.
+inline void write_default_mutators(ByteOut& o, int mode) {
     o.put<uint16_t>(30);
     o.count(30);
     o.vec(0.f, 0.f, GRAVITY_Z);
     const float f1[] = {CAR_MASS_BT, CARWORLD_COLLISION_FRICTION, CARWORLD_COLLISION_RESTITUTION, BALL_MASS_BT,
                         BALL_MAX_SPEED, BALL_DRAG, BALL_FRICTION, BALL_RESTITUTION, JUMP_ACCEL,
-                        JUMP_IMMEDIATE_FORCE, BOOST_ACCEL_GROUND, BOOST_ACCEL_AIR, BOOST_USED_PER_SECOND,
-                        DEMO_RESPAWN_TIME, BOOST_SPAWN_AMOUNT, BUMP_COOLDOWN_TIME, PAD_COOLDOWN_BIG,
+                        JUMP_IMMEDIATE_FORCE, BOOST_ACCEL_GROUND, BOOST_ACCEL_AIR, boost_used_per_second(mode),
+                        DEMO_RESPAWN_TIME, spawn_boost(mode), BUMP_COOLDOWN_TIME, PAD_COOLDOWN_BIG,
                         PAD_COOLDOWN_SMALL, 1.f, 1.f, BALL_COLLISION_RADIUS_SOCCAR};
     for (float f : f1) o.put<float>(f);
     o.b8(false);  // unlimitedFlips
@@ -188,7 +189,8 @@ inline bool records_from_raw(rsg_arena_snapshot& s) {
 
 // Arena::Serialize (Arena.cpp:546-573).
 inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<uint8_t>& out) {
-    if (s.mode != RSG_MODE_SOCCAR && s.mode != RSG_MODE_THE_VOID) return "mode must be SOCCAR or THE_VOID";
+    if (s.mode != RSG_MODE_SOCCAR && s.mode != RSG_MODE_HEATSEEKER && s.mode != RSG_MODE_THE_VOID)
+        return "mode must be SOCCAR, HEATSEEKER or THE_VOID";
     if (s.n_cars < 0 || s.n_cars > RSG_MAX_CARS) return "n_cars out of range";
     ByteOut o;
     o.count(4);
@@ -254,7 +256,7 @@ inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<u
         o.put<float>(cs.car_contact_cooldown);
     }
 
-    if (s.mode == RSG_MODE_SOCCAR) {
+    if (mode_has_arena(s.mode)) {
         // BoostPadState::Serialize (src/Sim/BoostPad/BoostPad.cpp:21-25)
         o.put<uint32_t>(RSG_NUM_PADS);
         for (int p = 0; p < RSG_NUM_PADS; p++) {
@@ -278,7 +280,7 @@ inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<u
     o.b8(false);
     o.put<uint64_t>(0);
 
-    write_default_mutators(o);
+    write_default_mutators(o, s.mode);
 
     // extension block: the raw records (built from the state records when the
     // snapshot carries none)
@@ -327,7 +329,8 @@ inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_ar
     s.tick_time = in.get<float>();
     s.tick_count = in.get<uint64_t>();
     s.last_car_id = in.get<uint32_t>();
-    if (s.mode != RSG_MODE_SOCCAR && s.mode != RSG_MODE_THE_VOID) return "game mode is not SOCCAR or THE_VOID";
+    if (s.mode != RSG_MODE_SOCCAR && s.mode != RSG_MODE_HEATSEEKER && s.mode != RSG_MODE_THE_VOID)
+        return "game mode is not SOCCAR, HEATSEEKER or THE_VOID";
     {
         ByteOut want;
         write_default_arena_config(want);
@@ -413,7 +416,7 @@ inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_ar
     // creation order is file order; ids are forced to the written ones
     for (int k = 0; k < s.n_cars; k++) creation_rank[k] = k;
 
-    if (s.mode == RSG_MODE_SOCCAR) {
+    if (mode_has_arena(s.mode)) {
         if (in.get<uint32_t>() != RSG_NUM_PADS) return "boost pad count is not 34";
         for (int p = 0; p < RSG_NUM_PADS; p++) {
             if (in.get<uint32_t>() != 3) return "boost pad field count is not 3";
@@ -435,7 +438,7 @@ inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_ar
     }
     {
         ByteOut want;
-        write_default_mutators(want);
+        write_default_mutators(want, s.mode);
         std::vector<uint8_t> got(want.d.size());
         in.raw(got.data(), got.size());
         if (in.bad) return "truncated";

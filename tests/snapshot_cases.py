@@ -5,7 +5,7 @@ import ctypes as C
 import random
 
 import parity
-from rocketsim_amd.state import ArenaSnapshot, MODE_SOCCAR
+from rocketsim_amd.state import ArenaSnapshot, MODE_HEATSEEKER, MODE_SOCCAR
 
 # Ball.h:45: DropshotInfo::lastDamageTick has no initializer, so the
 # reference serializes whatever its memory holds.  It is the last field of the
@@ -35,7 +35,7 @@ def snapshot_from_ref(A, presets, mode):
         s.controls[k] = A.get_controls(i)
         s.cars[k] = A.get_car(i)
     s.ball = A.get_ball()
-    if mode == MODE_SOCCAR:
+    if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
         for p, pd in enumerate(A.get_pads()):
             s.pads[p] = pd
     return s
@@ -52,7 +52,7 @@ def played_arenas(ref, mode, n_arenas, ticks, seed, snap_every):
         presets = [rng.randint(0, 5) for _ in range(4)]
         for i, p in enumerate(presets):
             A.add_car(i % 2, p)
-        if mode == MODE_SOCCAR:
+        if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
             A.reset_kickoff(rng.randint(0, 1000))
         for t in range(ticks):
             if t % 20 == 0:
