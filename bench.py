@@ -37,7 +37,7 @@ BYTES_PER_ARENA_TICK_2V2 = 2488
 # HBM bytes per launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE passes
 # over `bench.py --profile-probe` (counter factor 1.0 from a 64 MiB copy),
 # keyed by (arenas, ticks per launch, arenas per workgroup).
-HBM_TRAFFIC_PER_LAUNCH = {("rsg_step_arena_coop_kernel", 4096, 8, 4): (338808 + 584621) * 1024}
+HBM_TRAFFIC_PER_LAUNCH = {("rsg_step_arena_coop_kernel", 4096, 8, 4): (339319 + 586199) * 1024}
 
 
 def cpu_baseline(target_seconds=10.0):
