@@ -1,7 +1,7 @@
 // rsg_sim.h -- per-arena tick of the batched Soccar step: car game logic,
 // raycast vehicle, rigid-body integration and the solver's body write-back.
 //
-// Everything here is __host__ __device__ so the kernels in rsg_kernels.hip can
+// Everything here is __host__ __device__ so the kernels in rsg_batch.hip can
 // run it per arena (one lane per arena) and per phase (one lane per car /
 // wheel / pair).  The collision phases (world queries, narrowphase, contact
 // solver) live in rsg_collide.h and rsg_solver.h.
