@@ -21,6 +21,12 @@ import ball_pred_cases  # noqa: E402
 import parity  # noqa: E402
 
 
+def heatseeker():
+    ref.init()
+    hs = parity.sample_heatseeker(ref, n_traj=12, ticks=120, seed=20261207, stride=3)
+    parity.save_samples(os.path.join(HERE, "heatseeker_tf.npz"), hs)
+
+
 def main():
     ref.init()
     void = parity.sample_void(ref, n_traj=24, ticks_per_traj=8, seed=20261202)
@@ -38,6 +44,8 @@ def main():
     walls = parity.sample_soccar_melee(ref, n_traj=24, ticks_per_traj=100, seed=20261206, n_cars=4, stride=2,
                                        near_walls=True)
     parity.save_samples(os.path.join(HERE, "soccar_walls_tf.npz"), walls)
+
+    heatseeker()
     print("soccar_walls_tf.npz:", len(walls), "samples")
     states = ball_pred_cases.initial_states(ref, 10, seed=20261207)
     run = ball_pred_cases.reference_run(ref, states, 120)

@@ -349,6 +349,61 @@ def assert_parity(samples, results, max_frac=0.0):
     return nbad
 
 
+def sample_heatseeker(ref, n_traj, ticks, seed, stride):
+    """Heatseeker trajectories (GameMode::HEATSEEKER in the Soccar arena),
+    cycling through three scenarios: cars driven at the kickoff ball, whose
+    first touch starts the seeking; a ball already seeking a net at
+    2900-4600 uu/s; a seeking ball about to hit the back wall of its target
+    net, which turns it round.  Every `stride`-th tick is a sample."""
+    rng = random.Random(seed)
+    out = []
+    for k in range(n_traj):
+        kind = ("kickoff", "seeking", "back_wall")[k % 3]
+        A = ref.RefArena(MODE_HEATSEEKER)
+        n = rng.choice([2, 4])
+        teams = [i % 2 for i in range(n)]
+        presets = [rng.randint(0, 5) for _ in range(n)]
+        for t, p in zip(teams, presets):
+            A.add_car(t, p)
+        A.reset_kickoff(rng.randint(0, 1000))
+        b = A.get_ball()
+        if kind == "seeking":
+            d = rng.choice([-1.0, 1.0])
+            b.pos[:] = [rng.uniform(-3000, 3000), rng.uniform(-3500, 3500), rng.uniform(200, 1500)]
+            ang = rng.uniform(0, 2 * math.pi)
+            spd = rng.uniform(1500, 3500)
+            b.vel[:] = [spd * math.cos(ang), spd * math.sin(ang), rng.gauss(0, 300)]
+            b.hs_y_target_dir, b.hs_cur_target_speed = d, rng.choice([2900.0, 2985.0, 3500.0, 4600.0])
+            b.hs_time_since_hit = rng.uniform(0, 3)
+        elif kind == "back_wall":
+            d = rng.choice([-1.0, 1.0])
+            b.pos[:] = [rng.choice([-1, 1]) * rng.uniform(1100, 3000), d * rng.uniform(4500, 4800), rng.uniform(300, 1500)]
+            b.vel[:] = [rng.gauss(0, 300), d * rng.uniform(2000, 4000), rng.gauss(0, 300)]
+            b.hs_y_target_dir, b.hs_cur_target_speed, b.hs_time_since_hit = d, 3500.0, 0.5
+        if kind != "kickoff":
+            b.ang_vel[:] = [rng.gauss(0, 2) for _ in range(3)]
+            A.set_ball(b)
+        ctrls = [rand_controls(rng) for _ in range(n)]
+        for c in ctrls:
+            c.throttle, c.jump = 1.0, 0
+        hold = [rng.randint(10, 60) for _ in range(n)]
+        for t in range(ticks):
+            for i in range(n):
+                hold[i] -= 1
+                if hold[i] <= 0:
+                    ctrls[i] = rand_controls(rng)
+                    ctrls[i].throttle = rng.choice([1.0, 1.0, 0.5])
+                    hold[i] = rng.randint(10, 60)
+            if t % stride == 0:
+                out.append(_one_tick(A, MODE_HEATSEEKER, teams, presets, ctrls))
+            else:
+                for i in range(n):
+                    A.set_controls(i, ctrls[i])
+                A.step(1)
+        A.close()
+    return out
+
+
 def _one_tick(A, mode, teams, presets, ctrls):
     """One teacher-forced sample.  The oracle is first set to its own state as
     read through the API (positions and velocities in uu), repeatedly until

@@ -14,12 +14,14 @@ Tolerances are those of tests/parity.py; the Heatseeker fields of the ball
 (target net, target speed, time since the last speed-up) are compared too.
 """
 import math
-import random
+import os
 
 import pytest
 
 import parity
 from rocketsim_amd.state import MODE_HEATSEEKER
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def _oracle():
@@ -30,63 +32,13 @@ def _oracle():
     return ref
 
 
-def sample_heatseeker(ref, n_traj, ticks, seed, stride):
-    rng = random.Random(seed)
-    out = []
-    for k in range(n_traj):
-        kind = ("kickoff", "seeking", "back_wall")[k % 3]
-        A = ref.RefArena(MODE_HEATSEEKER)
-        n = rng.choice([2, 4])
-        teams = [i % 2 for i in range(n)]
-        presets = [rng.randint(0, 5) for _ in range(n)]
-        for t, p in zip(teams, presets):
-            A.add_car(t, p)
-        A.reset_kickoff(rng.randint(0, 1000))
-        b = A.get_ball()
-        if kind == "seeking":
-            d = rng.choice([-1.0, 1.0])
-            b.pos[:] = [rng.uniform(-3000, 3000), rng.uniform(-3500, 3500), rng.uniform(200, 1500)]
-            ang = rng.uniform(0, 2 * math.pi)
-            spd = rng.uniform(1500, 3500)
-            b.vel[:] = [spd * math.cos(ang), spd * math.sin(ang), rng.gauss(0, 300)]
-            b.hs_y_target_dir, b.hs_cur_target_speed = d, rng.choice([2900.0, 2985.0, 3500.0, 4600.0])
-            b.hs_time_since_hit = rng.uniform(0, 3)
-        elif kind == "back_wall":
-            d = rng.choice([-1.0, 1.0])
-            b.pos[:] = [rng.choice([-1, 1]) * rng.uniform(1100, 3000), d * rng.uniform(4500, 4800), rng.uniform(300, 1500)]
-            b.vel[:] = [rng.gauss(0, 300), d * rng.uniform(2000, 4000), rng.gauss(0, 300)]
-            b.hs_y_target_dir, b.hs_cur_target_speed, b.hs_time_since_hit = d, 3500.0, 0.5
-        if kind != "kickoff":
-            b.ang_vel[:] = [rng.gauss(0, 2) for _ in range(3)]
-            A.set_ball(b)
-        ctrls = [parity.rand_controls(rng) for _ in range(n)]
-        for c in ctrls:
-            c.throttle, c.jump = 1.0, 0
-        hold = [rng.randint(10, 60) for _ in range(n)]
-        for t in range(ticks):
-            for i in range(n):
-                hold[i] -= 1
-                if hold[i] <= 0:
-                    ctrls[i] = parity.rand_controls(rng)
-                    ctrls[i].throttle = rng.choice([1.0, 1.0, 0.5])
-                    hold[i] = rng.randint(10, 60)
-            if t % stride == 0:
-                out.append(parity._one_tick(A, MODE_HEATSEEKER, teams, presets, ctrls))
-            else:
-                for i in range(n):
-                    A.set_controls(i, ctrls[i])
-                A.step(1)
-        A.close()
-    return out
-
-
 def test_heatseeker_scenarios_exercise_the_mode():
     """The oracle's trajectories behind the GPU test reach every Heatseeker
     branch: seeking starts, speeds up and turns round at a back wall."""
     ref = _oracle()
     if ref is None:
         pytest.skip("oracle library not built on this machine")
-    samples = sample_heatseeker(ref, n_traj=9, ticks=90, seed=3, stride=1)
+    samples = parity.sample_heatseeker(ref, n_traj=9, ticks=90, seed=3, stride=1)
     started = sum(1 for s in samples if s.ball_in.hs_y_target_dir == 0 and s.ball_out.hs_y_target_dir != 0)
     turned = sum(1 for s in samples if s.ball_in.hs_y_target_dir * s.ball_out.hs_y_target_dir < 0)
     seeking = sum(1 for s in samples if s.ball_in.hs_y_target_dir != 0)
@@ -118,13 +70,7 @@ def _one_rotator_step_apart(s, res):
     return abs(na - nb) <= 1e-4 * na and math.acos(cosang) <= 2.05 * UE3_STEP
 
 
-@pytest.mark.gpu
-def test_heatseeker_live_oracle_parity():
-    ref = _oracle()
-    if ref is None:
-        pytest.skip("oracle library not built on this machine")
-    samples = sample_heatseeker(ref, n_traj=36, ticks=240, seed=21, stride=2)
-    results = parity.run_on_gpu(samples, MODE_HEATSEEKER)
+def _check(samples, results):
     off = [i for i, (s, r) in enumerate(zip(samples, results)) if parity.check_samples([s], [r])[0]]
     rounding = [i for i in off if _one_rotator_step_apart(samples[i], results[i])]
     rest = [i for i in off if i not in rounding]
@@ -132,6 +78,21 @@ def test_heatseeker_live_oracle_parity():
     _, msgs = parity.check_samples([samples[i] for i in rest], [results[i] for i in rest])
     assert not rest, f"{len(rest)}/{len(samples)} samples off\n" + "\n".join(msgs)
     assert len(rounding) <= 0.01 * len(samples)
+
+
+@pytest.mark.gpu
+def test_heatseeker_fixture_parity():
+    samples = parity.load_samples(os.path.join(HERE, "golden", "heatseeker_tf.npz"))
+    _check(samples, parity.run_on_gpu(samples, MODE_HEATSEEKER))
+
+
+@pytest.mark.gpu
+def test_heatseeker_live_oracle_parity():
+    ref = _oracle()
+    if ref is None:
+        pytest.skip("oracle library not built on this machine")
+    samples = parity.sample_heatseeker(ref, n_traj=36, ticks=240, seed=21, stride=2)
+    _check(samples, parity.run_on_gpu(samples, MODE_HEATSEEKER))
 
 
 @pytest.mark.gpu
