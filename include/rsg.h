@@ -236,7 +236,7 @@ int rsg_game_events_reset(rsg_game_events* t, int arena);
 int64_t rsg_snapshot_serialize(const rsg_arena_snapshot* s, uint8_t* out, uint64_t cap);
 
 /* Reads the reference's bytes (with or without the extension block).  Only
- * SOCCAR and THE_VOID with the default ArenaConfig, MutatorConfig and preset
+ * SOCCAR, HEATSEEKER and THE_VOID with the mode's default ArenaConfig, MutatorConfig and preset
  * car configs are accepted; anything else is an error.  Without the
  * extension, states are as Car/Ball::SetState leave them in a new arena
  * (tick counters and per-wheel values 0, broadphase records of newly created
