@@ -194,7 +194,8 @@ enum {
     RSG_PRED_VEL = 12,                /* 3 floats, uu/s */
     RSG_PRED_ANG_VEL = 15,            /* 3 floats, rad/s */
     RSG_PRED_TICKS_SINCE_UPDATE = 18, /* uint64 as two words, low first */
-    RSG_PRED_WORDS = 20
+    RSG_PRED_HS = 20,                 /* 3 floats: Heatseeker target net, target speed, time since hit */
+    RSG_PRED_WORDS = 23
 };
 typedef struct rsg_ball_pred_view_t {
     const float* traj;      /* [num_pred_ticks][RSG_PRED_WORDS][n]: word w of

@@ -813,6 +813,9 @@ static BallState ball_state_from(const rsg_ball_state* s) {
     BallState bs;
     vset(bs.pos, s->pos); rset(bs.rotMat, s->rot); vset(bs.vel, s->vel); vset(bs.angVel, s->ang_vel);
     bs.tickCountSinceUpdate = s->tick_count_since_update;
+    bs.hsInfo.yTargetDir = s->hs_y_target_dir;
+    bs.hsInfo.curTargetSpeed = s->hs_cur_target_speed;
+    bs.hsInfo.timeSinceHit = s->hs_time_since_hit;
     return bs;
 }
 
@@ -820,6 +823,9 @@ static void ball_state_to(const BallState& bs, rsg_ball_state* s) {
     std::memset(s, 0, sizeof(*s));
     s->tick_count_since_update = bs.tickCountSinceUpdate;
     vget(bs.pos, s->pos); rget(bs.rotMat, s->rot); vget(bs.vel, s->vel); vget(bs.angVel, s->ang_vel);
+    s->hs_y_target_dir = bs.hsInfo.yTargetDir;
+    s->hs_cur_target_speed = bs.hsInfo.curTargetSpeed;
+    s->hs_time_since_hit = bs.hsInfo.timeSinceHit;
 }
 
 // new BallPredTracker(arena, numPredTicks): predicts from the arena's ball at once.

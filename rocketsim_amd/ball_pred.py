@@ -13,7 +13,7 @@ ball of arena i (its own ball-only arena, as the reference's tracker keeps).
 import ctypes as C
 
 from . import native
-from .state import BallPredView, BallState, MODE_SOCCAR
+from .state import BallPredView, BallState, MODE_HEATSEEKER, MODE_SOCCAR
 
 
 class BallPredTracker:
@@ -25,7 +25,7 @@ class BallPredTracker:
             mode, n, tick_rate = arena.mode, arena.n_arenas, arena.tick_rate
         if mode is None or n is None:
             raise ValueError("BallPredTracker needs an ArenaBatch or mode and n")
-        if mode == MODE_SOCCAR:
+        if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
             from . import batch
             if not batch._mesh_loaded:
                 batch.init_mesh()

@@ -34,6 +34,7 @@ RSG_FN void pred_put(uint32_t* r, size_t s, const Ball& ball) {
     r[15 * s] = fw(w.x); r[16 * s] = fw(w.y); r[17 * s] = fw(w.z);
     r[18 * s] = (uint32_t)ball.ticks_since_set;
     r[19 * s] = (uint32_t)(ball.ticks_since_set >> 32);
+    r[20 * s] = fw(ball.hs_dir); r[21 * s] = fw(ball.hs_speed); r[22 * s] = fw(ball.hs_time);
 }
 
 RSG_FN V3 pred_v3(const uint32_t* r, size_t s, int w) {
@@ -48,6 +49,7 @@ RSG_FN void pred_set(Ball& ball, const uint32_t* r, size_t s) {
     ball.b.vel = pred_v3(r, s, RSG_PRED_VEL) * UU_TO_BT;
     ball.b.ang = pred_v3(r, s, RSG_PRED_ANG_VEL);
     ball.ticks_since_set = 0;
+    ball.hs_dir = wf(r[20 * s]); ball.hs_speed = wf(r[21 * s]); ball.hs_time = wf(r[22 * s]);
     ball_init_body(ball);
 }
 
@@ -91,6 +93,7 @@ RSG_FN void ball_pred_update(const BatchDev& pa, const MeshView* mp, const PredD
         cb.b.vel = br.v(BW_VEL);
         cb.b.ang = br.v(BW_ANGVEL);
         cb.ticks_since_set = br.u64(BW_TICKS_SINCE_SET);
+        cb.hs_dir = br.f(BW_HS); cb.hs_speed = br.f(BW_HS + 1); cb.hs_time = br.f(BW_HS + 2);
         pred_put(cur, 1, cb);
         // int ticksSinceLastUpdate = arena->tickCount - lastUpdateTickCount
         uint64_t tick = arena_ref(src, i).u64(AW_TICK);

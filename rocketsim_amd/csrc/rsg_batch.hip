@@ -292,7 +292,7 @@ rsg_ball_pred_kernel(BatchDev pa, MeshView mesh, PredDev p, int kind, BatchDev s
     const int i = blockIdx.x * per_wave + threadIdx.x;
     if (i >= pa.n) return;
     ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(rsg_lds + (size_t)threadIdx.x * (size_t)stride);
-    ball_pred_update(pa, pa.mode == RSG_MODE_SOCCAR ? &mesh : nullptr, p, kind, src, i, A);
+    ball_pred_update(pa, mode_has_arena(pa.mode) ? &mesh : nullptr, p, kind, src, i, A);
 }
 
 // Differentiable aerial shot (rsg_shot.h): one lane per car-ball pair.
@@ -1394,7 +1394,7 @@ int pred_launch(rsg_ball_pred* p, int kind, const BatchDev* src, hipStream_t s) 
     rsg_batch* a = p->arena;
     if (upload(a) != 0) return -1;
     MeshView mv{};
-    if (a->dev.mode == RSG_MODE_SOCCAR) {
+    if (mode_has_arena(a->dev.mode)) {
         if (!g_mesh.ready) return fail("rsg_ball_pred: no arena mesh (call rsg_init_mesh first)");
         mv = g_mesh.view;
     }
@@ -1432,6 +1432,9 @@ int pred_stage(rsg_ball_pred* p, const rsg_ball_state* in, const int32_t* ticks)
         for (int k = 0; k < 18; k++) std::memcpy(&w[k * n + i], &f[k], 4);
         w[18 * n + i] = (uint32_t)b.tick_count_since_update;
         w[19 * n + i] = (uint32_t)(b.tick_count_since_update >> 32);
+        std::memcpy(&w[20 * n + i], &b.hs_y_target_dir, 4);
+        std::memcpy(&w[21 * n + i], &b.hs_cur_target_speed, 4);
+        std::memcpy(&w[22 * n + i], &b.hs_time_since_hit, 4);
     }
     if (pred_wait(p) != 0) return -1;  // the previous update may still read the staging buffers
     HIP_OK(hipMemcpy(p->cur, w.data(), w.size() * 4, hipMemcpyHostToDevice));
@@ -1448,6 +1451,9 @@ void pred_row_to_api(const uint32_t* r, size_t s, rsg_ball_state& o) {
     std::memcpy(o.vel, f + 12, 12);
     std::memcpy(o.ang_vel, f + 15, 12);
     o.tick_count_since_update = (uint64_t)r[18 * s] | ((uint64_t)r[19 * s] << 32);
+    std::memcpy(&o.hs_y_target_dir, &r[20 * s], 4);
+    std::memcpy(&o.hs_cur_target_speed, &r[21 * s], 4);
+    std::memcpy(&o.hs_time_since_hit, &r[22 * s], 4);
 }
 
 // One tracker's ring -> predData order.

@@ -99,7 +99,7 @@ class BallState(C.Structure):
 
 
 # Ball prediction record words (include/rsg_state.h RSG_PRED_*)
-PRED_POS, PRED_ROT, PRED_VEL, PRED_ANG_VEL, PRED_TICKS_SINCE_UPDATE, PRED_WORDS = 0, 3, 12, 15, 18, 20
+PRED_POS, PRED_ROT, PRED_VEL, PRED_ANG_VEL, PRED_TICKS_SINCE_UPDATE, PRED_HS, PRED_WORDS = 0, 3, 12, 15, 18, 20, 23
 
 
 class BallPredView(C.Structure):

@@ -56,6 +56,9 @@ void stage(HostPred& h, const rsg_ball_state* in) {
         for (int k = 0; k < 18; k++) std::memcpy(&h.cur[k * n + i], &f[k], 4);
         h.cur[18 * n + i] = (uint32_t)b.tick_count_since_update;
         h.cur[19 * n + i] = (uint32_t)(b.tick_count_since_update >> 32);
+        std::memcpy(&h.cur[20 * n + i], &b.hs_y_target_dir, 4);
+        std::memcpy(&h.cur[21 * n + i], &b.hs_cur_target_speed, 4);
+        std::memcpy(&h.cur[22 * n + i], &b.hs_time_since_hit, 4);
     }
 }
 
@@ -147,6 +150,9 @@ int bp_get(void* p, int tracker, rsg_ball_state* out) {
         std::memcpy(o.vel, f + 12, 12);
         std::memcpy(o.ang_vel, f + 15, 12);
         o.tick_count_since_update = (uint64_t)r[18 * n] | ((uint64_t)r[19 * n] << 32);
+        std::memcpy(&o.hs_y_target_dir, &r[20 * n], 4);
+        std::memcpy(&o.hs_cur_target_speed, &r[21 * n], 4);
+        std::memcpy(&o.hs_time_since_hit, &r[22 * n], 4);
     }
     return h.size[tracker];
 }

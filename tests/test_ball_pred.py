@@ -31,7 +31,7 @@ import pytest
 
 import ball_pred_cases
 import parity
-from rocketsim_amd.state import BallState, MODE_SOCCAR, MODE_THE_VOID
+from rocketsim_amd.state import BallState, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -260,4 +260,53 @@ def test_gpu_state_for_time_and_empty_error():
         assert v.n == 3 and v.num_pred_ticks == T and v.traj and v.head and v.size
         assert not math.isnan(tr.pred_data(2)[T - 1].pos[2])
     finally:
+        tr.close()
+
+
+@pytest.mark.gpu
+def test_gpu_heatseeker_predictions_one_tick_parity_with_oracle():
+    """A Heatseeker tracker predicts the seeking ball: the reference's
+    tracker makes its arena in the source arena's mode (BallPredTracker.cpp:7)
+    and carries the seeking state in each BallState.  Each GPU prediction,
+    set on the oracle and stepped one tick, gives the next one, seeking words
+    included.  A seeking turn can land one rotator step away from the
+    reference's (the last-place difference of atan2f, as in the Heatseeker
+    parity test), so up to 1 % of ticks may differ."""
+    ref = _oracle()
+    if ref is None:
+        pytest.skip("oracle library not built on this machine")
+    import random
+    from rocketsim_amd.ball_pred import BallPredTracker
+    n, T = 24, 240
+    rng = random.Random(5)
+    A = ref.RefArena(MODE_HEATSEEKER)
+    states = []
+    for _ in range(n):
+        s = A.get_ball()
+        d = rng.choice([-1.0, 1.0])
+        s.pos[:] = [rng.uniform(-3000, 3000), d * rng.uniform(0, 4000), rng.uniform(200, 1500)]
+        s.vel[:] = [rng.gauss(0, 800), d * rng.uniform(1000, 3000), rng.gauss(0, 400)]
+        s.ang_vel[:] = [rng.gauss(0, 2) for _ in range(3)]
+        s.hs_y_target_dir, s.hs_cur_target_speed = d, rng.choice([2900.0, 3500.0, 4600.0])
+        s.hs_time_since_hit = rng.uniform(0, 2)
+        states.append(s)
+    tr = BallPredTracker(mode=MODE_HEATSEEKER, n=n, num_pred_ticks=T)
+    try:
+        tr.force_update_all_pred(states)
+        bad = total = turned = 0
+        for i in range(n):
+            pd = tr.pred_data(i)
+            for t in range(T - 1):
+                s = BallState.from_buffer_copy(bytes(pd[t]))
+                A.set_ball(s)
+                A.step(1)
+                r = A.get_ball()
+                r.bp_cell, r.bp_seq = 0, 0
+                total += 1
+                bad += 1 if parity.compare_ball(r, pd[t + 1]) else 0
+                turned += 1 if pd[t].hs_y_target_dir * pd[t + 1].hs_y_target_dir < 0 else 0
+        print(f"Heatseeker one-tick parity along GPU predictions: {bad}/{total} outside tolerance, {turned} turns")
+        assert bad <= 0.01 * total and turned >= 1
+    finally:
+        A.close()
         tr.close()
