@@ -131,6 +131,17 @@ int rsg_batch_set_tick_counts(rsg_batch* b, const uint64_t* in);
  * (Arena::SetGoalScoreCallback, Arena.h:24,102): goals scored by blue (ball
  * in the orange net), by orange, and the tick of the last goal. */
 int rsg_batch_get_goals(rsg_batch* b, uint32_t* blue, uint32_t* orange, uint64_t* last_goal_tick);
+/* Status of every arena (RSG_STATUS_* bits, rsg_state.h), in place of the
+ * reference's exceptions (RS_ERR_CLOSE, src/Framework.h:67-72): writes one
+ * word per arena to `per_arena` and their OR to `any` (either may be NULL)
+ * and returns the number of arenas with a bit set.  Waits for the batch's
+ * last step.  rsg_batch_clear_status zeroes the words.  With
+ * rsg_batch_set_strict(b, 1), rsg_batch_step and rsg_batch_graph_launch wait
+ * for their work and return -1, naming the first such arena in
+ * rsg_last_error(), as the reference's Arena::Step would throw. */
+int rsg_batch_get_status(rsg_batch* b, uint32_t* per_arena, uint32_t* any);
+int rsg_batch_clear_status(rsg_batch* b);
+int rsg_batch_set_strict(rsg_batch* b, int strict);
 
 /* Arena::ResetToRandomKickoff (Arena.cpp:113-223) for one arena, or all
  * arenas when arena < 0; the kickoff order is drawn from `seed`. */

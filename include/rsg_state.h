@@ -130,6 +130,23 @@ typedef struct rsg_pad_state {
 
 enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8, RSG_SNAPSHOT_RAW_WORDS = 1024 };
 
+/* Per-arena status bits (sticky until rsg_batch_clear_status).  The
+ * reference grows its contact and solver arrays without bound and throws on
+ * its own capacity errors (RS_ERR_CLOSE, src/Framework.h:67-72;
+ * btRSBroadphase.cpp:206,229); the kernels hold fixed capacities per arena
+ * and set a bit instead of dropping work silently:
+ *   RSG_STATUS_NONFINITE         a car or the ball left a tick with a
+ *                                non-finite position or velocity;
+ *   RSG_STATUS_CONTACT_OVERFLOW  a tick needed more contact manifolds than
+ *                                RSG_MAX_MANIFOLDS (the extra pairs were not
+ *                                collided);
+ *   RSG_STATUS_SOLVER_OVERFLOW   a tick needed more than RSG_MAX_SOLVER_ROWS
+ *                                contact or friction rows (the extra contacts
+ *                                were not solved).
+ * A tick that sets any of them has left the reference's results. */
+enum { RSG_STATUS_NONFINITE = 1u, RSG_STATUS_CONTACT_OVERFLOW = 2u, RSG_STATUS_SOLVER_OVERFLOW = 4u };
+enum { RSG_MAX_MANIFOLDS = 48, RSG_MAX_SOLVER_ROWS = 96 };
+
 /* A game event, replacing the reference's callbacks: the goal-score callback
  * (Arena::SetGoalScoreCallback, src/Sim/Arena/Arena.h:24,102; fired at the
  * end of the tick, Arena.cpp:748-752) and the car-bump callback

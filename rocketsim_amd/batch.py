@@ -208,6 +208,21 @@ class ArenaBatch:
         native.check(self.L.rsg_batch_get_goals(self.h, b, o, t))
         return list(b), list(o), list(t)
 
+    def get_status(self):
+        """Per-arena RSG_STATUS_* bits (non-finite state, contact or solver
+        capacity exceeded) since the last clear_status()."""
+        out = (C.c_uint32 * self.n_arenas)()
+        native.check(self.L.rsg_batch_get_status(self.h, out, None))
+        return list(out)
+
+    def clear_status(self):
+        native.check(self.L.rsg_batch_clear_status(self.h))
+
+    def set_strict(self, on=True):
+        """step() and graph_launch() raise when an arena carries a status bit,
+        as the reference's Arena::Step throws on its own errors."""
+        native.check(self.L.rsg_batch_set_strict(self.h, int(bool(on))))
+
     def is_ball_scored(self):
         out = (C.c_uint8 * self.n_arenas)()
         native.check(self.L.rsg_batch_is_ball_scored(self.h, out))

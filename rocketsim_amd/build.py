@@ -23,19 +23,49 @@ HIP_FLAGS = [
 ]
 
 
-def build_native(verbose=True):
+# Translation units of librsg.so: the C ABI (host code) and one file per
+# kernel family, compiled in parallel and linked into one shared library.
+NATIVE_SOURCES = ["rsg_batch.hip", "rsg_k_plain.hip", "rsg_k_lds.hip", "rsg_k_coop.hip", "rsg_k_pred.hip",
+                  "rsg_k_diff.hip", "rsg_k_events.hip"]
+
+
+def build_native(verbose=True, extra_flags=(), out_path=None, obj_dir=None):
+    """Compile every translation unit of NATIVE_SOURCES for gfx950 (in
+    parallel) and link librsg.so.  `extra_flags` (e.g. -D switches) and
+    `out_path` / `obj_dir` let a caller build a variant beside the product."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIBDIR, exist_ok=True)
-    src = os.path.join(CSRC, "rsg_batch.hip")
-    tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-I" + os.path.join(ROOT, "include"), src, "-o", tmp]
+    out_path = out_path or LIB_PATH
+    obj_dir = obj_dir or os.path.join(LIBDIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    inc = "-I" + os.path.join(ROOT, "include")
+
+    def compile_one(name):
+        src = os.path.join(CSRC, name)
+        obj = os.path.join(obj_dir, name.replace(".hip", ".o"))
+        cmd = [HIPCC, *HIP_FLAGS, *extra_flags, inc, "-c", src, "-o", obj]
+        if verbose:
+            print("[build]", " ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        return name, obj, r
+
+    jobs = min(len(NATIVE_SOURCES), max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(compile_one, NATIVE_SOURCES))
+    for name, _, r in results:
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed on {name}")
+    tmp = out_path + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", *[o for _, o, _ in results], "-o", tmp]
     if verbose:
         print("[build]", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building librsg.so")
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+        raise RuntimeError("hipcc failed linking librsg.so")
+    os.replace(tmp, out_path)
+    return out_path
 
 
 TORCH_EXT_PATH = os.path.join(LIBDIR, "rsg_torch.so")

@@ -71,7 +71,7 @@ RSG_FN bool pred_matches(const uint32_t* a, size_t sa, const uint32_t* b, size_t
 // UpdatePredFromArena (:18-23) on arena i of `src`.  `pa` holds the trackers'
 // ball-only prediction arenas; `A` is the working copy of arena i.
 RSG_FN void ball_pred_update(const BatchDev& pa, const MeshView* mp, const PredDev& p, int kind, const BatchDev& src,
-                             int i, ArenaLocal& A) {
+                             int i, ArenaLocal& A, StepSpace* ws) {
     load_arena(pa, i, A);
     TickParams P;
     P.mode = pa.mode;
@@ -116,7 +116,7 @@ RSG_FN void ball_pred_update(const BatchDev& pa, const MeshView* mp, const PredD
                 size -= k;
                 pred_set(A.ball, traj + (size_t)((head + size - 1) % T) * row_words, s);
                 for (; size < T; size++) {
-                    arena_tick(A, P, mp);
+                    arena_tick(A, P, mp, ws);
                     pred_put(traj + (size_t)((head + size) % T) * row_words, s, A.ball);
                 }
             }
@@ -127,7 +127,7 @@ RSG_FN void ball_pred_update(const BatchDev& pa, const MeshView* mp, const PredD
         uint32_t* r0 = traj + (size_t)head * row_words;
         for (int w = 0; w < RSG_PRED_WORDS; w++) r0[w * s] = cur[w];
         for (int t = 1; t < T; t++) {
-            arena_tick(A, P, mp);
+            arena_tick(A, P, mp, ws);
             pred_put(traj + (size_t)((head + t) % T) * row_words, s, A.ball);
         }
         size = T;

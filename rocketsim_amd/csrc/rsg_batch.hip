@@ -1,5 +1,5 @@
-// rsg_batch.hip -- kernels of the batched tick step and the C ABI of
-// include/rsg.h.
+// rsg_batch.hip -- the C ABI of include/rsg.h (host code).  The kernels live
+// in rsg_k_*.hip and are enqueued through rsg_kernels.h.
 //
 // Kernel map:
 //   rsg_step_arena_kernel       one lane per arena, K ticks per launch, arena
@@ -37,6 +37,7 @@
 #include "rsg_serialize.h"
 #include "rsg_game_events.h"
 #include "rsg_shot.h"
+#include "rsg_kernels.h"
 
 using namespace rsg;
 
@@ -86,315 +87,6 @@ int upload_array(T** dst, const std::vector<T>& src) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// kernels
-
-// Waves per SIMD the step kernel is compiled for (register budget).
-#ifndef RSG_STEP_WAVES_PER_EU
-#define RSG_STEP_WAVES_PER_EU 1
-#endif
-
-RSG_FN void step_arena(const BatchDev& d, const MeshView& mesh, int a, int ticks, ArenaLocal& A) {
-    RSG_PROF_MARK(15);
-    load_arena(d, a, A);
-    RSG_PROF_MARK(15);
-    TickParams P;
-    P.mode = d.mode;
-    P.dt = d.tick_time;
-    P.seed = d.seed;
-    P.arena_id = (uint32_t)a;
-    P.synth_controls = d.synth_controls != 0;
-    P.auto_reset = d.auto_reset != 0;
-    const MeshView* mp = mode_has_arena(d.mode) ? &mesh : nullptr;
-    for (int t = 0; t < ticks; t++) arena_tick(A, P, mp);
-    store_arena(d, a, A);
-}
-
-// One lane per arena; the arena's working copy lives in the lane's private
-// (scratch) memory.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
-rsg_step_arena_kernel(BatchDev d, MeshView mesh, int ticks) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= d.n) return;
-    ArenaLocal A;
-    step_arena(d, mesh, a, ticks, A);
-}
-
-// One lane per arena; the arena's working copy lives in LDS, `stride` bytes
-// per lane (the ArenaLocal prefix up to the batch's largest car count).
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
-rsg_step_arena_lds_kernel(BatchDev d, MeshView mesh, int ticks, int stride) {
-    extern __shared__ __align__(16) unsigned char rsg_lds[];
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= d.n) return;
-    ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(rsg_lds + (size_t)threadIdx.x * (size_t)stride);
-    step_arena(d, mesh, a, ticks, A);
-}
-
-// Cooperative variant: `lanes` lanes per arena (4 per car slot), arena in
-// LDS.  Lane 0 of each arena runs the serial parts of the tick; the lanes
-// share:
-//   * the wheel rays, one per lane, cast at the start of the pre-tick into
-//     LDS (unless a car respawns that tick: then lane 0 casts them in the
-//     loop as usual);
-//   * the car pre-ticks, one car per lane, when they are independent of
-//     their order (no wheel on a car);
-//   * the car post-ticks, one car per lane, and the boost-pad tests, pads
-//     spread over the lanes.
-// Per-arena LDS block: ArenaLocal prefix, ray results, two flag words.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
-rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int arena_bytes, int lanes) {
-    extern __shared__ __align__(16) unsigned char rsg_lds[];
-    const int slot = (int)threadIdx.x / lanes;
-    const int k = (int)threadIdx.x - slot * lanes;
-    const int a = (int)blockIdx.x * ((int)blockDim.x / lanes) + slot;
-    const bool live = a < d.n;
-    unsigned char* base = rsg_lds + (size_t)slot * (size_t)stride;
-    ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(base);
-    RayHit* hits = reinterpret_cast<RayHit*>(base + arena_bytes);
-    int* ahead = reinterpret_cast<int*>(base + arena_bytes + (size_t)(lanes) * sizeof(RayHit));
-    TickParams P;
-    P.mode = d.mode;
-    P.dt = d.tick_time;
-    P.seed = d.seed;
-    P.arena_id = (uint32_t)a;
-    P.synth_controls = d.synth_controls != 0;
-    P.auto_reset = d.auto_reset != 0;
-    const MeshView* mp = mode_has_arena(d.mode) ? &mesh : nullptr;
-    int* concurrent = ahead + 1;
-    if (live && k == 0) load_arena(d, a, A);
-    __syncthreads();
-    for (int t = 0; t < ticks; t++) {
-        if (live && k == 0) *ahead = tick_begin(A, P) ? 1 : 0;
-        __syncthreads();
-        if (live && *ahead && k < 4 * A.n_cars) tick_cast_ray(A, P, mp, k >> 2, k & 3, hits[k]);
-        __syncthreads();
-        if (live && k == 0) {
-            const RayHit* pre = *ahead ? hits : nullptr;
-            *concurrent = car_pre_ticks_independent(A, pre) ? 1 : 0;
-            if (!*concurrent) {
-                World world(A, mp, mode_has_arena(d.mode));
-                for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], pre);
-            }
-        }
-        __syncthreads();
-        if (live && *concurrent && k < A.n_cars) {
-            World world(A, mp, mode_has_arena(d.mode));
-            car_pre_tick(A, P, world, k, hits);
-        }
-        __syncthreads();
-        if (live && k == 0) {
-            pads_pre_tick(A, P);
-            tick_world(A, P, mp);
-        }
-        __syncthreads();
-        if (live && k < A.n_cars) car_post_tick(A, P, k);
-        __syncthreads();
-        if (live && mode_has_arena(d.mode))
-            for (int p = k; p < PAD_NUM; p += lanes) pad_check_cars(A, p);
-        __syncthreads();
-        if (live && k == 0) tick_end(A, P);
-        __syncthreads();
-    }
-    if (live && k == 0) store_arena(d, a, A);
-}
-
-// ---------------------------------------------------------------------------
-// Airborne-car rollouts and their adjoint (rsg_adjoint.h).  One lane per car;
-// arrays are structure-of-arrays by word: state [13][n] (position uu,
-// quaternion x y z w, velocity uu/s, angular velocity rad/s), controls
-// [ticks][4][n] (pitch, yaw, roll, boost), target [7][n] (quaternion,
-// position uu), checkpoints [ceil(ticks / K)][13][n] in Bullet units.
-
-constexpr int AIR_MAX_K = 32;
-
-RSG_FN AirState air_load(const float* a, int n, int i, bool uu) {
-    float k = uu ? UU_TO_BT : 1.f;
-    AirState s;
-    s.p = v3(a[0 * n + i], a[1 * n + i], a[2 * n + i]) * k;
-    s.q = q4(a[3 * n + i], a[4 * n + i], a[5 * n + i], a[6 * n + i]);
-    s.v = v3(a[7 * n + i], a[8 * n + i], a[9 * n + i]) * k;
-    s.w = v3(a[10 * n + i], a[11 * n + i], a[12 * n + i]);
-    return s;
-}
-
-RSG_FN void air_store(float* a, int n, int i, const AirState& s, float kpv) {
-    a[0 * n + i] = s.p.x * kpv; a[1 * n + i] = s.p.y * kpv; a[2 * n + i] = s.p.z * kpv;
-    a[3 * n + i] = s.q.x; a[4 * n + i] = s.q.y; a[5 * n + i] = s.q.z; a[6 * n + i] = s.q.w;
-    a[7 * n + i] = s.v.x * kpv; a[8 * n + i] = s.v.y * kpv; a[9 * n + i] = s.v.z * kpv;
-    a[10 * n + i] = s.w.x; a[11 * n + i] = s.w.y; a[12 * n + i] = s.w.z;
-}
-
-RSG_FN AirCtrl air_ctrl(const float* c, int n, int i, int t) {
-    const float* b = c + (size_t)t * 4 * n;
-    AirCtrl k;
-    k.pitch = b[i]; k.yaw = b[n + i]; k.roll = b[2 * n + i]; k.boost = b[3 * n + i];
-    return k;
-}
-
-__global__ void __launch_bounds__(256) rsg_air_forward_kernel(int n, int ticks, int K, float dt, const float* s0,
-                                                              const float* ctrl, const float* target, float* s_out,
-                                                              float* loss, float* ckpt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    AirState s = air_load(s0, n, i, true);
-    for (int t = 0; t < ticks; t++) {
-        if (ckpt && t % K == 0) air_store(ckpt + (size_t)(t / K) * AIR_STATE_WORDS * n, n, i, s, 1.f);
-        air_step(s, air_ctrl(ctrl, n, i, t), dt);
-    }
-    if (s_out) air_store(s_out, n, i, s, BT_TO_UU);
-    if (loss) {
-        Q4 tq = q4(target[i], target[n + i], target[2 * n + i], target[3 * n + i]);
-        V3 tp = v3(target[4 * n + i], target[5 * n + i], target[6 * n + i]) * UU_TO_BT;
-        loss[i] = air_loss(s, tq, tp);
-    }
-}
-
-__global__ void __launch_bounds__(256) rsg_air_backward_kernel(int n, int ticks, int K, float dt, const float* ctrl,
-                                                               const float* target, const float* ckpt,
-                                                               const float* dloss, float* dctrl, float* ds0) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int nseg = (ticks + K - 1) / K;
-    AirState st[AIR_MAX_K];
-    // final state: recompute the last segment from its checkpoint
-    AirState s = air_load(ckpt + (size_t)(nseg - 1) * AIR_STATE_WORDS * n, n, i, false);
-    for (int t = (nseg - 1) * K; t < ticks; t++) air_step(s, air_ctrl(ctrl, n, i, t), dt);
-    Q4 tq = q4(target[i], target[n + i], target[2 * n + i], target[3 * n + i]);
-    V3 tp = v3(target[4 * n + i], target[5 * n + i], target[6 * n + i]) * UU_TO_BT;
-    AirState g;
-    air_loss_grad(s, tq, tp, dloss[i], g);
-    for (int seg = nseg - 1; seg >= 0; seg--) {
-        int t0 = seg * K, t1 = t0 + K < ticks ? t0 + K : ticks;
-        AirState x = air_load(ckpt + (size_t)seg * AIR_STATE_WORDS * n, n, i, false);
-        for (int t = t0; t < t1; t++) {
-            st[t - t0] = x;
-            air_step(x, air_ctrl(ctrl, n, i, t), dt);
-        }
-        for (int t = t1 - 1; t >= t0; t--) {
-            AirCtrl gc = {0.f, 0.f, 0.f, 0.f};
-            air_step_adjoint(st[t - t0], air_ctrl(ctrl, n, i, t), dt, g, gc);
-            float* b = dctrl + (size_t)t * 4 * n;
-            b[i] = gc.pitch; b[n + i] = gc.yaw; b[2 * n + i] = gc.roll; b[3 * n + i] = gc.boost;
-        }
-    }
-    // initial-state gradient in interface units (uu for position and velocity)
-    if (ds0) air_store(ds0, n, i, g, UU_TO_BT);
-}
-
-// Ball prediction (rsg_ball_pred.h): one update of every tracker,
-// `per_wave` trackers per 64-lane block (the rest of the wave idles, so small
-// batches still reach every SIMD).  Tracker i's prediction arena (a
-// ball-only arena of `pa`) is held in LDS, `stride` bytes per tracker.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
-rsg_ball_pred_kernel(BatchDev pa, MeshView mesh, PredDev p, int kind, BatchDev src, int stride, int per_wave) {
-    extern __shared__ __align__(16) unsigned char rsg_lds[];
-    if ((int)threadIdx.x >= per_wave) return;
-    const int i = blockIdx.x * per_wave + threadIdx.x;
-    if (i >= pa.n) return;
-    ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(rsg_lds + (size_t)threadIdx.x * (size_t)stride);
-    ball_pred_update(pa, mode_has_arena(pa.mode) ? &mesh : nullptr, p, kind, src, i, A);
-}
-
-// Differentiable aerial shot (rsg_shot.h): one lane per car-ball pair.
-__global__ void __launch_bounds__(256) rsg_shot_forward_kernel(int n, int ticks, int K, float dt, const float* state0,
-                                                               const float* ctrl, const float* target,
-                                                               float* state_out, float* loss, float* ckpt,
-                                                               uint32_t* flags) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    ShotConsts k = shot_consts(dt);
-    ShotState<float> s;
-    shot_load(state0, n, i, s);
-    uint32_t fl = 0;
-    V3 tg = v3(target[i], target[n + i], target[2 * n + i]);
-    float l = shot_forward(s, ctrl, n, i, ticks, K, ckpt, tg, k, fl);
-    if (state_out) shot_store(state_out, n, i, s);
-    if (loss) loss[i] = l;
-    if (flags) flags[i] = fl;
-}
-
-// Backward of the shot: one group of SHOT_GROUP lanes per pair, lane j < 26
-// carrying the tangent along input j of every tick (22 state words, then
-// pitch, yaw, roll, boost), so a pair's 26 passes per tick run side by side
-// instead of one after another.  Every lane of a group recomputes the same
-// segment from its checkpoint; lane 0 keeps the segment's states in LDS
-// ([K][SHOT_LDS_WORDS] per group), and after each tick the group gathers the
-// new state adjoint from lanes 0-21 with cross-lane shuffles.  A group spans
-// half a wave, so no workgroup barrier is needed: a group whose pair is past
-// n leaves at once.
-constexpr int SHOT_GROUP = 32;
-constexpr int SHOT_LDS_WORDS = 24;   // a checkpoint's 23 words, padded
-
-__global__ void __launch_bounds__(256) rsg_shot_backward_kernel(int n, int ticks, int K, float dt, const float* ctrl,
-                                                                const float* target, const float* ckpt,
-                                                                const float* dloss, float* dctrl, float* ds0) {
-    extern __shared__ float shot_seg[];
-    const int lane = threadIdx.x & (SHOT_GROUP - 1);
-    const int grp = threadIdx.x / SHOT_GROUP;
-    const int i = blockIdx.x * (blockDim.x / SHOT_GROUP) + grp;
-    if (i >= n) return;
-    float* seg = shot_seg + (size_t)grp * K * SHOT_LDS_WORDS;
-    ShotConsts k = shot_consts(dt);
-    V3 tg = v3(target[i], target[n + i], target[2 * n + i]);
-    const int nseg = (ticks + K - 1) / K;
-    ShotState<float> s;
-    int last_extra;
-    shot_ckpt_load(ckpt + (size_t)(nseg - 1) * SHOT_CKPT_WORDS * n, n, i, s, last_extra);
-    uint32_t fl = 0;
-    for (int t = (nseg - 1) * K; t < ticks; t++) shot_step(s, shot_ctrl(ctrl, n, i, t), t, last_extra, fl, k);
-    float g[SHOT_STATE_WORDS];
-    shot_loss_grad(s, tg, dloss[i], g);
-    float mine = 0.f;
-    for (int sg = nseg - 1; sg >= 0; sg--) {
-        const int t0 = sg * K, t1 = t0 + K < ticks ? t0 + K : ticks;
-        ShotState<float> x;
-        int lx;
-        shot_ckpt_load(ckpt + (size_t)sg * SHOT_CKPT_WORDS * n, n, i, x, lx);
-        for (int t = t0; t < t1; t++) {
-            if (lane == 0) {
-                float* r = seg + (t - t0) * SHOT_LDS_WORDS;
-                for (int w = 0; w < SHOT_STATE_WORDS; w++) r[w] = shot_word(x, w);
-                r[SHOT_STATE_WORDS] = __int_as_float(lx);
-            }
-            shot_step(x, shot_ctrl(ctrl, n, i, t), t, lx, fl, k);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        for (int t = t1 - 1; t >= t0; t--) {
-            const float* r = seg + (t - t0) * SHOT_LDS_WORDS;
-            ShotState<float> st;
-            for (int w = 0; w < SHOT_STATE_WORDS; w++) shot_word(st, w) = r[w];
-            const int le = __float_as_int(r[SHOT_STATE_WORDS]);
-            mine = shot_vjp_entry(st, shot_ctrl(ctrl, n, i, t), t, le, k, g, lane);
-            for (int w = 0; w < SHOT_STATE_WORDS; w++) g[w] = __shfl(mine, w, SHOT_GROUP);
-            if (lane >= SHOT_STATE_WORDS && lane < SHOT_INPUTS)
-                dctrl[(size_t)t * 4 * n + (size_t)(lane - SHOT_STATE_WORDS) * n + i] = mine;
-        }
-        // the next segment's states overwrite this one's
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-    }
-    if (ds0 && lane < SHOT_STATE_WORDS) ds0[(size_t)lane * n + i] = mine * shot_unit(lane);
-}
-
-// GameEventTracker::Update for every arena (rsg_game_events.h), one lane each.
-__global__ void __launch_bounds__(256) rsg_game_events_kernel(BatchDev d, GameEventDev g, rsg_game_event_config cfg) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= d.n) return;
-    game_events_update(d, g, cfg, a);
-}
-
-__global__ void __launch_bounds__(64) rsg_reset_kernel(BatchDev d, int only_arena, uint64_t seed) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= d.n) return;
-    if (only_arena >= 0 && a != only_arena) return;
-    ArenaLocal A;
-    load_arena(d, a, A);
-    arena_kickoff_reset(A, seed, (uint32_t)a);
-    store_arena(d, a, A);
-}
-
-// ---------------------------------------------------------------------------
 // batch object
 
 struct rsg_batch {
@@ -408,6 +100,7 @@ struct rsg_batch {
     hipStream_t last_stream = nullptr;
     int arenas_per_block = 0;  // 0: derived from the device's SIMD count
     int kernel = RSG_KERNEL_AUTO;
+    bool strict = false;       // fail a step that leaves an arena with a status bit set
     // captured multi-launch step (rsg_batch_graph_*)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -518,7 +211,8 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     if ((e = hipMalloc(&b->dev.car, car_words(b) * 4)) != hipSuccess ||
         (e = hipMalloc(&b->dev.ball, ball_words(b) * 4)) != hipSuccess ||
         (e = hipMalloc(&b->dev.pad, pad_words(b) * 4)) != hipSuccess ||
-        (e = hipMalloc(&b->dev.arena, arena_words(b) * 4)) != hipSuccess) {
+        (e = hipMalloc(&b->dev.arena, arena_words(b) * 4)) != hipSuccess ||
+        (e = hipMalloc(&b->dev.space, (size_t)b->dev.n * sizeof(StepSpace))) != hipSuccess) {
         fail(std::string("rsg_batch_create: hipMalloc: ") + hipGetErrorString(e));
         rsg_batch_destroy(b);
         return nullptr;
@@ -541,6 +235,7 @@ void rsg_batch_destroy(rsg_batch* b) {
     if (b->dev.ball) (void)hipFree(b->dev.ball);
     if (b->dev.pad) (void)hipFree(b->dev.pad);
     if (b->dev.arena) (void)hipFree(b->dev.arena);
+    if (b->dev.space) (void)hipFree(b->dev.space);
     if (b->dev.ev_buf) (void)hipFree(b->dev.ev_buf);
     if (b->dev.ev_count) (void)hipFree(b->dev.ev_count);
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
@@ -666,20 +361,44 @@ int launch_step(const rsg_batch* b, const BatchDev& dev, const Launch& L, int ti
         if (!g_mesh.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
         mv = g_mesh.view;
     }
-    if (L.kernel == RSG_KERNEL_COOP) {
-        int lanes = coop_lanes(b);
-        int stride = coop_stride(b);
-        size_t lds = (size_t)block * (size_t)stride;
-        hipLaunchKernelGGL(rsg_step_arena_coop_kernel, dim3(grid), dim3(block * lanes), lds, s, dev, mv, ticks,
-                           stride, lds_stride(b), lanes);
-    } else if (L.kernel == RSG_KERNEL_LDS) {
-        int stride = lds_stride(b);
-        size_t lds = (size_t)block * (size_t)stride;
-        hipLaunchKernelGGL(rsg_step_arena_lds_kernel, dim3(grid), dim3(block), lds, s, dev, mv, ticks, stride);
+    if (L.kernel == RSG_KERNEL_COOP)
+        HIP_OK(launch_step_coop(dev, mv, ticks, grid, block, coop_stride(b), lds_stride(b), coop_lanes(b), s));
+    else if (L.kernel == RSG_KERNEL_LDS)
+        HIP_OK(launch_step_lds(dev, mv, ticks, grid, block, lds_stride(b), s));
+    else
+        HIP_OK(launch_step_plain(dev, mv, ticks, grid, block, s));
+    return 0;
+}
+}  // namespace
+
+namespace {
+// The per-arena status words (RSG_STATUS_*), from wherever they are newest.
+int read_status(rsg_batch* b, std::vector<uint32_t>& st) {
+    const size_t n = (size_t)b->dev.n;
+    st.resize(n);
+    if (b->dev_newer && !b->host_dirty) {
+        if (b->last_stream) HIP_OK(hipStreamSynchronize(b->last_stream));
+        HIP_OK(hipMemcpy(st.data(), b->dev.arena + (size_t)AW_STATUS * n, n * 4, hipMemcpyDeviceToHost));
     } else {
-        hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(block), 0, s, dev, mv, ticks);
+        std::memcpy(st.data(), b->h_arena.data() + (size_t)AW_STATUS * n, n * 4);
     }
-    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+const char* status_text(uint32_t s) {
+    if (s & RSG_STATUS_NONFINITE) return "non-finite car or ball state";
+    if (s & RSG_STATUS_CONTACT_OVERFLOW) return "more contact manifolds than RSG_MAX_MANIFOLDS";
+    return "more contact or friction rows than RSG_MAX_SOLVER_ROWS";
+}
+
+// Strict mode: wait for the step and fail if an arena carries a status bit,
+// as the reference throws on its capacity errors.
+int check_strict(rsg_batch* b, const char* who) {
+    if (!b->strict) return 0;
+    std::vector<uint32_t> st;
+    if (read_status(b, st) != 0) return -1;
+    for (size_t a = 0; a < st.size(); a++)
+        if (st[a]) return fail(std::string(who) + ": arena " + std::to_string(a) + ": " + status_text(st[a]));
     return 0;
 }
 }  // namespace
@@ -693,6 +412,36 @@ int rsg_batch_step(rsg_batch* b, int ticks, void* stream) {
     if (launch_step(b, b->dev, choose_launch(b), ticks, s) != 0) return -1;
     b->last_stream = s;
     b->dev_newer = true;
+    return check_strict(b, "rsg_batch_step");
+}
+
+int rsg_batch_get_status(rsg_batch* b, uint32_t* per_arena, uint32_t* any) {
+    if (!b) return fail("null batch");
+    std::vector<uint32_t> st;
+    if (read_status(b, st) != 0) return -1;
+    uint32_t all = 0;
+    int flagged = 0;
+    for (uint32_t s : st) {
+        all |= s;
+        flagged += s ? 1 : 0;
+    }
+    if (per_arena) std::memcpy(per_arena, st.data(), st.size() * 4);
+    if (any) *any = all;
+    return flagged;
+}
+
+int rsg_batch_clear_status(rsg_batch* b) {
+    if (!b) return fail("null batch");
+    if (download(b) != 0) return -1;
+    std::fill(b->h_arena.begin() + (size_t)AW_STATUS * b->dev.n, b->h_arena.begin() + (size_t)(AW_STATUS + 1) * b->dev.n,
+              0u);
+    b->host_dirty = true;
+    return 0;
+}
+
+int rsg_batch_set_strict(rsg_batch* b, int strict) {
+    if (!b) return fail("null batch");
+    b->strict = strict != 0;
     return 0;
 }
 
@@ -731,7 +480,7 @@ int rsg_batch_graph_launch(rsg_batch* b, void* stream) {
     HIP_OK(hipGraphLaunch(b->graph_exec, s));
     b->last_stream = s;
     b->dev_newer = true;
-    return 0;
+    return check_strict(b, "rsg_batch_graph_launch");
 }
 
 int rsg_batch_graph_ticks(rsg_batch* b) { return b ? b->graph_ticks : fail("null batch"); }
@@ -743,10 +492,8 @@ int rsg_air_rollout(int n, int ticks, int ckpt_every, float tick_rate, const flo
     if (tick_rate <= 0.f) return fail("rsg_air_rollout: tick_rate must be > 0");
     if (!state0 || !controls || (loss && !target)) return fail("rsg_air_rollout: null argument");
     if (n == 0) return 0;
-    int block = 256, grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(rsg_air_forward_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream, n, ticks,
-                       ckpt_every, 1.f / tick_rate, state0, controls, target, state_out, loss, checkpoints);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_air_forward(n, ticks, ckpt_every, 1.f / tick_rate, state0, controls, target, state_out, loss,
+                              checkpoints, (hipStream_t)stream));
     return 0;
 }
 
@@ -759,10 +506,8 @@ int rsg_air_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate, 
     if (!controls || !target || !checkpoints || !dloss || !dcontrols)
         return fail("rsg_air_rollout_backward: null argument");
     if (n == 0) return 0;
-    int block = 256, grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(rsg_air_backward_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream, n, ticks,
-                       ckpt_every, 1.f / tick_rate, controls, target, checkpoints, dloss, dcontrols, dstate0);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_air_backward(n, ticks, ckpt_every, 1.f / tick_rate, controls, target, checkpoints, dloss,
+                               dcontrols, dstate0, (hipStream_t)stream));
     return 0;
 }
 
@@ -773,11 +518,8 @@ int rsg_shot_rollout(int n, int ticks, int ckpt_every, float tick_rate, const fl
     if (ckpt_every < 1 || ckpt_every > SHOT_MAX_K) return fail("rsg_shot_rollout: ckpt_every must be within [1, 32]");
     if (!(tick_rate >= 15.f && tick_rate <= 120.f)) return fail("rsg_shot_rollout: tick_rate must be within [15, 120]");
     if (!state0 || !controls || !target || !checkpoints) return fail("rsg_shot_rollout: null argument");
-    // 64-lane workgroups spread the pairs over every CU
-    int block = 64, grid = (n + block - 1) / block;
-    hipLaunchKernelGGL(rsg_shot_forward_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream, n, ticks, ckpt_every,
-                       1.f / tick_rate, state0, controls, target, state_out, loss, checkpoints, flags);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_shot_forward(n, ticks, ckpt_every, 1.f / tick_rate, state0, controls, target, state_out, loss,
+                               checkpoints, flags, (hipStream_t)stream));
     return 0;
 }
 
@@ -791,11 +533,8 @@ int rsg_shot_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate,
         return fail("rsg_shot_rollout_backward: tick_rate must be within [15, 120]");
     if (!controls || !target || !checkpoints || !dloss || !dcontrols)
         return fail("rsg_shot_rollout_backward: null argument");
-    const int block = 256, per_block = block / SHOT_GROUP, grid = (n + per_block - 1) / per_block;
-    const size_t lds = (size_t)per_block * ckpt_every * SHOT_LDS_WORDS * sizeof(float);
-    hipLaunchKernelGGL(rsg_shot_backward_kernel, dim3(grid), dim3(block), lds, (hipStream_t)stream, n, ticks,
-                       ckpt_every, 1.f / tick_rate, controls, target, checkpoints, dloss, dcontrols, dstate0);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_shot_backward(n, ticks, ckpt_every, 1.f / tick_rate, controls, target, checkpoints, dloss,
+                                dcontrols, dstate0, (hipStream_t)stream));
     return 0;
 }
 
@@ -884,10 +623,7 @@ int rsg_batch_reset_kickoff(rsg_batch* b, int arena, uint64_t seed) {
     if (arena >= b->dev.n) return fail("rsg_batch_reset_kickoff: bad arena");
     if (upload(b) != 0) return -1;
     if (download(b) != 0) return -1;
-    int block = 64;
-    int grid = (b->dev.n + block - 1) / block;
-    hipLaunchKernelGGL(rsg_reset_kernel, dim3(grid), dim3(block), 0, b->last_stream, b->dev, arena, seed);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_reset(b->dev, arena, seed, b->last_stream));
     b->dev_newer = true;
     return 0;
 }
@@ -1175,9 +911,7 @@ int rsg_game_events_update(rsg_game_events* t, rsg_batch* b, void* stream) {
     g.ev_buf = t->ev_buf;
     g.ev_count = t->ev_count;
     g.ev_cap = t->ev_cap;
-    int block = 256, grid = (t->n + block - 1) / block;
-    hipLaunchKernelGGL(rsg_game_events_kernel, dim3(grid), dim3(block), 0, s, b->dev, g, t->cfg);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_game_events(b->dev, g, t->cfg, s));
     t->last_stream = s;
     return 0;
 }
@@ -1409,9 +1143,7 @@ int pred_launch(rsg_ball_pred* p, int kind, const BatchDev* src, hipStream_t s) 
     int per = p->per_wave > 0 ? p->per_wave : pred_per_wave(p->n);
     int stride = pred_stride();
     int grid = (p->n + per - 1) / per;
-    hipLaunchKernelGGL(rsg_ball_pred_kernel, dim3(grid), dim3(64), (size_t)per * (size_t)stride, s, a->dev, mv, d,
-                       kind, src ? *src : a->dev, stride, per);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_ball_pred(a->dev, mv, d, kind, src ? *src : a->dev, stride, per, grid, s));
     a->dev_newer = true;
     p->last_stream = s;
     p->pending = true;

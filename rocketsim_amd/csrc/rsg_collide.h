@@ -56,10 +56,23 @@ RSG_FN void quicksort_by_key(int* idx, const int* key, int lo, int hi) {
     }
 }
 
+// Per-arena work space of the world step: contact manifolds, solver rows
+// and the penetration query's pools (about 55 KB).  Callers place it in
+// private memory (one lane per arena) or in a per-arena block of global
+// memory, where a lane's consecutive words share cache lines (private memory
+// interleaves the lanes of a wave word by word).
+struct StepSpace {
+    Manifold mf[MAX_MANIFOLDS];
+    SRow rows[MAX_ROWS];
+    SRow fric[MAX_ROWS];
+    EpaSpace epa;
+};
+
 struct World {
     ArenaLocal& A;
     const MeshView* mesh;
     bool soccar;
+    StepSpace* ws;  // null for worlds that only cast rays
 
     V3 aabb_min[MAX_BODIES], aabb_max[MAX_BODIES];
     int order[MAX_BODIES];  // dynamic proxies in cell-list order
@@ -67,12 +80,11 @@ struct World {
     bool ray_order_ready;
     Pair pairs[MAX_PAIRS];
     int n_pairs;
-    Manifold mf[MAX_MANIFOLDS];
+    Manifold* mf;   // ws->mf
     int n_mf;
-    bool overflow;
 
-    RSG_FN World(ArenaLocal& a, const MeshView* m, bool s)
-        : A(a), mesh(m), soccar(s), ray_order_ready(false), n_pairs(0), n_mf(0), overflow(false) {}
+    RSG_FN World(ArenaLocal& a, const MeshView* m, bool s, StepSpace* w = nullptr)
+        : A(a), mesh(m), soccar(s), ws(w), ray_order_ready(false), n_pairs(0), mf(w ? w->mf : nullptr), n_mf(0) {}
 
     // Whether the subsimplex ray cast against a convex of bounding radius
     // `radius` about `center` can report a hit with fraction below 1: only
@@ -137,7 +149,7 @@ struct World {
     }
 
     RSG_FN int new_manifold(int b0, int b1) {
-        if (n_mf >= MAX_MANIFOLDS) { overflow = true; return -1; }
+        if (n_mf >= MAX_MANIFOLDS) { A.status |= RSG_STATUS_CONTACT_OVERFLOW; return -1; }
         Manifold& m = mf[n_mf];
         m.b0 = b0; m.b1 = b1;
         m.threshold = rs_min(threshold_of(b0), threshold_of(b1));
@@ -628,7 +640,7 @@ struct World {
         md *= md;
         V3 n, pb;
         float dist;
-        if (gjk_closest(box, tri, cx, xf_ident(), md, n, pb, dist)) add_contact(mi, n, pb, dist, t);
+        if (gjk_closest(box, tri, cx, xf_ident(), md, n, pb, dist, ws->epa)) add_contact(mi, n, pb, dist, t);
     }
 
     // Car hitbox vs the ball: btConvexConvexAlgorithm (GJK) on the box and the
@@ -641,7 +653,7 @@ struct World {
         md *= md;
         V3 n, pb;
         float dist;
-        if (gjk_closest(box, sph, cx, xf_from(A.ball.b.pos, A.ball.b.rot), md, n, pb, dist))
+        if (gjk_closest(box, sph, cx, xf_from(A.ball.b.pos, A.ball.b.rot), md, n, pb, dist, ws->epa))
             add_contact(mi, n, pb, dist, -1);
         refresh(mi);
     }
@@ -973,6 +985,8 @@ struct World {
         }
         // manifolds whose island is processed (all islands are batched)
         Solver S;
+        S.rows = ws->rows;
+        S.fric = ws->fric;
         solver_reset(S, dt);
         struct BodyAt {
             World* w;
@@ -1012,7 +1026,7 @@ struct World {
             b.pos = sb.xf.t;
             b.rot = sb.xf.R;
         }
-        if (S.overflow) overflow = true;
+        if (S.overflow) A.status |= RSG_STATUS_SOLVER_OVERFLOW;
     }
 
     RSG_FN static int uf_find(int* id, int x) {

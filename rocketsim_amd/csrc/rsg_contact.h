@@ -158,7 +158,13 @@ RSG_FN Xf xf_from(V3 t, const M3& R) { Xf r; r.R = R; r.t = t; return r; }
 // ---------------------------------------------------------------------------
 // Contact records
 
-constexpr int MAX_MANIFOLDS = 48;
+// Manifolds per arena and tick.  A host test build may lower the capacity
+// (RSG_TEST_MAX_MANIFOLDS) to reach the overflow path with ordinary play.
+#ifdef RSG_TEST_MAX_MANIFOLDS
+constexpr int MAX_MANIFOLDS = RSG_TEST_MAX_MANIFOLDS;
+#else
+constexpr int MAX_MANIFOLDS = RSG_MAX_MANIFOLDS;
+#endif
 constexpr int MAX_POINTS_PER_MANIFOLD = 4;
 
 struct CPoint {

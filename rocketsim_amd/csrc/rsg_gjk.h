@@ -665,7 +665,7 @@ struct Epa {
     V3 normal;
     float depth;
     SV* sv;               // shared pool (EPA vertices from index 4)
-    EpaFace fc[EPA_MAX_FACES];
+    EpaFace* fc;          // face pool, EPA_MAX_FACES entries
     int nextsv;
     int16_t hull_root, stock_root;
     int hull_count, stock_count;
@@ -687,8 +687,9 @@ struct Epa {
         if (f == root) root = fc[f].l[1];
         count--;
     }
-    RSG_FN void init(SV* pool) {
+    RSG_FN void init(SV* pool, EpaFace* faces) {
         sv = pool;
+        fc = faces;
         status = 9;
         normal = v3zero();
         depth = 0;
@@ -913,14 +914,14 @@ RSG_FN void g2_init_shape(MinkDiff& md, const CShape& s0, const Xf& t0, const CS
 
 // btGjkEpaSolver2::Penetration (with margins)
 RSG_FN bool g2_penetration(const CShape& s0, const Xf& t0, const CShape& s1, const Xf& t1, V3 guess,
-                           G2Results& res, SV* pool, Epa& epa) {
+                           G2Results& res, SV* pool, EpaFace* faces, Epa& epa) {
     Gjk2 gjk;
     g2_init_shape(gjk.shape, s0, t0, s1, t1, true);
     gjk.sv = pool;
     res.witness[0] = res.witness[1] = v3zero();
     int gs = gjk.evaluate(-guess);
     if (gs == 1) {
-        epa.init(pool);  // EPA vertices follow the four GJK entries of the pool
+        epa.init(pool, faces);  // EPA vertices follow the four GJK entries of the pool
         int es = epa.evaluate(gjk, -guess);
         if (es != 9) {
             V3 w0 = v3zero();
@@ -961,15 +962,25 @@ RSG_FN bool g2_distance(const CShape& s0, const Xf& t0, const CShape& s1, const 
     return false;
 }
 
-// btGjkEpaPenetrationDepthSolver::calcPenDepth
-RSG_FN bool calc_pen_depth(const CShape& a, const CShape& b, const Xf& ta, const Xf& tb, V3& v, V3& wa, V3& wb) {
+// Work space of a penetration query: the GJK/EPA vertex pool and EPA's face
+// pool.  The caller places it (private memory, or a per-arena block in global
+// memory that a lane reads without the per-lane interleaving of private
+// memory).
+struct EpaSpace {
     SV pool[SV_POOL];
+    EpaFace fc[EPA_MAX_FACES];
+};
+
+// btGjkEpaPenetrationDepthSolver::calcPenDepth
+RSG_FN bool calc_pen_depth(const CShape& a, const CShape& b, const Xf& ta, const Xf& tb, V3& v, V3& wa, V3& wb,
+                           EpaSpace& space) {
+    SV* pool = space.pool;
     Epa epa;
     V3 guesses[9] = {safe_normalized(tb.t - ta.t), safe_normalized(ta.t - tb.t), v3(0, 0, 1), v3(0, 1, 0),
                      v3(1, 0, 0), v3(1, 1, 0), v3(1, 1, 1), v3(0, 1, 1), v3(1, 0, 1)};
     for (int i = 0; i < 9; i++) {
         G2Results r;
-        if (g2_penetration(a, ta, b, tb, guesses[i], r, pool, epa)) {
+        if (g2_penetration(a, ta, b, tb, guesses[i], r, pool, space.fc, epa)) {
             wa = r.witness[0]; wb = r.witness[1]; v = r.normal;
             return true;
         }
@@ -988,7 +999,7 @@ RSG_FN bool calc_pen_depth(const CShape& a, const CShape& b, const Xf& ta, const
 // signed distance) when the shapes are closer than sqrt(max_dist2).
 
 RSG_FN bool gjk_closest(const CShape& A, const CShape& B, const Xf& ta, const Xf& tb, float max_dist2,
-                        V3& normal_out, V3& point_b_out, float& dist_out) {
+                        V3& normal_out, V3& point_b_out, float& dist_out, EpaSpace& space) {
     const float REL_ERROR2 = 1.0e-6f;
     float distance = 0.f;
     V3 normalInB = v3zero();
@@ -1051,7 +1062,7 @@ RSG_FN bool gjk_closest(const CShape& A, const CShape& B, const Xf& ta, const Xf
     if (!isValid || catchDegenerate) {
         V3 tA, tB;
         sep = v3zero();
-        bool ok2 = calc_pen_depth(A, B, lta, ltb, sep, tA, tB);
+        bool ok2 = calc_pen_depth(A, B, lta, ltb, sep, tA, tB, space);
         if (ok2) {
             V3 tn = tB - tA;
             float ls = len2(tn);

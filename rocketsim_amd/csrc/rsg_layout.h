@@ -81,7 +81,7 @@ enum ArenaWord : int {
     AW_GOALS_ORANGE = 4,
     AW_LAST_GOAL_TICK = 5, // 2
     AW_RESETS = 7,
-    AW_STATUS = 8,      // bit 0: non-finite state seen
+    AW_STATUS = 8,      // RSG_STATUS_* bits (sticky)
     AW_CAR_ORDER = 9,   // car update order: slot k = car index in bits 4k..4k+3
     AW_EVENT_SEQ = 10,  // number of events the arena has emitted
     ARENA_WORDS = 11
@@ -106,6 +106,9 @@ struct BatchDev {
     rsg_event* ev_buf;
     uint32_t* ev_count;
     uint32_t ev_cap;
+    // per-arena world-step work space (StepSpace blocks, one per arena) for
+    // the kernels that keep it in global memory; null when not allocated
+    unsigned char* space;
 };
 
 }  // namespace rsg

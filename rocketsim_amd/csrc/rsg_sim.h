@@ -117,7 +117,6 @@ struct Car {
     float lat[4], lon[4], push[4];
     uint32_t gen_redraw, gen_left;
     V3 vel_cache;
-    WheelT w[4];
 };
 
 struct Ball {
@@ -257,19 +256,19 @@ RSG_FN M3 yaw_basis(float yaw) {
               v3(-sj, cj * si, cj * ci));
 }
 
-RSG_FN V3 wheel_upwards_dir(const Car& c) {
+RSG_FN V3 wheel_upwards_dir(const Car& c, const WheelT* wh) {
     V3 sum = v3zero();
     for (int i = 0; i < 4; i++)
-        if (c.w[i].in_contact) sum += c.w[i].contact_normal;
+        if (wh[i].in_contact) sum += wh[i].contact_normal;
     if (sum == v3zero()) return car_up(c);
     return safe_normalized(sum);
 }
 
 // _UpdateWheels (Car.cpp:339-484)
-RSG_FN void car_update_wheels(Car& c, const CarPhys& cp, float dt, int n_contact, float fwd_speed_uu) {
+RSG_FN void car_update_wheels(Car& c, WheelT* wh, const CarPhys& cp, float dt, int n_contact, float fwd_speed_uu) {
     float abs_fwd = fabsf(fwd_speed_uu);
     bool wheels_world = false;
-    for (int i = 0; i < 4; i++) wheels_world |= c.w[i].with_world;
+    for (int i = 0; i < 4; i++) wheels_world |= wh[i].with_world;
 
     if (c.ctrl.handbrake) c.handbrake += POWERSLIDE_RISE_RATE * dt;
     else c.handbrake -= POWERSLIDE_FALL_RATE * dt;
@@ -310,7 +309,7 @@ RSG_FN void car_update_wheels(Car& c, const CarPhys& cp, float dt, int n_contact
     }
 
     for (int i = 0; i < 4; i++) {
-        WheelT& w = c.w[i];
+        WheelT& w = wh[i];
         if (w.ground != -1) {
             V3 lat_dir = col(w.basis, 1);
             V3 long_dir = cross(lat_dir, w.contact_normal);
@@ -340,7 +339,7 @@ RSG_FN void car_update_wheels(Car& c, const CarPhys& cp, float dt, int n_contact
     }
 
     if (wheels_world) {
-        V3 up_dir = wheel_upwards_dir(c);
+        V3 up_dir = wheel_upwards_dir(c, wh);
         bool full_stick = (real_throttle != 0) || (abs_fwd > STOPPING_FORWARD_VEL);
         float sticky_scale = 0.5f;
         if (full_stick) sticky_scale += 1 - fabsf(up_dir.z);
@@ -551,8 +550,8 @@ RSG_FN void car_update_auto_flip(Car& c, float dt, bool jump_pressed) {
 }
 
 // _UpdateAutoRoll (Car.cpp:819-853)
-RSG_FN void car_update_auto_roll(Car& c, int n_contact) {
-    V3 ground_up = n_contact > 0 ? wheel_upwards_dir(c) : c.world_normal;
+RSG_FN void car_update_auto_roll(Car& c, WheelT* wh, int n_contact) {
+    V3 ground_up = n_contact > 0 ? wheel_upwards_dir(c, wh) : c.world_normal;
     V3 ground_down = -ground_up;
     V3 fwd = car_forward(c), right = car_right(c);
     V3 cross_right = cross(ground_up, fwd);
@@ -569,11 +568,11 @@ RSG_FN void car_update_auto_roll(Car& c, int n_contact) {
 
 // btVehicleRL::calcFrictionImpulses (btVehicleRL.cpp:306-380)
 template <class World>
-RSG_FN void vehicle_calc_friction(Car& c, World& world, float dt) {
+RSG_FN void vehicle_calc_friction(Car& c, WheelT* wh, World& world, float dt) {
     float friction_scale = CAR_MASS_BT / 3;  // getMass() = 1 / inverseMass
     friction_scale = (1.f / c.b.inv_mass) / 3;
     for (int i = 0; i < 4; i++) {
-        WheelT& w = c.w[i];
+        WheelT& w = wh[i];
         if (w.ground != -1) {
             const Body* ground = w.ground >= 0 ? world.body_ptr(w.ground) : nullptr;
             V3 axle = col(w.basis, 1);
@@ -623,9 +622,9 @@ RSG_FN void wheel_ray(const Car& c, const CarPhys& cp, int i, V3& source, V3& ta
 // wheel's ray result cast earlier in the same pre-tick (nothing it depends on
 // has changed since).
 template <class World>
-RSG_FN void vehicle_ray_cast(Car& c, const CarPhys& cp, int i, World& world, int self_body, const SolverInfo& si,
+RSG_FN void vehicle_ray_cast(Car& c, WheelT* wh, const CarPhys& cp, int i, World& world, int self_body, const SolverInfo& si,
                              const RayHit* pre) {
-    WheelT& w = c.w[i];
+    WheelT& w = wh[i];
     wheel_transforms_ws(c, cp, i, w);
     float travel = cp.max_susp_travel;
     float rest = cp.wheel_rest[i];
@@ -675,9 +674,9 @@ RSG_FN void vehicle_ray_cast(Car& c, const CarPhys& cp, int i, World& world, int
 }
 
 // btVehicleRL::updateSuspension + applyFrictionImpulses (btVehicleRL.cpp:270-395)
-RSG_FN void vehicle_second(Car& c, const CarPhys& cp, float dt) {
+RSG_FN void vehicle_second(Car& c, WheelT* wh, const CarPhys& cp, float dt) {
     for (int i = 0; i < 4; i++) {
-        WheelT& w = c.w[i];
+        WheelT& w = wh[i];
         if (w.in_contact) {
             float force = (cp.wheel_rest[i] - w.susp_len) * SUSPENSION_STIFFNESS * w.clipped_inv;
             float damp = (w.susp_rel_vel < 0) ? WHEELS_DAMPING_COMPRESSION : WHEELS_DAMPING_RELAXATION;
@@ -689,7 +688,7 @@ RSG_FN void vehicle_second(Car& c, const CarPhys& cp, float dt) {
         }
     }
     for (int i = 0; i < 4; i++) {
-        WheelT& w = c.w[i];
+        WheelT& w = wh[i];
         if (w.susp_force != 0) {
             V3 off = w.contact_point - c.b.pos;
             float base = (w.susp_force * dt) + c.push[i];
@@ -699,7 +698,7 @@ RSG_FN void vehicle_second(Car& c, const CarPhys& cp, float dt) {
     }
     V3 up = car_up(c);
     for (int i = 0; i < 4; i++) {
-        WheelT& w = c.w[i];
+        WheelT& w = wh[i];
         if (!is_zero(w.impulse)) {
             V3 off = w.contact_point - c.b.pos;
             float d = dot(up, off);

@@ -20,7 +20,13 @@
 namespace rsg {
 
 constexpr int MAX_SOLVER_BODIES = 1 + RSG_MAX_CARS + 1;
-constexpr int MAX_ROWS = 96;
+// Contact rows and friction rows per arena and tick (each); a host test
+// build may lower the capacity (RSG_TEST_MAX_SOLVER_ROWS).
+#ifdef RSG_TEST_MAX_SOLVER_ROWS
+constexpr int MAX_ROWS = RSG_TEST_MAX_SOLVER_ROWS;
+#else
+constexpr int MAX_ROWS = RSG_MAX_SOLVER_ROWS;
+#endif
 
 struct SBody {
     int body;        // dynamic body index, -1 for the fixed body
@@ -56,9 +62,9 @@ struct Solver {
     int n_bodies;
     int fixed_id;
     int companion[MAX_SOLVER_BODIES];  // dynamic body -> solver body, -1 none
-    SRow rows[MAX_ROWS];
+    SRow* rows;      // MAX_ROWS contact rows (storage placed by the caller)
     int n_rows;
-    SRow fric[MAX_ROWS];
+    SRow* fric;      // MAX_ROWS friction rows
     int n_fric;
     SpecialInfo special[MAX_SOLVER_BODIES];
     float dt;

@@ -456,23 +456,25 @@ RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int c
     if (flag(c.flags, RSG_CF_DEMOED)) return;
 
     // updateVehicleFirst (btVehicleRL.cpp:215-228)
-    for (int i = 0; i < 4; i++) wheel_update_transform(c, cp, i, c.w[i]);
+    // btVehicleRL's wheel records: rebuilt every tick, used only in this pre-tick
+    WheelT wh[4];
+    for (int i = 0; i < 4; i++) wheel_update_transform(c, cp, i, wh[i]);
     RSG_PROF_MARK(1);
     for (int i = 0; i < 4; i++) {
-        vehicle_ray_cast(c, cp, i, world, 1 + ci, A.si, pre ? &pre[ci * 4 + i] : nullptr);
+        vehicle_ray_cast(c, wh, cp, i, world, 1 + ci, A.si, pre ? &pre[ci * 4 + i] : nullptr);
         RSG_PROF_MARK(6);
     }
-    vehicle_calc_friction(c, world, dt);
+    vehicle_calc_friction(c, wh, world, dt);
 
     bool jump_pressed = c.ctrl.jump && !c.last.jump;
     int n_contact = 0;
     for (int i = 0; i < 4; i++) {
-        setflag(c.flags, RSG_CF_WHEEL0 << i, c.w[i].in_contact);
-        n_contact += c.w[i].in_contact ? 1 : 0;
+        setflag(c.flags, RSG_CF_WHEEL0 << i, wh[i].in_contact);
+        n_contact += wh[i].in_contact ? 1 : 0;
     }
     setflag(c.flags, RSG_CF_ON_GROUND, n_contact >= 3);
     float fwd_speed_uu = dot(c.b.vel, car_forward(c)) * BT_TO_UU;
-    car_update_wheels(c, cp, dt, n_contact, fwd_speed_uu);
+    car_update_wheels(c, wh, cp, dt, n_contact, fwd_speed_uu);
     if (n_contact < 3)
         car_update_air_torque(c, dt, n_contact == 0);
     else
@@ -481,9 +483,9 @@ RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int c
     car_update_auto_flip(c, dt, jump_pressed);
     car_update_double_jump_or_flip(c, dt, jump_pressed, fwd_speed_uu);
     if (c.ctrl.throttle != 0.f && ((n_contact > 0 && n_contact < 4) || flag(c.flags, RSG_CF_WORLD_CONTACT)))
-        car_update_auto_roll(c, n_contact);
+        car_update_auto_roll(c, wh, n_contact);
     setflag(c.flags, RSG_CF_WORLD_CONTACT, false);
-    vehicle_second(c, cp, dt);
+    vehicle_second(c, wh, cp, dt);
     car_update_boost(c, dt, boost_used_per_second(P.mode));
 }
 
@@ -617,6 +619,12 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
         }
     }
     A.tick++;
+    // non-finite poses or velocities leave the reference's results for good
+    {
+        V3 s = A.ball.b.pos + A.ball.b.vel;
+        for (int i = 0; i < A.n_cars; i++) s += A.cars[i].b.pos + A.cars[i].b.vel + A.cars[i].b.ang;
+        if (!(isfinite(s.x) && isfinite(s.y) && isfinite(s.z))) A.status |= RSG_STATUS_NONFINITE;
+    }
     if (scored && P.auto_reset) arena_kickoff_reset(A, P.seed, P.arena_id);
     RSG_PROF_MARK(8);
 }
@@ -683,16 +691,16 @@ RSG_FN void ball_pre_tick_heatseeker(Ball& ball, float dt) {
 
 // The world step of the tick (btDiscreteDynamicsWorld::stepSimulation),
 // preceded by the ball's pre-tick, which runs after the pads' (Arena.cpp:715).
-RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const MeshView* mesh) {
+RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws) {
     if (P.mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(A.ball, P.dt);
-    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
+    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
     world.step(P.dt, P);
     RSG_PROF_MARK(7);
 }
 
 // The rest of Arena::Step after tick_begin, serially.  `pre` holds every
 // wheel's ray result (car-major, 4 per car) when cast ahead, else null.
-RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, const RayHit* pre) {
+RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, const RayHit* pre, StepSpace* ws) {
     {
         World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
         RSG_PROF_MARK(0);
@@ -703,7 +711,7 @@ RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, 
     // 3. pads pre-tick
     pads_pre_tick(A, P);
     // 4. world step
-    tick_world(A, P, mesh);
+    tick_world(A, P, mesh, ws);
     // 5. cars post-tick + pad collisions
     for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
     if (mode_has_arena(P.mode))
@@ -712,9 +720,9 @@ RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, 
 }
 
 // Arena::Step (src/Sim/Arena/Arena.cpp:689-756), one tick.
-RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh) {
+RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws) {
     tick_begin(A, P);
-    tick_rest(A, P, mesh, nullptr);
+    tick_rest(A, P, mesh, nullptr, ws);
 }
 
 }  // namespace rsg

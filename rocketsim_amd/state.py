@@ -37,6 +37,8 @@ CF_BODY_DISABLED = 1 << 16
 
 # rsg_batch_set_kernel
 KERNEL_AUTO, KERNEL_SCRATCH, KERNEL_LDS, KERNEL_COOP = 0, 1, 2, 3
+# per-arena status bits (include/rsg_state.h)
+STATUS_NONFINITE, STATUS_CONTACT_OVERFLOW, STATUS_SOLVER_OVERFLOW = 1, 2, 4
 
 F3 = C.c_float * 3
 F4 = C.c_float * 4

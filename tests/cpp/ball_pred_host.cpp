@@ -130,7 +130,9 @@ int bp_update(void* p, int kind, const rsg_ball_state* cur, const int32_t* ticks
     }
     MeshView mv = h.mesh.view();
     auto* A = new ArenaLocal;
-    for (int i = 0; i < h.n; i++) ball_pred_update(h.dev, h.soccar ? &mv : nullptr, d, kind, src, i, *A);
+    auto* ws = new StepSpace;
+    for (int i = 0; i < h.n; i++) ball_pred_update(h.dev, h.soccar ? &mv : nullptr, d, kind, src, i, *A, ws);
+    delete ws;
     delete A;
     return 0;
 }

@@ -1,0 +1,167 @@
+// Host build of the batched tick (rocketsim_amd/csrc/rsg_tick.h, the code the
+// step kernels run) for the CPU tests: a batch in the GPU's struct-of-arrays
+// word layout, stepped arena by arena on the host.
+//
+// Build switches:
+//   RSG_SSE_NORMALIZE        vector normalisation as the reference's x86-64
+//                            build does it (btVector3::normalize: rsqrtss plus
+//                            one Newton step, LinearMath/btVector3.h); without
+//                            it, the GPU's (divide by the length);
+//   RSG_TEST_MAX_MANIFOLDS,  lower contact capacities, so that ordinary play
+//   RSG_TEST_MAX_SOLVER_ROWS reaches the overflow status bits.
+#ifdef RSG_SSE_NORMALIZE
+#include <immintrin.h>
+
+template <class T>
+static inline T sse_normalized(T a) {
+    float l2 = (a.x * a.x + a.y * a.y) + a.z * a.z;
+    float y = _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(l2)));
+    float h = l2 * 0.5f;
+    h = h * y;
+    h = h * y;
+    y = y * (1.5f - h);
+    T r = a;
+    r.x = a.x * y;
+    r.y = a.y * y;
+    r.z = a.z * y;
+    return r;
+}
+#define RSG_NORMALIZE_HOOK(a) sse_normalized(a)
+#endif
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rsg.h"
+#include "../../rocketsim_amd/csrc/rsg_mesh_build.h"
+#include "../../rocketsim_amd/csrc/rsg_step.h"
+#include "../../rocketsim_amd/csrc/rsg_api_convert.h"
+
+using namespace rsg;
+
+namespace {
+
+struct HostBatch {
+    HostMesh mesh;
+    std::vector<int> cars_per_arena, car_offset;
+    std::vector<uint32_t> car, ball, pad, arena;
+    BatchDev dev{};
+};
+
+}  // namespace
+
+extern "C" {
+
+// A host batch of n arenas (cars_per_arena[a] cars each, teams and presets
+// per car); cmf = the arena mesh file (NULL for THE_VOID).
+void* th_create(int mode, int n, const int* cars_per_arena, const uint8_t* team, const uint8_t* preset,
+                const void* cmf, uint64_t cmf_size, float tick_rate) {
+    auto* h = new HostBatch();
+    if (cmf && !build_host_mesh(cmf, cmf_size, h->mesh).empty()) {
+        delete h;
+        return nullptr;
+    }
+    int total = 0, maxc = 1;
+    for (int a = 0; a < n; a++) {
+        h->cars_per_arena.push_back(cars_per_arena[a]);
+        h->car_offset.push_back(total);
+        total += cars_per_arena[a];
+        maxc = cars_per_arena[a] > maxc ? cars_per_arena[a] : maxc;
+    }
+    h->car.assign((size_t)CAR_WORDS * maxc * n, 0);
+    h->ball.assign((size_t)BALL_WORDS * n, 0);
+    h->pad.assign((size_t)PAD_WORDS * PAD_NUM * n, 0);
+    h->arena.assign((size_t)ARENA_WORDS * n, 0);
+    BatchDev& d = h->dev;
+    d.car = h->car.data();
+    d.ball = h->ball.data();
+    d.pad = h->pad.data();
+    d.arena = h->arena.data();
+    d.n = n;
+    d.max_cars = maxc;
+    d.mode = mode;
+    d.tick_time = 1 / tick_rate;
+    for (int a = 0; a < n; a++)
+        init_arena_record(d, a, cars_per_arena[a], team + h->car_offset[a], preset + h->car_offset[a]);
+    return h;
+}
+
+void th_destroy(void* p) { delete (HostBatch*)p; }
+
+void th_set_cars(void* p, const rsg_car_state* in) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++)
+        for (int i = 0; i < h.cars_per_arena[a]; i++) car_from_api(car_ref(h.dev, a, i), in[h.car_offset[a] + i]);
+}
+
+void th_get_cars(void* p, rsg_car_state* out) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++)
+        for (int i = 0; i < h.cars_per_arena[a]; i++) car_to_api(car_ref(h.dev, a, i), out[h.car_offset[a] + i]);
+}
+
+void th_set_balls(void* p, const rsg_ball_state* in) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++) ball_from_api(ball_ref(h.dev, a), in[a]);
+}
+
+void th_get_balls(void* p, rsg_ball_state* out) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++) ball_to_api(ball_ref(h.dev, a), out[a]);
+}
+
+void th_set_pads(void* p, const rsg_pad_state* in) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++)
+        for (int k = 0; k < PAD_NUM; k++) pad_from_api(pad_ref(h.dev, a, k), in[a * PAD_NUM + k]);
+}
+
+void th_get_pads(void* p, rsg_pad_state* out) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++)
+        for (int k = 0; k < PAD_NUM; k++) pad_to_api(pad_ref(h.dev, a, k), out[a * PAD_NUM + k]);
+}
+
+void th_set_controls(void* p, const rsg_controls* in) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++)
+        for (int i = 0; i < h.cars_per_arena[a]; i++)
+            store_ctrl(car_ref(h.dev, a, i), CW_CTRL, ctrl_from_api(in[h.car_offset[a] + i]));
+}
+
+void th_set_tick_counts(void* p, const uint64_t* in) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++) arena_ref(h.dev, a).su64(AW_TICK, in[a]);
+}
+
+void th_get_tick_counts(void* p, uint64_t* out) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++) out[a] = arena_ref(h.dev, a).u64(AW_TICK);
+}
+
+void th_set_car_order(void* p, const int32_t* order) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++) {
+        uint32_t ow = 0;
+        for (int k = 0; k < h.cars_per_arena[a]; k++) ow |= (uint32_t)order[h.car_offset[a] + k] << (4 * k);
+        arena_ref(h.dev, a).u(AW_CAR_ORDER) = ow;
+    }
+}
+
+void th_get_status(void* p, uint32_t* out) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++) out[a] = arena_ref(h.dev, a).u(AW_STATUS);
+}
+
+// Steps every arena `ticks` ticks with the plain kernel's code path.
+void th_step(void* p, int ticks) {
+    HostBatch& h = *(HostBatch*)p;
+    MeshView mv = h.mesh.view();
+    auto A = std::make_unique<ArenaLocal>();
+    auto ws = std::make_unique<StepSpace>();
+    for (int a = 0; a < h.dev.n; a++) step_arena(h.dev, mv, a, ticks, *A, ws.get());
+}
+
+}  // extern "C"

@@ -334,18 +334,58 @@ def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1, n
 # (btVector3::normalize, LinearMath/btVector3.h:304-340), whose last bits
 # depend on the x86 CPU; the GPU divides by the length.  GJK/EPA (which stop
 # at a 1e-4 squared distance) and the subsimplex ray cast can turn such ulp
-# differences into visibly different contacts in rare configurations.  A host
-# build of the same tick code with rsqrtss-emulated normalisation matches
-# the oracle on every sample of the drive and melee sets; with plain
-# normalisation about 1 tick in 3500 of the melee set diverges.  Contact-rich
-# sets therefore allow this many samples out of tolerance.
+# differences into visibly different contacts in rare configurations.
+#
+# Contact-rich sets therefore allow samples out of tolerance for this one
+# mechanism only, and check each such sample against it with the host build
+# of the same tick code (tests/cpp/tick_host.cpp):
+#   * built with the reference's SSE normalisation, it must match the oracle
+#     on that sample (so everything but the normalisation agrees), and
+#   * built with the GPU's normalisation, it must give the GPU's result
+#     within the one-tick tolerances (so the GPU's departure is the
+#     normalisation's).
+# At most CHAOTIC_FRACTION of a set may be out of tolerance even so.
 CHAOTIC_FRACTION = 0.005
 
 
-def assert_parity(samples, results, max_frac=0.0):
-    nbad, msgs = check_samples(samples, results)
-    allowed = int(max_frac * len(samples))
+def explain_outliers(samples, results, mode, bad_idx):
+    """Check the out-of-tolerance samples `bad_idx` against the normalisation
+    mechanism above.  Returns messages for the samples it does not explain."""
+    import tick_host
+    sub = [samples[i] for i in bad_idx]
+    sse_res, _ = tick_host.run_samples(tick_host.get("sse"), sub, mode)
+    plain_res, _ = tick_host.run_samples(tick_host.get("plain"), sub, mode)
+    out = []
+    for k, i in enumerate(bad_idx):
+        s = samples[i]
+        n_sse, m_sse = check_samples([s], [sse_res[k]])
+        # the host build with the GPU's normalisation in the reference's place
+        as_ref = Sample()
+        as_ref.mode, as_ref.teams, as_ref.presets = s.mode, s.teams, s.presets
+        as_ref.cars_out, as_ref.ball_out, as_ref.pads_out, as_ref.tick_out = plain_res[k]
+        n_plain, m_plain = check_samples([as_ref], [results[i][:4]])
+        if n_sse:
+            out.append(f"sample {i}: the SSE host build departs from the oracle too: {m_sse[0]}")
+        elif n_plain:
+            out.append(f"sample {i}: the GPU departs from the host build with its own normalisation: {m_plain[0]}")
+    return out
+
+
+def assert_parity(samples, results, mode, chaotic=False):
+    """Every sample within the tolerances, or (chaotic=True) at most
+    CHAOTIC_FRACTION of them out, each explained by the normalisation
+    mechanism (explain_outliers).  Prints and returns the out-of-tolerance
+    count."""
+    nbad, msgs, bad_idx = check_samples(samples, results, return_indices=True)
+    print(f"\n  {nbad}/{len(samples)} samples out of tolerance", flush=True)
+    if nbad == 0:
+        return 0
+    assert chaotic, f"{nbad}/{len(samples)} samples off\n" + "\n".join(msgs)
+    allowed = int(CHAOTIC_FRACTION * len(samples))
     assert nbad <= allowed, f"{nbad}/{len(samples)} samples off (allowed {allowed})\n" + "\n".join(msgs)
+    unexplained = explain_outliers(samples, results, mode, bad_idx)
+    assert not unexplained, "\n".join(unexplained + msgs)
+    print(f"  all {nbad} explained by the normalisation rounding (host builds)", flush=True)
     return nbad
 
 
@@ -402,6 +442,25 @@ def sample_heatseeker(ref, n_traj, ticks, seed, stride):
                 A.step(1)
         A.close()
     return out
+
+
+def sample_from_state(ref, mode, teams, presets, cars, ball, pads, tick, ctrls):
+    """A teacher-forced sample whose "before" state is given in API units (for
+    instance a state the GPU reached in a long batched run): a fresh oracle
+    arena is set to it and stepped once with `ctrls`."""
+    A = ref.RefArena(mode)
+    try:
+        for t, p in zip(teams, presets):
+            A.add_car(t, p)
+        for i, c in enumerate(cars):
+            A.set_car(i, c)
+        A.set_ball(ball)
+        if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
+            A.set_pads(pads)
+        A.tick_count = tick
+        return _one_tick(A, mode, teams, presets, ctrls)
+    finally:
+        A.close()
 
 
 def _one_tick(A, mode, teams, presets, ctrls):
@@ -505,13 +564,17 @@ def load_samples(path):
     return out
 
 
-def run_on_gpu(samples, mode):
-    """Step every sample once on the GPU, one sample per arena."""
+def run_on_gpu(samples, mode, kernel=0, per_block=0):
+    """Step every sample once on the GPU, one sample per arena.  `kernel` and
+    `per_block` pick the step kernel and arenas per workgroup (0: the
+    library's automatic choice)."""
     from rocketsim_amd.batch import ArenaBatch
     cpa = [len(s.teams) for s in samples]
     batch = ArenaBatch(cpa, sum((s.teams for s in samples), []), sum((s.presets for s in samples), []),
                        mode=mode)
     try:
+        batch.set_kernel(kernel)
+        batch.set_launch(per_block)
         batch.set_cars(sum((list(s.cars_in) for s in samples), []))
         batch.set_balls([s.ball_in for s in samples])
         batch.set_pads(sum((list(s.pads_in) for s in samples), []))
@@ -541,10 +604,11 @@ def run_on_gpu(samples, mode):
     return results
 
 
-def check_samples(samples, results, max_report=10):
-    """Return (n_bad_samples, messages)."""
+def check_samples(samples, results, max_report=10, return_indices=False):
+    """Return (n_bad_samples, messages[, indices of the bad samples])."""
     msgs = []
     nbad = 0
+    bad_idx = []
     for i, (s, res) in enumerate(zip(samples, results)):
         cars, ball, pads, tick = res[:4]
         bad = []
@@ -561,6 +625,9 @@ def check_samples(samples, results, max_report=10):
             bad.append(("tick", s.tick_out, tick, None))
         if bad:
             nbad += 1
+            bad_idx.append(i)
             if len(msgs) < max_report:
                 msgs.append(f"sample {i}: " + "; ".join(f"{b[0]} ref={b[1]} got={b[2]} err={b[3]}" for b in bad[:6]))
+    if return_indices:
+        return nbad, msgs, bad_idx
     return nbad, msgs

@@ -28,7 +28,7 @@ def _oracle():
 def test_soccar_cars_fixture_parity(name):
     samples = parity.load_samples(os.path.join(HERE, "golden", name))
     results = parity.run_on_gpu(samples, MODE_SOCCAR)
-    parity.assert_parity(samples, results, parity.CHAOTIC_FRACTION)
+    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True)
 
 
 @pytest.mark.gpu
@@ -38,7 +38,7 @@ def test_soccar_drive_live_oracle_parity():
         pytest.skip("oracle library not built on this machine")
     samples = parity.sample_soccar_drive(ref, n_traj=12, ticks_per_traj=200, seed=31, n_cars=6, stride=4)
     results = parity.run_on_gpu(samples, MODE_SOCCAR)
-    parity.assert_parity(samples, results, parity.CHAOTIC_FRACTION)
+    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True)
 
 
 @pytest.mark.gpu
@@ -48,7 +48,7 @@ def test_soccar_melee_live_oracle_parity():
         pytest.skip("oracle library not built on this machine")
     samples = parity.sample_soccar_melee(ref, n_traj=40, ticks_per_traj=90, seed=37, n_cars=4, stride=1)
     results = parity.run_on_gpu(samples, MODE_SOCCAR)
-    parity.assert_parity(samples, results, parity.CHAOTIC_FRACTION)
+    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True)
 
 
 @pytest.mark.gpu
@@ -59,4 +59,4 @@ def test_soccar_walls_live_oracle_parity():
     samples = parity.sample_soccar_melee(ref, n_traj=30, ticks_per_traj=90, seed=41, n_cars=6, stride=1,
                                          near_walls=True)
     results = parity.run_on_gpu(samples, MODE_SOCCAR)
-    parity.assert_parity(samples, results, parity.CHAOTIC_FRACTION)
+    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True)

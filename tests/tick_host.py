@@ -1,0 +1,105 @@
+"""ctypes front end of the host build of the batched tick (tests/cpp/tick_host.cpp):
+the code the step kernels run, compiled for the CPU with g++, in the GPU's
+struct-of-arrays layout.  Used by the CPU tests as a second implementation
+to explain where the GPU and the reference part."""
+import ctypes as C
+import os
+import subprocess
+
+from rocketsim_amd.state import BallState, CarState, Controls, MODE_HEATSEEKER, MODE_SOCCAR, NUM_PADS, PadState
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "cpp", "tick_host.cpp")
+
+
+def build(out_path, defines=(), extra=(), load_it=True):
+    """Compile the host tick library with the given -D switches (and load it,
+    unless load_it is False: a sanitizer build loads only under its runtime)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", *["-D" + d for d in defines],
+           *extra, "-I" + os.path.join(ROOT, "include"), SRC, "-o", out_path]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-4000:])
+    return load(out_path) if load_it else out_path
+
+
+_cache = {}
+VARIANTS = {"sse": ["RSG_SSE_NORMALIZE"], "plain": [],
+            # contact capacities low enough for ordinary play to reach them
+            "small_manifolds": ["RSG_TEST_MAX_MANIFOLDS=1"], "small_rows": ["RSG_TEST_MAX_SOLVER_ROWS=4"]}
+
+
+def get(variant):
+    """The host build `variant` (VARIANTS), compiled once per process."""
+    if variant not in _cache:
+        import tempfile
+        d = tempfile.mkdtemp(prefix="tick_host_")
+        _cache[variant] = build(os.path.join(d, f"tick_host_{variant}.so"), VARIANTS[variant])
+    return _cache[variant]
+
+
+def load(path):
+    L = C.CDLL(path)
+    L.th_create.restype = C.c_void_p
+    L.th_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
+                            C.c_char_p, C.c_uint64, C.c_float]
+    L.th_destroy.argtypes = [C.c_void_p]
+    for name, t in (("cars", CarState), ("balls", BallState), ("pads", PadState)):
+        getattr(L, "th_set_" + name).argtypes = [C.c_void_p, C.POINTER(t)]
+        getattr(L, "th_get_" + name).argtypes = [C.c_void_p, C.POINTER(t)]
+    L.th_set_controls.argtypes = [C.c_void_p, C.POINTER(Controls)]
+    L.th_set_tick_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    L.th_get_tick_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    L.th_set_car_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    L.th_get_status.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+    L.th_step.argtypes = [C.c_void_p, C.c_int]
+    return L
+
+
+def run_samples(L, samples, mode, cmf=None, inject=None):
+    """Step every teacher-forced sample once on the host build, one sample per
+    arena (the host counterpart of parity.run_on_gpu).  `inject(cars, balls)`
+    may edit the "before" records.  Returns (results, status words)."""
+    if mode in (MODE_SOCCAR, MODE_HEATSEEKER) and cmf is None:
+        from rocketsim_amd import arena_mesh
+        cmf = arena_mesh.soccar_cmf()
+    n = len(samples)
+    cpa = [len(s.teams) for s in samples]
+    teams = sum((list(s.teams) for s in samples), [])
+    presets = sum((list(s.presets) for s in samples), [])
+    total = sum(cpa)
+    h = L.th_create(mode, n, (C.c_int * n)(*cpa), (C.c_uint8 * max(total, 1))(*teams),
+                    (C.c_uint8 * max(total, 1))(*presets), cmf, len(cmf) if cmf else 0, 120.0)
+    if not h:
+        raise RuntimeError("th_create failed")
+    try:
+        cars = (CarState * max(total, 1))(*sum((list(s.cars_in) for s in samples), []))
+        balls = (BallState * n)(*[s.ball_in for s in samples])
+        if inject:
+            inject(cars, balls)
+        L.th_set_cars(h, cars)
+        L.th_set_balls(h, balls)
+        L.th_set_pads(h, (PadState * (n * NUM_PADS))(*sum((list(s.pads_in) for s in samples), [])))
+        L.th_set_controls(h, (Controls * max(total, 1))(*sum((list(s.ctrls) for s in samples), [])))
+        L.th_set_tick_counts(h, (C.c_uint64 * n)(*[s.tick_in for s in samples]))
+        L.th_set_car_order(h, (C.c_int32 * max(total, 1))(*sum((list(s.car_order) for s in samples), [])))
+        L.th_step(h, 1)
+        out_cars = (CarState * max(total, 1))()
+        out_balls = (BallState * n)()
+        out_pads = (PadState * (n * NUM_PADS))()
+        ticks = (C.c_uint64 * n)()
+        status = (C.c_uint32 * n)()
+        L.th_get_cars(h, out_cars)
+        L.th_get_balls(h, out_balls)
+        L.th_get_pads(h, out_pads)
+        L.th_get_tick_counts(h, ticks)
+        L.th_get_status(h, status)
+    finally:
+        L.th_destroy(h)
+    results, k = [], 0
+    for i, s in enumerate(samples):
+        c = len(s.teams)
+        results.append((out_cars[k:k + c], out_balls[i], out_pads[i * NUM_PADS:(i + 1) * NUM_PADS], ticks[i]))
+        k += c
+    return results, list(status)
