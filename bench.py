@@ -1,24 +1,27 @@
 """Throughput of the batched Soccar tick step on one MI355X.
 
-Workload (one "step" = one tick of every arena): 4096 independent Soccar 2v2
+Workload (one "step" = one tick of every arena): 16384 independent Soccar 2v2
 arenas, 4 Octane cars + ball each, on the synthetic arena mesh, cars driven by
 the seeded control process of include/rsg_workload.h (redraw every U{2..60}
 ticks), kickoff auto-reset on goal.  State is resident in HBM for the whole
-timed region.
+timed region.  The batch first plays a fixed pre-roll (240 ticks, 2 s of
+game time) from kickoff, so that every timed window is contact-rich play
+(cars on the ground and walls, ball touches), whatever --steps is.
 
-Prints one JSON line: whole-job arena-ticks/s, the roofline of the dominant
-kernel (HBM-bound: algorithmic bytes per arena-tick = full state read+write +
+Prints one JSON line: whole-job arena-ticks/s of the batched path, the plain
+one-arena-per-thread GPU kernel on the same workload and window
+(plain_gpu, with the speedup), the roofline of the dominant kernel
+(HBM-bound: algorithmic bytes per arena-tick = full state read+write +
 controls) and the reference CPU baseline (RocketSim compiled from source, one
-Arena per thread, 2v2) timed on this host on a bounded sample.
+Arena per pinned thread on every core this process may use, 2v2; plus the
+single-thread 1v1 figure of the reference's own benchmark shape).
 
-The batched path is the default: the kernel and launch shape come from the
-autotuner, ticks run in 8-tick launches captured in one hipGraph.
---compare-plain also times the plain one-arena-per-thread kernel (private
-scratch, 64 arenas per workgroup, one tick per launch) on the same workload.
---workload mixed is the uneven batch: 32768 arenas, 40/35/25 % 1v1/2v2/3v3,
-six hitbox presets, random boost.
+The batched path: kernel and launch shape from the autotuner, ticks in
+8-tick launches, the whole chunks captured in one hipGraph and any remainder
+launched eagerly.  --workload mixed is the uneven batch: 32768 arenas,
+40/35/25 % 1v1/2v2/3v3, six hitbox presets, random boost.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload soccar2v2|mixed] [--compare-plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload soccar2v2|mixed] [--arenas N]
 """
 import argparse
 import json
@@ -35,30 +38,65 @@ HBM_PEAK_GBS = 8000.0
 # pads 272 B + tick 8 B) + 4 x 24 B controls.
 BYTES_PER_ARENA_TICK_2V2 = 2488
 # HBM bytes per launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE passes
-# over `bench.py --profile-probe` (counter factor 1.0 from a 64 MiB copy),
-# keyed by (arenas, ticks per launch, arenas per workgroup).
-HBM_TRAFFIC_PER_LAUNCH = {("rsg_step_arena_coop_kernel", 4096, 8, 4): (339319 + 586199) * 1024}
+# over `bench.py --profile-probe` (counter factor from a 64 MiB copy), keyed by
+# (kernel, arenas, ticks per launch, arenas per workgroup).
+HBM_TRAFFIC_PER_LAUNCH = {}
+
+
+def _host_info():
+    """Cores this process may run on, the CPU model and the oracle's compiler."""
+    import platform
+    import subprocess
+    cpus = sorted(os.sched_getaffinity(0))
+    model = platform.processor()
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+            quota = None if q == "max" else round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    try:
+        cc = subprocess.run(["/opt/rocm/llvm/bin/clang++", "--version"], capture_output=True,
+                            text=True).stdout.splitlines()[0]
+    except (OSError, IndexError):
+        cc = "unknown"
+    return cpus, model, quota, cc
 
 
 def cpu_baseline(target_seconds=10.0):
-    """Reference Arena::Step, one arena per std::thread, all host threads."""
+    """The reference's Arena::Step on the host: one Soccar 2v2 arena per
+    std::thread, one thread pinned to each core this process may use, on a
+    bounded sample of the same workload; plus one thread of a 1v1 arena for
+    100k ticks (the reference's own single-thread benchmark shape)."""
     from oracle import ref
     from rocketsim_amd import arena_mesh
     if not ref.available():
         return None
     ref.init([arena_mesh.soccar_cmf()])
-    threads = min(16, os.cpu_count() or 1)
+    cpus, model, quota, cc = _host_info()
+    threads = len(cpus)
     # grow the sample until it takes at least half the target wall time
-    ticks = 2000
+    ticks = 1000
     while True:
-        n, secs = ref.run_workload(threads, 1, 2, 2, 0, ticks, 12345)
+        n, secs = ref.run_workload(threads, 1, 2, 2, 0, ticks, 12345, cpus)
         if secs >= 0.5 * target_seconds or ticks >= 4_000_000:
             break
         ticks = int(ticks * min(50.0, max(2.0, target_seconds / max(secs, 1e-3))))
+    n1, s1 = ref.run_workload(1, 1, 1, 1, 0, 100_000, 777, cpus[:1])
     return {
         "value": n / secs, "unit": "arena-ticks/s", "cores": threads, "kind": "reference",
-        "sample": f"{threads} threads x 1 Soccar 2v2 Octane arena x {ticks} ticks, same control process, "
+        "sample": f"{threads} pinned threads x 1 Soccar 2v2 Octane arena x {ticks} ticks, same control process, "
                   f"kickoff reset on goal ({secs:.1f} s wall)",
+        "cpu_model": model, "nproc": os.cpu_count(), "cgroup_cpu_quota": quota,
+        "compiler": cc + ", -O3 -DNDEBUG -ffp-contract=off (oracle/Makefile)",
+        "single_thread_1v1": {"value": n1 / s1, "unit": "ticks/s", "ticks": n1,
+                              "sample": "1 thread, 1 Soccar 1v1 Octane arena, 100k ticks, seeded controls"},
     }
 
 
@@ -93,18 +131,21 @@ def make_batch(workload, n_arenas, seed):
                           f"boost U[0,100], seeded controls, auto-reset on goal"), sum(counts) / n_arenas
 
 
-def run_timed(batch, steps, warmup, chunk, use_graph, stream):
-    """Warm up, then time exactly `steps` ticks (one graph replay, or launches
-    of `chunk` ticks) between two synchronisations.  Returns (wall s, event ms)."""
+def run_timed(batch, steps, warmup, preroll, chunk, use_graph, stream):
+    """Pre-roll `preroll` ticks from kickoff and warm up `warmup` ticks, then
+    time exactly `steps` ticks between two synchronisations: the whole
+    `chunk`-tick launches as one hipGraph replay (or eager launches) and the
+    remainder as one eager launch.  The HIP events are recorded on the stream
+    the kernels run on.  Returns (wall s, event ms, chunk, graph used)."""
     import torch
     sp = stream.cuda_stream
     chunk = max(1, min(chunk, steps))
-    launches = [chunk] * (steps // chunk) + ([steps % chunk] if steps % chunk else [])
-    for _ in range(max(1, warmup // chunk)):
-        batch.step(chunk, sp)
-    graph = use_graph and steps % chunk == 0
+    whole, rest = steps // chunk, steps % chunk
+    for n in [chunk] * ((preroll + warmup) // chunk) + ([(preroll + warmup) % chunk] if (preroll + warmup) % chunk else []):
+        batch.step(n, sp)
+    graph = use_graph and whole > 0
     if graph:
-        batch.graph_create(chunk, steps // chunk)
+        batch.graph_create(chunk, whole)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
@@ -113,8 +154,10 @@ def run_timed(batch, steps, warmup, chunk, use_graph, stream):
     if graph:
         batch.graph_launch(sp)
     else:
-        for n in launches:
-            batch.step(n, sp)
+        for _ in range(whole):
+            batch.step(chunk, sp)
+    if rest:
+        batch.step(rest, sp)
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -164,10 +207,9 @@ def bench_shot(n=16384, ticks=90, reps=5, seed=13):
     every 16 ticks): rollouts/s of forward + adjoint through the car-ball hit,
     through the LibTorch autograd Function."""
     import torch
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import shot_ref
     from rocketsim_amd.adjoint import shot_rollout
-    s0, ctrl, tgt = shot_ref.random_problem(n, ticks, seed)
+    from rocketsim_amd.workloads import shot_problem
+    s0, ctrl, tgt = shot_problem(n, ticks, seed)
     state0, target = s0.float().cuda(), tgt.float().cuda()
     params = torch.atanh(ctrl[..., :3].clamp(-0.999, 0.999)).float().cuda()
     params = torch.cat([params, torch.logit(ctrl[..., 3:].clamp(1e-3, 1 - 1e-3)).float().cuda()], -1)
@@ -246,8 +288,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=512)
     ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--preroll", type=int, default=240,
+                    help="ticks played from kickoff before the warm-up, so the timed window is contact-rich play")
     ap.add_argument("--workload", choices=["soccar2v2", "mixed"], default="soccar2v2")
-    ap.add_argument("--arenas", type=int, default=None, help="default 4096 (soccar2v2) or 32768 (mixed)")
+    ap.add_argument("--arenas", type=int, default=None, help="default 16384 (soccar2v2) or 32768 (mixed)")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--arenas-per-block", type=int, default=0,
                     help="arenas per workgroup of the step kernel (0 = autotune, or the kernel's default)")
@@ -256,8 +300,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=8,
                     help="ticks per kernel launch (the arena state stays on chip across a chunk)")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of one captured hipGraph")
-    ap.add_argument("--compare-plain", action="store_true",
-                    help="also time the plain one-arena-per-thread kernel (scratch, 64 arenas per workgroup, "
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain one-arena-per-thread kernel (private memory, 64 arenas per workgroup, "
                          "one tick per launch) on the same workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable air-control line")
@@ -265,7 +309,7 @@ def main():
     ap.add_argument("--no-ball-pred", action="store_true", help="skip the ball-prediction line")
     ap.add_argument("--profile-probe", action="store_true",
                     help="for rocprofv3 --pmc passes: fill and copy a 64 MiB buffer (calibrates the HBM byte "
-                         "counters), then two launches of --chunk ticks; prints no JSON")
+                         "counters), pre-roll, then two launches of --chunk ticks; prints no JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -273,7 +317,7 @@ def main():
               "so N GPUs would each step their own shard of the arena range with no per-tick exchange.",
               file=sys.stderr)
         sys.exit(2)
-    n_arenas = args.arenas or (4096 if args.workload == "soccar2v2" else 32768)
+    n_arenas = args.arenas or (16384 if args.workload == "soccar2v2" else 32768)
 
     import torch
     from rocketsim_amd import arena_mesh, native
@@ -285,9 +329,16 @@ def main():
     sp = stream.cuda_stream
     tuned = None
     if args.kernel == "auto" and args.arenas_per_block == 0 and not args.profile_probe:
-        # kernel and launch shape from the autotuner (times each on a copy of the state)
+        # kernel and launch shape from the autotuner (times each on a copy of
+        # the state, after the same pre-roll as the timed run)
+        for n in [args.chunk] * (args.preroll // args.chunk):
+            batch.step(n, sp)
         k, pb, ms = batch.autotune(8, sp)
         tuned = {"kernel": k, "arenas_per_block": pb, "ms_per_tick": ms}
+        batch.close()
+        batch, _, _, _ = make_batch(args.workload, n_arenas, args.seed)
+        batch.set_kernel(k)
+        batch.set_launch(pb)
     else:
         batch.set_kernel({"auto": 0, "scratch": 1, "lds": 2, "coop": 3}[args.kernel])
         batch.set_launch(args.arenas_per_block)
@@ -300,6 +351,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+        for n in [args.chunk] * (args.preroll // args.chunk):
+            batch.step(n, sp)
+        torch.cuda.synchronize()
         for _ in range(2):
             batch.step(args.chunk, sp)
             torch.cuda.synchronize()
@@ -307,20 +361,22 @@ def main():
         print("profile probe done", flush=True)
         return
 
-    wall, ev_ms, chunk, graph = run_timed(batch, args.steps, args.warmup, args.chunk, not args.no_graph, stream)
+    wall, ev_ms, chunk, graph = run_timed(batch, args.steps, args.warmup, args.preroll, args.chunk,
+                                          not args.no_graph, stream)
     arena_ticks = n_arenas * args.steps
     value = arena_ticks / wall
     kernel_ms = ev_ms / args.steps  # per tick (one launch covers `chunk` ticks)
     achieved = (n_arenas * bytes_per_at) / (kernel_ms * 1e-3) / 1e9
     goals = batch.get_goals()
+    status_flagged = sum(1 for x in batch.get_status() if x)
     batch.close()
 
     plain = None
-    if args.compare_plain:
+    if not args.no_plain:
         pb_batch, _, _, _ = make_batch(args.workload, n_arenas, args.seed)
         pb_batch.set_kernel(1)
         pb_batch.set_launch(64)
-        p_wall, p_ev, _, _ = run_timed(pb_batch, args.steps, args.warmup, 1, False, stream)
+        p_wall, p_ev, _, _ = run_timed(pb_batch, args.steps, args.warmup, args.preroll, 1, False, stream)
         pb_batch.close()
         plain = {"kernel": "rsg_step_arena_kernel", "arenas_per_block": 64, "ticks_per_launch": 1,
                  "value": arena_ticks / p_wall, "ms_per_tick": p_ev / args.steps,
@@ -346,18 +402,21 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc, "arenas": n_arenas, "cars_per_arena": cars_per_arena, "tick_rate": 120,
-                   "arenas_per_block": per_block, "ticks_per_launch": chunk, "kernel": kernel_name,
-                   "autotuned": tuned is not None, "hip_graph": graph,
-                   "goals_during_run": int(sum(goals[0]) + sum(goals[1]))},
+                   "preroll_ticks": args.preroll, "arenas_per_block": per_block, "ticks_per_launch": chunk,
+                   "kernel": kernel_name, "autotuned": tuned is not None, "hip_graph": graph,
+                   "goals_during_run": int(sum(goals[0]) + sum(goals[1])),
+                   "arenas_with_status_bits": status_flagged},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": HBM_TRAFFIC_PER_LAUNCH.get((kernel_name, n_arenas, chunk, per_block))
                      if args.workload == "soccar2v2" else None,
                      "traffic_unit": "B per launch",
-                     "kernel": kernel_name, "kernel_ms": kernel_ms,
+                     "kernel": kernel_name, "kernel_ms_per_tick": kernel_ms,
                      "bytes_per_arena_tick": bytes_per_at},
         "cpu_baseline": cpu,
     }
+    if cpu:
+        line["vs_cpu"] = value / cpu["value"]
     if plain is not None:
         line["plain_gpu"] = plain
     if adjoint is not None:

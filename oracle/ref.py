@@ -80,7 +80,7 @@ def lib():
         L.rsref_tracker_pop.argtypes = [C.c_void_p, C.POINTER(Event), C.c_int]
         L.rsref_run_workload.restype = C.c_double
         L.rsref_run_workload.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                         C.c_uint64, C.POINTER(C.c_uint64)]
+                                         C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -302,9 +302,11 @@ class RefGameEventTracker:
             pass
 
 
-def run_workload(threads, arenas_per_thread, n_blue, n_orange, preset, ticks, seed):
-    """Timed CPU baseline: returns (arena_ticks, wall_seconds)."""
+def run_workload(threads, arenas_per_thread, n_blue, n_orange, preset, ticks, seed, cpus=None):
+    """Timed CPU baseline: returns (arena_ticks, wall_seconds).  `cpus` pins
+    thread t to CPU cpus[t]."""
     L = lib()
     n = C.c_uint64(0)
-    secs = L.rsref_run_workload(threads, arenas_per_thread, n_blue, n_orange, preset, ticks, seed, C.byref(n))
+    pin = (C.c_int * threads)(*cpus) if cpus else None
+    secs = L.rsref_run_workload(threads, arenas_per_thread, n_blue, n_orange, preset, ticks, seed, C.byref(n), pin)
     return n.value, secs

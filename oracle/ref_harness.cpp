@@ -31,10 +31,13 @@
 #include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btVoronoiSimplexSolver.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btGjkEpaPenetrationDepthSolver.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/NarrowPhaseCollision/btSubSimplexConvexCast.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/CollisionDispatch/btBoxBoxDetector.h"
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <pthread.h>
+#include <sched.h>
 #include <thread>
 #include <vector>
 
@@ -374,9 +377,10 @@ void rsref_pad_config(void* h, int i, float* pos3, int* is_big) {
 // `arenas_per_thread` arenas of (n_blue, n_orange) cars of `preset`, stepped
 // `ticks` ticks one tick at a time with the rsg_workload.h control process and
 // a kickoff reset whenever a goal is scored.  Returns wall seconds; writes the
-// total number of arena-ticks to *arena_ticks.
+// total number of arena-ticks to *arena_ticks.  With `cpus` (one entry per
+// thread), thread t is pinned to CPU cpus[t].
 double rsref_run_workload(int threads, int arenas_per_thread, int n_blue, int n_orange,
-                          int preset, int ticks, uint64_t seed, uint64_t* arena_ticks) {
+                          int preset, int ticks, uint64_t seed, uint64_t* arena_ticks, const int* cpus) {
     std::atomic<uint64_t> total{0};
     std::atomic<int> ready{0};
     std::atomic<bool> go{false};
@@ -384,6 +388,12 @@ double rsref_run_workload(int threads, int arenas_per_thread, int n_blue, int n_
     std::vector<std::chrono::steady_clock::time_point> finish(threads);
     for (int t = 0; t < threads; t++) {
         pool.emplace_back([&, t]() {
+            if (cpus) {
+                cpu_set_t set;
+                CPU_ZERO(&set);
+                CPU_SET(cpus[t], &set);
+                pthread_setaffinity_np(pthread_self(), sizeof(set), &set);
+            }
             std::vector<RefArena> arenas(arenas_per_thread);
             std::vector<std::vector<rsg_ctrl_gen>> gens(arenas_per_thread);
             for (int a = 0; a < arenas_per_thread; a++) {
@@ -701,6 +711,40 @@ int rsref_ray_convex(int kind, float radius, const float* box_half, const float*
     *frac = res.m_fraction;
     for (int k = 0; k < 3; k++) normal[k] = res.m_normal[k];
     return 1;
+}
+
+// The reference's box-box detector (btBoxBoxDetector::getClosestPoints, i.e.
+// dBoxBox2) on two boxes of full half extents h0, h1 at transforms x0, x1
+// (basis rows then origin): every reported contact, in report order, as
+// normal on box 1, point on box 1 and signed distance.  Returns the count.
+namespace {
+struct MultiResult : public btDiscreteCollisionDetectorInterface::Result {
+    int n = 0;
+    float* normals; float* points; float* depths; int cap;
+    void setShapeIdentifiersA(int, int) override {}
+    void setShapeIdentifiersB(int, int) override {}
+    void addContactPoint(const btVector3& normal, const btVector3& point, btScalar depth) override {
+        if (n < cap) {
+            for (int k = 0; k < 3; k++) { normals[3 * n + k] = normal[k]; points[3 * n + k] = point[k]; }
+            depths[n] = depth;
+        }
+        n++;
+    }
+};
+}
+
+int rsref_box_box(const float* h0, const float* x0, const float* h1, const float* x1, float* normals,
+                  float* points, float* depths, int cap) {
+    btBoxShape b0(btVector3(h0[0], h0[1], h0[2]));
+    btBoxShape b1(btVector3(h1[0], h1[1], h1[2]));
+    btBoxBoxDetector det(&b0, &b1);
+    btDiscreteCollisionDetectorInterface::ClosestPointInput in;
+    in.m_transformA = xf_from_floats(x0);
+    in.m_transformB = xf_from_floats(x1);
+    MultiResult r;
+    r.normals = normals; r.points = points; r.depths = depths; r.cap = cap;
+    det.getClosestPoints(in, r, false);
+    return r.n;
 }
 
 // Box margin and implicit half extents the btBoxShape constructor derives.

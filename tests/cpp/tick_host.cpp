@@ -165,3 +165,20 @@ void th_step(void* p, int ticks) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// rsqrtss of eight fixed inputs on this CPU, the part of the reference's
+// normalisation whose last bits differ between x86 implementations.
+void th_rsqrt_signature(float* out) {
+    const float in[8] = {1.f, 2.f, 3.f, 0.5f, 1.2345f, 7.654321f, 1e-3f, 123.456f};
+    for (int i = 0; i < 8; i++) {
+#ifdef RSG_SSE_NORMALIZE
+        out[i] = _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(in[i])));
+#else
+        out[i] = 0.f;
+#endif
+    }
+}
+
+}  // extern "C"

@@ -145,6 +145,7 @@ def test_c1_sampled_arenas_teacher_forced():
             b.step(1)
             t += 1
             ctrls = b.get_controls()  # the controls the GPU drew and applied in that tick
+            print(f"\n  tick {t}: setting 256 sampled arenas into the oracle", flush=True)
             for a in rng.sample(range(n), 256):
                 samples.append(parity.sample_from_state(
                     ref, MODE_SOCCAR, [0, 0, 1, 1], [0, 0, 0, 0], list(cars[4 * a:4 * a + 4]), balls[a],

@@ -54,7 +54,23 @@ def load(path):
     L.th_set_car_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     L.th_get_status.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
     L.th_step.argtypes = [C.c_void_p, C.c_int]
+    L.th_rsqrt_signature.argtypes = [C.POINTER(C.c_float)]
     return L
+
+
+# rsqrtss of th_rsqrt_signature's inputs on the CPU that recorded the
+# committed fixtures (Intel Xeon @ 2.70GHz), as float bit patterns.
+RECORDING_CPU_RSQRT = [0x3f7ff000, 0x3f34f800, 0x3f13c800, 0x3fb4f800, 0x3f666000, 0x3eb91000, 0x41fd0000,
+                       0x3db85000]
+
+
+def same_rsqrt_as_recording_cpu():
+    """Whether this CPU's rsqrtss agrees with the fixtures' recording CPU's
+    on the signature inputs."""
+    import struct
+    out = (C.c_float * 8)()
+    get("sse").th_rsqrt_signature(out)
+    return [struct.unpack("<I", struct.pack("<f", x))[0] for x in out] == RECORDING_CPU_RSQRT
 
 
 def run_samples(L, samples, mode, cmf=None, inject=None):
