@@ -1,11 +1,21 @@
-// rsg_boxbox.h -- box vs box contacts for car hitboxes: the separating-axis
-// box-box test with face clipping and contact culling that the reference uses
-// for car-car pairs (btBoxBoxDetector / dBoxBox2, originally from the Open
-// Dynamics Engine; BulletCollision/CollisionDispatch/btBoxBoxDetector.cpp:
-// 27-767).  Fifteen axes (3 + 3 face normals, 9 edge crosses with a 5%
-// preference for face axes), then either one edge-edge point or the incident
-// face clipped against the reference face rectangle, at most four points
-// chosen by angle around their centroid.
+// rsg_boxbox.h -- contacts between two oriented boxes (car hitboxes), giving
+// the same points, normal and depths as the reference's car-car narrowphase
+// (btBoxBoxDetector, BulletCollision/CollisionDispatch/btBoxBoxDetector.cpp:
+// 272-767, itself ODE's dBoxBox).  Method (Gottschalk's separating-axis test
+// for OBBs, then Sutherland-Hodgman clipping):
+//   1. Fifteen candidate axes: the three face normals of each box and the nine
+//      edge-edge cross products.  The axis of least penetration wins; an
+//      edge axis must beat a face axis by 5 % to be chosen.  Any separating
+//      axis ends the test.
+//   2. An edge-edge axis gives one point: the closest approach of the two
+//      edges.  A face axis makes that face the reference face: the most
+//      anti-parallel face of the other box (the incident face) is clipped
+//      against the reference face's rectangle, and the clipped points below
+//      the reference face are kept; more than four are reduced to four
+//      spread by angle around the polygon's centroid, starting with the
+//      deepest.
+// Every sum, product and comparison is evaluated in the reference's order so
+// that the points come out bit for bit (tests/test_components.py).
 #pragma once
 
 #include "rsg_contact.h"
@@ -19,317 +29,297 @@ struct BoxBoxContacts {
     float depth[4];   // signed distance (negative = penetration)
 };
 
-// Column k of a basis (the reference reads its 4-stride matrices by column).
-RSG_FN V3 bcol(const M3& m, int k) { return v3(get(m.r[0], k), get(m.r[1], k), get(m.r[2], k)); }
+// One box: centre, the three axes (columns of its basis) and half extents.
+struct BoxFrame {
+    V3 c;
+    V3 ax[3];
+    float h[3];
+};
 
-// dLineClosestApproach
-RSG_FN void line_closest_approach(V3 pa, V3 ua, V3 pb, V3 ub, float& alpha, float& beta) {
-    V3 p = pb - pa;
-    float uaub = dot(ua, ub);
-    float q1 = dot(ua, p);
-    float q2 = -dot(ub, p);
-    float d = 1 - uaub * uaub;
-    if (d <= 0.0001f) {
-        alpha = 0;
-        beta = 0;
-    } else {
-        d = 1.f / d;
-        alpha = (q1 + uaub * q2) * d;
-        beta = (uaub * q1 + q2) * d;
-    }
+RSG_FN BoxFrame box_frame(const Xf& x, V3 half) {
+    BoxFrame b;
+    b.c = x.t;
+    for (int k = 0; k < 3; k++) b.ax[k] = v3(get(x.R.r[0], k), get(x.R.r[1], k), get(x.R.r[2], k));
+    b.h[0] = half.x; b.h[1] = half.y; b.h[2] = half.z;
+    return b;
 }
 
-// intersectRectQuad2: clip quad q (4 2D points) against the rectangle
-// [-h0,h0] x [-h1,h1]; returns the polygon in ret (up to 8 points).
-RSG_FN int intersect_rect_quad(const float h[2], const float p[8], float ret[16]) {
-    int nq = 4, nr = 0;
-    float buffer[16];
-    float qbuf[16];
-    for (int i = 0; i < 8; i++) qbuf[i] = p[i];
-    // q / r alternate between (qbuf|ret|buffer); track them as pointers
-    float* q = qbuf;
-    float* r = ret;
-    for (int dir = 0; dir <= 1; dir++) {
-        for (int sign = -1; sign <= 1; sign += 2) {
-            float* pq = q;
-            float* pr = r;
-            nr = 0;
-            for (int i = nq; i > 0; i--) {
-                if (sign * pq[dir] < h[dir]) {
-                    pr[0] = pq[0];
-                    pr[1] = pq[1];
-                    pr += 2;
-                    nr++;
-                    if (nr & 8) { q = r; goto done; }
+// Parameters of the points of closest approach of the lines pa + s ua and
+// pb + t ub (unit directions); both 0 for near-parallel lines.
+RSG_FN void closest_line_params(V3 pa, V3 ua, V3 pb, V3 ub, float& s, float& t) {
+    V3 d = pb - pa;
+    float cos_ab = dot(ua, ub);
+    float da = dot(ua, d);
+    float db = -dot(ub, d);
+    float den = 1 - cos_ab * cos_ab;
+    if (den <= 0.0001f) {
+        s = 0;
+        t = 0;
+        return;
+    }
+    den = 1.f / den;
+    s = (da + cos_ab * db) * den;
+    t = (cos_ab * da + db) * den;
+}
+
+// Sutherland-Hodgman: clips the polygon `poly` (n 2D points, x y pairs)
+// against the rectangle |x| < half[0], |y| < half[1], one half-plane at a
+// time (x >= -hx, x <= hx, y >= -hy, y <= hy).  At most 8 points are kept: a
+// clip step that would emit a ninth stops the whole clip with its first 8.
+// Returns the point count; the polygon is left in `poly`.
+RSG_FN int clip_to_rect(const float half[2], float poly[16], int n) {
+    float next[16];
+    for (int axis = 0; axis < 2; axis++) {
+        for (int side = -1; side <= 1; side += 2) {
+            const float bound = half[axis];
+            int m = 0;
+            bool full = false;
+            for (int i = 0; i < n && !full; i++) {
+                const float* a = poly + 2 * i;
+                const float* b = poly + 2 * (i + 1 < n ? i + 1 : 0);
+                const bool a_in = side * a[axis] < bound;
+                const bool b_in = side * b[axis] < bound;
+                if (a_in) {
+                    next[2 * m] = a[0];
+                    next[2 * m + 1] = a[1];
+                    full = ++m == 8;
                 }
-                float* nextq = (i > 1) ? pq + 2 : q;
-                if ((sign * pq[dir] < h[dir]) ^ (sign * nextq[dir] < h[dir])) {
-                    pr[1 - dir] = pq[1 - dir] + (nextq[1 - dir] - pq[1 - dir]) / (nextq[dir] - pq[dir]) * (sign * h[dir] - pq[dir]);
-                    pr[dir] = sign * h[dir];
-                    pr += 2;
-                    nr++;
-                    if (nr & 8) { q = r; goto done; }
+                if (!full && a_in != b_in) {
+                    const int o = 1 - axis;
+                    next[2 * m + o] = a[o] + (b[o] - a[o]) / (b[axis] - a[axis]) * (side * bound - a[axis]);
+                    next[2 * m + axis] = side * bound;
+                    full = ++m == 8;
                 }
-                pq += 2;
             }
-            q = r;
-            r = (q == ret) ? buffer : ret;
-            nq = nr;
+            for (int k = 0; k < 2 * m; k++) poly[k] = next[k];
+            n = m;
+            if (full) return n;
         }
     }
-done:
-    if (q != ret)
-        for (int i = 0; i < nr * 2; i++) ret[i] = q[i];
-    return nr;
+    return n;
 }
 
-// cullPoints2: choose m of the n polygon points, starting with i0, spread by
-// angle around the polygon's centroid.
-RSG_FN void cull_points(int n, const float p[], int m, int i0, int iret[]) {
-    const float PI_ODE = 3.14159265f;
-    float a, cx, cy, q;
+// Chooses `keep` of the n polygon points: `first`, then for each of the keep-1
+// directions spaced evenly in angle from it around the polygon's centroid,
+// the unused point whose direction is nearest (first one on ties).
+RSG_FN void spread_by_angle(int n, const float pts[16], int keep, int first, int out[4]) {
+    const float pi = 3.14159265f;
+    float cx, cy;
     if (n == 1) {
-        cx = p[0];
-        cy = p[1];
+        cx = pts[0];
+        cy = pts[1];
     } else if (n == 2) {
-        cx = 0.5f * (p[0] + p[2]);
-        cy = 0.5f * (p[1] + p[3]);
+        cx = 0.5f * (pts[0] + pts[2]);
+        cy = 0.5f * (pts[1] + pts[3]);
     } else {
-        a = 0; cx = 0; cy = 0;
-        for (int i = 0; i < n - 1; i++) {
-            q = p[i * 2] * p[i * 2 + 3] - p[i * 2 + 2] * p[i * 2 + 1];
-            a += q;
-            cx += q * (p[i * 2] + p[i * 2 + 2]);
-            cy += q * (p[i * 2 + 1] + p[i * 2 + 3]);
+        // area-weighted centroid (shoelace), closing edge last
+        float area2 = 0, sx = 0, sy = 0, w = 0;
+        for (int i = 0; i < n; i++) {
+            const int j = i + 1 < n ? i + 1 : 0;
+            w = pts[2 * i] * pts[2 * j + 1] - pts[2 * j] * pts[2 * i + 1];
+            if (j == 0) break;
+            area2 += w;
+            sx += w * (pts[2 * i] + pts[2 * j]);
+            sy += w * (pts[2 * i + 1] + pts[2 * j + 1]);
         }
-        q = p[n * 2 - 2] * p[1] - p[0] * p[n * 2 - 1];
-        if (fabsf(a + q) > FLT_EPS) a = 1.f / (3.0f * (a + q));
-        else a = 1e18f;
-        cx = a * (cx + q * (p[n * 2 - 2] + p[0]));
-        cy = a * (cy + q * (p[n * 2 - 1] + p[1]));
+        float inv = fabsf(area2 + w) > FLT_EPS ? 1.f / (3.0f * (area2 + w)) : 1e18f;
+        cx = inv * (sx + w * (pts[2 * n - 2] + pts[0]));
+        cy = inv * (sy + w * (pts[2 * n - 1] + pts[1]));
     }
     float ang[8];
-    for (int i = 0; i < n; i++) ang[i] = atan2f(p[i * 2 + 1] - cy, p[i * 2] - cx);
-    int avail[8];
-    for (int i = 0; i < n; i++) avail[i] = 1;
-    avail[i0] = 0;
-    iret[0] = i0;
-    int k = 1;
-    for (int j = 1; j < m; j++) {
-        a = (float)j * (2 * PI_ODE / m) + ang[i0];
-        if (a > PI_ODE) a -= 2 * PI_ODE;
-        float maxdiff = 1e9f, diff;
-        iret[k] = i0;
+    bool used[8];
+    for (int i = 0; i < n; i++) {
+        ang[i] = atan2f(pts[2 * i + 1] - cy, pts[2 * i] - cx);
+        used[i] = false;
+    }
+    used[first] = true;
+    out[0] = first;
+    for (int k = 1; k < keep; k++) {
+        float target = (float)k * (2 * pi / keep) + ang[first];
+        if (target > pi) target -= 2 * pi;
+        float best = 1e9f;
+        int pick = first;
         for (int i = 0; i < n; i++) {
-            if (avail[i]) {
-                diff = fabsf(ang[i] - a);
-                if (diff > PI_ODE) diff = 2 * PI_ODE - diff;
-                if (diff < maxdiff) { maxdiff = diff; iret[k] = i; }
+            if (used[i]) continue;
+            float d = fabsf(ang[i] - target);
+            if (d > pi) d = 2 * pi - d;
+            if (d < best) {
+                best = d;
+                pick = i;
             }
         }
-        avail[iret[k]] = 0;
-        k++;
+        used[pick] = true;
+        out[k] = pick;
     }
 }
 
-// Box 0 (centre x0.t, basis x0.R, half extents h0) against box 1.  Contacts
-// are reported as the reference reports them to the manifold: normal on
-// box 1, point on box 1, negative depth.
+// Box 0 (transform x0, half extents h0) against box 1.  Contacts are
+// reported as the manifold receives them: normal on box 1, point on box 1,
+// signed distance.
 RSG_FN void box_box(const Xf& x0, V3 h0, const Xf& x1, V3 h1, BoxBoxContacts& out) {
     out.n = 0;
-    const float fudge_factor = 1.05f;
-    const M3& R1 = x0.R;
-    const M3& R2 = x1.R;
-    V3 p1 = x0.t, p2 = x1.t;
-    V3 p = p2 - p1;
-    V3 pp = v3(dot(bcol(R1, 0), p), dot(bcol(R1, 1), p), dot(bcol(R1, 2), p));
-    float A[3] = {h0.x, h0.y, h0.z};
-    float B[3] = {h1.x, h1.y, h1.z};
-    float R[3][3], Q[3][3];
+    const BoxFrame b0 = box_frame(x0, h0), b1 = box_frame(x1, h1);
+    const V3 d = b1.c - b0.c;
+    // centre offset in box 0's frame; axis cosines and their magnitudes
+    float off[3], cs[3][3], ac[3][3];
+    for (int i = 0; i < 3; i++) off[i] = dot(b0.ax[i], d);
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) {
-            R[i][j] = dot(bcol(R1, i), bcol(R2, j));
-            Q[i][j] = fabsf(R[i][j]);
+            cs[i][j] = dot(b0.ax[i], b1.ax[j]);
+            ac[i][j] = fabsf(cs[i][j]);
         }
-    float s = -3.402823466e+38f, s2, l;
-    int norm_col = -1;   // >= 0: face normal = column of R1 (0..2) or R2 (3..5)
-    V3 normalC = v3zero();
-    bool invert_normal = false;
-    int code = 0;
+    const float* A = b0.h;
+    const float* B = b1.h;
 
-    // face axes
-    float e1v[6], e2v[6];
-    e1v[0] = pp.x; e2v[0] = A[0] + B[0] * Q[0][0] + B[1] * Q[0][1] + B[2] * Q[0][2];
-    e1v[1] = pp.y; e2v[1] = A[1] + B[0] * Q[1][0] + B[1] * Q[1][1] + B[2] * Q[1][2];
-    e1v[2] = pp.z; e2v[2] = A[2] + B[0] * Q[2][0] + B[1] * Q[2][1] + B[2] * Q[2][2];
-    e1v[3] = dot(bcol(R2, 0), p); e2v[3] = A[0] * Q[0][0] + A[1] * Q[1][0] + A[2] * Q[2][0] + B[0];
-    e1v[4] = dot(bcol(R2, 1), p); e2v[4] = A[0] * Q[0][1] + A[1] * Q[1][1] + A[2] * Q[2][1] + B[1];
-    e1v[5] = dot(bcol(R2, 2), p); e2v[5] = A[0] * Q[0][2] + A[1] * Q[1][2] + A[2] * Q[2][2] + B[2];
-    for (int k = 0; k < 6; k++) {
-        s2 = fabsf(e1v[k]) - e2v[k];
-        if (s2 > 0) return;
-        if (s2 > s) {
-            s = s2;
-            norm_col = k;
-            invert_normal = e1v[k] < 0;
-            code = k + 1;
-        }
+    // 1. separating-axis search.  `best` is the largest (least negative)
+    // separation, `axis_id` 1-3 box 0 faces, 4-6 box 1 faces, 7-15 edge pairs.
+    float best = -3.402823466e+38f;
+    int axis_id = 0;
+    bool flip = false;
+    V3 edge_n = v3zero();  // chosen edge axis in box 0's frame (unit)
+    for (int k = 0; k < 3; k++) {
+        const float dist = off[k];
+        const float reach = A[k] + B[0] * ac[k][0] + B[1] * ac[k][1] + B[2] * ac[k][2];
+        const float sep = fabsf(dist) - reach;
+        if (sep > 0) return;
+        if (sep > best) { best = sep; axis_id = 1 + k; flip = dist < 0; }
     }
-    // edge x edge axes
-    const float fudge2 = 1.0e-5f;
+    for (int k = 0; k < 3; k++) {
+        const float dist = dot(b1.ax[k], d);
+        const float reach = A[0] * ac[0][k] + A[1] * ac[1][k] + A[2] * ac[2][k] + B[k];
+        const float sep = fabsf(dist) - reach;
+        if (sep > 0) return;
+        if (sep > best) { best = sep; axis_id = 4 + k; flip = dist < 0; }
+    }
+    // edge axes: box 0 axis i x box 1 axis j, measured in box 0's frame; the
+    // cosine magnitudes get a small floor against near-parallel edges
     for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) Q[i][j] += fudge2;
-    float R11 = R[0][0], R12 = R[0][1], R13 = R[0][2], R21 = R[1][0], R22 = R[1][1], R23 = R[1][2],
-          R31 = R[2][0], R32 = R[2][1], R33 = R[2][2];
-    float Q11 = Q[0][0], Q12 = Q[0][1], Q13 = Q[0][2], Q21 = Q[1][0], Q22 = Q[1][1], Q23 = Q[1][2],
-          Q31 = Q[2][0], Q32 = Q[2][1], Q33 = Q[2][2];
-    struct EdgeAxis { float e1, e2, n1, n2, n3; };
-    EdgeAxis ax[9] = {
-        {pp.z * R21 - pp.y * R31, (A[1] * Q31 + A[2] * Q21 + B[1] * Q13 + B[2] * Q12), 0, -R31, R21},
-        {pp.z * R22 - pp.y * R32, (A[1] * Q32 + A[2] * Q22 + B[0] * Q13 + B[2] * Q11), 0, -R32, R22},
-        {pp.z * R23 - pp.y * R33, (A[1] * Q33 + A[2] * Q23 + B[0] * Q12 + B[1] * Q11), 0, -R33, R23},
-        {pp.x * R31 - pp.z * R11, (A[0] * Q31 + A[2] * Q11 + B[1] * Q23 + B[2] * Q22), R31, 0, -R11},
-        {pp.x * R32 - pp.z * R12, (A[0] * Q32 + A[2] * Q12 + B[0] * Q23 + B[2] * Q21), R32, 0, -R12},
-        {pp.x * R33 - pp.z * R13, (A[0] * Q33 + A[2] * Q13 + B[0] * Q22 + B[1] * Q21), R33, 0, -R13},
-        {pp.y * R11 - pp.x * R21, (A[0] * Q21 + A[1] * Q11 + B[1] * Q33 + B[2] * Q32), -R21, R11, 0},
-        {pp.y * R12 - pp.x * R22, (A[0] * Q22 + A[1] * Q12 + B[0] * Q33 + B[2] * Q31), -R22, R12, 0},
-        {pp.y * R13 - pp.x * R23, (A[0] * Q23 + A[1] * Q13 + B[0] * Q32 + B[1] * Q31), -R23, R13, 0},
-    };
-    for (int k = 0; k < 9; k++) {
-        const EdgeAxis& e = ax[k];
-        s2 = fabsf(e.e1) - e.e2;
-        if (s2 > FLT_EPS) return;
-        l = sqrtf(e.n1 * e.n1 + e.n2 * e.n2 + e.n3 * e.n3);
-        if (l > FLT_EPS) {
-            s2 /= l;
-            if (s2 * fudge_factor > s) {
-                s = s2;
-                norm_col = -1;
-                normalC = v3(e.n1 / l, e.n2 / l, e.n3 / l);
-                invert_normal = e.e1 < 0;
-                code = 7 + k;
+        for (int j = 0; j < 3; j++) ac[i][j] += 1.0e-5f;
+    for (int i = 0; i < 3; i++) {
+        const int u = (i + 1) % 3, w = (i + 2) % 3;         // cyclic successors
+        const int lo = u < w ? u : w, hi = u < w ? w : u;   // the other two, ascending
+        for (int j = 0; j < 3; j++) {
+            const int jlo = j == 0 ? 1 : 0, jhi = j == 2 ? 1 : 2;
+            const float dist = off[w] * cs[u][j] - off[u] * cs[w][j];
+            const float reach = A[lo] * ac[hi][j] + A[hi] * ac[lo][j] + B[jlo] * ac[i][jhi] + B[jhi] * ac[i][jlo];
+            float n3[3];
+            n3[i] = 0;
+            n3[u] = -cs[w][j];
+            n3[w] = cs[u][j];
+            float sep = fabsf(dist) - reach;
+            if (sep > FLT_EPS) return;
+            const float len = sqrtf(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
+            if (len > FLT_EPS) {
+                sep /= len;
+                if (sep * 1.05f > best) {
+                    best = sep;
+                    axis_id = 7 + 3 * i + j;
+                    edge_n = v3(n3[0] / len, n3[1] / len, n3[2] / len);
+                    flip = dist < 0;
+                }
             }
         }
     }
-    if (!code) return;
-    V3 normal;
-    if (norm_col >= 0) normal = norm_col < 3 ? bcol(R1, norm_col) : bcol(R2, norm_col - 3);
-    else normal = R1 * normalC;
-    if (invert_normal) normal = -normal;
-    float depth = -s;
+    if (!axis_id) return;
+    V3 normal;  // world normal pointing from box 0 to box 1
+    if (axis_id <= 3) normal = b0.ax[axis_id - 1];
+    else if (axis_id <= 6) normal = b1.ax[axis_id - 4];
+    else normal = x0.R * edge_n;  // back from box 0's frame
+    if (flip) normal = -normal;
+    const float pen = -best;
+    out.normal = -normal;
 
-    if (code > 6) {
-        // edge-edge
-        V3 pa = p1;
-        for (int j = 0; j < 3; j++) {
-            float sign = dot(normal, bcol(R1, j)) > 0 ? 1.0f : -1.0f;
-            pa += sign * A[j] * bcol(R1, j);
-        }
-        V3 pb = p2;
-        for (int j = 0; j < 3; j++) {
-            float sign = dot(normal, bcol(R2, j)) > 0 ? -1.0f : 1.0f;
-            pb += sign * B[j] * bcol(R2, j);
-        }
-        V3 ua = bcol(R1, (code - 7) / 3);
-        V3 ub = bcol(R2, (code - 7) % 3);
-        float alpha, beta;
-        line_closest_approach(pa, ua, pb, ub, alpha, beta);
-        pb += ub * beta;
-        out.normal = -normal;
+    // 2a. edge-edge: closest approach of the two edges nearest each other
+    if (axis_id > 6) {
+        V3 ea = b0.c, eb = b1.c;
+        for (int k = 0; k < 3; k++) ea += (dot(normal, b0.ax[k]) > 0 ? 1.0f : -1.0f) * A[k] * b0.ax[k];
+        for (int k = 0; k < 3; k++) eb += (dot(normal, b1.ax[k]) > 0 ? -1.0f : 1.0f) * B[k] * b1.ax[k];
+        const V3 ua = b0.ax[(axis_id - 7) / 3], ub = b1.ax[(axis_id - 7) % 3];
+        float s, t;
+        closest_line_params(ea, ua, eb, ub, s, t);
         out.n = 1;
-        out.point[0] = pb;
-        out.depth[0] = -depth;
+        out.point[0] = eb + ub * t;
+        out.depth[0] = -pen;
         return;
     }
 
-    // face-something: box a holds the reference face
-    const M3& Ra = code <= 3 ? R1 : R2;
-    const M3& Rb = code <= 3 ? R2 : R1;
-    V3 pa = code <= 3 ? p1 : p2;
-    V3 pb = code <= 3 ? p2 : p1;
-    const float* Sa = code <= 3 ? A : B;
-    const float* Sb = code <= 3 ? B : A;
-    V3 normal2 = code <= 3 ? normal : -normal;
-    V3 nr = v3(dot(bcol(Rb, 0), normal2), dot(bcol(Rb, 1), normal2), dot(bcol(Rb, 2), normal2));
-    V3 anr = v3(fabsf(nr.x), fabsf(nr.y), fabsf(nr.z));
-    int lanr, a1, a2;
-    if (anr.y > anr.x) {
-        if (anr.y > anr.z) { a1 = 0; lanr = 1; a2 = 2; }
-        else { a1 = 0; a2 = 1; lanr = 2; }
-    } else {
-        if (anr.x > anr.z) { lanr = 0; a1 = 1; a2 = 2; }
-        else { a1 = 0; a2 = 1; lanr = 2; }
+    // 2b. face contact: `ref` holds the reference face, `inc` the incident one
+    const bool ref_is_0 = axis_id <= 3;
+    const BoxFrame& ref = ref_is_0 ? b0 : b1;
+    const BoxFrame& inc = ref_is_0 ? b1 : b0;
+    const V3 ref_n = ref_is_0 ? normal : -normal;  // reference face normal, towards the incident box
+    const int face = ref_is_0 ? axis_id - 1 : axis_id - 4;
+    const int fu = face == 0 ? 1 : 0, fv = face == 2 ? 1 : 2;  // the face's in-plane axes
+    // the incident face: the incident box's axis most parallel to ref_n
+    float cn[3], acn[3];
+    for (int k = 0; k < 3; k++) {
+        cn[k] = dot(inc.ax[k], ref_n);
+        acn[k] = fabsf(cn[k]);
     }
-    V3 center;
-    if (get(nr, lanr) < 0) center = (pb - pa) + Sb[lanr] * bcol(Rb, lanr);
-    else center = (pb - pa) - Sb[lanr] * bcol(Rb, lanr);
-    int codeN = code <= 3 ? code - 1 : code - 4;
-    int code1, code2;
-    if (codeN == 0) { code1 = 1; code2 = 2; }
-    else if (codeN == 1) { code1 = 0; code2 = 2; }
-    else { code1 = 0; code2 = 1; }
-    float c1 = dot(center, bcol(Ra, code1));
-    float c2 = dot(center, bcol(Ra, code2));
-    float m11 = dot(bcol(Ra, code1), bcol(Rb, a1));
-    float m12 = dot(bcol(Ra, code1), bcol(Rb, a2));
-    float m21 = dot(bcol(Ra, code2), bcol(Rb, a1));
-    float m22 = dot(bcol(Ra, code2), bcol(Rb, a2));
-    float quad[8];
+    int up, ia, ib;  // incident face axis, then its in-plane axes
+    if (acn[1] > acn[0]) {
+        if (acn[1] > acn[2]) { up = 1; ia = 0; ib = 2; }
+        else { up = 2; ia = 0; ib = 1; }
+    } else {
+        if (acn[0] > acn[2]) { up = 0; ia = 1; ib = 2; }
+        else { up = 2; ia = 0; ib = 1; }
+    }
+    const V3 rel = inc.c - ref.c;
+    const V3 face_c = cn[up] < 0 ? rel + inc.h[up] * inc.ax[up] : rel - inc.h[up] * inc.ax[up];
+    // incident face corners in the reference face's 2D coordinates
+    const float fc_u = dot(face_c, ref.ax[fu]), fc_v = dot(face_c, ref.ax[fv]);
+    float m_ua = dot(ref.ax[fu], inc.ax[ia]), m_ub = dot(ref.ax[fu], inc.ax[ib]);
+    float m_va = dot(ref.ax[fv], inc.ax[ia]), m_vb = dot(ref.ax[fv], inc.ax[ib]);
+    float poly[16];
     {
-        float k1 = m11 * Sb[a1], k2 = m21 * Sb[a1], k3 = m12 * Sb[a2], k4 = m22 * Sb[a2];
-        quad[0] = c1 - k1 - k3; quad[1] = c2 - k2 - k4;
-        quad[2] = c1 - k1 + k3; quad[3] = c2 - k2 + k4;
-        quad[4] = c1 + k1 + k3; quad[5] = c2 + k2 + k4;
-        quad[6] = c1 + k1 - k3; quad[7] = c2 + k2 - k4;
+        const float ua = m_ua * inc.h[ia], va = m_va * inc.h[ia], ub = m_ub * inc.h[ib], vb = m_vb * inc.h[ib];
+        poly[0] = fc_u - ua - ub; poly[1] = fc_v - va - vb;
+        poly[2] = fc_u - ua + ub; poly[3] = fc_v - va + vb;
+        poly[4] = fc_u + ua + ub; poly[5] = fc_v + va + vb;
+        poly[6] = fc_u + ua - ub; poly[7] = fc_v + va - vb;
     }
-    float rect[2] = {Sa[code1], Sa[code2]};
-    float ret[16];
-    int n = intersect_rect_quad(rect, quad, ret);
+    const float rect[2] = {ref.h[fu], ref.h[fv]};
+    int n = clip_to_rect(rect, poly, 4);
     if (n < 1) return;
-    V3 point[8];
-    float dep[8];
-    float det1 = 1.f / (m11 * m22 - m12 * m21);
-    m11 *= det1; m12 *= det1; m21 *= det1; m22 *= det1;
-    int cnum = 0;
-    for (int j = 0; j < n; j++) {
-        float k1 = m22 * (ret[j * 2] - c1) - m12 * (ret[j * 2 + 1] - c2);
-        float k2 = -m21 * (ret[j * 2] - c1) + m11 * (ret[j * 2 + 1] - c2);
-        point[cnum] = center + k1 * bcol(Rb, a1) + k2 * bcol(Rb, a2);
-        dep[cnum] = Sa[codeN] - dot(normal2, point[cnum]);
-        if (dep[cnum] >= 0) {
-            ret[cnum * 2] = ret[j * 2];
-            ret[cnum * 2 + 1] = ret[j * 2 + 1];
-            cnum++;
+    // back to 3D (relative to the reference box's centre): invert the 2x2 map
+    const float inv_det = 1.f / (m_ua * m_vb - m_ub * m_va);
+    m_ua *= inv_det; m_ub *= inv_det; m_va *= inv_det; m_vb *= inv_det;
+    V3 pts[8];
+    float depth[8];
+    int kept = 0;
+    for (int k = 0; k < n; k++) {
+        const float du = poly[2 * k] - fc_u, dv = poly[2 * k + 1] - fc_v;
+        const float sa = m_vb * du - m_ub * dv;
+        const float sb = -m_va * du + m_ua * dv;
+        pts[kept] = face_c + sa * inc.ax[ia] + sb * inc.ax[ib];
+        depth[kept] = ref.h[face] - dot(ref_n, pts[kept]);
+        if (depth[kept] >= 0) {
+            poly[2 * kept] = poly[2 * k];
+            poly[2 * kept + 1] = poly[2 * k + 1];
+            kept++;
         }
     }
-    if (cnum < 1) return;
-    int maxc = 4;
-    if (maxc > cnum) maxc = cnum;
-    if (maxc < 1) maxc = 1;
-    out.normal = -normal;
-    if (cnum <= maxc) {
-        for (int j = 0; j < cnum; j++) {
-            V3 w = point[j] + pa;
-            if (code >= 4) w = w - normal * dep[j];
-            out.point[out.n] = w;
-            out.depth[out.n] = -dep[j];
-            out.n++;
-        }
+    if (kept < 1) return;
+    int pick[4];
+    int count = kept;
+    if (kept > 4) {
+        int deepest = 0;
+        for (int k = 1; k < kept; k++)
+            if (depth[k] > depth[deepest]) deepest = k;
+        spread_by_angle(kept, poly, 4, deepest, pick);
+        count = 4;
     } else {
-        int i1 = 0;
-        float maxdepth = dep[0];
-        for (int i = 1; i < cnum; i++)
-            if (dep[i] > maxdepth) { maxdepth = dep[i]; i1 = i; }
-        int iret[8];
-        cull_points(cnum, ret, maxc, i1, iret);
-        for (int j = 0; j < maxc; j++) {
-            V3 w = point[iret[j]] + pa;
-            if (code >= 4) w = w - normal * dep[iret[j]];
-            out.point[out.n] = w;
-            out.depth[out.n] = -dep[iret[j]];
-            out.n++;
-        }
+        for (int k = 0; k < kept; k++) pick[k] = k;
     }
+    // points on box 1: as found when box 1 is the incident box, moved down to
+    // the reference face when box 1 holds it
+    for (int k = 0; k < count; k++) {
+        V3 p = pts[pick[k]] + ref.c;
+        if (!ref_is_0) p = p - normal * depth[pick[k]];
+        out.point[k] = p;
+        out.depth[k] = -depth[pick[k]];
+    }
+    out.n = count;
 }
 
 }  // namespace rsg

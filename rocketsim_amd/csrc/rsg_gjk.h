@@ -380,14 +380,26 @@ RSG_FN bool ray_convex(V3 from, V3 to, const CShape& B, const Xf& tb, float& fra
 }
 
 // ---------------------------------------------------------------------------
-// btGjkEpaSolver2: GJK (Presson) + EPA, used when the cores overlap
+// Penetration depth of two overlapping convex shapes, as the reference's
+// btGjkEpaPenetrationDepthSolver computes it (btGjkEpa2.cpp, Bullet's
+// btGjkEpaSolver2 by N. Presson): a GJK distance query on the Minkowski
+// difference with margins (Gilbert, Johnson & Keerthi 1988, closest points
+// of the sub-simplices by signed volumes), and, when the origin is inside,
+// the expanding polytope algorithm (van den Bergen 2001): the polytope grows
+// towards the support point along its face nearest the origin until that
+// face's distance converges.  Tolerances, iteration limits, pool sizes and
+// the order in which faces are created, visited and compared follow the
+// reference, since the depth, normal and witness points must agree with it
+// bit for bit.
 
-constexpr float G2_ACCURACY = 0.0001f, G2_MIN_DISTANCE = 0.0001f, G2_DUP_EPS = 0.0001f;
-constexpr int G2_MAX_ITER = 128;
-constexpr float EPA_ACCURACY = 0.0001f, EPA_PLANE_EPS = 0.00001f;
-constexpr int EPA_MAX_VERTICES = 128, EPA_MAX_FACES = 256, EPA_MAX_ITER = 255;
-constexpr int EPA_STACK = 128;
+constexpr float GJK_EPS = 0.0001f;          // accuracy, minimum distance and duplicate-vertex tolerance
+constexpr int GJK_MAX_STEPS = 128;
+constexpr float POLY_EPS = 0.0001f;         // polytope convergence, minimum face area
+constexpr float POLY_PLANE_EPS = 0.00001f;  // a vertex this far below a face still sees it
+constexpr int POLY_MAX_VERTS = 128, POLY_MAX_FACES = 256, POLY_MAX_STEPS = 255;
+constexpr int POLY_WALK_DEPTH = 128;
 
+// Support mapping of the Minkowski difference A - B in A's frame.
 struct MinkDiff {
     const CShape* s0;
     const CShape* s1;
@@ -401,590 +413,592 @@ struct MinkDiff {
     RSG_FN V3 support(V3 d, int idx) const { return idx ? support1(d) : support0(d); }
 };
 
-struct SV { V3 d, w; };
+// A support point: the unit direction it was found along and the point.
+struct SupportPt { V3 dir, w; };
 
-struct G2Simplex {
-    int16_t c[4];
-    float p[4];
-    int rank;
+// A simplex of up to four pool entries with barycentric weights.
+struct Simplex {
+    int16_t idx[4];
+    float bary[4];
+    int n;
 };
 
-// vertex pool shared by the GJK stage (entries 0..3) and EPA (4..)
-constexpr int SV_POOL = 4 + EPA_MAX_VERTICES;
+// Pool entries 0-3 serve the GJK simplices; the polytope's vertices follow.
+constexpr int SUPPORT_POOL = 4 + POLY_MAX_VERTS;
 
-struct Gjk2 {
-    MinkDiff shape;
-    SV* sv;           // pool
-    V3 ray;
-    float distance;
-    G2Simplex simplices[2];
-    int16_t freel[4];
-    int nfree;
-    int current;
-    G2Simplex* simplex;
-    int status;       // 0 valid, 1 inside, 2 failed
+// Signed volume (triple product) a . (b x c), expanded in the reference's
+// term order.
+RSG_FN float triple(V3 a, V3 b, V3 c) {
+    return (a.y * b.z * c.x + a.z * b.x * c.y - a.x * b.z * c.y - a.y * b.x * c.z + a.x * b.y * c.z - a.z * b.y * c.x);
+}
 
-    RSG_FN void getsupport(V3 d, SV& s) const {
-        s.d = d * (1.f / len(d));
-        s.w = shape.support(s.d);
-    }
-    RSG_FN void removevertice(G2Simplex& sx) { freel[nfree++] = sx.c[--sx.rank]; }
-    RSG_FN void appendvertice(G2Simplex& sx, V3 v) {
-        sx.p[sx.rank] = 0;
-        sx.c[sx.rank] = freel[--nfree];
-        getsupport(v, sv[sx.c[sx.rank++]]);
-    }
-    RSG_FN static float det3(V3 a, V3 b, V3 c) {
-        return (a.y * b.z * c.x + a.z * b.x * c.y - a.x * b.z * c.y - a.y * b.x * c.z + a.x * b.y * c.z - a.z * b.y * c.x);
-    }
-    RSG_FN static float project2(V3 a, V3 b, float* w, uint32_t& m) {
-        V3 d = b - a;
-        float l = len2(d);
-        if (l > 0.f) {
-            float t = l > 0 ? -dot(a, d) / l : 0;
-            if (t >= 1) { w[0] = 0; w[1] = 1; m = 2; return len2(b); }
-            if (t <= 0) { w[0] = 1; w[1] = 0; m = 1; return len2(a); }
-            w[1] = t; w[0] = 1 - t; m = 3;
-            return len2(a + d * t);
-        }
-        return -1;
-    }
-    RSG_FN static float project3(V3 a, V3 b, V3 c, float* w, uint32_t& m) {
-        const int imd3[3] = {1, 2, 0};
-        V3 vt[3] = {a, b, c};
-        V3 dl[3] = {a - b, b - c, c - a};
-        V3 n = cross(dl[0], dl[1]);
-        float l = len2(n);
-        if (l > 0.f) {
-            float mindist = -1;
-            float subw[2] = {0.f, 0.f};
-            uint32_t subm = 0;
-            for (int i = 0; i < 3; i++) {
-                if (dot(vt[i], cross(dl[i], n)) > 0) {
-                    int j = imd3[i];
-                    float subd = project2(vt[i], vt[j], subw, subm);
-                    if (mindist < 0 || subd < mindist) {
-                        mindist = subd;
-                        m = ((subm & 1) ? 1u << i : 0u) + ((subm & 2) ? 1u << j : 0u);
-                        w[i] = subw[0];
-                        w[j] = subw[1];
-                        w[imd3[j]] = 0;
-                    }
-                }
-            }
-            if (mindist < 0) {
-                float d = dot(a, n);
-                float s = sqrtf(l);
-                V3 p = n * (d / l);
-                mindist = len2(p);
-                m = 7;
-                w[0] = len(cross(dl[1], b - p)) / s;
-                w[1] = len(cross(dl[2], c - p)) / s;
-                w[2] = 1 - (w[0] + w[1]);
-            }
-            return mindist;
-        }
-        return -1;
-    }
-    RSG_FN static float project4(V3 a, V3 b, V3 c, V3 d, float* w, uint32_t& m) {
-        const int imd3[3] = {1, 2, 0};
-        V3 vt[4] = {a, b, c, d};
-        V3 dl[3] = {a - d, b - d, c - d};
-        float vl = det3(dl[0], dl[1], dl[2]);
-        bool ng = (vl * dot(a, cross(b - c, a - b))) <= 0;
-        if (ng && fabsf(vl) > 0.f) {
-            float mindist = -1;
-            float subw[3] = {0.f, 0.f, 0.f};
-            uint32_t subm = 0;
-            for (int i = 0; i < 3; i++) {
-                int j = imd3[i];
-                float s = vl * dot(d, cross(dl[i], dl[j]));
-                if (s > 0) {
-                    float subd = project3(vt[i], vt[j], d, subw, subm);
-                    if (mindist < 0 || subd < mindist) {
-                        mindist = subd;
-                        m = ((subm & 1) ? 1u << i : 0u) + ((subm & 2) ? 1u << j : 0u) + ((subm & 4) ? 8u : 0u);
-                        w[i] = subw[0];
-                        w[j] = subw[1];
-                        w[imd3[j]] = 0;
-                        w[3] = subw[2];
-                    }
-                }
-            }
-            if (mindist < 0) {
-                mindist = 0;
-                m = 15;
-                w[0] = det3(c, b, d) / vl;
-                w[1] = det3(a, c, d) / vl;
-                w[2] = det3(b, a, d) / vl;
-                w[3] = 1 - (w[0] + w[1] + w[2]);
-            }
-            return mindist;
-        }
-        return -1;
-    }
+// Closest point to the origin on segment ab: squared distance (or -1 for a
+// degenerate segment), weights of a and b, and the mask of the vertices
+// that support it (bit 0 a, bit 1 b).
+RSG_FN float nearest_on_segment(V3 a, V3 b, float* wt, uint32_t& used) {
+    const V3 ab = b - a;
+    const float l2 = len2(ab);
+    if (!(l2 > 0.f)) return -1;
+    const float t = l2 > 0 ? -dot(a, ab) / l2 : 0;
+    if (t >= 1) { wt[0] = 0; wt[1] = 1; used = 2; return len2(b); }
+    if (t <= 0) { wt[0] = 1; wt[1] = 0; used = 1; return len2(a); }
+    wt[1] = t;
+    wt[0] = 1 - t;
+    used = 3;
+    return len2(a + ab * t);
+}
 
-    RSG_FN int evaluate(V3 guess) {
-        int iterations = 0;
-        float sqdist = 0, alpha = 0;
-        V3 lastw[4];
-        int clastw = 0;
-        freel[0] = 0; freel[1] = 1; freel[2] = 2; freel[3] = 3;
-        nfree = 4;
-        current = 0;
-        status = 0;
-        distance = 0;
-        simplices[0].rank = 0;
-        ray = guess;
-        float sqrl = len2(ray);
-        appendvertice(simplices[0], sqrl > 0 ? -ray : v3(1, 0, 0));
-        simplices[0].p[0] = 1;
-        ray = sv[simplices[0].c[0]].w;
-        sqdist = sqrl;
-        lastw[0] = lastw[1] = lastw[2] = lastw[3] = ray;
+// Closest point to the origin on triangle abc (bits 0-2 of `used`): on an
+// edge whose outer side holds the origin, else inside the face.
+RSG_FN float nearest_on_triangle(V3 a, V3 b, V3 c, float* wt, uint32_t& used) {
+    const V3 p[3] = {a, b, c};
+    const V3 e[3] = {a - b, b - c, c - a};  // edge i runs from p[i] to p[i+1], reversed
+    const V3 n = cross(e[0], e[1]);
+    const float n2 = len2(n);
+    if (!(n2 > 0.f)) return -1;
+    float best = -1;
+    float ew[2] = {0.f, 0.f};
+    uint32_t eu = 0;
+    for (int i = 0; i < 3; i++) {
+        if (!(dot(p[i], cross(e[i], n)) > 0)) continue;  // origin not outside edge i
+        const int j = i == 2 ? 0 : i + 1;
+        const float d2 = nearest_on_segment(p[i], p[j], ew, eu);
+        if (best < 0 || d2 < best) {
+            best = d2;
+            used = ((eu & 1) ? 1u << i : 0u) + ((eu & 2) ? 1u << j : 0u);
+            wt[i] = ew[0];
+            wt[j] = ew[1];
+            wt[j == 2 ? 0 : j + 1] = 0;
+        }
+    }
+    if (best < 0) {
+        const float k = dot(a, n);
+        const float nl = sqrtf(n2);
+        const V3 q = n * (k / n2);
+        best = len2(q);
+        used = 7;
+        wt[0] = len(cross(e[1], b - q)) / nl;
+        wt[1] = len(cross(e[2], c - q)) / nl;
+        wt[2] = 1 - (wt[0] + wt[1]);
+    }
+    return best;
+}
+
+// Closest point to the origin in tetrahedron abcd (bits 0-3): on a face
+// through d that the origin is outside of, else the origin itself.
+RSG_FN float nearest_in_tetra(V3 a, V3 b, V3 c, V3 d, float* wt, uint32_t& used) {
+    const V3 p[4] = {a, b, c, d};
+    const V3 e[3] = {a - d, b - d, c - d};
+    const float vol = triple(e[0], e[1], e[2]);
+    const bool facing = (vol * dot(a, cross(b - c, a - b))) <= 0;
+    if (!(facing && fabsf(vol) > 0.f)) return -1;
+    float best = -1;
+    float fw[3] = {0.f, 0.f, 0.f};
+    uint32_t fu = 0;
+    for (int i = 0; i < 3; i++) {
+        const int j = i == 2 ? 0 : i + 1;
+        if (!(vol * dot(d, cross(e[i], e[j])) > 0)) continue;
+        const float d2 = nearest_on_triangle(p[i], p[j], d, fw, fu);
+        if (best < 0 || d2 < best) {
+            best = d2;
+            used = ((fu & 1) ? 1u << i : 0u) + ((fu & 2) ? 1u << j : 0u) + ((fu & 4) ? 8u : 0u);
+            wt[i] = fw[0];
+            wt[j] = fw[1];
+            wt[j == 2 ? 0 : j + 1] = 0;
+            wt[3] = fw[2];
+        }
+    }
+    if (best < 0) {
+        best = 0;
+        used = 15;
+        wt[0] = triple(c, b, d) / vol;
+        wt[1] = triple(a, c, d) / vol;
+        wt[2] = triple(b, a, d) / vol;
+        wt[3] = 1 - (wt[0] + wt[1] + wt[2]);
+    }
+    return best;
+}
+
+// GJK on the Minkowski difference: the simplex closest to the origin.
+// Outcome: 0 separated (dist is the distance), 1 the origin is inside (or
+// touching), 2 out of steps.
+struct GjkQuery {
+    MinkDiff md;
+    SupportPt* pool;
+    V3 v;                 // current closest point to the origin
+    float dist;
+    Simplex sets[2];      // the live simplex and the one being built
+    int live;
+    int16_t spare[4];     // pool slots 0-3 not in the live simplex
+    int n_spare;
+    Simplex* result;
+    int outcome;
+
+    RSG_FN void support(V3 d, SupportPt& sp) const {
+        sp.dir = d * (1.f / len(d));
+        sp.w = md.support(sp.dir);
+    }
+    RSG_FN void push(Simplex& sx, V3 d) {
+        sx.bary[sx.n] = 0;
+        sx.idx[sx.n] = spare[--n_spare];
+        support(d, pool[sx.idx[sx.n++]]);
+    }
+    RSG_FN void pop(Simplex& sx) { spare[n_spare++] = sx.idx[--sx.n]; }
+    RSG_FN V3 wof(const Simplex& sx, int k) const { return pool[sx.idx[k]].w; }
+
+    RSG_FN int run(V3 guess) {
+        for (int k = 0; k < 4; k++) spare[k] = (int16_t)k;
+        n_spare = 4;
+        live = 0;
+        outcome = 0;
+        dist = 0;
+        sets[0].n = 0;
+        v = guess;
+        const float g2 = len2(v);
+        push(sets[0], g2 > 0 ? -v : v3(1, 0, 0));
+        sets[0].bary[0] = 1;
+        v = pool[sets[0].idx[0]].w;
+        float d2 = g2;
+        float lower = 0;  // best lower bound on the distance so far
+        V3 recent[4] = {v, v, v, v};
+        int slot = 0;
+        int steps = 0;
         do {
-            int next = 1 - current;
-            G2Simplex& cs = simplices[current];
-            G2Simplex& ns = simplices[next];
-            float rl = len(ray);
-            if (rl < G2_MIN_DISTANCE) { status = 1; break; }
-            appendvertice(cs, -ray);
-            V3 w = sv[cs.c[cs.rank - 1]].w;
-            bool found = false;
-            for (int i = 0; i < 4; i++)
-                if (len2(w - lastw[i]) < G2_DUP_EPS) { found = true; break; }
-            if (found) { removevertice(simplices[current]); break; }
-            clastw = (clastw + 1) & 3;
-            lastw[clastw] = w;
-            float omega = dot(ray, w) / rl;
-            alpha = fmaxf(omega, alpha);
-            if (((rl - alpha) - (G2_ACCURACY * rl)) <= 0) { removevertice(simplices[current]); break; }
-            float weights[4];
-            uint32_t mask = 0;
-            if (cs.rank == 2) sqdist = project2(sv[cs.c[0]].w, sv[cs.c[1]].w, weights, mask);
-            else if (cs.rank == 3) sqdist = project3(sv[cs.c[0]].w, sv[cs.c[1]].w, sv[cs.c[2]].w, weights, mask);
-            else sqdist = project4(sv[cs.c[0]].w, sv[cs.c[1]].w, sv[cs.c[2]].w, sv[cs.c[3]].w, weights, mask);
-            if (sqdist >= 0) {
-                ns.rank = 0;
-                ray = v3zero();
-                current = next;
-                for (int i = 0, ni = cs.rank; i < ni; i++) {
-                    if (mask & (1u << i)) {
-                        ns.c[ns.rank] = cs.c[i];
-                        ns.p[ns.rank++] = weights[i];
-                        ray += sv[cs.c[i]].w * weights[i];
-                    } else {
-                        freel[nfree++] = cs.c[i];
-                    }
+            Simplex& cur = sets[live];
+            Simplex& nxt = sets[1 - live];
+            const float vl = len(v);
+            if (vl < GJK_EPS) { outcome = 1; break; }
+            push(cur, -v);
+            const V3 w = wof(cur, cur.n - 1);
+            bool repeat = false;
+            for (int k = 0; k < 4 && !repeat; k++) repeat = len2(w - recent[k]) < GJK_EPS;
+            if (repeat) { pop(sets[live]); break; }
+            slot = (slot + 1) & 3;
+            recent[slot] = w;
+            lower = fmaxf(dot(v, w) / vl, lower);
+            if (((vl - lower) - (GJK_EPS * vl)) <= 0) { pop(sets[live]); break; }
+            float wt[4];
+            uint32_t used = 0;
+            if (cur.n == 2) d2 = nearest_on_segment(wof(cur, 0), wof(cur, 1), wt, used);
+            else if (cur.n == 3) d2 = nearest_on_triangle(wof(cur, 0), wof(cur, 1), wof(cur, 2), wt, used);
+            else d2 = nearest_in_tetra(wof(cur, 0), wof(cur, 1), wof(cur, 2), wof(cur, 3), wt, used);
+            if (!(d2 >= 0)) { pop(sets[live]); break; }
+            // keep the supporting vertices, rebuild v from their weights
+            nxt.n = 0;
+            v = v3zero();
+            live = 1 - live;
+            for (int k = 0; k < cur.n; k++) {
+                if (used & (1u << k)) {
+                    nxt.idx[nxt.n] = cur.idx[k];
+                    nxt.bary[nxt.n++] = wt[k];
+                    v += pool[cur.idx[k]].w * wt[k];
+                } else {
+                    spare[n_spare++] = cur.idx[k];
                 }
-                if (mask == 15) status = 1;
-            } else {
-                removevertice(simplices[current]);
-                break;
             }
-            status = (++iterations < G2_MAX_ITER) ? status : 2;
-        } while (status == 0);
-        simplex = &simplices[current];
-        if (status == 0) distance = len(ray);
-        else if (status == 1) distance = 0;
-        return status;
+            if (used == 15) outcome = 1;
+            outcome = (++steps < GJK_MAX_STEPS) ? outcome : 2;
+        } while (outcome == 0);
+        result = &sets[live];
+        if (outcome == 0) dist = len(v);
+        else if (outcome == 1) dist = 0;
+        return outcome;
     }
 
-    // EncloseOrigin, unrolled over the simplex rank (1 -> 4)
-    RSG_FN bool enclose4() {
-        return fabsf(det3(sv[simplex->c[0]].w - sv[simplex->c[3]].w, sv[simplex->c[1]].w - sv[simplex->c[3]].w,
-                          sv[simplex->c[2]].w - sv[simplex->c[3]].w)) > 0;
+    RSG_FN bool tetra_has_volume() const {
+        const Simplex& r = *result;
+        return fabsf(triple(wof(r, 0) - wof(r, 3), wof(r, 1) - wof(r, 3), wof(r, 2) - wof(r, 3))) > 0;
     }
-    RSG_FN bool enclose3() {
-        V3 nn = cross(sv[simplex->c[1]].w - sv[simplex->c[0]].w, sv[simplex->c[2]].w - sv[simplex->c[0]].w);
-        if (len2(nn) > 0) {
-            appendvertice(*simplex, nn);
-            if (enclose4()) return true;
-            removevertice(*simplex);
-            appendvertice(*simplex, -nn);
-            if (enclose4()) return true;
-            removevertice(*simplex);
-        }
-        return false;
-    }
-    RSG_FN bool enclose2() {
-        V3 d = sv[simplex->c[1]].w - sv[simplex->c[0]].w;
-        for (int i = 0; i < 3; i++) {
-            V3 axis = v3zero();
-            set(axis, i, 1.f);
-            V3 p = cross(d, axis);
-            if (len2(p) > 0) {
-                appendvertice(*simplex, p);
-                if (enclose3()) return true;
-                removevertice(*simplex);
-                appendvertice(*simplex, -p);
-                if (enclose3()) return true;
-                removevertice(*simplex);
+
+    // Trial direction k at simplex size n (n = 1: +-x, +-y, +-z; n = 2: +-
+    // (edge x axis) per axis; n = 3: +- face normal), or false when k is past
+    // the last one.  Degenerate cross products offer no trial.
+    RSG_FN bool trial_dir(int n, int& k, V3& d) const {
+        const Simplex& r = *result;
+        while (true) {
+            if (n == 1) {
+                if (k >= 6) return false;
+                V3 axis = v3zero();
+                set(axis, k >> 1, 1.f);
+                d = (k & 1) ? -axis : axis;
+                return true;
             }
-        }
-        return false;
-    }
-    RSG_FN bool enclose1() {
-        for (int i = 0; i < 3; i++) {
-            V3 axis = v3zero();
-            set(axis, i, 1.f);
-            appendvertice(*simplex, axis);
-            if (enclose2()) return true;
-            removevertice(*simplex);
-            appendvertice(*simplex, -axis);
-            if (enclose2()) return true;
-            removevertice(*simplex);
-        }
-        return false;
-    }
-    RSG_FN bool enclose_origin() {
-        switch (simplex->rank) {
-            case 1: return enclose1();
-            case 2: return enclose2();
-            case 3: return enclose3();
-            case 4: return enclose4();
-        }
-        return false;
-    }
-};
-
-struct EpaFace {
-    V3 n;
-    float d;
-    int16_t c[3];
-    int16_t f[3];
-    int16_t l[2];
-    uint8_t e[3];
-    uint8_t pass;
-};
-
-struct Epa {
-    int status;  // 0 valid, 1 touching, 2 degenerated, 3 nonconvex, 4 invalid hull, 5 out of faces,
-                 // 6 out of vertices, 7 accuracy reached, 8 fallback, 9 failed
-    G2Simplex result;
-    V3 normal;
-    float depth;
-    SV* sv;               // shared pool (EPA vertices from index 4)
-    EpaFace* fc;          // face pool, EPA_MAX_FACES entries
-    int nextsv;
-    int16_t hull_root, stock_root;
-    int hull_count, stock_count;
-
-    RSG_FN void bind(int fa, int ea, int fb, int eb) {
-        fc[fa].e[ea] = (uint8_t)eb; fc[fa].f[ea] = (int16_t)fb;
-        fc[fb].e[eb] = (uint8_t)ea; fc[fb].f[eb] = (int16_t)fa;
-    }
-    RSG_FN void list_append(int16_t& root, int& count, int f) {
-        fc[f].l[0] = -1;
-        fc[f].l[1] = root;
-        if (root >= 0) fc[root].l[0] = (int16_t)f;
-        root = (int16_t)f;
-        count++;
-    }
-    RSG_FN void list_remove(int16_t& root, int& count, int f) {
-        if (fc[f].l[1] >= 0) fc[fc[f].l[1]].l[0] = fc[f].l[0];
-        if (fc[f].l[0] >= 0) fc[fc[f].l[0]].l[1] = fc[f].l[1];
-        if (f == root) root = fc[f].l[1];
-        count--;
-    }
-    RSG_FN void init(SV* pool, EpaFace* faces) {
-        sv = pool;
-        fc = faces;
-        status = 9;
-        normal = v3zero();
-        depth = 0;
-        nextsv = 0;
-        hull_root = -1; stock_root = -1; hull_count = 0; stock_count = 0;
-        for (int i = 0; i < EPA_MAX_FACES; i++) list_append(stock_root, stock_count, EPA_MAX_FACES - i - 1);
-    }
-    RSG_FN bool edgedist(int face, int a, int b, float& dist) const {
-        V3 ba = sv[b].w - sv[a].w;
-        V3 n_ab = cross(ba, fc[face].n);
-        float a_dot_nab = dot(sv[a].w, n_ab);
-        if (a_dot_nab < 0) {
-            float ba_l2 = len2(ba);
-            float a_dot_ba = dot(sv[a].w, ba);
-            float b_dot_ba = dot(sv[b].w, ba);
-            if (a_dot_ba > 0) dist = len(sv[a].w);
-            else if (b_dot_ba < 0) dist = len(sv[b].w);
-            else {
-                float a_dot_b = dot(sv[a].w, sv[b].w);
-                dist = sqrtf(fmaxf((len2(sv[a].w) * len2(sv[b].w) - a_dot_b * a_dot_b) / ba_l2, 0.f));
+            if (n == 2) {
+                if (k >= 6) return false;
+                V3 axis = v3zero();
+                set(axis, k >> 1, 1.f);
+                const V3 c = cross(wof(r, 1) - wof(r, 0), axis);
+                if (!(len2(c) > 0)) { k = (k | 1) + 1; continue; }
+                d = (k & 1) ? -c : c;
+                return true;
             }
+            if (k >= 2) return false;
+            const V3 c = cross(wof(r, 1) - wof(r, 0), wof(r, 2) - wof(r, 0));
+            if (!(len2(c) > 0)) return false;
+            d = (k & 1) ? -c : c;
             return true;
         }
-        return false;
-    }
-    RSG_FN int newface(int a, int b, int c, bool forced) {
-        if (stock_root >= 0) {
-            int face = stock_root;
-            list_remove(stock_root, stock_count, face);
-            list_append(hull_root, hull_count, face);
-            EpaFace& F = fc[face];
-            F.pass = 0;
-            F.c[0] = (int16_t)a; F.c[1] = (int16_t)b; F.c[2] = (int16_t)c;
-            F.n = cross(sv[b].w - sv[a].w, sv[c].w - sv[a].w);
-            float l = len(F.n);
-            bool v = l > EPA_ACCURACY;
-            if (v) {
-                if (!(edgedist(face, a, b, F.d) || edgedist(face, b, c, F.d) || edgedist(face, c, a, F.d)))
-                    F.d = dot(sv[a].w, F.n) / l;
-                F.n = F.n * (1.f / l);
-                if (forced || F.d >= -EPA_PLANE_EPS) return face;
-                status = 3;
-            } else {
-                status = 2;
-            }
-            list_remove(hull_root, hull_count, face);
-            list_append(stock_root, stock_count, face);
-            return -1;
-        }
-        status = stock_root >= 0 ? 6 : 5;
-        return -1;
-    }
-    RSG_FN int findbest() const {
-        int minf = hull_root;
-        float mind = fc[minf].d * fc[minf].d;
-        for (int f = fc[minf].l[1]; f >= 0; f = fc[f].l[1]) {
-            float sqd = fc[f].d * fc[f].d;
-            if (sqd < mind) { minf = f; mind = sqd; }
-        }
-        return minf;
     }
 
-    struct Horizon { int cf, ff, nf; };
-
-    // expand(), iterative form of the reference's recursion
-    RSG_FN bool expand(int pass, int w, int f0, int e0, Horizon& hz) {
-        const int i1m3[3] = {1, 2, 0};
-        const int i2m3[3] = {2, 0, 1};
-        struct Frame { int16_t f; uint8_t e, phase; };
-        Frame stk[EPA_STACK];
-        int sp = 0;
-        stk[0].f = (int16_t)f0; stk[0].e = (uint8_t)e0; stk[0].phase = 0;
-        bool ret = false;
-        while (sp >= 0) {
-            Frame& fr = stk[sp];
-            int f = fr.f;
-            if (fr.phase == 0) {
-                if (fc[f].pass == pass) { ret = false; sp--; continue; }
-                int e1 = i1m3[fr.e];
-                if ((dot(fc[f].n, sv[w].w) - fc[f].d) < -EPA_PLANE_EPS) {
-                    int nf = newface(fc[f].c[e1], fc[f].c[fr.e], w, false);
-                    if (nf >= 0) {
-                        bind(nf, 0, f, fr.e);
-                        if (hz.cf >= 0) bind(hz.cf, 1, nf, 2);
-                        else hz.ff = nf;
-                        hz.cf = nf;
-                        hz.nf++;
-                        ret = true;
-                    } else {
-                        ret = false;
-                    }
-                    sp--;
-                    continue;
-                }
-                fc[f].pass = (uint8_t)pass;
-                fr.phase = 1;
-                if (sp + 1 >= EPA_STACK) { status = 4; return false; }
-                Frame& ch = stk[++sp];
-                ch.f = fc[f].f[e1]; ch.e = fc[f].e[e1]; ch.phase = 0;
-            } else if (fr.phase == 1) {
-                if (!ret) { sp--; continue; }
-                int e2 = i2m3[fr.e];
-                fr.phase = 2;
-                if (sp + 1 >= EPA_STACK) { status = 4; return false; }
-                Frame& ch = stk[++sp];
-                ch.f = fc[f].f[e2]; ch.e = fc[f].e[e2]; ch.phase = 0;
-            } else {
-                if (ret) {
-                    list_remove(hull_root, hull_count, f);
-                    list_append(stock_root, stock_count, f);
-                }
-                sp--;
+    // Grows the final simplex to a tetrahedron of non-zero volume around the
+    // origin, trying the trial directions depth first (the simplex of a
+    // touching contact is flat).
+    RSG_FN bool grow_to_tetra() {
+        Simplex& r = *result;
+        const int base = r.n;
+        if (base == 4) return tetra_has_volume();
+        int next_trial[5] = {0, 0, 0, 0, 0};
+        int n = base;
+        while (true) {
+            if (n == 4) {
+                if (tetra_has_volume()) return true;
+                pop(r);
+                n--;
+                continue;
             }
+            V3 d;
+            int k = next_trial[n];
+            const bool more = trial_dir(n, k, d);
+            next_trial[n] = k + 1;
+            if (!more) {
+                if (n == base) return false;
+                pop(r);
+                n--;
+                continue;
+            }
+            push(r, d);
+            n++;
+            next_trial[n] = 0;
         }
-        return ret;
-    }
-
-    RSG_FN int evaluate(Gjk2& gjk, V3 guess) {
-        G2Simplex& simplex = *gjk.simplex;
-        if (simplex.rank > 1 && gjk.enclose_origin()) {
-            while (hull_root >= 0) {
-                int f = hull_root;
-                list_remove(hull_root, hull_count, f);
-                list_append(stock_root, stock_count, f);
-            }
-            status = 0;
-            nextsv = 0;
-            if (Gjk2::det3(sv[simplex.c[0]].w - sv[simplex.c[3]].w, sv[simplex.c[1]].w - sv[simplex.c[3]].w,
-                           sv[simplex.c[2]].w - sv[simplex.c[3]].w) < 0) {
-                int16_t tc = simplex.c[0]; simplex.c[0] = simplex.c[1]; simplex.c[1] = tc;
-                float tp = simplex.p[0]; simplex.p[0] = simplex.p[1]; simplex.p[1] = tp;
-            }
-            int tetra[4] = {newface(simplex.c[0], simplex.c[1], simplex.c[2], true),
-                            newface(simplex.c[1], simplex.c[0], simplex.c[3], true),
-                            newface(simplex.c[2], simplex.c[1], simplex.c[3], true),
-                            newface(simplex.c[0], simplex.c[2], simplex.c[3], true)};
-            if (hull_count == 4) {
-                int best = findbest();
-                EpaFace outer = fc[best];
-                int pass = 0;
-                int iterations = 0;
-                bind(tetra[0], 0, tetra[1], 0);
-                bind(tetra[0], 1, tetra[2], 0);
-                bind(tetra[0], 2, tetra[3], 0);
-                bind(tetra[1], 1, tetra[3], 2);
-                bind(tetra[1], 2, tetra[2], 1);
-                bind(tetra[2], 2, tetra[3], 1);
-                status = 0;
-                for (; iterations < EPA_MAX_ITER; iterations++) {
-                    if (nextsv < EPA_MAX_VERTICES) {
-                        Horizon hz{-1, -1, 0};
-                        int w = 4 + nextsv++;
-                        bool valid = true;
-                        fc[best].pass = (uint8_t)(++pass);
-                        gjk.getsupport(fc[best].n, sv[w]);
-                        float wdist = dot(fc[best].n, sv[w].w) - fc[best].d;
-                        if (wdist > EPA_ACCURACY) {
-                            for (int j = 0; j < 3 && valid; j++)
-                                valid &= expand(pass, w, fc[best].f[j], fc[best].e[j], hz);
-                            if (valid && hz.nf >= 3) {
-                                bind(hz.cf, 1, hz.ff, 2);
-                                list_remove(hull_root, hull_count, best);
-                                list_append(stock_root, stock_count, best);
-                                best = findbest();
-                                outer = fc[best];
-                            } else {
-                                status = 4;
-                                break;
-                            }
-                        } else {
-                            status = 7;
-                            break;
-                        }
-                    } else {
-                        status = 6;
-                        break;
-                    }
-                }
-                V3 projection = outer.n * outer.d;
-                normal = outer.n;
-                depth = outer.d;
-                result.rank = 3;
-                result.c[0] = outer.c[0]; result.c[1] = outer.c[1]; result.c[2] = outer.c[2];
-                result.p[0] = len(cross(sv[outer.c[1]].w - projection, sv[outer.c[2]].w - projection));
-                result.p[1] = len(cross(sv[outer.c[2]].w - projection, sv[outer.c[0]].w - projection));
-                result.p[2] = len(cross(sv[outer.c[0]].w - projection, sv[outer.c[1]].w - projection));
-                float sum = result.p[0] + result.p[1] + result.p[2];
-                result.p[0] /= sum;
-                result.p[1] /= sum;
-                result.p[2] /= sum;
-                return status;
-            }
-        }
-        status = 8;
-        normal = -guess;
-        float nl = len(normal);
-        if (nl > 0) normal = normal * (1.f / nl);
-        else normal = v3(1, 0, 0);
-        depth = 0;
-        result.rank = 1;
-        result.c[0] = simplex.c[0];
-        result.p[0] = 1;
-        return status;
     }
 };
 
-struct G2Results {
+// A polytope face: outward unit normal, distance of its plane from the
+// origin, vertices (pool indices), neighbours across its edges (edge k runs
+// from v[k] to v[k+1]) with the neighbour's matching edge, the walk mark, and
+// the creation stamp that orders the live faces (newest first).
+struct PolyFace {
+    V3 n;
+    float d;
+    int16_t v[3];
+    int16_t nb[3];
+    uint8_t nb_edge[3];
+    uint8_t mark;
+    uint16_t stamp;
+    bool live;
+};
+
+// Expanding polytope over a vertex pool (entries 4 on) and a face pool.
+// Outcome: 0 valid, 1 touching, 2 degenerate face, 3 non-convex, 4 bad
+// horizon, 5 out of faces, 6 out of vertices, 7 converged, 8 fallback
+// (no enclosing tetrahedron), 9 failed.
+struct Polytope {
+    int outcome;
+    Simplex result;   // the converged face's vertices and barycentric weights
+    V3 normal;
+    float depth;
+    SupportPt* pool;
+    PolyFace* faces;
+    int n_used;       // faces[0 .. n_used) have been handed out at least once
+    int16_t free_list[POLY_MAX_FACES];
+    int n_free;
+    int n_live;
+    uint16_t clock;
+    int next_vert;
+
+    RSG_FN void reset(SupportPt* sp, PolyFace* fp) {
+        pool = sp;
+        faces = fp;
+        outcome = 9;
+        normal = v3zero();
+        depth = 0;
+        next_vert = 0;
+        n_used = 0;
+        n_free = 0;
+        n_live = 0;
+        clock = 0;
+    }
+    RSG_FN void link(int fa, int ea, int fb, int eb) {
+        faces[fa].nb_edge[ea] = (uint8_t)eb; faces[fa].nb[ea] = (int16_t)fb;
+        faces[fb].nb_edge[eb] = (uint8_t)ea; faces[fb].nb[eb] = (int16_t)fa;
+    }
+    RSG_FN void retire(int f) {
+        faces[f].live = false;
+        n_live--;
+        free_list[n_free++] = (int16_t)f;
+    }
+    // Distance of the face plane's nearest point to the origin when that
+    // point lies outside edge ab (the origin projects past the edge).
+    RSG_FN bool edge_dist(int f, int a, int b, float& out) const {
+        const V3 wa = pool[a].w, wb = pool[b].w;
+        const V3 ab = wb - wa;
+        const V3 edge_out = cross(ab, faces[f].n);
+        if (!(dot(wa, edge_out) < 0)) return false;
+        const float ab2 = len2(ab);
+        const float a_ab = dot(wa, ab);
+        const float b_ab = dot(wb, ab);
+        if (a_ab > 0) out = len(wa);
+        else if (b_ab < 0) out = len(wb);
+        else {
+            const float ab_dot = dot(wa, wb);
+            out = sqrtf(fmaxf((len2(wa) * len2(wb) - ab_dot * ab_dot) / ab2, 0.f));
+        }
+        return true;
+    }
+    // A new face abc, or -1 (outcome set) when it is degenerate, faces away
+    // from the origin (unless forced) or no face is free.
+    RSG_FN int add_face(int a, int b, int c, bool forced) {
+        int f;
+        if (n_free > 0) f = free_list[--n_free];
+        else if (n_used < POLY_MAX_FACES) f = n_used++;
+        else {
+            outcome = 5;
+            return -1;
+        }
+        PolyFace& F = faces[f];
+        F.live = true;
+        F.stamp = ++clock;
+        n_live++;
+        F.mark = 0;
+        F.v[0] = (int16_t)a; F.v[1] = (int16_t)b; F.v[2] = (int16_t)c;
+        F.n = cross(pool[b].w - pool[a].w, pool[c].w - pool[a].w);
+        const float nl = len(F.n);
+        if (nl > POLY_EPS) {
+            if (!(edge_dist(f, a, b, F.d) || edge_dist(f, b, c, F.d) || edge_dist(f, c, a, F.d)))
+                F.d = dot(pool[a].w, F.n) / nl;
+            F.n = F.n * (1.f / nl);
+            if (forced || F.d >= -POLY_PLANE_EPS) return f;
+            outcome = 3;
+        } else {
+            outcome = 2;
+        }
+        retire(f);
+        return -1;
+    }
+    // The live face nearest the origin (the newest one on equal distance).
+    RSG_FN int nearest_face() const {
+        int best = -1;
+        for (int f = 0; f < n_used; f++)
+            if (faces[f].live && (best < 0 || faces[f].stamp > faces[best].stamp)) best = f;
+        float bd = faces[best].d * faces[best].d;
+        for (int f = 0; f < n_used; f++) {
+            if (!faces[f].live || f == best) continue;
+            const float d2 = faces[f].d * faces[f].d;
+            if (d2 < bd || (d2 == bd && faces[f].stamp > faces[best].stamp)) {
+                best = f;
+                bd = d2;
+            }
+        }
+        return best;
+    }
+
+    struct Rim { int first, last, count; };  // the new faces along the horizon
+
+    // Walks the faces visible from vertex w, starting across edge `e0` of the
+    // face being replaced (into face f0), depth first: a face that w sees is
+    // marked, its two other neighbours walked, and it is retired once both
+    // walks succeed; a face w does not see gets a new face on the shared
+    // edge, linked to the previous new face along the horizon.
+    RSG_FN bool walk_horizon(int mark, int w, int f0, int e0, Rim& rim) {
+        struct Step { int16_t f; uint8_t e, stage; };
+        Step st[POLY_WALK_DEPTH];
+        int top = 0;
+        st[0].f = (int16_t)f0; st[0].e = (uint8_t)e0; st[0].stage = 0;
+        bool ok = false;
+        while (top >= 0) {
+            Step& s = st[top];
+            const int f = s.f;
+            const int ea = s.e == 2 ? 0 : s.e + 1;   // the edge after e
+            const int eb = ea == 2 ? 0 : ea + 1;      // and the one after that
+            if (s.stage == 0) {
+                if (faces[f].mark == mark) { ok = false; top--; continue; }
+                if ((dot(faces[f].n, pool[w].w) - faces[f].d) < -POLY_PLANE_EPS) {
+                    // w is below this face: the shared edge is on the horizon
+                    const int nf = add_face(faces[f].v[ea], faces[f].v[s.e], w, false);
+                    ok = nf >= 0;
+                    if (ok) {
+                        link(nf, 0, f, s.e);
+                        if (rim.last >= 0) link(rim.last, 1, nf, 2);
+                        else rim.first = nf;
+                        rim.last = nf;
+                        rim.count++;
+                    }
+                    top--;
+                    continue;
+                }
+                faces[f].mark = (uint8_t)mark;
+                s.stage = 1;
+                if (top + 1 >= POLY_WALK_DEPTH) { outcome = 4; return false; }
+                Step& c = st[++top];
+                c.f = faces[f].nb[ea]; c.e = faces[f].nb_edge[ea]; c.stage = 0;
+            } else if (s.stage == 1) {
+                if (!ok) { top--; continue; }
+                s.stage = 2;
+                if (top + 1 >= POLY_WALK_DEPTH) { outcome = 4; return false; }
+                Step& c = st[++top];
+                c.f = faces[f].nb[eb]; c.e = faces[f].nb_edge[eb]; c.stage = 0;
+            } else {
+                if (ok) retire(f);
+                top--;
+            }
+        }
+        return ok;
+    }
+
+    // Expands from the GJK query's final simplex (which encloses the origin).
+    RSG_FN int expand(GjkQuery& gjk, V3 guess) {
+        Simplex& sx = *gjk.result;
+        if (sx.n > 1 && gjk.grow_to_tetra()) {
+            outcome = 0;
+            next_vert = 0;
+            if (triple(gjk.wof(sx, 0) - gjk.wof(sx, 3), gjk.wof(sx, 1) - gjk.wof(sx, 3),
+                       gjk.wof(sx, 2) - gjk.wof(sx, 3)) < 0) {
+                // orient the tetrahedron so that its faces point outwards
+                const int16_t ti = sx.idx[0]; sx.idx[0] = sx.idx[1]; sx.idx[1] = ti;
+                const float tb = sx.bary[0]; sx.bary[0] = sx.bary[1]; sx.bary[1] = tb;
+            }
+            const int t0 = add_face(sx.idx[0], sx.idx[1], sx.idx[2], true);
+            const int t1 = add_face(sx.idx[1], sx.idx[0], sx.idx[3], true);
+            const int t2 = add_face(sx.idx[2], sx.idx[1], sx.idx[3], true);
+            const int t3 = add_face(sx.idx[0], sx.idx[2], sx.idx[3], true);
+            if (n_live == 4) {
+                int best = nearest_face();
+                PolyFace final_face = faces[best];
+                int mark = 0;
+                link(t0, 0, t1, 0);
+                link(t0, 1, t2, 0);
+                link(t0, 2, t3, 0);
+                link(t1, 1, t3, 2);
+                link(t1, 2, t2, 1);
+                link(t2, 2, t3, 1);
+                outcome = 0;
+                for (int step = 0; step < POLY_MAX_STEPS; step++) {
+                    if (next_vert >= POLY_MAX_VERTS) { outcome = 6; break; }
+                    Rim rim{-1, -1, 0};
+                    const int w = 4 + next_vert++;
+                    faces[best].mark = (uint8_t)(++mark);
+                    gjk.support(faces[best].n, pool[w]);
+                    const float gain = dot(faces[best].n, pool[w].w) - faces[best].d;
+                    if (!(gain > POLY_EPS)) { outcome = 7; break; }
+                    bool ok = true;
+                    for (int k = 0; k < 3 && ok; k++) ok &= walk_horizon(mark, w, faces[best].nb[k], faces[best].nb_edge[k], rim);
+                    if (!(ok && rim.count >= 3)) { outcome = 4; break; }
+                    link(rim.last, 1, rim.first, 2);
+                    retire(best);
+                    best = nearest_face();
+                    final_face = faces[best];
+                }
+                // barycentric weights of the origin's projection on the face
+                const V3 proj = final_face.n * final_face.d;
+                normal = final_face.n;
+                depth = final_face.d;
+                result.n = 3;
+                for (int k = 0; k < 3; k++) result.idx[k] = final_face.v[k];
+                for (int k = 0; k < 3; k++) {
+                    const int a = final_face.v[k == 2 ? 0 : k + 1], b = final_face.v[k == 0 ? 2 : k - 1];
+                    result.bary[k] = len(cross(pool[a].w - proj, pool[b].w - proj));
+                }
+                const float total = result.bary[0] + result.bary[1] + result.bary[2];
+                for (int k = 0; k < 3; k++) result.bary[k] /= total;
+                return outcome;
+            }
+        }
+        // no tetrahedron: the guess direction at zero depth
+        outcome = 8;
+        normal = -guess;
+        const float nl = len(normal);
+        normal = nl > 0 ? normal * (1.f / nl) : v3(1, 0, 0);
+        depth = 0;
+        result.n = 1;
+        result.idx[0] = sx.idx[0];
+        result.bary[0] = 1;
+        return outcome;
+    }
+};
+
+struct PenResult {
     V3 witness[2];
     V3 normal;
     float distance;
 };
 
-RSG_FN void g2_init_shape(MinkDiff& md, const CShape& s0, const Xf& t0, const CShape& s1, const Xf& t1, bool margins) {
+RSG_FN void mink_init(MinkDiff& md, const CShape& s0, const Xf& t0, const CShape& s1, const Xf& t1, bool margins) {
     md.s0 = &s0;
     md.s1 = &s1;
     md.to1 = transpose(t1.R) * t0.R;
-    // wtrs0.inverseTimes(wtrs1)
+    // shape 1's frame in shape 0's: t0^-1 * t1
     md.to0.R = transpose(t0.R) * t1.R;
     md.to0.t = mul_vm(t1.t - t0.t, t0.R);
     md.margins = margins;
 }
 
-// btGjkEpaSolver2::Penetration (with margins)
-RSG_FN bool g2_penetration(const CShape& s0, const Xf& t0, const CShape& s1, const Xf& t1, V3 guess,
-                           G2Results& res, SV* pool, EpaFace* faces, Epa& epa) {
-    Gjk2 gjk;
-    g2_init_shape(gjk.shape, s0, t0, s1, t1, true);
-    gjk.sv = pool;
+// Penetration of the shapes with their margins (btGjkEpaSolver2::Penetration):
+// GJK, then the polytope when the origin is inside the difference.
+RSG_FN bool penetration(const CShape& s0, const Xf& t0, const CShape& s1, const Xf& t1, V3 guess, PenResult& res,
+                        SupportPt* pool, PolyFace* faces, Polytope& poly) {
+    GjkQuery gjk;
+    mink_init(gjk.md, s0, t0, s1, t1, true);
+    gjk.pool = pool;
     res.witness[0] = res.witness[1] = v3zero();
-    int gs = gjk.evaluate(-guess);
-    if (gs == 1) {
-        epa.init(pool, faces);  // EPA vertices follow the four GJK entries of the pool
-        int es = epa.evaluate(gjk, -guess);
-        if (es != 9) {
-            V3 w0 = v3zero();
-            for (int i = 0; i < epa.result.rank; i++)
-                w0 += gjk.shape.support(pool[epa.result.c[i]].d, 0) * epa.result.p[i];
-            res.witness[0] = xf_apply(t0, w0);
-            res.witness[1] = xf_apply(t0, w0 - epa.normal * epa.depth);
-            res.normal = -epa.normal;
-            res.distance = -epa.depth;
-            return true;
-        }
-    }
-    return false;
+    if (gjk.run(-guess) != 1) return false;
+    poly.reset(pool, faces);
+    if (poly.expand(gjk, -guess) == 9) return false;
+    V3 w0 = v3zero();
+    for (int k = 0; k < poly.result.n; k++) w0 += gjk.md.support(pool[poly.result.idx[k]].dir, 0) * poly.result.bary[k];
+    res.witness[0] = xf_apply(t0, w0);
+    res.witness[1] = xf_apply(t0, w0 - poly.normal * poly.depth);
+    res.normal = -poly.normal;
+    res.distance = -poly.depth;
+    return true;
 }
 
-// btGjkEpaSolver2::Distance (without margins)
-RSG_FN bool g2_distance(const CShape& s0, const Xf& t0, const CShape& s1, const Xf& t1, V3 guess,
-                        G2Results& res, SV* pool) {
-    Gjk2 gjk;
-    g2_init_shape(gjk.shape, s0, t0, s1, t1, false);
-    gjk.sv = pool;
+// Distance of the shapes without margins (btGjkEpaSolver2::Distance).
+RSG_FN bool separation(const CShape& s0, const Xf& t0, const CShape& s1, const Xf& t1, V3 guess, PenResult& res,
+                       SupportPt* pool) {
+    GjkQuery gjk;
+    mink_init(gjk.md, s0, t0, s1, t1, false);
+    gjk.pool = pool;
     res.witness[0] = res.witness[1] = v3zero();
-    int gs = gjk.evaluate(guess);
-    if (gs == 0) {
-        V3 w0 = v3zero(), w1 = v3zero();
-        for (int i = 0; i < gjk.simplex->rank; i++) {
-            float p = gjk.simplex->p[i];
-            w0 += gjk.shape.support(pool[gjk.simplex->c[i]].d, 0) * p;
-            w1 += gjk.shape.support(-pool[gjk.simplex->c[i]].d, 1) * p;
-        }
-        res.witness[0] = xf_apply(t0, w0);
-        res.witness[1] = xf_apply(t0, w1);
-        res.normal = w0 - w1;
-        res.distance = len(res.normal);
-        res.normal = res.normal * (1.f / (res.distance > G2_MIN_DISTANCE ? res.distance : 1.f));
-        return true;
+    if (gjk.run(guess) != 0) return false;
+    V3 w0 = v3zero(), w1 = v3zero();
+    for (int k = 0; k < gjk.result->n; k++) {
+        const float b = gjk.result->bary[k];
+        w0 += gjk.md.support(pool[gjk.result->idx[k]].dir, 0) * b;
+        w1 += gjk.md.support(-pool[gjk.result->idx[k]].dir, 1) * b;
     }
-    return false;
+    res.witness[0] = xf_apply(t0, w0);
+    res.witness[1] = xf_apply(t0, w1);
+    res.normal = w0 - w1;
+    res.distance = len(res.normal);
+    res.normal = res.normal * (1.f / (res.distance > GJK_EPS ? res.distance : 1.f));
+    return true;
 }
 
-// Work space of a penetration query: the GJK/EPA vertex pool and EPA's face
-// pool.  The caller places it (private memory, or a per-arena block in global
-// memory that a lane reads without the per-lane interleaving of private
-// memory).
+// Work space of a penetration query: the support-point pool and the
+// polytope's face pool.  The caller places it (private memory, or a
+// per-arena block in global memory that a lane reads without the per-lane
+// interleaving of private memory).
 struct EpaSpace {
-    SV pool[SV_POOL];
-    EpaFace fc[EPA_MAX_FACES];
+    SupportPt pool[SUPPORT_POOL];
+    PolyFace faces[POLY_MAX_FACES];
 };
 
-// btGjkEpaPenetrationDepthSolver::calcPenDepth
+// Penetration depth with the reference's nine guess directions
+// (btGjkEpaPenetrationDepthSolver::calcPenDepth): the first guess for which
+// the penetration query succeeds gives the contact; a guess whose distance
+// query succeeds instead reports the shapes apart.
 RSG_FN bool calc_pen_depth(const CShape& a, const CShape& b, const Xf& ta, const Xf& tb, V3& v, V3& wa, V3& wb,
                            EpaSpace& space) {
-    SV* pool = space.pool;
-    Epa epa;
-    V3 guesses[9] = {safe_normalized(tb.t - ta.t), safe_normalized(ta.t - tb.t), v3(0, 0, 1), v3(0, 1, 0),
-                     v3(1, 0, 0), v3(1, 1, 0), v3(1, 1, 1), v3(0, 1, 1), v3(1, 0, 1)};
+    Polytope poly;
+    const V3 guesses[9] = {safe_normalized(tb.t - ta.t), safe_normalized(ta.t - tb.t), v3(0, 0, 1), v3(0, 1, 0),
+                           v3(1, 0, 0), v3(1, 1, 0), v3(1, 1, 1), v3(0, 1, 1), v3(1, 0, 1)};
     for (int i = 0; i < 9; i++) {
-        G2Results r;
-        if (g2_penetration(a, ta, b, tb, guesses[i], r, pool, space.fc, epa)) {
+        PenResult r;
+        if (penetration(a, ta, b, tb, guesses[i], r, space.pool, space.faces, poly)) {
             wa = r.witness[0]; wb = r.witness[1]; v = r.normal;
             return true;
         }
-        if (g2_distance(a, ta, b, tb, guesses[i], r, pool)) {
+        if (separation(a, ta, b, tb, guesses[i], r, space.pool)) {
             wa = r.witness[0]; wb = r.witness[1]; v = r.normal;
             return false;
         }

@@ -50,7 +50,10 @@ def _seed_for(ref, teams, presets, key, cache):
     if tk not in cache:
         cache[tk] = {}
         A = parity.pooled_arena(ref, MODE_SOCCAR, teams, presets)
-        for seed in range(4000):
+        # seeds spread over the int range: std::default_random_engine seeded
+        # with small integers never draws the first two slots first
+        for k in range(4000):
+            seed = (k * 2654435761 + 12345) % 2147483647
             A.reset_kickoff(seed)
             cache[tk].setdefault(_slot_key([A.get_car(i) for i in range(len(teams))], teams), seed)
     return cache[tk].get(key)
