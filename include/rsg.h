@@ -140,6 +140,21 @@ int rsg_batch_get_goals(rsg_batch* b, uint32_t* blue, uint32_t* orange, uint64_t
  * for their work and return -1, naming the first such arena in
  * rsg_last_error(), as the reference's Arena::Step would throw. */
 int rsg_batch_get_status(rsg_batch* b, uint32_t* per_arena, uint32_t* any);
+/* Device-side summary of the same words (no counterpart in the reference):
+ * enqueues on `stream`, after the batch's last step, kernels that write the
+ * OR of every arena's status word to out[0] and the number of arenas with a
+ * bit set to out[1]; `out` is two words of device memory.  No host round
+ * trip, so a stepping loop can test it on the device.  Allocates its
+ * partial-result buffer (two words per 256 arenas) on the first call. */
+int rsg_batch_status_summary(rsg_batch* b, uint32_t* out, void* stream);
+/* rsg_batch_status_summary without any allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops).  Same result; the partial results go
+ * to a file-scope device array sized for RSG_STATUS_NOALLOC_MAX_ARENAS
+ * arenas, and a larger batch is refused with an error.  Every call on a GPU
+ * shares that array: calls that overlap on one GPU race. */
+enum { RSG_STATUS_NOALLOC_MAX_ARENAS = 1 << 17 };
+int rsg_batch_status_summary_noalloc(rsg_batch* b, uint32_t* out, void* stream);
 int rsg_batch_clear_status(rsg_batch* b);
 int rsg_batch_set_strict(rsg_batch* b, int strict);
 

@@ -139,34 +139,196 @@ private:
     rsg_batch* h_ = nullptr;
 };
 
-// A single arena with the reference's method names (src/Sim/Arena/Arena.h),
-// backed by a batch of one.  Cars are fixed at creation, as a batch's are.
-class Arena {
+class Arena;
+
+// A car of an Arena (src/Sim/Car/Car.h): `controls` is read at the next Step,
+// as the reference's public Car::controls is.
+class Car {
 public:
-    static std::unique_ptr<Arena> Create(GameMode mode, const std::vector<CarSpec>& cars, uint64_t seed = 0) {
-        BatchConfig cfg;
-        cfg.mode = mode;
-        cfg.seed = seed;
-        return std::unique_ptr<Arena>(new Arena(ArenaBatch({cars}, cfg)));
-    }
-    void Step(int ticksToSimulate = 1) { b_.Step(ticksToSimulate); b_.Sync(); }
-    CarState GetCarState(int car) { return b_.GetCarStates().at((size_t)car); }
-    void SetCarState(int car, const CarState& s) { auto v = b_.GetCarStates(); v.at((size_t)car) = s; b_.SetCarStates(v); }
-    void SetCarControls(int car, const CarControls& c) { auto v = b_.GetControls(); v.at((size_t)car) = c; b_.SetControls(v); }
-    BallState GetBallState() { return b_.GetBallStates()[0]; }
-    void SetBallState(const BallState& s) { b_.SetBallStates({s}); }
-    std::vector<BoostPadState> GetBoostPadStates() { return b_.GetPadStates(); }
-    uint64_t TickCount() { return b_.GetTickCounts()[0]; }
-    void ResetToRandomKickoff(int seed = -1) { b_.ResetToRandomKickoff(0, (uint64_t)(int64_t)seed); }
-    bool IsBallScored() { return b_.IsBallScored()[0] != 0; }
-    std::vector<uint8_t> Serialize() { return b_.Serialize(0); }
-    void Deserialize(const std::vector<uint8_t>& bytes) { b_.Deserialize(0, bytes); }
-    ArenaBatch& Batch() { return b_; }
+    CarControls controls{};
+    uint32_t id = 0;
+    Team team = Team::BLUE;
+    int preset = RSG_PRESET_OCTANE;
+    inline CarState GetState();
+    inline void SetState(const CarState& s);
 
 private:
-    explicit Arena(ArenaBatch&& b) : b_(std::move(b)) {}
-    ArenaBatch b_;
+    friend class Arena;
+    Arena* arena_ = nullptr;
 };
+
+// The ball of an Arena (src/Sim/Ball/Ball.h).
+class Ball {
+public:
+    inline BallState GetState();
+    inline void SetState(const BallState& s);
+
+private:
+    friend class Arena;
+    Arena* arena_ = nullptr;
+};
+
+// A single arena with the reference's object API (src/Sim/Arena/Arena.h):
+// Create, AddCar / RemoveCar / GetCar / GetCars, car->controls,
+// car->GetState / SetState, ball->GetState / SetState, Step, Clone, kickoff
+// and snapshots, backed by a batch of one.  A batch's car count is fixed, so
+// adding or removing a car rebuilds the batch and carries every other state
+// over (ids included).  The index-based methods of earlier versions stay.
+class Arena {
+public:
+    static std::unique_ptr<Arena> Create(GameMode mode, uint64_t seed = 0) {
+        return std::unique_ptr<Arena>(new Arena(mode, seed, {}));
+    }
+    static std::unique_ptr<Arena> Create(GameMode mode, const std::vector<CarSpec>& cars, uint64_t seed = 0) {
+        return std::unique_ptr<Arena>(new Arena(mode, seed, cars));
+    }
+
+    // Arena::AddCar (Arena.cpp:50-62): the new car gets the next id and a
+    // kickoff spawn slot (the reference respawns it at a random respawn slot).
+    Car* AddCar(Team team, int preset = RSG_PRESET_OCTANE) {
+        std::vector<CarSpec> specs = specs_;
+        specs.push_back({team, preset});
+        rebuild(specs, -1);
+        return cars_.back().get();
+    }
+    // Arena::RemoveCar (Arena.cpp:80-100): false when no car has that id.
+    bool RemoveCar(uint32_t id) {
+        for (size_t i = 0; i < cars_.size(); i++) {
+            if (cars_[i]->id != id) continue;
+            std::vector<CarSpec> specs = specs_;
+            specs.erase(specs.begin() + (long)i);
+            rebuild(specs, (int)i);
+            return true;
+        }
+        return false;
+    }
+    Car* GetCar(uint32_t id) {
+        for (auto& c : cars_)
+            if (c->id == id) return c.get();
+        return nullptr;
+    }
+    std::vector<Car*> GetCars() {
+        std::vector<Car*> v;
+        for (auto& c : cars_) v.push_back(c.get());
+        return v;
+    }
+    Ball* ball = &ball_;
+
+    // Arena::Step: every car's `controls` go to the batch, then `ticksToSimulate` ticks.
+    void Step(int ticksToSimulate = 1) {
+        if (!cars_.empty()) {
+            std::vector<CarControls> c;
+            for (auto& car : cars_) c.push_back(car->controls);
+            b_->SetControls(c);
+        }
+        b_->Step(ticksToSimulate);
+        b_->Sync();
+    }
+    // Arena::Clone (Arena.cpp:646-687): a new arena in the same state.
+    std::unique_ptr<Arena> Clone() {
+        std::unique_ptr<Arena> a(new Arena(mode_, seed_, specs_));
+        a->Deserialize(Serialize());
+        for (size_t i = 0; i < cars_.size(); i++) a->cars_[i]->controls = cars_[i]->controls;
+        return a;
+    }
+
+    CarState GetCarState(int car) { return b_->GetCarStates().at((size_t)car); }
+    void SetCarState(int car, const CarState& s) { auto v = b_->GetCarStates(); v.at((size_t)car) = s; b_->SetCarStates(v); }
+    void SetCarControls(int car, const CarControls& c) { cars_.at((size_t)car)->controls = c; }
+    BallState GetBallState() { return b_->GetBallStates()[0]; }
+    void SetBallState(const BallState& s) { b_->SetBallStates({s}); }
+    std::vector<BoostPadState> GetBoostPadStates() { return b_->GetPadStates(); }
+    uint64_t TickCount() { return b_->GetTickCounts()[0]; }
+    void ResetToRandomKickoff(int seed = -1) { b_->ResetToRandomKickoff(0, (uint64_t)(int64_t)seed); }
+    bool IsBallScored() { return b_->IsBallScored()[0] != 0; }
+    std::vector<uint8_t> Serialize() { return b_->Serialize(0); }
+    void Deserialize(const std::vector<uint8_t>& bytes) { b_->Deserialize(0, bytes); }
+    ArenaBatch& Batch() { return *b_; }
+
+private:
+    friend class Car;
+    friend class Ball;
+    Arena(GameMode mode, uint64_t seed, const std::vector<CarSpec>& cars) : mode_(mode), seed_(seed) {
+        ball_.arena_ = this;
+        rebuild(cars, -2);
+    }
+    int slot_of(const Car* c) const {
+        for (size_t i = 0; i < cars_.size(); i++)
+            if (cars_[i].get() == c) return (int)i;
+        throw std::runtime_error("rsg: car does not belong to this arena");
+    }
+    // A new batch of one for `specs`, carrying the old batch's state over:
+    // removed < 0 adds cars at the end (-1) or starts fresh (-2); otherwise
+    // car `removed` is dropped and the others keep their ids (through a
+    // snapshot of the old batch).
+    void rebuild(const std::vector<CarSpec>& specs, int removed) {
+        BatchConfig cfg;
+        cfg.mode = mode_;
+        cfg.seed = seed_;
+        auto nb = std::unique_ptr<ArenaBatch>(new ArenaBatch({specs}, cfg));
+        std::vector<std::unique_ptr<Car>> cars;
+        if (removed == -1 && b_) {
+            auto old_cars = b_->GetCarStates();
+            auto cur = nb->GetCarStates();
+            for (size_t i = 0; i < old_cars.size(); i++) cur[i] = old_cars[i];
+            nb->SetCarStates(cur);
+            nb->SetBallStates(b_->GetBallStates());
+            nb->SetPadStates(b_->GetPadStates());
+            nb->SetTickCounts(b_->GetTickCounts());
+        } else if (removed >= 0 && b_) {
+            rsg_arena_snapshot snap;
+            check(rsg_batch_get_snapshot(b_->handle(), 0, &snap));
+            const uint32_t gone = cars_[(size_t)removed]->id;
+            int k = 0;
+            for (int i = 0; i < snap.n_cars; i++) {
+                if (snap.car_id[i] == gone) continue;
+                snap.car_id[k] = snap.car_id[i];
+                snap.car_team[k] = snap.car_team[i];
+                snap.car_preset[k] = snap.car_preset[i];
+                snap.controls[k] = snap.controls[i];
+                snap.cars[k] = snap.cars[i];
+                k++;
+            }
+            snap.n_cars = k;
+            snap.raw_words = 0;  // the exact records are per car slot; restore from the public state
+            check(rsg_batch_set_snapshot(nb->handle(), 0, &snap));
+        }
+        for (size_t i = 0; i < specs.size(); i++) {
+            std::unique_ptr<Car> c(new Car());
+            c->arena_ = this;
+            c->team = specs[i].team;
+            c->preset = specs[i].preset;
+            cars.push_back(std::move(c));
+        }
+        // ids: carried over from the old cars; new cars continue the count
+        uint32_t next = last_id_;
+        size_t j = 0;
+        for (size_t i = 0; i < cars_.size(); i++) {
+            if ((int)i == removed) continue;
+            cars[j]->id = cars_[i]->id;
+            cars[j]->controls = cars_[i]->controls;
+            j++;
+        }
+        for (; j < cars.size(); j++) cars[j]->id = ++next;
+        last_id_ = next;
+        b_ = std::move(nb);
+        cars_ = std::move(cars);
+        specs_ = specs;
+    }
+
+    GameMode mode_;
+    uint64_t seed_;
+    std::vector<CarSpec> specs_;
+    std::unique_ptr<ArenaBatch> b_;
+    std::vector<std::unique_ptr<Car>> cars_;
+    Ball ball_;
+    uint32_t last_id_ = 0;
+};
+
+inline CarState Car::GetState() { return arena_->GetCarState(arena_->slot_of(this)); }
+inline void Car::SetState(const CarState& s) { arena_->SetCarState(arena_->slot_of(this), s); }
+inline BallState Ball::GetState() { return arena_->GetBallState(); }
+inline void Ball::SetState(const BallState& s) { arena_->SetBallState(s); }
 
 }  // namespace rsg_amd
 

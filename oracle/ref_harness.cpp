@@ -22,6 +22,7 @@
 #include "Sim/GameEventTracker/GameEventTracker.h"
 #include "rsg_state.h"
 #include "rsg_workload.h"
+#include "rsg_rocketsim.hpp"
 #include "../libsrc/bullet3-3.24/BulletCollision/BroadphaseCollision/btRSBroadphase.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btBvhTriangleMeshShape.h"
 #include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btBoxShape.h"
@@ -95,24 +96,7 @@ void track_bp(RefArena* ra) {
 
 thread_local std::string g_err;
 
-void vset(Vec& v, const float* f) { v = Vec(f[0], f[1], f[2]); }
 void vget(const Vec& v, float* f) { f[0] = v.x; f[1] = v.y; f[2] = v.z; }
-void rset(RotMat& m, const float* f) {
-    m.forward = Vec(f[0], f[1], f[2]);
-    m.right = Vec(f[3], f[4], f[5]);
-    m.up = Vec(f[6], f[7], f[8]);
-}
-void rget(const RotMat& m, float* f) {
-    vget(m.forward, f); vget(m.right, f + 3); vget(m.up, f + 6);
-}
-void cset(CarControls& c, const rsg_controls& s) {
-    c.throttle = s.throttle; c.steer = s.steer; c.pitch = s.pitch; c.yaw = s.yaw; c.roll = s.roll;
-    c.jump = s.jump; c.boost = s.boost; c.handbrake = s.handbrake;
-}
-void cget(const CarControls& c, rsg_controls& s) {
-    s.throttle = c.throttle; s.steer = c.steer; s.pitch = c.pitch; s.yaw = c.yaw; s.roll = c.roll;
-    s.jump = c.jump; s.boost = c.boost; s.handbrake = c.handbrake; s._pad = 0;
-}
 
 }  // namespace
 
@@ -199,150 +183,43 @@ int rsref_num_cars(void* h) { return (int)((RefArena*)h)->cars.size(); }
 uint32_t rsref_car_id(void* h, int i) { return ((RefArena*)h)->cars[i]->id; }
 
 void rsref_set_controls(void* h, int i, const rsg_controls* c) {
-    cset(((RefArena*)h)->cars[i]->controls, *c);
+    ((RefArena*)h)->cars[i]->controls = rsg_amd::FromRsg(*c);
 }
 
 void rsref_get_car(void* h, int i, rsg_car_state* s) {
-    Car* car = ((RefArena*)h)->cars[i];
-    CarState cs = car->GetState();
-    std::memset(s, 0, sizeof(*s));
-    s->tick_count_since_update = cs.tickCountSinceUpdate;
-    vget(cs.pos, s->pos); rget(cs.rotMat, s->rot); vget(cs.vel, s->vel); vget(cs.angVel, s->ang_vel);
-    uint32_t f = 0;
-    if (cs.isOnGround) f |= RSG_CF_ON_GROUND;
-    for (int w = 0; w < 4; w++) if (cs.wheelsWithContact[w]) f |= (RSG_CF_WHEEL0 << w);
-    if (cs.hasJumped) f |= RSG_CF_HAS_JUMPED;
-    if (cs.hasDoubleJumped) f |= RSG_CF_HAS_DOUBLE_JUMPED;
-    if (cs.hasFlipped) f |= RSG_CF_HAS_FLIPPED;
-    if (cs.isFlipping) f |= RSG_CF_IS_FLIPPING;
-    if (cs.isJumping) f |= RSG_CF_IS_JUMPING;
-    if (cs.isBoosting) f |= RSG_CF_IS_BOOSTING;
-    if (cs.isSupersonic) f |= RSG_CF_SUPERSONIC;
-    if (cs.isAutoFlipping) f |= RSG_CF_AUTO_FLIPPING;
-    if (cs.worldContact.hasContact) f |= RSG_CF_WORLD_CONTACT;
-    if (cs.isDemoed) f |= RSG_CF_DEMOED;
-    if (cs.ballHitInfo.isValid) f |= RSG_CF_HIT_VALID;
-    if (car->_rigidBody.m_collisionFlags & btCollisionObject::CF_NO_CONTACT_RESPONSE) f |= RSG_CF_BODY_DISABLED;
-    s->flags = f;
-    vget(cs.flipRelTorque, s->flip_rel_torque);
-    s->jump_time = cs.jumpTime; s->flip_time = cs.flipTime;
-    s->air_time = cs.airTime; s->air_time_since_jump = cs.airTimeSinceJump;
-    s->boost = cs.boost; s->time_since_boosted = cs.timeSinceBoosted; s->boosting_time = cs.boostingTime;
-    s->supersonic_time = cs.supersonicTime; s->handbrake_val = cs.handbrakeVal;
-    s->auto_flip_timer = cs.autoFlipTimer; s->auto_flip_torque_scale = cs.autoFlipTorqueScale;
-    vget(cs.worldContact.contactNormal, s->world_contact_normal);
-    s->car_contact_other_id = cs.carContact.otherCarID;
-    s->car_contact_cooldown = cs.carContact.cooldownTimer;
-    s->demo_respawn_timer = cs.demoRespawnTimer;
-    vget(cs.ballHitInfo.relativePosOnBall, s->hit_rel_pos);
-    vget(cs.ballHitInfo.ballPos, s->hit_ball_pos);
-    vget(cs.ballHitInfo.extraHitVel, s->hit_extra_vel);
-    s->hit_tick = cs.ballHitInfo.tickCountWhenHit;
-    s->hit_extra_tick = cs.ballHitInfo.tickCountWhenExtraImpulseApplied;
-    cget(cs.lastControls, s->last_controls);
-    auto& wi = car->_bulletVehicle.m_wheelInfo;
-    s->wheel_engine_force = wi[0].m_engineForce;
-    s->wheel_brake = wi[0].m_brake;
-    s->wheel_steer = wi[0].m_steerAngle;
-    for (int w = 0; w < 4; w++) {
-        s->wheel_lat_friction[w] = wi[w].m_latFriction;
-        s->wheel_long_friction[w] = wi[w].m_longFriction;
-        s->wheel_extra_pushback[w] = wi[w].m_extraPushback;
-    }
+    // the public type adapter (include/rsg_rocketsim.hpp), plus the
+    // broadphase record this harness tracks
     auto* ra = (RefArena*)h;
+    *s = rsg_amd::ToRsg(*ra->cars[i]);
     s->bp_cell = ra->bp_cell[1 + i];
     s->bp_seq = ra->bp_seq[1 + i];
 }
 
 void rsref_set_car(void* h, int i, const rsg_car_state* s) {
-    Car* car = ((RefArena*)h)->cars[i];
-    CarState cs;
-    vset(cs.pos, s->pos); rset(cs.rotMat, s->rot); vset(cs.vel, s->vel); vset(cs.angVel, s->ang_vel);
-    uint32_t f = s->flags;
-    cs.isOnGround = f & RSG_CF_ON_GROUND;
-    for (int w = 0; w < 4; w++) cs.wheelsWithContact[w] = f & (RSG_CF_WHEEL0 << w);
-    cs.hasJumped = f & RSG_CF_HAS_JUMPED;
-    cs.hasDoubleJumped = f & RSG_CF_HAS_DOUBLE_JUMPED;
-    cs.hasFlipped = f & RSG_CF_HAS_FLIPPED;
-    cs.isFlipping = f & RSG_CF_IS_FLIPPING;
-    cs.isJumping = f & RSG_CF_IS_JUMPING;
-    cs.isBoosting = f & RSG_CF_IS_BOOSTING;
-    cs.isSupersonic = f & RSG_CF_SUPERSONIC;
-    cs.isAutoFlipping = f & RSG_CF_AUTO_FLIPPING;
-    cs.worldContact.hasContact = f & RSG_CF_WORLD_CONTACT;
-    cs.isDemoed = f & RSG_CF_DEMOED;
-    cs.ballHitInfo.isValid = f & RSG_CF_HIT_VALID;
-    vset(cs.flipRelTorque, s->flip_rel_torque);
-    cs.jumpTime = s->jump_time; cs.flipTime = s->flip_time;
-    cs.airTime = s->air_time; cs.airTimeSinceJump = s->air_time_since_jump;
-    cs.boost = s->boost; cs.timeSinceBoosted = s->time_since_boosted; cs.boostingTime = s->boosting_time;
-    cs.supersonicTime = s->supersonic_time; cs.handbrakeVal = s->handbrake_val;
-    cs.autoFlipTimer = s->auto_flip_timer; cs.autoFlipTorqueScale = s->auto_flip_torque_scale;
-    vset(cs.worldContact.contactNormal, s->world_contact_normal);
-    cs.carContact.otherCarID = s->car_contact_other_id;
-    cs.carContact.cooldownTimer = s->car_contact_cooldown;
-    cs.demoRespawnTimer = s->demo_respawn_timer;
-    vset(cs.ballHitInfo.relativePosOnBall, s->hit_rel_pos);
-    vset(cs.ballHitInfo.ballPos, s->hit_ball_pos);
-    vset(cs.ballHitInfo.extraHitVel, s->hit_extra_vel);
-    cs.ballHitInfo.tickCountWhenHit = s->hit_tick;
-    cs.ballHitInfo.tickCountWhenExtraImpulseApplied = s->hit_extra_tick;
-    cset(cs.lastControls, s->last_controls);
-    car->SetState(cs);
-    car->_internalState.tickCountSinceUpdate = s->tick_count_since_update;
-    if (f & RSG_CF_BODY_DISABLED) {
-        car->_rigidBody.m_activationState1 = DISABLE_SIMULATION;
-        car->_rigidBody.m_collisionFlags |= btCollisionObject::CF_NO_CONTACT_RESPONSE;
-    } else {
-        car->_rigidBody.m_activationState1 = ACTIVE_TAG;
-        car->_rigidBody.m_collisionFlags &= ~btCollisionObject::CF_NO_CONTACT_RESPONSE;
-    }
-    auto& wi = car->_bulletVehicle.m_wheelInfo;
-    for (int w = 0; w < 4; w++) {
-        wi[w].m_engineForce = s->wheel_engine_force;
-        wi[w].m_brake = s->wheel_brake;
-        wi[w].m_steerAngle = (w < 2) ? s->wheel_steer : 0.f;
-        wi[w].m_latFriction = s->wheel_lat_friction[w];
-        wi[w].m_longFriction = s->wheel_long_friction[w];
-        wi[w].m_extraPushback = s->wheel_extra_pushback[w];
-    }
+    rsg_amd::ApplyRsg(*s, *((RefArena*)h)->cars[i]);
 }
 
 void rsref_get_ball(void* h, rsg_ball_state* s) {
-    BallState bs = ((RefArena*)h)->arena->ball->GetState();
-    std::memset(s, 0, sizeof(*s));
-    s->tick_count_since_update = bs.tickCountSinceUpdate;
-    vget(bs.pos, s->pos); rget(bs.rotMat, s->rot); vget(bs.vel, s->vel); vget(bs.angVel, s->ang_vel);
     auto* ra = (RefArena*)h;
+    *s = rsg_amd::ToRsg(ra->arena->ball->GetState());
     s->bp_cell = ra->bp_cell[0];
     s->bp_seq = ra->bp_seq[0];
-    s->hs_y_target_dir = bs.hsInfo.yTargetDir;
-    s->hs_cur_target_speed = bs.hsInfo.curTargetSpeed;
-    s->hs_time_since_hit = bs.hsInfo.timeSinceHit;
 }
 
 void rsref_set_ball(void* h, const rsg_ball_state* s) {
     Ball* b = ((RefArena*)h)->arena->ball;
-    BallState bs;
-    vset(bs.pos, s->pos); rset(bs.rotMat, s->rot); vset(bs.vel, s->vel); vset(bs.angVel, s->ang_vel);
-    bs.hsInfo.yTargetDir = s->hs_y_target_dir;
-    bs.hsInfo.curTargetSpeed = s->hs_cur_target_speed;
-    bs.hsInfo.timeSinceHit = s->hs_time_since_hit;
-    b->SetState(bs);
+    b->SetState(rsg_amd::FromRsg(*s));
     b->_internalState.tickCountSinceUpdate = s->tick_count_since_update;
 }
 
 int rsref_num_pads(void* h) { return (int)((RefArena*)h)->arena->_boostPads.size(); }
 
 void rsref_get_pad(void* h, int i, rsg_pad_state* s) {
-    BoostPadState ps = ((RefArena*)h)->arena->_boostPads[i]->GetState();
-    s->is_active = ps.isActive; s->cooldown = ps.cooldown; s->prev_locked_car_id = ps.prevLockedCarID;
+    *s = rsg_amd::ToRsg(((RefArena*)h)->arena->_boostPads[i]->GetState());
 }
 
 void rsref_set_pad(void* h, int i, const rsg_pad_state* s) {
-    BoostPadState ps;
-    ps.isActive = s->is_active != 0; ps.cooldown = s->cooldown; ps.prevLockedCarID = s->prev_locked_car_id;
-    ((RefArena*)h)->arena->_boostPads[i]->SetState(ps);
+    ((RefArena*)h)->arena->_boostPads[i]->SetState(rsg_amd::FromRsg(*s));
 }
 
 uint64_t rsref_get_tick_count(void* h) { return ((RefArena*)h)->arena->tickCount; }
@@ -415,9 +292,9 @@ double rsref_run_workload(int threads, int arenas_per_thread, int n_blue, int n_
                     for (size_t i = 0; i < arenas[a].cars.size(); i++) {
                         rsg_controls c;
                         // Keep the previous draw unless a redraw is due.
-                        cget(arenas[a].cars[i]->controls, c);
+                        c = rsg_amd::ToRsg(arenas[a].cars[i]->controls);
                         rsg_ctrl_next(seed, arena_id, (uint32_t)i, &gens[a][i], &c);
-                        cset(arenas[a].cars[i]->controls, c);
+                        arenas[a].cars[i]->controls = rsg_amd::FromRsg(c);
                     }
                     arenas[a].arena->Step(1);
                     if (arenas[a].arena->IsBallScored())
@@ -853,24 +730,9 @@ extern "C" int rsref_pop_events(void* h, rsg_event* out, int max_out) {
 // ---------------------------------------------------------------------------
 // BallPredTracker (src/Sim/BallPredTracker/BallPredTracker.h:7-36).
 
-static BallState ball_state_from(const rsg_ball_state* s) {
-    BallState bs;
-    vset(bs.pos, s->pos); rset(bs.rotMat, s->rot); vset(bs.vel, s->vel); vset(bs.angVel, s->ang_vel);
-    bs.tickCountSinceUpdate = s->tick_count_since_update;
-    bs.hsInfo.yTargetDir = s->hs_y_target_dir;
-    bs.hsInfo.curTargetSpeed = s->hs_cur_target_speed;
-    bs.hsInfo.timeSinceHit = s->hs_time_since_hit;
-    return bs;
-}
+static BallState ball_state_from(const rsg_ball_state* s) { return rsg_amd::FromRsg(*s); }
 
-static void ball_state_to(const BallState& bs, rsg_ball_state* s) {
-    std::memset(s, 0, sizeof(*s));
-    s->tick_count_since_update = bs.tickCountSinceUpdate;
-    vget(bs.pos, s->pos); rget(bs.rotMat, s->rot); vget(bs.vel, s->vel); vget(bs.angVel, s->ang_vel);
-    s->hs_y_target_dir = bs.hsInfo.yTargetDir;
-    s->hs_cur_target_speed = bs.hsInfo.curTargetSpeed;
-    s->hs_time_since_hit = bs.hsInfo.timeSinceHit;
-}
+static void ball_state_to(const BallState& bs, rsg_ball_state* s) { *s = rsg_amd::ToRsg(bs); }
 
 // new BallPredTracker(arena, numPredTicks): predicts from the arena's ball at once.
 extern "C" void* rsref_ball_pred_create(void* h, int num_pred_ticks) {
@@ -935,7 +797,7 @@ extern "C" void* rsref_deserialize(const uint8_t* data, uint64_t size) {
 
 extern "C" uint32_t rsref_last_car_id(void* h) { return ((RefArena*)h)->arena->_lastCarID; }
 extern "C" int rsref_car_team(void* h, int i) { return (int)((RefArena*)h)->cars[i]->team; }
-extern "C" void rsref_get_controls(void* h, int i, rsg_controls* c) { cget(((RefArena*)h)->cars[i]->controls, *c); }
+extern "C" void rsref_get_controls(void* h, int i, rsg_controls* c) { *c = rsg_amd::ToRsg(((RefArena*)h)->cars[i]->controls); }
 
 // ---------------------------------------------------------------------------
 // GameEventTracker (src/Sim/GameEventTracker/GameEventTracker.h:44-106) with

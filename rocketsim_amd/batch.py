@@ -215,6 +215,12 @@ class ArenaBatch:
         native.check(self.L.rsg_batch_get_status(self.h, out, None))
         return list(out)
 
+    def status_summary(self, out, stream=None):
+        """Enqueues the device-side summary of the status words into `out`, a
+        torch uint32/int32 tensor of 2 elements on the GPU: [OR of the bits,
+        number of flagged arenas]."""
+        native.check(self.L.rsg_batch_status_summary(self.h, C.c_void_p(out.data_ptr()), stream))
+
     def clear_status(self):
         native.check(self.L.rsg_batch_clear_status(self.h))
 

@@ -26,7 +26,7 @@ HIP_FLAGS = [
 # Translation units of librsg.so: the C ABI (host code) and one file per
 # kernel family, compiled in parallel and linked into one shared library.
 NATIVE_SOURCES = ["rsg_batch.hip", "rsg_k_plain.hip", "rsg_k_lds.hip", "rsg_k_coop.hip", "rsg_k_pred.hip",
-                  "rsg_k_diff.hip", "rsg_k_events.hip"]
+                  "rsg_k_diff.hip", "rsg_k_events.hip", "rsg_k_status.hip"]
 
 
 def build_native(verbose=True, extra_flags=(), out_path=None, obj_dir=None):

@@ -101,6 +101,7 @@ struct rsg_batch {
     int arenas_per_block = 0;  // 0: derived from the device's SIMD count
     int kernel = RSG_KERNEL_AUTO;
     bool strict = false;       // fail a step that leaves an arena with a status bit set
+    uint32_t* status_partials = nullptr;  // rsg_batch_status_summary's partial results
     // captured multi-launch step (rsg_batch_graph_*)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -236,6 +237,7 @@ void rsg_batch_destroy(rsg_batch* b) {
     if (b->dev.pad) (void)hipFree(b->dev.pad);
     if (b->dev.arena) (void)hipFree(b->dev.arena);
     if (b->dev.space) (void)hipFree(b->dev.space);
+    if (b->status_partials) (void)hipFree(b->status_partials);
     if (b->dev.ev_buf) (void)hipFree(b->dev.ev_buf);
     if (b->dev.ev_count) (void)hipFree(b->dev.ev_count);
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
@@ -428,6 +430,59 @@ int rsg_batch_get_status(rsg_batch* b, uint32_t* per_arena, uint32_t* any) {
     if (per_arena) std::memcpy(per_arena, st.data(), st.size() * 4);
     if (any) *any = all;
     return flagged;
+}
+
+int rsg_batch_status_summary(rsg_batch* b, uint32_t* out, void* stream) {
+    if (!b || !out) return fail("rsg_batch_status_summary: null argument");
+    if (upload(b) != 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (b->last_stream && b->last_stream != s) HIP_OK(hipStreamSynchronize(b->last_stream));
+    if (!b->status_partials)
+        HIP_OK(hipMalloc(&b->status_partials, (size_t)(2 * status_blocks(b->dev.n) + 2) * sizeof(uint32_t)));
+    HIP_OK(launch_status_summary(b->dev.arena + (size_t)AW_STATUS * b->dev.n, b->dev.n, b->status_partials, out, s));
+    return 0;
+}
+
+// Partial results of rsg_batch_status_summary_noalloc, for batches of up to
+// RSG_STATUS_NOALLOC_MAX_ARENAS arenas (2 words per 256 arenas: 4 KiB).
+// Each GPU holds its own copy until the process ends; host code reaches it
+// only through the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) uint32_t scratch[2 * (RSG_STATUS_NOALLOC_MAX_ARENAS / STATUS_BLOCK)];
+
+namespace {
+std::mutex g_scratch_mutex;
+uint32_t* g_scratch_ptr[64] = {};  // per device
+
+// The address of `scratch` on the current device, looked up once per device.
+int scratch_on_device(uint32_t** out) {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail("rsg_batch_status_summary_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (uint32_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+int rsg_batch_status_summary_noalloc(rsg_batch* b, uint32_t* out, void* stream) {
+    if (!b || !out) return fail("rsg_batch_status_summary_noalloc: null argument");
+    if (b->dev.n > RSG_STATUS_NOALLOC_MAX_ARENAS)
+        return fail("rsg_batch_status_summary_noalloc: " + std::to_string(b->dev.n) +
+                    " arenas exceed RSG_STATUS_NOALLOC_MAX_ARENAS (" + std::to_string(RSG_STATUS_NOALLOC_MAX_ARENAS) +
+                    "); use rsg_batch_status_summary");
+    uint32_t* partials = nullptr;
+    if (scratch_on_device(&partials) != 0) return -1;
+    if (upload(b) != 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (b->last_stream && b->last_stream != s) HIP_OK(hipStreamSynchronize(b->last_stream));
+    // every partial slot the final kernel reads is written by this call's first kernel
+    HIP_OK(launch_status_summary(b->dev.arena + (size_t)AW_STATUS * b->dev.n, b->dev.n, partials, out, s));
+    return 0;
 }
 
 int rsg_batch_clear_status(rsg_batch* b) {

@@ -47,4 +47,11 @@ hipError_t launch_ball_pred(const BatchDev& pa, const MeshView& mv, const PredDe
 hipError_t launch_game_events(const BatchDev& d, const GameEventDev& g, const rsg_game_event_config& cfg,
                               hipStream_t s);
 
+// Status summary (rsg_k_status.hip): the OR of the n status words and the
+// number of nonzero ones into out[0], out[1] (device memory), through
+// 2 * status_blocks(n) words of partial results.
+constexpr int STATUS_BLOCK = 256;
+inline int status_blocks(int n) { return (n + STATUS_BLOCK - 1) / STATUS_BLOCK; }
+hipError_t launch_status_summary(const uint32_t* status, int n, uint32_t* partials, uint32_t* out, hipStream_t s);
+
 }  // namespace rsg

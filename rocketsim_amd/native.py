@@ -70,6 +70,8 @@ def lib():
         L.rsg_batch_get_status.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.rsg_batch_clear_status.argtypes = [C.c_void_p]
         L.rsg_batch_set_strict.argtypes = [C.c_void_p, C.c_int]
+        L.rsg_batch_status_summary.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rsg_batch_status_summary_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.rsg_ball_pred_create.restype = C.c_void_p
         L.rsg_ball_pred_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float]
         L.rsg_ball_pred_destroy.argtypes = [C.c_void_p]
@@ -118,7 +120,8 @@ EXPORTED = [
     "rsg_shot_rollout", "rsg_shot_rollout_backward",
     "rsg_batch_get_tick_counts", "rsg_batch_set_tick_counts", "rsg_batch_get_goals",
     "rsg_batch_reset_kickoff", "rsg_batch_is_ball_scored",
-    "rsg_batch_get_status", "rsg_batch_clear_status", "rsg_batch_set_strict",
+    "rsg_batch_get_status", "rsg_batch_clear_status", "rsg_batch_set_strict", "rsg_batch_status_summary",
+    "rsg_batch_status_summary_noalloc",
     "rsg_ball_pred_create", "rsg_ball_pred_destroy", "rsg_ball_pred_force_update", "rsg_ball_pred_update_manual",
     "rsg_ball_pred_update_from_batch", "rsg_ball_pred_get", "rsg_ball_pred_get_all",
     "rsg_ball_pred_state_for_time", "rsg_ball_pred_view", "rsg_ball_pred_set_launch",

@@ -50,6 +50,36 @@ int main(int argc, char** argv) {
         rsg_amd::CarState s1 = arena->GetCarState(0), s2 = copy->GetCarState(0);
         bool same = std::memcmp(&b1, &b2, sizeof(b1)) == 0 && std::memcmp(&s1, &s2, sizeof(s1)) == 0;
         std::printf("resume %d %zu\n", same ? 1 : 0, bytes.size());
+
+        // the reference's object API: Create, AddCar, car->controls, Step,
+        // GetCar / GetCars, RemoveCar, Clone
+        auto obj = rsg_amd::Arena::Create(rsg_amd::GameMode::SOCCAR);
+        rsg_amd::Car* blue = obj->AddCar(rsg_amd::Team::BLUE);
+        obj->AddCar(rsg_amd::Team::ORANGE, RSG_PRESET_DOMINUS);
+        obj->ResetToRandomKickoff(7);
+        rsg_amd::BallState ob = obj->ball->GetState();
+        ob.vel[0] = 300.f; ob.vel[1] = -200.f; ob.vel[2] = 650.f;
+        obj->ball->SetState(ob);
+        blue = obj->GetCar(1);
+        blue->controls.throttle = 1.f;
+        blue->controls.boost = 1;
+        obj->Step(std::atoi(argv[2]));
+        ob = obj->ball->GetState();
+        rsg_amd::CarState os = blue->GetState();
+        std::printf("objball %.6f %.6f %.6f\n", ob.pos[0], ob.pos[1], ob.pos[2]);
+        std::printf("objcar0 %.6f %.6f %.6f\n", os.pos[0], os.pos[1], os.pos[2]);
+        rsg_amd::Car* third = obj->AddCar(rsg_amd::Team::BLUE, RSG_PRESET_PLANK);
+        bool ids = obj->GetCars().size() == 3 && third->id == 3 && obj->GetCar(2) != nullptr;
+        rsg_amd::CarState after = obj->GetCar(1)->GetState();  // carried over by AddCar
+        bool kept = std::memcmp(&os, &after, sizeof(os)) == 0;
+        ids = ids && obj->RemoveCar(2) && obj->GetCars().size() == 2 && obj->GetCar(2) == nullptr &&
+              obj->GetCar(1) != nullptr && obj->GetCar(3) != nullptr;
+        auto twin = obj->Clone();
+        obj->Step(20);
+        twin->Step(20);
+        rsg_amd::BallState t1 = obj->ball->GetState(), t2 = twin->ball->GetState();
+        bool clone_same = std::memcmp(&t1, &t2, sizeof(t1)) == 0;
+        std::printf("objapi %d %d %d\n", ids ? 1 : 0, kept ? 1 : 0, clone_same ? 1 : 0);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;

@@ -376,6 +376,28 @@ def explain_outliers(samples, results, mode, bad_idx):
     return out
 
 
+# One step of the rotator rounding (Math::RoundAngleUE3, src/Math/Math.cpp:
+# 75-91): 4 of the 65536 units of a turn.
+UE3_STEP = 4 * 2 * math.pi / 65536
+
+
+def seeking_direction_rounding(ref_ball, got_ball):
+    """Whether the only difference between two ball records is a Heatseeker
+    seeking ball's direction, at most two rotator steps apart at equal speed
+    (position following from it): the seeking turn rounds the direction to
+    rotator units after two atan2f calls, and where the exact angle lies
+    within an ulp of a rounding boundary the reference's glibc atan2f and the
+    GPU's land one step apart.  The seeking words themselves must match."""
+    bad = [b[0] for b in compare_ball(ref_ball, got_ball)]
+    if not bad or set(bad) - {"ball.vel", "ball.pos"} or ref_ball.hs_y_target_dir == 0:
+        return False
+    a = [float(x) for x in ref_ball.vel]
+    b = [float(x) for x in got_ball.vel]
+    na, nb = math.sqrt(sum(x * x for x in a)), math.sqrt(sum(x * x for x in b))
+    cosang = max(-1.0, min(1.0, sum(x * y for x, y in zip(a, b)) / (na * nb)))
+    return abs(na - nb) <= 1e-4 * na and math.acos(cosang) <= 2.05 * UE3_STEP
+
+
 def assert_parity(samples, results, mode, chaotic=False):
     """Every sample within the tolerances, or (chaotic=True) at most
     CHAOTIC_FRACTION of them out, each explained by the normalisation

@@ -58,5 +58,26 @@ def test_facade_matches_python_layer(tmp_path):
         assert vals["car0"] == pytest.approx(list(car.pos), abs=1e-3)
         assert int(vals["tick"][0]) == b.get_tick_counts()[0]
         assert vals["resume"][0] == 1.0 and vals["resume"][1] > 0
+        # the object API (AddCar, car->controls, ball->SetState) steps the same arena
+        assert vals["objball"] == vals["ball"] and vals["objcar0"] == vals["car0"]
+        assert vals["objapi"] == [1.0, 1.0, 1.0]
     finally:
         b.close()
+
+
+@pytest.mark.gpu
+def test_reference_typed_program_through_facade(tmp_path):
+    """oracle/adapter_demo.cpp: a program holding RocketSim::Arena / Car /
+    CarState moves its state each tick into an rsg_amd::Arena through
+    include/rsg_rocketsim.hpp, steps both, and compares the GPU's records,
+    converted back to RocketSim types, with the reference's (built where the
+    reference sources are, by oracle/Makefile)."""
+    from rocketsim_amd import arena_mesh
+    exe = os.path.join(ROOT, "oracle", "_ref", "adapter_demo")
+    if not os.path.exists(exe):
+        pytest.skip("adapter demo not built on this machine (needs the reference sources)")
+    cmf = tmp_path / "soccar.cmf"
+    cmf.write_bytes(arena_mesh.soccar_cmf())
+    r = subprocess.run([exe, str(cmf), "120"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "120 ticks match" in r.stdout

@@ -35,27 +35,38 @@ def _oracle():
     return ref
 
 
-def _slot_key(cars, teams):
-    """The spawn positions of blue then orange cars in team order (uu,
-    rounded), i.e. the slot order a kickoff drew."""
-    blue = [tuple(round(v, 1) for v in c.pos[:2]) for c, t in zip(cars, teams) if t == 0]
-    orange = [tuple(round(v, 1) for v in c.pos[:2]) for c, t in zip(cars, teams) if t == 1]
-    return tuple(blue), tuple(orange)
+def _ranked(cars, teams, order):
+    """Cars keyed by (team, rank in the car update order): a kickoff gives the
+    r-th car of each team, in that order, the r-th slot of its draw
+    (Arena.cpp:153-195)."""
+    out, seen = {}, [0, 0]
+    for i in order:
+        out[(teams[i], seen[teams[i]])] = cars[i]
+        seen[teams[i]] += 1
+    return out
+
+
+def _slot_key(cars, teams, order):
+    """The spawn positions (uu, rounded) of each team's cars by update rank,
+    i.e. the slot order a kickoff drew."""
+    r = _ranked(cars, teams, order)
+    return tuple(sorted((k, tuple(round(v, 1) for v in c.pos[:2])) for k, c in r.items()))
 
 
 def _seed_for(ref, teams, presets, key, cache):
     """A reference seed whose ResetToRandomKickoff draws the slot order `key`
-    for cars of `teams` (in car order)."""
+    for cars of `teams`."""
     tk = (tuple(teams), tuple(presets))
     if tk not in cache:
         cache[tk] = {}
         A = parity.pooled_arena(ref, MODE_SOCCAR, teams, presets)
+        order = A.car_order()
         # seeds spread over the int range: std::default_random_engine seeded
         # with small integers never draws the first two slots first
         for k in range(4000):
             seed = (k * 2654435761 + 12345) % 2147483647
             A.reset_kickoff(seed)
-            cache[tk].setdefault(_slot_key([A.get_car(i) for i in range(len(teams))], teams), seed)
+            cache[tk].setdefault(_slot_key([A.get_car(i) for i in range(len(teams))], teams, order), seed)
     return cache[tk].get(key)
 
 
@@ -94,8 +105,9 @@ def test_goal_auto_reset_matches_reference_kickoff(per_team):
             assert tick == s.tick_out
             # the reference's kickoff with the GPU's slot order, from the
             # reference's post-tick state
-            seed = _seed_for(ref, s.teams, s.presets, _slot_key(cars, s.teams), cache)
-            assert seed is not None, f"arena {i}: GPU slot order {_slot_key(cars, s.teams)} is no kickoff"
+            key = _slot_key(cars, s.teams, s.car_order)
+            seed = _seed_for(ref, s.teams, s.presets, key, cache)
+            assert seed is not None, f"arena {i}: GPU slot order {key} is no kickoff"
             A.reset_kickoff(seed)
             kick = parity.Sample()
             kick.mode, kick.teams, kick.presets = MODE_SOCCAR, s.teams, s.presets
@@ -137,19 +149,21 @@ def test_reset_kickoff_matches_reference():
         b.step(30)
         b.reset_kickoff(-1, 12345)
         cars, balls, pads = b.get_cars(), b.get_balls(), b.get_pads()
+        orders = b.get_car_order()
         k = 0
         for a, n in enumerate(counts):
             t, p = teams[k:k + n], presets[k:k + n]
-            got = cars[k:k + n]
-            seed = _seed_for(ref, t, p, _slot_key(got, t), cache)
+            got = list(cars[k:k + n])
+            seed = _seed_for(ref, t, p, _slot_key(got, t, orders[k:k + n]), cache)
             assert seed is not None
             A = parity.pooled_arena(ref, MODE_SOCCAR, t, p)
             A.reset_kickoff(seed)
-            for j in range(n):
-                r = A.get_car(j)
-                assert max(abs(x - y) for x, y in zip(got[j].pos[:], r.pos[:])) <= 1e-3
-                assert max(abs(x - y) for x, y in zip(got[j].rot[:], r.rot[:])) <= 1e-6
-                assert (got[j].boost, got[j].flags, list(got[j].vel[:])) == (r.boost, r.flags, list(r.vel[:]))
+            ref_ranked = _ranked([A.get_car(j) for j in range(n)], t, A.car_order())
+            for key, g in _ranked(got, t, orders[k:k + n]).items():
+                r = ref_ranked[key]
+                assert max(abs(x - y) for x, y in zip(g.pos[:], r.pos[:])) <= 1e-3
+                assert max(abs(x - y) for x, y in zip(g.rot[:], r.rot[:])) <= 1e-6
+                assert (g.boost, g.flags, list(g.vel[:])) == (r.boost, r.flags, list(r.vel[:]))
             rb = A.get_ball()
             assert max(abs(x - y) for x, y in zip(balls[a].pos[:], rb.pos[:])) <= 1e-3
             assert list(balls[a].vel[:]) == list(rb.vel[:])
@@ -206,9 +220,18 @@ def test_demolished_car_respawns_like_reference():
         key = tuple(round(x, 1) for x in got.pos[:2])
         assert key in options, f"sample {i}: GPU respawn at {key}, reference's at {sorted(options)}"
         seen.add((s.teams[v], key))
-        # everything but the drawn slot as the reference's respawn on that slot
+        # everything but the drawn slot as the reference's respawn on that
+        # slot; the respawned car's broadphase cell is the one of its slot.
+        # The registration order of the bodies is left out here: the
+        # reference run that drew the GPU's slot has another history (every
+        # other parity test compares it).
+        opt = type(options[key]).from_buffer_copy(bytes(options[key]))
+        assert opt.bp_cell == got.bp_cell, f"sample {i}: cell {got.bp_cell} vs {opt.bp_cell}"
         s.cars_out = list(s.cars_out)
-        s.cars_out[v] = options[key]
+        s.cars_out[v] = opt
+        for j, c in enumerate(s.cars_out):
+            c.bp_seq = r[0][j].bp_seq
+        s.ball_out.bp_seq = r[1].bp_seq
         nbad, msgs = parity.check_samples([s], [r])
         assert nbad == 0, f"sample {i}: " + "\n".join(msgs)
     assert len(seen) >= 4  # several slots were drawn

@@ -45,29 +45,14 @@ def test_heatseeker_scenarios_exercise_the_mode():
     assert seeking > 50 and turned >= 1 and started + turned >= 1
 
 
-# One step of the rotator rounding (Math::RoundAngleUE3, src/Math/Math.cpp:
-# 75-91): 4 of 65536 units of a turn.
-UE3_STEP = 4 * 2 * math.pi / 65536
-
-
 def _one_rotator_step_apart(s, res):
     """Whether the only difference of a sample is the seeking ball's new
-    direction, one or two rotator steps apart at equal speed.  The seeking
-    turn rounds the direction to rotator units after two atan2f calls; the
-    reference's atan2f (glibc 2.35, about 1 ulp) and the GPU's round
-    differently in the last place, and where the exact angle lies within an
-    ulp of a rounding boundary the two land one step apart."""
+    direction, one or two rotator steps apart at equal speed
+    (parity.seeking_direction_rounding), every car matching."""
     cars, ball = res[0], res[1]
     if any(parity.compare_car(s.cars_out[j], cars[j]) for j in range(len(s.teams))):
         return False
-    bad = [b[0] for b in parity.compare_ball(s.ball_out, ball)]
-    if not bad or set(bad) - {"ball.vel", "ball.pos"} or s.ball_in.hs_y_target_dir == 0:
-        return False
-    a = [float(x) for x in s.ball_out.vel]
-    b = [float(x) for x in ball.vel]
-    na, nb = math.sqrt(sum(x * x for x in a)), math.sqrt(sum(x * x for x in b))
-    cosang = max(-1.0, min(1.0, sum(x * y for x, y in zip(a, b)) / (na * nb)))
-    return abs(na - nb) <= 1e-4 * na and math.acos(cosang) <= 2.05 * UE3_STEP
+    return s.ball_in.hs_y_target_dir != 0 and parity.seeking_direction_rounding(s.ball_out, ball)
 
 
 def _check(samples, results):
