@@ -70,7 +70,8 @@ int rsg_batch_sync(rsg_batch* b);
 /* Full-state access (Car::Get/SetState Car.h:151-152, Ball::Get/SetState
  * Ball.h:68-69, BoostPad::Get/SetState BoostPad.h:45-46), including the
  * per-wheel values the reference keeps in btVehicleRL between ticks.
- * Flattened arrays: cars in (arena, car) order, pads 34 per arena. */
+ * Flattened arrays: cars in (arena, car) order, pads 34 per arena.  The car
+ * and control arrays may be NULL when the batch holds no cars. */
 int rsg_batch_get_cars(rsg_batch* b, rsg_car_state* out);
 int rsg_batch_set_cars(rsg_batch* b, const rsg_car_state* in);
 int rsg_batch_get_balls(rsg_batch* b, rsg_ball_state* out);

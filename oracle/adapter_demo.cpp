@@ -91,16 +91,15 @@ int main(int argc, char** argv) {
         }
         // reference -> GPU, through the adapter
         std::vector<rsg_car_state> cs;
-        std::vector<rsg_controls> ctl;
-        for (Car* c : cars) {
-            cs.push_back(rsg_amd::ToRsg(*c));
-            ctl.push_back(rsg_amd::ToRsg(c->controls));
+        std::vector<rsg_amd::Car*> gcars = gpu->GetCars();
+        for (int i = 0; i < 4; i++) {
+            cs.push_back(rsg_amd::ToRsg(*cars[i]));
+            gcars[i]->controls = rsg_amd::ToRsg(cars[i]->controls);  // what Step hands to the batch
         }
         // broadphase records from the batch's own view of the same poses
         auto cur = gpu->Batch().GetCarStates();
         for (int i = 0; i < 4; i++) { cs[i].bp_cell = cur[i].bp_cell; cs[i].bp_seq = cur[i].bp_seq; }
         gpu->Batch().SetCarStates(cs);
-        gpu->Batch().SetControls(ctl);
         rsg_ball_state bs = rsg_amd::ToRsg(ref->ball->GetState());
         auto cur_ball = gpu->GetBallState();
         bs.bp_cell = cur_ball.bp_cell;

@@ -56,16 +56,78 @@ RSG_FN void quicksort_by_key(int* idx, const int* key, int lo, int hi) {
     }
 }
 
-// Per-arena work space of the world step: contact manifolds, solver rows
-// and the penetration query's pools (about 55 KB).  Callers place it in
-// private memory (one lane per arena) or in a per-arena block of global
-// memory, where a lane's consecutive words share cache lines (private memory
-// interleaves the lanes of a wave word by word).
+// The world step's per-arena tables: proxy bounds, cell-list order, the
+// overlapping pairs of the tick and the pairs that get a contact manifold
+// (plan[i] = pair index; manifold i belongs to it).
+struct WorldTables {
+    V3 aabb_min[MAX_BODIES], aabb_max[MAX_BODIES];
+    int order[MAX_BODIES];  // dynamic proxies in cell-list order
+    Pair pairs[MAX_PAIRS];
+    int n_pairs;
+    int plan[MAX_MANIFOLDS];
+    int n_mf;
+};
+
+// The island pass's tables (btSimulationIslandManager): union-find over the
+// dynamic bodies, the bodies sorted by island, the island manifolds.
+struct IslandTables {
+    int uf_id[MAX_BODIES], tag[MAX_BODIES];
+    int elem[MAX_BODIES], ekey[MAX_BODIES];
+    int bodies[MAX_BODIES];
+    int im[MAX_MANIFOLDS], ikey[MAX_MANIFOLDS];
+};
+
+// What the contact-added callbacks do to the arena (Arena.cpp:244-414),
+// recorded while the pairs' narrowphase runs and applied afterwards in pair
+// order, which is the order the reference's dispatcher meets the contacts
+// in.  The narrowphase reads only poses, shapes and the pre-tick contact
+// response flags, none of which these effects change, so recording them
+// changes no result.
+enum EffectKind { EFF_CAR_WORLD = 1, EFF_BALL_WORLD = 2, EFF_CAR_BALL = 3, EFF_CAR_CAR = 4 };
+struct Effect {
+    int code;   // kind | car a << 8 | car b << 16
+    V3 v0, v1;  // normal (world contacts) or local contact points
+};
+// Effects kept per pair (a host test build may lower it: RSG_TEST_PAIR_EFFECTS)
+#ifdef RSG_TEST_PAIR_EFFECTS
+constexpr int PAIR_EFFECTS = RSG_TEST_PAIR_EFFECTS;
+#else
+constexpr int PAIR_EFFECTS = 4;
+#endif
+struct PairEffects {
+    int n;
+    int redo;  // more effects than fit, or the pair needs the penetration pass: run it again in order
+    Effect e[PAIR_EFFECTS];
+};
+// One arena's record of a shared narrowphase, one entry per manifold.
+struct PairLog {
+    PairEffects eff[MAX_MANIFOLDS];
+};
+
+// Per-arena work space of the world step: contact manifolds, solver rows,
+// the penetration query's pools and the tables above (about 60 KB).
+// Callers place it in private memory (one lane per arena) or in a per-arena
+// block of global memory, where a lane's consecutive words share cache
+// lines (private memory interleaves the lanes of a wave word by word).
 struct StepSpace {
     Manifold mf[MAX_MANIFOLDS];
     SRow rows[MAX_ROWS];
     SRow fric[MAX_ROWS];
     EpaSpace epa;
+    WorldTables wt;
+    IslandTables isl;
+    Solver solver;
+};
+// A kernel that keeps one StepSpace per lane in private memory needs its
+// size times the lanes the device holds in flight (64 per wave, 1024 waves
+// at one wave per SIMD) within the 4 GiB a queue's scratch can address.
+static_assert(sizeof(StepSpace) <= 60 * 1024, "StepSpace must fit a lane's private memory");
+
+// An arena's block of a batch's global work space: the world step's work
+// space and the record of a shared narrowphase.
+struct ArenaSpace {
+    StepSpace step;
+    PairLog log;
 };
 
 struct World {
@@ -74,17 +136,20 @@ struct World {
     bool soccar;
     StepSpace* ws;  // null for worlds that only cast rays
 
-    V3 aabb_min[MAX_BODIES], aabb_max[MAX_BODIES];
-    int order[MAX_BODIES];  // dynamic proxies in cell-list order
-    int ray_order[MAX_BODIES];  // the same, as of the start of the tick (wheel rays)
+    WorldTables* T;  // ws->wt
+    int ray_order[MAX_BODIES];  // dynamic proxies in cell-list order as of the start of the tick (wheel rays)
     bool ray_order_ready;
-    Pair pairs[MAX_PAIRS];
-    int n_pairs;
     Manifold* mf;   // ws->mf
-    int n_mf;
+    // Narrowphase of one planned pair: where its callbacks' effects go (null:
+    // applied at once) and the penetration work space (null: a query that
+    // needs it marks the pair for a second, in-order run).
+    PairEffects* rec;
+    EpaSpace* epa;
+    bool aborted;
 
     RSG_FN World(ArenaLocal& a, const MeshView* m, bool s, StepSpace* w = nullptr)
-        : A(a), mesh(m), soccar(s), ws(w), ray_order_ready(false), n_pairs(0), mf(w ? w->mf : nullptr), n_mf(0) {}
+        : A(a), mesh(m), soccar(s), ws(w), T(w ? &w->wt : nullptr), ray_order_ready(false), mf(w ? w->mf : nullptr),
+          rec(nullptr), epa(w ? &w->epa : nullptr), aborted(false) {}
 
     // Whether the subsimplex ray cast against a convex of bounding radius
     // `radius` about `center` can report a hit with fraction below 1: only
@@ -148,18 +213,11 @@ struct World {
         return 1e30f;  // planes and the arena mesh have huge bounding spheres
     }
 
-    RSG_FN int new_manifold(int b0, int b1) {
-        if (n_mf >= MAX_MANIFOLDS) { A.status |= RSG_STATUS_CONTACT_OVERFLOW; return -1; }
-        Manifold& m = mf[n_mf];
+    RSG_FN void open_manifold(int mi, int b0, int b1) {
+        Manifold& m = mf[mi];
         m.b0 = b0; m.b1 = b1;
         m.threshold = rs_min(threshold_of(b0), threshold_of(b1));
         m.n = 0;
-        return n_mf++;
-    }
-    RSG_FN void release_manifold(int i) {
-        int last = n_mf - 1;
-        if (i != last) mf[i] = mf[last];
-        n_mf--;
     }
 
     // -----------------------------------------------------------------------
@@ -264,9 +322,6 @@ struct World {
             } else {
                 CShape box = car_box_shape(q);
                 Xf cx = car_child_xf(q);
-#ifdef RSG_HOST_RAY_HOOK
-                RSG_HOST_RAY_HOOK(from, to, box, cx);
-#endif
                 if (!ray_may_hit(from, to, cx.t, len(box.half + v3(box.margin, box.margin, box.margin)))) continue;
                 if (ray_convex(from, to, box, cx, f, nn)) {
                     if (len2(nn) > 0.0001f && f < frac) { frac = f; n = normalized(nn); hit_obj = q; }
@@ -323,27 +378,58 @@ struct World {
         else { f = CAR_COLLISION_FRICTION; r = CAR_COLLISION_RESTITUTION; }
     }
 
-    // Arena::_BulletContactAddedCallback (src/Sim/Arena/Arena.cpp:244-314)
+    // Arena::_BulletContactAddedCallback (src/Sim/Arena/Arena.cpp:244-314):
+    // the point's combined material and flags here, the arena's side through
+    // effect().
     RSG_FN void contact_added(CPoint& cp, int b0, int b1) {
         if (!has_response(b0) || !has_response(b1)) return;
         if (b0 > 0 && b1 < 0) {
             // car + world (Arena.cpp:407-414)
-            Car& c = A.cars[b0 - 1];
-            setflag(c.flags, RSG_CF_WORLD_CONTACT, true);
-            c.world_normal = cp.normal;
             cp.friction = CARWORLD_COLLISION_FRICTION;
             cp.restitution = CARWORLD_COLLISION_RESTITUTION;
+            effect(EFF_CAR_WORLD, b0 - 1, 0, cp.normal, v3zero());
         } else if (b0 == 0 && b1 < 0) {
             // ball + world: Ball::_OnWorldCollision, then the averaged
             // "special" row (Arena.cpp:296-304)
-            if (A.mode == RSG_MODE_HEATSEEKER) ball_world_hit_heatseeker(cp.normal);
+            if (A.mode == RSG_MODE_HEATSEEKER) effect(EFF_BALL_WORLD, 0, 0, cp.normal, v3zero());
             cp.special = true;
         } else if (b0 > 0 && b1 == 0) {
-            car_ball_hit(cp, b0 - 1);
+            cp.friction = CARBALL_COLLISION_FRICTION;
+            cp.restitution = CARBALL_COLLISION_RESTITUTION;
+            effect(EFF_CAR_BALL, b0 - 1, 0, cp.local_b, v3zero());
         } else if (b0 > 0 && b1 > 0) {
-            car_car_hit(cp, b0 - 1, b1 - 1);
+            cp.friction = CARCAR_COLLISION_FRICTION;
+            cp.restitution = CARCAR_COLLISION_RESTITUTION;
+            effect(EFF_CAR_CAR, b0 - 1, b1 - 1, cp.local_a, cp.local_b);
         }
         if (b1 == -1 - ST_MESH && cp.tri >= 0) adjust_internal_edge(cp, *mesh, cp.tri);
+    }
+
+    RSG_FN void effect(int kind, int a, int b, V3 v0, V3 v1) {
+        if (!rec) { apply_effect(kind, a, b, v0, v1); return; }
+        if (kind == EFF_CAR_WORLD && rec->n > 0 && (rec->e[rec->n - 1].code & 0xff) == EFF_CAR_WORLD) {
+            rec->e[rec->n - 1].v0 = v0;  // the flag is set either way; the last normal stays
+            return;
+        }
+        if (rec->n >= PAIR_EFFECTS) { rec->redo = 1; return; }
+        Effect& e = rec->e[rec->n++];
+        e.code = kind | (a << 8) | (b << 16);
+        e.v0 = v0;
+        e.v1 = v1;
+    }
+
+    RSG_FN void apply_effect(int kind, int a, int b, V3 v0, V3 v1) {
+        if (kind == EFF_CAR_WORLD) {
+            Car& c = A.cars[a];
+            setflag(c.flags, RSG_CF_WORLD_CONTACT, true);
+            c.world_normal = v0;
+        } else if (kind == EFF_BALL_WORLD) {
+            ball_world_hit_heatseeker(v0);
+        } else if (kind == EFF_CAR_BALL) {
+            car_ball_hit(a, v0);
+        } else if (kind == EFF_CAR_CAR) {
+            car_car_hit(a, b, v0, v1);
+        }
     }
 
     // Ball::_OnWorldCollision for Heatseeker (src/Sim/Ball/Ball.cpp:326-358):
@@ -368,13 +454,11 @@ struct World {
     // Arena::_BtCallback_OnCarBallCollision + Ball::_OnHit
     // (src/Sim/Arena/Arena.cpp:316-322, src/Sim/Ball/Ball.cpp:236-290).  The
     // manifold's body0 is the car, so the point on the ball is local point B.
-    RSG_FN void car_ball_hit(CPoint& cp, int ci) {
+    RSG_FN void car_ball_hit(int ci, V3 local_b) {
         Car& c = A.cars[ci];
         Body& bb = A.ball.b;
-        cp.friction = CARBALL_COLLISION_FRICTION;
-        cp.restitution = CARBALL_COLLISION_RESTITUTION;
         setflag(c.flags, RSG_CF_HIT_VALID, true);
-        c.hit_rel_pos = cp.local_b * BT_TO_UU;
+        c.hit_rel_pos = local_b * BT_TO_UU;
         c.hit_tick = A.tick;
         V3 ball_pos = bb.pos * BT_TO_UU;
         c.hit_ball_pos = ball_pos;
@@ -411,9 +495,7 @@ struct World {
     // Arena::_BtCallback_OnCarCarCollision (src/Sim/Arena/Arena.cpp:324-395):
     // bumps and demolitions, tested from both cars.  car1 is the manifold's
     // body0 (point A), car2 its body1 (point B).
-    RSG_FN void car_car_hit(CPoint& cp, int c1, int c2) {
-        cp.friction = CARCAR_COLLISION_FRICTION;
-        cp.restitution = CARCAR_COLLISION_RESTITUTION;
+    RSG_FN void car_car_hit(int c1, int c2, V3 local_a, V3 local_b) {
         for (int it = 0; it < 2; it++) {
             bool swapped = it == 1;
             Car& me = A.cars[swapped ? c2 : c1];
@@ -429,7 +511,7 @@ struct World {
                 float speed_towards = dot(vel, dir_to_other);
                 float other_away = dot(other_vel, vel_dir);
                 if (speed_towards > other_away) {
-                    V3 local = swapped ? cp.local_b : cp.local_a;
+                    V3 local = swapped ? local_b : local_a;
                     if ((local.x * BT_TO_UU) > BUMP_MIN_FORWARD_DIST) {
                         bool is_demo = flag(me.flags, RSG_CF_SUPERSONIC);
                         if (is_demo) is_demo = me.team != other.team;
@@ -614,10 +696,11 @@ struct World {
             RSG_FN void operator()(int t) { w->car_triangle(mi, code, t, amin, amax, cx, he); }
         } cb{this, mi, code, amin, amax, cx, he};
         mesh_query_aabb(*mesh, amin, amax, cb);
-        refresh(mi);
+        if (!aborted) refresh(mi);
     }
 
     RSG_FN void car_triangle(int mi, int code, int t, V3 amin, V3 amax, const Xf& cx, V3 he) {
+        if (aborted) return;
         V3 v[3];
         mesh_triangle(*mesh, t, v);
         if (!triangle_aabb_overlap(v, amin, amax)) return;
@@ -640,7 +723,9 @@ struct World {
         md *= md;
         V3 n, pb;
         float dist;
-        if (gjk_closest(box, tri, cx, xf_ident(), md, n, pb, dist, ws->epa)) add_contact(mi, n, pb, dist, t);
+        int r = gjk_closest(box, tri, cx, xf_ident(), md, n, pb, dist, epa);
+        if (r < 0) aborted = true;
+        else if (r) add_contact(mi, n, pb, dist, t);
     }
 
     // Car hitbox vs the ball: btConvexConvexAlgorithm (GJK) on the box and the
@@ -653,8 +738,9 @@ struct World {
         md *= md;
         V3 n, pb;
         float dist;
-        if (gjk_closest(box, sph, cx, xf_from(A.ball.b.pos, A.ball.b.rot), md, n, pb, dist, ws->epa))
-            add_contact(mi, n, pb, dist, -1);
+        int r = gjk_closest(box, sph, cx, xf_from(A.ball.b.pos, A.ball.b.rot), md, n, pb, dist, epa);
+        if (r < 0) { aborted = true; return; }
+        if (r) add_contact(mi, n, pb, dist, -1);
         refresh(mi);
     }
 
@@ -712,8 +798,16 @@ struct World {
     // -----------------------------------------------------------------------
     // The step
 
-    RSG_FN void step(float dt, const TickParams& P) {
-        (void)P;
+    // btDiscreteDynamicsWorld::internalSingleStepSimulation
+    // (btDiscreteDynamicsWorld.cpp:395-440) on one lane.
+    RSG_FN void step(float dt) {
+        step_begin(dt);
+        run_pairs();
+        step_end(dt, nullptr);
+    }
+
+    // Gravity, damping, bounds, broadphase and the manifold plan.
+    RSG_FN void step_begin(float dt) {
         const int nb = num_bodies();
         const V3 g = v3(0.f, 0.f, GRAVITY_Z) * UU_TO_BT;
         for (int i = 0; i < nb; i++) {
@@ -738,12 +832,12 @@ struct World {
             mn = mn - t; mx = mx + t;
             shape_aabb(i, pp, pr, mn2, mx2);
             mn2 = mn2 - t; mx2 = mx2 + t;
-            aabb_min[i] = v3(fminf(mn.x, mn2.x), fminf(mn.y, mn2.y), fminf(mn.z, mn2.z));
-            aabb_max[i] = v3(fmaxf(mx.x, mx2.x), fmaxf(mx.y, mx2.y), fmaxf(mx.z, mx2.z));
+            T->aabb_min[i] = v3(fminf(mn.x, mn2.x), fminf(mn.y, mn2.y), fminf(mn.z, mn2.z));
+            T->aabb_max[i] = v3(fmaxf(mx.x, mx2.x), fmaxf(mx.y, mx2.y), fmaxf(mx.z, mx2.z));
             // btRSBroadphase::setAabb: a proxy whose cell changed is removed
             // from its old cells and appended to the new ones, i.e. it moves
             // to the end of every cell list it is in.
-            int c = bp_cell_of(aabb_min[i]);
+            int c = bp_cell_of(T->aabb_min[i]);
             if (c != A.bp_cell[i]) {
                 A.bp_cell[i] = c;
                 if (nb > 1) {
@@ -754,21 +848,25 @@ struct World {
             }
         }
         // cell-list order of the dynamic proxies (registration order)
-        for (int i = 0; i < nb; i++) order[i] = i;
+        for (int i = 0; i < nb; i++) T->order[i] = i;
         for (int i = 1; i < nb; i++) {
-            int v = order[i], j = i - 1;
-            while (j >= 0 && A.bp_seq[order[j]] > A.bp_seq[v]) { order[j + 1] = order[j]; j--; }
-            order[j + 1] = v;
+            int v = T->order[i], j = i - 1;
+            while (j >= 0 && A.bp_seq[T->order[j]] > A.bp_seq[v]) { T->order[j + 1] = T->order[j]; j--; }
+            T->order[j + 1] = v;
         }
         // keep the order numbers small: rank them 0..nb-1
-        for (int r = 0; r < nb; r++) A.bp_seq[order[r]] = (uint32_t)r;
+        for (int r = 0; r < nb; r++) A.bp_seq[T->order[r]] = (uint32_t)r;
         find_pairs();
+        plan_pairs();
         RSG_PROF_MARK(3);
-        dispatch_pairs();
+    }
+
+    // The narrowphase's recorded effects (when it was shared: `log`),
+    // islands and solver, integration.
+    RSG_FN void step_end(float dt, const PairLog* log) {
+        const int nb = num_bodies();
+        if (log) apply_recorded(log);
         RSG_PROF_MARK(4);
-#ifdef RSG_HOST_DEBUG_HOOK
-        RSG_HOST_DEBUG_HOOK(*this);
-#endif
         solve_islands(dt);
         RSG_PROF_MARK(5);
         // integrateTransforms: active bodies
@@ -790,7 +888,7 @@ struct World {
 
     // btRSBroadphase::calculateOverlappingPairs
     RSG_FN void find_pairs() {
-        n_pairs = 0;
+        T->n_pairs = 0;
         const int nb = num_bodies();
         for (int p = 0; p < nb; p++) {
             int c = A.bp_cell[p];
@@ -798,10 +896,10 @@ struct World {
                 if (!static_in_cell(s, c)) continue;
                 V3 smin, smax;
                 static_proxy_aabb(mesh, s, smin, smax);
-                if (aabb_overlap(aabb_min[p], aabb_max[p], smin, smax)) {
-                    pairs[n_pairs].a = -1 - s;
-                    pairs[n_pairs].b = p;
-                    n_pairs++;
+                if (aabb_overlap(T->aabb_min[p], T->aabb_max[p], smin, smax)) {
+                    T->pairs[T->n_pairs].a = -1 - s;
+                    T->pairs[T->n_pairs].b = p;
+                    T->n_pairs++;
                 }
             }
             if (nb > 1) {
@@ -812,17 +910,17 @@ struct World {
                     if (bp_cells_adjacent(A.bp_cell[q], c)) cnt++;
                 if (cnt > 1) {
                     for (int r = 0; r < nb; r++) {
-                        int q = order[r];
+                        int q = T->order[r];
                         if (q == p || !bp_cells_adjacent(A.bp_cell[q], c)) continue;
-                        if (!aabb_overlap(aabb_min[p], aabb_max[p], aabb_min[q], aabb_max[q])) continue;
+                        if (!aabb_overlap(T->aabb_min[p], T->aabb_max[p], T->aabb_min[q], T->aabb_max[q])) continue;
                         int lo = p < q ? p : q, hi = p < q ? q : p;
                         bool found = false;
-                        for (int k = 0; k < n_pairs; k++)
-                            if (pairs[k].a == lo && pairs[k].b == hi) found = true;
+                        for (int k = 0; k < T->n_pairs; k++)
+                            if (T->pairs[k].a == lo && T->pairs[k].b == hi) found = true;
                         if (found) continue;
-                        pairs[n_pairs].a = lo;
-                        pairs[n_pairs].b = hi;
-                        n_pairs++;
+                        T->pairs[T->n_pairs].a = lo;
+                        T->pairs[T->n_pairs].b = hi;
+                        T->n_pairs++;
                     }
                 }
             }
@@ -831,74 +929,113 @@ struct World {
 
     RSG_FN bool needs_collision(int a, int b) { return active(a) || active(b); }
 
-    RSG_FN void dispatch_pairs() {
-        n_mf = 0;
-        for (int k = 0; k < n_pairs; k++) {
-            int a = pairs[k].a, b = pairs[k].b;
+    // The dispatcher's manifolds (btCollisionDispatcher::dispatchAllCollisionPairs,
+    // btCollisionDispatcher.cpp:221-271), decided before any narrowphase runs:
+    // every algorithm's creation and release of its manifold depends only on
+    // bounding boxes.  Ball + static and car + static pairs and the car + car
+    // box-box child create their manifold with the pair, and the compound
+    // ones release it unless the child AABB tests pass; the swapped compound
+    // of ball + car creates it once the child AABB meets the ball's.  Each
+    // kept manifold is (body b, body a) of its pair.  A manifold past the
+    // capacity is dropped where the pair would have created it.
+    RSG_FN void plan_pairs() {
+        int n = 0;
+        for (int k = 0; k < T->n_pairs; k++) {
+            int a = T->pairs[k].a, b = T->pairs[k].b;
             if (!needs_collision(a, b)) continue;
+            bool creates = true, keeps = true;
             if (a < 0) {
-                int s = -1 - a;
-                if (b == 0) {
-                    if (s == ST_MESH) {
-                        int mi = new_manifold(0, a);
-                        if (mi >= 0) ball_vs_mesh(mi);
-                        RSG_PROF_MARK(11);
-                    } else {
-                        int mi = new_manifold(0, a);
-                        if (mi >= 0) ball_vs_plane(mi, s);
-                        RSG_PROF_MARK(12);
-                    }
-                } else {
-                    // compound car vs static: the child algorithm's manifold is
-                    // created up front and released if the child AABB misses
-                    int mi = new_manifold(b, a);
-                    if (mi < 0) continue;
+                if (b != 0) {
                     V3 cmin, cmax, smin, smax;
                     car_child_aabb(b, cmin, cmax);
-                    static_shape_aabb(mesh, s, smin, smax);
-                    bool ov = aabb_overlap(cmin, cmax, smin, smax);
-                    if (ov) {
-                        if (s == ST_MESH) car_vs_mesh(mi, b);
-                        else car_vs_plane(mi, b, s);
-                    }
-                    if (!ov) release_manifold(mi);
-                    if (s == ST_MESH) RSG_PROF_MARK(13);
-                    else RSG_PROF_MARK(12);
+                    static_shape_aabb(mesh, -1 - a, smin, smax);
+                    keeps = aabb_overlap(cmin, cmax, smin, smax);
                 }
             } else if (a == 0) {
-                // ball + car compound (swapped compound algorithm): the child
-                // convex-convex algorithm creates its manifold (car, ball) on
-                // its first run, which needs the child AABB to overlap the ball's
                 V3 cmin, cmax;
                 car_child_aabb(b, cmin, cmax);
                 float r = make_ball_phys().radius + 0.08f;
                 V3 bmin = A.ball.b.pos - v3(r, r, r), bmax = A.ball.b.pos + v3(r, r, r);
-                if (aabb_overlap(cmin, cmax, bmin, bmax)) {
-                    int mi = new_manifold(b, 0);
-                    if (mi >= 0) car_vs_ball(mi, b);
-                }
-                RSG_PROF_MARK(14);
+                creates = keeps = aabb_overlap(cmin, cmax, bmin, bmax);
             } else {
-                // car + car: compound-compound without AABB trees falls back to
-                // nested compound algorithms whose box-box child owns a
-                // manifold (car b, car a) created with the pair; it is released
-                // unless both child AABB tests pass.
-                int mi = new_manifold(b, a);
-                if (mi < 0) continue;
                 V3 amin, amax, bcmin, bcmax, bmin, bmax;
                 car_child_aabb(a, amin, amax);
                 Body& bb = A.cars[b - 1].b;
                 shape_aabb(b, bb.pos, bb.rot, bcmin, bcmax);
-                bool keep = false;
+                keeps = false;
                 if (aabb_overlap(amin, amax, bcmin, bcmax)) {
                     car_child_aabb(b, bmin, bmax);
-                    if (aabb_overlap(bmin, bmax, amin, amax)) {
-                        car_vs_car(mi, b, a);
-                        keep = true;
-                    }
+                    keeps = aabb_overlap(bmin, bmax, amin, amax);
                 }
-                if (!keep) release_manifold(mi);
-                RSG_PROF_MARK(14);
+            }
+            if (!creates) continue;
+            if (n >= MAX_MANIFOLDS) { A.status |= RSG_STATUS_CONTACT_OVERFLOW; continue; }
+            if (!keeps) continue;
+            T->plan[n++] = k;
+        }
+        T->n_mf = n;
+    }
+
+    // The narrowphase of planned pair `mi` into manifold mi.
+    RSG_FN void run_pair(int mi) {
+        const Pair pr = T->pairs[T->plan[mi]];
+        const int a = pr.a, b = pr.b;
+        aborted = false;
+        open_manifold(mi, b, a);
+        if (a < 0) {
+            int s = -1 - a;
+            if (b == 0) {
+                if (s == ST_MESH) ball_vs_mesh(mi);
+                else ball_vs_plane(mi, s);
+            } else {
+                if (s == ST_MESH) car_vs_mesh(mi, b);
+                else car_vs_plane(mi, b, s);
+            }
+        } else if (a == 0) {
+            car_vs_ball(mi, b);
+        } else {
+            car_vs_car(mi, b, a);
+        }
+    }
+
+    // The narrowphase on one lane, pair by pair, effects applied as they come.
+    RSG_FN void run_pairs() {
+        rec = nullptr;
+        epa = &ws->epa;
+        for (int mi = 0; mi < T->n_mf; mi++) run_pair(mi);
+    }
+
+    // One lane's share of the narrowphase when the lanes of an arena run the
+    // planned pairs side by side: effects recorded in `log`, no penetration
+    // pass (`space` null) or the lane's own.
+    RSG_FN void run_pairs_recorded(int first, int stride, EpaSpace* space, PairLog* log) {
+        epa = space;
+        for (int mi = first; mi < T->n_mf; mi += stride) {
+            PairEffects& pe = log->eff[mi];
+            pe.n = 0;
+            pe.redo = 0;
+            rec = &pe;
+            run_pair(mi);
+            if (aborted) pe.redo = 1;
+        }
+        rec = nullptr;
+        epa = &ws->epa;
+    }
+
+    // The recorded effects in pair order; a pair marked for it runs again,
+    // with its effects applied as they come.
+    RSG_FN void apply_recorded(const PairLog* log) {
+        rec = nullptr;
+        epa = &ws->epa;
+        for (int mi = 0; mi < T->n_mf; mi++) {
+            const PairEffects& pe = log->eff[mi];
+            if (pe.redo) {
+                run_pair(mi);
+                continue;
+            }
+            for (int j = 0; j < pe.n; j++) {
+                const Effect& e = pe.e[j];
+                apply_effect(e.code & 0xff, (e.code >> 8) & 0xff, e.code >> 16, e.v0, e.v1);
             }
         }
     }
@@ -908,22 +1045,24 @@ struct World {
 
     RSG_FN void solve_islands(float dt) {
         const int nb = num_bodies();
-        // union-find over non-static objects (index = body index)
-        int uf_id[MAX_BODIES], uf_sz[MAX_BODIES];
-        for (int i = 0; i < nb; i++) { uf_id[i] = i; uf_sz[i] = 1; }
-        for (int k = 0; k < n_pairs; k++) {
-            int a = pairs[k].a, b = pairs[k].b;
+        IslandTables& I = ws->isl;
+        int* uf_id = I.uf_id;
+        // union-find over non-static objects (index = body index); the
+        // reference's set sizes never decide a union, so none are kept
+        for (int i = 0; i < nb; i++) uf_id[i] = i;
+        for (int k = 0; k < T->n_pairs; k++) {
+            int a = T->pairs[k].a, b = T->pairs[k].b;
             if (a < 0 || b < 0) continue;
             if (!has_response(a) || !has_response(b)) continue;
             int i = uf_find(uf_id, a), j = uf_find(uf_id, b);
             if (i == j) continue;
             uf_id[i] = j;
-            uf_sz[j] += uf_sz[i];
         }
-        int tag[MAX_BODIES];
+        int* tag = I.tag;
         for (int i = 0; i < nb; i++) tag[i] = uf_find(uf_id, i);
         // sortIslands: elements sorted by island id (quicksort)
-        int elem[MAX_BODIES], ekey[MAX_BODIES];
+        int* elem = I.elem;
+        int* ekey = I.ekey;
         for (int i = 0; i < nb; i++) { elem[i] = i; ekey[i] = uf_find(uf_id, i); }
         if (nb > 1) quicksort_by_key(elem, ekey, 0, nb - 1);
         // activation: wake islands that contain an active (non sleeping) body
@@ -953,9 +1092,10 @@ struct World {
             s0 = e0;
         }
         // island manifolds in dispatcher order
-        int im[MAX_MANIFOLDS], ikey[MAX_MANIFOLDS];
+        int* im = I.im;
+        int* ikey = I.ikey;
         int n_im = 0;
-        for (int m = 0; m < n_mf; m++) {
+        for (int m = 0; m < T->n_mf; m++) {
             int b0 = mf[m].b0, b1 = mf[m].b1;
             bool b0_sleep = b0 < 0 ? true : (b0 == 0 ? sleeping[0] : false);
             bool b1_sleep = b1 < 0 ? true : (b1 == 0 ? sleeping[0] : false);
@@ -968,7 +1108,7 @@ struct World {
         }
         if (n_im > 1) quicksort_by_key(im, ikey, 0, n_im - 1);
         // collect bodies of non-sleeping islands, in island order
-        int bodies[MAX_BODIES];
+        int* bodies = I.bodies;
         int n_bodies = 0;
         for (int s0 = 0; s0 < nb;) {
             int id = ekey[elem[s0]];
@@ -984,7 +1124,7 @@ struct World {
             s0 = e0;
         }
         // manifolds whose island is processed (all islands are batched)
-        Solver S;
+        Solver& S = ws->solver;
         S.rows = ws->rows;
         S.fric = ws->fric;
         solver_reset(S, dt);

@@ -1008,12 +1008,14 @@ RSG_FN bool calc_pen_depth(const CShape& a, const CShape& b, const Xf& ta, const
 }
 
 // ---------------------------------------------------------------------------
-// btGjkPairDetector::getClosestPointsNonVirtual.  Returns true with the
+// btGjkPairDetector::getClosestPointsNonVirtual.  Returns 1 with the
 // reported contact (normal on B pointing to A, point on B in world space,
-// signed distance) when the shapes are closer than sqrt(max_dist2).
+// signed distance) when the shapes are closer than sqrt(max_dist2), else 0.
+// Without a penetration work space (`space` null) a query that needs the
+// penetration pass returns -1 and reports nothing.
 
-RSG_FN bool gjk_closest(const CShape& A, const CShape& B, const Xf& ta, const Xf& tb, float max_dist2,
-                        V3& normal_out, V3& point_b_out, float& dist_out, EpaSpace& space) {
+RSG_FN int gjk_closest(const CShape& A, const CShape& B, const Xf& ta, const Xf& tb, float max_dist2,
+                       V3& normal_out, V3& point_b_out, float& dist_out, EpaSpace* space) {
     const float REL_ERROR2 = 1.0e-6f;
     float distance = 0.f;
     V3 normalInB = v3zero();
@@ -1074,9 +1076,10 @@ RSG_FN bool gjk_closest(const CShape& A, const CShape& B, const Xf& ta, const Xf
     }
     bool catchDegenerate = degenerate != 0 && (distance + margin) < 0.01f;
     if (!isValid || catchDegenerate) {
+        if (!space) return -1;
         V3 tA, tB;
         sep = v3zero();
-        bool ok2 = calc_pen_depth(A, B, lta, ltb, sep, tA, tB, space);
+        bool ok2 = calc_pen_depth(A, B, lta, ltb, sep, tA, tB, *space);
         if (ok2) {
             V3 tn = tB - tA;
             float ls = len2(tn);
@@ -1108,9 +1111,9 @@ RSG_FN bool gjk_closest(const CShape& A, const CShape& B, const Xf& ta, const Xf
         normal_out = normalInB;
         point_b_out = pointOnB + offset;
         dist_out = distance;
-        return true;
+        return 1;
     }
-    return false;
+    return 0;
 }
 
 }  // namespace rsg

@@ -15,10 +15,14 @@ using namespace rsg;
 //     loop as usual);
 //   * the car pre-ticks, one car per lane, when they are independent of
 //     their order (no wheel on a car);
+//   * the narrowphase of the world step, one planned pair per lane (the
+//     contact callbacks' effects on the arena recorded per pair and applied
+//     by lane 0 in pair order; a pair that needs the penetration pass is
+//     run again there);
 //   * the car post-ticks, one car per lane, and the boost-pad tests, pads
 //     spread over the lanes.
 // Per-arena LDS block: ArenaLocal prefix, ray results, two flag words.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_COOP_WAVES_PER_EU)))
 rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int arena_bytes, int lanes) {
     extern __shared__ __align__(16) unsigned char rsg_lds[];
     const int slot = (int)threadIdx.x / lanes;
@@ -37,6 +41,8 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
     P.synth_controls = d.synth_controls != 0;
     P.auto_reset = d.auto_reset != 0;
     const MeshView* mp = mode_has_arena(d.mode) ? &mesh : nullptr;
+    StepSpace* ws = live ? arena_space(d, a) : nullptr;
+    PairLog* log = live ? arena_log(d, a) : nullptr;
     int* concurrent = ahead + 1;
     if (live && k == 0) load_arena(d, a, A);
     __syncthreads();
@@ -61,8 +67,12 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
         __syncthreads();
         if (live && k == 0) {
             pads_pre_tick(A, P);
-            tick_world(A, P, mp, arena_space(d, a));
+            tick_world_begin(A, P, mp, ws);
         }
+        __syncthreads();
+        if (live) tick_world_pairs(A, P, mp, ws, log, k, lanes);
+        __syncthreads();
+        if (live && k == 0) tick_world_end(A, P, mp, ws, log);
         __syncthreads();
         if (live && k < A.n_cars) car_post_tick(A, P, k);
         __syncthreads();

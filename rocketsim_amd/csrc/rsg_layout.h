@@ -106,7 +106,7 @@ struct BatchDev {
     rsg_event* ev_buf;
     uint32_t* ev_count;
     uint32_t ev_cap;
-    // per-arena world-step work space (StepSpace blocks, one per arena) for
+    // per-arena world-step work space (ArenaSpace blocks, one per arena) for
     // the kernels that keep it in global memory; null when not allocated
     unsigned char* space;
 };

@@ -694,7 +694,29 @@ RSG_FN void ball_pre_tick_heatseeker(Ball& ball, float dt) {
 RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws) {
     if (P.mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(A.ball, P.dt);
     World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
-    world.step(P.dt, P);
+    world.step(P.dt);
+    RSG_PROF_MARK(7);
+}
+
+// The same world step in three parts for an arena whose lanes share the
+// narrowphase: the first and last on one lane, the pairs on every lane
+// (`lane` of `lanes`, effects recorded in `log`), with the arena unchanged
+// in between.  A pair whose contact needs the penetration pass is run again
+// by the last part, on the arena's own work space.
+RSG_FN void tick_world_begin(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws) {
+    if (P.mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(A.ball, P.dt);
+    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
+    world.step_begin(P.dt);
+}
+RSG_FN void tick_world_pairs(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws, PairLog* log,
+                             int lane, int lanes) {
+    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
+    world.run_pairs_recorded(lane, lanes, nullptr, log);
+}
+RSG_FN void tick_world_end(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws,
+                           const PairLog* log) {
+    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
+    world.step_end(P.dt, log);
     RSG_PROF_MARK(7);
 }
 

@@ -24,6 +24,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise NativeMissing(f"{LIB_PATH} is not built; run `python -m rocketsim_amd.build`")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64, and
+        # a process that loads librsg.so first (resolving it to the system
+        # runtime) leaves PyTorch unable to find a GPU afterwards.  Loading
+        # PyTorch first makes librsg.so bind to the runtime already loaded.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.rsg_last_error.restype = C.c_char_p
         L.rsg_init_mesh.argtypes = [C.c_void_p, C.c_uint64]

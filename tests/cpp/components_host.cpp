@@ -50,7 +50,7 @@ int ch_gjk(const float* box_implicit, float box_margin, const float* xa, int kin
     auto space = std::make_unique<EpaSpace>();
     V3 n, p;
     float d;
-    if (!gjk_closest(box, b, xf_from_floats(xa), xf_from_floats(xb), max_dist2, n, p, d, *space)) return 0;
+    if (gjk_closest(box, b, xf_from_floats(xa), xf_from_floats(xb), max_dist2, n, p, d, space.get()) != 1) return 0;
     put(out_n, n);
     put(out_p, p);
     *out_d = d;

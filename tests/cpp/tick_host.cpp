@@ -8,7 +8,13 @@
 //                            one Newton step, LinearMath/btVector3.h); without
 //                            it, the GPU's (divide by the length);
 //   RSG_TEST_MAX_MANIFOLDS,  lower contact capacities, so that ordinary play
-//   RSG_TEST_MAX_SOLVER_ROWS reaches the overflow status bits.
+//   RSG_TEST_MAX_SOLVER_ROWS reaches the overflow status bits;
+//   RSG_TEST_SPLIT_WORLD=n   the world step as the cooperative kernel runs
+//                            it: its narrowphase shared by n lanes (run here
+//                            one after another, last lane first) with no
+//                            penetration work space, the effects applied in
+//                            pair order afterwards;
+//   RSG_TEST_PAIR_EFFECTS=n  room for n recorded effects per pair.
 #ifdef RSG_SSE_NORMALIZE
 #include <immintrin.h>
 
@@ -155,13 +161,52 @@ void th_get_status(void* p, uint32_t* out) {
     for (int a = 0; a < h.dev.n; a++) out[a] = arena_ref(h.dev, a).u(AW_STATUS);
 }
 
-// Steps every arena `ticks` ticks with the plain kernel's code path.
+#ifdef RSG_TEST_SPLIT_WORLD
+// One tick in the order of rsg_step_arena_coop_kernel's phases, with the
+// world step in its three parts.
+static void tick_split(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws, PairLog* log) {
+    tick_begin(A, P);
+    {
+        World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
+        for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], nullptr);
+    }
+    pads_pre_tick(A, P);
+    tick_world_begin(A, P, mesh, ws);
+    for (int lane = RSG_TEST_SPLIT_WORLD - 1; lane >= 0; lane--)
+        tick_world_pairs(A, P, mesh, ws, log, lane, RSG_TEST_SPLIT_WORLD);
+    tick_world_end(A, P, mesh, ws, log);
+    for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
+    if (mode_has_arena(P.mode))
+        for (int p = 0; p < PAD_NUM; p++) pad_check_cars(A, p);
+    tick_end(A, P);
+}
+#endif
+
+// Steps every arena `ticks` ticks with the plain kernel's code path (or the
+// split world step, RSG_TEST_SPLIT_WORLD).
 void th_step(void* p, int ticks) {
     HostBatch& h = *(HostBatch*)p;
     MeshView mv = h.mesh.view();
     auto A = std::make_unique<ArenaLocal>();
     auto ws = std::make_unique<StepSpace>();
+#ifdef RSG_TEST_SPLIT_WORLD
+    auto log = std::make_unique<PairLog>();
+    for (int a = 0; a < h.dev.n; a++) {
+        load_arena(h.dev, a, *A);
+        TickParams P;
+        P.mode = h.dev.mode;
+        P.dt = h.dev.tick_time;
+        P.seed = h.dev.seed;
+        P.arena_id = (uint32_t)a;
+        P.synth_controls = h.dev.synth_controls != 0;
+        P.auto_reset = h.dev.auto_reset != 0;
+        const MeshView* mp = mode_has_arena(h.dev.mode) ? &mv : nullptr;
+        for (int t = 0; t < ticks; t++) tick_split(*A, P, mp, ws.get(), log.get());
+        store_arena(h.dev, a, *A);
+    }
+#else
     for (int a = 0; a < h.dev.n; a++) step_arena(h.dev, mv, a, ticks, *A, ws.get());
+#endif
 }
 
 }  // extern "C"

@@ -5,7 +5,11 @@ step kernels run, compiled with g++) on the CPU.
     recorded ticks on every committed fixture, with no exception: the tick
     code is the reference's arithmetic, and the one thing the GPU does
     differently is the last bits of a normalised vector.  (The parity tests
-    use this build to explain each GPU sample out of tolerance.)
+    use this build to explain each GPU sample out of tolerance.)  The same
+    holds with the world step split as the cooperative kernel splits it
+    (narrowphase shared by 16 lanes, contact effects applied afterwards in
+    pair order, penetration queries run again in order), and with room for
+    one effect per pair, so that every pair with more runs again in order.
   * Capacity overflow is signalled, not silent: with the contact capacities
     lowered, arenas that exceed them carry RSG_STATUS_CONTACT_OVERFLOW or
     RSG_STATUS_SOLVER_OVERFLOW, and every arena whose result differs from the
@@ -33,12 +37,13 @@ FIXTURES = [("void_tf.npz", MODE_THE_VOID), ("soccar_ball_tf.npz", MODE_SOCCAR),
             ("soccar_walls_tf.npz", MODE_SOCCAR), ("heatseeker_tf.npz", MODE_HEATSEEKER)]
 
 
+@pytest.mark.parametrize("variant", ["sse", "sse_split", "sse_split_1eff"])
 @pytest.mark.parametrize("name,mode", FIXTURES)
-def test_sse_host_build_reproduces_oracle_fixtures(name, mode):
+def test_sse_host_build_reproduces_oracle_fixtures(name, mode, variant):
     if not tick_host.same_rsqrt_as_recording_cpu():
         pytest.skip("this CPU's rsqrtss differs from the fixtures' recording CPU's")
     samples = parity.load_samples(os.path.join(HERE, "golden", name))
-    results, status = tick_host.run_samples(tick_host.get("sse"), samples, mode)
+    results, status = tick_host.run_samples(tick_host.get(variant), samples, mode)
     nbad, msgs = parity.check_samples(samples, results)
     assert nbad == 0, "\n".join(msgs)
     assert not any(status)

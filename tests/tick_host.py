@@ -26,6 +26,11 @@ def build(out_path, defines=(), extra=(), load_it=True):
 
 _cache = {}
 VARIANTS = {"sse": ["RSG_SSE_NORMALIZE"], "plain": [],
+            # the world step as the cooperative kernel runs it (narrowphase
+            # over 16 lanes, effects applied afterwards in pair order)
+            "sse_split": ["RSG_SSE_NORMALIZE", "RSG_TEST_SPLIT_WORLD=16"],
+            # ... keeping one effect per pair, so pairs with more run again in order
+            "sse_split_1eff": ["RSG_SSE_NORMALIZE", "RSG_TEST_SPLIT_WORLD=16", "RSG_TEST_PAIR_EFFECTS=1"],
             # contact capacities low enough for ordinary play to reach them
             "small_manifolds": ["RSG_TEST_MAX_MANIFOLDS=1"], "small_rows": ["RSG_TEST_MAX_SOLVER_ROWS=4"]}
 
