@@ -40,6 +40,10 @@ typedef struct rsg_batch_desc {
     const int* cars_per_arena;    /* n_arenas entries (each 0..RSG_MAX_CARS) */
     const uint8_t* car_team;      /* RSG_TEAM_*, one per car */
     const uint8_t* car_preset;    /* RSG_PRESET_*, one per car */
+    /* MutatorConfig of every arena, set before its cars are added (Arena::Create,
+     * SetMutatorConfig, then AddCar: the ball is made with the config's radius
+     * and mass, cars keep the car mass); NULL for the mode's defaults */
+    const rsg_mutator_config* mutators;
 } rsg_batch_desc;
 
 /* Last error message of the calling thread ("" if none). */
@@ -102,8 +106,9 @@ int rsg_batch_get_launch(rsg_batch* b);
  *   RSG_KERNEL_LDS      one lane per arena, working copy in LDS;
  *   RSG_KERNEL_COOP     4 lanes per car slot per arena, working copy in LDS,
  *                       the wheel rays of a tick cast by all lanes at once;
- *   RSG_KERNEL_AUTO     the first of COOP, LDS, SCRATCH whose footprint
- *                       covers the batch in one round of waves.
+ *   RSG_KERNEL_AUTO     COOP wherever its per-arena block fits, else LDS
+ *                       when its footprint covers the batch in one round
+ *                       of waves, else SCRATCH.
  * Results do not depend on it; rsg_batch_get_kernel returns the kernel in
  * use.  (rsg_batch_set_launch's arenas_per_block counts arenas, not lanes.) */
 enum { RSG_KERNEL_AUTO = 0, RSG_KERNEL_SCRATCH = 1, RSG_KERNEL_LDS = 2, RSG_KERNEL_COOP = 3 };
@@ -223,8 +228,26 @@ int rsg_shot_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate,
                               const float* target, const float* checkpoints, const float* dloss, float* dcontrols,
                               float* dstate0, void* stream);
 
-/* Arena::IsBallScored (Arena.cpp:893-901) per arena. */
+/* Arena::IsBallScored (Arena.cpp:893-901) per arena, with the arena's
+ * goalBaseThresholdY and ballRadius. */
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
+
+/* MutatorConfig(GameMode) (src/Sim/MutatorConfig/MutatorConfig.cpp:5-40):
+ * the defaults of `mode` (RSG_MODE_*). */
+int rsg_mutator_config_default(int mode, rsg_mutator_config* out);
+/* Arena::SetMutatorConfig (src/Sim/Arena/Arena.h:53, Arena.cpp:17-48) on
+ * arena `arena`, or on every arena of the batch (-1), with the reference's
+ * side effects: the new gravity goes to the bodies that are awake (a
+ * demolished car and a sleeping ball keep theirs); a new ball radius or mass
+ * remakes the ball at rest at the centre, at the radius's height; a new car
+ * mass gives every car the ball's mass and the car mass's inertia (the
+ * reference passes ballMass to setMassProps).  A batch starts with the
+ * mode's defaults in every arena.  Fails on non-finite fields, on a
+ * non-positive ball radius, ball mass or car mass and on an unknown
+ * demo_mode. */
+int rsg_batch_set_mutator_config(rsg_batch* b, int arena, const rsg_mutator_config* cfg);
+/* Arena::GetMutatorConfig (Arena.h:52). */
+int rsg_batch_get_mutator_config(rsg_batch* b, int arena, rsg_mutator_config* out);
 
 /* ---- Game-event tracking: one GameEventTracker
  * (src/Sim/GameEventTracker/GameEventTracker.cpp:51-166) per arena of a

@@ -17,6 +17,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <optional>
 #include <vector>
 
 #include "rsg.h"
@@ -27,13 +28,21 @@ inline void check(int rc) {
     if (rc < 0) throw std::runtime_error(std::string("rsg: ") + rsg_last_error());
 }
 
-enum class GameMode : int { SOCCAR = RSG_MODE_SOCCAR, THE_VOID = RSG_MODE_THE_VOID };
+enum class GameMode : int { SOCCAR = RSG_MODE_SOCCAR, HEATSEEKER = RSG_MODE_HEATSEEKER, THE_VOID = RSG_MODE_THE_VOID };
 enum class Team : uint8_t { BLUE = RSG_TEAM_BLUE, ORANGE = RSG_TEAM_ORANGE };
 
 using CarState = rsg_car_state;
 using BallState = rsg_ball_state;
 using BoostPadState = rsg_pad_state;
 using CarControls = rsg_controls;
+using MutatorConfig = rsg_mutator_config;
+
+// MutatorConfig(GameMode) (src/Sim/MutatorConfig/MutatorConfig.cpp:5-40).
+inline MutatorConfig DefaultMutatorConfig(GameMode mode) {
+    MutatorConfig m;
+    check(rsg_mutator_config_default((int)mode, &m));
+    return m;
+}
 
 // RocketSim::InitFromMem (src/RocketSim.h:32) for the Soccar collision mesh.
 inline void InitFromMem(const void* cmf_data, uint64_t size) { check(rsg_init_mesh(cmf_data, size)); }
@@ -49,6 +58,9 @@ struct BatchConfig {
     uint64_t seed = 0;
     bool auto_reset = false;      // kickoff reset after a goal
     bool synth_controls = false;  // draw controls on the device (include/rsg_workload.h)
+    // every arena's MutatorConfig, set before its cars are added (none: the
+    // mode's defaults)
+    std::optional<MutatorConfig> mutators;
 };
 
 class ArenaBatch {
@@ -74,6 +86,7 @@ public:
         d.cars_per_arena = counts.data();
         d.car_team = teams.data();
         d.car_preset = presets.data();
+        d.mutators = cfg.mutators ? &*cfg.mutators : nullptr;
         h_ = rsg_batch_create(&d);
         if (!h_) throw std::runtime_error(std::string("rsg: ") + rsg_last_error());
     }
@@ -110,6 +123,13 @@ public:
     void ResetToRandomKickoff(int arena = -1, uint64_t seed = 0) { check(rsg_batch_reset_kickoff(h_, arena, seed)); }
     std::vector<uint8_t> IsBallScored() { std::vector<uint8_t> v(NumArenas()); check(rsg_batch_is_ball_scored(h_, v.data())); return v; }
     void SetLaunch(int arenas_per_block) { check(rsg_batch_set_launch(h_, arenas_per_block)); }
+    // Arena::SetMutatorConfig / GetMutatorConfig on one arena, or all (-1).
+    void SetMutatorConfig(const MutatorConfig& m, int arena = -1) { check(rsg_batch_set_mutator_config(h_, arena, &m)); }
+    MutatorConfig GetMutatorConfig(int arena = 0) {
+        MutatorConfig m;
+        check(rsg_batch_get_mutator_config(h_, arena, &m));
+        return m;
+    }
 
     // Arena::Serialize (Arena.cpp:546-573) of one arena: the reference's bytes
     // plus this library's extension block.
@@ -224,7 +244,9 @@ public:
         b_->Step(ticksToSimulate);
         b_->Sync();
     }
-    // Arena::Clone (Arena.cpp:646-687): a new arena in the same state.
+    // Arena::Clone (Arena.cpp:646-687): a new arena in the same state, its
+    // mutator config included (the reference's Clone leaves the new arena at
+    // its mode's defaults).
     std::unique_ptr<Arena> Clone() {
         std::unique_ptr<Arena> a(new Arena(mode_, seed_, specs_));
         a->Deserialize(Serialize());
@@ -240,6 +262,13 @@ public:
     std::vector<BoostPadState> GetBoostPadStates() { return b_->GetPadStates(); }
     uint64_t TickCount() { return b_->GetTickCounts()[0]; }
     void ResetToRandomKickoff(int seed = -1) { b_->ResetToRandomKickoff(0, (uint64_t)(int64_t)seed); }
+    // Arena::SetMutatorConfig / GetMutatorConfig (Arena.h:52-53).  Cars added
+    // or removed later rebuild the batch with this config given at creation,
+    // so every car gets its gravity and the car mass's default body (the
+    // reference leaves cars that existed at SetMutatorConfig with the mass it
+    // gave them).
+    void SetMutatorConfig(const MutatorConfig& m) { b_->SetMutatorConfig(m, 0); }
+    MutatorConfig GetMutatorConfig() { return b_->GetMutatorConfig(0); }
     bool IsBallScored() { return b_->IsBallScored()[0] != 0; }
     std::vector<uint8_t> Serialize() { return b_->Serialize(0); }
     void Deserialize(const std::vector<uint8_t>& bytes) { b_->Deserialize(0, bytes); }
@@ -265,6 +294,7 @@ private:
         BatchConfig cfg;
         cfg.mode = mode_;
         cfg.seed = seed_;
+        if (b_) cfg.mutators = b_->GetMutatorConfig(0);
         auto nb = std::unique_ptr<ArenaBatch>(new ArenaBatch({specs}, cfg));
         std::vector<std::unique_ptr<Car>> cars;
         if (removed == -1 && b_) {

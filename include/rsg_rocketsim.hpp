@@ -212,6 +212,78 @@ inline RocketSim::BoostPadState FromRsg(const rsg_pad_state& s) {
     return p;
 }
 
+// MutatorConfig, field for field (src/Sim/MutatorConfig/MutatorConfig.h:16-79).
+inline rsg_mutator_config ToRsg(const RocketSim::MutatorConfig& m) {
+    rsg_mutator_config s;
+    std::memset(&s, 0, sizeof(s));
+    ToRsg(m.gravity, s.gravity);
+    s.car_mass = m.carMass;
+    s.car_world_friction = m.carWorldFriction;
+    s.car_world_restitution = m.carWorldRestitution;
+    s.ball_mass = m.ballMass;
+    s.ball_max_speed = m.ballMaxSpeed;
+    s.ball_drag = m.ballDrag;
+    s.ball_world_friction = m.ballWorldFriction;
+    s.ball_world_restitution = m.ballWorldRestitution;
+    s.jump_accel = m.jumpAccel;
+    s.jump_immediate_force = m.jumpImmediateForce;
+    s.boost_accel_ground = m.boostAccelGround;
+    s.boost_accel_air = m.boostAccelAir;
+    s.boost_used_per_second = m.boostUsedPerSecond;
+    s.respawn_delay = m.respawnDelay;
+    s.bump_cooldown_time = m.bumpCooldownTime;
+    s.boost_pad_cooldown_big = m.boostPadCooldown_Big;
+    s.boost_pad_cooldown_small = m.boostPadCooldown_Small;
+    s.car_spawn_boost_amount = m.carSpawnBoostAmount;
+    s.ball_hit_extra_force_scale = m.ballHitExtraForceScale;
+    s.bump_force_scale = m.bumpForceScale;
+    s.ball_radius = m.ballRadius;
+    s.unlimited_flips = m.unlimitedFlips;
+    s.unlimited_double_jumps = m.unlimitedDoubleJumps;
+    s.recharge_boost_enabled = m.rechargeBoostEnabled;
+    s.demo_mode = (uint8_t)m.demoMode;
+    s.recharge_boost_per_second = m.rechargeBoostPerSecond;
+    s.recharge_boost_delay = m.rechargeBoostDelay;
+    s.enable_team_demos = m.enableTeamDemos;
+    s.goal_base_threshold_y = m.goalBaseThresholdY;
+    return s;
+}
+
+inline RocketSim::MutatorConfig FromRsg(const rsg_mutator_config& s, RocketSim::GameMode mode) {
+    RocketSim::MutatorConfig m(mode);
+    m.gravity = VecFromRsg(s.gravity);
+    m.carMass = s.car_mass;
+    m.carWorldFriction = s.car_world_friction;
+    m.carWorldRestitution = s.car_world_restitution;
+    m.ballMass = s.ball_mass;
+    m.ballMaxSpeed = s.ball_max_speed;
+    m.ballDrag = s.ball_drag;
+    m.ballWorldFriction = s.ball_world_friction;
+    m.ballWorldRestitution = s.ball_world_restitution;
+    m.jumpAccel = s.jump_accel;
+    m.jumpImmediateForce = s.jump_immediate_force;
+    m.boostAccelGround = s.boost_accel_ground;
+    m.boostAccelAir = s.boost_accel_air;
+    m.boostUsedPerSecond = s.boost_used_per_second;
+    m.respawnDelay = s.respawn_delay;
+    m.bumpCooldownTime = s.bump_cooldown_time;
+    m.boostPadCooldown_Big = s.boost_pad_cooldown_big;
+    m.boostPadCooldown_Small = s.boost_pad_cooldown_small;
+    m.carSpawnBoostAmount = s.car_spawn_boost_amount;
+    m.ballHitExtraForceScale = s.ball_hit_extra_force_scale;
+    m.bumpForceScale = s.bump_force_scale;
+    m.ballRadius = s.ball_radius;
+    m.unlimitedFlips = s.unlimited_flips != 0;
+    m.unlimitedDoubleJumps = s.unlimited_double_jumps != 0;
+    m.rechargeBoostEnabled = s.recharge_boost_enabled != 0;
+    m.demoMode = (RocketSim::DemoMode)s.demo_mode;
+    m.rechargeBoostPerSecond = s.recharge_boost_per_second;
+    m.rechargeBoostDelay = s.recharge_boost_delay;
+    m.enableTeamDemos = s.enable_team_demos != 0;
+    m.goalBaseThresholdY = s.goal_base_threshold_y;
+    return m;
+}
+
 }  // namespace rsg_amd
 
 #endif  // RSG_ROCKETSIM_HPP

@@ -147,6 +147,33 @@ enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8, RSG_SNAPSHOT_RAW_WORDS = 1024 };
 enum { RSG_STATUS_NONFINITE = 1u, RSG_STATUS_CONTACT_OVERFLOW = 2u, RSG_STATUS_SOLVER_OVERFLOW = 4u };
 enum { RSG_MAX_MANIFOLDS = 48, RSG_MAX_SOLVER_ROWS = 96 };
 
+/* MutatorConfig (src/Sim/MutatorConfig/MutatorConfig.h:16-79), field for
+ * field, in the reference's units (uu, uu/s, uu/s^2, seconds; masses in
+ * Bullet units as the reference keeps them).  rsg_mutator_config_default
+ * gives a mode's defaults (MutatorConfig(GameMode), MutatorConfig.cpp:5-40). */
+enum { RSG_DEMO_NORMAL = 0, RSG_DEMO_ON_CONTACT = 1, RSG_DEMO_DISABLED = 2 };
+typedef struct rsg_mutator_config {
+    float gravity[3];
+    float car_mass;
+    float car_world_friction, car_world_restitution;
+    float ball_mass, ball_max_speed, ball_drag;
+    float ball_world_friction, ball_world_restitution;
+    float jump_accel, jump_immediate_force;
+    float boost_accel_ground, boost_accel_air, boost_used_per_second;
+    float respawn_delay, bump_cooldown_time;
+    float boost_pad_cooldown_big, boost_pad_cooldown_small;
+    float car_spawn_boost_amount;
+    float ball_hit_extra_force_scale, bump_force_scale;
+    float ball_radius;
+    uint8_t unlimited_flips, unlimited_double_jumps;
+    uint8_t recharge_boost_enabled;
+    uint8_t demo_mode;                 /* RSG_DEMO_* (DemoMode) */
+    float recharge_boost_per_second, recharge_boost_delay;
+    uint8_t enable_team_demos;
+    uint8_t _pad[3];
+    float goal_base_threshold_y;
+} rsg_mutator_config;
+
 /* A game event, replacing the reference's callbacks: the goal-score callback
  * (Arena::SetGoalScoreCallback, src/Sim/Arena/Arena.h:24,102; fired at the
  * end of the tick, Arena.cpp:748-752) and the car-bump callback
@@ -190,6 +217,7 @@ typedef struct rsg_arena_snapshot {
     rsg_car_state cars[RSG_MAX_CARS];
     rsg_ball_state ball;
     rsg_pad_state pads[RSG_NUM_PADS];
+    rsg_mutator_config mutators;       /* Arena::_mutatorConfig */
     /* This library's exact records of the arena, opaque: positions and
      * velocities in its internal units, the values the reference keeps
      * outside CarState / BallState (per-wheel vehicle values, contact flags,

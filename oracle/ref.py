@@ -9,7 +9,7 @@ the timed CPU baseline -- never as the product path.
 import ctypes as C
 import os
 
-from rocketsim_amd.state import BallState, CarState, Controls, PadState
+from rocketsim_amd.state import BallState, CarState, Controls, MutatorConfig, PadState
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_ref", "librsref.so")
@@ -69,6 +69,8 @@ def lib():
         L.rsref_deserialize.argtypes = [C.c_char_p, C.c_uint64]
         L.rsref_last_car_id.restype = C.c_uint32
         L.rsref_last_car_id.argtypes = [C.c_void_p]
+        L.rsref_get_mutators.argtypes = [C.c_void_p, C.POINTER(MutatorConfig)]
+        L.rsref_set_mutators.argtypes = [C.c_void_p, C.POINTER(MutatorConfig)]
         L.rsref_car_team.argtypes = [C.c_void_p, C.c_int]
         L.rsref_get_controls.argtypes = [C.c_void_p, C.c_int, C.POINTER(Controls)]
         from rocketsim_amd.state import GameEventConfig
@@ -182,6 +184,14 @@ class RefArena:
         if not self.h:
             raise RuntimeError(self.L.rsref_last_error().decode())
         return self
+
+    def get_mutators(self):
+        m = MutatorConfig()
+        self.L.rsref_get_mutators(self.h, C.byref(m))
+        return m
+
+    def set_mutators(self, m):
+        self.L.rsref_set_mutators(self.h, C.byref(m))
 
     def last_car_id(self):
         return self.L.rsref_last_car_id(self.h)

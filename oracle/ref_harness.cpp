@@ -796,6 +796,21 @@ extern "C" void* rsref_deserialize(const uint8_t* data, uint64_t size) {
 }
 
 extern "C" uint32_t rsref_last_car_id(void* h) { return ((RefArena*)h)->arena->_lastCarID; }
+extern "C" void rsref_get_mutators(void* h, rsg_mutator_config* out) {
+    *out = rsg_amd::ToRsg(((RefArena*)h)->arena->GetMutatorConfig());
+}
+extern "C" void rsref_set_mutators(void* h, const rsg_mutator_config* in) {
+    RefArena* ra = (RefArena*)h;
+    Arena* a = ra->arena;
+    const MutatorConfig& old = a->GetMutatorConfig();
+    bool ball_remade = in->ball_radius != old.ballRadius || in->ball_mass != old.ballMass;
+    a->SetMutatorConfig(rsg_amd::FromRsg(*in, a->gameMode));
+    if (ball_remade) {
+        // the new ball body's proxy is registered after every other one
+        ra->bp_cell[0] = proxy_cell(ra, 0);
+        ra->bp_seq[0] = ++ra->bp_counter;
+    }
+}
 extern "C" int rsref_car_team(void* h, int i) { return (int)((RefArena*)h)->cars[i]->team; }
 extern "C" void rsref_get_controls(void* h, int i, rsg_controls* c) { *c = rsg_amd::ToRsg(((RefArena*)h)->cars[i]->controls); }
 

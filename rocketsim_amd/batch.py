@@ -9,7 +9,7 @@ import ctypes as C
 
 from . import native
 from .state import (ArenaSnapshot, BallState, BatchDesc, CarState, Controls, Event, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID,
-                    NUM_PADS, PadState)
+                    MutatorConfig, NUM_PADS, PadState)
 
 
 _mesh_loaded = False
@@ -47,7 +47,7 @@ def deserialize_snapshot(data):
 
 class ArenaBatch:
     def __init__(self, cars_per_arena, teams, presets, mode=MODE_SOCCAR, tick_rate=120.0,
-                 seed=0, auto_reset=False, synth_controls=False):
+                 seed=0, auto_reset=False, synth_controls=False, mutators=None):
         L = native.lib()
         if mode in (MODE_SOCCAR, MODE_HEATSEEKER) and not _mesh_loaded:
             init_mesh()
@@ -58,9 +58,11 @@ class ArenaBatch:
         self._cpa = (C.c_int * n)(*cars_per_arena)
         self._team = (C.c_uint8 * max(total, 1))(*teams)
         self._preset = (C.c_uint8 * max(total, 1))(*presets)
+        self._mut = mutators
         d = BatchDesc(n_arenas=n, mode=mode, tick_rate=tick_rate, seed=seed,
                       auto_reset=int(auto_reset), synth_controls=int(synth_controls),
-                      cars_per_arena=self._cpa, car_team=self._team, car_preset=self._preset)
+                      cars_per_arena=self._cpa, car_team=self._team, car_preset=self._preset,
+                      mutators=C.pointer(mutators) if mutators is not None else None)
         h = L.rsg_batch_create(C.byref(d))
         if not h:
             raise RuntimeError(L.rsg_last_error().decode())
@@ -228,6 +230,15 @@ class ArenaBatch:
         """step() and graph_launch() raise when an arena carries a status bit,
         as the reference's Arena::Step throws on its own errors."""
         native.check(self.L.rsg_batch_set_strict(self.h, int(bool(on))))
+
+    def set_mutator_config(self, cfg, arena=-1):
+        """Arena::SetMutatorConfig on one arena, or on all (-1)."""
+        native.check(self.L.rsg_batch_set_mutator_config(self.h, arena, C.byref(cfg)))
+
+    def get_mutator_config(self, arena=0):
+        out = MutatorConfig()
+        native.check(self.L.rsg_batch_get_mutator_config(self.h, arena, C.byref(out)))
+        return out
 
     def is_ball_scored(self):
         out = (C.c_uint8 * self.n_arenas)()

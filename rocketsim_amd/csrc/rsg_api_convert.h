@@ -139,6 +139,104 @@ inline void ball_from_api(const CarRef& r, const rsg_ball_state& s) {
     r.sf(BW_HS, s.hs_y_target_dir);
     r.sf(BW_HS + 1, s.hs_cur_target_speed);
     r.sf(BW_HS + 2, s.hs_time_since_hit);
+    // Ball::SetState wakes a moving ball (src/Sim/Ball/Ball.cpp:46-47)
+    if (s.vel[0] != 0.f || s.vel[1] != 0.f || s.vel[2] != 0.f || s.ang_vel[0] != 0.f || s.ang_vel[1] != 0.f ||
+        s.ang_vel[2] != 0.f)
+        r.u(BW_AWAKE) = 1;
+}
+
+// The arena's MutatorConfig (the mode's defaults where the batch keeps none).
+inline rsg_mutator_config arena_mutators(const BatchDev& h, int a) {
+    return h.mut ? h.mut[a] : mutator_defaults(h.mode);
+}
+
+// btRigidBody::setGravity (BulletDynamics/Dynamics/btRigidBody.cpp:160-167):
+// the acceleration kept, the force it gives the body's current mass.
+inline void set_body_gravity(const CarRef& r, int acc_word, int force_word, const float* gravity_uu, float inv_mass) {
+    V3 acc = v3(gravity_uu[0], gravity_uu[1], gravity_uu[2]) * UU_TO_BT;
+    r.sv(acc_word, acc);
+    r.sv(force_word, acc * (1.f / inv_mass));
+}
+
+// A body as the reference makes it, added to a world with gravity
+// `gravity_uu` (btDiscreteDynamicsWorld::addRigidBody): the car's box and mass
+// (Car::_BulletSetup, src/Sim/Car/Car.cpp:195-215; mass words 0 = the car
+// default), the ball's sphere and mass (Ball::_BulletSetup,
+// src/Sim/Ball/Ball.cpp:53-115), awake.
+inline void init_car_body_words(const CarRef& cr, const float* gravity_uu) {
+    cr.sf(CW_BODY_INV_MASS, 0.f);
+    cr.sf(CW_BODY_INERTIA_MASS, 0.f);
+    set_body_gravity(cr, CW_BODY_GRAVITY_ACC, CW_BODY_GRAVITY, gravity_uu, 1.f / CAR_MASS_BT);
+}
+inline void init_ball_body_words(const CarRef& br, float radius_uu, float mass, const float* gravity_uu) {
+    br.sf(BW_RADIUS, radius_uu);
+    br.sf(BW_MASS, mass);
+    br.u(BW_AWAKE) = 1;
+    set_body_gravity(br, BW_GRAVITY_ACC, BW_GRAVITY, gravity_uu, make_ball_phys(radius_uu, mass).inv_mass);
+}
+
+// btRSBroadphase::createProxy refuses a dynamic proxy whose AABB diagonal
+// exceeds the cell size (btRSBroadphase.cpp:227-229); a ball of radius
+// `radius_uu` made at (0, 0, radius) passes if this holds.
+inline bool ball_fits_broadphase(float radius_uu) {
+    BallPhys bp = make_ball_phys(radius_uu, BALL_MASS_BT);
+    float ext = bp.radius + 0.08f;
+    V3 c = v3(0, 0, bp.radius);
+    V3 d = (c + v3(ext, ext, ext)) - (c - v3(ext, ext, ext));
+    float cell = bp_cell_size();
+    return !(dot(d, d) > cell * cell);
+}
+
+// Arena::SetMutatorConfig (src/Sim/Arena/Arena.cpp:17-48) on arena `a`'s
+// records, from config `old` to `m`:
+//   * the world gravity goes to the bodies that are awake
+//     (btDiscreteDynamicsWorld::setGravity: a demolished car's disabled body
+//     and a sleeping ball keep theirs);
+//   * a new ball radius or mass remakes the ball body (Ball::_BulletSetup):
+//     at rest at (0, 0, radius), awake, with the world gravity, registered
+//     again in the broadphase after every other proxy;
+//   * a new car mass gives every car body mass ballMass and the box inertia
+//     of carMass (the reference passes mutatorConfig.ballMass to
+//     setMassProps, Arena.cpp:38-39), with m_gravity = mass * acceleration.
+// Ball friction, restitution and drag, and everything else, are read from
+// the config each tick.
+inline void apply_mutator_config(const BatchDev& h, int a, const rsg_mutator_config& old, const rsg_mutator_config& m) {
+    const bool ball_changed = m.ball_radius != old.ball_radius || m.ball_mass != old.ball_mass;
+    const bool car_mass_changed = m.car_mass != old.car_mass;
+    const int n_cars = (int)arena_ref(h, a).u(AW_NUM_CARS);
+    CarRef br = ball_ref(h, a);
+    for (int i = 0; i < n_cars; i++) {
+        CarRef cr = car_ref(h, a, i);
+        if (cr.u(CW_FLAGS) & RSG_CF_BODY_DISABLED) continue;
+        float im = cr.f(CW_BODY_INV_MASS);
+        set_body_gravity(cr, CW_BODY_GRAVITY_ACC, CW_BODY_GRAVITY, m.gravity, im != 0.f ? im : 1.f / CAR_MASS_BT);
+    }
+    if (br.u(BW_AWAKE))
+        set_body_gravity(br, BW_GRAVITY_ACC, BW_GRAVITY, m.gravity, make_ball_phys(br.f(BW_RADIUS), br.f(BW_MASS)).inv_mass);
+    if (ball_changed) {
+        init_ball_body_words(br, m.ball_radius, m.ball_mass, m.gravity);
+        BallPhys bp = make_ball_phys(m.ball_radius, m.ball_mass);
+        br.sv(BW_POS, v3(0, 0, bp.radius));
+        store_rot(br, BW_ROT, m3_identity());
+        br.sv(BW_VEL, v3zero());
+        br.sv(BW_ANGVEL, v3zero());
+        float ext = bp.radius + 0.08f;
+        br.u(BW_BP_CELL) = (uint32_t)bp_cell_of(v3(0, 0, bp.radius) - v3(ext, ext, ext));
+        uint32_t last = br.u(BW_BP_SEQ);
+        for (int i = 0; i < n_cars; i++) {
+            uint32_t q = car_ref(h, a, i).u(CW_BP_SEQ);
+            last = q > last ? q : last;
+        }
+        br.u(BW_BP_SEQ) = last + 1;
+    }
+    if (car_mass_changed) {
+        for (int i = 0; i < n_cars; i++) {
+            CarRef cr = car_ref(h, a, i);
+            cr.sf(CW_BODY_INV_MASS, 1.f / m.ball_mass);
+            cr.sf(CW_BODY_INERTIA_MASS, m.car_mass);
+            cr.sv(CW_BODY_GRAVITY, m.ball_mass * cr.v(CW_BODY_GRAVITY_ACC));
+        }
+    }
 }
 
 // A new arena's records (Arena::Create + AddCar, src/Sim/Arena/Arena.cpp:50-62,
@@ -146,6 +244,7 @@ inline void ball_from_api(const CarRef& r, const rsg_ball_state& s) {
 // active; the proxies registered where the reference creates them.
 inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_t* team, const uint8_t* preset) {
     const int mode = h.mode;
+    const rsg_mutator_config mut = arena_mutators(h, a);
     CarRef ar = arena_ref(h, a);
     ar.u(AW_NUM_CARS) = (uint32_t)n_cars;
     uint32_t ow = 0;  // default car update order: reverse creation
@@ -157,8 +256,9 @@ inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_
         cr.u(CW_ID) = (uint32_t)(i + 1);
         cr.su64(CW_HIT_TICK, ~0ull);
         cr.su64(CW_HIT_EXTRA_TICK, ~0ull);
-        cr.sf(CW_BOOST, spawn_boost(mode));
+        cr.sf(CW_BOOST, mut.car_spawn_boost_amount);
         store_rot(cr, CW_ROT, m3_identity());
+        init_car_body_words(cr, mut.gravity);
         // proxy created at the identity transform (Car::_BulletSetup,
         // src/Sim/Car/Car.cpp:212-233), AABB without contact threshold
         CarPhys cp = make_car_phys(preset[i]);
@@ -169,9 +269,10 @@ inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_
     br.sv(BW_POS, v3(0, 0, BALL_REST_Z) * UU_TO_BT);
     store_rot(br, BW_ROT, m3_identity());
     br.sf(BW_HS + 1, HS_INITIAL_TARGET_SPEED);
+    init_ball_body_words(br, mut.ball_radius, mut.ball_mass, mut.gravity);
     {
         // proxy created at (0, 0, radius) (Ball::_BulletSetup, src/Sim/Ball/Ball.cpp:85-114)
-        BallPhys bp = make_ball_phys();
+        BallPhys bp = make_ball_phys(mut.ball_radius, mut.ball_mass);
         float ext = bp.radius + 0.08f;
         V3 c = v3(0, 0, bp.radius);
         br.u(BW_BP_CELL) = (uint32_t)bp_cell_of(c - v3(ext, ext, ext));

@@ -13,6 +13,7 @@ constexpr int MAX_CARS = RSG_MAX_CARS;
 struct ArenaLocal {
     int n_cars;
     int mode;           // RSG_MODE_*
+    rsg_mutator_config mut;  // Arena::_mutatorConfig
     Ball ball;
     Pad pads[PAD_NUM];
     uint64_t tick;

@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -86,6 +87,29 @@ int upload_array(T** dst, const std::vector<T>& src) {
 
 }  // namespace
 
+namespace {
+// Why a MutatorConfig cannot be simulated ("" if it can): non-finite
+// fields, a non-positive ball radius, ball mass or car mass, an unknown
+// demo mode.
+std::string mutator_config_problem(const rsg_mutator_config& m) {
+    const float fields[] = {m.gravity[0], m.gravity[1], m.gravity[2], m.car_mass, m.car_world_friction,
+                            m.car_world_restitution, m.ball_mass, m.ball_max_speed, m.ball_drag,
+                            m.ball_world_friction, m.ball_world_restitution, m.jump_accel, m.jump_immediate_force,
+                            m.boost_accel_ground, m.boost_accel_air, m.boost_used_per_second, m.respawn_delay,
+                            m.bump_cooldown_time, m.boost_pad_cooldown_big, m.boost_pad_cooldown_small,
+                            m.car_spawn_boost_amount, m.ball_hit_extra_force_scale, m.bump_force_scale,
+                            m.ball_radius, m.recharge_boost_per_second, m.recharge_boost_delay,
+                            m.goal_base_threshold_y};
+    for (float f : fields)
+        if (!std::isfinite(f)) return "non-finite field";
+    if (!(m.ball_radius > 0.f) || !(m.ball_mass > 0.f) || !(m.car_mass > 0.f))
+        return "ball_radius, ball_mass and car_mass must be positive";
+    if (m.demo_mode > RSG_DEMO_DISABLED) return "unknown demo_mode";
+    return "";
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // batch object
 
@@ -95,6 +119,7 @@ struct rsg_batch {
     std::vector<uint8_t> team, preset;
     int total_cars = 0;
     std::vector<uint32_t> h_car, h_ball, h_pad, h_arena;
+    std::vector<rsg_mutator_config> h_mut;  // per-arena MutatorConfig (the kernels only read it)
     bool host_dirty = false;  // host mirror newer than device
     bool dev_newer = false;   // device newer than host mirror
     hipStream_t last_stream = nullptr;
@@ -122,6 +147,7 @@ BatchDev host_view(rsg_batch* b) {
     h.ball = b->h_ball.data();
     h.pad = b->h_pad.data();
     h.arena = b->h_arena.data();
+    h.mut = b->h_mut.data();
     return h;
 }
 
@@ -131,6 +157,8 @@ int upload(rsg_batch* b) {
     HIP_OK(hipMemcpy(b->dev.ball, b->h_ball.data(), ball_words(b) * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(b->dev.pad, b->h_pad.data(), pad_words(b) * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(b->dev.arena, b->h_arena.data(), arena_words(b) * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(const_cast<rsg_mutator_config*>(b->dev.mut), b->h_mut.data(),
+                     b->h_mut.size() * sizeof(rsg_mutator_config), hipMemcpyHostToDevice));
     b->host_dirty = false;
     return 0;
 }
@@ -167,6 +195,13 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
         return nullptr;
     }
     if (!desc->cars_per_arena) { fail("rsg_batch_create: cars_per_arena is required"); return nullptr; }
+    if (desc->mutators) {
+        std::string why = mutator_config_problem(*desc->mutators);
+        if (!why.empty() || !ball_fits_broadphase(desc->mutators->ball_radius)) {
+            fail("rsg_batch_create: mutators: " + (why.empty() ? std::string("the ball's AABB exceeds the broadphase cell size") : why));
+            return nullptr;
+        }
+    }
     if (mode_has_arena(desc->mode) && !g_mesh.ready) {
         fail("rsg_batch_create: no arena collision mesh loaded (call rsg_init_mesh first)");
         return nullptr;
@@ -208,16 +243,21 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     b->h_ball.assign(ball_words(b), 0);
     b->h_pad.assign(pad_words(b), 0);
     b->h_arena.assign(arena_words(b), 0);
+    b->h_mut.assign((size_t)b->dev.n, desc->mutators ? *desc->mutators : mutator_defaults(b->dev.mode));
     hipError_t e = hipSuccess;
+    rsg_mutator_config* dmut = nullptr;
     if ((e = hipMalloc(&b->dev.car, car_words(b) * 4)) != hipSuccess ||
         (e = hipMalloc(&b->dev.ball, ball_words(b) * 4)) != hipSuccess ||
         (e = hipMalloc(&b->dev.pad, pad_words(b) * 4)) != hipSuccess ||
         (e = hipMalloc(&b->dev.arena, arena_words(b) * 4)) != hipSuccess ||
-        (e = hipMalloc(&b->dev.space, (size_t)b->dev.n * sizeof(ArenaSpace))) != hipSuccess) {
+        (e = hipMalloc(&b->dev.space, (size_t)b->dev.n * sizeof(ArenaSpace))) != hipSuccess ||
+        (e = hipMalloc(&dmut, (size_t)b->dev.n * sizeof(rsg_mutator_config))) != hipSuccess) {
+        b->dev.mut = dmut;
         fail(std::string("rsg_batch_create: hipMalloc: ") + hipGetErrorString(e));
         rsg_batch_destroy(b);
         return nullptr;
     }
+    b->dev.mut = dmut;
     // Host-side initial state: cars with config, ball at rest, pads active.
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++)
@@ -237,6 +277,7 @@ void rsg_batch_destroy(rsg_batch* b) {
     if (b->dev.pad) (void)hipFree(b->dev.pad);
     if (b->dev.arena) (void)hipFree(b->dev.arena);
     if (b->dev.space) (void)hipFree(b->dev.space);
+    if (b->dev.mut) (void)hipFree(const_cast<rsg_mutator_config*>(b->dev.mut));
     if (b->status_partials) (void)hipFree(b->status_partials);
     if (b->dev.ev_buf) (void)hipFree(b->dev.ev_buf);
     if (b->dev.ev_count) (void)hipFree(b->dev.ev_count);
@@ -881,14 +922,52 @@ int rsg_batch_get_goals(rsg_batch* b, uint32_t* blue, uint32_t* orange, uint64_t
     return 0;
 }
 
+int rsg_mutator_config_default(int mode, rsg_mutator_config* out) {
+    if (!out) return fail("rsg_mutator_config_default: null argument");
+    if (mode != RSG_MODE_SOCCAR && mode != RSG_MODE_HEATSEEKER && mode != RSG_MODE_THE_VOID)
+        return fail("rsg_mutator_config_default: unsupported game mode");
+    *out = mutator_defaults(mode);
+    return 0;
+}
+
+int rsg_batch_set_mutator_config(rsg_batch* b, int arena, const rsg_mutator_config* cfg) {
+    if (!b || !cfg) return fail("rsg_batch_set_mutator_config: null argument");
+    if (arena < -1 || arena >= b->dev.n) return fail("rsg_batch_set_mutator_config: arena out of range");
+    const rsg_mutator_config& m = *cfg;
+    std::string why = mutator_config_problem(m);
+    if (!why.empty()) return fail("rsg_batch_set_mutator_config: " + why);
+    const int a0 = arena < 0 ? 0 : arena, a1 = arena < 0 ? b->dev.n : arena + 1;
+    for (int a = a0; a < a1; a++) {
+        const rsg_mutator_config& old = b->h_mut[(size_t)a];
+        if ((m.ball_radius != old.ball_radius || m.ball_mass != old.ball_mass) && !ball_fits_broadphase(m.ball_radius))
+            return fail("rsg_batch_set_mutator_config: the remade ball's AABB exceeds the broadphase cell size "
+                        "(btRSBroadphase::createProxy refuses it)");
+    }
+    if (download(b) != 0) return -1;
+    BatchDev h = host_view(b);
+    for (int a = a0; a < a1; a++) {
+        apply_mutator_config(h, a, b->h_mut[(size_t)a], m);
+        b->h_mut[(size_t)a] = m;
+    }
+    b->host_dirty = true;
+    return 0;
+}
+
+int rsg_batch_get_mutator_config(rsg_batch* b, int arena, rsg_mutator_config* out) {
+    if (!b || !out) return fail("rsg_batch_get_mutator_config: null argument");
+    if (arena < 0 || arena >= b->dev.n) return fail("rsg_batch_get_mutator_config: arena out of range");
+    *out = b->h_mut[(size_t)arena];
+    return 0;
+}
+
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out) {
     if (!b || !out) return fail("rsg_batch_is_ball_scored: null argument");
     if (download(b) != 0) return -1;
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++) {
         float y = ball_ref(h, a).v(BW_POS).y * BT_TO_UU;
-        out[a] = (mode_has_arena(b->dev.mode) &&
-                  fabsf(y) > (SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y + BALL_COLLISION_RADIUS_SOCCAR)) ? 1 : 0;
+        const rsg_mutator_config& m = b->h_mut[(size_t)a];
+        out[a] = (mode_has_arena(b->dev.mode) && fabsf(y) > (m.goal_base_threshold_y + m.ball_radius)) ? 1 : 0;
     }
     return 0;
 }
@@ -1063,6 +1142,7 @@ int rsg_batch_get_snapshot(rsg_batch* b, int arena, rsg_arena_snapshot* out) {
         last_id = out->car_id[k] > last_id ? out->car_id[k] : last_id;
     }
     out->last_car_id = last_id;
+    out->mutators = b->h_mut[(size_t)arena];
     ball_to_api(ball_ref(h, arena), out->ball);
     if (mode_has_arena(b->dev.mode))
         for (int p = 0; p < PAD_NUM; p++) pad_to_api(pad_ref(h, arena, p), out->pads[p]);
@@ -1093,6 +1173,9 @@ int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s)
     if (s->tick_time != b->dev.tick_time) return fail("rsg_batch_set_snapshot: tick time differs from the batch's");
     const int n = b->cars_per_arena[arena];
     if (s->n_cars != n) return fail("rsg_batch_set_snapshot: car count differs from the arena slot's");
+    if (s->mutators.demo_mode > RSG_DEMO_DISABLED || !(s->mutators.ball_radius > 0.f) ||
+        !(s->mutators.ball_mass > 0.f) || !(s->mutators.car_mass > 0.f))
+        return fail("rsg_batch_set_snapshot: mutator config out of range");
     // Cars go to the slots in creation (ascending id) order; each must have
     // the slot's team and preset.
     int by_id[RSG_MAX_CARS], slot_of[RSG_MAX_CARS];
@@ -1134,6 +1217,8 @@ int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s)
             car_from_api(cr, s->cars[k]);
             cr.u(CW_ID) = s->car_id[k];
             store_ctrl(cr, CW_CTRL, ctrl_from_api(s->controls[k]));
+            // a new car body, then SetMutatorConfig's gravity (Arena::DeserializeNew)
+            init_car_body_words(cr, s->mutators.gravity);
         }
     }
     if (raw) {
@@ -1144,8 +1229,13 @@ int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s)
             if (w != AW_NUM_CARS && w != AW_CAR_ORDER) ar.u(w) = ra.u(w);
     } else {
         ar.su64(AW_TICK, s->tick_count);
+        // the mode's default ball body, then SetMutatorConfig's gravity (the
+        // deserialized config counts as unchanged: no ball remake)
+        const rsg_mutator_config def = mutator_defaults(b->dev.mode);
+        init_ball_body_words(ball_ref(h, arena), def.ball_radius, def.ball_mass, s->mutators.gravity);
         ball_from_api(ball_ref(h, arena), s->ball);
     }
+    b->h_mut[(size_t)arena] = s->mutators;
     ar.u(AW_CAR_ORDER) = ow;
     if (mode_has_arena(b->dev.mode))
         for (int p = 0; p < PAD_NUM; p++) pad_from_api(pad_ref(h, arena, p), s->pads[p]);

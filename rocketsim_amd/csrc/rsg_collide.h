@@ -208,7 +208,7 @@ struct World {
     }
 
     RSG_FN float threshold_of(int code) {
-        if (code == 0) return make_ball_phys().angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
+        if (code == 0) return make_ball_phys(A.ball.radius_uu, A.ball.mass).angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
         if (code > 0) return make_car_phys(A.cars[code - 1].preset).angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
         return 1e30f;  // planes and the arena mesh have huge bounding spheres
     }
@@ -225,7 +225,7 @@ struct World {
     // and predicted pose, merged, +- the global contact threshold.
     RSG_FN void shape_aabb(int code, V3 pos, const M3& rot, V3& mn, V3& mx) {
         if (code == 0) {
-            float r = make_ball_phys().radius + 0.08f;  // btSphereShape::getAabb (RocketSim +0.08)
+            float r = make_ball_phys(A.ball.radius_uu, A.ball.mass).radius + 0.08f;  // btSphereShape::getAabb (RocketSim +0.08)
             mn = pos - v3(r, r, r);
             mx = pos + v3(r, r, r);
         } else {
@@ -314,7 +314,7 @@ struct World {
             float f;
             V3 nn;
             if (q == 0) {
-                CShape sph = shape_sphere(make_ball_phys().radius);
+                CShape sph = shape_sphere(make_ball_phys(A.ball.radius_uu, A.ball.mass).radius);
                 if (!ray_may_hit(from, to, A.ball.b.pos, sph.margin)) continue;
                 if (ray_convex(from, to, sph, xf_from(A.ball.b.pos, A.ball.b.rot), f, nn)) {
                     if (len2(nn) > 0.0001f && f < frac) { frac = f; n = normalized(nn); hit_obj = q; }
@@ -374,7 +374,7 @@ struct World {
 
     RSG_FN void body_material(int code, float& f, float& r) {
         if (code < 0) { f = ARENA_COLLISION_BASE_FRICTION; r = ARENA_COLLISION_BASE_RESTITUTION; }
-        else if (code == 0) { f = BALL_FRICTION; r = BALL_RESTITUTION; }
+        else if (code == 0) { f = A.mut.ball_world_friction; r = A.mut.ball_world_restitution; }
         else { f = CAR_COLLISION_FRICTION; r = CAR_COLLISION_RESTITUTION; }
     }
 
@@ -385,8 +385,8 @@ struct World {
         if (!has_response(b0) || !has_response(b1)) return;
         if (b0 > 0 && b1 < 0) {
             // car + world (Arena.cpp:407-414)
-            cp.friction = CARWORLD_COLLISION_FRICTION;
-            cp.restitution = CARWORLD_COLLISION_RESTITUTION;
+            cp.friction = A.mut.car_world_friction;
+            cp.restitution = A.mut.car_world_restitution;
             effect(EFF_CAR_WORLD, b0 - 1, 0, cp.normal, v3zero());
         } else if (b0 == 0 && b1 < 0) {
             // ball + world: Ball::_OnWorldCollision, then the averaged
@@ -473,7 +473,8 @@ struct World {
                 V3 hit_dir = rs_normalized(rel_pos * v3(1, 1, BALL_CAR_EXTRA_IMPULSE_Z_SCALE));
                 V3 fwd_adj = fwd * dot(hit_dir, fwd) * (1 - BALL_CAR_EXTRA_IMPULSE_FORWARD_SCALE);
                 hit_dir = rs_normalized(hit_dir - fwd_adj);
-                V3 added = (hit_dir * rel_speed) * curve_eval(curve_ball_extra_impulse(), rel_speed) * 1.f;
+                V3 added = (hit_dir * rel_speed) * curve_eval(curve_ball_extra_impulse(), rel_speed) *
+                           A.mut.ball_hit_extra_force_scale;
                 c.hit_extra_vel = added;
                 A.ball.vel_cache += added * UU_TO_BT;
             }
@@ -513,21 +514,24 @@ struct World {
                 if (speed_towards > other_away) {
                     V3 local = swapped ? local_b : local_a;
                     if ((local.x * BT_TO_UU) > BUMP_MIN_FORWARD_DIST) {
-                        bool is_demo = flag(me.flags, RSG_CF_SUPERSONIC);
-                        if (is_demo) is_demo = me.team != other.team;
+                        bool is_demo = A.mut.demo_mode == RSG_DEMO_ON_CONTACT    ? true
+                                       : A.mut.demo_mode == RSG_DEMO_DISABLED ? false
+                                                                              : flag(me.flags, RSG_CF_SUPERSONIC);
+                        if (is_demo && !A.mut.enable_team_demos) is_demo = me.team != other.team;
                         emit_event(A, is_demo ? RSG_EVENT_DEMO : RSG_EVENT_BUMP, me.id, other.id);
                         if (is_demo) {
                             setflag(other.flags, RSG_CF_DEMOED, true);
-                            other.demo_timer = DEMO_RESPAWN_TIME;
+                            other.demo_timer = A.mut.respawn_delay;
                         } else {
                             bool ground = flag(other.flags, RSG_CF_ON_GROUND);
                             float base = curve_eval(ground ? curve_bump_ground() : curve_bump_air(), speed_towards);
                             V3 up = ground ? car_up(other) : v3(0, 0, 1);
-                            V3 imp = vel_dir * base + up * curve_eval(curve_bump_upward(), speed_towards) * 1.f;
+                            V3 imp = vel_dir * base + up * curve_eval(curve_bump_upward(), speed_towards) *
+                                                      A.mut.bump_force_scale;
                             other.vel_cache += imp * UU_TO_BT;
                         }
                         me.other_id = other.id;
-                        me.car_cooldown = BUMP_COOLDOWN_TIME;
+                        me.car_cooldown = A.mut.bump_cooldown_time;
                     }
                 }
             }
@@ -553,7 +557,7 @@ struct World {
         Xf convex_in_plane = xf_mul(xf_inverse(px), bx);
         V3 dir = plane_in_convex.R * -pd.n;
         V3 vtx;
-        float r = make_ball_phys().radius;
+        float r = make_ball_phys(A.ball.radius_uu, A.ball.mass).radius;
         if (len2(dir) < FLT_EPS * FLT_EPS) vtx = normalized(v3(-1, -1, -1)) * r;
         else vtx = r * normalized(dir);
         V3 vin = xf_apply(convex_in_plane, vtx);
@@ -595,7 +599,7 @@ struct World {
     RSG_FN void sphere_triangle(int mi, const V3 v[3], int tri) {
         Body& b = A.ball.b;
         V3 center = b.pos;  // triangle space = world (mesh at identity)
-        float radius = make_ball_phys().radius;
+        float radius = make_ball_phys(A.ball.radius_uu, A.ball.mass).radius;
         float thr = mf[mi].threshold + 0.f;
         float rwt = radius + thr;
         V3 normal = cross(v[1] - v[0], v[2] - v[0]);
@@ -733,7 +737,7 @@ struct World {
     RSG_FN void car_vs_ball(int mi, int code) {
         Xf cx = car_child_xf(code);
         CShape box = car_box_shape(code);
-        CShape sph = shape_sphere(make_ball_phys().radius);
+        CShape sph = shape_sphere(make_ball_phys(A.ball.radius_uu, A.ball.mass).radius);
         float md = box.margin + sph.margin + mf[mi].threshold;
         md *= md;
         V3 n, pb;
@@ -760,7 +764,7 @@ struct World {
     // btConvexConcaveCollisionAlgorithm::processCollision for the ball
     RSG_FN void ball_vs_mesh(int mi) {
         Body& b = A.ball.b;
-        float r = make_ball_phys().radius + 0.08f;
+        float r = make_ball_phys(A.ball.radius_uu, A.ball.mass).radius + 0.08f;
         V3 amin = b.pos - v3(r, r, r), amax = b.pos + v3(r, r, r);  // convex AABB in mesh space, + margin 0
         struct CB {
             World* w; int mi; V3 amin, amax;
@@ -778,7 +782,7 @@ struct World {
         {
             V3 tn = normalized(cross(v[1] - v[0], v[2] - v[0]));
             Body& b = A.ball.b;
-            float r = make_ball_phys().radius;
+            float r = make_ball_phys(A.ball.radius_uu, A.ball.mass).radius;
             V3 d = inverse(b.rot) * tn;  // getBasis().inverse() * n
             V3 lp = (len2(d) < FLT_EPS * FLT_EPS) ? normalized(v3(-1, -1, -1)) * r : r * normalized(d);
             V3 wp = b.rot * lp + b.pos;
@@ -809,15 +813,18 @@ struct World {
     // Gravity, damping, bounds, broadphase and the manifold plan.
     RSG_FN void step_begin(float dt) {
         const int nb = num_bodies();
-        const V3 g = v3(0.f, 0.f, GRAVITY_Z) * UU_TO_BT;
+        // btRigidBody::applyGravity: each body's own gravity
         for (int i = 0; i < nb; i++) {
             Body& b = *body_ptr(i);
-            if (b.active) b.force += g * (1.f / b.inv_mass);
+            if (b.active) b.force += b.grav;
         }
-        // predictUnconstraintMotion (damping on every dynamic body)
+        // predictUnconstraintMotion (damping on every dynamic body; the
+        // ball's linear damping is the arena's ballDrag, clamped to [0, 1]
+        // by btRigidBody::setDamping)
         {
             Body& bb = A.ball.b;
-            bb.vel *= powf(1.f - BALL_DRAG, dt);
+            const float drag = fminf(fmaxf(A.mut.ball_drag, 0.f), 1.f);
+            bb.vel *= powf(1.f - drag, dt);
             bb.ang *= powf(1.f - 0.f, dt);
         }
         RSG_PROF_MARK(2);
@@ -954,7 +961,7 @@ struct World {
             } else if (a == 0) {
                 V3 cmin, cmax;
                 car_child_aabb(b, cmin, cmax);
-                float r = make_ball_phys().radius + 0.08f;
+                float r = make_ball_phys(A.ball.radius_uu, A.ball.mass).radius + 0.08f;
                 V3 bmin = A.ball.b.pos - v3(r, r, r), bmax = A.ball.b.pos + v3(r, r, r);
                 creates = keeps = aabb_overlap(cmin, cmax, bmin, bmax);
             } else {
@@ -1107,6 +1114,7 @@ struct World {
             n_im++;
         }
         if (n_im > 1) quicksort_by_key(im, ikey, 0, n_im - 1);
+        RSG_PROF_MARK(11);
         // collect bodies of non-sleeping islands, in island order
         int* bodies = I.bodies;
         int n_bodies = 0;
@@ -1147,7 +1155,13 @@ struct World {
             int k = bodies[i];
             if (S.special[k].num > 0) convert_special(S, k, body_at);
         }
+        RSG_PROF_MARK(12);
+        RSG_PROF_COUNT(26, S.n_rows);
+        RSG_PROF_COUNT(27, S.n_fric);
+        RSG_PROF_COUNT(28, T->n_mf);
+        RSG_PROF_COUNT(29, S.n_bodies);
         solver_iterate(S);
+        RSG_PROF_MARK(13);
         // write back (btSequentialImpulseConstraintSolver::writeBackBodies)
         for (int i = 0; i < S.n_bodies; i++) {
             SBody& sb = S.bodies[i];

@@ -212,6 +212,18 @@ struct CarPhys {
     float angular_motion_disc;
 };
 
+// btBoxShape::calculateLocalInertia (BulletCollision/CollisionShapes/
+// btBoxShape.cpp:30-45) of a box with half extents `he` (margin included)
+// for `mass`, inverted as btRigidBody::setMassProps does.
+RSG_FN V3 box_inv_inertia(V3 he, float mass) {
+    float lx = 2.f * he.x, ly = 2.f * he.y, lz = 2.f * he.z;
+    V3 inertia = v3(mass / (12.0f) * (ly * ly + lz * lz),
+                    mass / (12.0f) * (lx * lx + lz * lz),
+                    mass / (12.0f) * (lx * lx + ly * ly));
+    return v3(inertia.x != 0.f ? 1.f / inertia.x : 0.f, inertia.y != 0.f ? 1.f / inertia.y : 0.f,
+              inertia.z != 0.f ? 1.f / inertia.z : 0.f);
+}
+
 RSG_FN CarPhys make_car_phys(int preset) {
     PresetRaw r = preset_raw(preset);
     CarPhys c;
@@ -230,11 +242,7 @@ RSG_FN CarPhys make_car_phys(int preset) {
     V3 lmin = c.box_offset - he, lmax = c.box_offset + he;
     c.local_aabb_half = 0.5f * (lmax - lmin);
     c.local_aabb_center = 0.5f * (lmax + lmin);
-    float lx = 2.f * he.x, ly = 2.f * he.y, lz = 2.f * he.z;
-    V3 inertia = v3(CAR_MASS_BT / (12.0f) * (ly * ly + lz * lz),
-                    CAR_MASS_BT / (12.0f) * (lx * lx + lz * lz),
-                    CAR_MASS_BT / (12.0f) * (lx * lx + ly * ly));
-    c.inv_inertia_local = v3(1.f / inertia.x, 1.f / inertia.y, 1.f / inertia.z);
+    c.inv_inertia_local = box_inv_inertia(he, CAR_MASS_BT);
     c.inv_mass = 1.f / CAR_MASS_BT;
     for (int i = 0; i < 4; i++) {
         bool front = i < 2;
@@ -273,11 +281,11 @@ struct BallPhys {
     float angular_motion_disc;
 };
 
-RSG_FN BallPhys make_ball_phys() {
+RSG_FN BallPhys make_ball_phys(float radius_uu = BALL_COLLISION_RADIUS_SOCCAR, float mass = BALL_MASS_BT) {
     BallPhys b;
-    b.radius = BALL_COLLISION_RADIUS_SOCCAR * UU_TO_BT;
-    b.inv_mass = 1.f / BALL_MASS_BT;
-    float elem = 0.4f * BALL_MASS_BT * b.radius * b.radius;
+    b.radius = radius_uu * UU_TO_BT;
+    b.inv_mass = 1.f / mass;
+    float elem = 0.4f * mass * b.radius * b.radius;
     b.inv_inertia = 1.f / elem;
     // RocketSim adds 0.08 to the sphere's bounding radius
     // (BulletCollision/CollisionShapes/btCollisionShape.cpp:130-134).
@@ -321,10 +329,48 @@ RSG_FN SpawnPos kickoff_spawn_heatseeker(int i) {
 // (Arena.cpp:478-520, 893-901).
 RSG_FN bool mode_has_arena(int mode) { return mode == RSG_MODE_SOCCAR || mode == RSG_MODE_HEATSEEKER; }
 
-// MutatorConfig defaults per mode (src/Sim/MutatorConfig/MutatorConfig.cpp:
-// 20-40): Heatseeker spawns cars with full boost and never drains it.
-RSG_FN float spawn_boost(int mode) { return mode == RSG_MODE_HEATSEEKER ? BOOST_MAX : BOOST_SPAWN_AMOUNT; }
-RSG_FN float boost_used_per_second(int mode) { return mode == RSG_MODE_HEATSEEKER ? 0.f : BOOST_USED_PER_SECOND; }
+// MutatorConfig(GameMode) (src/Sim/MutatorConfig/MutatorConfig.h:16-79,
+// MutatorConfig.cpp:5-40): the field defaults, then the mode's changes
+// (Heatseeker spawns cars with full boost and never drains it).
+RSG_FN rsg_mutator_config mutator_defaults(int mode) {
+    rsg_mutator_config m;
+    m.gravity[0] = 0.f; m.gravity[1] = 0.f; m.gravity[2] = GRAVITY_Z;
+    m.car_mass = CAR_MASS_BT;
+    m.car_world_friction = CARWORLD_COLLISION_FRICTION;
+    m.car_world_restitution = CARWORLD_COLLISION_RESTITUTION;
+    m.ball_mass = BALL_MASS_BT;
+    m.ball_max_speed = BALL_MAX_SPEED;
+    m.ball_drag = BALL_DRAG;
+    m.ball_world_friction = BALL_FRICTION;
+    m.ball_world_restitution = BALL_RESTITUTION;
+    m.jump_accel = JUMP_ACCEL;
+    m.jump_immediate_force = JUMP_IMMEDIATE_FORCE;
+    m.boost_accel_ground = BOOST_ACCEL_GROUND;
+    m.boost_accel_air = BOOST_ACCEL_AIR;
+    m.boost_used_per_second = BOOST_USED_PER_SECOND;
+    m.respawn_delay = DEMO_RESPAWN_TIME;
+    m.bump_cooldown_time = BUMP_COOLDOWN_TIME;
+    m.boost_pad_cooldown_big = PAD_COOLDOWN_BIG;
+    m.boost_pad_cooldown_small = PAD_COOLDOWN_SMALL;
+    m.car_spawn_boost_amount = BOOST_SPAWN_AMOUNT;
+    m.ball_hit_extra_force_scale = 1.f;
+    m.bump_force_scale = 1.f;
+    m.ball_radius = BALL_COLLISION_RADIUS_SOCCAR;
+    m.unlimited_flips = 0;
+    m.unlimited_double_jumps = 0;
+    m.recharge_boost_enabled = 0;
+    m.demo_mode = RSG_DEMO_NORMAL;
+    m.recharge_boost_per_second = RECHARGE_BOOST_PER_SECOND;
+    m.recharge_boost_delay = RECHARGE_BOOST_DELAY;
+    m.enable_team_demos = 0;
+    m._pad[0] = m._pad[1] = m._pad[2] = 0;
+    m.goal_base_threshold_y = SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y;
+    if (mode == RSG_MODE_HEATSEEKER) {
+        m.car_spawn_boost_amount = BOOST_MAX;
+        m.boost_used_per_second = 0.f;
+    }
+    return m;
+}
 
 // Heatseeker ball (src/RLConst.h:168-188).
 constexpr float HS_INITIAL_TARGET_SPEED = 2900, HS_TARGET_SPEED_INCREMENT = 85, HS_MIN_SPEEDUP_INTERVAL = 1,

@@ -84,18 +84,19 @@ RSG_FN bool ge_shooter_passer(const BatchDev& d, int a, uint32_t team, uint64_t 
 // Arena::IsBallProbablyGoingIn for Soccar (src/Sim/Arena/Arena.cpp:771-807):
 // the ball's ballistic path reaches the goal plane within max_time inside the
 // approximate goal mouth.  goal_team: the team whose net it is.
-RSG_FN bool ge_probably_going_in(V3 pos_bt, V3 vel_bt, float max_time, float extra_margin, uint32_t& goal_team) {
+RSG_FN bool ge_probably_going_in(V3 pos_bt, V3 vel_bt, float max_time, float extra_margin, const rsg_mutator_config& M,
+                                 uint32_t& goal_team) {
     V3 p = pos_bt * BT_TO_UU, v = vel_bt * BT_TO_UU;
     if (fabsf(v.y) < FLT_EPS) return false;
     float sgn = (float)((v.y > 0) - (v.y < 0));
-    float goal_y = SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y * sgn;
+    float goal_y = M.goal_base_threshold_y * sgn;
     float dist = fabsf(p.y - goal_y);
     float t = dist / fabsf(v.y);
     if (t > max_time) return false;
-    V3 g = v3(0.f, 0.f, GRAVITY_Z);
+    V3 g = v3(M.gravity[0], M.gravity[1], M.gravity[2]);
     V3 e = (p + v * t) + ((g * t) * t) / 2.f;
     const float half_width = 892.755f, height = 642.775f;
-    float margin = BALL_COLLISION_RADIUS_SOCCAR * 0.1f + extra_margin;
+    float margin = M.ball_radius * 0.1f + extra_margin;
     if (e.z > height + margin) return false;
     if (fabsf(e.x) > half_width + margin) return false;
     goal_team = sgn < 0 ? (uint32_t)RSG_TEAM_BLUE : (uint32_t)RSG_TEAM_ORANGE;
@@ -107,8 +108,8 @@ RSG_FN void game_events_update(const BatchDev& d, const GameEventDev& g, const r
     CarRef ar = arena_ref(d, a), br = ball_ref(d, a);
     const uint64_t tick = ar.u64(AW_TICK);
     const V3 bpos = br.v(BW_POS), bvel = br.v(BW_VEL);
-    const bool scored =
-        fabsf(bpos.y * BT_TO_UU) > (SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y + BALL_COLLISION_RADIUS_SOCCAR);
+    const rsg_mutator_config M = d.mut ? d.mut[a] : mutator_defaults(d.mode);
+    const bool scored = fabsf(bpos.y * BT_TO_UU) > (M.goal_base_threshold_y + M.ball_radius);
     const float tick_time = d.tick_time;
     const float tickrate = 1.f / tick_time;
     const uint64_t count = br.u64(BW_TICKS_SINCE_SET);
@@ -135,7 +136,7 @@ RSG_FN void game_events_update(const BatchDev& d, const GameEventDev& g, const r
                 float speed_sq = v.x * v.x + v.y * v.y + v.z * v.z;
                 uint32_t goal_team = 0;
                 if (speed_sq >= cfg.shot_min_speed * cfg.shot_min_speed &&
-                    ge_probably_going_in(bpos, bvel, cfg.shot_min_score_time, cfg.pred_score_extra_margin, goal_team)) {
+                    ge_probably_going_in(bpos, bvel, cfg.shot_min_score_time, cfg.pred_score_extra_margin, M, goal_team)) {
                     uint32_t shooter_team = goal_team == RSG_TEAM_BLUE ? (uint32_t)RSG_TEAM_ORANGE : (uint32_t)RSG_TEAM_BLUE;
                     const uint64_t min_delay = (uint64_t)(cfg.shot_touch_min_delay * tickrate);
                     int s, p;
@@ -154,7 +155,7 @@ RSG_FN void game_events_update(const BatchDev& d, const GameEventDev& g, const r
             }
         } else {
             uint32_t unused_team = 0;
-            if (!ge_probably_going_in(bpos, bvel, cfg.shot_min_score_time, cfg.pred_score_extra_margin, unused_team)) {
+            if (!ge_probably_going_in(bpos, bvel, cfg.shot_min_score_time, cfg.pred_score_extra_margin, M, unused_team)) {
                 // no longer going in: a save if the defending team touched it
                 uint32_t defending = (fl & GE_SHOT_GOAL_ORANGE) ? (uint32_t)RSG_TEAM_ORANGE : (uint32_t)RSG_TEAM_BLUE;
                 int s, p;

@@ -15,6 +15,8 @@
 
 #include <stdint.h>
 
+#include "rsg_state.h"
+
 namespace rsg {
 
 enum CarWord : int {
@@ -62,14 +64,30 @@ enum CarWord : int {
     CW_DEMO_BODY_ROT = 87, // 9  rigid-body basis while demolished (CW_ROT then
                            //    holds the basis last reported, which the
                            //    reference freezes at demolition)
-    CAR_WORDS = 96
+    // the rigid body's own mass properties and gravity, which the reference
+    // sets when the body is made and changes only through
+    // Arena::SetMutatorConfig (src/Sim/Arena/Arena.cpp:17-48); 0 in the mass
+    // words means the car default
+    CW_BODY_INV_MASS = 96,     // inverse mass
+    CW_BODY_INERTIA_MASS = 97, // mass the box inertia is computed with
+    CW_BODY_GRAVITY_ACC = 98,  // 3  btRigidBody::m_gravity_acceleration (m/s^2)
+    CW_BODY_GRAVITY = 101,     // 3  btRigidBody::m_gravity, the force applied each tick
+    CAR_WORDS = 104
 };
 
 enum BallWord : int {
     BW_POS = 0, BW_ROT = 3, BW_VEL = 12, BW_ANGVEL = 15, BW_TICKS_SINCE_SET = 18,
     BW_BP_CELL = 20, BW_BP_SEQ = 21,
     BW_HS = 22,          // 3  Heatseeker: target net, target speed, time since hit
-    BALL_WORDS = 25
+    // the ball body's shape and mass as last made (Ball::_BulletSetup,
+    // src/Sim/Ball/Ball.cpp:53-115), whether it is awake (not
+    // ISLAND_SLEEPING) and its gravity, as for cars
+    BW_RADIUS = 25,      // uu
+    BW_MASS = 26,
+    BW_AWAKE = 27,
+    BW_GRAVITY_ACC = 28, // 3
+    BW_GRAVITY = 31,     // 3
+    BALL_WORDS = 34
 };
 
 enum PadWord : int { PW_ACTIVE = 0, PW_COOLDOWN = 1, PW_PREV_LOCKED = 2, PAD_WORDS = 3 };
@@ -109,6 +127,8 @@ struct BatchDev {
     // per-arena world-step work space (ArenaSpace blocks, one per arena) for
     // the kernels that keep it in global memory; null when not allocated
     unsigned char* space;
+    // per-arena MutatorConfig (n records); null: the mode's defaults
+    const rsg_mutator_config* mut;
 };
 
 }  // namespace rsg

@@ -17,9 +17,12 @@
 #define RSG_FN inline
 #endif
 
-// Phase marks for an instrumented build (no-ops otherwise).
+// Phase marks and counters for an instrumented build (no-ops otherwise).
 #ifndef RSG_PROF_MARK
 #define RSG_PROF_MARK(id)
+#endif
+#ifndef RSG_PROF_COUNT
+#define RSG_PROF_COUNT(id, v)
 #endif
 
 namespace rsg {

@@ -28,7 +28,7 @@
 namespace rsg {
 
 constexpr uint32_t SNAP_EXT_MAGIC = 0x58475352u;  // "RSGX"
-constexpr uint32_t SNAP_EXT_VERSION = 2;
+constexpr uint32_t SNAP_EXT_VERSION = 3;
 
 static_assert(RSG_MAX_CARS * CAR_WORDS + BALL_WORDS + ARENA_WORDS <= RSG_SNAPSHOT_RAW_WORDS,
               "snapshot raw record too small");
@@ -105,27 +105,56 @@ inline void preset_config_words(int preset, float out[1 + 3 + 3 + 5 + 5]) {
     *o++ = r.back_rad;
 }
 
-// The default MutatorConfig of `mode` (MutatorConfig.h:16-75 with
-// MutatorConfig.cpp:5-40: Heatseeker spawns with 100 boost and uses none), in
-// MUTATOR_CONFIG_SERIALIZATION_FIELDS order.
-inline void write_default_mutators(ByteOut& o, int mode) {
-    o.put<uint16_t>(30);
-    o.count(30);
-    o.vec(0.f, 0.f, GRAVITY_Z);
-    const float f1[] = {CAR_MASS_BT, CARWORLD_COLLISION_FRICTION, CARWORLD_COLLISION_RESTITUTION, BALL_MASS_BT,
-                        BALL_MAX_SPEED, BALL_DRAG, BALL_FRICTION, BALL_RESTITUTION, JUMP_ACCEL,
-                        JUMP_IMMEDIATE_FORCE, BOOST_ACCEL_GROUND, BOOST_ACCEL_AIR, boost_used_per_second(mode),
-                        DEMO_RESPAWN_TIME, spawn_boost(mode), BUMP_COOLDOWN_TIME, PAD_COOLDOWN_BIG,
-                        PAD_COOLDOWN_SMALL, 1.f, 1.f, BALL_COLLISION_RADIUS_SOCCAR};
+// MutatorConfig::Serialize (MutatorConfig.cpp:42-45): the field count, then
+// MUTATOR_CONFIG_SERIALIZATION_FIELDS (MutatorConfig.h:81-88) in that order.
+constexpr int MUTATOR_FIELDS = 30;
+inline void write_mutators(ByteOut& o, const rsg_mutator_config& m) {
+    o.put<uint16_t>(MUTATOR_FIELDS);
+    o.count(MUTATOR_FIELDS);
+    o.vec(m.gravity);
+    const float f1[] = {m.car_mass, m.car_world_friction, m.car_world_restitution, m.ball_mass, m.ball_max_speed,
+                        m.ball_drag, m.ball_world_friction, m.ball_world_restitution, m.jump_accel,
+                        m.jump_immediate_force, m.boost_accel_ground, m.boost_accel_air, m.boost_used_per_second,
+                        m.respawn_delay, m.car_spawn_boost_amount, m.bump_cooldown_time, m.boost_pad_cooldown_big,
+                        m.boost_pad_cooldown_small, m.ball_hit_extra_force_scale, m.bump_force_scale, m.ball_radius};
     for (float f : f1) o.put<float>(f);
-    o.b8(false);  // unlimitedFlips
-    o.b8(false);  // unlimitedDoubleJumps
-    o.b8(false);  // rechargeBoostEnabled
-    o.put<float>(RECHARGE_BOOST_PER_SECOND);
-    o.put<float>(RECHARGE_BOOST_DELAY);
-    o.put<uint8_t>(0);  // demoMode NORMAL
-    o.b8(false);        // enableTeamDemos
-    o.put<float>(SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y);
+    o.b8(m.unlimited_flips);
+    o.b8(m.unlimited_double_jumps);
+    o.b8(m.recharge_boost_enabled);
+    o.put<float>(m.recharge_boost_per_second);
+    o.put<float>(m.recharge_boost_delay);
+    o.put<uint8_t>(m.demo_mode);
+    o.b8(m.enable_team_demos);
+    o.put<float>(m.goal_base_threshold_y);
+}
+
+// MutatorConfig::Deserialize (MutatorConfig.cpp:47-55), which refuses a
+// config written with another field count.
+inline std::string read_mutators(ByteIn& in, rsg_mutator_config& m) {
+    uint16_t fields = in.get<uint16_t>();
+    uint32_t count = in.get<uint32_t>();
+    if (in.bad) return "truncated";
+    if (fields != MUTATOR_FIELDS || count != (uint32_t)MUTATOR_FIELDS)
+        return "mutator config is from a different version of RocketSim, fields don't match";
+    in.vec(m.gravity);
+    float* f1[] = {&m.car_mass, &m.car_world_friction, &m.car_world_restitution, &m.ball_mass, &m.ball_max_speed,
+                   &m.ball_drag, &m.ball_world_friction, &m.ball_world_restitution, &m.jump_accel,
+                   &m.jump_immediate_force, &m.boost_accel_ground, &m.boost_accel_air, &m.boost_used_per_second,
+                   &m.respawn_delay, &m.car_spawn_boost_amount, &m.bump_cooldown_time, &m.boost_pad_cooldown_big,
+                   &m.boost_pad_cooldown_small, &m.ball_hit_extra_force_scale, &m.bump_force_scale, &m.ball_radius};
+    for (float* f : f1) *f = in.get<float>();
+    m.unlimited_flips = in.b8();
+    m.unlimited_double_jumps = in.b8();
+    m.recharge_boost_enabled = in.b8();
+    m.recharge_boost_per_second = in.get<float>();
+    m.recharge_boost_delay = in.get<float>();
+    m.demo_mode = in.get<uint8_t>();
+    m.enable_team_demos = in.b8();
+    m._pad[0] = m._pad[1] = m._pad[2] = 0;
+    m.goal_base_threshold_y = in.get<float>();
+    if (in.bad) return "truncated";
+    if (m.demo_mode > RSG_DEMO_DISABLED) return "unknown DemoMode";
+    return "";
 }
 
 inline void write_default_arena_config(ByteOut& o) {
@@ -280,7 +309,7 @@ inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<u
     o.b8(false);
     o.put<uint64_t>(0);
 
-    write_default_mutators(o, s.mode);
+    write_mutators(o, s.mutators);
 
     // extension block: the raw records (built from the state records when the
     // snapshot carries none)
@@ -317,8 +346,8 @@ inline void fresh_broadphase(rsg_arena_snapshot& s, const int* creation_rank) {
 }
 
 // Arena::DeserializeNew (Arena.cpp:575-644) into a snapshot.  Only what this
-// library simulates is accepted: Soccar or THE_VOID, the default ArenaConfig
-// and MutatorConfig, preset car configs.  States are as Car/Ball::SetState
+// library simulates is accepted: Soccar, Heatseeker or THE_VOID, the default
+// ArenaConfig, preset car configs.  States are as Car/Ball::SetState
 // leave them (tick counters 0, no hidden values) unless the extension block
 // is present; then they and the raw records come from it.
 inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_arena_snapshot& s) {
@@ -437,12 +466,8 @@ inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_ar
         in.raw(skip, sizeof(skip));
     }
     {
-        ByteOut want;
-        write_default_mutators(want, s.mode);
-        std::vector<uint8_t> got(want.d.size());
-        in.raw(got.data(), got.size());
-        if (in.bad) return "truncated";
-        if (got != want.d) return "non-default MutatorConfig (not supported)";
+        std::string err = read_mutators(in, s.mutators);
+        if (!err.empty()) return err;
     }
     fresh_broadphase(s, creation_rank);
 

@@ -39,6 +39,7 @@ struct Body {
     M3 inv_iw;         // world inverse inertia, R diag(invI) R^T
     V3 inv_i_local;
     float inv_mass;
+    V3 grav;           // gravity force (btRigidBody::m_gravity: acceleration times mass)
     bool active;       // takes part in gravity / solver / integration
     bool no_rot;       // ball: translation-only integration
 };
@@ -121,6 +122,7 @@ struct Car {
 
 struct Ball {
     Body b;
+    float radius_uu, mass;  // the body's sphere and mass as last made
     uint64_t ticks_since_set;
     V3 vel_cache;
     // BallState::HeatseekerInfo (src/Sim/Ball/Ball.h:24-31)
@@ -348,7 +350,7 @@ RSG_FN void car_update_wheels(Car& c, WheelT* wh, const CarPhys& cp, float dt, i
 }
 
 // _UpdateBoost (Car.cpp:486-533), Soccar mutators (no recharge).
-RSG_FN void car_update_boost(Car& c, float dt, float used_per_second) {
+RSG_FN void car_update_boost(Car& c, float dt, const rsg_mutator_config& M) {
     bool has_boost = c.boost > 0;
     bool boosting = flag(c.flags, RSG_CF_IS_BOOSTING);
     if (has_boost) {
@@ -363,18 +365,21 @@ RSG_FN void car_update_boost(Car& c, float dt, float used_per_second) {
     setflag(c.flags, RSG_CF_IS_BOOSTING, boosting);
     if (boosting) c.boosting_time += dt; else c.boosting_time = 0;
     if (boosting) {
-        c.boost = rs_max(c.boost - used_per_second * dt, 0.f);
-        float accel = flag(c.flags, RSG_CF_ON_GROUND) ? BOOST_ACCEL_GROUND : BOOST_ACCEL_AIR;
+        c.boost = rs_max(c.boost - M.boost_used_per_second * dt, 0.f);
+        float accel = flag(c.flags, RSG_CF_ON_GROUND) ? M.boost_accel_ground : M.boost_accel_air;
         body_apply_central_force(c.b, accel * UU_TO_BT * car_forward(c) * CAR_MASS_BT);
         c.time_since_boosted = 0;
     } else {
         c.time_since_boosted += dt;
+        // recharge (MutatorConfig::rechargeBoostEnabled)
+        if (M.recharge_boost_enabled && c.time_since_boosted >= M.recharge_boost_delay)
+            c.boost += M.recharge_boost_per_second * dt;
     }
     c.boost = rs_min(c.boost, BOOST_MAX);
 }
 
 // _UpdateJump (Car.cpp:535-582)
-RSG_FN void car_update_jump(Car& c, float dt, bool jump_pressed) {
+RSG_FN void car_update_jump(Car& c, float dt, bool jump_pressed, const rsg_mutator_config& M) {
     bool on_ground = flag(c.flags, RSG_CF_ON_GROUND);
     bool is_jumping = flag(c.flags, RSG_CF_IS_JUMPING);
     bool has_jumped = flag(c.flags, RSG_CF_HAS_JUMPED);
@@ -390,12 +395,12 @@ RSG_FN void car_update_jump(Car& c, float dt, bool jump_pressed) {
     } else if (on_ground && jump_pressed) {
         is_jumping = true;
         c.jump_time = 0;
-        V3 f = car_up(c) * JUMP_IMMEDIATE_FORCE * UU_TO_BT * CAR_MASS_BT;
+        V3 f = car_up(c) * M.jump_immediate_force * UU_TO_BT * CAR_MASS_BT;
         body_apply_central_impulse(c.b, f);
     }
     if (is_jumping) {
         has_jumped = true;
-        V3 total = car_up(c) * JUMP_ACCEL;
+        V3 total = car_up(c) * M.jump_accel;
         if (c.jump_time < JUMP_MIN_TIME) total = total * JUMP_PRE_MIN_ACCEL_SCALE;
         body_apply_central_force(c.b, total * UU_TO_BT * CAR_MASS_BT);
     }
@@ -458,7 +463,8 @@ RSG_FN void car_update_air_torque(Car& c, float dt, bool update_air_control) {
 }
 
 // _UpdateDoubleJumpOrFlip (Car.cpp:671-781)
-RSG_FN void car_update_double_jump_or_flip(Car& c, float dt, bool jump_pressed, float fwd_speed_uu) {
+RSG_FN void car_update_double_jump_or_flip(Car& c, float dt, bool jump_pressed, float fwd_speed_uu,
+                                            const rsg_mutator_config& M) {
     if (flag(c.flags, RSG_CF_ON_GROUND)) {
         setflag(c.flags, RSG_CF_HAS_DOUBLE_JUMPED, false);
         setflag(c.flags, RSG_CF_HAS_FLIPPED, false);
@@ -475,7 +481,8 @@ RSG_FN void car_update_double_jump_or_flip(Car& c, float dt, bool jump_pressed, 
         if (jump_pressed && c.air_time_since_jump < DOUBLEJUMP_MAX_DELAY) {
             float in_mag = fabsf(c.ctrl.yaw) + fabsf(c.ctrl.pitch) + fabsf(c.ctrl.roll);
             bool is_flip = in_mag >= 0.5f;  // CarConfig::dodgeDeadzone
-            bool can_use = !flag(c.flags, RSG_CF_HAS_DOUBLE_JUMPED) && !flag(c.flags, RSG_CF_HAS_FLIPPED);
+            bool can_use = (!flag(c.flags, RSG_CF_HAS_DOUBLE_JUMPED) && !flag(c.flags, RSG_CF_HAS_FLIPPED)) ||
+                           (is_flip ? M.unlimited_flips : M.unlimited_double_jumps);
             if (flag(c.flags, RSG_CF_AUTO_FLIPPING)) can_use = false;
             if (can_use) {
                 if (is_flip) {

@@ -150,19 +150,23 @@ RSG_FN void store_car(const CarRef& r, const Car& c) {
     r.u(CW_GEN_TICKS_LEFT) = c.gen_left;
 }
 
-RSG_FN void car_init_body(Car& c) {
+// The car body's mass properties: the preset's unless the arena's
+// SetMutatorConfig replaced them (`inv_mass`, `inertia_mass` nonzero).
+RSG_FN void car_init_body(Car& c, float inv_mass, float inertia_mass) {
     CarPhys cp = make_car_phys(c.preset);
     c.b.force = v3zero();
     c.b.torque = v3zero();
     c.b.inv_i_local = cp.inv_inertia_local;
-    c.b.inv_mass = cp.inv_mass;
+    if (inertia_mass != 0.f)
+        c.b.inv_i_local = box_inv_inertia(cp.box_implicit + v3(cp.box_margin, cp.box_margin, cp.box_margin), inertia_mass);
+    c.b.inv_mass = inv_mass != 0.f ? inv_mass : cp.inv_mass;
     c.b.no_rot = false;
     c.b.active = true;
     body_update_inertia(c.b);
 }
 
 RSG_FN void ball_init_body(Ball& b) {
-    BallPhys bp = make_ball_phys();
+    BallPhys bp = make_ball_phys(b.radius_uu, b.mass);
     b.b.force = v3zero();
     b.b.torque = v3zero();
     b.b.inv_i_local = v3(bp.inv_inertia, bp.inv_inertia, bp.inv_inertia);
@@ -178,6 +182,7 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.tick = ar.u64(AW_TICK);
     A.n_cars = (int)ar.u(AW_NUM_CARS);
     A.mode = d.mode;
+    A.mut = d.mut ? d.mut[a] : mutator_defaults(d.mode);
     A.goals_blue = ar.u(AW_GOALS_BLUE);
     A.goals_orange = ar.u(AW_GOALS_ORANGE);
     A.last_goal_tick = ar.u64(AW_LAST_GOAL_TICK);
@@ -209,7 +214,8 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     for (int i = 0; i < A.n_cars; i++) {
         CarRef cr = car_ref(d, a, i);
         load_car(cr, A.cars[i]);
-        car_init_body(A.cars[i]);
+        car_init_body(A.cars[i], cr.f(CW_BODY_INV_MASS), cr.f(CW_BODY_INERTIA_MASS));
+        A.cars[i].b.grav = cr.v(CW_BODY_GRAVITY);
         A.cars[i].b.active = !flag(A.cars[i].flags, RSG_CF_BODY_DISABLED);
         load_bp(cr, CW_BP_CELL, CW_BP_SEQ, A.bp_cell[1 + i], A.bp_seq[1 + i]);
     }
@@ -219,7 +225,11 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.ball.b.vel = br.v(BW_VEL);
     A.ball.b.ang = br.v(BW_ANGVEL);
     A.ball.ticks_since_set = br.u64(BW_TICKS_SINCE_SET);
+    A.ball.radius_uu = br.f(BW_RADIUS);
+    A.ball.mass = br.f(BW_MASS);
     ball_init_body(A.ball);
+    A.ball.b.active = br.u(BW_AWAKE) != 0;
+    A.ball.b.grav = br.v(BW_GRAVITY);
     A.ball.hs_dir = br.f(BW_HS);
     A.ball.hs_speed = br.f(BW_HS + 1);
     A.ball.hs_time = br.f(BW_HS + 2);
@@ -266,6 +276,7 @@ RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
     br.sf(BW_HS, A.ball.hs_dir);
     br.sf(BW_HS + 1, A.ball.hs_speed);
     br.sf(BW_HS + 2, A.ball.hs_time);
+    br.u(BW_AWAKE) = A.ball.b.active ? 1u : 0u;
     if (mode_has_arena(d.mode)) {
         for (int p = 0; p < PAD_NUM; p++) {
             CarRef pr = pad_ref(d, a, p);
@@ -306,7 +317,7 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
             sp.y += 250.f * num_at_respawn[ri];
             num_at_respawn[ri]++;
         }
-        car_respawn_state(c, 0, spawn_boost(A.mode));
+        car_respawn_state(c, 0, A.mut.car_spawn_boost_amount);
         V3 pos = v3(sp.x, sp.y, CAR_SPAWN_REST_Z);
         float yaw = sp.yaw;
         if (c.team != RSG_TEAM_BLUE) {
@@ -330,6 +341,7 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
         const float sy = ((h >> 56) & 1u) ? 1.f : -1.f;
         A.ball.b.pos = v3(HS_BALL_START_X, HS_BALL_START_Y * sy, HS_BALL_START_Z) * UU_TO_BT;
         A.ball.b.vel = v3(0.f, HS_BALL_START_VY * sy, HS_BALL_START_VZ) * UU_TO_BT;
+        A.ball.b.active = true;  // Ball::SetState wakes a moving ball
     }
     A.ball.hs_dir = 0;
     A.ball.hs_speed = HS_INITIAL_TARGET_SPEED;
@@ -446,7 +458,7 @@ RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int c
         c.demo_timer = rs_max(c.demo_timer - dt, 0.f);
         if (c.demo_timer == 0) {
             int slot = (int)(rsg_draw(P.seed ^ 0xDE40ull, P.arena_id, (uint32_t)ci, (uint32_t)A.tick, 9) % 4u);
-            car_respawn_state(c, slot, spawn_boost(P.mode));
+            car_respawn_state(c, slot, A.mut.car_spawn_boost_amount);
         }
         c.b.active = false;  // DISABLE_SIMULATION + no contact response
     } else {
@@ -479,14 +491,14 @@ RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int c
         car_update_air_torque(c, dt, n_contact == 0);
     else
         setflag(c.flags, RSG_CF_IS_FLIPPING, false);
-    car_update_jump(c, dt, jump_pressed);
+    car_update_jump(c, dt, jump_pressed, A.mut);
     car_update_auto_flip(c, dt, jump_pressed);
-    car_update_double_jump_or_flip(c, dt, jump_pressed, fwd_speed_uu);
+    car_update_double_jump_or_flip(c, dt, jump_pressed, fwd_speed_uu, A.mut);
     if (c.ctrl.throttle != 0.f && ((n_contact > 0 && n_contact < 4) || flag(c.flags, RSG_CF_WORLD_CONTACT)))
         car_update_auto_roll(c, wh, n_contact);
     setflag(c.flags, RSG_CF_WORLD_CONTACT, false);
     vehicle_second(c, wh, cp, dt);
-    car_update_boost(c, dt, boost_used_per_second(P.mode));
+    car_update_boost(c, dt, A.mut);
 }
 
 // Whether the car pre-ticks of this tick are independent of their order, so
@@ -582,7 +594,7 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
                     float add = pad_is_big(p) ? PAD_BOOST_AMOUNT_BIG : PAD_BOOST_AMOUNT_SMALL;
                     c.boost = rs_min(c.boost + add, BOOST_MAX);
                     pad.active = false;
-                    pad.cooldown = pad_is_big(p) ? PAD_COOLDOWN_BIG : PAD_COOLDOWN_SMALL;
+                    pad.cooldown = pad_is_big(p) ? A.mut.boost_pad_cooldown_big : A.mut.boost_pad_cooldown_small;
                 }
             }
             pad.prev_locked = locked_id;
@@ -597,7 +609,7 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
             A.ball.vel_cache = v3zero();
         }
         V3 v = b.vel, w = b.ang;
-        float max_bt = BALL_MAX_SPEED * UU_TO_BT;
+        float max_bt = A.mut.ball_max_speed * UU_TO_BT;
         if (len2(v) > max_bt * max_bt) v = normalized(v) * max_bt;
         if (len2(w) > (BALL_MAX_ANG_SPEED * BALL_MAX_ANG_SPEED)) w = normalized(w) * BALL_MAX_ANG_SPEED;
         b.vel = v;
@@ -611,7 +623,7 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
     bool scored = false;
     if (soccar) {
         float y = A.ball.b.pos.y * BT_TO_UU;
-        if (fabsf(y) > (SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y + BALL_COLLISION_RADIUS_SOCCAR)) {
+        if (fabsf(y) > (A.mut.goal_base_threshold_y + A.mut.ball_radius)) {
             scored = true;
             if (y > 0) A.goals_blue++; else A.goals_orange++;
             A.last_goal_tick = A.tick;

@@ -113,6 +113,31 @@ class PadState(C.Structure):
     _fields_ = [("is_active", C.c_uint32), ("cooldown", C.c_float), ("prev_locked_car_id", C.c_uint32)]
 
 
+DEMO_NORMAL, DEMO_ON_CONTACT, DEMO_DISABLED = 0, 1, 2
+
+
+class MutatorConfig(C.Structure):
+    """rsg_mutator_config: MutatorConfig (src/Sim/MutatorConfig/MutatorConfig.h:16-79)."""
+    _fields_ = [
+        ("gravity", C.c_float * 3), ("car_mass", C.c_float),
+        ("car_world_friction", C.c_float), ("car_world_restitution", C.c_float),
+        ("ball_mass", C.c_float), ("ball_max_speed", C.c_float), ("ball_drag", C.c_float),
+        ("ball_world_friction", C.c_float), ("ball_world_restitution", C.c_float),
+        ("jump_accel", C.c_float), ("jump_immediate_force", C.c_float),
+        ("boost_accel_ground", C.c_float), ("boost_accel_air", C.c_float), ("boost_used_per_second", C.c_float),
+        ("respawn_delay", C.c_float), ("bump_cooldown_time", C.c_float),
+        ("boost_pad_cooldown_big", C.c_float), ("boost_pad_cooldown_small", C.c_float),
+        ("car_spawn_boost_amount", C.c_float),
+        ("ball_hit_extra_force_scale", C.c_float), ("bump_force_scale", C.c_float),
+        ("ball_radius", C.c_float),
+        ("unlimited_flips", C.c_uint8), ("unlimited_double_jumps", C.c_uint8),
+        ("recharge_boost_enabled", C.c_uint8), ("demo_mode", C.c_uint8),
+        ("recharge_boost_per_second", C.c_float), ("recharge_boost_delay", C.c_float),
+        ("enable_team_demos", C.c_uint8), ("_pad", C.c_uint8 * 3),
+        ("goal_base_threshold_y", C.c_float),
+    ]
+
+
 class ArenaSnapshot(C.Structure):
     """rsg_arena_snapshot (include/rsg_state.h): what Arena::Serialize writes
     for one arena (src/Sim/Arena/Arena.cpp:546-573) plus the hidden values."""
@@ -122,7 +147,7 @@ class ArenaSnapshot(C.Structure):
         ("car_id", C.c_uint32 * MAX_CARS), ("car_team", C.c_uint8 * MAX_CARS),
         ("car_preset", C.c_uint8 * MAX_CARS), ("_pad", C.c_uint8 * 6),
         ("controls", Controls * MAX_CARS), ("cars", CarState * MAX_CARS),
-        ("ball", BallState), ("pads", PadState * NUM_PADS),
+        ("ball", BallState), ("pads", PadState * NUM_PADS), ("mutators", MutatorConfig),
         ("raw_layout", C.c_uint32), ("raw_words", C.c_uint32), ("raw", C.c_uint32 * SNAPSHOT_RAW_WORDS),
     ]
 
@@ -153,6 +178,7 @@ class BatchDesc(C.Structure):
         ("cars_per_arena", C.POINTER(C.c_int)),
         ("car_team", C.POINTER(C.c_uint8)),
         ("car_preset", C.POINTER(C.c_uint8)),
+        ("mutators", C.POINTER(MutatorConfig)),
     ]
 
 

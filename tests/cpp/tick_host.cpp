@@ -53,6 +53,7 @@ struct HostBatch {
     HostMesh mesh;
     std::vector<int> cars_per_arena, car_offset;
     std::vector<uint32_t> car, ball, pad, arena;
+    std::vector<rsg_mutator_config> mut;
     BatchDev dev{};
 };
 
@@ -61,9 +62,10 @@ struct HostBatch {
 extern "C" {
 
 // A host batch of n arenas (cars_per_arena[a] cars each, teams and presets
-// per car); cmf = the arena mesh file (NULL for THE_VOID).
+// per car); cmf = the arena mesh file (NULL for THE_VOID); mut = the
+// MutatorConfig set before the cars are added (NULL: the mode's defaults).
 void* th_create(int mode, int n, const int* cars_per_arena, const uint8_t* team, const uint8_t* preset,
-                const void* cmf, uint64_t cmf_size, float tick_rate) {
+                const void* cmf, uint64_t cmf_size, float tick_rate, const rsg_mutator_config* mut) {
     auto* h = new HostBatch();
     if (cmf && !build_host_mesh(cmf, cmf_size, h->mesh).empty()) {
         delete h;
@@ -89,9 +91,21 @@ void* th_create(int mode, int n, const int* cars_per_arena, const uint8_t* team,
     d.max_cars = maxc;
     d.mode = mode;
     d.tick_time = 1 / tick_rate;
+    h->mut.assign((size_t)n, mut ? *mut : mutator_defaults(mode));
+    d.mut = h->mut.data();
     for (int a = 0; a < n; a++)
         init_arena_record(d, a, cars_per_arena[a], team + h->car_offset[a], preset + h->car_offset[a]);
     return h;
+}
+
+// Arena::SetMutatorConfig on every arena (rsg_batch_set_mutator_config's
+// host side).
+void th_set_mutators(void* p, const rsg_mutator_config* m) {
+    HostBatch& h = *(HostBatch*)p;
+    for (int a = 0; a < h.dev.n; a++) {
+        apply_mutator_config(h.dev, a, h.mut[(size_t)a], *m);
+        h.mut[(size_t)a] = *m;
+    }
 }
 
 void th_destroy(void* p) { delete (HostBatch*)p; }

@@ -270,20 +270,27 @@ def sample_soccar_drive(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1):
     return out
 
 
-def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1, near_walls=False):
+def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1, near_walls=False, mutators=None,
+                        mutators_first=False):
     """Cars placed 250-700 uu around the ball and driven at it (some
     supersonic, some jumping): car-ball hits with the extra impulse,
     car-car box-box contacts, bumps and demolitions, wheels resting on
     other cars.  With near_walls the ball starts by a side wall, a back wall
-    or a corner, so cars drive onto the mesh's fillets and walls."""
+    or a corner, so cars drive onto the mesh's fillets and walls.  With
+    `mutators` the arena's MutatorConfig is set before its cars are added
+    (`mutators_first`) or once they are."""
     rng = random.Random(seed)
     out = []
     for _ in range(n_traj):
         A = ref.RefArena(MODE_SOCCAR)
         teams = [i % 2 for i in range(n_cars)]
         presets = [rng.randint(0, 5) for _ in range(n_cars)]
+        if mutators is not None and mutators_first:
+            A.set_mutators(mutators)
         for t, p in zip(teams, presets):
             A.add_car(t, p)
+        if mutators is not None and not mutators_first:
+            A.set_mutators(mutators)
         bx, by = rng.uniform(-3000, 3000), rng.uniform(-4000, 4000)
         if near_walls:
             bx, by = rng.choice([-1, 1]) * rng.uniform(3300, 3800), rng.uniform(-4500, 4500)
@@ -603,18 +610,22 @@ def load_samples(path):
     return out
 
 
-def run_on_gpu(samples, mode, kernel=0, per_block=0, auto_reset=False, seed=0):
+def run_on_gpu(samples, mode, kernel=0, per_block=0, auto_reset=False, seed=0, mutators=None, mutators_first=False):
     """Step every sample once on the GPU, one sample per arena.  `kernel` and
     `per_block` pick the step kernel and arenas per workgroup (0: the
     library's automatic choice); `auto_reset` / `seed` set the batch's kickoff
-    reset on goal."""
+    reset on goal; `mutators` is every arena's MutatorConfig, given at
+    creation (`mutators_first`: before the cars are added) or set after."""
     from rocketsim_amd.batch import ArenaBatch
     cpa = [len(s.teams) for s in samples]
     batch = ArenaBatch(cpa, sum((s.teams for s in samples), []), sum((s.presets for s in samples), []),
-                       mode=mode, auto_reset=auto_reset, seed=seed)
+                       mode=mode, auto_reset=auto_reset, seed=seed,
+                       mutators=mutators if mutators_first else None)
     try:
         batch.set_kernel(kernel)
         batch.set_launch(per_block)
+        if mutators is not None and not mutators_first:
+            batch.set_mutator_config(mutators)
         batch.set_cars(sum((list(s.cars_in) for s in samples), []))
         batch.set_balls([s.ball_in for s in samples])
         batch.set_pads(sum((list(s.pads_in) for s in samples), []))

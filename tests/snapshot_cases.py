@@ -38,6 +38,7 @@ def snapshot_from_ref(A, presets, mode):
     if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
         for p, pd in enumerate(A.get_pads()):
             s.pads[p] = pd
+    s.mutators = A.get_mutators()
     return s
 
 

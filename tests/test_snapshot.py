@@ -155,10 +155,14 @@ def test_deserialize_refuses_unsupported(ref):
     bad_cfg = rb[:cfg_at] + struct.pack("<f", 14.0) + rb[cfg_at + 4:]
     with pytest.raises(RuntimeError, match="preset"):
         deserialize_snapshot(bad_cfg)
-    # mutator config: gravity z changed
-    g_at = len(rb) - snapshot_cases.MUTATOR_BYTES + 2 + 4 + 8
-    bad_mut = rb[:g_at] + struct.pack("<f", -500.0) + rb[g_at + 4:]
-    with pytest.raises(RuntimeError, match="MutatorConfig"):
+    # mutator config: any values are read (gravity z here); a config with
+    # another field count is refused, as MutatorConfig::Deserialize does
+    m_at = len(rb) - snapshot_cases.MUTATOR_BYTES
+    g_at = m_at + 2 + 4 + 8
+    low_g = rb[:g_at] + struct.pack("<f", -500.0) + rb[g_at + 4:]
+    assert deserialize_snapshot(low_g).mutators.gravity[2] == -500.0
+    bad_mut = rb[:m_at] + struct.pack("<H", 29) + rb[m_at + 2:]
+    with pytest.raises(RuntimeError, match="different version"):
         deserialize_snapshot(bad_mut)
 
 

@@ -6,7 +6,8 @@ import ctypes as C
 import os
 import subprocess
 
-from rocketsim_amd.state import BallState, CarState, Controls, MODE_HEATSEEKER, MODE_SOCCAR, NUM_PADS, PadState
+from rocketsim_amd.state import (BallState, CarState, Controls, MODE_HEATSEEKER, MODE_SOCCAR, MutatorConfig, NUM_PADS,
+                                 PadState)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -48,7 +49,7 @@ def load(path):
     L = C.CDLL(path)
     L.th_create.restype = C.c_void_p
     L.th_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
-                            C.c_char_p, C.c_uint64, C.c_float]
+                            C.c_char_p, C.c_uint64, C.c_float, C.POINTER(MutatorConfig)]
     L.th_destroy.argtypes = [C.c_void_p]
     for name, t in (("cars", CarState), ("balls", BallState), ("pads", PadState)):
         getattr(L, "th_set_" + name).argtypes = [C.c_void_p, C.POINTER(t)]
@@ -60,6 +61,7 @@ def load(path):
     L.th_get_status.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
     L.th_step.argtypes = [C.c_void_p, C.c_int]
     L.th_rsqrt_signature.argtypes = [C.POINTER(C.c_float)]
+    L.th_set_mutators.argtypes = [C.c_void_p, C.POINTER(MutatorConfig)]
     return L
 
 
@@ -78,10 +80,12 @@ def same_rsqrt_as_recording_cpu():
     return [struct.unpack("<I", struct.pack("<f", x))[0] for x in out] == RECORDING_CPU_RSQRT
 
 
-def run_samples(L, samples, mode, cmf=None, inject=None):
+def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators_first=False):
     """Step every teacher-forced sample once on the host build, one sample per
     arena (the host counterpart of parity.run_on_gpu).  `inject(cars, balls)`
-    may edit the "before" records.  Returns (results, status words)."""
+    may edit the "before" records; `mutators` is every arena's MutatorConfig,
+    set before its cars are added (`mutators_first`) or after, as
+    Arena::SetMutatorConfig.  Returns (results, status words)."""
     if mode in (MODE_SOCCAR, MODE_HEATSEEKER) and cmf is None:
         from rocketsim_amd import arena_mesh
         cmf = arena_mesh.soccar_cmf()
@@ -91,10 +95,13 @@ def run_samples(L, samples, mode, cmf=None, inject=None):
     presets = sum((list(s.presets) for s in samples), [])
     total = sum(cpa)
     h = L.th_create(mode, n, (C.c_int * n)(*cpa), (C.c_uint8 * max(total, 1))(*teams),
-                    (C.c_uint8 * max(total, 1))(*presets), cmf, len(cmf) if cmf else 0, 120.0)
+                    (C.c_uint8 * max(total, 1))(*presets), cmf, len(cmf) if cmf else 0, 120.0,
+                    C.byref(mutators) if mutators is not None and mutators_first else None)
     if not h:
         raise RuntimeError("th_create failed")
     try:
+        if mutators is not None and not mutators_first:
+            L.th_set_mutators(h, C.byref(mutators))
         cars = (CarState * max(total, 1))(*sum((list(s.cars_in) for s in samples), []))
         balls = (BallState * n)(*[s.ball_in for s in samples])
         if inject:
