@@ -1050,12 +1050,11 @@ struct World {
     // -----------------------------------------------------------------------
     // Islands + solve
 
-    RSG_FN void solve_islands(float dt) {
+    // Union-find over the dynamic bodies joined by overlapping pairs with
+    // contact response (btSimulationIslandManager::findUnions); the
+    // reference's set sizes never decide a union, so none are kept.
+    RSG_FN void island_unions(int* uf_id) {
         const int nb = num_bodies();
-        IslandTables& I = ws->isl;
-        int* uf_id = I.uf_id;
-        // union-find over non-static objects (index = body index); the
-        // reference's set sizes never decide a union, so none are kept
         for (int i = 0; i < nb; i++) uf_id[i] = i;
         for (int k = 0; k < T->n_pairs; k++) {
             int a = T->pairs[k].a, b = T->pairs[k].b;
@@ -1065,6 +1064,44 @@ struct World {
             if (i == j) continue;
             uf_id[i] = j;
         }
+    }
+
+    // The islands and solver of a tick in which no manifold holds a point:
+    // no rows, so the solver's write-back leaves each awake body's velocity
+    // plus its external force and torque impulses (the full path's
+    // expressions, with zero solver deltas) and its pose as it was.  Islands
+    // only decide whether a sleeping ball wakes: it does when its island
+    // holds an awake body.
+    RSG_FN void solve_without_contacts(float dt) {
+        const int nb = num_bodies();
+        if (!A.ball.b.active) {
+            int* uf_id = ws->isl.uf_id;
+            island_unions(uf_id);
+            const int ball_root = uf_find(uf_id, 0);
+            for (int i = 1; i < nb; i++)
+                if (A.cars[i - 1].b.active && uf_find(uf_id, i) == ball_root) { A.ball.b.active = true; break; }
+        }
+        for (int i = 0; i < nb; i++) {
+            Body& b = *body_ptr(i);
+            if (!b.active || b.inv_mass == 0.f) continue;
+            const V3 ext_f = b.force * b.inv_mass * dt;
+            const V3 ext_t = mul_vm(b.torque, b.inv_iw) * dt;
+            b.vel = (b.vel + v3zero()) + ext_f;
+            b.ang = (b.ang + v3zero()) + ext_t;
+        }
+    }
+
+    RSG_FN void solve_islands(float dt) {
+        bool any_point = false;
+        for (int m = 0; m < T->n_mf; m++) any_point = any_point || mf[m].n > 0;
+        if (!any_point) {
+            solve_without_contacts(dt);
+            return;
+        }
+        const int nb = num_bodies();
+        IslandTables& I = ws->isl;
+        int* uf_id = I.uf_id;
+        island_unions(uf_id);
         int* tag = I.tag;
         for (int i = 0; i < nb; i++) tag[i] = uf_find(uf_id, i);
         // sortIslands: elements sorted by island id (quicksort)
