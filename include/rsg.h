@@ -39,11 +39,14 @@ typedef struct rsg_batch_desc {
     int synth_controls;           /* draw controls on device each tick (include/rsg_workload.h) */
     const int* cars_per_arena;    /* n_arenas entries (each 0..RSG_MAX_CARS) */
     const uint8_t* car_team;      /* RSG_TEAM_*, one per car */
-    const uint8_t* car_preset;    /* RSG_PRESET_*, one per car */
+    const uint8_t* car_preset;    /* RSG_PRESET_*, one per car (ignored with car_configs) */
     /* MutatorConfig of every arena, set before its cars are added (Arena::Create,
      * SetMutatorConfig, then AddCar: the ball is made with the config's radius
      * and mass, cars keep the car mass); NULL for the mode's defaults */
     const rsg_mutator_config* mutators;
+    /* each car's CarConfig (AddCar(team, config), Arena.h:71), one per car;
+     * NULL for the presets of car_preset */
+    const rsg_car_config* car_configs;
 } rsg_batch_desc;
 
 /* Last error message of the calling thread ("" if none). */
@@ -231,6 +234,10 @@ int rsg_shot_rollout_backward(int n, int ticks, int ckpt_every, float tick_rate,
 /* Arena::IsBallScored (Arena.cpp:893-901) per arena, with the arena's
  * goalBaseThresholdY and ballRadius. */
 int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out);
+
+/* The CarConfig presets CAR_CONFIG_OCTANE ... CAR_CONFIG_PSYCLOPS
+ * (src/Sim/Car/CarConfig/CarConfig.cpp:93-100), by RSG_PRESET_*. */
+int rsg_car_config_preset(int preset, rsg_car_config* out);
 
 /* MutatorConfig(GameMode) (src/Sim/MutatorConfig/MutatorConfig.cpp:5-40):
  * the defaults of `mode` (RSG_MODE_*). */

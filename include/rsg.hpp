@@ -47,9 +47,19 @@ inline MutatorConfig DefaultMutatorConfig(GameMode mode) {
 // RocketSim::InitFromMem (src/RocketSim.h:32) for the Soccar collision mesh.
 inline void InitFromMem(const void* cmf_data, uint64_t size) { check(rsg_init_mesh(cmf_data, size)); }
 
+using CarConfig = rsg_car_config;
+
+// The reference's CAR_CONFIG_* presets (CarConfig.cpp:93-100).
+inline CarConfig PresetCarConfig(int preset) {
+    CarConfig c;
+    check(rsg_car_config_preset(preset, &c));
+    return c;
+}
+
 struct CarSpec {
     Team team;
-    int preset;  // RSG_PRESET_*
+    int preset = RSG_PRESET_OCTANE;  // RSG_PRESET_*
+    std::optional<CarConfig> config;  // any CarConfig, in place of the preset
 };
 
 struct BatchConfig {
@@ -69,13 +79,19 @@ public:
     ArenaBatch(const std::vector<std::vector<CarSpec>>& arenas, const BatchConfig& cfg = {}) {
         std::vector<int> counts;
         std::vector<uint8_t> teams, presets;
+        std::vector<CarConfig> configs;
+        bool custom = false;
         for (const auto& a : arenas) {
             counts.push_back((int)a.size());
             for (const auto& c : a) {
                 teams.push_back((uint8_t)c.team);
                 presets.push_back((uint8_t)c.preset);
+                custom = custom || c.config.has_value();
             }
         }
+        if (custom)
+            for (const auto& a : arenas)
+                for (const auto& c : a) configs.push_back(c.config ? *c.config : PresetCarConfig(c.preset));
         rsg_batch_desc d{};
         d.n_arenas = (int)arenas.size();
         d.mode = (int)cfg.mode;
@@ -87,6 +103,7 @@ public:
         d.car_team = teams.data();
         d.car_preset = presets.data();
         d.mutators = cfg.mutators ? &*cfg.mutators : nullptr;
+        d.car_configs = custom ? configs.data() : nullptr;
         h_ = rsg_batch_create(&d);
         if (!h_) throw std::runtime_error(std::string("rsg: ") + rsg_last_error());
     }
@@ -168,7 +185,8 @@ public:
     CarControls controls{};
     uint32_t id = 0;
     Team team = Team::BLUE;
-    int preset = RSG_PRESET_OCTANE;
+    int preset = RSG_PRESET_OCTANE;  // RSG_PRESET_CUSTOM when `config` is none of the presets
+    CarConfig config{};
     inline CarState GetState();
     inline void SetState(const CarState& s);
 
@@ -207,7 +225,14 @@ public:
     // kickoff spawn slot (the reference respawns it at a random respawn slot).
     Car* AddCar(Team team, int preset = RSG_PRESET_OCTANE) {
         std::vector<CarSpec> specs = specs_;
-        specs.push_back({team, preset});
+        specs.push_back({team, preset, std::nullopt});
+        rebuild(specs, -1);
+        return cars_.back().get();
+    }
+    // Arena::AddCar(team, config) with any CarConfig (Arena.h:71).
+    Car* AddCar(Team team, const CarConfig& config) {
+        std::vector<CarSpec> specs = specs_;
+        specs.push_back({team, RSG_PRESET_OCTANE, config});
         rebuild(specs, -1);
         return cars_.back().get();
     }
@@ -315,6 +340,7 @@ private:
                 snap.car_id[k] = snap.car_id[i];
                 snap.car_team[k] = snap.car_team[i];
                 snap.car_preset[k] = snap.car_preset[i];
+                snap.car_config[k] = snap.car_config[i];
                 snap.controls[k] = snap.controls[i];
                 snap.cars[k] = snap.cars[i];
                 k++;
@@ -327,7 +353,8 @@ private:
             std::unique_ptr<Car> c(new Car());
             c->arena_ = this;
             c->team = specs[i].team;
-            c->preset = specs[i].preset;
+            c->config = specs[i].config ? *specs[i].config : PresetCarConfig(specs[i].preset);
+            c->preset = specs[i].config ? RSG_PRESET_CUSTOM : specs[i].preset;
             cars.push_back(std::move(c));
         }
         // ids: carried over from the old cars; new cars continue the count

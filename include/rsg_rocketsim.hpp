@@ -212,6 +212,40 @@ inline RocketSim::BoostPadState FromRsg(const rsg_pad_state& s) {
     return p;
 }
 
+// CarConfig, field for field (src/Sim/Car/CarConfig/CarConfig.h:6-43).
+inline rsg_car_config ToRsg(const RocketSim::CarConfig& c) {
+    rsg_car_config s;
+    std::memset(&s, 0, sizeof(s));
+    ToRsg(c.hitboxSize, s.hitbox_size);
+    ToRsg(c.hitboxPosOffset, s.hitbox_pos_offset);
+    const RocketSim::WheelPairConfig* src[2] = {&c.frontWheels, &c.backWheels};
+    rsg_wheel_pair_config* dst[2] = {&s.front_wheels, &s.back_wheels};
+    for (int k = 0; k < 2; k++) {
+        dst[k]->wheel_radius = src[k]->wheelRadius;
+        dst[k]->suspension_rest_length = src[k]->suspensionRestLength;
+        ToRsg(src[k]->connectionPointOffset, dst[k]->connection_point_offset);
+    }
+    s.three_wheels = c.threeWheels ? 1 : 0;
+    s.dodge_deadzone = c.dodgeDeadzone;
+    return s;
+}
+
+inline RocketSim::CarConfig FromRsg(const rsg_car_config& s) {
+    RocketSim::CarConfig c;
+    c.hitboxSize = VecFromRsg(s.hitbox_size);
+    c.hitboxPosOffset = VecFromRsg(s.hitbox_pos_offset);
+    RocketSim::WheelPairConfig* dst[2] = {&c.frontWheels, &c.backWheels};
+    const rsg_wheel_pair_config* src[2] = {&s.front_wheels, &s.back_wheels};
+    for (int k = 0; k < 2; k++) {
+        dst[k]->wheelRadius = src[k]->wheel_radius;
+        dst[k]->suspensionRestLength = src[k]->suspension_rest_length;
+        dst[k]->connectionPointOffset = VecFromRsg(src[k]->connection_point_offset);
+    }
+    c.threeWheels = s.three_wheels != 0;
+    c.dodgeDeadzone = s.dodge_deadzone;
+    return c;
+}
+
 // MutatorConfig, field for field (src/Sim/MutatorConfig/MutatorConfig.h:16-79).
 inline rsg_mutator_config ToRsg(const RocketSim::MutatorConfig& m) {
     rsg_mutator_config s;

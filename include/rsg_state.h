@@ -37,12 +37,33 @@ extern "C" {
  * batched step. */
 enum { RSG_MODE_SOCCAR = 0, RSG_MODE_HEATSEEKER = 2, RSG_MODE_THE_VOID = 5 };
 
-/* Car hitbox presets, same order as src/Sim/Car/CarConfig/CarConfig.cpp:20-100. */
+/* Car config presets, same order as src/Sim/Car/CarConfig/CarConfig.cpp:20-100
+ * (CAR_CONFIG_OCTANE ... CAR_CONFIG_PSYCLOPS).  RSG_PRESET_CUSTOM marks a car
+ * whose config is none of them (rsg_car_config). */
 enum {
     RSG_PRESET_OCTANE = 0, RSG_PRESET_DOMINUS = 1, RSG_PRESET_PLANK = 2,
     RSG_PRESET_BREAKOUT = 3, RSG_PRESET_HYBRID = 4, RSG_PRESET_MERC = 5,
-    RSG_PRESET_COUNT = 6
+    RSG_PRESET_PSYCLOPS = 6, RSG_PRESET_COUNT = 7, RSG_PRESET_CUSTOM = 255
 };
+
+/* CarConfig (src/Sim/Car/CarConfig/CarConfig.h:6-43), field for field, in uu:
+ * the full hitbox size, its offset from the car's origin, the two wheel pairs
+ * (connection point with y positive: the left wheel mirrors it), three-wheel
+ * behaviour and the flip deadzone. */
+typedef struct rsg_wheel_pair_config {
+    float wheel_radius;
+    float suspension_rest_length;
+    float connection_point_offset[3];
+} rsg_wheel_pair_config;
+
+typedef struct rsg_car_config {
+    float hitbox_size[3];
+    float hitbox_pos_offset[3];
+    rsg_wheel_pair_config front_wheels, back_wheels;
+    uint8_t three_wheels;
+    uint8_t _pad[3];
+    float dodge_deadzone;
+} rsg_car_config;
 
 enum { RSG_TEAM_BLUE = 0, RSG_TEAM_ORANGE = 1 };
 
@@ -211,8 +232,10 @@ typedef struct rsg_arena_snapshot {
      * order, which Serialize also follows) */
     uint32_t car_id[RSG_MAX_CARS];
     uint8_t car_team[RSG_MAX_CARS];
-    uint8_t car_preset[RSG_MAX_CARS];  /* RSG_PRESET_* */
+    uint8_t car_preset[RSG_MAX_CARS];  /* RSG_PRESET_*, RSG_PRESET_CUSTOM for other configs */
     uint8_t _pad[6];
+    /* each car's CarConfig; an all-zero entry stands for car_preset's */
+    rsg_car_config car_config[RSG_MAX_CARS];
     rsg_controls controls[RSG_MAX_CARS];   /* Car::controls */
     rsg_car_state cars[RSG_MAX_CARS];
     rsg_ball_state ball;

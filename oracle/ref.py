@@ -9,7 +9,7 @@ the timed CPU baseline -- never as the product path.
 import ctypes as C
 import os
 
-from rocketsim_amd.state import BallState, CarState, Controls, MutatorConfig, PadState
+from rocketsim_amd.state import BallState, CarConfig, CarState, Controls, MutatorConfig, PadState
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_ref", "librsref.so")
@@ -70,6 +70,9 @@ def lib():
         L.rsref_last_car_id.restype = C.c_uint32
         L.rsref_last_car_id.argtypes = [C.c_void_p]
         L.rsref_get_mutators.argtypes = [C.c_void_p, C.POINTER(MutatorConfig)]
+        L.rsref_add_car_config.restype = C.c_uint32
+        L.rsref_add_car_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(CarConfig)]
+        L.rsref_car_config_preset.argtypes = [C.c_int, C.POINTER(CarConfig)]
         L.rsref_set_mutators.argtypes = [C.c_void_p, C.POINTER(MutatorConfig)]
         L.rsref_car_team.argtypes = [C.c_void_p, C.c_int]
         L.rsref_get_controls.argtypes = [C.c_void_p, C.c_int, C.POINTER(Controls)]
@@ -88,6 +91,14 @@ def lib():
 
 
 _initialised = False
+
+
+def car_config_preset(preset):
+    """The reference's CAR_CONFIG_* preset as an rsg_car_config."""
+    c = CarConfig()
+    if lib().rsref_car_config_preset(preset, C.byref(c)) != 0:
+        raise ValueError(preset)
+    return c
 
 
 def init(meshes=None):
@@ -184,6 +195,12 @@ class RefArena:
         if not self.h:
             raise RuntimeError(self.L.rsref_last_error().decode())
         return self
+
+    def add_car_config(self, team, cfg):
+        cid = self.L.rsref_add_car_config(self.h, team, C.byref(cfg))
+        if cid == 0:
+            raise RuntimeError("rsref_add_car_config failed")
+        return cid
 
     def get_mutators(self):
         m = MutatorConfig()

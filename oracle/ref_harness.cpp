@@ -54,6 +54,7 @@ const CarConfig* preset_config(int p) {
         case RSG_PRESET_BREAKOUT: return &CAR_CONFIG_BREAKOUT;
         case RSG_PRESET_HYBRID: return &CAR_CONFIG_HYBRID;
         case RSG_PRESET_MERC: return &CAR_CONFIG_MERC;
+        case RSG_PRESET_PSYCLOPS: return &CAR_CONFIG_PSYCLOPS;
         default: return nullptr;
     }
 }
@@ -177,6 +178,24 @@ uint32_t rsref_add_car(void* h, int team, int preset) {
     ra->bp_cell.push_back(proxy_cell(ra, (int)ra->cars.size()));
     ra->bp_seq.push_back(++ra->bp_counter);
     return c->id;
+}
+
+// AddCar with any CarConfig.
+uint32_t rsref_add_car_config(void* h, int team, const rsg_car_config* cfg) {
+    auto* ra = (RefArena*)h;
+    Car* c = ra->arena->AddCar((Team)team, rsg_amd::FromRsg(*cfg));
+    ra->cars.push_back(c);
+    ra->bp_cell.push_back(proxy_cell(ra, (int)ra->cars.size()));
+    ra->bp_seq.push_back(++ra->bp_counter);
+    return c->id;
+}
+
+// The reference's preset CarConfig, through the type adapter.
+int rsref_car_config_preset(int preset, rsg_car_config* out) {
+    const CarConfig* cfg = preset_config(preset);
+    if (!cfg) return -1;
+    *out = rsg_amd::ToRsg(*cfg);
+    return 0;
 }
 
 int rsref_num_cars(void* h) { return (int)((RefArena*)h)->cars.size(); }

@@ -9,7 +9,7 @@ import ctypes as C
 
 from . import native
 from .state import (ArenaSnapshot, BallState, BatchDesc, CarState, Controls, Event, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID,
-                    MutatorConfig, NUM_PADS, PadState)
+                    CarConfig, MutatorConfig, NUM_PADS, PadState)
 
 
 _mesh_loaded = False
@@ -47,14 +47,22 @@ def deserialize_snapshot(data):
 
 class ArenaBatch:
     def __init__(self, cars_per_arena, teams, presets, mode=MODE_SOCCAR, tick_rate=120.0,
-                 seed=0, auto_reset=False, synth_controls=False, mutators=None):
+                 seed=0, auto_reset=False, synth_controls=False, mutators=None, car_configs=None):
+        """`presets`: RSG_PRESET_* per car, or None with `car_configs` (one
+        CarConfig per car); `mutators`: every arena's MutatorConfig, set
+        before its cars are added."""
         L = native.lib()
         if mode in (MODE_SOCCAR, MODE_HEATSEEKER) and not _mesh_loaded:
             init_mesh()
         n = len(cars_per_arena)
         total = sum(cars_per_arena)
+        if car_configs is not None:
+            presets = [0] * total
+            if len(car_configs) != total:
+                raise ValueError("car_configs must have one entry per car")
         if len(teams) != total or len(presets) != total:
             raise ValueError("teams/presets must have one entry per car")
+        self._cfgs = (CarConfig * max(total, 1))(*car_configs) if car_configs is not None else None
         self._cpa = (C.c_int * n)(*cars_per_arena)
         self._team = (C.c_uint8 * max(total, 1))(*teams)
         self._preset = (C.c_uint8 * max(total, 1))(*presets)
@@ -62,7 +70,8 @@ class ArenaBatch:
         d = BatchDesc(n_arenas=n, mode=mode, tick_rate=tick_rate, seed=seed,
                       auto_reset=int(auto_reset), synth_controls=int(synth_controls),
                       cars_per_arena=self._cpa, car_team=self._team, car_preset=self._preset,
-                      mutators=C.pointer(mutators) if mutators is not None else None)
+                      mutators=C.pointer(mutators) if mutators is not None else None,
+                      car_configs=self._cfgs)
         h = L.rsg_batch_create(C.byref(d))
         if not h:
             raise RuntimeError(L.rsg_last_error().decode())

@@ -5,7 +5,9 @@
 // src/Sim/Ball/Ball.cpp:27-51).  Host code only.
 #pragma once
 
+#include <cmath>
 #include <cstring>
+#include <string>
 
 #include "../../include/rsg_state.h"
 #include "rsg_tick.h"
@@ -175,6 +177,46 @@ inline void init_ball_body_words(const CarRef& br, float radius_uu, float mass, 
     set_body_gravity(br, BW_GRAVITY_ACC, BW_GRAVITY, gravity_uu, make_ball_phys(radius_uu, mass).inv_mass);
 }
 
+// A car's CarConfig: its entry in the batch's table (the presets without one).
+inline rsg_car_config config_of(const BatchDev& h, int index) {
+    return h.cfgs ? h.cfgs[index] : preset_config(index);
+}
+
+// The preset a config equals, or RSG_PRESET_CUSTOM.
+inline int preset_of(const rsg_car_config& c) {
+    for (int p = 0; p < RSG_PRESET_COUNT; p++) {
+        rsg_car_config q = preset_config(p);
+        if (std::memcmp(&q, &c, sizeof(c)) == 0) return p;
+    }
+    return RSG_PRESET_CUSTOM;
+}
+
+// Why a CarConfig cannot be simulated ("" if it can): non-finite fields,
+// a hitbox with a non-positive size, a negative wheel radius or rest length,
+// or a hitbox whose AABB diagonal exceeds the broadphase cell
+// (btRSBroadphase::createProxy refuses it, btRSBroadphase.cpp:227-229).
+inline std::string car_config_problem(const rsg_car_config& c) {
+    const float f[] = {c.hitbox_size[0], c.hitbox_size[1], c.hitbox_size[2], c.hitbox_pos_offset[0],
+                       c.hitbox_pos_offset[1], c.hitbox_pos_offset[2], c.front_wheels.wheel_radius,
+                       c.front_wheels.suspension_rest_length, c.front_wheels.connection_point_offset[0],
+                       c.front_wheels.connection_point_offset[1], c.front_wheels.connection_point_offset[2],
+                       c.back_wheels.wheel_radius, c.back_wheels.suspension_rest_length,
+                       c.back_wheels.connection_point_offset[0], c.back_wheels.connection_point_offset[1],
+                       c.back_wheels.connection_point_offset[2], c.dodge_deadzone};
+    for (float x : f)
+        if (!std::isfinite(x)) return "non-finite car config field";
+    if (!(c.hitbox_size[0] > 0.f) || !(c.hitbox_size[1] > 0.f) || !(c.hitbox_size[2] > 0.f))
+        return "car hitbox sizes must be positive";
+    if (c.front_wheels.wheel_radius < 0.f || c.back_wheels.wheel_radius < 0.f ||
+        c.front_wheels.suspension_rest_length < 0.f || c.back_wheels.suspension_rest_length < 0.f)
+        return "negative wheel radius or suspension rest length";
+    CarPhys cp = make_car_phys(c);
+    V3 d = (cp.local_aabb_center + cp.local_aabb_half) - (cp.local_aabb_center - cp.local_aabb_half);
+    float cell = bp_cell_size();
+    if (dot(d, d) > cell * cell) return "car hitbox AABB exceeds the broadphase cell size";
+    return "";
+}
+
 // btRSBroadphase::createProxy refuses a dynamic proxy whose AABB diagonal
 // exceeds the cell size (btRSBroadphase.cpp:227-229); a ball of radius
 // `radius_uu` made at (0, 0, radius) passes if this holds.
@@ -242,6 +284,7 @@ inline void apply_mutator_config(const BatchDev& h, int a, const rsg_mutator_con
 // A new arena's records (Arena::Create + AddCar, src/Sim/Arena/Arena.cpp:50-62,
 // 416-540): cars with their config and spawn boost, the ball at rest, pads
 // active; the proxies registered where the reference creates them.
+// `preset` holds each car's index in the batch's config table.
 inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_t* team, const uint8_t* preset) {
     const int mode = h.mode;
     const rsg_mutator_config mut = arena_mutators(h, a);
@@ -261,7 +304,7 @@ inline void init_arena_record(const BatchDev& h, int a, int n_cars, const uint8_
         init_car_body_words(cr, mut.gravity);
         // proxy created at the identity transform (Car::_BulletSetup,
         // src/Sim/Car/Car.cpp:212-233), AABB without contact threshold
-        CarPhys cp = make_car_phys(preset[i]);
+        CarPhys cp = make_car_phys(config_of(h, preset[i]));
         cr.u(CW_BP_CELL) = (uint32_t)bp_cell_of(cp.local_aabb_center - cp.local_aabb_half);
         cr.u(CW_BP_SEQ) = (uint32_t)(1 + i);
     }

@@ -116,10 +116,12 @@ std::string mutator_config_problem(const rsg_mutator_config& m) {
 struct rsg_batch {
     BatchDev dev;
     std::vector<int> cars_per_arena, car_offset;
-    std::vector<uint8_t> team, preset;
+    std::vector<uint8_t> team, preset;  // preset: each car's index in the config table
     int total_cars = 0;
     std::vector<uint32_t> h_car, h_ball, h_pad, h_arena;
     std::vector<rsg_mutator_config> h_mut;  // per-arena MutatorConfig (the kernels only read it)
+    std::vector<rsg_car_config> h_cfgs;     // car config table: the presets, then other configs
+    rsg_car_config* d_cfgs = nullptr;
     bool host_dirty = false;  // host mirror newer than device
     bool dev_newer = false;   // device newer than host mirror
     hipStream_t last_stream = nullptr;
@@ -148,6 +150,7 @@ BatchDev host_view(rsg_batch* b) {
     h.pad = b->h_pad.data();
     h.arena = b->h_arena.data();
     h.mut = b->h_mut.data();
+    h.cfgs = b->h_cfgs.data();
     return h;
 }
 
@@ -231,13 +234,42 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     b->total_cars = total;
     b->dev.max_cars = maxc > 0 ? maxc : 1;
     b->team.assign(desc->car_team, desc->car_team + total);
-    b->preset.assign(desc->car_preset, desc->car_preset + total);
+    // the config table: the presets first, then each other config once
+    for (int p = 0; p < RSG_PRESET_COUNT; p++) b->h_cfgs.push_back(preset_config(p));
+    b->preset.resize((size_t)total);
     for (int i = 0; i < total; i++) {
-        if (b->team[i] > 1 || b->preset[i] >= RSG_PRESET_COUNT) {
+        if (b->team[i] > 1) {
             delete b;
-            fail("rsg_batch_create: bad team or preset");
+            fail("rsg_batch_create: bad team");
             return nullptr;
         }
+        if (!desc->car_configs) {
+            if (!desc->car_preset || desc->car_preset[i] >= RSG_PRESET_COUNT) {
+                delete b;
+                fail("rsg_batch_create: bad preset");
+                return nullptr;
+            }
+            b->preset[(size_t)i] = desc->car_preset[i];
+            continue;
+        }
+        const rsg_car_config& c = desc->car_configs[i];
+        std::string why = car_config_problem(c);
+        if (!why.empty()) {
+            delete b;
+            fail("rsg_batch_create: car " + std::to_string(i) + ": " + why);
+            return nullptr;
+        }
+        size_t k = 0;
+        while (k < b->h_cfgs.size() && std::memcmp(&b->h_cfgs[k], &c, sizeof(c)) != 0) k++;
+        if (k == b->h_cfgs.size()) {
+            if (k >= 255) {
+                delete b;
+                fail("rsg_batch_create: more than 255 distinct car configs");
+                return nullptr;
+            }
+            b->h_cfgs.push_back(c);
+        }
+        b->preset[(size_t)i] = (uint8_t)k;
     }
     b->h_car.assign(car_words(b), 0);
     b->h_ball.assign(ball_words(b), 0);
@@ -251,13 +283,17 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
         (e = hipMalloc(&b->dev.pad, pad_words(b) * 4)) != hipSuccess ||
         (e = hipMalloc(&b->dev.arena, arena_words(b) * 4)) != hipSuccess ||
         (e = hipMalloc(&b->dev.space, (size_t)b->dev.n * sizeof(ArenaSpace))) != hipSuccess ||
-        (e = hipMalloc(&dmut, (size_t)b->dev.n * sizeof(rsg_mutator_config))) != hipSuccess) {
+        (e = hipMalloc(&dmut, (size_t)b->dev.n * sizeof(rsg_mutator_config))) != hipSuccess ||
+        (e = hipMalloc(&b->d_cfgs, b->h_cfgs.size() * sizeof(rsg_car_config))) != hipSuccess ||
+        (e = hipMemcpy(b->d_cfgs, b->h_cfgs.data(), b->h_cfgs.size() * sizeof(rsg_car_config),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
         b->dev.mut = dmut;
         fail(std::string("rsg_batch_create: hipMalloc: ") + hipGetErrorString(e));
         rsg_batch_destroy(b);
         return nullptr;
     }
     b->dev.mut = dmut;
+    b->dev.cfgs = b->d_cfgs;
     // Host-side initial state: cars with config, ball at rest, pads active.
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++)
@@ -278,6 +314,7 @@ void rsg_batch_destroy(rsg_batch* b) {
     if (b->dev.arena) (void)hipFree(b->dev.arena);
     if (b->dev.space) (void)hipFree(b->dev.space);
     if (b->dev.mut) (void)hipFree(const_cast<rsg_mutator_config*>(b->dev.mut));
+    if (b->d_cfgs) (void)hipFree(b->d_cfgs);
     if (b->status_partials) (void)hipFree(b->status_partials);
     if (b->dev.ev_buf) (void)hipFree(b->dev.ev_buf);
     if (b->dev.ev_count) (void)hipFree(b->dev.ev_count);
@@ -922,6 +959,13 @@ int rsg_batch_get_goals(rsg_batch* b, uint32_t* blue, uint32_t* orange, uint64_t
     return 0;
 }
 
+int rsg_car_config_preset(int preset, rsg_car_config* out) {
+    if (!out) return fail("rsg_car_config_preset: null argument");
+    if (preset < 0 || preset >= RSG_PRESET_COUNT) return fail("rsg_car_config_preset: unknown preset");
+    *out = preset_config(preset);
+    return 0;
+}
+
 int rsg_mutator_config_default(int mode, rsg_mutator_config* out) {
     if (!out) return fail("rsg_mutator_config_default: null argument");
     if (mode != RSG_MODE_SOCCAR && mode != RSG_MODE_HEATSEEKER && mode != RSG_MODE_THE_VOID)
@@ -1135,7 +1179,8 @@ int rsg_batch_get_snapshot(rsg_batch* b, int arena, rsg_arena_snapshot* out) {
         CarRef cr = car_ref(h, arena, slot);
         uint32_t cfg = cr.u(CW_CONFIG);
         out->car_id[k] = cr.u(CW_ID);
-        out->car_preset[k] = (uint8_t)(cfg & 0xFF);
+        out->car_config[k] = b->h_cfgs[cfg & 0xFF];
+        out->car_preset[k] = (uint8_t)preset_of(out->car_config[k]);
         out->car_team[k] = (uint8_t)((cfg >> 8) & 0xFF);
         out->controls[k] = ctrl_to_api(load_ctrl(cr, CW_CTRL));
         car_to_api(cr, out->cars[k]);
@@ -1185,9 +1230,10 @@ int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s)
         int k = by_id[i];
         if (i > 0 && s->car_id[k] == s->car_id[by_id[i - 1]]) return fail("rsg_batch_set_snapshot: repeated car id");
         int g = b->car_offset[arena] + i;
-        if (s->car_team[k] != b->team[g] || s->car_preset[k] != b->preset[g])
+        const rsg_car_config want = snapshot_car_config(*s, k);
+        if (s->car_team[k] != b->team[g] || std::memcmp(&want, &b->h_cfgs[b->preset[g]], sizeof(want)) != 0)
             return fail("rsg_batch_set_snapshot: car " + std::to_string(s->car_id[k]) +
-                        " has another team or preset than slot " + std::to_string(i));
+                        " has another team or car config than slot " + std::to_string(i));
         slot_of[k] = i;
     }
     const bool raw = s->raw_words > 0;
@@ -1199,8 +1245,7 @@ int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s)
         for (int k = 0; k < n; k++) {
             CarRef rc = car_ref(v, 0, k);
             uint32_t cfg = rc.u(CW_CONFIG);
-            if (rc.u(CW_ID) != s->car_id[k] || (cfg & 0xFF) != s->car_preset[k] ||
-                ((cfg >> 8) & 0xFF) != s->car_team[k])
+            if (rc.u(CW_ID) != s->car_id[k] || ((cfg >> 8) & 0xFF) != s->car_team[k])
                 return fail("rsg_batch_set_snapshot: raw records disagree with the snapshot's cars");
         }
     if (download(b) != 0) return -1;
@@ -1213,6 +1258,9 @@ int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s)
         if (raw) {
             CarRef rc = car_ref(v, 0, k);
             for (int w = 0; w < CAR_WORDS; w++) cr.u(w) = rc.u(w);
+            // the config index is this batch's (the snapshot's configs matched)
+            const int g = b->car_offset[arena] + slot_of[k];
+            cr.u(CW_CONFIG) = (uint32_t)b->preset[g] | ((uint32_t)b->team[g] << 8);
         } else {
             car_from_api(cr, s->cars[k]);
             cr.u(CW_ID) = s->car_id[k];

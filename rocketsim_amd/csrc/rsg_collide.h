@@ -209,7 +209,7 @@ struct World {
 
     RSG_FN float threshold_of(int code) {
         if (code == 0) return make_ball_phys(A.ball.radius_uu, A.ball.mass).angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
-        if (code > 0) return make_car_phys(A.cars[code - 1].preset).angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
+        if (code > 0) return A.cars[code - 1].phys.angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
         return 1e30f;  // planes and the arena mesh have huge bounding spheres
     }
 
@@ -229,7 +229,7 @@ struct World {
             mn = pos - v3(r, r, r);
             mx = pos + v3(r, r, r);
         } else {
-            CarPhys cp = make_car_phys(A.cars[code - 1].preset);
+            const CarPhys& cp = A.cars[code - 1].phys;
             M3 ab = abs_m3(rot);
             V3 c = rot * cp.local_aabb_center + pos;
             V3 e = v3(dot(ab.r[0], cp.local_aabb_half), dot(ab.r[1], cp.local_aabb_half), dot(ab.r[2], cp.local_aabb_half));
@@ -241,7 +241,7 @@ struct World {
     // Child box AABB of a car (btBoxShape::getAabb at hull transform, no threshold).
     RSG_FN void car_child_aabb(int code, V3& mn, V3& mx) {
         Car& c = A.cars[code - 1];
-        CarPhys cp = make_car_phys(c.preset);
+        const CarPhys& cp = c.phys;
         Xf x = xf_mul(xf_from(c.b.pos, c.b.rot), xf_from(cp.box_offset, m3_identity()));
         V3 he = cp.box_implicit + v3(cp.box_margin, cp.box_margin, cp.box_margin);
         M3 ab = abs_m3(x.R);
@@ -576,7 +576,7 @@ struct World {
         PlaneDef pd = plane_def(s, mesh->plane_unit);
         Xf px = xf_from(pd.origin, m3_identity());
         Car& c = A.cars[code - 1];
-        CarPhys cp = make_car_phys(c.preset);
+        const CarPhys& cp = c.phys;
         Xf cx = xf_mul(xf_from(c.b.pos, c.b.rot), xf_from(cp.box_offset, m3_identity()));
         Xf plane_in_convex = xf_mul(xf_inverse(cx), px);
         Xf convex_in_plane = xf_mul(xf_inverse(px), cx);
@@ -678,11 +678,11 @@ struct World {
 
     RSG_FN Xf car_child_xf(int code) {
         Car& c = A.cars[code - 1];
-        CarPhys cp = make_car_phys(c.preset);
+        const CarPhys& cp = c.phys;
         return xf_mul(xf_from(c.b.pos, c.b.rot), xf_from(cp.box_offset, m3_identity()));
     }
     RSG_FN CShape car_box_shape(int code) {
-        CarPhys cp = make_car_phys(A.cars[code - 1].preset);
+        const CarPhys& cp = A.cars[code - 1].phys;
         return shape_box(cp.box_implicit, cp.box_margin);
     }
 
@@ -690,7 +690,7 @@ struct World {
     // per-triangle early out, then btConvexConvexAlgorithm (GJK) per triangle.
     RSG_FN void car_vs_mesh(int mi, int code) {
         Xf cx = car_child_xf(code);
-        CarPhys cp = make_car_phys(A.cars[code - 1].preset);
+        const CarPhys& cp = A.cars[code - 1].phys;
         V3 he = cp.box_implicit + v3(cp.box_margin, cp.box_margin, cp.box_margin);
         M3 ab = abs_m3(cx.R);
         V3 e = v3(dot(ab.r[0], he), dot(ab.r[1], he), dot(ab.r[2], he));
@@ -752,7 +752,8 @@ struct World {
     // box0 = the car of manifold body0.
     RSG_FN void car_vs_car(int mi, int c0, int c1) {
         Xf x0 = car_child_xf(c0), x1 = car_child_xf(c1);
-        CarPhys p0 = make_car_phys(A.cars[c0 - 1].preset), p1 = make_car_phys(A.cars[c1 - 1].preset);
+        const CarPhys& p0 = A.cars[c0 - 1].phys;
+        const CarPhys& p1 = A.cars[c1 - 1].phys;
         V3 h0 = p0.box_implicit + v3(p0.box_margin, p0.box_margin, p0.box_margin);
         V3 h1 = p1.box_implicit + v3(p1.box_margin, p1.box_margin, p1.box_margin);
         BoxBoxContacts bc;

@@ -159,6 +159,9 @@ RSG_FN float curve_eval(const Curve& c, float input, float default_out = 1.f) {
 
 // Curves are tiny; keep them as inline constexpr data usable on host and device.
 RSG_FN Curve curve_steer_angle() { return Curve{6, {0, 500, 1000, 1500, 1750, 3000}, {0.53356f, 0.31930f, 0.18203f, 0.10570f, 0.08507f, 0.03454f}}; }
+// Three-wheel cars (CarConfig::threeWheels, src/RLConst.h:429-434, 470-475)
+RSG_FN Curve curve_steer_angle_three_wheels() { return Curve{2, {0, 2300}, {0.342473f, 0.034837f}}; }
+RSG_FN Curve curve_lat_friction_three_wheels() { return Curve{2, {0, 1}, {0.30f, 0.25f}}; }
 RSG_FN Curve curve_powerslide_steer() { return Curve{2, {0, 2500}, {0.39235f, 0.12610f}}; }
 RSG_FN Curve curve_drive_torque() { return Curve{3, {0, 1400, 1410}, {1.0f, 0.1f, 0.0f}}; }
 RSG_FN Curve curve_non_sticky() { return Curve{3, {0, 0.7075f, 1}, {0.1f, 0.5f, 1.0f}}; }
@@ -171,23 +174,56 @@ RSG_FN Curve curve_bump_ground() { return Curve{3, {0.f, 1400.f, 2200.f}, {(5.f 
 RSG_FN Curve curve_bump_air() { return Curve{3, {0.f, 1400.f, 2200.f}, {(5.f / 6.f), 1390.f, 1945.f}}; }
 RSG_FN Curve curve_bump_upward() { return Curve{3, {0.f, 1400.f, 2200.f}, {(2.f / 6.f), 278.f, 417.f}}; }
 
-// Car presets (CarConfig.cpp:20-100): hitbox size, hitbox offset, front/back
-// wheel radius, suspension rest length, connection point.
-struct PresetRaw {
-    float hitbox[3], offset[3];
-    float front_rad, back_rad, front_rest, back_rest;
-    float front_conn[3], back_conn[3];
-};
+// Car config presets (src/Sim/Car/CarConfig/CarConfig.cpp:20-100, the
+// reference's tables in its order; the Psyclops hitbox adds 0.134 to the
+// Octane's in float as the reference does).
+RSG_FN rsg_wheel_pair_config wheel_pair(float radius, float rest, float x, float y, float z) {
+    rsg_wheel_pair_config w;
+    w.wheel_radius = radius;
+    w.suspension_rest_length = rest;
+    w.connection_point_offset[0] = x;
+    w.connection_point_offset[1] = y;
+    w.connection_point_offset[2] = z;
+    return w;
+}
 
-RSG_FN PresetRaw preset_raw(int p) {
+RSG_FN rsg_car_config car_config(float sx, float sy, float sz, float ox, float oy, float oz,
+                                 rsg_wheel_pair_config front, rsg_wheel_pair_config back, bool three_wheels) {
+    rsg_car_config c;
+    c.hitbox_size[0] = sx; c.hitbox_size[1] = sy; c.hitbox_size[2] = sz;
+    c.hitbox_pos_offset[0] = ox; c.hitbox_pos_offset[1] = oy; c.hitbox_pos_offset[2] = oz;
+    c.front_wheels = front;
+    c.back_wheels = back;
+    c.three_wheels = three_wheels ? 1 : 0;
+    c._pad[0] = c._pad[1] = c._pad[2] = 0;
+    c.dodge_deadzone = 0.5f;  // CarConfig.h:37
+    return c;
+}
+
+RSG_FN rsg_car_config preset_config(int p) {
     switch (p) {
         default:
-        case 0: return PresetRaw{{120.507f, 86.6994f, 38.6591f}, {13.8757f, 0, 20.755f}, 12.50f, 15.00f, 38.755f, 37.055f, {51.25f, 25.90f, 20.755f}, {-33.75f, 29.50f, 20.755f}};
-        case 1: return PresetRaw{{130.427f, 85.7799f, 33.8f}, {9.f, 0, 15.75f}, 12.00f, 13.50f, 33.95f, 33.85f, {50.30f, 31.10f, 15.75f}, {-34.75f, 33.00f, 15.75f}};
-        case 2: return PresetRaw{{131.32f, 87.1704f, 31.8944f}, {9.00857f, 0, 12.0942f}, 12.50f, 17.00f, 31.9242f, 27.9242f, {49.97f, 27.80f, 12.0942f}, {-35.43f, 20.28f, 12.0942f}};
-        case 3: return PresetRaw{{133.992f, 83.021f, 32.8f}, {12.5f, 0, 11.75f}, 13.50f, 15.00f, 29.7f, 29.666f, {51.50f, 26.67f, 11.75f}, {-35.75f, 35.00f, 11.75f}};
-        case 4: return PresetRaw{{129.519f, 84.6879f, 36.6591f}, {13.8757f, 0, 20.755f}, 12.50f, 15.00f, 38.755f, 37.055f, {51.25f, 25.90f, 20.755f}, {-34.00f, 29.50f, 20.755f}};
-        case 5: return PresetRaw{{123.22f, 79.2103f, 44.1591f}, {11.3757f, 0, 21.505f}, 15.00f, 15.00f, 39.505f, 39.105f, {51.25f, 25.90f, 21.505f}, {-33.75f, 29.50f, 21.505f}};
+        case 0: return car_config(120.507f, 86.6994f, 38.6591f, 13.8757f, 0, 20.755f,
+                                  wheel_pair(12.50f, 38.755f, 51.25f, 25.90f, 20.755f),
+                                  wheel_pair(15.00f, 37.055f, -33.75f, 29.50f, 20.755f), false);
+        case 1: return car_config(130.427f, 85.7799f, 33.8f, 9.f, 0, 15.75f,
+                                  wheel_pair(12.00f, 33.95f, 50.30f, 31.10f, 15.75f),
+                                  wheel_pair(13.50f, 33.85f, -34.75f, 33.00f, 15.75f), false);
+        case 2: return car_config(131.32f, 87.1704f, 31.8944f, 9.00857f, 0, 12.0942f,
+                                  wheel_pair(12.50f, 31.9242f, 49.97f, 27.80f, 12.0942f),
+                                  wheel_pair(17.00f, 27.9242f, -35.43f, 20.28f, 12.0942f), false);
+        case 3: return car_config(133.992f, 83.021f, 32.8f, 12.5f, 0, 11.75f,
+                                  wheel_pair(13.50f, 29.7f, 51.50f, 26.67f, 11.75f),
+                                  wheel_pair(15.00f, 29.666f, -35.75f, 35.00f, 11.75f), false);
+        case 4: return car_config(129.519f, 84.6879f, 36.6591f, 13.8757f, 0, 20.755f,
+                                  wheel_pair(12.50f, 38.755f, 51.25f, 25.90f, 20.755f),
+                                  wheel_pair(15.00f, 37.055f, -34.00f, 29.50f, 20.755f), false);
+        case 5: return car_config(123.22f, 79.2103f, 44.1591f, 11.3757f, 0, 21.505f,
+                                  wheel_pair(15.00f, 39.505f, 51.25f, 25.90f, 21.505f),
+                                  wheel_pair(15.00f, 39.105f, -33.75f, 29.50f, 21.505f), false);
+        case 6: return car_config(120.507f + 0.134f, 86.6994f + 0.134f, 38.6591f + 0.134f, 13.8757f, 0, 15.0f,
+                                  wheel_pair(12.50f, 33.0f, 51.25f, 5.000f, 15.000f),
+                                  wheel_pair(15.00f, 31.3f, -33.75f, 29.50f, 15.000f), true);
     }
 }
 
@@ -210,6 +246,8 @@ struct CarPhys {
     float wheel_force_scale[4];
     float max_susp_travel; // m
     float angular_motion_disc;
+    bool three_wheels;     // CarConfig::threeWheels
+    float dodge_deadzone;  // CarConfig::dodgeDeadzone
 };
 
 // btBoxShape::calculateLocalInertia (BulletCollision/CollisionShapes/
@@ -224,10 +262,11 @@ RSG_FN V3 box_inv_inertia(V3 he, float mass) {
               inertia.z != 0.f ? 1.f / inertia.z : 0.f);
 }
 
-RSG_FN CarPhys make_car_phys(int preset) {
-    PresetRaw r = preset_raw(preset);
+RSG_FN CarPhys make_car_phys(const rsg_car_config& r) {
     CarPhys c;
-    V3 size = v3(r.hitbox[0], r.hitbox[1], r.hitbox[2]);
+    c.three_wheels = r.three_wheels != 0;
+    c.dodge_deadzone = r.dodge_deadzone;
+    V3 size = v3(r.hitbox_size[0], r.hitbox_size[1], r.hitbox_size[2]);
     V3 half = (size * UU_TO_BT) / 2.f;
     c.box_implicit = half - v3(CONVEX_DISTANCE_MARGIN, CONVEX_DISTANCE_MARGIN, CONVEX_DISTANCE_MARGIN);
     float mind = half.x;
@@ -237,7 +276,7 @@ RSG_FN CarPhys make_car_phys(int preset) {
     mind = get(half, ax);
     float safe = 0.1f * mind;
     c.box_margin = safe < CONVEX_DISTANCE_MARGIN ? safe : CONVEX_DISTANCE_MARGIN;
-    c.box_offset = v3(r.offset[0], r.offset[1], r.offset[2]) * UU_TO_BT;
+    c.box_offset = v3(r.hitbox_pos_offset[0], r.hitbox_pos_offset[1], r.hitbox_pos_offset[2]) * UU_TO_BT;
     V3 he = c.box_implicit + v3(c.box_margin, c.box_margin, c.box_margin);
     V3 lmin = c.box_offset - he, lmax = c.box_offset + he;
     c.local_aabb_half = 0.5f * (lmax - lmin);
@@ -247,14 +286,14 @@ RSG_FN CarPhys make_car_phys(int preset) {
     for (int i = 0; i < 4; i++) {
         bool front = i < 2;
         bool left = i % 2;
-        V3 conn = front ? v3(r.front_conn[0], r.front_conn[1], r.front_conn[2])
-                        : v3(r.back_conn[0], r.back_conn[1], r.back_conn[2]);
+        const rsg_wheel_pair_config& wp = front ? r.front_wheels : r.back_wheels;
+        V3 conn = v3(wp.connection_point_offset[0], wp.connection_point_offset[1], wp.connection_point_offset[2]);
         if (left) conn.y *= -1;
         c.wheel_conn[i] = conn * UU_TO_BT;
-        float rest = front ? r.front_rest : r.back_rest;
+        float rest = wp.suspension_rest_length;
         rest -= MAX_SUSPENSION_TRAVEL;
         c.wheel_rest[i] = rest * UU_TO_BT;
-        c.wheel_radius[i] = (front ? r.front_rad : r.back_rad) * UU_TO_BT;
+        c.wheel_radius[i] = wp.wheel_radius * UU_TO_BT;
         c.wheel_force_scale[i] = front ? SUSPENSION_FORCE_SCALE_FRONT : SUSPENSION_FORCE_SCALE_BACK;
     }
     c.max_susp_travel = ((MAX_SUSPENSION_TRAVEL * UU_TO_BT) * 100) / 100;
@@ -272,6 +311,8 @@ RSG_FN CarPhys make_car_phys(int preset) {
     }
     return c;
 }
+
+RSG_FN CarPhys make_car_phys(int preset) { return make_car_phys(preset_config(preset)); }
 
 // Ball (src/Sim/Ball/Ball.cpp:53-115, btSphereShape.cpp:37-59)
 struct BallPhys {

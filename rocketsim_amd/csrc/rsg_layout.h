@@ -57,7 +57,7 @@ enum CarWord : int {
     CW_CTRL = 75,          // 5 floats + 1 bits word
     CW_GEN_REDRAW = 81,
     CW_GEN_TICKS_LEFT = 82,
-    CW_CONFIG = 83,        // preset | team << 8
+    CW_CONFIG = 83,        // config table index | team << 8
     CW_ID = 84,
     CW_BP_CELL = 85,       // broadphase cell the proxy is registered in
     CW_BP_SEQ = 86,        // registration order among the arena's dynamic proxies
@@ -129,6 +129,8 @@ struct BatchDev {
     unsigned char* space;
     // per-arena MutatorConfig (n records); null: the mode's defaults
     const rsg_mutator_config* mut;
+    // car config table: CW_CONFIG's low byte indexes it; null: the presets
+    const rsg_car_config* cfgs;
 };
 
 }  // namespace rsg

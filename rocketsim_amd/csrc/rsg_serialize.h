@@ -88,21 +88,29 @@ struct ByteIn {
     size_t left() const { return bad || pos > n ? 0 : n - pos; }
 };
 
-// CarConfig of a preset (CarConfig.h:21-43), in its serialized field order:
-// dodgeDeadzone, hitboxPosOffset, hitboxSize, front connection point, rest
-// length, radius, back connection point, rest length, radius.
-inline void preset_config_words(int preset, float out[1 + 3 + 3 + 5 + 5]) {
-    PresetRaw r = preset_raw(preset);
+// A snapshot car's CarConfig: car_config[k], or car_preset[k]'s when that
+// entry is all zero.
+inline rsg_car_config snapshot_car_config(const rsg_arena_snapshot& s, int k) {
+    static const rsg_car_config zero = {};
+    if (std::memcmp(&s.car_config[k], &zero, sizeof(zero)) != 0) return s.car_config[k];
+    return preset_config(s.car_preset[k] < RSG_PRESET_COUNT ? s.car_preset[k] : 0);
+}
+
+// A CarConfig in its serialized field order (CAR_CONFIG_SERIALIZATION_FIELDS,
+// CarConfig.h:45-49): dodgeDeadzone, hitboxPosOffset, hitboxSize, front
+// connection point, rest length, radius, back connection point, rest length,
+// radius.  threeWheels is not serialized.
+inline void config_words(const rsg_car_config& r, float out[1 + 3 + 3 + 5 + 5]) {
     float* o = out;
-    *o++ = 0.5f;  // dodgeDeadzone (CarConfig.h:37)
-    for (int i = 0; i < 3; i++) *o++ = r.offset[i];
-    for (int i = 0; i < 3; i++) *o++ = r.hitbox[i];
-    for (int i = 0; i < 3; i++) *o++ = r.front_conn[i];
-    *o++ = r.front_rest;
-    *o++ = r.front_rad;
-    for (int i = 0; i < 3; i++) *o++ = r.back_conn[i];
-    *o++ = r.back_rest;
-    *o++ = r.back_rad;
+    *o++ = r.dodge_deadzone;
+    for (int i = 0; i < 3; i++) *o++ = r.hitbox_pos_offset[i];
+    for (int i = 0; i < 3; i++) *o++ = r.hitbox_size[i];
+    const rsg_wheel_pair_config* w[2] = {&r.front_wheels, &r.back_wheels};
+    for (int k = 0; k < 2; k++) {
+        for (int i = 0; i < 3; i++) *o++ = w[k]->connection_point_offset[i];
+        *o++ = w[k]->suspension_rest_length;
+        *o++ = w[k]->wheel_radius;
+    }
 }
 
 // MutatorConfig::Serialize (MutatorConfig.cpp:42-45): the field count, then
@@ -187,6 +195,13 @@ inline void raw_from_records(const rsg_arena_snapshot& s, std::vector<uint32_t>&
         cr.u(CW_ID) = s.car_id[k];
         car_from_api(cr, s.cars[k]);
         store_ctrl(cr, CW_CTRL, ctrl_from_api(s.controls[k]));
+        init_car_body_words(cr, s.mutators.gravity);
+    }
+    {
+        // the mode's default ball body with the config's gravity, as
+        // Arena::DeserializeNew leaves it
+        const rsg_mutator_config def = mutator_defaults(s.mode);
+        init_ball_body_words(ball_ref(v, 0), def.ball_radius, def.ball_mass, s.mutators.gravity);
     }
     ball_from_api(ball_ref(v, 0), s.ball);
     CarRef ar = arena_ref(v, 0);
@@ -204,8 +219,7 @@ inline bool records_from_raw(rsg_arena_snapshot& s) {
     for (int k = 0; k < s.n_cars; k++) {
         CarRef cr = car_ref(v, 0, k);
         uint32_t cfg = cr.u(CW_CONFIG);
-        if (cr.u(CW_ID) != s.car_id[k] || (cfg & 0xFF) != s.car_preset[k] || ((cfg >> 8) & 0xFF) != s.car_team[k])
-            return false;
+        if (cr.u(CW_ID) != s.car_id[k] || ((cfg >> 8) & 0xFF) != s.car_team[k]) return false;
     }
     for (int k = 0; k < s.n_cars; k++) {
         CarRef cr = car_ref(v, 0, k);
@@ -231,7 +245,7 @@ inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<u
 
     o.put<uint32_t>((uint32_t)s.n_cars);
     for (int k = 0; k < s.n_cars; k++) {
-        if (s.car_preset[k] >= RSG_PRESET_COUNT || s.car_team[k] > 1) return "bad car team or preset";
+        if (s.car_team[k] > 1) return "bad car team";
         const rsg_controls& c = s.controls[k];
         const rsg_car_state& cs = s.cars[k];
         o.put<uint8_t>(s.car_team[k]);
@@ -242,7 +256,7 @@ inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<u
         o.put<float>(c.pitch); o.put<float>(c.yaw); o.put<float>(c.roll);
         o.b8(c.boost != 0); o.b8(c.jump != 0); o.b8(c.handbrake != 0);
         float cfg[17];
-        preset_config_words(s.car_preset[k], cfg);
+        config_words(snapshot_car_config(s, k), cfg);
         o.count(9);
         o.put<float>(cfg[0]);
         o.vec(cfg + 1); o.vec(cfg + 4);
@@ -334,7 +348,7 @@ inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<u
 // Car::_BulletSetup and Ball::_BulletSetup create them, in creation order).
 inline void fresh_broadphase(rsg_arena_snapshot& s, const int* creation_rank) {
     for (int k = 0; k < s.n_cars; k++) {
-        CarPhys cp = make_car_phys(s.car_preset[k]);
+        CarPhys cp = make_car_phys(snapshot_car_config(s, k));
         s.cars[k].bp_cell = bp_cell_of(cp.local_aabb_center - cp.local_aabb_half);
         s.cars[k].bp_seq = (uint32_t)(1 + creation_rank[k]);
     }
@@ -389,14 +403,25 @@ inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_ar
         in.vec(cfg + 1); in.vec(cfg + 4);
         in.vec(cfg + 7); cfg[10] = in.get<float>(); cfg[11] = in.get<float>();
         in.vec(cfg + 12); cfg[15] = in.get<float>(); cfg[16] = in.get<float>();
-        int preset = -1;
-        for (int p = 0; p < RSG_PRESET_COUNT && preset < 0; p++) {
-            float want[17];
-            preset_config_words(p, want);
-            if (std::memcmp(want, cfg, sizeof(cfg)) == 0) preset = p;
+        {
+            // Car::_Deserialize reads the serialized fields; threeWheels
+            // keeps CarConfig's default (false)
+            rsg_car_config& rc = s.car_config[k];
+            std::memset(&rc, 0, sizeof(rc));
+            const float* w = cfg;
+            rc.dodge_deadzone = *w++;
+            for (int i = 0; i < 3; i++) rc.hitbox_pos_offset[i] = *w++;
+            for (int i = 0; i < 3; i++) rc.hitbox_size[i] = *w++;
+            rsg_wheel_pair_config* wp[2] = {&rc.front_wheels, &rc.back_wheels};
+            for (int q = 0; q < 2; q++) {
+                for (int i = 0; i < 3; i++) wp[q]->connection_point_offset[i] = *w++;
+                wp[q]->suspension_rest_length = *w++;
+                wp[q]->wheel_radius = *w++;
+            }
+            std::string why = car_config_problem(rc);
+            if (!why.empty()) return why;
+            s.car_preset[k] = (uint8_t)preset_of(rc);
         }
-        if (preset < 0) return "car config matches no preset (custom configs are not supported)";
-        s.car_preset[k] = (uint8_t)preset;
         uint32_t f = 0;
         cs.hit_tick = ~0ull;
         cs.hit_extra_tick = ~0ull;

@@ -101,7 +101,8 @@ struct Car {
     Body b;
     M3 api_rot;        // CarState::rotMat: the body basis as of the last post-tick
                        // update of a car that was not demolished
-    int preset, team;
+    int preset, team;  // preset: the car's entry in the batch's config table
+    CarPhys phys;      // its CarConfig's physics values
     uint32_t id;
     uint32_t flags;
     V3 flip_torque;
@@ -303,7 +304,7 @@ RSG_FN void car_update_wheels(Car& c, WheelT* wh, const CarPhys& cp, float dt, i
     }
 
     {
-        float steer_angle = curve_eval(curve_steer_angle(), abs_fwd);
+        float steer_angle = curve_eval(cp.three_wheels ? curve_steer_angle_three_wheels() : curve_steer_angle(), abs_fwd);
         if (c.handbrake != 0.f)
             steer_angle += (curve_eval(curve_powerslide_steer(), abs_fwd) - steer_angle) * c.handbrake;
         steer_angle *= c.ctrl.steer;
@@ -320,7 +321,7 @@ RSG_FN void car_update_wheels(Car& c, WheelT* wh, const CarPhys& cp, float dt, i
             V3 cv = (cross(c.b.ang, delta) + c.b.vel) * BT_TO_UU;
             float base_fr = fabsf(dot(cv, lat_dir));
             if (base_fr > 5) curve_in = base_fr / (fabsf(dot(cv, long_dir)) + base_fr);
-            float lat_f = curve_eval(curve_lat_friction(), curve_in);
+            float lat_f = curve_eval(cp.three_wheels ? curve_lat_friction_three_wheels() : curve_lat_friction(), curve_in);
             float long_f = curve_eval(curve_long_friction(), curve_in);
             if (c.handbrake != 0.f) {
                 float hb = c.handbrake;
@@ -343,7 +344,7 @@ RSG_FN void car_update_wheels(Car& c, WheelT* wh, const CarPhys& cp, float dt, i
     if (wheels_world) {
         V3 up_dir = wheel_upwards_dir(c, wh);
         bool full_stick = (real_throttle != 0) || (abs_fwd > STOPPING_FORWARD_VEL);
-        float sticky_scale = 0.5f;
+        float sticky_scale = cp.three_wheels ? 0.f : 0.5f;
         if (full_stick) sticky_scale += 1 - fabsf(up_dir.z);
         body_apply_central_force(c.b, up_dir * sticky_scale * (GRAVITY_Z * UU_TO_BT) * CAR_MASS_BT);
     }
@@ -480,7 +481,7 @@ RSG_FN void car_update_double_jump_or_flip(Car& c, float dt, bool jump_pressed, 
 
         if (jump_pressed && c.air_time_since_jump < DOUBLEJUMP_MAX_DELAY) {
             float in_mag = fabsf(c.ctrl.yaw) + fabsf(c.ctrl.pitch) + fabsf(c.ctrl.roll);
-            bool is_flip = in_mag >= 0.5f;  // CarConfig::dodgeDeadzone
+            bool is_flip = in_mag >= c.phys.dodge_deadzone;
             bool can_use = (!flag(c.flags, RSG_CF_HAS_DOUBLE_JUMPED) && !flag(c.flags, RSG_CF_HAS_FLIPPED)) ||
                            (is_flip ? M.unlimited_flips : M.unlimited_double_jumps);
             if (flag(c.flags, RSG_CF_AUTO_FLIPPING)) can_use = false;
@@ -725,6 +726,7 @@ RSG_FN void car_respawn_state(Car& c, int slot, float boost_amount) {
     c = Car();
     c.b = keep.b;
     c.preset = keep.preset; c.team = keep.team; c.id = keep.id;
+    c.phys = keep.phys;
     c.ctrl = keep.ctrl;
     c.gen_redraw = keep.gen_redraw; c.gen_left = keep.gen_left;
     // Per-wheel values persist across SetState in the reference.

@@ -153,7 +153,7 @@ RSG_FN void store_car(const CarRef& r, const Car& c) {
 // The car body's mass properties: the preset's unless the arena's
 // SetMutatorConfig replaced them (`inv_mass`, `inertia_mass` nonzero).
 RSG_FN void car_init_body(Car& c, float inv_mass, float inertia_mass) {
-    CarPhys cp = make_car_phys(c.preset);
+    const CarPhys& cp = c.phys;
     c.b.force = v3zero();
     c.b.torque = v3zero();
     c.b.inv_i_local = cp.inv_inertia_local;
@@ -214,6 +214,7 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     for (int i = 0; i < A.n_cars; i++) {
         CarRef cr = car_ref(d, a, i);
         load_car(cr, A.cars[i]);
+        A.cars[i].phys = make_car_phys(d.cfgs ? d.cfgs[A.cars[i].preset] : preset_config(A.cars[i].preset));
         car_init_body(A.cars[i], cr.f(CW_BODY_INV_MASS), cr.f(CW_BODY_INERTIA_MASS));
         A.cars[i].b.grav = cr.v(CW_BODY_GRAVITY);
         A.cars[i].b.active = !flag(A.cars[i].flags, RSG_CF_BODY_DISABLED);
@@ -436,7 +437,7 @@ RSG_FN void tick_cast_ray(ArenaLocal& A, const TickParams& P, const MeshView* me
     while (pos < A.n_cars && A.order[pos] != ci) pos++;
     world.ray_caster_pos = pos;
     const Car& c = A.cars[ci];
-    CarPhys cp = make_car_phys(c.preset);
+    const CarPhys& cp = c.phys;
     V3 source, target;
     wheel_ray(c, cp, wheel, source, target);
     out = world.raycast(source, target, 1 + ci);
@@ -447,7 +448,7 @@ RSG_FN void tick_cast_ray(ArenaLocal& A, const TickParams& P, const MeshView* me
 RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int ci, const RayHit* pre) {
     const float dt = P.dt;
     Car& c = A.cars[ci];
-    CarPhys cp = make_car_phys(c.preset);
+    const CarPhys& cp = c.phys;
     c.ctrl.throttle = clampf(c.ctrl.throttle, -1, 1);
     c.ctrl.steer = clampf(c.ctrl.steer, -1, 1);
     c.ctrl.pitch = clampf(c.ctrl.pitch, -1, 1);
@@ -573,7 +574,7 @@ RSG_FN void pad_check_cars(ArenaLocal& A, int p) {
         int iy = (int)(cp_uu.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2));
         int x0 = ix - 1 > 0 ? ix - 1 : 0, x1 = ix + 1 < PAD_GRID_CELLS_X - 1 ? ix + 1 : PAD_GRID_CELLS_X - 1;
         int y0 = iy - 1 > 0 ? iy - 1 : 0, y1 = iy + 1 < PAD_GRID_CELLS_Y - 1 ? iy + 1 : PAD_GRID_CELLS_Y - 1;
-        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A.pads[p], p, c, ci, make_car_phys(c.preset));
+        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A.pads[p], p, c, ci, c.phys);
     }
 }
 

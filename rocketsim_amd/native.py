@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 from .state import (ArenaSnapshot, BallPredView, BallState, BatchDesc, CarState, Controls, Event, GameEventConfig,
-                    MutatorConfig, PadState)
+                    CarConfig, MutatorConfig, PadState)
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "librsg.so")
@@ -76,6 +76,7 @@ def lib():
         L.rsg_batch_reset_kickoff.argtypes = [C.c_void_p, C.c_int, C.c_uint64]
         L.rsg_batch_is_ball_scored.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
         L.rsg_mutator_config_default.argtypes = [C.c_int, C.POINTER(MutatorConfig)]
+        L.rsg_car_config_preset.argtypes = [C.c_int, C.POINTER(CarConfig)]
         L.rsg_batch_set_mutator_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(MutatorConfig)]
         L.rsg_batch_get_mutator_config.argtypes = [C.c_void_p, C.c_int, C.POINTER(MutatorConfig)]
         L.rsg_batch_get_status.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -131,7 +132,7 @@ EXPORTED = [
     "rsg_shot_rollout", "rsg_shot_rollout_backward",
     "rsg_batch_get_tick_counts", "rsg_batch_set_tick_counts", "rsg_batch_get_goals",
     "rsg_batch_reset_kickoff", "rsg_batch_is_ball_scored", "rsg_mutator_config_default",
-    "rsg_batch_set_mutator_config", "rsg_batch_get_mutator_config",
+    "rsg_batch_set_mutator_config", "rsg_batch_get_mutator_config", "rsg_car_config_preset",
     "rsg_batch_get_status", "rsg_batch_clear_status", "rsg_batch_set_strict", "rsg_batch_status_summary",
     "rsg_batch_status_summary_noalloc",
     "rsg_ball_pred_create", "rsg_ball_pred_destroy", "rsg_ball_pred_force_update", "rsg_ball_pred_update_manual",

@@ -114,6 +114,19 @@ class PadState(C.Structure):
 
 
 DEMO_NORMAL, DEMO_ON_CONTACT, DEMO_DISABLED = 0, 1, 2
+PRESET_PSYCLOPS, PRESET_COUNT, PRESET_CUSTOM = 6, 7, 255
+
+
+class WheelPairConfig(C.Structure):
+    _fields_ = [("wheel_radius", C.c_float), ("suspension_rest_length", C.c_float),
+                ("connection_point_offset", C.c_float * 3)]
+
+
+class CarConfig(C.Structure):
+    """rsg_car_config: CarConfig (src/Sim/Car/CarConfig/CarConfig.h:6-43), uu."""
+    _fields_ = [("hitbox_size", C.c_float * 3), ("hitbox_pos_offset", C.c_float * 3),
+                ("front_wheels", WheelPairConfig), ("back_wheels", WheelPairConfig),
+                ("three_wheels", C.c_uint8), ("_pad", C.c_uint8 * 3), ("dodge_deadzone", C.c_float)]
 
 
 class MutatorConfig(C.Structure):
@@ -146,6 +159,7 @@ class ArenaSnapshot(C.Structure):
         ("last_car_id", C.c_uint32), ("n_cars", C.c_int32),
         ("car_id", C.c_uint32 * MAX_CARS), ("car_team", C.c_uint8 * MAX_CARS),
         ("car_preset", C.c_uint8 * MAX_CARS), ("_pad", C.c_uint8 * 6),
+        ("car_config", CarConfig * MAX_CARS),
         ("controls", Controls * MAX_CARS), ("cars", CarState * MAX_CARS),
         ("ball", BallState), ("pads", PadState * NUM_PADS), ("mutators", MutatorConfig),
         ("raw_layout", C.c_uint32), ("raw_words", C.c_uint32), ("raw", C.c_uint32 * SNAPSHOT_RAW_WORDS),
@@ -179,6 +193,7 @@ class BatchDesc(C.Structure):
         ("car_team", C.POINTER(C.c_uint8)),
         ("car_preset", C.POINTER(C.c_uint8)),
         ("mutators", C.POINTER(MutatorConfig)),
+        ("car_configs", C.POINTER(CarConfig)),
     ]
 
 

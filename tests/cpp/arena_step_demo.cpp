@@ -68,8 +68,11 @@ int main(int argc, char** argv) {
         rsg_amd::CarState os = blue->GetState();
         std::printf("objball %.6f %.6f %.6f\n", ob.pos[0], ob.pos[1], ob.pos[2]);
         std::printf("objcar0 %.6f %.6f %.6f\n", os.pos[0], os.pos[1], os.pos[2]);
-        rsg_amd::Car* third = obj->AddCar(rsg_amd::Team::BLUE, RSG_PRESET_PLANK);
-        bool ids = obj->GetCars().size() == 3 && third->id == 3 && obj->GetCar(2) != nullptr;
+        rsg_amd::CarConfig custom = rsg_amd::PresetCarConfig(RSG_PRESET_PSYCLOPS);
+        custom.dodge_deadzone = 0.7f;  // a config that is none of the presets
+        rsg_amd::Car* third = obj->AddCar(rsg_amd::Team::BLUE, custom);
+        bool ids = obj->GetCars().size() == 3 && third->id == 3 && obj->GetCar(2) != nullptr &&
+                   third->preset == RSG_PRESET_CUSTOM && third->config.dodge_deadzone == 0.7f;
         rsg_amd::CarState after = obj->GetCar(1)->GetState();  // carried over by AddCar
         bool kept = std::memcmp(&os, &after, sizeof(os)) == 0;
         ids = ids && obj->RemoveCar(2) && obj->GetCars().size() == 2 && obj->GetCar(2) == nullptr &&

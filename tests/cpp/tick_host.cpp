@@ -54,6 +54,8 @@ struct HostBatch {
     std::vector<int> cars_per_arena, car_offset;
     std::vector<uint32_t> car, ball, pad, arena;
     std::vector<rsg_mutator_config> mut;
+    std::vector<rsg_car_config> cfgs;  // config table: presets, then the cars' other configs
+    std::vector<uint8_t> cfg_index;    // each car's entry
     BatchDev dev{};
 };
 
@@ -62,10 +64,12 @@ struct HostBatch {
 extern "C" {
 
 // A host batch of n arenas (cars_per_arena[a] cars each, teams and presets
-// per car); cmf = the arena mesh file (NULL for THE_VOID); mut = the
-// MutatorConfig set before the cars are added (NULL: the mode's defaults).
+// per car, or each car's CarConfig in car_configs); cmf = the arena mesh file
+// (NULL for THE_VOID); mut = the MutatorConfig set before the cars are added
+// (NULL: the mode's defaults).
 void* th_create(int mode, int n, const int* cars_per_arena, const uint8_t* team, const uint8_t* preset,
-                const void* cmf, uint64_t cmf_size, float tick_rate, const rsg_mutator_config* mut) {
+                const void* cmf, uint64_t cmf_size, float tick_rate, const rsg_mutator_config* mut,
+                const rsg_car_config* car_configs) {
     auto* h = new HostBatch();
     if (cmf && !build_host_mesh(cmf, cmf_size, h->mesh).empty()) {
         delete h;
@@ -93,8 +97,17 @@ void* th_create(int mode, int n, const int* cars_per_arena, const uint8_t* team,
     d.tick_time = 1 / tick_rate;
     h->mut.assign((size_t)n, mut ? *mut : mutator_defaults(mode));
     d.mut = h->mut.data();
+    for (int p = 0; p < RSG_PRESET_COUNT; p++) h->cfgs.push_back(preset_config(p));
+    for (int i = 0; i < total; i++) {
+        if (!car_configs) { h->cfg_index.push_back(preset[i]); continue; }
+        size_t k = 0;
+        while (k < h->cfgs.size() && std::memcmp(&h->cfgs[k], &car_configs[i], sizeof(rsg_car_config)) != 0) k++;
+        if (k == h->cfgs.size()) h->cfgs.push_back(car_configs[i]);
+        h->cfg_index.push_back((uint8_t)k);
+    }
+    d.cfgs = h->cfgs.data();
     for (int a = 0; a < n; a++)
-        init_arena_record(d, a, cars_per_arena[a], team + h->car_offset[a], preset + h->car_offset[a]);
+        init_arena_record(d, a, cars_per_arena[a], team + h->car_offset[a], h->cfg_index.data() + h->car_offset[a]);
     return h;
 }
 

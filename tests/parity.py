@@ -124,11 +124,12 @@ def compare_pads(ref, got):
 
 class Sample:
     __slots__ = ("mode", "teams", "presets", "car_order", "cars_in", "ball_in", "pads_in", "ctrls", "tick_in",
-                 "cars_out", "ball_out", "pads_out", "tick_out", "events", "live")
+                 "cars_out", "ball_out", "pads_out", "tick_out", "events", "live", "configs")
 
     def __init__(self):
         self.events = None  # goal/bump/demo events of the tick (live oracle samples)
         self.live = False   # stepped by the oracle in this process (not loaded from a fixture)
+        self.configs = None  # the cars' CarConfigs when not presets
 
 
 def _rand_quat(rng):
@@ -271,24 +272,29 @@ def sample_soccar_drive(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1):
 
 
 def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1, near_walls=False, mutators=None,
-                        mutators_first=False):
+                        mutators_first=False, configs=None):
     """Cars placed 250-700 uu around the ball and driven at it (some
     supersonic, some jumping): car-ball hits with the extra impulse,
     car-car box-box contacts, bumps and demolitions, wheels resting on
     other cars.  With near_walls the ball starts by a side wall, a back wall
     or a corner, so cars drive onto the mesh's fillets and walls.  With
     `mutators` the arena's MutatorConfig is set before its cars are added
-    (`mutators_first`) or once they are."""
+    (`mutators_first`) or once they are.  With `configs` (CarConfigs) each car
+    takes one of them at random instead of a preset."""
     rng = random.Random(seed)
     out = []
     for _ in range(n_traj):
         A = ref.RefArena(MODE_SOCCAR)
         teams = [i % 2 for i in range(n_cars)]
         presets = [rng.randint(0, 5) for _ in range(n_cars)]
+        cfgs = [rng.choice(configs) for _ in range(n_cars)] if configs else None
         if mutators is not None and mutators_first:
             A.set_mutators(mutators)
-        for t, p in zip(teams, presets):
-            A.add_car(t, p)
+        for i, (t, p) in enumerate(zip(teams, presets)):
+            if cfgs:
+                A.add_car_config(t, cfgs[i])
+            else:
+                A.add_car(t, p)
         if mutators is not None and not mutators_first:
             A.set_mutators(mutators)
         bx, by = rng.uniform(-3000, 3000), rng.uniform(-4000, 4000)
@@ -330,6 +336,7 @@ def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1, n
                     hold[i] = rng.randint(10, 60)
             if t % stride == 0:
                 out.append(_one_tick(A, MODE_SOCCAR, teams, presets, ctrls))
+                out[-1].configs = cfgs
             else:
                 for i in range(n_cars):
                     A.set_controls(i, ctrls[i])
@@ -618,9 +625,14 @@ def run_on_gpu(samples, mode, kernel=0, per_block=0, auto_reset=False, seed=0, m
     creation (`mutators_first`: before the cars are added) or set after."""
     from rocketsim_amd.batch import ArenaBatch
     cpa = [len(s.teams) for s in samples]
+    car_configs = None
+    if any(getattr(s, "configs", None) for s in samples):
+        from tick_host import preset_config
+        car_configs = sum((list(s.configs) if getattr(s, "configs", None) else [preset_config(p) for p in s.presets]
+                           for s in samples), [])
     batch = ArenaBatch(cpa, sum((s.teams for s in samples), []), sum((s.presets for s in samples), []),
                        mode=mode, auto_reset=auto_reset, seed=seed,
-                       mutators=mutators if mutators_first else None)
+                       mutators=mutators if mutators_first else None, car_configs=car_configs)
     try:
         batch.set_kernel(kernel)
         batch.set_launch(per_block)

@@ -12,7 +12,7 @@ import pytest
 from rocketsim_amd import native
 from rocketsim_amd import state
 from rocketsim_amd.state import (ArenaSnapshot, BallPredView, BallState, BatchDesc, CarState, Controls, Event,
-                                 GameEventConfig, MutatorConfig, PadState)
+                                 CarConfig, GameEventConfig, MutatorConfig, PadState, WheelPairConfig)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -36,7 +36,8 @@ def test_library_exports_all_symbols():
 RECORDS = {"rsg_controls": Controls, "rsg_car_state": CarState, "rsg_ball_state": BallState,
            "rsg_pad_state": PadState, "rsg_batch_desc": BatchDesc, "rsg_event": Event,
            "rsg_ball_pred_view_t": BallPredView, "rsg_arena_snapshot": ArenaSnapshot,
-           "rsg_game_event_config": GameEventConfig, "rsg_mutator_config": MutatorConfig}
+           "rsg_game_event_config": GameEventConfig, "rsg_mutator_config": MutatorConfig,
+           "rsg_car_config": CarConfig, "rsg_wheel_pair_config": WheelPairConfig}
 # enum constants of include/rsg_state.h mirrored in rocketsim_amd.state
 CONSTANTS = {"RSG_PRED_POS": "PRED_POS", "RSG_PRED_ROT": "PRED_ROT", "RSG_PRED_VEL": "PRED_VEL",
              "RSG_PRED_ANG_VEL": "PRED_ANG_VEL", "RSG_PRED_TICKS_SINCE_UPDATE": "PRED_TICKS_SINCE_UPDATE",

@@ -25,7 +25,7 @@ import pytest
 
 import snapshot_cases
 from rocketsim_amd.batch import deserialize_snapshot, serialize_snapshot
-from rocketsim_amd.state import ArenaSnapshot, Controls, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID
+from rocketsim_amd.state import ArenaSnapshot, Controls, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID, PRESET_CUSTOM
 
 
 def _oracle():
@@ -52,6 +52,7 @@ def _by_id(snap):
         s.car_id[dst] = snap.car_id[k]
         s.car_team[dst] = snap.car_team[k]
         s.car_preset[dst] = snap.car_preset[k]
+        s.car_config[dst] = snap.car_config[k]
         s.controls[dst] = snap.controls[k]
         s.cars[dst] = snap.cars[k]
     return s
@@ -150,11 +151,14 @@ def test_deserialize_refuses_unsupported(ref):
         deserialize_snapshot(bad_mode)
     with pytest.raises(RuntimeError, match="truncated|count"):
         deserialize_snapshot(rb[:len(rb) // 2])
-    # car config: hitboxPosOffset.x of the first car changed
+    # car config: any config is read (hitboxPosOffset.x of the first car
+    # changed: a custom car); one the kernels cannot simulate is refused
     cfg_at = 4 + 1 + 4 + 8 + 4 + 42 + 1 + 4 + 1 + 4 + (4 + 5 * 4 + 3) + 4 + 4
-    bad_cfg = rb[:cfg_at] + struct.pack("<f", 14.0) + rb[cfg_at + 4:]
-    with pytest.raises(RuntimeError, match="preset"):
-        deserialize_snapshot(bad_cfg)
+    custom = deserialize_snapshot(rb[:cfg_at] + struct.pack("<f", 14.0) + rb[cfg_at + 4:])
+    assert custom.car_preset[0] == PRESET_CUSTOM and custom.car_config[0].hitbox_pos_offset[0] == 14.0
+    size_at = cfg_at + 16
+    with pytest.raises(RuntimeError, match="positive"):
+        deserialize_snapshot(rb[:size_at] + struct.pack("<f", -1.0) + rb[size_at + 4:])
     # mutator config: any values are read (gravity z here); a config with
     # another field count is refused, as MutatorConfig::Deserialize does
     m_at = len(rb) - snapshot_cases.MUTATOR_BYTES

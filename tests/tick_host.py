@@ -6,8 +6,8 @@ import ctypes as C
 import os
 import subprocess
 
-from rocketsim_amd.state import (BallState, CarState, Controls, MODE_HEATSEEKER, MODE_SOCCAR, MutatorConfig, NUM_PADS,
-                                 PadState)
+from rocketsim_amd.state import (BallState, CarConfig, CarState, Controls, MODE_HEATSEEKER, MODE_SOCCAR, MutatorConfig,
+                                 NUM_PADS, PadState)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -49,7 +49,7 @@ def load(path):
     L = C.CDLL(path)
     L.th_create.restype = C.c_void_p
     L.th_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
-                            C.c_char_p, C.c_uint64, C.c_float, C.POINTER(MutatorConfig)]
+                            C.c_char_p, C.c_uint64, C.c_float, C.POINTER(MutatorConfig), C.POINTER(CarConfig)]
     L.th_destroy.argtypes = [C.c_void_p]
     for name, t in (("cars", CarState), ("balls", BallState), ("pads", PadState)):
         getattr(L, "th_set_" + name).argtypes = [C.c_void_p, C.POINTER(t)]
@@ -80,6 +80,25 @@ def same_rsqrt_as_recording_cpu():
     return [struct.unpack("<I", struct.pack("<f", x))[0] for x in out] == RECORDING_CPU_RSQRT
 
 
+def _car_configs(samples):
+    """The samples' cars' configs (a sample may carry `configs`, one per car),
+    or None when every car uses a preset."""
+    if not any(getattr(s, "configs", None) for s in samples):
+        return None
+    out = []
+    for s in samples:
+        cfgs = getattr(s, "configs", None)
+        out += list(cfgs) if cfgs else [preset_config(p) for p in s.presets]
+    return (CarConfig * len(out))(*out)
+
+
+def preset_config(p):
+    from rocketsim_amd import native
+    c = CarConfig()
+    native.check(native.lib().rsg_car_config_preset(p, C.byref(c)))
+    return c
+
+
 def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators_first=False):
     """Step every teacher-forced sample once on the host build, one sample per
     arena (the host counterpart of parity.run_on_gpu).  `inject(cars, balls)`
@@ -96,7 +115,8 @@ def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators
     total = sum(cpa)
     h = L.th_create(mode, n, (C.c_int * n)(*cpa), (C.c_uint8 * max(total, 1))(*teams),
                     (C.c_uint8 * max(total, 1))(*presets), cmf, len(cmf) if cmf else 0, 120.0,
-                    C.byref(mutators) if mutators is not None and mutators_first else None)
+                    C.byref(mutators) if mutators is not None and mutators_first else None,
+                    _car_configs(samples))
     if not h:
         raise RuntimeError("th_create failed")
     try:
