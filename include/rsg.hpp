@@ -51,7 +51,7 @@ using CarConfig = rsg_car_config;
 
 // The reference's CAR_CONFIG_* presets (CarConfig.cpp:93-100).
 inline CarConfig PresetCarConfig(int preset) {
-    CarConfig c;
+    CarConfig c{};
     check(rsg_car_config_preset(preset, &c));
     return c;
 }
@@ -271,10 +271,13 @@ public:
     }
     // Arena::Clone (Arena.cpp:646-687): a new arena in the same state, its
     // mutator config included (the reference's Clone leaves the new arena at
-    // its mode's defaults).
+    // its mode's defaults).  The copy goes through the full snapshot, which
+    // keeps every car's config whole (the byte format drops threeWheels).
     std::unique_ptr<Arena> Clone() {
         std::unique_ptr<Arena> a(new Arena(mode_, seed_, specs_));
-        a->Deserialize(Serialize());
+        rsg_arena_snapshot snap;
+        check(rsg_batch_get_snapshot(b_->handle(), 0, &snap));
+        check(rsg_batch_set_snapshot(a->b_->handle(), 0, &snap));
         for (size_t i = 0; i < cars_.size(); i++) a->cars_[i]->controls = cars_[i]->controls;
         return a;
     }

@@ -182,12 +182,20 @@ inline rsg_car_config config_of(const BatchDev& h, int index) {
     return h.cfgs ? h.cfgs[index] : preset_config(index);
 }
 
+// Two CarConfigs with equal fields (the padding bytes are not compared).
+inline bool same_car_config(const rsg_car_config& a, const rsg_car_config& b) {
+    rsg_car_config x = a, y = b;
+    std::memset(x._pad, 0, sizeof(x._pad));
+    std::memset(y._pad, 0, sizeof(y._pad));
+    x.three_wheels = x.three_wheels != 0;
+    y.three_wheels = y.three_wheels != 0;
+    return std::memcmp(&x, &y, sizeof(x)) == 0;
+}
+
 // The preset a config equals, or RSG_PRESET_CUSTOM.
 inline int preset_of(const rsg_car_config& c) {
-    for (int p = 0; p < RSG_PRESET_COUNT; p++) {
-        rsg_car_config q = preset_config(p);
-        if (std::memcmp(&q, &c, sizeof(c)) == 0) return p;
-    }
+    for (int p = 0; p < RSG_PRESET_COUNT; p++)
+        if (same_car_config(preset_config(p), c)) return p;
     return RSG_PRESET_CUSTOM;
 }
 

@@ -260,14 +260,17 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
             return nullptr;
         }
         size_t k = 0;
-        while (k < b->h_cfgs.size() && std::memcmp(&b->h_cfgs[k], &c, sizeof(c)) != 0) k++;
+        while (k < b->h_cfgs.size() && !same_car_config(b->h_cfgs[k], c)) k++;
         if (k == b->h_cfgs.size()) {
             if (k >= 255) {
                 delete b;
                 fail("rsg_batch_create: more than 255 distinct car configs");
                 return nullptr;
             }
-            b->h_cfgs.push_back(c);
+            rsg_car_config n = c;
+            std::memset(n._pad, 0, sizeof(n._pad));
+            n.three_wheels = n.three_wheels != 0;
+            b->h_cfgs.push_back(n);
         }
         b->preset[(size_t)i] = (uint8_t)k;
     }
@@ -1231,7 +1234,7 @@ int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s)
         if (i > 0 && s->car_id[k] == s->car_id[by_id[i - 1]]) return fail("rsg_batch_set_snapshot: repeated car id");
         int g = b->car_offset[arena] + i;
         const rsg_car_config want = snapshot_car_config(*s, k);
-        if (s->car_team[k] != b->team[g] || std::memcmp(&want, &b->h_cfgs[b->preset[g]], sizeof(want)) != 0)
+        if (s->car_team[k] != b->team[g] || !same_car_config(want, b->h_cfgs[b->preset[g]]))
             return fail("rsg_batch_set_snapshot: car " + std::to_string(s->car_id[k]) +
                         " has another team or car config than slot " + std::to_string(i));
         slot_of[k] = i;

@@ -189,7 +189,7 @@ RSG_FN rsg_wheel_pair_config wheel_pair(float radius, float rest, float x, float
 
 RSG_FN rsg_car_config car_config(float sx, float sy, float sz, float ox, float oy, float oz,
                                  rsg_wheel_pair_config front, rsg_wheel_pair_config back, bool three_wheels) {
-    rsg_car_config c;
+    rsg_car_config c{};
     c.hitbox_size[0] = sx; c.hitbox_size[1] = sy; c.hitbox_size[2] = sz;
     c.hitbox_pos_offset[0] = ox; c.hitbox_pos_offset[1] = oy; c.hitbox_pos_offset[2] = oz;
     c.front_wheels = front;

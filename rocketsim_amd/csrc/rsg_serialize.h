@@ -92,7 +92,7 @@ struct ByteIn {
 // entry is all zero.
 inline rsg_car_config snapshot_car_config(const rsg_arena_snapshot& s, int k) {
     static const rsg_car_config zero = {};
-    if (std::memcmp(&s.car_config[k], &zero, sizeof(zero)) != 0) return s.car_config[k];
+    if (!same_car_config(s.car_config[k], zero)) return s.car_config[k];
     return preset_config(s.car_preset[k] < RSG_PRESET_COUNT ? s.car_preset[k] : 0);
 }
 
