@@ -53,9 +53,17 @@ typedef struct rsg_batch_desc {
 const char* rsg_last_error(void);
 
 /* Replaces RocketSim::InitFromMem (src/RocketSim.h:32, src/RocketSim.cpp:92-185)
- * for the Soccar arena: one collision mesh in the reference's .cmf format
- * (src/CollisionMeshFile/CollisionMeshFile.cpp:11-61).  Must be called once
- * before creating a SOCCAR batch. */
+ * for the Soccar arena (Soccar and Heatseeker batches): n collision mesh
+ * files in the reference's .cmf format
+ * (src/CollisionMeshFile/CollisionMeshFile.cpp:11-61), 1 <= n <= 28, in the
+ * order the reference's list holds them.  Each mesh becomes one static body
+ * of every arena, in this order and before the floor, ceiling and side walls
+ * (Arena::_SetupArenaCollisionShapes, src/Sim/Arena/Arena.cpp:990-1068).
+ * Unlike the reference, a later call replaces the meshes for batches stepped
+ * afterwards (it waits for the device first).  Must be called before
+ * creating a SOCCAR or HEATSEEKER batch. */
+int rsg_init_meshes(const void* const* cmf_data, const uint64_t* sizes, int n);
+/* rsg_init_meshes with one mesh file. */
 int rsg_init_mesh(const void* cmf_data, uint64_t size);
 
 /* Replaces Arena::Create + AddCar (src/Sim/Arena/Arena.h:71,111).  Cars start

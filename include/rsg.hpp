@@ -46,6 +46,16 @@ inline MutatorConfig DefaultMutatorConfig(GameMode mode) {
 
 // RocketSim::InitFromMem (src/RocketSim.h:32) for the Soccar collision mesh.
 inline void InitFromMem(const void* cmf_data, uint64_t size) { check(rsg_init_mesh(cmf_data, size)); }
+// RocketSim::InitFromMem with the Soccar arena's list of mesh files, in order.
+inline void InitFromMem(const std::vector<std::vector<uint8_t>>& cmf_files) {
+    std::vector<const void*> data;
+    std::vector<uint64_t> sizes;
+    for (const auto& f : cmf_files) {
+        data.push_back(f.data());
+        sizes.push_back(f.size());
+    }
+    check(rsg_init_meshes(data.data(), sizes.data(), (int)data.size()));
+}
 
 using CarConfig = rsg_car_config;
 

@@ -217,6 +217,44 @@ def from_cmf(data):
 
 
 _CACHE = None
+_SPLIT_CACHE = None
+
+# The pieces of soccar_split_cmfs(), in load order.
+SPLIT_PIECES = ("blue goal", "orange goal", "-x -y", "+x -y", "-x +y", "+x +y")
+
+
+def split_mesh(V, T):
+    """Split the arena's triangles into SPLIT_PIECES by centroid: the two goal
+    boxes (behind the back walls), then the four quadrants of the rest.  Each
+    piece keeps its triangles in their original order and its used vertices
+    in theirs.  Returns a list of (verts, tris)."""
+    V = np.asarray(V, dtype=np.float32)
+    T = np.asarray(T, dtype=np.int32)
+    c = V[T].mean(axis=1)
+    goal_b = c[:, 1] < -EXTENT_Y + 1.0
+    goal_o = c[:, 1] > EXTENT_Y - 1.0
+    rest = ~(goal_b | goal_o)
+    nx, ny = c[:, 0] < 0, c[:, 1] < 0
+    masks = [goal_b, goal_o, rest & nx & ny, rest & ~nx & ny, rest & nx & ~ny, rest & ~nx & ~ny]
+    out = []
+    for m in masks:
+        t = T[m]
+        used = np.unique(t)
+        remap = np.full(len(V), -1, dtype=np.int32)
+        remap[used] = np.arange(len(used), dtype=np.int32)
+        out.append((V[used], remap[t]))
+    return out
+
+
+def soccar_split_cmfs():
+    """The synthetic Soccar arena as six .cmf files (SPLIT_PIECES), the way
+    the reference's arena comes as a list of mesh files (src/RocketSim.cpp:
+    20-28,92-185); each becomes its own static body."""
+    global _SPLIT_CACHE
+    if _SPLIT_CACHE is None:
+        V, T = build_mesh()
+        _SPLIT_CACHE = [to_cmf(v, t) for v, t in split_mesh(V, T)]
+    return list(_SPLIT_CACHE)
 
 
 def soccar_cmf():

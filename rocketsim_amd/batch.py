@@ -13,19 +13,43 @@ from .state import (ArenaSnapshot, BallState, BatchDesc, CarState, Controls, Eve
 
 
 _mesh_loaded = False
+_mesh_files = None  # the .cmf files loaded last, in order
+
+
+def init_meshes(cmfs):
+    """Load the Soccar arena's collision meshes (RocketSim::InitFromMem,
+    src/RocketSim.cpp:92-185): `cmfs` is the list of .cmf file bytes, in the
+    order the reference's list would hold them; each becomes one static body
+    of every arena stepped afterwards."""
+    global _mesh_loaded, _mesh_files
+    files = [bytes(c) for c in cmfs]
+    bufs = [C.create_string_buffer(c, len(c)) for c in files]
+    n = len(files)
+    ptrs = (C.c_void_p * max(n, 1))(*[C.cast(b, C.c_void_p) for b in bufs])
+    sizes = (C.c_uint64 * max(n, 1))(*[len(c) for c in files])
+    native.check(native.lib().rsg_init_meshes(ptrs, sizes, n))
+    _mesh_loaded = True
+    _mesh_files = files
 
 
 def init_mesh(cmf=None):
-    """Load the Soccar collision mesh (RocketSim::Init / InitFromMem,
-    src/RocketSim.cpp:92-185): `cmf` is the bytes of one .cmf file; by default
-    the procedural arena of rocketsim_amd.arena_mesh."""
-    global _mesh_loaded
+    """Load a one-mesh Soccar arena: `cmf` is the bytes of one .cmf file; by
+    default the procedural arena of rocketsim_amd.arena_mesh."""
     if cmf is None:
         from . import arena_mesh
         cmf = arena_mesh.soccar_cmf()
-    buf = C.create_string_buffer(bytes(cmf), len(cmf))
-    native.check(native.lib().rsg_init_mesh(buf, len(cmf)))
-    _mesh_loaded = True
+    init_meshes([cmf])
+
+
+def ensure_meshes(cmfs=None):
+    """init_meshes(cmfs) unless exactly these files are loaded already
+    (None: the procedural one-mesh arena)."""
+    if cmfs is None:
+        from . import arena_mesh
+        cmfs = [arena_mesh.soccar_cmf()]
+    files = [bytes(c) for c in cmfs]
+    if not _mesh_loaded or _mesh_files != files:
+        init_meshes(files)
 
 
 def serialize_snapshot(snap):

@@ -70,7 +70,7 @@ RSG_FN bool pred_matches(const uint32_t* a, size_t sa, const uint32_t* b, size_t
 // PRED_MANUAL = UpdatePredManual (:25-61), PRED_FROM_BATCH =
 // UpdatePredFromArena (:18-23) on arena i of `src`.  `pa` holds the trackers'
 // ball-only prediction arenas; `A` is the working copy of arena i.
-RSG_FN void ball_pred_update(const BatchDev& pa, const MeshView* mp, const PredDev& p, int kind, const BatchDev& src,
+RSG_FN void ball_pred_update(const BatchDev& pa, const ArenaGeom* mp, const PredDev& p, int kind, const BatchDev& src,
                              int i, ArenaLocal& A, StepSpace* ws) {
     load_arena(pa, i, A);
     TickParams P;

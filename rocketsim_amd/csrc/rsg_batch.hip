@@ -57,22 +57,28 @@ int fail(const std::string& msg) {
         if (_e != hipSuccess) return fail(std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
-// The Soccar collision mesh, shared read-only by every batch of the process
-// (RocketSim keeps its arena meshes in process-wide storage the same way,
-// src/RocketSim.cpp:92-185).  Device arrays are allocated once per init.
+// One collision mesh's device arrays.
 struct DeviceMesh {
-    bool ready = false;
-    HostMesh host;
     float* verts = nullptr;
     int32_t* tris = nullptr;
     QNode* nodes = nullptr;
     QSubtree* subtrees = nullptr;
     TriInfo* tri_info = nullptr;
-    uint8_t* cell_statics = nullptr;
     uint32_t* occ = nullptr;
-    MeshView view{};
 };
-DeviceMesh g_mesh;
+// The Soccar arena's static geometry, shared read-only by every batch of the
+// process (RocketSim keeps its arena meshes in process-wide storage the same
+// way, src/RocketSim.cpp:50-53,92-185).  Device arrays are allocated once per
+// init.
+struct DeviceArena {
+    bool ready = false;
+    HostArena host;
+    std::vector<DeviceMesh> meshes;
+    MeshView* views = nullptr;      // device copies of the meshes' views
+    uint32_t* cell_statics = nullptr;
+    ArenaGeom geom{};
+};
+DeviceArena g_mesh;
 std::mutex g_mesh_mutex;
 
 template <class T>
@@ -445,10 +451,10 @@ namespace {
 int launch_step(const rsg_batch* b, const BatchDev& dev, const Launch& L, int ticks, hipStream_t s) {
     int block = L.per_block;
     int grid = (dev.n + block - 1) / block;
-    MeshView mv{};
+    ArenaGeom mv{};
     if (mode_has_arena(dev.mode)) {
         if (!g_mesh.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
-        mv = g_mesh.view;
+        mv = g_mesh.geom;
     }
     if (L.kernel == RSG_KERNEL_COOP)
         HIP_OK(launch_step_coop(dev, mv, ticks, grid, block, coop_stride(b), lds_stride(b), coop_lanes(b), s));
@@ -1333,10 +1339,10 @@ int pred_wait(rsg_ball_pred* p) {
 int pred_launch(rsg_ball_pred* p, int kind, const BatchDev* src, hipStream_t s) {
     rsg_batch* a = p->arena;
     if (upload(a) != 0) return -1;
-    MeshView mv{};
+    ArenaGeom mv{};
     if (mode_has_arena(a->dev.mode)) {
         if (!g_mesh.ready) return fail("rsg_ball_pred: no arena mesh (call rsg_init_mesh first)");
-        mv = g_mesh.view;
+        mv = g_mesh.geom;
     }
     PredDev d;
     d.traj = p->traj;
@@ -1552,31 +1558,53 @@ int rsg_ball_pred_view(rsg_ball_pred* p, rsg_ball_pred_view_t* out) {
     return 0;
 }
 
-int rsg_init_mesh(const void* cmf_data, uint64_t size) {
+int rsg_init_meshes(const void* const* cmf_data, const uint64_t* sizes, int n) {
+    if (n > 0 && (!cmf_data || !sizes)) return fail("rsg_init_meshes: null argument");
     std::lock_guard<std::mutex> lock(g_mesh_mutex);
-    HostMesh m;
-    std::string err = build_host_mesh(cmf_data, size, m);
-    if (!err.empty()) return fail("rsg_init_mesh: invalid collision mesh file (" + err + ")");
+    HostArena H;
+    std::string err = build_host_arena(cmf_data, sizes, n, H);
+    if (!err.empty()) return fail("rsg_init_meshes: invalid collision mesh files (" + err + ")");
+    // kernels of earlier launches may still read the old arrays
     (void)hipDeviceSynchronize();
     g_mesh.ready = false;
-    g_mesh.host = std::move(m);
-    const HostMesh& H = g_mesh.host;
-    if (upload_array(&g_mesh.verts, H.verts) || upload_array(&g_mesh.tris, H.tris) ||
-        upload_array(&g_mesh.nodes, H.nodes) || upload_array(&g_mesh.subtrees, H.subtrees) ||
-        upload_array(&g_mesh.tri_info, H.tri_info) || upload_array(&g_mesh.cell_statics, H.cell_statics) ||
-        upload_array(&g_mesh.occ, H.occ))
-        return -1;
-    MeshView v = H.view();
-    v.verts = g_mesh.verts;
-    v.tris = g_mesh.tris;
-    v.nodes = g_mesh.nodes;
-    v.subtrees = g_mesh.subtrees;
-    v.tri_info = g_mesh.tri_info;
-    v.cell_statics = g_mesh.cell_statics;
-    v.occ = g_mesh.occ;
-    g_mesh.view = v;
+    for (DeviceMesh& m : g_mesh.meshes) {
+        (void)hipFree(m.verts); (void)hipFree(m.tris); (void)hipFree(m.nodes);
+        (void)hipFree(m.subtrees); (void)hipFree(m.tri_info); (void)hipFree(m.occ);
+    }
+    g_mesh.meshes.assign(H.meshes.size(), DeviceMesh());
+    g_mesh.host = std::move(H);
+    std::vector<MeshView> views;
+    for (size_t i = 0; i < g_mesh.host.meshes.size(); i++) {
+        const HostMesh& M = g_mesh.host.meshes[i];
+        DeviceMesh& D = g_mesh.meshes[i];
+        if (upload_array(&D.verts, M.verts) || upload_array(&D.tris, M.tris) || upload_array(&D.nodes, M.nodes) ||
+            upload_array(&D.subtrees, M.subtrees) || upload_array(&D.tri_info, M.tri_info) ||
+            upload_array(&D.occ, M.occ))
+            return -1;
+        MeshView v = M.view();
+        v.verts = D.verts;
+        v.tris = D.tris;
+        v.nodes = D.nodes;
+        v.subtrees = D.subtrees;
+        v.tri_info = D.tri_info;
+        v.occ = D.occ;
+        views.push_back(v);
+    }
+    if (upload_array(&g_mesh.views, views) || upload_array(&g_mesh.cell_statics, g_mesh.host.cell_statics)) return -1;
+    ArenaGeom g{};
+    g.meshes = g_mesh.views;
+    g.n_meshes = (int)views.size();
+    g.cell_statics = g_mesh.cell_statics;
+    g.plane_unit = g_mesh.host.plane_unit;
+    g_mesh.geom = g;
     g_mesh.ready = true;
     return 0;
+}
+
+int rsg_init_mesh(const void* cmf_data, uint64_t size) {
+    const void* data[1] = {cmf_data};
+    const uint64_t sizes[1] = {size};
+    return rsg_init_meshes(data, sizes, 1);
 }
 
 }  // extern "C"

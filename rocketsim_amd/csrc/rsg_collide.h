@@ -24,7 +24,7 @@
 namespace rsg {
 
 constexpr int MAX_BODIES = 1 + RSG_MAX_CARS;
-constexpr int MAX_PAIRS = MAX_BODIES * NUM_STATICS + MAX_BODIES * (MAX_BODIES - 1) / 2;
+constexpr int MAX_PAIRS = MAX_BODIES * MAX_STATICS + MAX_BODIES * (MAX_BODIES - 1) / 2;
 
 struct Pair {
     int a, b;  // body codes, a has the lower unique id (statics < ball < cars)
@@ -132,8 +132,9 @@ struct ArenaSpace {
 
 struct World {
     ArenaLocal& A;
-    const MeshView* mesh;
+    const ArenaGeom* geo;  // the arena's statics (null without an arena)
     bool soccar;
+    const MeshView* mesh;  // the mesh of the static pair being run
     StepSpace* ws;  // null for worlds that only cast rays
 
     WorldTables* T;  // ws->wt
@@ -147,8 +148,8 @@ struct World {
     EpaSpace* epa;
     bool aborted;
 
-    RSG_FN World(ArenaLocal& a, const MeshView* m, bool s, StepSpace* w = nullptr)
-        : A(a), mesh(m), soccar(s), ws(w), T(w ? &w->wt : nullptr), ray_order_ready(false), mf(w ? w->mf : nullptr),
+    RSG_FN World(ArenaLocal& a, const ArenaGeom* g, bool s, StepSpace* w = nullptr)
+        : A(a), geo(g), soccar(s), mesh(nullptr), ws(w), T(w ? &w->wt : nullptr), ray_order_ready(false), mf(w ? w->mf : nullptr),
           rec(nullptr), epa(w ? &w->epa : nullptr), aborted(false) {}
 
     // Whether the subsimplex ray cast against a convex of bounding radius
@@ -202,15 +203,15 @@ struct World {
 
     RSG_FN Xf xf_of(int code) {
         if (code >= 0) { Body* b = body_ptr(code); return xf_from(b->pos, b->rot); }
-        if (code == -1 - ST_MESH) return xf_ident();
-        PlaneDef p = plane_def(-1 - code);
+        if (static_is_mesh(*geo, -1 - code)) return xf_ident();
+        PlaneDef p = plane_def(-1 - code - geo->n_meshes);
         return xf_from(p.origin, m3_identity());
     }
 
     RSG_FN float threshold_of(int code) {
         if (code == 0) return make_ball_phys(A.ball.radius_uu, A.ball.mass).angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
         if (code > 0) return A.cars[code - 1].phys.angular_motion_disc * CONTACT_BREAKING_THRESHOLD;
-        return 1e30f;  // planes and the arena mesh have huge bounding spheres
+        return 1e30f;  // planes and the arena meshes have huge bounding spheres
     }
 
     RSG_FN void open_manifold(int mi, int b0, int b1) {
@@ -252,7 +253,7 @@ struct World {
 
     RSG_FN bool static_in_cell(int s, int c) {
         if (!soccar) return false;
-        return (mesh->cell_statics[c] >> s) & 1u;
+        return (geo->cell_statics[c] >> s) & 1u;
     }
 
     // -----------------------------------------------------------------------
@@ -274,10 +275,12 @@ struct World {
         V3 n = v3zero();
         int hit_obj = -100;
         // static objects registered in the cell, in registration order
-        for (int s = 0; s < NUM_STATICS && soccar; s++) {
+        const int ns = soccar ? num_statics(*geo) : 0;
+        for (int s = 0; s < ns; s++) {
             if (!static_in_cell(s, c)) continue;
             if (frac == 0.f) break;
-            if (s == ST_MESH) {
+            if (static_is_mesh(*geo, s)) {
+                const MeshView& m = geo->meshes[s];
                 struct RayCB {
                     const MeshView* m; V3 from, to; float* frac; V3* n; bool hit;
                     RSG_FN void operator()(int t) {
@@ -285,11 +288,11 @@ struct World {
                         mesh_triangle(*m, t, v);
                         if (ray_triangle(from, to, v[0], v[1], v[2], *frac, *n)) hit = true;
                     }
-                } cb{mesh, from, to, &frac, &n, false};
-                mesh_query_ray(*mesh, from, to, cb);
+                } cb{&m, from, to, &frac, &n, false};
+                mesh_query_ray(m, from, to, cb);
                 if (cb.hit) hit_obj = -1 - s;
             } else {
-                if (ray_plane(s, mesh->plane_unit, from, to, frac, n)) hit_obj = -1 - s;
+                if (ray_plane(s - geo->n_meshes, geo->plane_unit, from, to, frac, n)) hit_obj = -1 - s;
             }
         }
         RSG_PROF_MARK(10);
@@ -402,7 +405,7 @@ struct World {
             cp.restitution = CARCAR_COLLISION_RESTITUTION;
             effect(EFF_CAR_CAR, b0 - 1, b1 - 1, cp.local_a, cp.local_b);
         }
-        if (b1 == -1 - ST_MESH && cp.tri >= 0) adjust_internal_edge(cp, *mesh, cp.tri);
+        if (b1 < 0 && cp.tri >= 0 && static_is_mesh(*geo, -1 - b1)) adjust_internal_edge(cp, geo->meshes[-1 - b1], cp.tri);
     }
 
     RSG_FN void effect(int kind, int a, int b, V3 v0, V3 v1) {
@@ -548,8 +551,8 @@ struct World {
     // Narrowphase
 
     // btConvexPlaneCollisionAlgorithm::processCollision for the ball
-    RSG_FN void ball_vs_plane(int mi, int s) {
-        PlaneDef pd = plane_def(s, mesh->plane_unit);
+    RSG_FN void ball_vs_plane(int mi, int plane) {
+        PlaneDef pd = plane_def(plane, geo->plane_unit);
         Xf px = xf_from(pd.origin, m3_identity());
         Body& b = A.ball.b;
         Xf bx = xf_from(b.pos, b.rot);
@@ -572,8 +575,8 @@ struct World {
     }
 
     // Child box of a car vs a plane (same algorithm, box support with margin)
-    RSG_FN void car_vs_plane(int mi, int code, int s) {
-        PlaneDef pd = plane_def(s, mesh->plane_unit);
+    RSG_FN void car_vs_plane(int mi, int code, int plane) {
+        PlaneDef pd = plane_def(plane, geo->plane_unit);
         Xf px = xf_from(pd.origin, m3_identity());
         Car& c = A.cars[code - 1];
         const CarPhys& cp = c.phys;
@@ -686,7 +689,7 @@ struct World {
         return shape_box(cp.box_implicit, cp.box_margin);
     }
 
-    // Car hitbox vs the arena mesh: btConvexConcaveCollisionAlgorithm with the
+    // Car hitbox vs an arena mesh (World::mesh): btConvexConcaveCollisionAlgorithm with the
     // per-triangle early out, then btConvexConvexAlgorithm (GJK) per triangle.
     RSG_FN void car_vs_mesh(int mi, int code) {
         Xf cx = car_child_xf(code);
@@ -762,7 +765,8 @@ struct World {
         refresh(mi);
     }
 
-    // btConvexConcaveCollisionAlgorithm::processCollision for the ball
+    // btConvexConcaveCollisionAlgorithm::processCollision for the ball vs
+    // an arena mesh (World::mesh)
     RSG_FN void ball_vs_mesh(int mi) {
         Body& b = A.ball.b;
         float r = make_ball_phys(A.ball.radius_uu, A.ball.mass).radius + 0.08f;
@@ -900,10 +904,11 @@ struct World {
         const int nb = num_bodies();
         for (int p = 0; p < nb; p++) {
             int c = A.bp_cell[p];
-            for (int s = 0; s < NUM_STATICS; s++) {
+            const int ns = soccar ? num_statics(*geo) : 0;
+            for (int s = 0; s < ns; s++) {
                 if (!static_in_cell(s, c)) continue;
                 V3 smin, smax;
-                static_proxy_aabb(mesh, s, smin, smax);
+                static_proxy_aabb(geo, s, smin, smax);
                 if (aabb_overlap(T->aabb_min[p], T->aabb_max[p], smin, smax)) {
                     T->pairs[T->n_pairs].a = -1 - s;
                     T->pairs[T->n_pairs].b = p;
@@ -956,7 +961,7 @@ struct World {
                 if (b != 0) {
                     V3 cmin, cmax, smin, smax;
                     car_child_aabb(b, cmin, cmax);
-                    static_shape_aabb(mesh, -1 - a, smin, smax);
+                    static_shape_aabb(geo, -1 - a, smin, smax);
                     keeps = aabb_overlap(cmin, cmax, smin, smax);
                 }
             } else if (a == 0) {
@@ -991,13 +996,15 @@ struct World {
         aborted = false;
         open_manifold(mi, b, a);
         if (a < 0) {
-            int s = -1 - a;
+            const int s = -1 - a;
+            const bool is_mesh = static_is_mesh(*geo, s);
+            mesh = is_mesh ? &geo->meshes[s] : nullptr;
             if (b == 0) {
-                if (s == ST_MESH) ball_vs_mesh(mi);
-                else ball_vs_plane(mi, s);
+                if (is_mesh) ball_vs_mesh(mi);
+                else ball_vs_plane(mi, s - geo->n_meshes);
             } else {
-                if (s == ST_MESH) car_vs_mesh(mi, b);
-                else car_vs_plane(mi, b, s);
+                if (is_mesh) car_vs_mesh(mi, b);
+                else car_vs_plane(mi, b, s - geo->n_meshes);
             }
         } else if (a == 0) {
             car_vs_ball(mi, b);

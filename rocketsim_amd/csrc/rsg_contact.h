@@ -1,5 +1,5 @@
-// rsg_contact.h -- contact generation against the static arena (4 planes +
-// the shared triangle mesh), contact records, the reference's contact-added
+// rsg_contact.h -- contact generation against the static arena (the shared
+// triangle meshes + 4 planes), contact records, the reference's contact-added
 // rules and the internal-edge normal fix-up.
 //
 // Reference behaviour restated here:
@@ -20,9 +20,27 @@
 
 namespace rsg {
 
-// Static collision objects in the reference's creation order: the mesh, then
-// the floor, ceiling and the two side walls (Arena::_SetupArenaCollisionShapes).
-enum StaticId { ST_MESH = 0, ST_FLOOR = 1, ST_CEIL = 2, ST_WALL_NX = 3, ST_WALL_PX = 4, NUM_STATICS = 5 };
+// Static collision objects in the reference's creation order
+// (Arena::_SetupArenaCollisionShapes, src/Sim/Arena/Arena.cpp:990-1068): the
+// arena's collision meshes in load order, then the floor, the ceiling and the
+// two side walls.  In an arena of M meshes static object s is mesh s for
+// s < M and plane s - M otherwise; its body code is -1 - s.
+enum PlaneId { PL_FLOOR = 0, PL_CEIL = 1, PL_WALL_NX = 2, PL_WALL_PX = 3, NUM_PLANES = 4 };
+constexpr int MAX_MESHES = 28;
+constexpr int MAX_STATICS = MAX_MESHES + NUM_PLANES;  // one bit each in a cell's mask
+
+// An arena's static geometry as the kernels see it, shared by every arena
+// of the process (the reference keeps its meshes in process-wide storage,
+// src/RocketSim.cpp:50-53).
+struct ArenaGeom {
+    const MeshView* meshes;        // n_meshes views, in load order
+    int n_meshes;
+    const uint32_t* cell_statics;  // BP_NUM_CELLS masks: bit s = static object s registered in the cell
+    // Length of the static planes' stored unit normals (see plane_def).
+    float plane_unit;
+};
+RSG_FN int num_statics(const ArenaGeom& g) { return g.n_meshes + NUM_PLANES; }
+RSG_FN bool static_is_mesh(const ArenaGeom& g, int s) { return s < g.n_meshes; }
 
 struct PlaneDef {
     V3 n;        // plane normal
@@ -35,23 +53,23 @@ struct PlaneDef {
 // keeps planeNormal.normalized() (btStaticPlaneShape.cpp:21), whose SSE
 // reciprocal-square-root normalisation of a unit axis is the host CPU's
 // rsqrtss(1) after one Newton step (0.99999994 on common x86 cores), measured
-// once on the host (MeshView::plane_unit).
-RSG_FN PlaneDef plane_def(int s, float unit = 1.f) {
+// once on the host (ArenaGeom::plane_unit).
+RSG_FN PlaneDef plane_def(int plane, float unit = 1.f) {
     PlaneDef p;
-    switch (s) {
+    switch (plane) {
         default:
-        case ST_FLOOR: p.n = v3(0, 0, unit); p.origin = v3(0, 0, 0) * UU_TO_BT; p.axis = 2; p.backwards = false; break;
-        case ST_CEIL: p.n = v3(0, 0, -unit); p.origin = v3(0, 0, ARENA_HEIGHT) * UU_TO_BT; p.axis = 2; p.backwards = true; break;
-        case ST_WALL_NX: p.n = v3(unit, 0, 0); p.origin = v3(-ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = false; break;
-        case ST_WALL_PX: p.n = v3(-unit, 0, 0); p.origin = v3(ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = true; break;
+        case PL_FLOOR: p.n = v3(0, 0, unit); p.origin = v3(0, 0, 0) * UU_TO_BT; p.axis = 2; p.backwards = false; break;
+        case PL_CEIL: p.n = v3(0, 0, -unit); p.origin = v3(0, 0, ARENA_HEIGHT) * UU_TO_BT; p.axis = 2; p.backwards = true; break;
+        case PL_WALL_NX: p.n = v3(unit, 0, 0); p.origin = v3(-ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = false; break;
+        case PL_WALL_PX: p.n = v3(-unit, 0, 0); p.origin = v3(ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = true; break;
     }
     return p;
 }
 
 // btStaticPlaneShape::getAabb with RocketSim's single-axis slab (+-0.2 m,
 // unbounded behind the plane).
-RSG_FN void plane_shape_aabb(int s, V3& mn, V3& mx) {
-    PlaneDef p = plane_def(s);
+RSG_FN void plane_shape_aabb(int plane, V3& mn, V3& mx) {
+    PlaneDef p = plane_def(plane);
     const float L = 1e18f;
     mn = v3(-L, -L, -L);
     mx = v3(L, L, L);
@@ -69,15 +87,15 @@ RSG_FN void mesh_shape_aabb(const MeshView& m, V3& mn, V3& mx) {
     mx = c + he;
 }
 
-RSG_FN void static_shape_aabb(const MeshView* m, int s, V3& mn, V3& mx) {
-    if (s == ST_MESH) mesh_shape_aabb(*m, mn, mx);
-    else plane_shape_aabb(s, mn, mx);
+RSG_FN void static_shape_aabb(const ArenaGeom* g, int s, V3& mn, V3& mx) {
+    if (static_is_mesh(*g, s)) mesh_shape_aabb(g->meshes[s], mn, mx);
+    else plane_shape_aabb(s - g->n_meshes, mn, mx);
 }
 
 // Broadphase proxy AABB of a static object: shape AABB +- the global contact
 // breaking threshold (btCollisionWorld::updateSingleAabb).
-RSG_FN void static_proxy_aabb(const MeshView* m, int s, V3& mn, V3& mx) {
-    static_shape_aabb(m, s, mn, mx);
+RSG_FN void static_proxy_aabb(const ArenaGeom* g, int s, V3& mn, V3& mx) {
+    static_shape_aabb(g, s, mn, mx);
     V3 t = v3(CONTACT_BREAKING_THRESHOLD, CONTACT_BREAKING_THRESHOLD, CONTACT_BREAKING_THRESHOLD);
     mn = mn - t;
     mx = mx + t;
@@ -420,8 +438,8 @@ RSG_FN void plane_space(V3 n, V3& p, V3& q) {
 
 // Ray vs static plane: the plane shape emits two triangles around the ray's
 // AABB (btStaticPlaneShape::processAllTriangles) which are ray-tested.
-RSG_FN bool ray_plane(int s, float unit, V3 from_w, V3 to_w, float& hit_fraction, V3& hit_normal) {
-    PlaneDef pd = plane_def(s, unit);
+RSG_FN bool ray_plane(int plane, float unit, V3 from_w, V3 to_w, float& hit_fraction, V3& hit_normal) {
+    PlaneDef pd = plane_def(plane, unit);
     // world -> plane local (identity basis)
     V3 from = from_w - pd.origin, to = to_w - pd.origin;
     V3 amin = v3(fminf(from.x, to.x), fminf(from.y, to.y), fminf(from.z, to.z));

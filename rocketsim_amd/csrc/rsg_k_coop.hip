@@ -23,7 +23,7 @@ using namespace rsg;
 //     spread over the lanes.
 // Per-arena LDS block: ArenaLocal prefix, ray results, two flag words.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_COOP_WAVES_PER_EU)))
-rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int arena_bytes, int lanes) {
+rsg_step_arena_coop_kernel(BatchDev d, ArenaGeom geom, int ticks, int stride, int arena_bytes, int lanes) {
     extern __shared__ __align__(16) unsigned char rsg_lds[];
     const int slot = (int)threadIdx.x / lanes;
     const int k = (int)threadIdx.x - slot * lanes;
@@ -40,7 +40,7 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
     P.arena_id = (uint32_t)a;
     P.synth_controls = d.synth_controls != 0;
     P.auto_reset = d.auto_reset != 0;
-    const MeshView* mp = mode_has_arena(d.mode) ? &mesh : nullptr;
+    const ArenaGeom* mp = mode_has_arena(d.mode) ? &geom : nullptr;
     StepSpace* ws = live ? arena_space(d, a) : nullptr;
     PairLog* log = live ? arena_log(d, a) : nullptr;
     int* concurrent = ahead + 1;
@@ -87,10 +87,10 @@ rsg_step_arena_coop_kernel(BatchDev d, MeshView mesh, int ticks, int stride, int
 
 namespace rsg {
 
-hipError_t launch_step_coop(const BatchDev& d, const MeshView& mv, int ticks, int grid, int per_block, int stride,
+hipError_t launch_step_coop(const BatchDev& d, const ArenaGeom& geom, int ticks, int grid, int per_block, int stride,
                             int arena_bytes, int lanes, hipStream_t s) {
     const size_t lds = (size_t)per_block * (size_t)stride;
-    hipLaunchKernelGGL(rsg_step_arena_coop_kernel, dim3(grid), dim3(per_block * lanes), lds, s, d, mv, ticks,
+    hipLaunchKernelGGL(rsg_step_arena_coop_kernel, dim3(grid), dim3(per_block * lanes), lds, s, d, geom, ticks,
                        stride, arena_bytes, lanes);
     return hipGetLastError();
 }

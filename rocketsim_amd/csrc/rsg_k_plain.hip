@@ -10,12 +10,12 @@ using namespace rsg;
 // One lane per arena; the arena's working copy lives in the lane's private
 // (scratch) memory.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
-rsg_step_arena_kernel(BatchDev d, MeshView mesh, int ticks) {
+rsg_step_arena_kernel(BatchDev d, ArenaGeom geom, int ticks) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= d.n) return;
     ArenaLocal A;
     StepSpace ws;  // the naive mapping: every lane's work space in its private memory
-    step_arena(d, mesh, a, ticks, A, &ws);
+    step_arena(d, geom, a, ticks, A, &ws);
 }
 
 __global__ void __launch_bounds__(64) rsg_reset_kernel(BatchDev d, int only_arena, uint64_t seed) {
@@ -30,9 +30,9 @@ __global__ void __launch_bounds__(64) rsg_reset_kernel(BatchDev d, int only_aren
 
 namespace rsg {
 
-hipError_t launch_step_plain(const BatchDev& d, const MeshView& mv, int ticks, int grid, int per_block,
+hipError_t launch_step_plain(const BatchDev& d, const ArenaGeom& geom, int ticks, int grid, int per_block,
                              hipStream_t s) {
-    hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(per_block), 0, s, d, mv, ticks);
+    hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(per_block), 0, s, d, geom, ticks);
     return hipGetLastError();
 }
 

@@ -12,20 +12,20 @@ using namespace rsg;
 // batches still reach every SIMD).  Tracker i's prediction arena (a
 // ball-only arena of `pa`) is held in LDS, `stride` bytes per tracker.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RSG_STEP_WAVES_PER_EU)))
-rsg_ball_pred_kernel(BatchDev pa, MeshView mesh, PredDev p, int kind, BatchDev src, int stride, int per_wave) {
+rsg_ball_pred_kernel(BatchDev pa, ArenaGeom geom, PredDev p, int kind, BatchDev src, int stride, int per_wave) {
     extern __shared__ __align__(16) unsigned char rsg_lds[];
     if ((int)threadIdx.x >= per_wave) return;
     const int i = blockIdx.x * per_wave + threadIdx.x;
     if (i >= pa.n) return;
     ArenaLocal& A = *reinterpret_cast<ArenaLocal*>(rsg_lds + (size_t)threadIdx.x * (size_t)stride);
-    ball_pred_update(pa, mode_has_arena(pa.mode) ? &mesh : nullptr, p, kind, src, i, A, arena_space(pa, i));
+    ball_pred_update(pa, mode_has_arena(pa.mode) ? &geom : nullptr, p, kind, src, i, A, arena_space(pa, i));
 }
 
 namespace rsg {
 
-hipError_t launch_ball_pred(const BatchDev& pa, const MeshView& mv, const PredDev& p, int kind, const BatchDev& src,
+hipError_t launch_ball_pred(const BatchDev& pa, const ArenaGeom& geom, const PredDev& p, int kind, const BatchDev& src,
                             int stride, int per_wave, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(rsg_ball_pred_kernel, dim3(grid), dim3(64), (size_t)per_wave * (size_t)stride, s, pa, mv, p,
+    hipLaunchKernelGGL(rsg_ball_pred_kernel, dim3(grid), dim3(64), (size_t)per_wave * (size_t)stride, s, pa, geom, p,
                        kind, src, stride, per_wave);
     return hipGetLastError();
 }

@@ -18,11 +18,11 @@ namespace rsg {
 
 // Step kernels (rsg_k_plain.hip, rsg_k_lds.hip, rsg_k_coop.hip): `grid`
 // workgroups of `per_block` arenas advance `ticks` ticks.
-hipError_t launch_step_plain(const BatchDev& d, const MeshView& mv, int ticks, int grid, int per_block,
+hipError_t launch_step_plain(const BatchDev& d, const ArenaGeom& geom, int ticks, int grid, int per_block,
                              hipStream_t s);
-hipError_t launch_step_lds(const BatchDev& d, const MeshView& mv, int ticks, int grid, int per_block, int stride,
+hipError_t launch_step_lds(const BatchDev& d, const ArenaGeom& geom, int ticks, int grid, int per_block, int stride,
                            hipStream_t s);
-hipError_t launch_step_coop(const BatchDev& d, const MeshView& mv, int ticks, int grid, int per_block, int stride,
+hipError_t launch_step_coop(const BatchDev& d, const ArenaGeom& geom, int ticks, int grid, int per_block, int stride,
                             int arena_bytes, int lanes, hipStream_t s);
 // Kickoff reset of arena `only_arena` (all arenas when < 0).
 hipError_t launch_reset(const BatchDev& d, int only_arena, uint64_t seed, hipStream_t s);
@@ -40,7 +40,7 @@ hipError_t launch_shot_backward(int n, int ticks, int K, float dt, const float* 
                                 const float* ckpt, const float* dloss, float* dctrl, float* ds0, hipStream_t s);
 
 // Ball prediction (rsg_k_pred.hip): `grid` workgroups of `per_wave` trackers.
-hipError_t launch_ball_pred(const BatchDev& pa, const MeshView& mv, const PredDev& p, int kind, const BatchDev& src,
+hipError_t launch_ball_pred(const BatchDev& pa, const ArenaGeom& geom, const PredDev& p, int kind, const BatchDev& src,
                             int stride, int per_wave, int grid, hipStream_t s);
 
 // GameEventTracker update of every arena (rsg_k_events.hip).

@@ -1,5 +1,5 @@
-// rsg_mesh.h -- the shared, read-only arena collision mesh: triangles, a
-// 16-bit quantized AABB tree over them, and the broadphase cell table.
+// rsg_mesh.h -- a shared, read-only arena collision mesh: triangles and a
+// 16-bit quantized AABB tree over them.
 //
 // One copy lives in HBM for the whole batch (every arena uses the same
 // mesh), so after the first ticks it is L2-resident.  The tree follows the
@@ -49,7 +49,6 @@ struct MeshView {
     const QNode* nodes;
     const QSubtree* subtrees;
     const TriInfo* tri_info; // may be null
-    const uint8_t* cell_statics; // BP_NUM_CELLS bytes: bit s = static object s registered in the cell
     int n_tris, n_nodes, n_subtrees;
     V3 bvh_min, bvh_max, quant;
     V3 aabb_min, aabb_max;   // local AABB of the mesh (no margin)
@@ -63,8 +62,6 @@ struct MeshView {
     int occ_nx, occ_ny, occ_nz;
     V3 occ_origin;
     float occ_inv_cell;
-    // Length of the static planes' stored unit normals (see plane_def).
-    float plane_unit;
 };
 
 constexpr float MESH_OCC_CELL = 2.f;     // Bullet units

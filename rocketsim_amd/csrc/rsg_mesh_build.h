@@ -1,6 +1,7 @@
-// rsg_mesh_build.h -- host-side construction of the shared arena mesh data
-// (MeshView in rsg_mesh.h): quantized AABB tree, broadphase cell table of the
-// static objects, and the internal-edge table.
+// rsg_mesh_build.h -- host-side construction of the shared arena geometry
+// (MeshView in rsg_mesh.h, ArenaGeom in rsg_contact.h): per mesh the
+// quantized AABB tree and the internal-edge table, per arena the broadphase
+// cell table of its static objects.
 //
 // The tree is built exactly as the reference builds its BVH
 // (btOptimizedBvh::build, BulletCollision/CollisionShapes/btOptimizedBvh.cpp:28-188;
@@ -27,12 +28,10 @@ struct HostMesh {
     std::vector<QNode> nodes;
     std::vector<QSubtree> subtrees;
     std::vector<TriInfo> tri_info;
-    std::vector<uint8_t> cell_statics;  // BP_NUM_CELLS: bit s = static s registered
     V3 bvh_min, bvh_max, quant;
     V3 aabb_min, aabb_max;
     std::vector<uint32_t> occ;          // MeshView::occ bitmask
     int occ_dims[3] = {0, 0, 0};
-    float plane_unit = 1.f;             // MeshView::plane_unit
 
     MeshView view() const {
         MeshView v{};
@@ -41,7 +40,6 @@ struct HostMesh {
         v.nodes = nodes.data();
         v.subtrees = subtrees.data();
         v.tri_info = tri_info.empty() ? nullptr : tri_info.data();
-        v.cell_statics = cell_statics.empty() ? nullptr : cell_statics.data();
         v.n_tris = (int)tris.size() / 3;
         v.n_nodes = (int)nodes.size();
         v.n_subtrees = (int)subtrees.size();
@@ -51,7 +49,6 @@ struct HostMesh {
         v.occ_nx = occ_dims[0]; v.occ_ny = occ_dims[1]; v.occ_nz = occ_dims[2];
         v.occ_origin = bvh_min;
         v.occ_inv_cell = 1.f / MESH_OCC_CELL;
-        v.plane_unit = plane_unit;
         return v;
     }
 };
@@ -378,23 +375,46 @@ inline void build_tri_info(HostMesh& M) {
 }
 
 // ---------------------------------------------------------------------------
+// An arena's static geometry on the host: its collision meshes in load order
+// and the broadphase cell masks of all its static objects (ArenaGeom).
+struct HostArena {
+    std::vector<HostMesh> meshes;
+    std::vector<MeshView> views;        // host views of `meshes` (refresh_views)
+    std::vector<uint32_t> cell_statics; // BP_NUM_CELLS masks
+    float plane_unit = 1.f;             // ArenaGeom::plane_unit
+
+    void refresh_views() {
+        views.clear();
+        for (const HostMesh& m : meshes) views.push_back(m.view());
+    }
+    ArenaGeom geom() const {
+        ArenaGeom g{};
+        g.meshes = views.data();
+        g.n_meshes = (int)views.size();
+        g.cell_statics = cell_statics.empty() ? nullptr : cell_statics.data();
+        g.plane_unit = plane_unit;
+        return g;
+    }
+};
+
 // Static objects per broadphase cell (_UpdateCellsStatic<true>,
 // BulletCollision/BroadphaseCollision/btRSBroadphase.cpp:96-185): for each
 // cell (i,j,k) covered by the static proxy's AABB (max clamped to the grid
-// max), the proxy is registered in the 27 cells around it -- for the mesh only
-// if some triangle's AABB overlaps the cell's box.  The reference reaches the
+// max), the proxy is registered in the 27 cells around it -- for a mesh only
+// if one of its triangles' AABB overlaps the cell's box.  The reference reaches the
 // mesh through a btTriangleMeshShape pointer there, and processAllTriangles is
 // not virtual in its Bullet copy (btTriangleMeshShape.h:54), so the test is
 // the brute-force filtered walk over every triangle
 // (btTriangleMeshShape.cpp:116-141), not the BVH query.
 
-inline void build_cell_statics(HostMesh& M) {
-    M.cell_statics.assign(BP_NUM_CELLS, 0);
-    MeshView mv = M.view();
+inline void build_cell_statics(HostArena& H) {
+    H.refresh_views();
+    H.cell_statics.assign(BP_NUM_CELLS, 0u);
+    const ArenaGeom g = H.geom();
     V3 gmax = bp_max_pos();
-    for (int s = 0; s < NUM_STATICS; s++) {
+    for (int s = 0; s < num_statics(g); s++) {
         V3 mn, mx;
-        static_proxy_aabb(&mv, s, mn, mx);
+        static_proxy_aabb(&g, s, mn, mx);
         mx = v3(fminf(mx.x, gmax.x), fminf(mx.y, gmax.y), fminf(mx.z, gmax.z));
         int i0, j0, k0, i1, j1, k1;
         bp_cell_indices(mn, i0, j0, k0);
@@ -402,7 +422,8 @@ inline void build_cell_statics(HostMesh& M) {
         for (int i = i0; i <= i1; i++)
             for (int j = j0; j <= j1; j++)
                 for (int k = k0; k <= k1; k++) {
-                    if (s == ST_MESH) {
+                    if (static_is_mesh(g, s)) {
+                        const MeshView& mv = g.meshes[s];
                         V3 cmin = bp_min_pos() + v3((float)i, (float)j, (float)k) * bp_cell_size();
                         V3 cmax = cmin + v3(bp_cell_size(), bp_cell_size(), bp_cell_size());
                         bool hit = false;
@@ -419,7 +440,7 @@ inline void build_cell_statics(HostMesh& M) {
                                 int ci = i + di, cj = j + dj, ck = k + dk;
                                 if (ci < 0 || cj < 0 || ck < 0) continue;
                                 if (ci >= BP_CELLS_X || cj >= BP_CELLS_Y || ck >= BP_CELLS_Z) continue;
-                                M.cell_statics[bp_cell_index(ci, cj, ck)] |= (uint8_t)(1u << s);
+                                H.cell_statics[bp_cell_index(ci, cj, ck)] |= 1u << s;
                             }
                 }
     }
@@ -460,16 +481,33 @@ inline void build_occupancy(HostMesh& M) {
     }
 }
 
-// Full preparation of the arena mesh from .cmf bytes.
+// Full preparation of one collision mesh from .cmf bytes (one entry of
+// RocketSim::InitFromMem's list, src/RocketSim.cpp:129-155: the mesh, its
+// BVH and its triangle info map).
 inline std::string build_host_mesh(const void* data, uint64_t size, HostMesh& M) {
     std::string err = parse_cmf(data, size, M);
     if (!err.empty()) return err;
     BvhBuilder b(M);
     b.build();
     build_tri_info(M);
-    build_cell_statics(M);
     build_occupancy(M);
-    M.plane_unit = host_sse_normalized(v3(0.f, 0.f, 1.f)).z;
+    return "";
+}
+
+// An arena from its n collision mesh files, in the order given (the
+// reference adds one static body per mesh in list order, then the planes,
+// src/Sim/Arena/Arena.cpp:1000-1068).  Returns "" or why the files were refused.
+inline std::string build_host_arena(const void* const* data, const uint64_t* sizes, int n, HostArena& H) {
+    if (n < 1 || n > MAX_MESHES)
+        return "between 1 and " + std::to_string(MAX_MESHES) + " mesh files are supported, got " + std::to_string(n);
+    H.meshes.assign((size_t)n, HostMesh());
+    for (int i = 0; i < n; i++) {
+        if (!data[i]) return "mesh file " + std::to_string(i) + " is null";
+        std::string err = build_host_mesh(data[i], sizes[i], H.meshes[(size_t)i]);
+        if (!err.empty()) return "mesh file " + std::to_string(i) + ": " + err;
+    }
+    build_cell_statics(H);
+    H.plane_unit = host_sse_normalized(v3(0.f, 0.f, 1.f)).z;
     return "";
 }
 

@@ -28,7 +28,7 @@ RSG_FN PairLog* arena_log(const BatchDev& d, int a) {
     return &reinterpret_cast<ArenaSpace*>(d.space)[a].log;
 }
 
-RSG_FN void step_arena(const BatchDev& d, const MeshView& mesh, int a, int ticks, ArenaLocal& A, StepSpace* ws) {
+RSG_FN void step_arena(const BatchDev& d, const ArenaGeom& geom, int a, int ticks, ArenaLocal& A, StepSpace* ws) {
     RSG_PROF_MARK(15);
     load_arena(d, a, A);
     RSG_PROF_MARK(15);
@@ -39,7 +39,7 @@ RSG_FN void step_arena(const BatchDev& d, const MeshView& mesh, int a, int ticks
     P.arena_id = (uint32_t)a;
     P.synth_controls = d.synth_controls != 0;
     P.auto_reset = d.auto_reset != 0;
-    const MeshView* mp = mode_has_arena(d.mode) ? &mesh : nullptr;
+    const ArenaGeom* mp = mode_has_arena(d.mode) ? &geom : nullptr;
     for (int t = 0; t < ticks; t++) arena_tick(A, P, mp, ws);
     store_arena(d, a, A);
 }

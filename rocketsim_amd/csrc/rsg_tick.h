@@ -431,8 +431,8 @@ RSG_FN bool tick_begin(ArenaLocal& A, const TickParams& P) {
 // Wheel ray `wheel` of car `ci` as the pre-tick loop will cast it, computed
 // ahead of the loop (valid when tick_begin returned true): poses, broadphase
 // cells and contact-response flags are those the car sees at its turn.
-RSG_FN void tick_cast_ray(ArenaLocal& A, const TickParams& P, const MeshView* mesh, int ci, int wheel, RayHit& out) {
-    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
+RSG_FN void tick_cast_ray(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, int ci, int wheel, RayHit& out) {
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr);
     int pos = 0;
     while (pos < A.n_cars && A.order[pos] != ci) pos++;
     world.ray_caster_pos = pos;
@@ -704,9 +704,9 @@ RSG_FN void ball_pre_tick_heatseeker(Ball& ball, float dt) {
 
 // The world step of the tick (btDiscreteDynamicsWorld::stepSimulation),
 // preceded by the ball's pre-tick, which runs after the pads' (Arena.cpp:715).
-RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws) {
+RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws) {
     if (P.mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(A.ball, P.dt);
-    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     world.step(P.dt);
     RSG_PROF_MARK(7);
 }
@@ -716,28 +716,28 @@ RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const MeshView* mesh,
 // (`lane` of `lanes`, effects recorded in `log`), with the arena unchanged
 // in between.  A pair whose contact needs the penetration pass is run again
 // by the last part, on the arena's own work space.
-RSG_FN void tick_world_begin(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws) {
+RSG_FN void tick_world_begin(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws) {
     if (P.mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(A.ball, P.dt);
-    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     world.step_begin(P.dt);
 }
-RSG_FN void tick_world_pairs(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws, PairLog* log,
+RSG_FN void tick_world_pairs(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws, PairLog* log,
                              int lane, int lanes) {
-    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     world.run_pairs_recorded(lane, lanes, nullptr, log);
 }
-RSG_FN void tick_world_end(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws,
+RSG_FN void tick_world_end(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws,
                            const PairLog* log) {
-    World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr, ws);
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     world.step_end(P.dt, log);
     RSG_PROF_MARK(7);
 }
 
 // The rest of Arena::Step after tick_begin, serially.  `pre` holds every
 // wheel's ray result (car-major, 4 per car) when cast ahead, else null.
-RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, const RayHit* pre, StepSpace* ws) {
+RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, const RayHit* pre, StepSpace* ws) {
     {
-        World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
+        World world(A, geo, mode_has_arena(P.mode) && geo != nullptr);
         RSG_PROF_MARK(0);
         // 2. car pre-tick, in the arena's car update order
         for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], pre);
@@ -746,7 +746,7 @@ RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, 
     // 3. pads pre-tick
     pads_pre_tick(A, P);
     // 4. world step
-    tick_world(A, P, mesh, ws);
+    tick_world(A, P, geo, ws);
     // 5. cars post-tick + pad collisions
     for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
     if (mode_has_arena(P.mode))
@@ -755,9 +755,9 @@ RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const MeshView* mesh, 
 }
 
 // Arena::Step (src/Sim/Arena/Arena.cpp:689-756), one tick.
-RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws) {
+RSG_FN void arena_tick(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws) {
     tick_begin(A, P);
-    tick_rest(A, P, mesh, nullptr, ws);
+    tick_rest(A, P, geo, nullptr, ws);
 }
 
 }  // namespace rsg

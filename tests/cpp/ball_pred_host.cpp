@@ -35,7 +35,7 @@ using namespace rsg;
 namespace {
 
 struct HostPred {
-    HostMesh mesh;
+    HostArena geo;
     bool soccar = false;
     int n = 0, T = 0;
     std::vector<uint32_t> car, ball, pad, arena;  // the prediction arenas
@@ -70,7 +70,8 @@ extern "C" {
 void* bp_create(const void* cmf, uint64_t cmf_size, int n, int T, float tick_rate) {
     auto* h = new HostPred();
     if (cmf) {
-        if (!build_host_mesh(cmf, cmf_size, h->mesh).empty()) { delete h; return nullptr; }
+        const void* files[1] = {cmf};
+        if (!build_host_arena(files, &cmf_size, 1, h->geo).empty()) { delete h; return nullptr; }
         h->soccar = true;
     }
     h->n = n;
@@ -128,7 +129,7 @@ int bp_update(void* p, int kind, const rsg_ball_state* cur, const int32_t* ticks
         if (kind == PRED_MANUAL)
             for (int i = 0; i < h.n; i++) h.ticks[i] = ticks[i];
     }
-    MeshView mv = h.mesh.view();
+    const ArenaGeom mv = h.geo.geom();
     auto* A = new ArenaLocal;
     auto* ws = new StepSpace;
     for (int i = 0; i < h.n; i++) ball_pred_update(h.dev, h.soccar ? &mv : nullptr, d, kind, src, i, *A, ws);

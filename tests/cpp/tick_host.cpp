@@ -50,7 +50,7 @@ using namespace rsg;
 namespace {
 
 struct HostBatch {
-    HostMesh mesh;
+    HostArena geo;
     std::vector<int> cars_per_arena, car_offset;
     std::vector<uint32_t> car, ball, pad, arena;
     std::vector<rsg_mutator_config> mut;
@@ -64,14 +64,14 @@ struct HostBatch {
 extern "C" {
 
 // A host batch of n arenas (cars_per_arena[a] cars each, teams and presets
-// per car, or each car's CarConfig in car_configs); cmf = the arena mesh file
-// (NULL for THE_VOID); mut = the MutatorConfig set before the cars are added
-// (NULL: the mode's defaults).
+// per car, or each car's CarConfig in car_configs); cmfs = the arena's n_cmf
+// mesh files in order (none for THE_VOID); mut = the MutatorConfig set
+// before the cars are added (NULL: the mode's defaults).
 void* th_create(int mode, int n, const int* cars_per_arena, const uint8_t* team, const uint8_t* preset,
-                const void* cmf, uint64_t cmf_size, float tick_rate, const rsg_mutator_config* mut,
-                const rsg_car_config* car_configs) {
+                const void* const* cmfs, const uint64_t* cmf_sizes, int n_cmf, float tick_rate,
+                const rsg_mutator_config* mut, const rsg_car_config* car_configs) {
     auto* h = new HostBatch();
-    if (cmf && !build_host_mesh(cmf, cmf_size, h->mesh).empty()) {
+    if (n_cmf > 0 && !build_host_arena(cmfs, cmf_sizes, n_cmf, h->geo).empty()) {
         delete h;
         return nullptr;
     }
@@ -191,17 +191,17 @@ void th_get_status(void* p, uint32_t* out) {
 #ifdef RSG_TEST_SPLIT_WORLD
 // One tick in the order of rsg_step_arena_coop_kernel's phases, with the
 // world step in its three parts.
-static void tick_split(ArenaLocal& A, const TickParams& P, const MeshView* mesh, StepSpace* ws, PairLog* log) {
+static void tick_split(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws, PairLog* log) {
     tick_begin(A, P);
     {
-        World world(A, mesh, mode_has_arena(P.mode) && mesh != nullptr);
+        World world(A, geo, mode_has_arena(P.mode) && geo != nullptr);
         for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], nullptr);
     }
     pads_pre_tick(A, P);
-    tick_world_begin(A, P, mesh, ws);
+    tick_world_begin(A, P, geo, ws);
     for (int lane = RSG_TEST_SPLIT_WORLD - 1; lane >= 0; lane--)
-        tick_world_pairs(A, P, mesh, ws, log, lane, RSG_TEST_SPLIT_WORLD);
-    tick_world_end(A, P, mesh, ws, log);
+        tick_world_pairs(A, P, geo, ws, log, lane, RSG_TEST_SPLIT_WORLD);
+    tick_world_end(A, P, geo, ws, log);
     for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
     if (mode_has_arena(P.mode))
         for (int p = 0; p < PAD_NUM; p++) pad_check_cars(A, p);
@@ -213,7 +213,7 @@ static void tick_split(ArenaLocal& A, const TickParams& P, const MeshView* mesh,
 // split world step, RSG_TEST_SPLIT_WORLD).
 void th_step(void* p, int ticks) {
     HostBatch& h = *(HostBatch*)p;
-    MeshView mv = h.mesh.view();
+    const ArenaGeom mv = h.geo.geom();
     auto A = std::make_unique<ArenaLocal>();
     auto ws = std::make_unique<StepSpace>();
 #ifdef RSG_TEST_SPLIT_WORLD
@@ -227,7 +227,7 @@ void th_step(void* p, int ticks) {
         P.arena_id = (uint32_t)a;
         P.synth_controls = h.dev.synth_controls != 0;
         P.auto_reset = h.dev.auto_reset != 0;
-        const MeshView* mp = mode_has_arena(h.dev.mode) ? &mv : nullptr;
+        const ArenaGeom* mp = mode_has_arena(h.dev.mode) ? &mv : nullptr;
         for (int t = 0; t < ticks; t++) tick_split(*A, P, mp, ws.get(), log.get());
         store_arena(h.dev, a, *A);
     }

@@ -27,7 +27,16 @@ def heatseeker():
     parity.save_samples(os.path.join(HERE, "heatseeker_tf.npz"), hs)
 
 
+def split():
+    """The six-mesh arena's samples, drawn in a process of their own (the
+    oracle loads its meshes once per process)."""
+    import multi_mesh_case
+    s = multi_mesh_case.generate(os.path.join(HERE, "soccar_split_tf.npz"), seed=20261208, n_traj=6)
+    print("soccar_split_tf.npz:", len(s), "samples")
+
+
 def main():
+    split()
     ref.init()
     void = parity.sample_void(ref, n_traj=24, ticks_per_traj=8, seed=20261202)
     parity.save_samples(os.path.join(HERE, "void_tf.npz"), void)
@@ -54,4 +63,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["split"]:
+        split()
+    else:
+        main()

@@ -617,13 +617,17 @@ def load_samples(path):
     return out
 
 
-def run_on_gpu(samples, mode, kernel=0, per_block=0, auto_reset=False, seed=0, mutators=None, mutators_first=False):
+def run_on_gpu(samples, mode, kernel=0, per_block=0, auto_reset=False, seed=0, mutators=None, mutators_first=False,
+               cmfs=None):
     """Step every sample once on the GPU, one sample per arena.  `kernel` and
     `per_block` pick the step kernel and arenas per workgroup (0: the
     library's automatic choice); `auto_reset` / `seed` set the batch's kickoff
     reset on goal; `mutators` is every arena's MutatorConfig, given at
-    creation (`mutators_first`: before the cars are added) or set after."""
-    from rocketsim_amd.batch import ArenaBatch
+    creation (`mutators_first`: before the cars are added) or set after;
+    `cmfs` the arena's mesh files (None: the one-mesh synthetic arena)."""
+    from rocketsim_amd.batch import ArenaBatch, ensure_meshes
+    if mode != MODE_THE_VOID:
+        ensure_meshes(cmfs)
     cpa = [len(s.teams) for s in samples]
     car_configs = None
     if any(getattr(s, "configs", None) for s in samples):

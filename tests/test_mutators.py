@@ -102,9 +102,9 @@ def _host_batch(L, ref_arena, mode, teams, presets):
     """A one-arena host batch in the oracle arena's current state."""
     from rocketsim_amd import arena_mesh
     n = len(teams)
-    cmf = arena_mesh.soccar_cmf() if mode != MODE_THE_VOID else None
-    h = L.th_create(mode, 1, (C.c_int * 1)(n), (C.c_uint8 * n)(*teams), (C.c_uint8 * n)(*presets), cmf,
-                    len(cmf) if cmf else 0, 120.0, None, None)
+    cmfs = [arena_mesh.soccar_cmf()] if mode != MODE_THE_VOID else []
+    h = L.th_create(mode, 1, (C.c_int * 1)(n), (C.c_uint8 * n)(*teams), (C.c_uint8 * n)(*presets),
+                    *tick_host.mesh_args(cmfs), 120.0, None, None)
     assert h
     L.th_set_cars(h, (CarState * n)(*[ref_arena.get_car(i) for i in range(n)]))
     L.th_set_balls(h, (BallState * 1)(ref_arena.get_ball()))

@@ -49,7 +49,8 @@ def load(path):
     L = C.CDLL(path)
     L.th_create.restype = C.c_void_p
     L.th_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
-                            C.c_char_p, C.c_uint64, C.c_float, C.POINTER(MutatorConfig), C.POINTER(CarConfig)]
+                            C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_int, C.c_float,
+                            C.POINTER(MutatorConfig), C.POINTER(CarConfig)]
     L.th_destroy.argtypes = [C.c_void_p]
     for name, t in (("cars", CarState), ("balls", BallState), ("pads", PadState)):
         getattr(L, "th_set_" + name).argtypes = [C.c_void_p, C.POINTER(t)]
@@ -99,22 +100,37 @@ def preset_config(p):
     return c
 
 
-def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators_first=False):
+def mesh_args(cmfs):
+    """th_create's (files, sizes, count) arguments for a list of .cmf file
+    bytes (the buffers stay alive with the returned tuple's first item)."""
+    cmfs = [bytes(c) for c in (cmfs or [])]
+    bufs = [C.create_string_buffer(c, len(c)) for c in cmfs]
+    n = len(cmfs)
+    ptrs = (C.c_void_p * max(n, 1))(*[C.cast(b, C.c_void_p) for b in bufs])
+    ptrs._keep = bufs
+    return ptrs, (C.c_uint64 * max(n, 1))(*[len(c) for c in cmfs]), n
+
+
+def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators_first=False, cmfs=None):
     """Step every teacher-forced sample once on the host build, one sample per
     arena (the host counterpart of parity.run_on_gpu).  `inject(cars, balls)`
     may edit the "before" records; `mutators` is every arena's MutatorConfig,
     set before its cars are added (`mutators_first`) or after, as
     Arena::SetMutatorConfig.  Returns (results, status words)."""
-    if mode in (MODE_SOCCAR, MODE_HEATSEEKER) and cmf is None:
+    if cmfs is None and cmf is not None:
+        cmfs = [cmf]
+    if mode in (MODE_SOCCAR, MODE_HEATSEEKER) and cmfs is None:
         from rocketsim_amd import arena_mesh
-        cmf = arena_mesh.soccar_cmf()
+        cmfs = [arena_mesh.soccar_cmf()]
+    if mode not in (MODE_SOCCAR, MODE_HEATSEEKER):
+        cmfs = []
     n = len(samples)
     cpa = [len(s.teams) for s in samples]
     teams = sum((list(s.teams) for s in samples), [])
     presets = sum((list(s.presets) for s in samples), [])
     total = sum(cpa)
     h = L.th_create(mode, n, (C.c_int * n)(*cpa), (C.c_uint8 * max(total, 1))(*teams),
-                    (C.c_uint8 * max(total, 1))(*presets), cmf, len(cmf) if cmf else 0, 120.0,
+                    (C.c_uint8 * max(total, 1))(*presets), *mesh_args(cmfs), 120.0,
                     C.byref(mutators) if mutators is not None and mutators_first else None,
                     _car_configs(samples))
     if not h:
