@@ -55,8 +55,8 @@ const char* rsg_last_error(void);
 /* Replaces RocketSim::InitFromMem (src/RocketSim.h:32, src/RocketSim.cpp:92-185)
  * for the Soccar arena (Soccar and Heatseeker batches): n collision mesh
  * files in the reference's .cmf format
- * (src/CollisionMeshFile/CollisionMeshFile.cpp:11-61), 1 <= n <= 28, in the
- * order the reference's list holds them.  Each mesh becomes one static body
+ * (src/CollisionMeshFile/CollisionMeshFile.cpp:11-61), in the
+ * order the reference's list holds them (1 <= n <= 26).  Each mesh becomes one static body
  * of every arena, in this order and before the floor, ceiling and side walls
  * (Arena::_SetupArenaCollisionShapes, src/Sim/Arena/Arena.cpp:990-1068).
  * Unlike the reference, a later call replaces the meshes for batches stepped
@@ -65,6 +65,12 @@ const char* rsg_last_error(void);
 int rsg_init_meshes(const void* const* cmf_data, const uint64_t* sizes, int n);
 /* rsg_init_meshes with one mesh file. */
 int rsg_init_mesh(const void* cmf_data, uint64_t size);
+/* rsg_init_meshes for the arena of `mode`: RSG_MODE_SOCCAR or
+ * RSG_MODE_HEATSEEKER (the Soccar arena: 4 planes) or RSG_MODE_HOOPS (the
+ * Hoops arena: 6 planes, src/Sim/Arena/Arena.cpp:1036-1068; a mesh of 505
+ * vertices is a net, which cars and their wheel rays pass through,
+ * Arena.cpp:1005-1024).  1 <= n <= 26. */
+int rsg_init_arena_meshes(int mode, const void* const* cmf_data, const uint64_t* sizes, int n);
 
 /* Replaces Arena::Create + AddCar (src/Sim/Arena/Arena.h:71,111).  Cars start
  * at random kickoff positions (Arena::ResetToRandomKickoff). */

@@ -35,7 +35,7 @@ extern "C" {
 /* Game mode ids, same order as src/Sim/GameMode.h:7-17. SOCCAR, HEATSEEKER
  * (the Soccar arena with the seeking ball) and THE_VOID are supported by the
  * batched step. */
-enum { RSG_MODE_SOCCAR = 0, RSG_MODE_HEATSEEKER = 2, RSG_MODE_THE_VOID = 5 };
+enum { RSG_MODE_SOCCAR = 0, RSG_MODE_HOOPS = 1, RSG_MODE_HEATSEEKER = 2, RSG_MODE_THE_VOID = 5 };
 
 /* Car config presets, same order as src/Sim/Car/CarConfig/CarConfig.cpp:20-100
  * (CAR_CONFIG_OCTANE ... CAR_CONFIG_PSYCLOPS).  RSG_PRESET_CUSTOM marks a car

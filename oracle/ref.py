@@ -29,6 +29,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.rsref_last_error.restype = C.c_char_p
         L.rsref_init.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_int]
+        L.rsref_init_modes.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_int,
+                                       C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_int]
         L.rsref_arena_create.restype = C.c_void_p
         L.rsref_arena_create.argtypes = [C.c_int, C.c_float]
         L.rsref_arena_destroy.argtypes = [C.c_void_p]
@@ -101,22 +103,29 @@ def car_config_preset(preset):
     return c
 
 
-def init(meshes=None):
-    """RocketSim::InitFromMem with the given .cmf blobs (bytes) for SOCCAR,
-    by default the procedural Soccar arena of rocketsim_amd.arena_mesh.  The
-    reference initialises once per process; later calls are no-ops."""
-    global _initialised
-    if _initialised:
-        return
-    if meshes is None:
-        from rocketsim_amd import arena_mesh
-        meshes = [arena_mesh.soccar_cmf()]
-    L = lib()
+def _blob_args(meshes):
     n = len(meshes)
     bufs = [C.create_string_buffer(m, len(m)) for m in meshes]
     ptrs = (C.c_void_p * max(n, 1))(*[C.cast(b, C.c_void_p) for b in bufs])
-    sizes = (C.c_uint64 * max(n, 1))(*[len(m) for m in meshes])
-    if L.rsref_init(ptrs, sizes, n) != 0:
+    ptrs._keep = bufs
+    return ptrs, (C.c_uint64 * max(n, 1))(*[len(m) for m in meshes]), n
+
+
+def init(meshes=None, hoops_meshes=None):
+    """RocketSim::InitFromMem with the given .cmf blobs (bytes) for SOCCAR
+    and for HOOPS, by default the procedural arenas of
+    rocketsim_amd.arena_mesh.  The reference initialises once per process;
+    later calls are no-ops."""
+    global _initialised
+    if _initialised:
+        return
+    from rocketsim_amd import arena_mesh
+    if meshes is None:
+        meshes = [arena_mesh.soccar_cmf()]
+    if hoops_meshes is None:
+        hoops_meshes = arena_mesh.hoops_cmfs()
+    L = lib()
+    if L.rsref_init_modes(*_blob_args(meshes), *_blob_args(hoops_meshes)) != 0:
         raise RuntimeError(L.rsref_last_error().decode())
     _initialised = True
 

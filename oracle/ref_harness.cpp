@@ -105,8 +105,10 @@ extern "C" {
 
 const char* rsref_last_error() { return g_err.c_str(); }
 
-// blobs: n .cmf files for SOCCAR (may be 0 for a THE_VOID-only process).
-int rsref_init(const uint8_t* const* blobs, const uint64_t* sizes, int n) {
+// blobs: n .cmf files for SOCCAR (may be 0 for a THE_VOID-only process);
+// hoops: n_hoops .cmf files for HOOPS.
+int rsref_init_modes(const uint8_t* const* blobs, const uint64_t* sizes, int n, const uint8_t* const* hoops,
+                     const uint64_t* hoops_sizes, int n_hoops) {
     try {
         std::map<GameMode, std::vector<FileData>> m;
         for (int i = 0; i < n; i++) {
@@ -114,12 +116,17 @@ int rsref_init(const uint8_t* const* blobs, const uint64_t* sizes, int n) {
             m[GameMode::SOCCAR].push_back(FileData(blobs[i], blobs[i] + sizes[i]));
             m[GameMode::HEATSEEKER].push_back(FileData(blobs[i], blobs[i] + sizes[i]));
         }
+        for (int i = 0; i < n_hoops; i++) m[GameMode::HOOPS].push_back(FileData(hoops[i], hoops[i] + hoops_sizes[i]));
         RocketSim::InitFromMem(m, true);
         return 0;
     } catch (std::exception& e) {
         g_err = e.what();
         return -1;
     }
+}
+
+int rsref_init(const uint8_t* const* blobs, const uint64_t* sizes, int n) {
+    return rsref_init_modes(blobs, sizes, n, nullptr, nullptr, 0);
 }
 
 void* rsref_arena_create(int mode, float tick_rate) {

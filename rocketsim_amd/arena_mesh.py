@@ -257,6 +257,99 @@ def soccar_split_cmfs():
     return list(_SPLIT_CACHE)
 
 
+# -- the synthetic Hoops arena -------------------------------------------------
+# The Hoops arena's planes (floor, ceiling, side and end walls at
+# x = +-8900/3, y = +-3581, z = 1820 uu; src/RLConst.h:18-20) are added by the
+# arena itself.  Its meshes here: 45 degree bevels in the four vertical
+# corners, and per side a hoop -- a rim band and, below it, a funnel net of
+# exactly 505 vertices (the count by which the reference tells a net mesh,
+# src/Sim/Arena/Arena.cpp:1008-1018).  A ball dropped through a rim comes to
+# rest in the net low inside the hoop's scoring footprint
+# (Arena.cpp:759-768,902-915).
+
+HOOPS_EXTENT_X = 8900.0 / 3
+HOOPS_EXTENT_Y = 3581.0
+HOOPS_HEIGHT = 1820.0
+HOOPS_BEVEL = 600.0
+HOOP_Y = 2900.0          # hoop centre |y|
+HOOP_R = 600.0           # rim radius
+RIM_Z0, RIM_Z1 = 330.0, 380.0
+NET_BOTTOM_Z = 30.0
+NET_BOTTOM_SCALE = 0.4
+HOOP_SEGS = 24
+NET_RINGS = 21           # 21 rings of 24 + the bottom centre = 505 vertices
+
+_HOOPS_CACHE = None
+
+
+def _facing(V, tris, inside):
+    """Orient every triangle so its normal points towards inside(centroid)."""
+    out = []
+    for t in tris:
+        a, b, c = (np.asarray(V[i], dtype=np.float64) for i in t)
+        n = np.cross(b - a, c - a)
+        if np.dot(n, inside((a + b + c) / 3) - (a + b + c) / 3) < 0:
+            t = (t[0], t[2], t[1])
+        out.append(t)
+    return out
+
+
+def _hoop(sign):
+    """(rim (verts, tris), net (verts, tris)) of the hoop at y = sign * HOOP_Y."""
+    cy = sign * HOOP_Y
+    ang = [2 * math.pi * k / HOOP_SEGS for k in range(HOOP_SEGS)]
+    rim_v = [(HOOP_R * math.cos(a), cy + HOOP_R * math.sin(a), z) for z in (RIM_Z0, RIM_Z1) for a in ang]
+    rim_t = []
+    for k in range(HOOP_SEGS):
+        k2 = (k + 1) % HOOP_SEGS
+        rim_t += [(k, k2, HOOP_SEGS + k2), (k, HOOP_SEGS + k2, HOOP_SEGS + k)]
+    axis = lambda p: np.array([0.0, cy, p[2]])
+    rim_t = _facing(rim_v, rim_t, axis)
+    net_v = []
+    for i in range(NET_RINGS):
+        f = i / (NET_RINGS - 1)
+        z = RIM_Z0 + (NET_BOTTOM_Z - RIM_Z0) * f
+        r = HOOP_R * (1 + (NET_BOTTOM_SCALE - 1) * f)
+        net_v += [(r * math.cos(a), cy + r * math.sin(a), z) for a in ang]
+    net_v.append((0.0, cy, NET_BOTTOM_Z))
+    net_t = []
+    for i in range(NET_RINGS - 1):
+        for k in range(HOOP_SEGS):
+            k2 = (k + 1) % HOOP_SEGS
+            a, b = i * HOOP_SEGS + k, i * HOOP_SEGS + k2
+            c, d = a + HOOP_SEGS, b + HOOP_SEGS
+            net_t += [(a, b, d), (a, d, c)]
+    last, centre = (NET_RINGS - 1) * HOOP_SEGS, len(net_v) - 1
+    net_t += [(last + k, last + (k + 1) % HOOP_SEGS, centre) for k in range(HOOP_SEGS)]
+    net_t = _facing(net_v, net_t, lambda p: np.array([0.0, cy, p[2] + 200.0]))
+    assert len(net_v) == 505
+    return (rim_v, rim_t), (net_v, net_t)
+
+
+def _hoops_shell():
+    """The four corner bevels, facing the arena's centre."""
+    V, T = [], []
+    for sx in (-1, 1):
+        for sy in (-1, 1):
+            p = (sx * HOOPS_EXTENT_X, sy * (HOOPS_EXTENT_Y - HOOPS_BEVEL))
+            q = (sx * (HOOPS_EXTENT_X - HOOPS_BEVEL), sy * HOOPS_EXTENT_Y)
+            base = len(V)
+            V += [(p[0], p[1], 0.0), (q[0], q[1], 0.0), (q[0], q[1], HOOPS_HEIGHT), (p[0], p[1], HOOPS_HEIGHT)]
+            T += [(base, base + 1, base + 2), (base, base + 2, base + 3)]
+    return V, _facing(V, T, lambda p: np.array([0.0, 0.0, p[2]]))
+
+
+def hoops_cmfs():
+    """The synthetic Hoops arena as five .cmf files: the corner bevels, the
+    two rims, then the two nets (deterministic, cached)."""
+    global _HOOPS_CACHE
+    if _HOOPS_CACHE is None:
+        (rim_b, net_b), (rim_o, net_o) = _hoop(-1), _hoop(1)
+        parts = [_hoops_shell(), rim_b, rim_o, net_b, net_o]
+        _HOOPS_CACHE = [to_cmf(np.array(v, dtype=np.float32), np.array(t, dtype=np.int32)) for v, t in parts]
+    return list(_HOOPS_CACHE)
+
+
 def soccar_cmf():
     """The synthetic Soccar arena as .cmf bytes (deterministic, cached)."""
     global _CACHE

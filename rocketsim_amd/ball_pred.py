@@ -25,10 +25,8 @@ class BallPredTracker:
             mode, n, tick_rate = arena.mode, arena.n_arenas, arena.tick_rate
         if mode is None or n is None:
             raise ValueError("BallPredTracker needs an ArenaBatch or mode and n")
-        if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
-            from . import batch
-            if not batch._mesh_loaded:
-                batch.init_mesh()
+        from . import batch
+        batch.ensure_arena(mode)
         L = native.lib()
         h = L.rsg_ball_pred_create(mode, n, num_pred_ticks, tick_rate)
         if not h:

@@ -66,10 +66,10 @@ struct DeviceMesh {
     TriInfo* tri_info = nullptr;
     uint32_t* occ = nullptr;
 };
-// The Soccar arena's static geometry, shared read-only by every batch of the
-// process (RocketSim keeps its arena meshes in process-wide storage the same
-// way, src/RocketSim.cpp:50-53,92-185).  Device arrays are allocated once per
-// init.
+// An arena's static geometry (the Soccar arena, the Hoops arena), shared
+// read-only by every batch of the process (RocketSim keeps its arena meshes
+// in process-wide storage the same way, src/RocketSim.cpp:50-53,92-185).
+// Device arrays are allocated once per init.
 struct DeviceArena {
     bool ready = false;
     HostArena host;
@@ -78,7 +78,7 @@ struct DeviceArena {
     uint32_t* cell_statics = nullptr;
     ArenaGeom geom{};
 };
-DeviceArena g_mesh;
+DeviceArena g_arenas[2];  // by arena_kind(mode)
 std::mutex g_mesh_mutex;
 
 template <class T>
@@ -195,8 +195,9 @@ const char* rsg_last_error(void) { return g_err.c_str(); }
 
 rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     if (!desc || desc->n_arenas <= 0) { fail("rsg_batch_create: n_arenas must be > 0"); return nullptr; }
-    if (desc->mode != RSG_MODE_SOCCAR && desc->mode != RSG_MODE_HEATSEEKER && desc->mode != RSG_MODE_THE_VOID) {
-        fail("rsg_batch_create: only SOCCAR, HEATSEEKER and THE_VOID are supported");
+    if (desc->mode != RSG_MODE_SOCCAR && desc->mode != RSG_MODE_HOOPS && desc->mode != RSG_MODE_HEATSEEKER &&
+        desc->mode != RSG_MODE_THE_VOID) {
+        fail("rsg_batch_create: only SOCCAR, HOOPS, HEATSEEKER and THE_VOID are supported");
         return nullptr;
     }
     if (!(desc->tick_rate >= 15.f && desc->tick_rate <= 120.f)) {
@@ -211,7 +212,7 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
             return nullptr;
         }
     }
-    if (mode_has_arena(desc->mode) && !g_mesh.ready) {
+    if (mode_has_arena(desc->mode) && !g_arenas[arena_kind(desc->mode)].ready) {
         fail("rsg_batch_create: no arena collision mesh loaded (call rsg_init_mesh first)");
         return nullptr;
     }
@@ -453,8 +454,9 @@ int launch_step(const rsg_batch* b, const BatchDev& dev, const Launch& L, int ti
     int grid = (dev.n + block - 1) / block;
     ArenaGeom mv{};
     if (mode_has_arena(dev.mode)) {
-        if (!g_mesh.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
-        mv = g_mesh.geom;
+        const DeviceArena& ga = g_arenas[arena_kind(dev.mode)];
+        if (!ga.ready) return fail("rsg_batch_step: no arena mesh (call rsg_init_mesh first)");
+        mv = ga.geom;
     }
     if (L.kernel == RSG_KERNEL_COOP)
         HIP_OK(launch_step_coop(dev, mv, ticks, grid, block, coop_stride(b), lds_stride(b), coop_lanes(b), s));
@@ -977,7 +979,7 @@ int rsg_car_config_preset(int preset, rsg_car_config* out) {
 
 int rsg_mutator_config_default(int mode, rsg_mutator_config* out) {
     if (!out) return fail("rsg_mutator_config_default: null argument");
-    if (mode != RSG_MODE_SOCCAR && mode != RSG_MODE_HEATSEEKER && mode != RSG_MODE_THE_VOID)
+    if (mode != RSG_MODE_SOCCAR && mode != RSG_MODE_HOOPS && mode != RSG_MODE_HEATSEEKER && mode != RSG_MODE_THE_VOID)
         return fail("rsg_mutator_config_default: unsupported game mode");
     *out = mutator_defaults(mode);
     return 0;
@@ -1018,9 +1020,7 @@ int rsg_batch_is_ball_scored(rsg_batch* b, uint8_t* out) {
     if (download(b) != 0) return -1;
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++) {
-        float y = ball_ref(h, a).v(BW_POS).y * BT_TO_UU;
-        const rsg_mutator_config& m = b->h_mut[(size_t)a];
-        out[a] = (mode_has_arena(b->dev.mode) && fabsf(y) > (m.goal_base_threshold_y + m.ball_radius)) ? 1 : 0;
+        out[a] = ball_scored(b->dev.mode, ball_ref(h, a).v(BW_POS), b->h_mut[(size_t)a]) ? 1 : 0;
     }
     return 0;
 }
@@ -1341,8 +1341,9 @@ int pred_launch(rsg_ball_pred* p, int kind, const BatchDev* src, hipStream_t s) 
     if (upload(a) != 0) return -1;
     ArenaGeom mv{};
     if (mode_has_arena(a->dev.mode)) {
-        if (!g_mesh.ready) return fail("rsg_ball_pred: no arena mesh (call rsg_init_mesh first)");
-        mv = g_mesh.geom;
+        const DeviceArena& ga = g_arenas[arena_kind(a->dev.mode)];
+        if (!ga.ready) return fail("rsg_ball_pred: no arena mesh (call rsg_init_mesh first)");
+        mv = ga.geom;
     }
     PredDev d;
     d.traj = p->traj;
@@ -1558,11 +1559,13 @@ int rsg_ball_pred_view(rsg_ball_pred* p, rsg_ball_pred_view_t* out) {
     return 0;
 }
 
-int rsg_init_meshes(const void* const* cmf_data, const uint64_t* sizes, int n) {
+int rsg_init_arena_meshes(int mode, const void* const* cmf_data, const uint64_t* sizes, int n) {
+    if (!mode_has_arena(mode)) return fail("rsg_init_arena_meshes: the mode has no arena meshes");
     if (n > 0 && (!cmf_data || !sizes)) return fail("rsg_init_meshes: null argument");
+    DeviceArena& g_mesh = g_arenas[arena_kind(mode)];
     std::lock_guard<std::mutex> lock(g_mesh_mutex);
     HostArena H;
-    std::string err = build_host_arena(cmf_data, sizes, n, H);
+    std::string err = build_host_arena(cmf_data, sizes, n, arena_kind(mode), H);
     if (!err.empty()) return fail("rsg_init_meshes: invalid collision mesh files (" + err + ")");
     // kernels of earlier launches may still read the old arrays
     (void)hipDeviceSynchronize();
@@ -1591,14 +1594,17 @@ int rsg_init_meshes(const void* const* cmf_data, const uint64_t* sizes, int n) {
         views.push_back(v);
     }
     if (upload_array(&g_mesh.views, views) || upload_array(&g_mesh.cell_statics, g_mesh.host.cell_statics)) return -1;
-    ArenaGeom g{};
+    ArenaGeom g = g_mesh.host.geom();
     g.meshes = g_mesh.views;
     g.n_meshes = (int)views.size();
     g.cell_statics = g_mesh.cell_statics;
-    g.plane_unit = g_mesh.host.plane_unit;
     g_mesh.geom = g;
     g_mesh.ready = true;
     return 0;
+}
+
+int rsg_init_meshes(const void* const* cmf_data, const uint64_t* sizes, int n) {
+    return rsg_init_arena_meshes(RSG_MODE_SOCCAR, cmf_data, sizes, n);
 }
 
 int rsg_init_mesh(const void* cmf_data, uint64_t size) {

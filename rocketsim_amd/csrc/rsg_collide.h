@@ -204,8 +204,7 @@ struct World {
     RSG_FN Xf xf_of(int code) {
         if (code >= 0) { Body* b = body_ptr(code); return xf_from(b->pos, b->rot); }
         if (static_is_mesh(*geo, -1 - code)) return xf_ident();
-        PlaneDef p = plane_def(-1 - code - geo->n_meshes);
-        return xf_from(p.origin, m3_identity());
+        return xf_from(static_plane(*geo, -1 - code).origin, m3_identity());
     }
 
     RSG_FN float threshold_of(int code) {
@@ -277,7 +276,7 @@ struct World {
         // static objects registered in the cell, in registration order
         const int ns = soccar ? num_statics(*geo) : 0;
         for (int s = 0; s < ns; s++) {
-            if (!static_in_cell(s, c)) continue;
+            if (!static_in_cell(s, c) || ((geo->car_skip >> s) & 1u)) continue;
             if (frac == 0.f) break;
             if (static_is_mesh(*geo, s)) {
                 const MeshView& m = geo->meshes[s];
@@ -292,7 +291,7 @@ struct World {
                 mesh_query_ray(m, from, to, cb);
                 if (cb.hit) hit_obj = -1 - s;
             } else {
-                if (ray_plane(s - geo->n_meshes, geo->plane_unit, from, to, frac, n)) hit_obj = -1 - s;
+                if (ray_plane(static_plane(*geo, s), from, to, frac, n)) hit_obj = -1 - s;
             }
         }
         RSG_PROF_MARK(10);
@@ -473,7 +472,11 @@ struct World {
             V3 rel_vel = bb.vel * BT_TO_UU - c.b.vel * BT_TO_UU;
             float rel_speed = rs_min(len(rel_vel), BALL_CAR_EXTRA_IMPULSE_MAXDELTAVEL_UU);
             if (rel_speed > 0) {
-                V3 hit_dir = rs_normalized(rel_pos * v3(1, 1, BALL_CAR_EXTRA_IMPULSE_Z_SCALE));
+                // Hoops lifts harder for a car on the ground (Ball.cpp:272-276)
+                const bool hoops_z = A.mode == RSG_MODE_HOOPS && flag(c.flags, RSG_CF_ON_GROUND) &&
+                                     car_up(c).z > BALL_CAR_EXTRA_IMPULSE_Z_SCALE_HOOPS_NORMAL_Z_THRESH;
+                const float z_scale = hoops_z ? BALL_CAR_EXTRA_IMPULSE_Z_SCALE_HOOPS_GROUND : BALL_CAR_EXTRA_IMPULSE_Z_SCALE;
+                V3 hit_dir = rs_normalized(rel_pos * v3(1, 1, z_scale));
                 V3 fwd_adj = fwd * dot(hit_dir, fwd) * (1 - BALL_CAR_EXTRA_IMPULSE_FORWARD_SCALE);
                 hit_dir = rs_normalized(hit_dir - fwd_adj);
                 V3 added = (hit_dir * rel_speed) * curve_eval(curve_ball_extra_impulse(), rel_speed) *
@@ -551,8 +554,8 @@ struct World {
     // Narrowphase
 
     // btConvexPlaneCollisionAlgorithm::processCollision for the ball
-    RSG_FN void ball_vs_plane(int mi, int plane) {
-        PlaneDef pd = plane_def(plane, geo->plane_unit);
+    RSG_FN void ball_vs_plane(int mi, int s) {
+        const PlaneDef& pd = static_plane(*geo, s);
         Xf px = xf_from(pd.origin, m3_identity());
         Body& b = A.ball.b;
         Xf bx = xf_from(b.pos, b.rot);
@@ -575,8 +578,8 @@ struct World {
     }
 
     // Child box of a car vs a plane (same algorithm, box support with margin)
-    RSG_FN void car_vs_plane(int mi, int code, int plane) {
-        PlaneDef pd = plane_def(plane, geo->plane_unit);
+    RSG_FN void car_vs_plane(int mi, int code, int s) {
+        const PlaneDef& pd = static_plane(*geo, s);
         Xf px = xf_from(pd.origin, m3_identity());
         Car& c = A.cars[code - 1];
         const CarPhys& cp = c.phys;
@@ -905,8 +908,9 @@ struct World {
         for (int p = 0; p < nb; p++) {
             int c = A.bp_cell[p];
             const int ns = soccar ? num_statics(*geo) : 0;
+            const uint32_t skip = p > 0 ? geo->car_skip : 0u;
             for (int s = 0; s < ns; s++) {
-                if (!static_in_cell(s, c)) continue;
+                if (!static_in_cell(s, c) || ((skip >> s) & 1u)) continue;
                 V3 smin, smax;
                 static_proxy_aabb(geo, s, smin, smax);
                 if (aabb_overlap(T->aabb_min[p], T->aabb_max[p], smin, smax)) {
@@ -1001,10 +1005,10 @@ struct World {
             mesh = is_mesh ? &geo->meshes[s] : nullptr;
             if (b == 0) {
                 if (is_mesh) ball_vs_mesh(mi);
-                else ball_vs_plane(mi, s - geo->n_meshes);
+                else ball_vs_plane(mi, s);
             } else {
                 if (is_mesh) car_vs_mesh(mi, b);
-                else car_vs_plane(mi, b, s - geo->n_meshes);
+                else car_vs_plane(mi, b, s);
             }
         } else if (a == 0) {
             car_vs_ball(mi, b);

@@ -21,6 +21,12 @@ constexpr float UU_TO_BT = 1.f / 50.f;
 constexpr double PI_D = 3.14159265358979323846;
 constexpr float GRAVITY_Z = -650.f;
 constexpr float ARENA_EXTENT_X = 4096, ARENA_EXTENT_Y = 5120, ARENA_HEIGHT = 2048;
+constexpr float ARENA_EXTENT_X_HOOPS = 8900 / 3.f, ARENA_EXTENT_Y_HOOPS = 3581, ARENA_HEIGHT_HOOPS = 1820;
+// Hoops (src/RLConst.h:50-51,118-139): the ball's kickoff launch, radius,
+// goal height and the stronger upward extra hit impulse of cars on the ground.
+constexpr float BALL_HOOPS_LAUNCH_Z_VEL = 1000, BALL_HOOPS_LAUNCH_DELAY = 0.265f;
+constexpr float BALL_COLLISION_RADIUS_HOOPS = 96.3831f;
+constexpr float HOOPS_GOAL_SCORE_THRESHOLD_Z = 270.f;
 constexpr float ARENA_COLLISION_BASE_FRICTION = 0.6f, ARENA_COLLISION_BASE_RESTITUTION = 0.3f;
 constexpr float CAR_MASS_BT = 180.f;
 constexpr float BALL_MASS_BT = CAR_MASS_BT / 6.f;
@@ -84,6 +90,8 @@ constexpr float CAR_AUTOFLIP_ROLL_THRESH = 2.8f;
 constexpr float CAR_AUTOROLL_FORCE = 100;
 constexpr float CAR_AUTOROLL_TORQUE = 80;
 constexpr float BALL_CAR_EXTRA_IMPULSE_Z_SCALE = 0.35f;
+constexpr float BALL_CAR_EXTRA_IMPULSE_Z_SCALE_HOOPS_GROUND = BALL_CAR_EXTRA_IMPULSE_Z_SCALE * 1.55f;
+constexpr float BALL_CAR_EXTRA_IMPULSE_Z_SCALE_HOOPS_NORMAL_Z_THRESH = 0.1f;
 constexpr float BALL_CAR_EXTRA_IMPULSE_FORWARD_SCALE = 0.65f;
 constexpr float BALL_CAR_EXTRA_IMPULSE_MAXDELTAVEL_UU = 4600.f;
 constexpr float CAR_SPAWN_REST_Z = 17.f;
@@ -112,6 +120,7 @@ constexpr float PAD_BOX_HEIGHT = 64, PAD_BOX_RAD_BIG = 160, PAD_BOX_RAD_SMALL = 
 constexpr float PAD_COOLDOWN_BIG = 10, PAD_COOLDOWN_SMALL = 4;
 constexpr float PAD_BOOST_AMOUNT_BIG = 100, PAD_BOOST_AMOUNT_SMALL = 12;
 constexpr int PAD_NUM_BIG = 6, PAD_NUM_SMALL = 28, PAD_NUM = 34;
+constexpr int PAD_NUM_SMALL_HOOPS = 14;  // Hoops: 6 big + 14 small pads in the first 20 slots
 
 // Boost pad grid (src/Sim/BoostPad/BoostPadGrid/BoostPadGrid.h)
 constexpr float PAD_GRID_EXTENT_Z = PAD_CYL_HEIGHT + 250.f;
@@ -334,8 +343,27 @@ RSG_FN BallPhys make_ball_phys(float radius_uu = BALL_COLLISION_RADIUS_SOCCAR, f
     return b;
 }
 
-// Car spawn tables (src/RLConst.h:388-427), blue side; orange is mirrored.
+// Car spawn tables (src/RLConst.h:355-427), blue side; orange is mirrored.
 struct SpawnPos { float x, y, yaw; };
+RSG_FN SpawnPos kickoff_spawn_hoops(int i) {
+    switch (i) {
+        default:
+        case 0: return SpawnPos{-1536, -3072, (float)(0.78539816339744830962 * 2)};
+        case 1: return SpawnPos{1536, -3072, (float)(0.78539816339744830962 * 2)};
+        case 2: return SpawnPos{-256, -2816, (float)(0.78539816339744830962 * 2)};
+        case 3: return SpawnPos{256, -2816, (float)(0.78539816339744830962 * 2)};
+        case 4: return SpawnPos{0, -3200, (float)(0.78539816339744830962 * 2)};
+    }
+}
+RSG_FN SpawnPos respawn_spawn_hoops(int i) {
+    switch (i) {
+        default:
+        case 0: return SpawnPos{-1920, -3072, (float)(PI_D / 2)};
+        case 1: return SpawnPos{-1152, -3072, (float)(PI_D / 2)};
+        case 2: return SpawnPos{1920, -3072, (float)(PI_D / 2)};
+        case 3: return SpawnPos{1152, -3072, (float)(PI_D / 2)};
+    }
+}
 RSG_FN SpawnPos kickoff_spawn(int i) {
     switch (i) {
         default:
@@ -366,9 +394,33 @@ RSG_FN SpawnPos kickoff_spawn_heatseeker(int i) {
     }
 }
 
-// Modes played in the Soccar arena: its mesh, boost pads and goals
-// (Arena.cpp:478-520, 893-901).
-RSG_FN bool mode_has_arena(int mode) { return mode == RSG_MODE_SOCCAR || mode == RSG_MODE_HEATSEEKER; }
+// Modes played in an arena (meshes, planes, boost pads and goals,
+// Arena.cpp:478-520, 893-915): Soccar and Heatseeker in the Soccar arena,
+// Hoops in its own.
+RSG_FN bool mode_has_arena(int mode) {
+    return mode == RSG_MODE_SOCCAR || mode == RSG_MODE_HEATSEEKER || mode == RSG_MODE_HOOPS;
+}
+// The arena a mode's meshes belong to (0: Soccar, 1: Hoops).
+RSG_FN int arena_kind(int mode) { return mode == RSG_MODE_HOOPS ? 1 : 0; }
+// Boost pads of the mode's arena (Arena.cpp:509-531).
+RSG_FN int pad_count(int mode) { return mode == RSG_MODE_HOOPS ? PAD_NUM_BIG + PAD_NUM_SMALL_HOOPS : PAD_NUM; }
+RSG_FN SpawnPos kickoff_spawn_of(int mode, int i) {
+    if (mode == RSG_MODE_HOOPS) return kickoff_spawn_hoops(i);
+    if (mode == RSG_MODE_HEATSEEKER) return kickoff_spawn_heatseeker(i);
+    return kickoff_spawn(i);
+}
+// Car::Respawn's table (Car.cpp:49); Heatseeker respawns at the Soccar slots.
+RSG_FN SpawnPos respawn_spawn_of(int mode, int i) {
+    return mode == RSG_MODE_HOOPS ? respawn_spawn_hoops(i) : respawn_spawn(i);
+}
+// BallWithinHoopsGoalXYMarginSq (Arena.cpp:759-768): negative inside a
+// hoop's footprint (squared margin, uu).
+RSG_FN float hoops_goal_xy_margin_sq(float x, float y) {
+    const float SCALE_Y = 0.9f, OFFSET_Y = 2770.f, RADIUS_SQ = 716 * 716;
+    float dy = fabsf(y) * SCALE_Y - OFFSET_Y;
+    float dist_sq = x * x + dy * dy;
+    return dist_sq - RADIUS_SQ;
+}
 
 // MutatorConfig(GameMode) (src/Sim/MutatorConfig/MutatorConfig.h:16-79,
 // MutatorConfig.cpp:5-40): the field defaults, then the mode's changes
@@ -406,6 +458,7 @@ RSG_FN rsg_mutator_config mutator_defaults(int mode) {
     m.enable_team_demos = 0;
     m._pad[0] = m._pad[1] = m._pad[2] = 0;
     m.goal_base_threshold_y = SOCCAR_GOAL_SCORE_BASE_THRESHOLD_Y;
+    if (mode == RSG_MODE_HOOPS) m.ball_radius = BALL_COLLISION_RADIUS_HOOPS;
     if (mode == RSG_MODE_HEATSEEKER) {
         m.car_spawn_boost_amount = BOOST_MAX;
         m.boost_used_per_second = 0.f;
@@ -422,8 +475,34 @@ constexpr float HS_INITIAL_TARGET_SPEED = 2900, HS_TARGET_SPEED_INCREMENT = 85, 
 constexpr float HS_BALL_START_X = -1000, HS_BALL_START_Y = -2220, HS_BALL_START_Z = 92.75f, HS_BALL_START_VY = -65,
                 HS_BALL_START_VZ = 650;
 
+// Hoops boost pad locations (src/RLConst.h:314-340): 6 big then 14 small.
+RSG_FN V3 pad_location_hoops(int i) {
+    switch (i) {
+        case 0: return v3(-2176.f, 2944.f, 72.f);
+        case 1: return v3(2176.f, -2944.f, 72.f);
+        case 2: return v3(-2176.f, -2944.f, 72.f);
+        case 3: return v3(-2432.f, 0.f, 72.f);
+        case 4: return v3(2432.f, 0.f, 72.f);
+        case 5: return v3(2175.99f, 2944.f, 72.f);
+        case 6: return v3(1536.f, -1024.f, 64.f);
+        case 7: return v3(-1280.f, -2304.f, 64.f);
+        case 8: return v3(0.f, -2816.f, 64.f);
+        case 9: return v3(-1536.f, -1024.f, 64.f);
+        case 10: return v3(1280.f, -2304.f, 64.f);
+        case 11: return v3(-512.f, 512.f, 64.f);
+        case 12: return v3(-1536.f, 1024.f, 64.f);
+        case 13: return v3(1536.f, 1024.f, 64.f);
+        case 14: return v3(1280.f, 2304.f, 64.f);
+        case 15: return v3(0.f, 2816.f, 64.f);
+        case 16: return v3(512.f, 512.f, 64.f);
+        case 17: return v3(512.f, -512.f, 64.f);
+        case 18: return v3(-512.f, -512.f, 64.f);
+        default: return v3(-1280.f, 2304.f, 64.f);
+    }
+}
+
 // Boost pad locations (src/RLConst.h:264-309): 6 big then 28 small.
-RSG_FN V3 pad_location(int i) {
+RSG_FN V3 pad_location_soccar(int i) {
     switch (i) {
         case 0: return v3(-3584.f, 0.f, 73.f);
         case 1: return v3(3584.f, 0.f, 73.f);
@@ -460,6 +539,23 @@ RSG_FN V3 pad_location(int i) {
         case 32: return v3(1792.f, 4184.f, 70.f);
         default: return v3(0.f, 4240.f, 70.f);
     }
+}
+
+// Arena::IsBallScored (Arena.cpp:893-918) for a ball at `pos_bt` (Bullet
+// units): past the goal line (Soccar arena) or low inside a hoop (Hoops).
+RSG_FN bool ball_scored(int mode, V3 pos_bt, const rsg_mutator_config& m) {
+    if (mode == RSG_MODE_HOOPS) {
+        if (!(pos_bt.z < HOOPS_GOAL_SCORE_THRESHOLD_Z * UU_TO_BT)) return false;
+        V3 p = pos_bt * BT_TO_UU;
+        return hoops_goal_xy_margin_sq(p.x, p.y) < 0;
+    }
+    if (mode != RSG_MODE_SOCCAR && mode != RSG_MODE_HEATSEEKER) return false;
+    return fabsf(pos_bt.y * BT_TO_UU) > (m.goal_base_threshold_y + m.ball_radius);
+}
+
+// Boost pad i of the mode's arena.
+RSG_FN V3 pad_location(int mode, int i) {
+    return mode == RSG_MODE_HOOPS ? pad_location_hoops(i) : pad_location_soccar(i);
 }
 
 }  // namespace rsg

@@ -22,25 +22,14 @@ namespace rsg {
 
 // Static collision objects in the reference's creation order
 // (Arena::_SetupArenaCollisionShapes, src/Sim/Arena/Arena.cpp:990-1068): the
-// arena's collision meshes in load order, then the floor, the ceiling and the
-// two side walls.  In an arena of M meshes static object s is mesh s for
-// s < M and plane s - M otherwise; its body code is -1 - s.
-enum PlaneId { PL_FLOOR = 0, PL_CEIL = 1, PL_WALL_NX = 2, PL_WALL_PX = 3, NUM_PLANES = 4 };
-constexpr int MAX_MESHES = 28;
-constexpr int MAX_STATICS = MAX_MESHES + NUM_PLANES;  // one bit each in a cell's mask
-
-// An arena's static geometry as the kernels see it, shared by every arena
-// of the process (the reference keeps its meshes in process-wide storage,
-// src/RocketSim.cpp:50-53).
-struct ArenaGeom {
-    const MeshView* meshes;        // n_meshes views, in load order
-    int n_meshes;
-    const uint32_t* cell_statics;  // BP_NUM_CELLS masks: bit s = static object s registered in the cell
-    // Length of the static planes' stored unit normals (see plane_def).
-    float plane_unit;
-};
-RSG_FN int num_statics(const ArenaGeom& g) { return g.n_meshes + NUM_PLANES; }
-RSG_FN bool static_is_mesh(const ArenaGeom& g, int s) { return s < g.n_meshes; }
+// arena's collision meshes in load order, then its planes (the floor, the
+// ceiling, the two side walls and, in Hoops, the two end walls).  In an arena
+// of M meshes static object s is mesh s for s < M and plane s - M otherwise;
+// its body code is -1 - s.
+enum PlaneId { PL_FLOOR = 0, PL_CEIL = 1, PL_WALL_NX = 2, PL_WALL_PX = 3, PL_WALL_NY = 4, PL_WALL_PY = 5,
+               MAX_PLANES = 6 };
+constexpr int MAX_MESHES = 26;
+constexpr int MAX_STATICS = MAX_MESHES + MAX_PLANES;  // one bit each in a cell's mask
 
 struct PlaneDef {
     V3 n;        // plane normal
@@ -49,27 +38,48 @@ struct PlaneDef {
     bool backwards;
 };
 
-// `unit` is the stored length of the axis-aligned normals: the reference
-// keeps planeNormal.normalized() (btStaticPlaneShape.cpp:21), whose SSE
-// reciprocal-square-root normalisation of a unit axis is the host CPU's
-// rsqrtss(1) after one Newton step (0.99999994 on common x86 cores), measured
-// once on the host (ArenaGeom::plane_unit).
-RSG_FN PlaneDef plane_def(int plane, float unit = 1.f) {
-    PlaneDef p;
-    switch (plane) {
-        default:
-        case PL_FLOOR: p.n = v3(0, 0, unit); p.origin = v3(0, 0, 0) * UU_TO_BT; p.axis = 2; p.backwards = false; break;
-        case PL_CEIL: p.n = v3(0, 0, -unit); p.origin = v3(0, 0, ARENA_HEIGHT) * UU_TO_BT; p.axis = 2; p.backwards = true; break;
-        case PL_WALL_NX: p.n = v3(unit, 0, 0); p.origin = v3(-ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = false; break;
-        case PL_WALL_PX: p.n = v3(-unit, 0, 0); p.origin = v3(ARENA_EXTENT_X, 0, ARENA_HEIGHT / 2) * UU_TO_BT; p.axis = 0; p.backwards = true; break;
-    }
-    return p;
+// An arena's static geometry as the kernels see it, shared by every arena
+// of the process (the reference keeps its meshes in process-wide storage,
+// src/RocketSim.cpp:50-53).
+struct ArenaGeom {
+    const MeshView* meshes;        // n_meshes views, in load order
+    int n_meshes;
+    int n_planes;
+    const uint32_t* cell_statics;  // BP_NUM_CELLS masks: bit s = static object s registered in the cell
+    // Statics that cars and their wheel rays pass through: the Hoops net
+    // meshes, whose collision group only the ball's filter mask holds
+    // (Arena.cpp:1005-1024, Ball.cpp:111-114, Car.cpp:227-233).
+    uint32_t car_skip;
+    PlaneDef planes[MAX_PLANES];
+};
+RSG_FN int num_statics(const ArenaGeom& g) { return g.n_meshes + g.n_planes; }
+RSG_FN bool static_is_mesh(const ArenaGeom& g, int s) { return s < g.n_meshes; }
+RSG_FN const PlaneDef& static_plane(const ArenaGeom& g, int s) { return g.planes[s - g.n_meshes]; }
+
+// The planes of an arena kind (0: Soccar, 1: Hoops) in creation order
+// (Arena.cpp:1033-1068); returns their count.  `unit` is the stored length
+// of the axis-aligned normals: the reference keeps planeNormal.normalized()
+// (btStaticPlaneShape.cpp:21), whose SSE reciprocal-square-root normalisation
+// of a unit axis is the host CPU's rsqrtss(1) after one Newton step
+// (0.99999994 on common x86 cores), measured once on the host.
+RSG_FN int arena_planes(int kind, float unit, PlaneDef out[MAX_PLANES]) {
+    const bool hoops = kind == 1;
+    const float ex = hoops ? ARENA_EXTENT_X_HOOPS : ARENA_EXTENT_X;
+    const float ey = ARENA_EXTENT_Y_HOOPS;
+    const float h = hoops ? ARENA_HEIGHT_HOOPS : ARENA_HEIGHT;
+    out[PL_FLOOR] = PlaneDef{v3(0, 0, unit), v3(0, 0, 0) * UU_TO_BT, 2, false};
+    out[PL_CEIL] = PlaneDef{v3(0, 0, -unit), v3(0, 0, h) * UU_TO_BT, 2, true};
+    out[PL_WALL_NX] = PlaneDef{v3(unit, 0, 0), v3(-ex, 0, h / 2) * UU_TO_BT, 0, false};
+    out[PL_WALL_PX] = PlaneDef{v3(-unit, 0, 0), v3(ex, 0, h / 2) * UU_TO_BT, 0, true};
+    if (!hoops) return 4;
+    out[PL_WALL_NY] = PlaneDef{v3(0, unit, 0), v3(0, -ey, h / 2) * UU_TO_BT, 1, false};
+    out[PL_WALL_PY] = PlaneDef{v3(0, -unit, 0), v3(0, ey, h / 2) * UU_TO_BT, 1, true};
+    return 6;
 }
 
 // btStaticPlaneShape::getAabb with RocketSim's single-axis slab (+-0.2 m,
 // unbounded behind the plane).
-RSG_FN void plane_shape_aabb(int plane, V3& mn, V3& mx) {
-    PlaneDef p = plane_def(plane);
+RSG_FN void plane_shape_aabb(const PlaneDef& p, V3& mn, V3& mx) {
     const float L = 1e18f;
     mn = v3(-L, -L, -L);
     mx = v3(L, L, L);
@@ -89,7 +99,7 @@ RSG_FN void mesh_shape_aabb(const MeshView& m, V3& mn, V3& mx) {
 
 RSG_FN void static_shape_aabb(const ArenaGeom* g, int s, V3& mn, V3& mx) {
     if (static_is_mesh(*g, s)) mesh_shape_aabb(g->meshes[s], mn, mx);
-    else plane_shape_aabb(s - g->n_meshes, mn, mx);
+    else plane_shape_aabb(static_plane(*g, s), mn, mx);
 }
 
 // Broadphase proxy AABB of a static object: shape AABB +- the global contact
@@ -438,8 +448,7 @@ RSG_FN void plane_space(V3 n, V3& p, V3& q) {
 
 // Ray vs static plane: the plane shape emits two triangles around the ray's
 // AABB (btStaticPlaneShape::processAllTriangles) which are ray-tested.
-RSG_FN bool ray_plane(int plane, float unit, V3 from_w, V3 to_w, float& hit_fraction, V3& hit_normal) {
-    PlaneDef pd = plane_def(plane, unit);
+RSG_FN bool ray_plane(const PlaneDef& pd, V3 from_w, V3 to_w, float& hit_fraction, V3& hit_normal) {
     // world -> plane local (identity basis)
     V3 from = from_w - pd.origin, to = to_w - pd.origin;
     V3 amin = v3(fminf(from.x, to.x), fminf(from.y, to.y), fminf(from.z, to.z));

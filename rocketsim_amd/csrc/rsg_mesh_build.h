@@ -381,7 +381,9 @@ struct HostArena {
     std::vector<HostMesh> meshes;
     std::vector<MeshView> views;        // host views of `meshes` (refresh_views)
     std::vector<uint32_t> cell_statics; // BP_NUM_CELLS masks
-    float plane_unit = 1.f;             // ArenaGeom::plane_unit
+    int n_planes = 0;
+    PlaneDef planes[MAX_PLANES];
+    uint32_t car_skip = 0;              // ArenaGeom::car_skip
 
     void refresh_views() {
         views.clear();
@@ -392,7 +394,9 @@ struct HostArena {
         g.meshes = views.data();
         g.n_meshes = (int)views.size();
         g.cell_statics = cell_statics.empty() ? nullptr : cell_statics.data();
-        g.plane_unit = plane_unit;
+        g.n_planes = n_planes;
+        g.car_skip = car_skip;
+        for (int i = 0; i < MAX_PLANES; i++) g.planes[i] = planes[i];
         return g;
     }
 };
@@ -494,10 +498,17 @@ inline std::string build_host_mesh(const void* data, uint64_t size, HostMesh& M)
     return "";
 }
 
-// An arena from its n collision mesh files, in the order given (the
-// reference adds one static body per mesh in list order, then the planes,
-// src/Sim/Arena/Arena.cpp:1000-1068).  Returns "" or why the files were refused.
-inline std::string build_host_arena(const void* const* data, const uint64_t* sizes, int n, HostArena& H) {
+// Vertex count by which the reference recognises a Hoops net mesh
+// (HOOPS_NET_NUM_VERTS, src/Sim/Arena/Arena.cpp:1008-1018).
+constexpr int HOOPS_NET_NUM_VERTS = 505;
+
+// An arena of kind `kind` (0: Soccar, 1: Hoops) from its n collision mesh
+// files, in the order given (the reference adds one static body per mesh in
+// list order, then the planes, src/Sim/Arena/Arena.cpp:990-1068).  Returns ""
+// or why the files were refused.
+inline std::string build_host_arena(const void* const* data, const uint64_t* sizes, int n, int kind,
+                                    HostArena& H) {
+    if (kind != 0 && kind != 1) return "unknown arena kind";
     if (n < 1 || n > MAX_MESHES)
         return "between 1 and " + std::to_string(MAX_MESHES) + " mesh files are supported, got " + std::to_string(n);
     H.meshes.assign((size_t)n, HostMesh());
@@ -505,9 +516,10 @@ inline std::string build_host_arena(const void* const* data, const uint64_t* siz
         if (!data[i]) return "mesh file " + std::to_string(i) + " is null";
         std::string err = build_host_mesh(data[i], sizes[i], H.meshes[(size_t)i]);
         if (!err.empty()) return "mesh file " + std::to_string(i) + ": " + err;
+        if (kind == 1 && (int)H.meshes[(size_t)i].verts.size() / 3 == HOOPS_NET_NUM_VERTS) H.car_skip |= 1u << i;
     }
+    H.n_planes = arena_planes(kind, host_sse_normalized(v3(0.f, 0.f, 1.f)).z, H.planes);
     build_cell_statics(H);
-    H.plane_unit = host_sse_normalized(v3(0.f, 0.f, 1.f)).z;
     return "";
 }
 

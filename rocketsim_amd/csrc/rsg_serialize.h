@@ -232,8 +232,9 @@ inline bool records_from_raw(rsg_arena_snapshot& s) {
 
 // Arena::Serialize (Arena.cpp:546-573).
 inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<uint8_t>& out) {
-    if (s.mode != RSG_MODE_SOCCAR && s.mode != RSG_MODE_HEATSEEKER && s.mode != RSG_MODE_THE_VOID)
-        return "mode must be SOCCAR, HEATSEEKER or THE_VOID";
+    if (s.mode != RSG_MODE_SOCCAR && s.mode != RSG_MODE_HOOPS && s.mode != RSG_MODE_HEATSEEKER &&
+        s.mode != RSG_MODE_THE_VOID)
+        return "mode must be SOCCAR, HOOPS, HEATSEEKER or THE_VOID";
     if (s.n_cars < 0 || s.n_cars > RSG_MAX_CARS) return "n_cars out of range";
     ByteOut o;
     o.count(4);
@@ -301,8 +302,9 @@ inline std::string snapshot_serialize(const rsg_arena_snapshot& s, std::vector<u
 
     if (mode_has_arena(s.mode)) {
         // BoostPadState::Serialize (src/Sim/BoostPad/BoostPad.cpp:21-25)
-        o.put<uint32_t>(RSG_NUM_PADS);
-        for (int p = 0; p < RSG_NUM_PADS; p++) {
+        const int np = pad_count(s.mode);
+        o.put<uint32_t>((uint32_t)np);
+        for (int p = 0; p < np; p++) {
             o.count(3);
             o.b8(s.pads[p].is_active != 0);
             o.put<float>(s.pads[p].cooldown);
@@ -372,8 +374,9 @@ inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_ar
     s.tick_time = in.get<float>();
     s.tick_count = in.get<uint64_t>();
     s.last_car_id = in.get<uint32_t>();
-    if (s.mode != RSG_MODE_SOCCAR && s.mode != RSG_MODE_HEATSEEKER && s.mode != RSG_MODE_THE_VOID)
-        return "game mode is not SOCCAR, HEATSEEKER or THE_VOID";
+    if (s.mode != RSG_MODE_SOCCAR && s.mode != RSG_MODE_HOOPS && s.mode != RSG_MODE_HEATSEEKER &&
+        s.mode != RSG_MODE_THE_VOID)
+        return "game mode is not SOCCAR, HOOPS, HEATSEEKER or THE_VOID";
     {
         ByteOut want;
         write_default_arena_config(want);
@@ -471,8 +474,9 @@ inline std::string snapshot_deserialize(const uint8_t* data, size_t size, rsg_ar
     for (int k = 0; k < s.n_cars; k++) creation_rank[k] = k;
 
     if (mode_has_arena(s.mode)) {
-        if (in.get<uint32_t>() != RSG_NUM_PADS) return "boost pad count is not 34";
-        for (int p = 0; p < RSG_NUM_PADS; p++) {
+        const int np = pad_count(s.mode);
+        if (in.get<uint32_t>() != (uint32_t)np) return "boost pad count is not the arena's";
+        for (int p = 0; p < np; p++) {
             if (in.get<uint32_t>() != 3) return "boost pad field count is not 3";
             s.pads[p].is_active = in.b8() ? 1u : 0u;
             s.pads[p].cooldown = in.get<float>();

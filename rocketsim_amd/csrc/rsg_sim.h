@@ -716,9 +716,9 @@ RSG_FN void vehicle_second(Car& c, WheelT* wh, const CarPhys& cp, float dt) {
     }
 }
 
-// Car::Respawn (Car.cpp:43-56) at a given respawn slot.
-RSG_FN void car_respawn_state(Car& c, int slot, float boost_amount) {
-    SpawnPos sp = respawn_spawn(slot);
+// Car::Respawn (Car.cpp:43-56) at a given respawn slot of the mode's table.
+RSG_FN void car_respawn_state(Car& c, int mode, int slot, float boost_amount) {
+    SpawnPos sp = respawn_spawn_of(mode, slot);
     V3 pos_uu = v3(sp.x, sp.y * (c.team == RSG_TEAM_BLUE ? 1.f : -1.f), CAR_RESPAWN_Z);
     float yaw = (float)((double)sp.yaw + (c.team == RSG_TEAM_BLUE ? 0.0 : PI_D));
     // CarState() defaults then SetState (Car.cpp:23-36)

@@ -295,7 +295,7 @@ RSG_FN void store_arena(const BatchDev& d, int a, const ArenaLocal& A) {
 // team.
 RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id) {
     const bool hs = A.mode == RSG_MODE_HEATSEEKER;
-    const int n_slots = hs ? 4 : 5;
+    const int n_slots = hs ? 4 : 5;  // CAR_SPAWN_LOCATION_AMOUNT(_HEATSEEKER)
     int slots[5] = {0, 1, 2, 3, 4};
     uint64_t h = rsg_draw(seed ^ 0x4B1C0FFull, arena_id, 0xFFFFu, A.resets, 7);
     for (int i = n_slots - 1; i > 0; i--) {
@@ -310,15 +310,15 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
         int k = idx_team[c.team]++;
         SpawnPos sp;
         if (k < n_slots) {
-            sp = hs ? kickoff_spawn_heatseeker(slots[k]) : kickoff_spawn(slots[k]);
+            sp = kickoff_spawn_of(A.mode, slots[k]);
         } else {
             int ri = (k - n_slots) % n_slots;
             ri = ri < 4 ? ri : 3;
-            sp = respawn_spawn(ri);
+            sp = respawn_spawn_of(A.mode, ri);
             sp.y += 250.f * num_at_respawn[ri];
             num_at_respawn[ri]++;
         }
-        car_respawn_state(c, 0, A.mut.car_spawn_boost_amount);
+        car_respawn_state(c, A.mode, 0, A.mut.car_spawn_boost_amount);
         V3 pos = v3(sp.x, sp.y, CAR_SPAWN_REST_Z);
         float yaw = sp.yaw;
         if (c.team != RSG_TEAM_BLUE) {
@@ -350,7 +350,7 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
     A.ball.ticks_since_set = 0;
     A.ball.vel_cache = v3zero();
     body_update_inertia(A.ball.b);
-    for (int p = 0; p < PAD_NUM; p++) {
+    for (int p = 0; p < pad_count(A.mode); p++) {
         A.pads[p].active = true;
         A.pads[p].cooldown = 0;
         A.pads[p].prev_locked = 0;
@@ -368,8 +368,8 @@ RSG_FN bool pad_is_big(int p) { return p < PAD_NUM_BIG; }
 RSG_FN int pad_cell_x(V3 p) { return (int)(p.x / PAD_GRID_CELL_SIZE_X + (PAD_GRID_CELLS_X / 2)); }
 RSG_FN int pad_cell_y(V3 p) { return (int)(p.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2)); }
 
-RSG_FN void pad_check_collide(Pad& pad, int p, const Car& c, int car_idx, const CarPhys& cp) {
-    V3 pos_uu = pad_location(p);
+RSG_FN void pad_check_collide(int mode, Pad& pad, int p, const Car& c, int car_idx, const CarPhys& cp) {
+    V3 pos_uu = pad_location(mode, p);
     V3 pos_bt = pos_uu * UU_TO_BT;
     V3 cpos = c.b.pos;
     bool colliding = false;
@@ -459,7 +459,7 @@ RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int c
         c.demo_timer = rs_max(c.demo_timer - dt, 0.f);
         if (c.demo_timer == 0) {
             int slot = (int)(rsg_draw(P.seed ^ 0xDE40ull, P.arena_id, (uint32_t)ci, (uint32_t)A.tick, 9) % 4u);
-            car_respawn_state(c, slot, A.mut.car_spawn_boost_amount);
+            car_respawn_state(c, A.mode, slot, A.mut.car_spawn_boost_amount);
         }
         c.b.active = false;  // DISABLE_SIMULATION + no contact response
     } else {
@@ -516,7 +516,7 @@ RSG_FN bool car_pre_ticks_independent(const ArenaLocal& A, const RayHit* pre) {
 // Pads pre-tick (BoostPad.cpp:51-60).
 RSG_FN void pads_pre_tick(ArenaLocal& A, const TickParams& P) {
     if (!mode_has_arena(P.mode) || A.n_cars == 0) return;
-    for (int p = 0; p < PAD_NUM; p++) {
+    for (int p = 0; p < pad_count(P.mode); p++) {
         Pad& pad = A.pads[p];
         if (pad.cooldown > 0) pad.cooldown = rs_max(pad.cooldown - P.dt, 0.f);
         pad.active = (pad.cooldown == 0);
@@ -562,7 +562,7 @@ RSG_FN void car_post_tick(ArenaLocal& A, const TickParams& P, int ci) {
 // test reads only that car's pose, boost and flags, which no later post-tick
 // changes, and the last colliding car locks the pad).
 RSG_FN void pad_check_cars(ArenaLocal& A, int p) {
-    V3 pl = pad_location(p);
+    V3 pl = pad_location(A.mode, p);
     int px = pad_cell_x(pl), py = pad_cell_y(pl);
     for (int oi = 0; oi < A.n_cars; oi++) {
         const int ci = A.order[oi];
@@ -574,7 +574,7 @@ RSG_FN void pad_check_cars(ArenaLocal& A, int p) {
         int iy = (int)(cp_uu.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2));
         int x0 = ix - 1 > 0 ? ix - 1 : 0, x1 = ix + 1 < PAD_GRID_CELLS_X - 1 ? ix + 1 : PAD_GRID_CELLS_X - 1;
         int y0 = iy - 1 > 0 ? iy - 1 : 0, y1 = iy + 1 < PAD_GRID_CELLS_Y - 1 ? iy + 1 : PAD_GRID_CELLS_Y - 1;
-        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A.pads[p], p, c, ci, c.phys);
+        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A.mode, A.pads[p], p, c, ci, c.phys);
     }
 }
 
@@ -585,7 +585,7 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
     const bool soccar = mode_has_arena(P.mode);
     // 6. pads post-tick (BoostPad.cpp:88-107)
     if (soccar && A.n_cars > 0) {
-        for (int p = 0; p < PAD_NUM; p++) {
+        for (int p = 0; p < pad_count(P.mode); p++) {
             Pad& pad = A.pads[p];
             uint32_t locked_id = 0;
             if (pad.cur_locked >= 0) {
@@ -624,7 +624,7 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
     bool scored = false;
     if (soccar) {
         float y = A.ball.b.pos.y * BT_TO_UU;
-        if (fabsf(y) > (A.mut.goal_base_threshold_y + A.mut.ball_radius)) {
+        if (ball_scored(A.mode, A.ball.b.pos, A.mut)) {
             scored = true;
             if (y > 0) A.goals_blue++; else A.goals_orange++;
             A.last_goal_tick = A.tick;
@@ -702,10 +702,32 @@ RSG_FN void ball_pre_tick_heatseeker(Ball& ball, float dt) {
     ball.hs_time += dt;
 }
 
+// Ball::_PreTickUpdate for Hoops (src/Sim/Ball/Ball.cpp:209-233): the tick
+// at which BALL_HOOPS_LAUNCH_DELAY passes since the ball was last set, a ball
+// still frozen at the kickoff spot gets an upward impulse and wakes.
+RSG_FN void ball_pre_tick_hoops(Ball& ball, float dt) {
+    float cur = (float)ball.ticks_since_set * dt;
+    float prev = cur - dt;
+    if (!(prev < BALL_HOOPS_LAUNCH_DELAY && cur >= BALL_HOOPS_LAUNCH_DELAY)) return;
+    Body& b = ball.b;
+    V3 vel = b.vel * BT_TO_UU, pos = b.pos * BT_TO_UU;
+    if (!(is_zero(vel) && is_zero(b.ang) && pos.x == 0.f && pos.y == 0.f)) return;
+    float mass = b.inv_mass == 0.f ? 0.f : 1.f / b.inv_mass;  // btRigidBody::getMass
+    V3 impulse = (v3(0, 0, BALL_HOOPS_LAUNCH_Z_VEL) * mass) * UU_TO_BT;
+    b.vel += impulse * b.inv_mass;  // applyCentralImpulse (linear factor 1)
+    b.active = true;
+}
+
+// The ball's pre-tick of the mode.
+RSG_FN void ball_pre_tick(Ball& ball, int mode, float dt) {
+    if (mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(ball, dt);
+    else if (mode == RSG_MODE_HOOPS) ball_pre_tick_hoops(ball, dt);
+}
+
 // The world step of the tick (btDiscreteDynamicsWorld::stepSimulation),
 // preceded by the ball's pre-tick, which runs after the pads' (Arena.cpp:715).
 RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws) {
-    if (P.mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(A.ball, P.dt);
+    ball_pre_tick(A.ball, P.mode, P.dt);
     World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     world.step(P.dt);
     RSG_PROF_MARK(7);
@@ -717,7 +739,7 @@ RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo,
 // in between.  A pair whose contact needs the penetration pass is run again
 // by the last part, on the arena's own work space.
 RSG_FN void tick_world_begin(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws) {
-    if (P.mode == RSG_MODE_HEATSEEKER) ball_pre_tick_heatseeker(A.ball, P.dt);
+    ball_pre_tick(A.ball, P.mode, P.dt);
     World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     world.step_begin(P.dt);
 }
@@ -750,7 +772,7 @@ RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, 
     // 5. cars post-tick + pad collisions
     for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
     if (mode_has_arena(P.mode))
-        for (int p = 0; p < PAD_NUM; p++) pad_check_cars(A, p);
+        for (int p = 0; p < pad_count(P.mode); p++) pad_check_cars(A, p);
     tick_end(A, P);
 }
 

@@ -36,6 +36,7 @@ def lib():
         L.rsg_last_error.restype = C.c_char_p
         L.rsg_init_mesh.argtypes = [C.c_void_p, C.c_uint64]
         L.rsg_init_meshes.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_int]
+        L.rsg_init_arena_meshes.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_int]
         L.rsg_batch_create.restype = C.c_void_p
         L.rsg_batch_create.argtypes = [C.POINTER(BatchDesc)]
         L.rsg_batch_destroy.argtypes = [C.c_void_p]
@@ -121,7 +122,7 @@ def check(rc):
 
 # Every symbol include/rsg.h declares, for the ABI export test.
 EXPORTED = [
-    "rsg_last_error", "rsg_init_mesh", "rsg_init_meshes", "rsg_batch_create", "rsg_batch_destroy",
+    "rsg_last_error", "rsg_init_mesh", "rsg_init_meshes", "rsg_init_arena_meshes", "rsg_batch_create", "rsg_batch_destroy",
     "rsg_batch_num_arenas", "rsg_batch_num_cars", "rsg_batch_car_offset",
     "rsg_batch_step", "rsg_batch_sync",
     "rsg_batch_get_cars", "rsg_batch_set_cars", "rsg_batch_get_balls", "rsg_batch_set_balls",

@@ -9,8 +9,10 @@ plus the per-wheel vehicle values carried between ticks
 import ctypes as C
 
 MODE_SOCCAR = 0
+MODE_HOOPS = 1
 MODE_HEATSEEKER = 2
 MODE_THE_VOID = 5
+ARENA_MODES = (MODE_SOCCAR, MODE_HOOPS, MODE_HEATSEEKER)  # modes played in an arena with meshes and pads
 
 PRESET_OCTANE, PRESET_DOMINUS, PRESET_PLANK, PRESET_BREAKOUT, PRESET_HYBRID, PRESET_MERC = range(6)
 PRESET_NAMES = ["octane", "dominus", "plank", "breakout", "hybrid", "merc"]

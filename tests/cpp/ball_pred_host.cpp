@@ -71,7 +71,7 @@ void* bp_create(const void* cmf, uint64_t cmf_size, int n, int T, float tick_rat
     auto* h = new HostPred();
     if (cmf) {
         const void* files[1] = {cmf};
-        if (!build_host_arena(files, &cmf_size, 1, h->geo).empty()) { delete h; return nullptr; }
+        if (!build_host_arena(files, &cmf_size, 1, 0, h->geo).empty()) { delete h; return nullptr; }
         h->soccar = true;
     }
     h->n = n;

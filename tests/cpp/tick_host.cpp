@@ -71,7 +71,7 @@ void* th_create(int mode, int n, const int* cars_per_arena, const uint8_t* team,
                 const void* const* cmfs, const uint64_t* cmf_sizes, int n_cmf, float tick_rate,
                 const rsg_mutator_config* mut, const rsg_car_config* car_configs) {
     auto* h = new HostBatch();
-    if (n_cmf > 0 && !build_host_arena(cmfs, cmf_sizes, n_cmf, h->geo).empty()) {
+    if (n_cmf > 0 && !build_host_arena(cmfs, cmf_sizes, n_cmf, arena_kind(mode), h->geo).empty()) {
         delete h;
         return nullptr;
     }

@@ -27,6 +27,13 @@ def heatseeker():
     parity.save_samples(os.path.join(HERE, "heatseeker_tf.npz"), hs)
 
 
+def hoops():
+    ref.init()
+    hp = parity.sample_hoops(ref, n_traj=16, ticks=120, seed=20261209, stride=3)
+    parity.save_samples(os.path.join(HERE, "hoops_tf.npz"), hp)
+    print("hoops_tf.npz:", len(hp), "samples")
+
+
 def split():
     """The six-mesh arena's samples, drawn in a process of their own (the
     oracle loads its meshes once per process)."""
@@ -55,6 +62,7 @@ def main():
     parity.save_samples(os.path.join(HERE, "soccar_walls_tf.npz"), walls)
 
     heatseeker()
+    hoops()
     print("soccar_walls_tf.npz:", len(walls), "samples")
     states = ball_pred_cases.initial_states(ref, 10, seed=20261207)
     run = ball_pred_cases.reference_run(ref, states, 120)
@@ -65,5 +73,7 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["split"]:
         split()
+    elif sys.argv[1:] == ["hoops"]:
+        hoops()
     else:
         main()

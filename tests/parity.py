@@ -25,7 +25,8 @@ import random
 import numpy as np
 
 from rocketsim_amd.state import (BallState, CarState, Controls, PadState, CF_HAS_JUMPED,
-                                 CF_ON_GROUND, CF_SUPERSONIC, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID, NUM_PADS)
+                                 CF_ON_GROUND, CF_SUPERSONIC, ARENA_MODES, MODE_HEATSEEKER, MODE_HOOPS, MODE_SOCCAR,
+                                 MODE_THE_VOID, NUM_PADS)
 
 EXACT_CAR = {"flags", "car_contact_other_id", "hit_tick", "hit_extra_tick", "tick_count_since_update",
              "last_controls.jump", "last_controls.boost", "last_controls.handbrake", "last_controls._pad",
@@ -367,13 +368,15 @@ def sample_soccar_melee(ref, n_traj, ticks_per_traj, seed, n_cars=4, stride=1, n
 CHAOTIC_FRACTION = 0.005
 
 
-def explain_outliers(samples, results, mode, bad_idx):
+def explain_outliers(samples, results, mode, bad_idx, cmfs=None):
     """Check the out-of-tolerance samples `bad_idx` against the normalisation
-    mechanism above.  Returns messages for the samples it does not explain."""
+    mechanism above (host builds on the arena's mesh files `cmfs`, None: the
+    one-mesh synthetic arena).  Returns messages for the samples it does not
+    explain."""
     import tick_host
     sub = [samples[i] for i in bad_idx]
-    sse_res, _ = tick_host.run_samples(tick_host.get("sse"), sub, mode)
-    plain_res, _ = tick_host.run_samples(tick_host.get("plain"), sub, mode)
+    sse_res, _ = tick_host.run_samples(tick_host.get("sse"), sub, mode, cmfs=cmfs)
+    plain_res, _ = tick_host.run_samples(tick_host.get("plain"), sub, mode, cmfs=cmfs)
     out = []
     for k, i in enumerate(bad_idx):
         s = samples[i]
@@ -412,11 +415,11 @@ def seeking_direction_rounding(ref_ball, got_ball):
     return abs(na - nb) <= 1e-4 * na and math.acos(cosang) <= 2.05 * UE3_STEP
 
 
-def assert_parity(samples, results, mode, chaotic=False):
+def assert_parity(samples, results, mode, chaotic=False, cmfs=None):
     """Every sample within the tolerances, or (chaotic=True) at most
     CHAOTIC_FRACTION of them out, each explained by the normalisation
-    mechanism (explain_outliers).  Prints and returns the out-of-tolerance
-    count."""
+    mechanism (explain_outliers, on the arena's mesh files `cmfs`).  Prints
+    and returns the out-of-tolerance count."""
     nbad, msgs, bad_idx = check_samples(samples, results, return_indices=True)
     print(f"\n  {nbad}/{len(samples)} samples out of tolerance", flush=True)
     if nbad == 0:
@@ -424,10 +427,77 @@ def assert_parity(samples, results, mode, chaotic=False):
     assert chaotic, f"{nbad}/{len(samples)} samples off\n" + "\n".join(msgs)
     allowed = int(CHAOTIC_FRACTION * len(samples))
     assert nbad <= allowed, f"{nbad}/{len(samples)} samples off (allowed {allowed})\n" + "\n".join(msgs)
-    unexplained = explain_outliers(samples, results, mode, bad_idx)
+    unexplained = explain_outliers(samples, results, mode, bad_idx, cmfs=cmfs)
     assert not unexplained, "\n".join(unexplained + msgs)
     print(f"  all {nbad} explained by the normalisation rounding (host builds)", flush=True)
     return nbad
+
+
+def sample_hoops(ref, n_traj, ticks, seed, stride):
+    """Hoops trajectories (GameMode::HOOPS in the synthetic Hoops arena,
+    rocketsim_amd.arena_mesh.hoops_cmfs), cycling through four scenarios: the
+    kickoff, whose frozen ball is launched 0.265 s after the reset (every tick
+    around the launch is a sample); a ball dropped through a rim into its net
+    (a goal); cars driven under a hoop through its net, which only the ball
+    collides with; cars driven at a ground ball (the Hoops extra hit lift).
+    Every `stride`-th tick is a sample."""
+    rng = random.Random(seed)
+    out = []
+    for k in range(n_traj):
+        kind = ("kickoff", "hoop", "net", "melee")[k % 4]
+        A = ref.RefArena(MODE_HOOPS)
+        n = rng.choice([2, 4])
+        teams = [i % 2 for i in range(n)]
+        presets = [rng.randint(0, 5) for _ in range(n)]
+        for t, p in zip(teams, presets):
+            A.add_car(t, p)
+        A.reset_kickoff(rng.randint(0, 1000))
+        sy = rng.choice([-1, 1])
+        b = A.get_ball()
+        if kind in ("hoop", "net"):
+            b.pos[:] = [rng.uniform(-300, 300), sy * (2900 + rng.uniform(-300, 300)), rng.uniform(500, 900)]
+            b.vel[:] = [rng.gauss(0, 200), rng.gauss(0, 200), rng.uniform(-600, 0)]
+        elif kind == "melee":
+            b.pos[:] = [rng.uniform(-2000, 2000), rng.uniform(-2500, 2500), 96.4]
+            b.vel[:] = [rng.gauss(0, 300), rng.gauss(0, 300), 0]
+        if kind != "kickoff":
+            b.ang_vel[:] = [rng.gauss(0, 2) for _ in range(3)]
+            A.set_ball(b)
+            for i in range(n):
+                s = A.get_car(i)
+                if kind == "melee":
+                    ang, dist = rng.uniform(0, 2 * math.pi), rng.uniform(250, 700)
+                    x, y = b.pos[0] + dist * math.cos(ang), b.pos[1] + dist * math.sin(ang)
+                    yaw = math.atan2(b.pos[1] - y, b.pos[0] - x) + rng.gauss(0, 0.3)
+                else:
+                    x, y = rng.uniform(-500, 500), sy * rng.uniform(1900, 2300)
+                    yaw = math.atan2(sy, 0.0) + rng.gauss(0, 0.3)
+                cy, syw = math.cos(yaw), math.sin(yaw)
+                spd = rng.choice([0, 800, 1500])
+                s.pos[:] = [x, y, 17.0]
+                s.rot[:] = [cy, syw, 0, -syw, cy, 0, 0, 0, 1]
+                s.vel[:] = [spd * cy, spd * syw, 0]
+                s.ang_vel[:] = [0, 0, 0]
+                s.flags = CF_ON_GROUND | 0x1E
+                s.boost = 50
+                A.set_car(i, s)
+        ctrls = [rand_controls(rng) for _ in range(n)]
+        for c in ctrls:
+            c.throttle, c.jump = 1.0, 0
+        hold = [rng.randint(10, 60) for _ in range(n)]
+        for t in range(ticks):
+            for i in range(n):
+                hold[i] -= 1
+                if hold[i] <= 0:
+                    ctrls[i] = rand_controls(rng)
+                    ctrls[i].throttle = 1.0
+                    hold[i] = rng.randint(10, 60)
+            if t % stride == 0 or (kind == "kickoff" and 28 <= t <= 36):
+                out.append(_one_tick(A, MODE_HOOPS, teams, presets, ctrls))
+            else:
+                A.step(1)
+        A.close()
+    return out
 
 
 def sample_heatseeker(ref, n_traj, ticks, seed, stride):
@@ -509,10 +579,18 @@ def sample_from_state(ref, mode, teams, presets, cars, ball, pads, tick, ctrls):
     for i, c in enumerate(cars):
         A.set_car(i, c)
     A.set_ball(ball)
-    if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
-        A.set_pads(pads)
+    if mode in ARENA_MODES:
+        A.set_pads(pads[:len(A.get_pads())])
     A.tick_count = tick
     return _one_tick(A, mode, teams, presets, ctrls)
+
+
+def _arena_pads(A, mode):
+    """The oracle arena's boost pads in the batch's NUM_PADS slots: a Hoops
+    arena has 20 and leaves the rest, like an arena without pads, at their
+    initial state."""
+    pads = A.get_pads() if mode in ARENA_MODES else []
+    return pads + [PadState(1, 0.0, 0) for _ in range(NUM_PADS - len(pads))]
 
 
 def _one_tick(A, mode, teams, presets, ctrls):
@@ -538,7 +616,7 @@ def _one_tick(A, mode, teams, presets, ctrls):
     s.car_order = A.car_order()
     s.cars_in = [A.get_car(i) for i in range(n)]
     s.ball_in = A.get_ball()
-    s.pads_in = A.get_pads() if mode in (MODE_SOCCAR, MODE_HEATSEEKER) else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
+    s.pads_in = _arena_pads(A, mode)
     s.tick_in = A.tick_count
     s.ctrls = [Controls.from_buffer_copy(c) for c in ctrls]
     for i in range(n):
@@ -548,7 +626,7 @@ def _one_tick(A, mode, teams, presets, ctrls):
     s.events = [(e.type, e.tick, e.a, e.b) for e in A.pop_events()]
     s.cars_out = [A.get_car(i) for i in range(n)]
     s.ball_out = A.get_ball()
-    s.pads_out = A.get_pads() if mode in (MODE_SOCCAR, MODE_HEATSEEKER) else [PadState(1, 0.0, 0) for _ in range(NUM_PADS)]
+    s.pads_out = _arena_pads(A, mode)
     s.tick_out = A.tick_count
     return s
 
@@ -626,8 +704,8 @@ def run_on_gpu(samples, mode, kernel=0, per_block=0, auto_reset=False, seed=0, m
     creation (`mutators_first`: before the cars are added) or set after;
     `cmfs` the arena's mesh files (None: the one-mesh synthetic arena)."""
     from rocketsim_amd.batch import ArenaBatch, ensure_meshes
-    if mode != MODE_THE_VOID:
-        ensure_meshes(cmfs)
+    if mode in ARENA_MODES:
+        ensure_meshes(cmfs, mode)
     cpa = [len(s.teams) for s in samples]
     car_configs = None
     if any(getattr(s, "configs", None) for s in samples):
@@ -686,7 +764,7 @@ def check_samples(samples, results, max_report=10, return_indices=False):
         bad += compare_ball(s.ball_out, ball)
         if bp_order(s.ball_out, s.cars_out) != bp_order(ball, cars):
             bad.append(("bp_order", bp_order(s.ball_out, s.cars_out), bp_order(ball, cars), None))
-        if s.mode in (MODE_SOCCAR, MODE_HEATSEEKER):
+        if s.mode in ARENA_MODES:
             bad += compare_pads(s.pads_out, pads)
         if tick != s.tick_out:
             bad.append(("tick", s.tick_out, tick, None))

@@ -51,12 +51,12 @@ def test_init_meshes_refuses_bad_lists():
         pytest.skip("librsg.so not built")
     L = native.lib()
     assert L.rsg_init_meshes(None, None, 0) != 0
-    assert b"between 1 and 28" in L.rsg_last_error()
+    assert b"between 1 and 26" in L.rsg_last_error()
     one = arena_mesh.soccar_cmf()
-    bufs = [C.create_string_buffer(one, len(one))] * 29
-    ptrs = (C.c_void_p * 29)(*[C.cast(b, C.c_void_p) for b in bufs])
-    sizes = (C.c_uint64 * 29)(*([len(one)] * 29))
-    assert L.rsg_init_meshes(ptrs, sizes, 29) != 0
+    bufs = [C.create_string_buffer(one, len(one))] * 27
+    ptrs = (C.c_void_p * 27)(*[C.cast(b, C.c_void_p) for b in bufs])
+    sizes = (C.c_uint64 * 27)(*([len(one)] * 27))
+    assert L.rsg_init_meshes(ptrs, sizes, 27) != 0
     bad = (C.c_void_p * 1)(C.cast(C.create_string_buffer(b"\x01\x00", 2), C.c_void_p))
     assert L.rsg_init_meshes(bad, (C.c_uint64 * 1)(2), 1) != 0
     assert b"mesh file 0" in L.rsg_last_error()
@@ -69,14 +69,14 @@ def test_split_arena_host_build_matches_oracle(tmp_path):
     samples = multi_mesh_case.generate(str(tmp_path / "split.npz"), seed=8101, n_traj=4)
     L = tick_host.get("sse")
     results, _ = tick_host.run_samples(L, samples, MODE_SOCCAR, cmfs=arena_mesh.soccar_split_cmfs())
-    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True)
+    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True, cmfs=arena_mesh.soccar_split_cmfs())
 
 
 def test_split_fixture_host_build():
     samples = parity.load_samples(FIXTURE)
     L = tick_host.get("sse")
     results, _ = tick_host.run_samples(L, samples, MODE_SOCCAR, cmfs=arena_mesh.soccar_split_cmfs())
-    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True)
+    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True, cmfs=arena_mesh.soccar_split_cmfs())
 
 
 @pytest.mark.gpu
@@ -89,7 +89,7 @@ def test_split_arena_fixture_parity(kernel, per_block):
     finally:
         from rocketsim_amd.batch import ensure_meshes
         ensure_meshes(None)
-    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True)
+    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True, cmfs=arena_mesh.soccar_split_cmfs())
 
 
 @pytest.mark.gpu
@@ -103,4 +103,4 @@ def test_split_arena_live_oracle_parity(tmp_path):
     finally:
         from rocketsim_amd.batch import ensure_meshes
         ensure_meshes(None)
-    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True)
+    parity.assert_parity(samples, results, MODE_SOCCAR, chaotic=True, cmfs=arena_mesh.soccar_split_cmfs())

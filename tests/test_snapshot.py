@@ -146,9 +146,12 @@ def test_deserialize_refuses_unsupported(ref):
     rb = A.serialize()
     A.close()
     deserialize_snapshot(rb)   # accepted as is
-    bad_mode = rb[:4] + bytes([1]) + rb[5:]                       # GameMode HOOPS
+    bad_mode = rb[:4] + bytes([3]) + rb[5:]                       # GameMode SNOWDAY
     with pytest.raises(RuntimeError, match="mode"):
         deserialize_snapshot(bad_mode)
+    as_hoops = rb[:4] + bytes([1]) + rb[5:]                       # HOOPS: 20 pads, not 34
+    with pytest.raises(RuntimeError, match="pad count"):
+        deserialize_snapshot(as_hoops)
     with pytest.raises(RuntimeError, match="truncated|count"):
         deserialize_snapshot(rb[:len(rb) // 2])
     # car config: any config is read (hitboxPosOffset.x of the first car

@@ -6,7 +6,8 @@ import ctypes as C
 import os
 import subprocess
 
-from rocketsim_amd.state import (BallState, CarConfig, CarState, Controls, MODE_HEATSEEKER, MODE_SOCCAR, MutatorConfig,
+from rocketsim_amd.state import (ARENA_MODES, BallState, CarConfig, CarState, Controls, MODE_HEATSEEKER, MODE_HOOPS,
+                                 MODE_SOCCAR, MutatorConfig,
                                  NUM_PADS, PadState)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -119,10 +120,10 @@ def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators
     Arena::SetMutatorConfig.  Returns (results, status words)."""
     if cmfs is None and cmf is not None:
         cmfs = [cmf]
-    if mode in (MODE_SOCCAR, MODE_HEATSEEKER) and cmfs is None:
+    if mode in ARENA_MODES and cmfs is None:
         from rocketsim_amd import arena_mesh
-        cmfs = [arena_mesh.soccar_cmf()]
-    if mode not in (MODE_SOCCAR, MODE_HEATSEEKER):
+        cmfs = arena_mesh.hoops_cmfs() if mode == MODE_HOOPS else [arena_mesh.soccar_cmf()]
+    if mode not in ARENA_MODES:
         cmfs = []
     n = len(samples)
     cpa = [len(s.teams) for s in samples]
