@@ -5,7 +5,7 @@ import ctypes as C
 import random
 
 import parity
-from rocketsim_amd.state import ArenaSnapshot, MODE_HEATSEEKER, MODE_SOCCAR
+from rocketsim_amd.state import ARENA_MODES, ArenaSnapshot, MODE_HEATSEEKER, MODE_SOCCAR
 
 # Ball.h:45: DropshotInfo::lastDamageTick has no initializer, so the
 # reference serializes whatever its memory holds.  It is the last field of the
@@ -35,7 +35,7 @@ def snapshot_from_ref(A, presets, mode):
         s.controls[k] = A.get_controls(i)
         s.cars[k] = A.get_car(i)
     s.ball = A.get_ball()
-    if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
+    if mode in ARENA_MODES:
         for p, pd in enumerate(A.get_pads()):
             s.pads[p] = pd
     s.mutators = A.get_mutators()
@@ -53,7 +53,7 @@ def played_arenas(ref, mode, n_arenas, ticks, seed, snap_every):
         presets = [rng.randint(0, 5) for _ in range(4)]
         for i, p in enumerate(presets):
             A.add_car(i % 2, p)
-        if mode in (MODE_SOCCAR, MODE_HEATSEEKER):
+        if mode in ARENA_MODES:
             A.reset_kickoff(rng.randint(0, 1000))
         for t in range(ticks):
             if t % 20 == 0:

@@ -25,7 +25,7 @@ import pytest
 
 import snapshot_cases
 from rocketsim_amd.batch import deserialize_snapshot, serialize_snapshot
-from rocketsim_amd.state import ArenaSnapshot, Controls, MODE_HEATSEEKER, MODE_SOCCAR, MODE_THE_VOID, PRESET_CUSTOM
+from rocketsim_amd.state import ArenaSnapshot, Controls, MODE_HEATSEEKER, MODE_HOOPS, MODE_SOCCAR, MODE_THE_VOID, PRESET_CUSTOM
 
 
 def _oracle():
@@ -70,7 +70,7 @@ def _assert_records_close(x_snap, y_snap):
         assert bytes(x) == bytes(y)
 
 
-@pytest.mark.parametrize("mode", [MODE_SOCCAR, MODE_HEATSEEKER, MODE_THE_VOID])
+@pytest.mark.parametrize("mode", [MODE_SOCCAR, MODE_HOOPS, MODE_HEATSEEKER, MODE_THE_VOID])
 def test_serialize_matches_reference_bytes(ref, mode):
     n = 0
     for A, presets in snapshot_cases.played_arenas(ref, mode, n_arenas=2, ticks=240, seed=11 + mode, snap_every=40):
