@@ -908,7 +908,7 @@ struct World {
         for (int p = 0; p < nb; p++) {
             int c = A.bp_cell[p];
             const int ns = soccar ? num_statics(*geo) : 0;
-            const uint32_t skip = p > 0 ? geo->car_skip : 0u;
+            const uint32_t skip = (soccar && p > 0) ? geo->car_skip : 0u;  // no statics, no geometry
             for (int s = 0; s < ns; s++) {
                 if (!static_in_cell(s, c) || ((skip >> s) & 1u)) continue;
                 V3 smin, smax;

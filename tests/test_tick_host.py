@@ -103,3 +103,39 @@ def test_nonfinite_state_is_flagged_and_stays_in_bounds(tmp_path):
     line = [x for x in r.stdout.splitlines() if x.startswith("STATUS")][0].split()
     assert int(line[1]) & STATUS_NONFINITE and int(line[2]) & STATUS_NONFINITE
     assert int(line[3]) == 0
+
+
+_UB_SCRIPT = r"""
+import os, sys
+sys.path.insert(0, {root!r}); sys.path.insert(0, {tests!r})
+import parity, tick_host
+from rocketsim_amd import arena_mesh
+from rocketsim_amd.state import MODE_HEATSEEKER, MODE_HOOPS, MODE_SOCCAR, MODE_THE_VOID
+L = tick_host.load({so!r})
+for name, mode, cmfs in (("void_tf.npz", MODE_THE_VOID, None), ("soccar_split_tf.npz", MODE_SOCCAR, arena_mesh.soccar_split_cmfs()),
+                         ("hoops_tf.npz", MODE_HOOPS, None), ("heatseeker_tf.npz", MODE_HEATSEEKER, None),
+                         ("soccar_walls_tf.npz", MODE_SOCCAR, None)):
+    samples = parity.load_samples(os.path.join({tests!r}, "golden", name))[:256]
+    tick_host.run_samples(L, samples, mode, cmfs=cmfs)
+print("OK")
+"""
+
+
+@pytest.mark.parametrize("variant", ["sse", "sse_split"])
+def test_undefined_behaviour_sanitizer_on_every_arena_kind(tmp_path, variant):
+    """An UndefinedBehaviorSanitizer + AddressSanitizer build of the tick code
+    (aborting on the first report) steps samples of every arena kind: no
+    statics (THE_VOID), the split Soccar arena, Hoops, Heatseeker.  It caught a
+    read of the arena geometry through a null pointer in an arena without
+    statics, which faulted the GPU."""
+    so = str(tmp_path / f"tick_host_ub_{variant}.so")
+    tick_host.build(so, tick_host.VARIANTS[variant],
+                    ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"],
+                    load_it=False)
+    libs = [subprocess.run(["g++", f"-print-file-name={n}"], capture_output=True, text=True).stdout.strip()
+            for n in ("libasan.so", "libubsan.so")]
+    env = dict(os.environ, LD_PRELOAD=" ".join(libs), ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    code = _UB_SCRIPT.format(root=ROOT, tests=HERE, so=so)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=900)
+    assert r.returncode == 0 and "OK" in r.stdout, r.stdout[-2000:] + r.stderr[-4000:]
