@@ -24,10 +24,13 @@
 namespace rsg {
 
 constexpr int MAX_BODIES = 1 + RSG_MAX_CARS;
-constexpr int MAX_PAIRS = MAX_BODIES * MAX_STATICS + MAX_BODIES * (MAX_BODIES - 1) / 2;
+// Overlapping pairs of a tick: every dynamic pair, and up to 16 statics per
+// body on average (a body's box meets a few of an arena's statics; more pairs
+// set RSG_STATUS_CONTACT_OVERFLOW and are dropped).
+constexpr int MAX_PAIRS = MAX_BODIES * 16 + MAX_BODIES * (MAX_BODIES - 1) / 2;
 
 struct Pair {
-    int a, b;  // body codes, a has the lower unique id (statics < ball < cars)
+    int16_t a, b;  // body codes, a has the lower unique id (statics < ball < cars)
 };
 
 // Bullet's btAlignedObjectArray::quickSortInternal (Hoare partition on the
@@ -122,6 +125,9 @@ struct StepSpace {
 // size times the lanes the device holds in flight (64 per wave, 1024 waves
 // at one wave per SIMD) within the 4 GiB a queue's scratch can address.
 static_assert(sizeof(StepSpace) <= 60 * 1024, "StepSpace must fit a lane's private memory");
+// The plain kernel's lane holds a StepSpace, an ArenaLocal and about 2 KB of
+// other private data within the device's 64 KiB per lane.
+static_assert(sizeof(StepSpace) + sizeof(ArenaLocal) <= 62 * 1024, "the plain kernel's lane must fit 64 KiB");
 
 // An arena's block of a batch's global work space: the world step's work
 // space and the record of a shared narrowphase.
@@ -914,8 +920,9 @@ struct World {
                 V3 smin, smax;
                 static_proxy_aabb(geo, s, smin, smax);
                 if (aabb_overlap(T->aabb_min[p], T->aabb_max[p], smin, smax)) {
-                    T->pairs[T->n_pairs].a = -1 - s;
-                    T->pairs[T->n_pairs].b = p;
+                    if (T->n_pairs >= MAX_PAIRS) { A.status |= RSG_STATUS_CONTACT_OVERFLOW; continue; }
+                    T->pairs[T->n_pairs].a = (int16_t)(-1 - s);
+                    T->pairs[T->n_pairs].b = (int16_t)p;
                     T->n_pairs++;
                 }
             }
@@ -935,8 +942,9 @@ struct World {
                         for (int k = 0; k < T->n_pairs; k++)
                             if (T->pairs[k].a == lo && T->pairs[k].b == hi) found = true;
                         if (found) continue;
-                        T->pairs[T->n_pairs].a = lo;
-                        T->pairs[T->n_pairs].b = hi;
+                        if (T->n_pairs >= MAX_PAIRS) { A.status |= RSG_STATUS_CONTACT_OVERFLOW; continue; }
+                        T->pairs[T->n_pairs].a = (int16_t)lo;
+                        T->pairs[T->n_pairs].b = (int16_t)hi;
                         T->n_pairs++;
                     }
                 }

@@ -32,6 +32,16 @@ namespace rsg {
 
 hipError_t launch_step_plain(const BatchDev& d, const ArenaGeom& geom, int ticks, int grid, int per_block,
                              hipStream_t s) {
+    // A lane's private segment above the device's 64 KiB would abort the
+    // queue at dispatch; refuse the launch instead.
+    static int private_ok = -1;
+    if (private_ok < 0) {
+        hipFuncAttributes attr;
+        if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(rsg_step_arena_kernel)) != hipSuccess)
+            return hipErrorInvalidDeviceFunction;
+        private_ok = attr.localSizeBytes <= 64 * 1024 ? 1 : 0;
+    }
+    if (!private_ok) return hipErrorLaunchOutOfResources;
     hipLaunchKernelGGL(rsg_step_arena_kernel, dim3(grid), dim3(per_block), 0, s, d, geom, ticks);
     return hipGetLastError();
 }
