@@ -826,6 +826,15 @@ struct World {
 
     // Gravity, damping, bounds, broadphase and the manifold plan.
     RSG_FN void step_begin(float dt) {
+        step_prologue(dt);
+        for (int i = 0; i < num_bodies(); i++) body_aabb(i, dt);
+        step_broadphase();
+    }
+
+    // The world step's start in parts: gravity and damping; each body's
+    // proxy AABB (independent per body, so the arena's lanes may share them);
+    // broadphase cells, pairs and the manifold plan.
+    RSG_FN void step_prologue(float dt) {
         const int nb = num_bodies();
         // btRigidBody::applyGravity: each body's own gravity
         for (int i = 0; i < nb; i++) {
@@ -842,19 +851,28 @@ struct World {
             bb.ang *= powf(1.f - 0.f, dt);
         }
         RSG_PROF_MARK(2);
-        // AABBs and broadphase cells
+    }
+
+    // Proxy AABB of body i: its shape at the current and the predicted pose,
+    // merged, +- the contact threshold (btCollisionWorld::updateSingleAabb
+    // over predictIntegratedTransform).
+    RSG_FN void body_aabb(int i, float dt) {
+        Body& b = *body_ptr(i);
+        V3 pp; M3 pr;
+        integrate_transform(b.pos, b.rot, b.vel, b.ang, dt, b.no_rot, pp, pr);
+        V3 mn, mx, mn2, mx2;
+        shape_aabb(i, b.pos, b.rot, mn, mx);
+        V3 t = v3(CONTACT_BREAKING_THRESHOLD, CONTACT_BREAKING_THRESHOLD, CONTACT_BREAKING_THRESHOLD);
+        mn = mn - t; mx = mx + t;
+        shape_aabb(i, pp, pr, mn2, mx2);
+        mn2 = mn2 - t; mx2 = mx2 + t;
+        T->aabb_min[i] = v3(fminf(mn.x, mn2.x), fminf(mn.y, mn2.y), fminf(mn.z, mn2.z));
+        T->aabb_max[i] = v3(fmaxf(mx.x, mx2.x), fmaxf(mx.y, mx2.y), fmaxf(mx.z, mx2.z));
+    }
+
+    RSG_FN void step_broadphase() {
+        const int nb = num_bodies();
         for (int i = 0; i < nb; i++) {
-            Body& b = *body_ptr(i);
-            V3 pp; M3 pr;
-            integrate_transform(b.pos, b.rot, b.vel, b.ang, dt, b.no_rot, pp, pr);
-            V3 mn, mx, mn2, mx2;
-            shape_aabb(i, b.pos, b.rot, mn, mx);
-            V3 t = v3(CONTACT_BREAKING_THRESHOLD, CONTACT_BREAKING_THRESHOLD, CONTACT_BREAKING_THRESHOLD);
-            mn = mn - t; mx = mx + t;
-            shape_aabb(i, pp, pr, mn2, mx2);
-            mn2 = mn2 - t; mx2 = mx2 + t;
-            T->aabb_min[i] = v3(fminf(mn.x, mn2.x), fminf(mn.y, mn2.y), fminf(mn.z, mn2.z));
-            T->aabb_max[i] = v3(fmaxf(mx.x, mx2.x), fmaxf(mx.y, mx2.y), fmaxf(mx.z, mx2.z));
             // btRSBroadphase::setAabb: a proxy whose cell changed is removed
             // from its old cells and appended to the new ones, i.e. it moves
             // to the end of every cell list it is in.
@@ -885,26 +903,29 @@ struct World {
     // The narrowphase's recorded effects (when it was shared: `log`),
     // islands and solver, integration.
     RSG_FN void step_end(float dt, const PairLog* log) {
-        const int nb = num_bodies();
+        step_solve(dt, log);
+        for (int i = 0; i < num_bodies(); i++) body_integrate(i, dt);
+    }
+    RSG_FN void step_solve(float dt, const PairLog* log) {
         if (log) apply_recorded(log);
         RSG_PROF_MARK(4);
         solve_islands(dt);
         RSG_PROF_MARK(5);
-        // integrateTransforms: active bodies
-        for (int i = 0; i < nb; i++) {
-            Body& b = *body_ptr(i);
-            if (!b.active) continue;
+    }
+
+    // integrateTransforms for body i (active bodies only), then
+    // clearForces; independent per body.
+    RSG_FN void body_integrate(int i, float dt) {
+        Body& b = *body_ptr(i);
+        if (b.active) {
             V3 np; M3 nr;
             integrate_transform(b.pos, b.rot, b.vel, b.ang, dt, b.no_rot, np, nr);
             b.pos = np;
             b.rot = nr;
             body_update_inertia(b);
         }
-        for (int i = 0; i < nb; i++) {
-            Body& b = *body_ptr(i);
-            b.force = v3zero();
-            b.torque = v3zero();
-        }
+        b.force = v3zero();
+        b.torque = v3zero();
     }
 
     // btRSBroadphase::calculateOverlappingPairs

@@ -19,6 +19,8 @@ using namespace rsg;
 //     contact callbacks' effects on the arena recorded per pair and applied
 //     by lane 0 in pair order; a pair that needs the penetration pass is
 //     run again there);
+//   * the world step's per-body parts, one body per lane: the proxy AABBs
+//     (with the predicted pose) and the integration of the new poses;
 //   * the car post-ticks, one car per lane, and the boost-pad tests, pads
 //     spread over the lanes.
 // Per-arena LDS block: ArenaLocal prefix, ray results, two flag words.
@@ -67,12 +69,18 @@ rsg_step_arena_coop_kernel(BatchDev d, ArenaGeom geom, int ticks, int stride, in
         __syncthreads();
         if (live && k == 0) {
             pads_pre_tick(A, P);
-            tick_world_begin(A, P, mp, ws);
+            tick_world_prologue(A, P, mp, ws);
         }
+        __syncthreads();
+        if (live) tick_world_aabbs(A, P, mp, ws, k, lanes);
+        __syncthreads();
+        if (live && k == 0) tick_world_broadphase(A, P, mp, ws);
         __syncthreads();
         if (live) tick_world_pairs(A, P, mp, ws, log, k, lanes);
         __syncthreads();
-        if (live && k == 0) tick_world_end(A, P, mp, ws, log);
+        if (live && k == 0) tick_world_solve(A, P, mp, ws, log);
+        __syncthreads();
+        if (live) tick_world_integrate(A, P, mp, ws, k, lanes);
         __syncthreads();
         if (live && k < A.n_cars) car_post_tick(A, P, k);
         __syncthreads();

@@ -755,6 +755,37 @@ RSG_FN void tick_world_end(ArenaLocal& A, const TickParams& P, const ArenaGeom* 
     RSG_PROF_MARK(7);
 }
 
+// The begin and end parts in finer pieces, for lanes that share an arena:
+// the per-body work (proxy AABBs, integration) goes one body per lane
+// (`lane` of `lanes`), the rest stays on one lane, with the arena unchanged
+// by other lanes in between.  In order they equal tick_world_begin and
+// tick_world_end.
+RSG_FN void tick_world_prologue(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws) {
+    ball_pre_tick(A.ball, P.mode, P.dt);
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
+    world.step_prologue(P.dt);
+}
+RSG_FN void tick_world_aabbs(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws, int lane,
+                             int lanes) {
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
+    for (int i = lane; i < world.num_bodies(); i += lanes) world.body_aabb(i, P.dt);
+}
+RSG_FN void tick_world_broadphase(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws) {
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
+    world.step_broadphase();
+}
+RSG_FN void tick_world_solve(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws,
+                             const PairLog* log) {
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
+    world.step_solve(P.dt, log);
+}
+RSG_FN void tick_world_integrate(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, StepSpace* ws, int lane,
+                                 int lanes) {
+    World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
+    for (int i = lane; i < world.num_bodies(); i += lanes) world.body_integrate(i, P.dt);
+    RSG_PROF_MARK(7);
+}
+
 // The rest of Arena::Step after tick_begin, serially.  `pre` holds every
 // wheel's ray result (car-major, 4 per car) when cast ahead, else null.
 RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, const RayHit* pre, StepSpace* ws) {

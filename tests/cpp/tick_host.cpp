@@ -198,10 +198,15 @@ static void tick_split(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo,
         for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], nullptr);
     }
     pads_pre_tick(A, P);
-    tick_world_begin(A, P, geo, ws);
+    tick_world_prologue(A, P, geo, ws);
+    for (int lane = RSG_TEST_SPLIT_WORLD - 1; lane >= 0; lane--)
+        tick_world_aabbs(A, P, geo, ws, lane, RSG_TEST_SPLIT_WORLD);
+    tick_world_broadphase(A, P, geo, ws);
     for (int lane = RSG_TEST_SPLIT_WORLD - 1; lane >= 0; lane--)
         tick_world_pairs(A, P, geo, ws, log, lane, RSG_TEST_SPLIT_WORLD);
-    tick_world_end(A, P, geo, ws, log);
+    tick_world_solve(A, P, geo, ws, log);
+    for (int lane = RSG_TEST_SPLIT_WORLD - 1; lane >= 0; lane--)
+        tick_world_integrate(A, P, geo, ws, lane, RSG_TEST_SPLIT_WORLD);
     for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
     if (mode_has_arena(P.mode))
         for (int p = 0; p < PAD_NUM; p++) pad_check_cars(A, p);
