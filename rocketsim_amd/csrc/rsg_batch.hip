@@ -1059,7 +1059,10 @@ void rsg_game_event_config_default(rsg_game_event_config* out) {
 
 rsg_game_events* rsg_game_events_create(rsg_batch* b, const rsg_game_event_config* cfg, int capacity) {
     if (!b) { fail("rsg_game_events_create: null batch"); return nullptr; }
-    if (b->dev.mode != RSG_MODE_SOCCAR) { fail("rsg_game_events_create: SOCCAR batches only"); return nullptr; }
+    if (b->dev.mode != RSG_MODE_SOCCAR && b->dev.mode != RSG_MODE_HOOPS) {
+        fail("rsg_game_events_create: SOCCAR and HOOPS batches only");
+        return nullptr;
+    }
     if (capacity < 1) { fail("rsg_game_events_create: capacity must be >= 1"); return nullptr; }
     auto* t = new rsg_game_events();
     t->n = b->dev.n;
@@ -1095,7 +1098,7 @@ void rsg_game_events_destroy(rsg_game_events* t) {
 
 int rsg_game_events_update(rsg_game_events* t, rsg_batch* b, void* stream) {
     if (!t || !b) return fail("rsg_game_events_update: null argument");
-    if (b->dev.n != t->n || b->dev.mode != RSG_MODE_SOCCAR)
+    if (b->dev.n != t->n || (b->dev.mode != RSG_MODE_SOCCAR && b->dev.mode != RSG_MODE_HOOPS))
         return fail("rsg_game_events_update: not the batch the tracker was made for");
     if (upload(b) != 0) return -1;
     hipStream_t s = (hipStream_t)stream;

@@ -81,12 +81,70 @@ RSG_FN bool ge_shooter_passer(const BatchDev& d, int a, uint32_t team, uint64_t 
     return shooter >= 0;
 }
 
+// RS_TEAM_FROM_Y (src/Sim/Car/Car.h): the team whose half y is in.
+RSG_FN uint32_t ge_team_from_y(float y) { return y < 0 ? (uint32_t)RSG_TEAM_BLUE : (uint32_t)RSG_TEAM_ORANGE; }
+
+// Arena::IsBallProbablyGoingIn for Hoops (src/Sim/Arena/Arena.cpp:808-885):
+// a falling ball already low inside a hoop, or a ballistic path that clears
+// the rim on the way up and comes down inside a hoop's footprint (with a
+// rough bounce off the end wall).
+RSG_FN bool ge_probably_going_in_hoops(V3 p, V3 v, const rsg_mutator_config& M, uint32_t& goal_team) {
+    const float APPROX_RIM_HEIGHT = 365;
+    float min_height = APPROX_RIM_HEIGHT + M.ball_radius * 1.2f;
+    if (v.z < -FLT_EPS && p.z < min_height) {
+        if (hoops_goal_xy_margin_sq(p.x, p.y) < 0) {
+            goal_team = ge_team_from_y(p.y);
+            return true;
+        }
+    }
+    float margin = M.ball_radius * 1.0f;
+    float margin_sq = margin * margin;
+    float up_t, down_t;
+    {
+        float g = M.gravity[2];
+        if (g > -FLT_EPS) return false;
+        float vz = v.z;
+        float h = p.z - min_height;
+        float sqrt_in = vz * vz - 2 * g * h;
+        if (sqrt_in > 0) {
+            float sq = sqrtf(sqrt_in);
+            up_t = (-vz + sq) / g;
+            down_t = (-vz - sq) / g;
+        } else {
+            // never reaches the rim height: in only if it starts inside a hoop
+            // (the reference leaves the team unset here)
+            return hoops_goal_xy_margin_sq(p.x, p.y) < -margin_sq;
+        }
+    }
+    if (up_t >= 0) {
+        V3 up = p + (v * up_t);
+        float up_margin_sq = hoops_goal_xy_margin_sq(up.x, up.y);
+        float clearance = 60 + M.ball_radius;
+        if (up_margin_sq > -margin_sq && up_margin_sq < (clearance * clearance)) return false;  // the rim
+    }
+    V3 down = p + (v * down_t);
+    down.y = fabsf(down.y);
+    {
+        float wall_y = ARENA_EXTENT_Y_HOOPS - M.ball_radius;
+        if (down.y > wall_y) {
+            float over = down.y - wall_y;
+            down.y -= over * (1 + M.ball_world_restitution);
+        }
+    }
+    if (hoops_goal_xy_margin_sq(down.x, down.y) < -margin_sq) {
+        goal_team = ge_team_from_y(down.y);
+        return true;
+    }
+    return false;
+}
+
 // Arena::IsBallProbablyGoingIn for Soccar (src/Sim/Arena/Arena.cpp:771-807):
 // the ball's ballistic path reaches the goal plane within max_time inside the
-// approximate goal mouth.  goal_team: the team whose net it is.
-RSG_FN bool ge_probably_going_in(V3 pos_bt, V3 vel_bt, float max_time, float extra_margin, const rsg_mutator_config& M,
-                                 uint32_t& goal_team) {
+// approximate goal mouth; Hoops as above.  goal_team: the team whose net it is.
+RSG_FN bool ge_probably_going_in(int mode, V3 pos_bt, V3 vel_bt, float max_time, float extra_margin,
+                                 const rsg_mutator_config& M, uint32_t& goal_team) {
     V3 p = pos_bt * BT_TO_UU, v = vel_bt * BT_TO_UU;
+    if (mode == RSG_MODE_HOOPS) return ge_probably_going_in_hoops(p, v, M, goal_team);
     if (fabsf(v.y) < FLT_EPS) return false;
     float sgn = (float)((v.y > 0) - (v.y < 0));
     float goal_y = M.goal_base_threshold_y * sgn;
@@ -109,7 +167,7 @@ RSG_FN void game_events_update(const BatchDev& d, const GameEventDev& g, const r
     const uint64_t tick = ar.u64(AW_TICK);
     const V3 bpos = br.v(BW_POS), bvel = br.v(BW_VEL);
     const rsg_mutator_config M = d.mut ? d.mut[a] : mutator_defaults(d.mode);
-    const bool scored = fabsf(bpos.y * BT_TO_UU) > (M.goal_base_threshold_y + M.ball_radius);
+    const bool scored = ball_scored(d.mode, bpos, M);
     const float tick_time = d.tick_time;
     const float tickrate = 1.f / tick_time;
     const uint64_t count = br.u64(BW_TICKS_SINCE_SET);
@@ -134,9 +192,9 @@ RSG_FN void game_events_update(const BatchDev& d, const GameEventDev& g, const r
             } else {
                 V3 v = bvel * BT_TO_UU;
                 float speed_sq = v.x * v.x + v.y * v.y + v.z * v.z;
-                uint32_t goal_team = 0;
+                uint32_t goal_team = 0;  // left BLUE where the reference leaves it unset
                 if (speed_sq >= cfg.shot_min_speed * cfg.shot_min_speed &&
-                    ge_probably_going_in(bpos, bvel, cfg.shot_min_score_time, cfg.pred_score_extra_margin, M, goal_team)) {
+                    ge_probably_going_in(d.mode, bpos, bvel, cfg.shot_min_score_time, cfg.pred_score_extra_margin, M, goal_team)) {
                     uint32_t shooter_team = goal_team == RSG_TEAM_BLUE ? (uint32_t)RSG_TEAM_ORANGE : (uint32_t)RSG_TEAM_BLUE;
                     const uint64_t min_delay = (uint64_t)(cfg.shot_touch_min_delay * tickrate);
                     int s, p;
@@ -155,7 +213,7 @@ RSG_FN void game_events_update(const BatchDev& d, const GameEventDev& g, const r
             }
         } else {
             uint32_t unused_team = 0;
-            if (!ge_probably_going_in(bpos, bvel, cfg.shot_min_score_time, cfg.pred_score_extra_margin, M, unused_team)) {
+            if (!ge_probably_going_in(d.mode, bpos, bvel, cfg.shot_min_score_time, cfg.pred_score_extra_margin, M, unused_team)) {
                 // no longer going in: a save if the defending team touched it
                 uint32_t defending = (fl & GE_SHOT_GOAL_ORANGE) ? (uint32_t)RSG_TEAM_ORANGE : (uint32_t)RSG_TEAM_BLUE;
                 int s, p;

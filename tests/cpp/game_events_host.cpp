@@ -29,7 +29,7 @@ struct HostGE {
 extern "C" {
 
 void* ge_create(int n, int cars, const uint8_t* teams, const uint8_t* presets, const rsg_game_event_config* cfg,
-                float tick_rate) {
+                float tick_rate, int mode) {
     auto* h = new HostGE();
     h->n = n;
     h->cars = cars;
@@ -50,7 +50,7 @@ void* ge_create(int n, int cars, const uint8_t* teams, const uint8_t* presets, c
     d.arena = h->arena.data();
     d.n = n;
     d.max_cars = cars;
-    d.mode = RSG_MODE_SOCCAR;
+    d.mode = mode;
     d.tick_time = 1 / tick_rate;
     for (int a = 0; a < n; a++) init_arena_record(d, a, cars, teams + (size_t)a * cars, presets + (size_t)a * cars);
     return h;

@@ -19,7 +19,7 @@ import pytest
 
 import game_event_cases
 from rocketsim_amd.state import (BallState, CarState, Event, EVENT_SAVE, EVENT_SCORER, EVENT_SHOT, GameEventConfig,
-                                 MODE_SOCCAR)
+                                 MODE_HOOPS, MODE_SOCCAR)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -79,7 +79,7 @@ def host_lib(tmp_path_factory):
     L = C.CDLL(so)
     L.ge_create.restype = C.c_void_p
     L.ge_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
-                            C.POINTER(GameEventConfig), C.c_float]
+                            C.POINTER(GameEventConfig), C.c_float, C.c_int]
     L.ge_destroy.argtypes = [C.c_void_p]
     L.ge_update.argtypes = [C.c_void_p, C.POINTER(CarState), C.POINTER(BallState), C.POINTER(C.c_uint64),
                             C.POINTER(C.c_int32)]
@@ -100,7 +100,7 @@ def test_host_tracker_matches_reference(reference_runs, host_lib, every):
     n = len(ref_events)
     L = host_lib
     cfg = GameEventConfig()
-    h = L.ge_create(n, 4, (C.c_uint8 * (4 * n))(*teams), (C.c_uint8 * (4 * n))(*presets), C.byref(cfg), 120.0)
+    h = L.ge_create(n, 4, (C.c_uint8 * (4 * n))(*teams), (C.c_uint8 * (4 * n))(*presets), C.byref(cfg), 120.0, MODE_SOCCAR)
     got = []
     try:
         for cars, balls, ticks, orders in updates:
@@ -165,3 +165,64 @@ def test_gpu_tracker_on_a_stepping_batch():
     finally:
         tr.close()
         b.close()
+
+
+@pytest.fixture(scope="module")
+def hoops_runs():
+    ref = _oracle()
+    if ref is None:
+        pytest.skip("oracle library not built")
+    return game_event_cases.run_reference(ref, n_arenas=6, ticks=1200, update_every=1, seed=71, mode=MODE_HOOPS)
+
+
+def test_hoops_scenarios_have_shots_and_goals(hoops_runs):
+    """Lobs at the hoops (Arena::IsBallProbablyGoingIn's Hoops branch,
+    Arena.cpp:808-885): the reference's tracker reports shots and goals."""
+    kinds = _kinds(hoops_runs[3])
+    assert kinds[(EVENT_SHOT, False)] + kinds[(EVENT_SHOT, True)] > 0
+    assert kinds[(EVENT_SCORER, False)] + kinds[(EVENT_SCORER, True)] > 0
+
+
+def test_host_tracker_matches_reference_hoops(hoops_runs, host_lib):
+    teams, presets, updates, ref_events = hoops_runs
+    n = len(ref_events)
+    L = host_lib
+    cfg = GameEventConfig()
+    h = L.ge_create(n, 4, (C.c_uint8 * (4 * n))(*teams), (C.c_uint8 * (4 * n))(*presets), C.byref(cfg), 120.0,
+                    MODE_HOOPS)
+    got = []
+    try:
+        for cars, balls, ticks, orders in updates:
+            L.ge_update(h, (CarState * (4 * n))(*[c for per in cars for c in per]), (BallState * n)(*balls),
+                        (C.c_uint64 * n)(*ticks), (C.c_int32 * (4 * n))(*[o for per in orders for o in per]))
+            buf = (Event * 1024)()
+            got.extend(buf[:L.ge_pop(h, buf, 1024)])
+    finally:
+        L.ge_destroy(h)
+    want = [[game_event_cases.event_key(e) for e in per] for per in ref_events]
+    assert _by_arena(got, n) == want
+
+
+@pytest.mark.gpu
+def test_gpu_tracker_matches_reference_hoops(hoops_runs):
+    from rocketsim_amd.batch import ArenaBatch
+    from rocketsim_amd.game_events import GameEventTracker
+    teams, presets, updates, ref_events = hoops_runs
+    n = len(ref_events)
+    b = ArenaBatch([4] * n, teams, presets, mode=MODE_HOOPS)
+    tr = GameEventTracker(b)
+    got = []
+    try:
+        for cars, balls, ticks, orders in updates:
+            b.set_cars([c for per in cars for c in per])
+            b.set_balls(balls)
+            b.set_tick_counts(ticks)
+            b.set_car_order([o for per in orders for o in per])
+            tr.update(b)
+            got.extend(tr.poll())
+        assert tr.lost == 0
+    finally:
+        tr.close()
+        b.close()
+    want = [[game_event_cases.event_key(e) for e in per] for per in ref_events]
+    assert _by_arena(got, n) == want
