@@ -28,7 +28,9 @@ inline void check(int rc) {
     if (rc < 0) throw std::runtime_error(std::string("rsg: ") + rsg_last_error());
 }
 
-enum class GameMode : int { SOCCAR = RSG_MODE_SOCCAR, HEATSEEKER = RSG_MODE_HEATSEEKER, THE_VOID = RSG_MODE_THE_VOID };
+enum class GameMode : int {
+    SOCCAR = RSG_MODE_SOCCAR, HOOPS = RSG_MODE_HOOPS, HEATSEEKER = RSG_MODE_HEATSEEKER, THE_VOID = RSG_MODE_THE_VOID
+};
 enum class Team : uint8_t { BLUE = RSG_TEAM_BLUE, ORANGE = RSG_TEAM_ORANGE };
 
 using CarState = rsg_car_state;
@@ -46,15 +48,16 @@ inline MutatorConfig DefaultMutatorConfig(GameMode mode) {
 
 // RocketSim::InitFromMem (src/RocketSim.h:32) for the Soccar collision mesh.
 inline void InitFromMem(const void* cmf_data, uint64_t size) { check(rsg_init_mesh(cmf_data, size)); }
-// RocketSim::InitFromMem with the Soccar arena's list of mesh files, in order.
-inline void InitFromMem(const std::vector<std::vector<uint8_t>>& cmf_files) {
+// RocketSim::InitFromMem with a mode's list of mesh files, in order (the
+// Soccar arena by default; GameMode::HOOPS for the Hoops arena).
+inline void InitFromMem(const std::vector<std::vector<uint8_t>>& cmf_files, GameMode mode = GameMode::SOCCAR) {
     std::vector<const void*> data;
     std::vector<uint64_t> sizes;
     for (const auto& f : cmf_files) {
         data.push_back(f.data());
         sizes.push_back(f.size());
     }
-    check(rsg_init_meshes(data.data(), sizes.data(), (int)data.size()));
+    check(rsg_init_arena_meshes((int)mode, data.data(), sizes.data(), (int)data.size()));
 }
 
 using CarConfig = rsg_car_config;
