@@ -40,7 +40,11 @@ BYTES_PER_ARENA_TICK_2V2 = 2488
 # HBM bytes per launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE passes
 # over `bench.py --profile-probe` (counter factor from a 64 MiB copy), keyed by
 # (kernel, arenas, ticks per launch, arenas per workgroup).
-HBM_TRAFFIC_PER_LAUNCH = {}
+HBM_TRAFFIC_PER_LAUNCH = {
+    # profiles/rsg_step_arena_coop_kernel_soccar2v2_16384.md: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
+    # the mean of the probe's two 8-tick launches after the 240-tick pre-roll
+    ("rsg_step_arena_coop_kernel", 16384, 8, 4): (2715608 + 1446796) * 1024,
+}
 
 
 def _host_info():
