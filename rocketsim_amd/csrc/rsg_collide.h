@@ -300,7 +300,6 @@ struct World {
                 if (ray_plane(static_plane(*geo, s), from, to, frac, n)) hit_obj = -1 - s;
             }
         }
-        RSG_PROF_MARK(10);
         // then the dynamic proxies registered in the cell (cell-list order),
         // except the casting car (btCollisionWorld::RayResultCallback::needsCollision)
         const int nb = num_bodies();
@@ -336,7 +335,6 @@ struct World {
                 }
             }
         }
-        RSG_PROF_MARK(9);
         if (hit_obj != -100) {
             // btDefaultVehicleRaycaster::castRay: the closest hit must respond
             if (hit_obj >= 0 && !ray_target_responds(hit_obj)) return h;
@@ -850,7 +848,6 @@ struct World {
             bb.vel *= powf(1.f - drag, dt);
             bb.ang *= powf(1.f - 0.f, dt);
         }
-        RSG_PROF_MARK(2);
     }
 
     // Proxy AABB of body i: its shape at the current and the predicted pose,
@@ -897,7 +894,6 @@ struct World {
         for (int r = 0; r < nb; r++) A.bp_seq[T->order[r]] = (uint32_t)r;
         find_pairs();
         plan_pairs();
-        RSG_PROF_MARK(3);
     }
 
     // The narrowphase's recorded effects (when it was shared: `log`),
@@ -908,9 +904,7 @@ struct World {
     }
     RSG_FN void step_solve(float dt, const PairLog* log) {
         if (log) apply_recorded(log);
-        RSG_PROF_MARK(4);
         solve_islands(dt);
-        RSG_PROF_MARK(5);
     }
 
     // integrateTransforms for body i (active bodies only), then
@@ -1192,7 +1186,6 @@ struct World {
             n_im++;
         }
         if (n_im > 1) quicksort_by_key(im, ikey, 0, n_im - 1);
-        RSG_PROF_MARK(11);
         // collect bodies of non-sleeping islands, in island order
         int* bodies = I.bodies;
         int n_bodies = 0;
@@ -1233,13 +1226,7 @@ struct World {
             int k = bodies[i];
             if (S.special[k].num > 0) convert_special(S, k, body_at);
         }
-        RSG_PROF_MARK(12);
-        RSG_PROF_COUNT(26, S.n_rows);
-        RSG_PROF_COUNT(27, S.n_fric);
-        RSG_PROF_COUNT(28, T->n_mf);
-        RSG_PROF_COUNT(29, S.n_bodies);
         solver_iterate(S);
-        RSG_PROF_MARK(13);
         // write back (btSequentialImpulseConstraintSolver::writeBackBodies)
         for (int i = 0; i < S.n_bodies; i++) {
             SBody& sb = S.bodies[i];

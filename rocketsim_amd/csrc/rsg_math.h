@@ -17,14 +17,6 @@
 #define RSG_FN inline
 #endif
 
-// Phase marks and counters for an instrumented build (no-ops otherwise).
-#ifndef RSG_PROF_MARK
-#define RSG_PROF_MARK(id)
-#endif
-#ifndef RSG_PROF_COUNT
-#define RSG_PROF_COUNT(id, v)
-#endif
-
 namespace rsg {
 
 struct V3 {

@@ -29,9 +29,7 @@ RSG_FN PairLog* arena_log(const BatchDev& d, int a) {
 }
 
 RSG_FN void step_arena(const BatchDev& d, const ArenaGeom& geom, int a, int ticks, ArenaLocal& A, StepSpace* ws) {
-    RSG_PROF_MARK(15);
     load_arena(d, a, A);
-    RSG_PROF_MARK(15);
     TickParams P;
     P.mode = d.mode;
     P.dt = d.tick_time;

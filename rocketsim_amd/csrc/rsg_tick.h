@@ -472,10 +472,8 @@ RSG_FN void car_pre_tick(ArenaLocal& A, const TickParams& P, World& world, int c
     // btVehicleRL's wheel records: rebuilt every tick, used only in this pre-tick
     WheelT wh[4];
     for (int i = 0; i < 4; i++) wheel_update_transform(c, cp, i, wh[i]);
-    RSG_PROF_MARK(1);
     for (int i = 0; i < 4; i++) {
         vehicle_ray_cast(c, wh, cp, i, world, 1 + ci, A.si, pre ? &pre[ci * 4 + i] : nullptr);
-        RSG_PROF_MARK(6);
     }
     vehicle_calc_friction(c, wh, world, dt);
 
@@ -639,7 +637,6 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
         if (!(isfinite(s.x) && isfinite(s.y) && isfinite(s.z))) A.status |= RSG_STATUS_NONFINITE;
     }
     if (scored && P.auto_reset) arena_kickoff_reset(A, P.seed, P.arena_id);
-    RSG_PROF_MARK(8);
 }
 
 // Angle::FromVec (src/Math/MathTypes/MathTypes.cpp:80-100): yaw and pitch
@@ -730,7 +727,6 @@ RSG_FN void tick_world(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo,
     ball_pre_tick(A.ball, P.mode, P.dt);
     World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     world.step(P.dt);
-    RSG_PROF_MARK(7);
 }
 
 // The same world step in three parts for an arena whose lanes share the
@@ -752,7 +748,6 @@ RSG_FN void tick_world_end(ArenaLocal& A, const TickParams& P, const ArenaGeom* 
                            const PairLog* log) {
     World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     world.step_end(P.dt, log);
-    RSG_PROF_MARK(7);
 }
 
 // The begin and end parts in finer pieces, for lanes that share an arena:
@@ -783,7 +778,6 @@ RSG_FN void tick_world_integrate(ArenaLocal& A, const TickParams& P, const Arena
                                  int lanes) {
     World world(A, geo, mode_has_arena(P.mode) && geo != nullptr, ws);
     for (int i = lane; i < world.num_bodies(); i += lanes) world.body_integrate(i, P.dt);
-    RSG_PROF_MARK(7);
 }
 
 // The rest of Arena::Step after tick_begin, serially.  `pre` holds every
@@ -791,11 +785,9 @@ RSG_FN void tick_world_integrate(ArenaLocal& A, const TickParams& P, const Arena
 RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, const RayHit* pre, StepSpace* ws) {
     {
         World world(A, geo, mode_has_arena(P.mode) && geo != nullptr);
-        RSG_PROF_MARK(0);
         // 2. car pre-tick, in the arena's car update order
         for (int oi = 0; oi < A.n_cars; oi++) car_pre_tick(A, P, world, A.order[oi], pre);
     }
-    RSG_PROF_MARK(1);
     // 3. pads pre-tick
     pads_pre_tick(A, P);
     // 4. world step
