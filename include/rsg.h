@@ -47,6 +47,10 @@ typedef struct rsg_batch_desc {
     /* each car's CarConfig (AddCar(team, config), Arena.h:71), one per car;
      * NULL for the presets of car_preset */
     const rsg_car_config* car_configs;
+    /* ArenaConfig::noBallRot == false (src/Sim/Arena/ArenaConfig/ArenaConfig.h,
+     * Ball.cpp:109): the ball's pose rotates with its angular velocity; 0 (the
+     * reference's default): it keeps its rotation */
+    int ball_rotation;
 } rsg_batch_desc;
 
 /* Last error message of the calling thread ("" if none). */

@@ -81,6 +81,7 @@ struct BatchConfig {
     uint64_t seed = 0;
     bool auto_reset = false;      // kickoff reset after a goal
     bool synth_controls = false;  // draw controls on the device (include/rsg_workload.h)
+    bool ball_rotation = false;   // ArenaConfig::noBallRot == false: the ball's pose rotates
     // every arena's MutatorConfig, set before its cars are added (none: the
     // mode's defaults)
     std::optional<MutatorConfig> mutators;
@@ -112,6 +113,7 @@ public:
         d.seed = cfg.seed;
         d.auto_reset = cfg.auto_reset ? 1 : 0;
         d.synth_controls = cfg.synth_controls ? 1 : 0;
+        d.ball_rotation = cfg.ball_rotation ? 1 : 0;
         d.cars_per_arena = counts.data();
         d.car_team = teams.data();
         d.car_preset = presets.data();

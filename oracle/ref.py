@@ -33,6 +33,8 @@ def lib():
                                        C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_int]
         L.rsref_arena_create.restype = C.c_void_p
         L.rsref_arena_create.argtypes = [C.c_int, C.c_float]
+        L.rsref_arena_create_cfg.restype = C.c_void_p
+        L.rsref_arena_create_cfg.argtypes = [C.c_int, C.c_float, C.c_int]
         L.rsref_arena_destroy.argtypes = [C.c_void_p]
         L.rsref_add_car.restype = C.c_uint32
         L.rsref_add_car.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -133,9 +135,9 @@ def init(meshes=None, hoops_meshes=None):
 class RefArena:
     """One reference Arena with cars added in a fixed order."""
 
-    def __init__(self, mode, tick_rate=120.0):
+    def __init__(self, mode, tick_rate=120.0, no_ball_rot=True):
         L = lib()
-        self.h = L.rsref_arena_create(mode, tick_rate)
+        self.h = L.rsref_arena_create_cfg(mode, tick_rate, 1 if no_ball_rot else 0)
         if not self.h:
             raise RuntimeError(L.rsref_last_error().decode())
         self.L = L

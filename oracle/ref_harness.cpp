@@ -129,10 +129,12 @@ int rsref_init(const uint8_t* const* blobs, const uint64_t* sizes, int n) {
     return rsref_init_modes(blobs, sizes, n, nullptr, nullptr, 0);
 }
 
-void* rsref_arena_create(int mode, float tick_rate) {
+void* rsref_arena_create_cfg(int mode, float tick_rate, int no_ball_rot) {
     try {
         auto* ra = new RefArena();
-        ra->arena = Arena::Create((GameMode)mode, ArenaConfig(), tick_rate);
+        ArenaConfig cfg;
+        cfg.noBallRot = no_ball_rot != 0;
+        ra->arena = Arena::Create((GameMode)mode, cfg, tick_rate);
         // The solver info's timestep is only written by the first world step
         // (btDiscreteDynamicsWorld.cpp:414); before that it holds Bullet's
         // default 1/60, which the wheel pushback of the very first tick reads
@@ -167,6 +169,8 @@ void* rsref_arena_create(int mode, float tick_rate) {
         return nullptr;
     }
 }
+
+void* rsref_arena_create(int mode, float tick_rate) { return rsref_arena_create_cfg(mode, tick_rate, 1); }
 
 void rsref_arena_destroy(void* h) {
     auto* ra = (RefArena*)h;

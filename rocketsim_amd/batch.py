@@ -79,7 +79,8 @@ def deserialize_snapshot(data):
 
 class ArenaBatch:
     def __init__(self, cars_per_arena, teams, presets, mode=MODE_SOCCAR, tick_rate=120.0,
-                 seed=0, auto_reset=False, synth_controls=False, mutators=None, car_configs=None):
+                 seed=0, auto_reset=False, synth_controls=False, mutators=None, car_configs=None,
+                 ball_rotation=False):
         """`presets`: RSG_PRESET_* per car, or None with `car_configs` (one
         CarConfig per car); `mutators`: every arena's MutatorConfig, set
         before its cars are added."""
@@ -102,7 +103,7 @@ class ArenaBatch:
                       auto_reset=int(auto_reset), synth_controls=int(synth_controls),
                       cars_per_arena=self._cpa, car_team=self._team, car_preset=self._preset,
                       mutators=C.pointer(mutators) if mutators is not None else None,
-                      car_configs=self._cfgs)
+                      car_configs=self._cfgs, ball_rotation=int(ball_rotation))
         h = L.rsg_batch_create(C.byref(d))
         if not h:
             raise RuntimeError(L.rsg_last_error().decode())

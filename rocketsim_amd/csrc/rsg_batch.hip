@@ -221,6 +221,7 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     b->dev.mode = desc->mode;
     b->dev.auto_reset = desc->auto_reset;
     b->dev.synth_controls = desc->synth_controls;
+    b->dev.ball_rot = desc->ball_rotation != 0;
     b->dev.seed = desc->seed;
     b->dev.tick_time = 1 / desc->tick_rate;
     int maxc = 0, total = 0;

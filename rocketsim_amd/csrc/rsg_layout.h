@@ -118,6 +118,7 @@ struct BatchDev {
     int mode;          // RSG_MODE_*
     int auto_reset;    // kickoff reset after a goal
     int synth_controls;// draw controls from the seeded process each tick
+    int ball_rot;      // the ball integrates its rotation (ArenaConfig::noBallRot == false)
     uint64_t seed;
     float tick_time;
     // event ring (null when events are off): append index from *ev_count

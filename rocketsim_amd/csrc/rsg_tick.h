@@ -229,6 +229,7 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.ball.radius_uu = br.f(BW_RADIUS);
     A.ball.mass = br.f(BW_MASS);
     ball_init_body(A.ball);
+    A.ball.b.no_rot = d.ball_rot == 0;  // btRigidBody::m_noRot (Ball.cpp:109)
     A.ball.b.active = br.u(BW_AWAKE) != 0;
     A.ball.b.grav = br.v(BW_GRAVITY);
     A.ball.hs_dir = br.f(BW_HS);

@@ -196,6 +196,7 @@ class BatchDesc(C.Structure):
         ("car_preset", C.POINTER(C.c_uint8)),
         ("mutators", C.POINTER(MutatorConfig)),
         ("car_configs", C.POINTER(CarConfig)),
+        ("ball_rotation", C.c_int),
     ]
 
 

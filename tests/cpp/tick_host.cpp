@@ -214,6 +214,9 @@ static void tick_split(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo,
 }
 #endif
 
+// ArenaConfig::noBallRot == false for every arena (the batch's ball_rotation).
+void th_set_ball_rotation(void* p, int on) { ((HostBatch*)p)->dev.ball_rot = on != 0; }
+
 // Steps every arena `ticks` ticks with the plain kernel's code path (or the
 // split world step, RSG_TEST_SPLIT_WORLD).
 void th_step(void* p, int ticks) {

@@ -60,6 +60,7 @@ def load(path):
     L.th_set_tick_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.th_get_tick_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.th_set_car_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    L.th_set_ball_rotation.argtypes = [C.c_void_p, C.c_int]
     L.th_get_status.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
     L.th_step.argtypes = [C.c_void_p, C.c_int]
     L.th_rsqrt_signature.argtypes = [C.POINTER(C.c_float)]
@@ -112,7 +113,8 @@ def mesh_args(cmfs):
     return ptrs, (C.c_uint64 * max(n, 1))(*[len(c) for c in cmfs]), n
 
 
-def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators_first=False, cmfs=None):
+def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators_first=False, cmfs=None,
+                ball_rotation=False):
     """Step every teacher-forced sample once on the host build, one sample per
     arena (the host counterpart of parity.run_on_gpu).  `inject(cars, balls)`
     may edit the "before" records; `mutators` is every arena's MutatorConfig,
@@ -139,6 +141,7 @@ def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators
     try:
         if mutators is not None and not mutators_first:
             L.th_set_mutators(h, C.byref(mutators))
+        L.th_set_ball_rotation(h, 1 if ball_rotation else 0)
         cars = (CarState * max(total, 1))(*sum((list(s.cars_in) for s in samples), []))
         balls = (BallState * n)(*[s.ball_in for s in samples])
         if inject:
