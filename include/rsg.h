@@ -51,6 +51,12 @@ typedef struct rsg_batch_desc {
      * Ball.cpp:109): the ball's pose rotates with its angular velocity; 0 (the
      * reference's default): it keeps its rotation */
     int ball_rotation;
+    /* ArenaConfig::useCustomBoostPads / customBoostPads (ArenaConfig.h,
+     * Arena.cpp:501-508): n_custom_pads (1..34) pads in place of the mode's,
+     * in this order; NULL / 0 for the mode's pads.  Every car is tested
+     * against every custom pad (Arena.cpp:724-728). */
+    const rsg_boost_pad_config* custom_pads;
+    int n_custom_pads;
 } rsg_batch_desc;
 
 /* Last error message of the calling thread ("" if none). */

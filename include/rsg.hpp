@@ -82,6 +82,7 @@ struct BatchConfig {
     bool auto_reset = false;      // kickoff reset after a goal
     bool synth_controls = false;  // draw controls on the device (include/rsg_workload.h)
     bool ball_rotation = false;   // ArenaConfig::noBallRot == false: the ball's pose rotates
+    std::vector<rsg_boost_pad_config> custom_pads;  // ArenaConfig::customBoostPads (empty: the mode's)
     // every arena's MutatorConfig, set before its cars are added (none: the
     // mode's defaults)
     std::optional<MutatorConfig> mutators;
@@ -114,6 +115,8 @@ public:
         d.auto_reset = cfg.auto_reset ? 1 : 0;
         d.synth_controls = cfg.synth_controls ? 1 : 0;
         d.ball_rotation = cfg.ball_rotation ? 1 : 0;
+        d.custom_pads = cfg.custom_pads.empty() ? nullptr : cfg.custom_pads.data();
+        d.n_custom_pads = (int)cfg.custom_pads.size();
         d.cars_per_arena = counts.data();
         d.car_team = teams.data();
         d.car_preset = presets.data();

@@ -67,6 +67,14 @@ typedef struct rsg_car_config {
 
 enum { RSG_TEAM_BLUE = 0, RSG_TEAM_ORANGE = 1 };
 
+/* BoostPadConfig (src/Sim/BoostPad/BoostPad.h:13-19): a custom pad of
+ * ArenaConfig::customBoostPads. */
+typedef struct rsg_boost_pad_config {
+    float pos[3];     /* uu */
+    uint8_t is_big;
+    uint8_t _pad[3];
+} rsg_boost_pad_config;
+
 typedef struct rsg_controls {
     float throttle, steer, pitch, yaw, roll;
     uint8_t jump, boost, handbrake, _pad;

@@ -34,7 +34,7 @@ def lib():
         L.rsref_arena_create.restype = C.c_void_p
         L.rsref_arena_create.argtypes = [C.c_int, C.c_float]
         L.rsref_arena_create_cfg.restype = C.c_void_p
-        L.rsref_arena_create_cfg.argtypes = [C.c_int, C.c_float, C.c_int]
+        L.rsref_arena_create_cfg.argtypes = [C.c_int, C.c_float, C.c_int, C.POINTER(C.c_float), C.c_int]
         L.rsref_arena_destroy.argtypes = [C.c_void_p]
         L.rsref_add_car.restype = C.c_uint32
         L.rsref_add_car.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -135,9 +135,12 @@ def init(meshes=None, hoops_meshes=None):
 class RefArena:
     """One reference Arena with cars added in a fixed order."""
 
-    def __init__(self, mode, tick_rate=120.0, no_ball_rot=True):
+    def __init__(self, mode, tick_rate=120.0, no_ball_rot=True, custom_pads=None):
+        """custom_pads: ArenaConfig::customBoostPads as [(x, y, z, is_big), ...]."""
         L = lib()
-        self.h = L.rsref_arena_create_cfg(mode, tick_rate, 1 if no_ball_rot else 0)
+        flat = [float(v) for p in (custom_pads or []) for v in (p[0], p[1], p[2], 1.0 if p[3] else 0.0)]
+        buf = (C.c_float * max(len(flat), 1))(*flat)
+        self.h = L.rsref_arena_create_cfg(mode, tick_rate, 1 if no_ball_rot else 0, buf, len(flat) // 4)
         if not self.h:
             raise RuntimeError(L.rsref_last_error().decode())
         self.L = L

@@ -129,11 +129,21 @@ int rsref_init(const uint8_t* const* blobs, const uint64_t* sizes, int n) {
     return rsref_init_modes(blobs, sizes, n, nullptr, nullptr, 0);
 }
 
-void* rsref_arena_create_cfg(int mode, float tick_rate, int no_ball_rot) {
+// pads: n_pads custom boost pads as (x, y, z, is_big) (ArenaConfig::customBoostPads), or none.
+void* rsref_arena_create_cfg(int mode, float tick_rate, int no_ball_rot, const float* pads, int n_pads) {
     try {
         auto* ra = new RefArena();
         ArenaConfig cfg;
         cfg.noBallRot = no_ball_rot != 0;
+        if (n_pads > 0) {
+            cfg.useCustomBoostPads = true;
+            for (int i = 0; i < n_pads; i++) {
+                BoostPadConfig pc;
+                pc.pos = Vec(pads[4 * i], pads[4 * i + 1], pads[4 * i + 2]);
+                pc.isBig = pads[4 * i + 3] != 0.f;
+                cfg.customBoostPads.push_back(pc);
+            }
+        }
         ra->arena = Arena::Create((GameMode)mode, cfg, tick_rate);
         // The solver info's timestep is only written by the first world step
         // (btDiscreteDynamicsWorld.cpp:414); before that it holds Bullet's
@@ -170,7 +180,7 @@ void* rsref_arena_create_cfg(int mode, float tick_rate, int no_ball_rot) {
     }
 }
 
-void* rsref_arena_create(int mode, float tick_rate) { return rsref_arena_create_cfg(mode, tick_rate, 1); }
+void* rsref_arena_create(int mode, float tick_rate) { return rsref_arena_create_cfg(mode, tick_rate, 1, nullptr, 0); }
 
 void rsref_arena_destroy(void* h) {
     auto* ra = (RefArena*)h;

@@ -8,7 +8,7 @@ the get/set methods replace Car/Ball/BoostPad Get/SetState.
 import ctypes as C
 
 from . import native
-from .state import (ARENA_MODES, ArenaSnapshot, BallState, BatchDesc, CarState, Controls, Event, MODE_HEATSEEKER,
+from .state import (ARENA_MODES, ArenaSnapshot, BallState, BatchDesc, BoostPadConfig, CarState, Controls, Event, MODE_HEATSEEKER,
                     MODE_HOOPS, MODE_SOCCAR, MODE_THE_VOID,
                     CarConfig, MutatorConfig, NUM_PADS, PadState)
 
@@ -80,7 +80,7 @@ def deserialize_snapshot(data):
 class ArenaBatch:
     def __init__(self, cars_per_arena, teams, presets, mode=MODE_SOCCAR, tick_rate=120.0,
                  seed=0, auto_reset=False, synth_controls=False, mutators=None, car_configs=None,
-                 ball_rotation=False):
+                 ball_rotation=False, custom_pads=None):
         """`presets`: RSG_PRESET_* per car, or None with `car_configs` (one
         CarConfig per car); `mutators`: every arena's MutatorConfig, set
         before its cars are added."""
@@ -104,6 +104,12 @@ class ArenaBatch:
                       cars_per_arena=self._cpa, car_team=self._team, car_preset=self._preset,
                       mutators=C.pointer(mutators) if mutators is not None else None,
                       car_configs=self._cfgs, ball_rotation=int(ball_rotation))
+        if custom_pads:
+            # ArenaConfig::customBoostPads: [(x, y, z, is_big), ...]
+            self._custom = (BoostPadConfig * len(custom_pads))(
+                *[BoostPadConfig((C.c_float * 3)(x, y, z), int(bool(big))) for x, y, z, big in custom_pads])
+            d.custom_pads = self._custom
+            d.n_custom_pads = len(custom_pads)
         h = L.rsg_batch_create(C.byref(d))
         if not h:
             raise RuntimeError(L.rsg_last_error().decode())

@@ -14,6 +14,8 @@ struct ArenaLocal {
     int n_cars;
     int mode;           // RSG_MODE_*
     rsg_mutator_config mut;  // Arena::_mutatorConfig
+    const rsg_boost_pad_config* custom_pads;  // the batch's custom pads (null: the mode's)
+    int n_pads;         // boost pads in use
     Ball ball;
     Pad pads[PAD_NUM];
     uint64_t tick;

@@ -128,6 +128,8 @@ struct rsg_batch {
     std::vector<rsg_mutator_config> h_mut;  // per-arena MutatorConfig (the kernels only read it)
     std::vector<rsg_car_config> h_cfgs;     // car config table: the presets, then other configs
     rsg_car_config* d_cfgs = nullptr;
+    std::vector<rsg_boost_pad_config> h_custom_pads;  // ArenaConfig::customBoostPads
+    rsg_boost_pad_config* d_custom_pads = nullptr;
     bool host_dirty = false;  // host mirror newer than device
     bool dev_newer = false;   // device newer than host mirror
     hipStream_t last_stream = nullptr;
@@ -157,6 +159,7 @@ BatchDev host_view(rsg_batch* b) {
     h.arena = b->h_arena.data();
     h.mut = b->h_mut.data();
     h.cfgs = b->h_cfgs.data();
+    h.custom_pads = b->h_custom_pads.empty() ? nullptr : b->h_custom_pads.data();
     return h;
 }
 
@@ -222,6 +225,23 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     b->dev.auto_reset = desc->auto_reset;
     b->dev.synth_controls = desc->synth_controls;
     b->dev.ball_rot = desc->ball_rotation != 0;
+    if (desc->n_custom_pads != 0) {
+        std::string why;
+        if (!mode_has_arena(desc->mode)) why = "the mode has no boost pads";
+        else if (!desc->custom_pads || desc->n_custom_pads < 0 || desc->n_custom_pads > RSG_NUM_PADS)
+            why = "custom_pads must hold 1.." + std::to_string(RSG_NUM_PADS) + " pads";
+        else
+            for (int i = 0; i < desc->n_custom_pads && why.empty(); i++)
+                for (float x : desc->custom_pads[i].pos)
+                    if (!std::isfinite(x)) why = "non-finite custom pad position";
+        if (!why.empty()) {
+            delete b;
+            fail("rsg_batch_create: " + why);
+            return nullptr;
+        }
+        b->h_custom_pads.assign(desc->custom_pads, desc->custom_pads + desc->n_custom_pads);
+        for (auto& c : b->h_custom_pads) c.is_big = c.is_big != 0, c._pad[0] = c._pad[1] = c._pad[2] = 0;
+    }
     b->dev.seed = desc->seed;
     b->dev.tick_time = 1 / desc->tick_rate;
     int maxc = 0, total = 0;
@@ -297,7 +317,11 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
         (e = hipMalloc(&dmut, (size_t)b->dev.n * sizeof(rsg_mutator_config))) != hipSuccess ||
         (e = hipMalloc(&b->d_cfgs, b->h_cfgs.size() * sizeof(rsg_car_config))) != hipSuccess ||
         (e = hipMemcpy(b->d_cfgs, b->h_cfgs.data(), b->h_cfgs.size() * sizeof(rsg_car_config),
-                       hipMemcpyHostToDevice)) != hipSuccess) {
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (!b->h_custom_pads.empty() &&
+         ((e = hipMalloc(&b->d_custom_pads, b->h_custom_pads.size() * sizeof(rsg_boost_pad_config))) != hipSuccess ||
+          (e = hipMemcpy(b->d_custom_pads, b->h_custom_pads.data(),
+                         b->h_custom_pads.size() * sizeof(rsg_boost_pad_config), hipMemcpyHostToDevice)) != hipSuccess))) {
         b->dev.mut = dmut;
         fail(std::string("rsg_batch_create: hipMalloc: ") + hipGetErrorString(e));
         rsg_batch_destroy(b);
@@ -305,6 +329,8 @@ rsg_batch* rsg_batch_create(const rsg_batch_desc* desc) {
     }
     b->dev.mut = dmut;
     b->dev.cfgs = b->d_cfgs;
+    b->dev.custom_pads = b->d_custom_pads;
+    b->dev.n_custom_pads = (int)b->h_custom_pads.size();
     // Host-side initial state: cars with config, ball at rest, pads active.
     BatchDev h = host_view(b);
     for (int a = 0; a < b->dev.n; a++)
@@ -326,6 +352,7 @@ void rsg_batch_destroy(rsg_batch* b) {
     if (b->dev.space) (void)hipFree(b->dev.space);
     if (b->dev.mut) (void)hipFree(const_cast<rsg_mutator_config*>(b->dev.mut));
     if (b->d_cfgs) (void)hipFree(b->d_cfgs);
+    if (b->d_custom_pads) (void)hipFree(b->d_custom_pads);
     if (b->status_partials) (void)hipFree(b->status_partials);
     if (b->dev.ev_buf) (void)hipFree(b->dev.ev_buf);
     if (b->dev.ev_count) (void)hipFree(b->dev.ev_count);
@@ -1177,6 +1204,7 @@ int rsg_snapshot_deserialize(const uint8_t* data, uint64_t size, rsg_arena_snaps
 int rsg_batch_get_snapshot(rsg_batch* b, int arena, rsg_arena_snapshot* out) {
     if (!b || !out) return fail("rsg_batch_get_snapshot: null argument");
     if (arena < 0 || arena >= b->dev.n) return fail("rsg_batch_get_snapshot: arena out of range");
+    if (b->dev.n_custom_pads) return fail("rsg_batch_get_snapshot: batches with custom boost pads have no snapshots");
     if (download(b) != 0) return -1;
     BatchDev h = host_view(b);
     std::memset(out, 0, sizeof(*out));
@@ -1227,6 +1255,7 @@ int rsg_batch_get_snapshot(rsg_batch* b, int arena, rsg_arena_snapshot* out) {
 int rsg_batch_set_snapshot(rsg_batch* b, int arena, const rsg_arena_snapshot* s) {
     if (!b || !s) return fail("rsg_batch_set_snapshot: null argument");
     if (arena < 0 || arena >= b->dev.n) return fail("rsg_batch_set_snapshot: arena out of range");
+    if (b->dev.n_custom_pads) return fail("rsg_batch_set_snapshot: batches with custom boost pads have no snapshots");
     if (s->mode != b->dev.mode) return fail("rsg_batch_set_snapshot: game mode differs from the batch's");
     if (s->tick_time != b->dev.tick_time) return fail("rsg_batch_set_snapshot: tick time differs from the batch's");
     const int n = b->cars_per_arena[arena];

@@ -85,7 +85,7 @@ rsg_step_arena_coop_kernel(BatchDev d, ArenaGeom geom, int ticks, int stride, in
         if (live && k < A.n_cars) car_post_tick(A, P, k);
         __syncthreads();
         if (live && mode_has_arena(d.mode))
-            for (int p = k; p < pad_count(d.mode); p += lanes) pad_check_cars(A, p);
+            for (int p = k; p < A.n_pads; p += lanes) pad_check_cars(A, p);
         __syncthreads();
         if (live && k == 0) tick_end(A, P);
         __syncthreads();

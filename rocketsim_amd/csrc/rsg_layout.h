@@ -119,6 +119,8 @@ struct BatchDev {
     int auto_reset;    // kickoff reset after a goal
     int synth_controls;// draw controls from the seeded process each tick
     int ball_rot;      // the ball integrates its rotation (ArenaConfig::noBallRot == false)
+    const rsg_boost_pad_config* custom_pads;  // ArenaConfig::customBoostPads, or null
+    int n_custom_pads;
     uint64_t seed;
     float tick_time;
     // event ring (null when events are off): append index from *ev_count

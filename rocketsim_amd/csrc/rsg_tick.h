@@ -183,6 +183,8 @@ RSG_FN void load_arena(const BatchDev& d, int a, ArenaLocal& A) {
     A.n_cars = (int)ar.u(AW_NUM_CARS);
     A.mode = d.mode;
     A.mut = d.mut ? d.mut[a] : mutator_defaults(d.mode);
+    A.custom_pads = d.n_custom_pads > 0 ? d.custom_pads : nullptr;
+    A.n_pads = d.n_custom_pads > 0 ? d.n_custom_pads : pad_count(d.mode);
     A.goals_blue = ar.u(AW_GOALS_BLUE);
     A.goals_orange = ar.u(AW_GOALS_ORANGE);
     A.last_goal_tick = ar.u64(AW_LAST_GOAL_TICK);
@@ -351,7 +353,7 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
     A.ball.ticks_since_set = 0;
     A.ball.vel_cache = v3zero();
     body_update_inertia(A.ball.b);
-    for (int p = 0; p < pad_count(A.mode); p++) {
+    for (int p = 0; p < A.n_pads; p++) {
         A.pads[p].active = true;
         A.pads[p].cooldown = 0;
         A.pads[p].prev_locked = 0;
@@ -362,20 +364,26 @@ RSG_FN void arena_kickoff_reset(ArenaLocal& A, uint64_t seed, uint32_t arena_id)
 // ---------------------------------------------------------------------------
 // Boost pads (src/Sim/BoostPad/BoostPad.cpp, BoostPadGrid.cpp)
 
-RSG_FN bool pad_is_big(int p) { return p < PAD_NUM_BIG; }
+RSG_FN bool pad_is_big(const ArenaLocal& A, int p) {
+    return A.custom_pads ? A.custom_pads[p].is_big != 0 : p < PAD_NUM_BIG;
+}
+RSG_FN V3 arena_pad_location(const ArenaLocal& A, int p) {
+    if (A.custom_pads) return v3(A.custom_pads[p].pos[0], A.custom_pads[p].pos[1], A.custom_pads[p].pos[2]);
+    return pad_location(A.mode, p);
+}
 
 // BoostPadGrid::Add cell of each pad; the grid maps each occupied (x,y) cell
 // to one pad (src/Sim/BoostPad/BoostPadGrid/BoostPadGrid.cpp:27-41).
 RSG_FN int pad_cell_x(V3 p) { return (int)(p.x / PAD_GRID_CELL_SIZE_X + (PAD_GRID_CELLS_X / 2)); }
 RSG_FN int pad_cell_y(V3 p) { return (int)(p.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2)); }
 
-RSG_FN void pad_check_collide(int mode, Pad& pad, int p, const Car& c, int car_idx, const CarPhys& cp) {
-    V3 pos_uu = pad_location(mode, p);
+RSG_FN void pad_check_collide(const ArenaLocal& A, Pad& pad, int p, const Car& c, int car_idx, const CarPhys& cp) {
+    V3 pos_uu = arena_pad_location(A, p);
     V3 pos_bt = pos_uu * UU_TO_BT;
     V3 cpos = c.b.pos;
     bool colliding = false;
     if (pad.prev_locked == c.id) {
-        float box_rad = (pad_is_big(p) ? PAD_BOX_RAD_BIG : PAD_BOX_RAD_SMALL) * UU_TO_BT;
+        float box_rad = (pad_is_big(A, p) ? PAD_BOX_RAD_BIG : PAD_BOX_RAD_SMALL) * UU_TO_BT;
         V3 bmin = pos_bt - v3(box_rad, box_rad, 0);
         V3 bmax = pos_bt + v3(box_rad, box_rad, PAD_BOX_HEIGHT * UU_TO_BT);
         // car->_rigidBody.getAabb: compound AABB without contact threshold
@@ -387,7 +395,7 @@ RSG_FN void pad_check_collide(int mode, Pad& pad, int p, const Car& c, int car_i
         colliding = (bmax.x > cmin.x && bmax.y > cmin.y && bmax.z > cmin.z) &&
                     (bmin.x < cmax.x && bmin.y < cmax.y && bmin.z < cmax.z);
     } else {
-        float rad = (pad_is_big(p) ? PAD_CYL_RAD_BIG : PAD_CYL_RAD_SMALL) * UU_TO_BT;
+        float rad = (pad_is_big(A, p) ? PAD_CYL_RAD_BIG : PAD_CYL_RAD_SMALL) * UU_TO_BT;
         float dx = cpos.x - pos_bt.x, dy = cpos.y - pos_bt.y;
         if ((dx * dx + dy * dy) < (rad * rad))
             colliding = fabsf(cpos.z - pos_bt.z) < (PAD_CYL_HEIGHT * UU_TO_BT);
@@ -515,7 +523,7 @@ RSG_FN bool car_pre_ticks_independent(const ArenaLocal& A, const RayHit* pre) {
 // Pads pre-tick (BoostPad.cpp:51-60).
 RSG_FN void pads_pre_tick(ArenaLocal& A, const TickParams& P) {
     if (!mode_has_arena(P.mode) || A.n_cars == 0) return;
-    for (int p = 0; p < pad_count(P.mode); p++) {
+    for (int p = 0; p < A.n_pads; p++) {
         Pad& pad = A.pads[p];
         if (pad.cooldown > 0) pad.cooldown = rs_max(pad.cooldown - P.dt, 0.f);
         pad.active = (pad.cooldown == 0);
@@ -561,6 +569,15 @@ RSG_FN void car_post_tick(ArenaLocal& A, const TickParams& P, int ci) {
 // test reads only that car's pose, boost and flags, which no later post-tick
 // changes, and the last colliding car locks the pad).
 RSG_FN void pad_check_cars(ArenaLocal& A, int p) {
+    if (A.custom_pads) {
+        // custom pads: every car against every pad, without the grid's
+        // early outs (Arena.cpp:723-728)
+        for (int oi = 0; oi < A.n_cars; oi++) {
+            const int ci = A.order[oi];
+            pad_check_collide(A, A.pads[p], p, A.cars[ci], ci, A.cars[ci].phys);
+        }
+        return;
+    }
     V3 pl = pad_location(A.mode, p);
     int px = pad_cell_x(pl), py = pad_cell_y(pl);
     for (int oi = 0; oi < A.n_cars; oi++) {
@@ -573,7 +590,7 @@ RSG_FN void pad_check_cars(ArenaLocal& A, int p) {
         int iy = (int)(cp_uu.y / PAD_GRID_CELL_SIZE_Y + (PAD_GRID_CELLS_Y / 2));
         int x0 = ix - 1 > 0 ? ix - 1 : 0, x1 = ix + 1 < PAD_GRID_CELLS_X - 1 ? ix + 1 : PAD_GRID_CELLS_X - 1;
         int y0 = iy - 1 > 0 ? iy - 1 : 0, y1 = iy + 1 < PAD_GRID_CELLS_Y - 1 ? iy + 1 : PAD_GRID_CELLS_Y - 1;
-        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A.mode, A.pads[p], p, c, ci, c.phys);
+        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) pad_check_collide(A, A.pads[p], p, c, ci, c.phys);
     }
 }
 
@@ -584,17 +601,17 @@ RSG_FN void tick_end(ArenaLocal& A, const TickParams& P) {
     const bool soccar = mode_has_arena(P.mode);
     // 6. pads post-tick (BoostPad.cpp:88-107)
     if (soccar && A.n_cars > 0) {
-        for (int p = 0; p < pad_count(P.mode); p++) {
+        for (int p = 0; p < A.n_pads; p++) {
             Pad& pad = A.pads[p];
             uint32_t locked_id = 0;
             if (pad.cur_locked >= 0) {
                 Car& c = A.cars[pad.cur_locked];
                 locked_id = c.id;
                 if (pad.active) {
-                    float add = pad_is_big(p) ? PAD_BOOST_AMOUNT_BIG : PAD_BOOST_AMOUNT_SMALL;
+                    float add = pad_is_big(A, p) ? PAD_BOOST_AMOUNT_BIG : PAD_BOOST_AMOUNT_SMALL;
                     c.boost = rs_min(c.boost + add, BOOST_MAX);
                     pad.active = false;
-                    pad.cooldown = pad_is_big(p) ? A.mut.boost_pad_cooldown_big : A.mut.boost_pad_cooldown_small;
+                    pad.cooldown = pad_is_big(A, p) ? A.mut.boost_pad_cooldown_big : A.mut.boost_pad_cooldown_small;
                 }
             }
             pad.prev_locked = locked_id;
@@ -796,7 +813,7 @@ RSG_FN void tick_rest(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo, 
     // 5. cars post-tick + pad collisions
     for (int oi = 0; oi < A.n_cars; oi++) car_post_tick(A, P, A.order[oi]);
     if (mode_has_arena(P.mode))
-        for (int p = 0; p < pad_count(P.mode); p++) pad_check_cars(A, p);
+        for (int p = 0; p < A.n_pads; p++) pad_check_cars(A, p);
     tick_end(A, P);
 }
 

@@ -187,6 +187,11 @@ class GameEventConfig(C.Structure):
         super().__init__(**defaults)
 
 
+class BoostPadConfig(C.Structure):
+    """rsg_boost_pad_config: BoostPadConfig (src/Sim/BoostPad/BoostPad.h:13-19)."""
+    _fields_ = [("pos", C.c_float * 3), ("is_big", C.c_uint8), ("_pad", C.c_uint8 * 3)]
+
+
 class BatchDesc(C.Structure):
     _fields_ = [
         ("n_arenas", C.c_int), ("mode", C.c_int), ("tick_rate", C.c_float),
@@ -197,6 +202,8 @@ class BatchDesc(C.Structure):
         ("mutators", C.POINTER(MutatorConfig)),
         ("car_configs", C.POINTER(CarConfig)),
         ("ball_rotation", C.c_int),
+        ("custom_pads", C.POINTER(BoostPadConfig)),
+        ("n_custom_pads", C.c_int),
     ]
 
 

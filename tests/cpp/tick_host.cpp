@@ -56,6 +56,7 @@ struct HostBatch {
     std::vector<rsg_mutator_config> mut;
     std::vector<rsg_car_config> cfgs;  // config table: presets, then the cars' other configs
     std::vector<uint8_t> cfg_index;    // each car's entry
+    std::vector<rsg_boost_pad_config> custom_pads;
     BatchDev dev{};
 };
 
@@ -213,6 +214,14 @@ static void tick_split(ArenaLocal& A, const TickParams& P, const ArenaGeom* geo,
     tick_end(A, P);
 }
 #endif
+
+// ArenaConfig::customBoostPads for every arena (the batch's custom_pads).
+void th_set_custom_pads(void* p, const rsg_boost_pad_config* pads, int n) {
+    HostBatch& h = *(HostBatch*)p;
+    h.custom_pads.assign(pads, pads + n);
+    h.dev.custom_pads = h.custom_pads.empty() ? nullptr : h.custom_pads.data();
+    h.dev.n_custom_pads = n;
+}
 
 // ArenaConfig::noBallRot == false for every arena (the batch's ball_rotation).
 void th_set_ball_rotation(void* p, int on) { ((HostBatch*)p)->dev.ball_rot = on != 0; }

@@ -696,7 +696,7 @@ def load_samples(path):
 
 
 def run_on_gpu(samples, mode, kernel=0, per_block=0, auto_reset=False, seed=0, mutators=None, mutators_first=False,
-               cmfs=None, ball_rotation=False):
+               cmfs=None, ball_rotation=False, custom_pads=None):
     """Step every sample once on the GPU, one sample per arena.  `kernel` and
     `per_block` pick the step kernel and arenas per workgroup (0: the
     library's automatic choice); `auto_reset` / `seed` set the batch's kickoff
@@ -715,7 +715,7 @@ def run_on_gpu(samples, mode, kernel=0, per_block=0, auto_reset=False, seed=0, m
     batch = ArenaBatch(cpa, sum((s.teams for s in samples), []), sum((s.presets for s in samples), []),
                        mode=mode, auto_reset=auto_reset, seed=seed,
                        mutators=mutators if mutators_first else None, car_configs=car_configs,
-                       ball_rotation=ball_rotation)
+                       ball_rotation=ball_rotation, custom_pads=custom_pads)
     try:
         batch.set_kernel(kernel)
         batch.set_launch(per_block)

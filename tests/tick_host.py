@@ -61,6 +61,7 @@ def load(path):
     L.th_get_tick_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
     L.th_set_car_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     L.th_set_ball_rotation.argtypes = [C.c_void_p, C.c_int]
+    L.th_set_custom_pads.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
     L.th_get_status.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
     L.th_step.argtypes = [C.c_void_p, C.c_int]
     L.th_rsqrt_signature.argtypes = [C.POINTER(C.c_float)]
@@ -114,7 +115,7 @@ def mesh_args(cmfs):
 
 
 def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators_first=False, cmfs=None,
-                ball_rotation=False):
+                ball_rotation=False, custom_pads=None):
     """Step every teacher-forced sample once on the host build, one sample per
     arena (the host counterpart of parity.run_on_gpu).  `inject(cars, balls)`
     may edit the "before" records; `mutators` is every arena's MutatorConfig,
@@ -142,6 +143,11 @@ def run_samples(L, samples, mode, cmf=None, inject=None, mutators=None, mutators
         if mutators is not None and not mutators_first:
             L.th_set_mutators(h, C.byref(mutators))
         L.th_set_ball_rotation(h, 1 if ball_rotation else 0)
+        if custom_pads:
+            from rocketsim_amd.state import BoostPadConfig
+            arr = (BoostPadConfig * len(custom_pads))(
+                *[BoostPadConfig((C.c_float * 3)(x, y, z), int(bool(big))) for x, y, z, big in custom_pads])
+            L.th_set_custom_pads(h, arr, len(custom_pads))
         cars = (CarState * max(total, 1))(*sum((list(s.cars_in) for s in samples), []))
         balls = (BallState * n)(*[s.ball_in for s in samples])
         if inject:
