@@ -22,8 +22,9 @@
 #define RSG_ROCKETSIM_HPP
 
 #include <cstring>
+#include <stdexcept>
 
-#include "rsg.h"
+#include "rsg.hpp"
 
 namespace rsg_amd {
 
@@ -316,6 +317,26 @@ inline RocketSim::MutatorConfig FromRsg(const rsg_mutator_config& s, RocketSim::
     m.enableTeamDemos = s.enable_team_demos != 0;
     m.goalBaseThresholdY = s.goal_base_threshold_y;
     return m;
+}
+
+// ArenaConfig (src/Sim/Arena/ArenaConfig/ArenaConfig.h) onto a BatchConfig:
+// noBallRot and the custom boost pads carry over; a broadphase other than the
+// default grid cannot be simulated and throws, as RS_ERR_CLOSE would.
+inline void ApplyArenaConfig(const RocketSim::ArenaConfig& a, BatchConfig& cfg) {
+    const RocketSim::ArenaConfig def;
+    if (!a.useCustomBroadphase || a.maxAABBLen != def.maxAABBLen || a.minPos.x != def.minPos.x ||
+        a.minPos.y != def.minPos.y || a.minPos.z != def.minPos.z || a.maxPos.x != def.maxPos.x ||
+        a.maxPos.y != def.maxPos.y || a.maxPos.z != def.maxPos.z)
+        throw std::runtime_error("rsg: only the default broadphase grid of ArenaConfig is supported");
+    cfg.ball_rotation = !a.noBallRot;
+    cfg.custom_pads.clear();
+    if (a.useCustomBoostPads)
+        for (const auto& p : a.customBoostPads) {
+            rsg_boost_pad_config c{};
+            c.pos[0] = p.pos.x; c.pos[1] = p.pos.y; c.pos[2] = p.pos.z;
+            c.is_big = p.isBig ? 1 : 0;
+            cfg.custom_pads.push_back(c);
+        }
 }
 
 }  // namespace rsg_amd
