@@ -167,8 +167,10 @@ enum { RSG_NUM_PADS = 34, RSG_MAX_CARS = 8, RSG_SNAPSHOT_RAW_WORDS = 1024 };
  *   RSG_STATUS_NONFINITE         a car or the ball left a tick with a
  *                                non-finite position or velocity;
  *   RSG_STATUS_CONTACT_OVERFLOW  a tick needed more contact manifolds than
- *                                RSG_MAX_MANIFOLDS (the extra pairs were not
- *                                collided);
+ *                                RSG_MAX_MANIFOLDS, or had more overlapping
+ *                                pairs than its pair table holds (16 statics
+ *                                per body on average plus every body pair;
+ *                                the extra pairs were not collided);
  *   RSG_STATUS_SOLVER_OVERFLOW   a tick needed more than RSG_MAX_SOLVER_ROWS
  *                                contact or friction rows (the extra contacts
  *                                were not solved).
