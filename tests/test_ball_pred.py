@@ -316,3 +316,51 @@ def test_gpu_heatseeker_predictions_one_tick_parity_with_oracle():
     finally:
         A.close()
         tr.close()
+
+
+@pytest.mark.gpu
+def test_gpu_hoops_predictions_one_tick_parity_with_oracle():
+    """A Hoops tracker (BallPredTracker.cpp:7: the source arena's mode) predicts
+    in the Hoops arena: its planes, rims and nets.  Each GPU prediction of a
+    ball lobbed at a hoop, set on the oracle and stepped one tick, gives the
+    next one within the tolerances.  The balls move, so the kickoff launch
+    rule (which counts ticks since the ball was set) never applies."""
+    ref = _oracle()
+    if ref is None:
+        pytest.skip("oracle library not built on this machine")
+    import random
+    from rocketsim_amd.ball_pred import BallPredTracker
+    from rocketsim_amd.state import MODE_HOOPS
+    n, T = 24, 240
+    rng = random.Random(7)
+    A = ref.RefArena(MODE_HOOPS)
+    states = []
+    for _ in range(n):
+        s = A.get_ball()
+        d = rng.choice([-1.0, 1.0])
+        s.pos[:] = [rng.uniform(-600, 600), d * rng.uniform(1500, 2600), rng.uniform(200, 900)]
+        s.vel[:] = [rng.gauss(0, 200), d * rng.uniform(300, 1200), rng.uniform(200, 900)]
+        s.ang_vel[:] = [rng.gauss(0, 2) for _ in range(3)]
+        states.append(s)
+    tr = BallPredTracker(mode=MODE_HOOPS, n=n, num_pred_ticks=T)
+    try:
+        tr.force_update_all_pred(states)
+        bad = total = 0
+        low = 0
+        for i in range(n):
+            pd = tr.pred_data(i)
+            for t in range(T - 1):
+                s = BallState.from_buffer_copy(bytes(pd[t]))
+                A.set_ball(s)
+                A.step(1)
+                r = A.get_ball()
+                r.bp_cell, r.bp_seq = 0, 0
+                total += 1
+                bad += 1 if parity.compare_ball(r, pd[t + 1]) else 0
+                low += 1 if pd[t + 1].pos[2] < 270 and abs(abs(pd[t + 1].pos[1]) - 2900) < 600 else 0
+        print(f"Hoops one-tick parity along GPU predictions: {bad}/{total} outside tolerance, "
+              f"{low} ticks low inside a hoop")
+        assert bad <= 0.005 * total and low > 0
+    finally:
+        A.close()
+        tr.close()
